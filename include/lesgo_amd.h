@@ -1,0 +1,217 @@
+/*
+ * lesgo_amd.h -- C ABI of the MI355X (gfx950) implementation of the per-timestep
+ * RHS + pressure-projection path of the lesgo-lineage LES in
+ * adiaiyer/Population_Balance.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and returns an
+ * int status (LP_OK == 0).  A Fortran host binds them through ISO_C_BINDING
+ * (population_balance_amd/fortran/lesgo_amd_iface.f90); Python binds them through
+ * ctypes (population_balance_amd/lesgo.py).
+ *
+ * Array conventions (identical to the reference's Fortran memory):
+ *   - a 3-D field f(ld,ny,1:nz) is column-major, i.e. C f[k][jy][jx] with jx fastest,
+ *     row pitch ld = 2*(nx/2+1), level k = jz-1.  Columns nx..ld-1 are FFT padding.
+ *   - p(ld,ny,0:nz) has nz+1 levels, level k = jz.
+ *   - "big" (3/2-padded) fields are (ld_big, ny2, nz).
+ *   - spectra overlay the same memory as complex(lh,ny) (interleaved re/im).
+ *   - element type: double for an LP_FP64 context, float for an LP_MIXED context
+ *     (the exceptions, always double, are listed with the field ids below).
+ *   - every array argument may be host memory or device memory: device pointers are
+ *     used in place, host pointers are staged through the context (synchronous).
+ *
+ * Error behaviour: the reference STOPs on a zero pivot (tridag_array.f90:35-38,70-75)
+ * and on an invalid lbc_mom (wallstress.f90:207-209); here those return
+ * LP_ERR_PIVOT / LP_ERR_ARG and lp_last_error() holds the message.
+ */
+#ifndef LESGO_AMD_H
+#define LESGO_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LP_ABI_VERSION 1
+
+enum lp_status {
+  LP_OK = 0,
+  LP_ERR_ARG = 1,         /* bad argument / unsupported configuration            */
+  LP_ERR_HIP = 2,         /* HIP runtime error                                   */
+  LP_ERR_PIVOT = 3,       /* zero pivot in the tridiagonal solve                 */
+  LP_ERR_NAN = 4,         /* non-finite value detected by lp_check_finite        */
+  LP_ERR_NOMEM = 5,       /* device allocation failed                            */
+  LP_ERR_NODEVICE = 6     /* no HIP device visible                               */
+};
+
+enum lp_precision {
+  LP_FP64 = 0,            /* every field and every operation in binary64          */
+  LP_MIXED = 1            /* FFT/convection/SGS in binary32; pressure tridiagonal,
+                             mean mode, p, and the Lagrangian F_LM/F_MM/F_QN/F_NN
+                             averages in binary64                                 */
+};
+
+enum lp_lbc_mom { LP_LBC_WALL = 0, LP_LBC_STRESS_FREE = 1 };
+
+/* Grid: replaces the compile-time parameters of param.f90:57-62,70-72. */
+typedef struct lp_grid {
+  int32_t nx, ny, nz;            /* nz = staggered levels (reference nz, lbz = 1)      */
+  int32_t ld, lh;                /* lh = nx/2+1, ld = 2*lh                             */
+  int32_t nx2, ny2;              /* 3nx/2, 3ny/2                                       */
+  int32_t lh_big, ld_big;        /* nx2/2+1, 2*lh_big                                  */
+  int32_t pad_;
+  double L_x, L_y, L_z;          /* domain; z_i = 1                                   */
+  double dx, dy, dz;             /* L_x/nx, L_y/ny, L_z/(nz-1)                          */
+} lp_grid;
+
+/* Droplet (PCon) parameters: param.f90:303-440,560-682 re-expressed at run time. */
+typedef struct lp_droplet_config {
+  int32_t npcon;                 /* number of size bins (0 = droplets off)             */
+  int32_t pbe;                   /* compute the breakup source Rhs_pbe (PBE_FLAG)      */
+  int32_t pbe_feedback;          /* add Rhs_pbe to RHS_PCon (commented out in the
+                                    reference, scalars_module_crossflow_mod.f90:4470)  */
+  int32_t settling, inertia;     /* settling, PCon_acc                                 */
+  int32_t active;                /* active_pcon: droplet buoyancy into RHSz            */
+  int32_t inlet_x, inlet_y;      /* PCON_INLET_X / PCON_INLET_Y                        */
+  int32_t src_i, src_j, src_k;   /* point-source centre (1-based node indices)         */
+  int32_t pad_;
+  double src_sigma;              /* Gaussian source width in cells                     */
+  double Sc;                     /* SGS Schmidt number (model_sc = 1)                  */
+  double rho_c, rho_w;           /* droplet / sea-water density                        */
+  double sigma_s, nu_c, nu_d, k_b, d_min; /* surface tension, viscosities, breakup
+                                    constant, minimum daughter size (param.f90:401)    */
+  double g;                      /* gravity (dimensional, nondimensionalised inside)   */
+  double u_star;                 /* velocity scale                                     */
+  const double *diameter;        /* [npcon] bin diameters (m)                          */
+  const double *settling_vel;    /* [npcon] rise/settling velocity (nondimensional)    */
+  const double *Q_src;           /* [npcon] source strength (nondimensional)           */
+} lp_droplet_config;
+
+/* Run-time configuration: replaces param.f90 / sgsmodule.f90 parameters. */
+typedef struct lp_config {
+  lp_grid grid;
+  double dt;                     /* param.f90:105                                      */
+  int32_t precision;             /* enum lp_precision                                  */
+  int32_t model;                 /* SGS model: 1 Smagorinsky, 5 Lagrangian scale-dep.  */
+  int32_t inilag, initu;         /* param.f90:270                                      */
+  int32_t cs_count, dyn_init;    /* param.f90:127,249                                  */
+  int32_t fp32_casts;            /* reproduce REAL(4) conversions of lagrange_Sdep.f90:
+                                    311,313,329,361-363,386,388,397,405-407,430-434    */
+  int32_t lbc_mom;               /* enum lp_lbc_mom                                    */
+  double z0, d0;                 /* roughness and displacement height (uniform)        */
+  double vonk;                   /* 0.4                                                */
+  double Co;                     /* model 1 constant (param.f90:160)                   */
+  int32_t nnn;                   /* model 1 wall-damping exponent                      */
+  int32_t use_mean_p_force;
+  double mean_p_force;
+  double cs2_init;               /* Cs_opt2 at jt == 1 with inilag (0.03)              */
+  double opftime;                /* 1.5 (sgsmodule.f90:7)                              */
+  double filter_size;            /* 1 (test_filtermodule.f90:8)                        */
+  lp_droplet_config drop;
+} lp_config;
+
+/* State field ids.  Shapes are (ld,ny,nz) unless noted.  In an LP_MIXED context the
+ * fields marked [f64] stay binary64, all others are binary32. */
+enum lp_field {
+  LP_U = 0, LP_V, LP_W,
+  LP_DUDX, LP_DUDY, LP_DUDZ, LP_DVDX, LP_DVDY, LP_DVDZ, LP_DWDX, LP_DWDY, LP_DWDZ,
+  LP_RHSX, LP_RHSY, LP_RHSZ, LP_RHSX_F, LP_RHSY_F, LP_RHSZ_F,
+  LP_TXX, LP_TXY, LP_TXZ, LP_TYY, LP_TYZ, LP_TZZ,
+  LP_DIVTX, LP_DIVTY, LP_DIVTZ,
+  LP_P,                          /* (ld,ny,0:nz)  [f64]                                */
+  LP_DPDX, LP_DPDY, LP_DPDZ,
+  LP_F_LM, LP_F_MM, LP_F_QN, LP_F_NN,   /* [f64]                                      */
+  LP_CS_OPT2, LP_BETA,
+  LP_NU_T, LP_DISSIP, LP_MAGS,
+  LP_U_LAG, LP_V_LAG, LP_W_LAG,
+  LP_DUDT, LP_DVDT, LP_DWDT,
+  LP_USTAR,                      /* (ld,ny) wall u* of wallstress                       */
+  LP_PCON,                       /* (ld,ny,nz,npcon) [f64]                             */
+  LP_RHS_PCON, LP_RHS_PCON_F,    /* (ld,ny,nz,npcon) [f64]                             */
+  LP_RHS_PBE,                    /* (ld,ny,nz,npcon) [f64]                             */
+  LP_BETA_PCON,                  /* (ld,ny,nz)                                         */
+  LP_NFIELDS
+};
+
+typedef struct lp_ctx lp_ctx;
+
+/* ---------------- lifecycle ---------------- */
+int lp_abi_version(void);
+const char *lp_last_error(void);
+/* Fill the derived members of *g from nx, ny, nz and the domain (param.f90:57-72). */
+int lp_grid_init(lp_grid *g, int32_t nx, int32_t ny, int32_t nz,
+                 double L_x, double L_y, double L_z);
+/* Defaults of the reference-equivalent configuration (model 5, inilag, cs_count=1,
+ * DYN_init=2, wall BC, z0=1e-4, fp32_casts on, FP64). */
+void lp_config_default(lp_config *c);
+/* Create a context on the current HIP device: twiddles, wavenumbers (fft.f90:35-76),
+ * G_test / G_test_test (test_filtermodule.f90:29-57, main.f90:179-183), device state,
+ * scratch.  The droplet arrays of cfg->drop are copied. */
+int lp_create(const lp_config *cfg, lp_ctx **out);
+int lp_destroy(lp_ctx *ctx);
+int lp_sync(lp_ctx *ctx);
+
+/* ---------------- state access ---------------- */
+int lp_field_info(lp_ctx *ctx, int32_t field, void **dev_ptr, size_t *bytes,
+                  int32_t *elem_size);
+int lp_upload(lp_ctx *ctx, int32_t field, const void *host, size_t bytes);
+int lp_download(lp_ctx *ctx, int32_t field, void *host, size_t bytes);
+int lp_set_time(lp_ctx *ctx, int32_t jt, int32_t jt_total);
+int lp_get_time(lp_ctx *ctx, int32_t *jt, int32_t *jt_total);
+
+/* ---------------- spectral primitives (explicit arrays) ---------------- */
+/* fft.f90:51-76 -> kx, ky, k2 (lh,ny) binary64 */
+int lp_init_wavenumber(lp_ctx *ctx, double *kx, double *ky, double *k2);
+/* test_filtermodule.f90:29-57 -> G (lh,ny) binary64, ifilter = 1 (sharp cutoff) */
+int lp_test_filter_init(lp_ctx *ctx, double alpha, double *G);
+/* filt_da.f90:3-36; overwrites f with its Nyquist-filtered self; nlev planes */
+int lp_filt_da(lp_ctx *ctx, void *f, void *dfdx, void *dfdy, int32_t nlev);
+/* ddx.f90:1-31, ddy.f90:1-31, ddxy.f90:1-29 */
+int lp_ddx(lp_ctx *ctx, void *dfdx, const void *f, int32_t nlev);
+int lp_ddy(lp_ctx *ctx, void *dfdy, const void *f, int32_t nlev);
+int lp_ddxy(lp_ctx *ctx, void *dfdx, void *dfdy, const void *f, int32_t nlev);
+/* ddz_uv.f90:5-60 (level 1 untouched), ddz_w.f90:6-51; nz levels */
+int lp_ddz_uv(lp_ctx *ctx, void *dfdz, const void *f);
+int lp_ddz_w(lp_ctx *ctx, void *dfdz, const void *f);
+/* padd.f90:1-26 / unpadd.f90:1-22 on nplanes complex planes (lh,ny) <-> (lh_big,ny2) */
+int lp_padd(lp_ctx *ctx, void *u_big, const void *u, int32_t nplanes);
+int lp_unpadd(lp_ctx *ctx, void *cc, const void *cc_big, int32_t nplanes);
+/* dealias1.f90:1-31 (u untouched) / dealias2.f90:1-33 (u_big is clobbered, as the
+ * reference leaves its scaled spectrum there); nlev planes */
+int lp_dealias1(lp_ctx *ctx, const void *u, void *u_big, int32_t nlev);
+int lp_dealias2(lp_ctx *ctx, void *u, void *u_big, int32_t nlev);
+/* test_filtermodule.f90:12-27 on nplanes (ld,ny) planes, G (lh,ny) binary64 */
+int lp_test_filter(lp_ctx *ctx, void *f, const double *G, int32_t nplanes);
+
+/* ---------------- physics routines (explicit arrays) ---------------- */
+/* divstress_uv.f90:2-129 / divstress_w.f90:4-85 */
+int lp_divstress_uv(lp_ctx *ctx, void *divt, const void *tx, const void *ty,
+                    const void *tz);
+int lp_divstress_w(lp_ctx *ctx, void *divt, const void *tx, const void *ty,
+                   const void *tz);
+/* tridag_array.f90:1-163: a,b,c (lh,ny,nz+1) binary64, r,u complex (lh,ny,nz+1)
+ * binary64; modes jy = ny/2+1, jx = lh and (1,1) are skipped as in the reference. */
+int lp_tridag_array(lp_ctx *ctx, const double *a, const double *b, const double *c,
+                    const double *r, double *u);
+
+/* ---------------- physics routines on the context state ----------------
+ * The reference's no-argument routines read and write module globals
+ * (sim_param, sgsmodule); these read and write the context's fields. */
+int lp_wallstress(lp_ctx *ctx);                 /* wallstress.f90:5-215            */
+int lp_sgs_stag(lp_ctx *ctx);                   /* sgs_stag.f90:3-401 (+lagrange)  */
+int lp_lagrange_sdep(lp_ctx *ctx);              /* lagrange_Sdep.f90:4-496         */
+int lp_interpolag_sdep(lp_ctx *ctx);            /* interpolag_Sdep.f90:2-373       */
+int lp_convec(lp_ctx *ctx);                     /* convec.f90:9-316 -> RHSx/y/z    */
+int lp_press_stag_array(lp_ctx *ctx);           /* press_stag_array.f90:1-361      */
+int lp_project(lp_ctx *ctx);                    /* forcing.f90:122-294             */
+int lp_timestep_conditions(lp_ctx *ctx, double *cfl, double *visc_stab);
+                                                /* scalars_module2.f90:1336-1379   */
+int lp_rmsdiv(lp_ctx *ctx, double *rmsdiv);     /* rmsdiv.f90:1-44                 */
+/* One or more device-resident time steps of the hot loop (main.f90:207-1004). */
+int lp_step(lp_ctx *ctx, int32_t nsteps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

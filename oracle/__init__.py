@@ -1,0 +1,275 @@
+"""ctypes binding of the CPU binary64 oracle (oracle/lesgo_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() (as the
+checker) and bench.py's cpu_baseline leg.  The product package never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from population_balance_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblesgo_oracle.so")
+_lib = None
+
+
+def build(force=False):
+    """Compile the oracle with its Makefile (gcc, OpenMP)."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", _HERE, "-B" if force else "all"])
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        vp, i32, dp = C.c_void_p, C.c_int32, C.POINTER(C.c_double)
+        L.or_create.restype = vp
+        L.or_create.argtypes = [C.POINTER(abi.lp_config)]
+        L.or_destroy.argtypes = [vp]
+        L.or_field.restype = dp
+        L.or_field.argtypes = [vp, i32]
+        L.or_field_elems.restype = C.c_size_t
+        L.or_field_elems.argtypes = [vp, i32]
+        L.or_set_time.argtypes = [vp, i32, i32]
+        L.or_get_time.argtypes = [vp, C.POINTER(i32), C.POINTER(i32)]
+        L.or_set_threads.argtypes = [C.c_int]
+        L.or_set_threads.restype = C.c_int
+        L.or_last_error.restype = C.c_char_p
+        L.or_rfft2_fwd.argtypes = [vp, C.c_int, vp, C.c_int, vp]
+        L.or_rfft2_inv.argtypes = [vp, C.c_int, vp, vp, C.c_int]
+        L.or_cfft1.argtypes = [C.c_int, C.c_int, vp, vp]
+        for n in ("or_init_wavenumber",):
+            getattr(L, n).argtypes = [vp, vp, vp, vp]
+        L.or_test_filter_init.argtypes = [vp, C.c_double, vp]
+        L.or_filt_da.argtypes = [vp, vp, vp, vp, C.c_int]
+        L.or_ddx.argtypes = [vp, vp, vp, C.c_int]
+        L.or_ddy.argtypes = [vp, vp, vp, C.c_int]
+        L.or_ddxy.argtypes = [vp, vp, vp, vp, C.c_int]
+        L.or_ddz_uv.argtypes = [vp, vp, vp]
+        L.or_ddz_w.argtypes = [vp, vp, vp]
+        L.or_padd.argtypes = [vp, vp, vp, C.c_int]
+        L.or_unpadd.argtypes = [vp, vp, vp, C.c_int]
+        L.or_dealias1.argtypes = [vp, vp, vp, C.c_int]
+        L.or_dealias2.argtypes = [vp, vp, vp, C.c_int]
+        L.or_test_filter.argtypes = [vp, vp, vp, C.c_int]
+        L.or_divstress_uv.argtypes = [vp, vp, vp, vp, vp]
+        L.or_divstress_w.argtypes = [vp, vp, vp, vp, vp]
+        L.or_tridag_array.argtypes = [vp, vp, vp, vp, vp, vp]
+        for n in ("or_wallstress", "or_sgs_stag", "or_lagrange_sdep", "or_interpolag_sdep",
+                  "or_convec", "or_press_stag_array", "or_project"):
+            getattr(L, n).argtypes = [vp]
+            getattr(L, n).restype = C.c_int
+        L.or_tridag_array.restype = C.c_int
+        L.or_timestep_conditions.argtypes = [vp, dp, dp]
+        L.or_rmsdiv.argtypes = [vp, dp]
+        L.or_step.argtypes = [vp, C.c_int]
+        L.or_step.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class Oracle:
+    """One oracle context: configuration, wavenumbers, filters and the state fields
+    (all binary64, shapes from population_balance_amd.abi.field_shape)."""
+
+    def __init__(self, cfg, threads=None):
+        self.L = lib()
+        if threads:
+            self.L.or_set_threads(int(threads))
+        self.cfg = cfg
+        self.g = cfg.grid
+        self._cfg_ref = cfg
+        self.h = self.L.or_create(C.byref(cfg))
+        if not self.h:
+            raise RuntimeError("or_create failed")
+
+    def close(self):
+        if self.h:
+            self.L.or_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc}): {self.L.or_last_error().decode()}")
+
+    # ---- state ----
+    def field(self, name):
+        """numpy view (no copy) of a state field."""
+        fid = abi.FIELD[name]
+        n = self.L.or_field_elems(self.h, fid)
+        ptr = self.L.or_field(self.h, fid)
+        arr = np.ctypeslib.as_array(ptr, shape=(n,))
+        return arr.reshape(abi.field_shape(self.g, name, self.cfg.drop.npcon))
+
+    def set_field(self, name, value):
+        self.field(name)[...] = value
+
+    def set_time(self, jt, jt_total=None):
+        self.L.or_set_time(self.h, jt, jt if jt_total is None else jt_total)
+
+    def time(self):
+        a, b = C.c_int32(), C.c_int32()
+        self.L.or_get_time(self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    # ---- primitives ----
+    def plane_shape(self, big=False):
+        g = self.g
+        return (g.ny2, g.ld_big) if big else (g.ny, g.ld)
+
+    def rfft2_fwd(self, plane, big=False):
+        g = self.g
+        nx, ny = (g.nx2, g.ny2) if big else (g.nx, g.ny)
+        lh = nx // 2 + 1
+        out = np.zeros((ny, lh), np.complex128)
+        src = np.ascontiguousarray(plane, np.float64)
+        self.L.or_rfft2_fwd(self.h, int(big), _p(src), src.shape[-1], _p(out))
+        return out
+
+    def rfft2_inv(self, spec, big=False, pitch=None):
+        g = self.g
+        nx, ny = (g.nx2, g.ny2) if big else (g.nx, g.ny)
+        pitch = pitch or (nx + 2)
+        src = np.ascontiguousarray(spec, np.complex128)
+        out = np.zeros((ny, pitch), np.float64)
+        self.L.or_rfft2_inv(self.h, int(big), _p(src), _p(out), pitch)
+        return out
+
+    def init_wavenumber(self):
+        g = self.g
+        kx, ky, k2 = (np.zeros((g.ny, g.lh)) for _ in range(3))
+        self.L.or_init_wavenumber(self.h, _p(kx), _p(ky), _p(k2))
+        return kx, ky, k2
+
+    def test_filter_init(self, alpha):
+        G = np.zeros((self.g.ny, self.g.lh))
+        self.L.or_test_filter_init(self.h, float(alpha), _p(G))
+        return G
+
+    def filt_da(self, f):
+        """returns (f_filtered, dfdx, dfdy); f is not modified (a copy is)."""
+        f = np.array(f, np.float64, order="C", copy=True)
+        dx, dy = np.zeros_like(f), np.zeros_like(f)
+        self.L.or_filt_da(self.h, _p(f), _p(dx), _p(dy), f.shape[0])
+        return f, dx, dy
+
+    def ddx(self, f):
+        f = np.ascontiguousarray(f, np.float64)
+        out = np.zeros_like(f)
+        self.L.or_ddx(self.h, _p(out), _p(f), f.shape[0])
+        return out
+
+    def ddy(self, f):
+        f = np.ascontiguousarray(f, np.float64)
+        out = np.zeros_like(f)
+        self.L.or_ddy(self.h, _p(out), _p(f), f.shape[0])
+        return out
+
+    def ddxy(self, f):
+        f = np.ascontiguousarray(f, np.float64)
+        a, b = np.zeros_like(f), np.zeros_like(f)
+        self.L.or_ddxy(self.h, _p(a), _p(b), _p(f), f.shape[0])
+        return a, b
+
+    def ddz_uv(self, f, dfdz=None):
+        f = np.ascontiguousarray(f, np.float64)
+        out = np.zeros_like(f) if dfdz is None else np.array(dfdz, np.float64, copy=True)
+        self.L.or_ddz_uv(self.h, _p(out), _p(f))
+        return out
+
+    def ddz_w(self, f):
+        f = np.ascontiguousarray(f, np.float64)
+        out = np.zeros_like(f)
+        self.L.or_ddz_w(self.h, _p(out), _p(f))
+        return out
+
+    def padd(self, u):
+        g = self.g
+        u = np.ascontiguousarray(u, np.complex128)
+        n = u.shape[0]
+        out = np.zeros((n, g.ny2, g.lh_big), np.complex128)
+        self.L.or_padd(self.h, _p(out), _p(u), n)
+        return out
+
+    def unpadd(self, cb):
+        g = self.g
+        cb = np.ascontiguousarray(cb, np.complex128)
+        n = cb.shape[0]
+        out = np.zeros((n, g.ny, g.lh), np.complex128)
+        self.L.or_unpadd(self.h, _p(out), _p(cb), n)
+        return out
+
+    def dealias1(self, u):
+        g = self.g
+        u = np.ascontiguousarray(u, np.float64)
+        out = np.zeros((u.shape[0], g.ny2, g.ld_big))
+        self.L.or_dealias1(self.h, _p(u), _p(out), u.shape[0])
+        return out
+
+    def dealias2(self, u_big):
+        g = self.g
+        ub = np.array(u_big, np.float64, copy=True)
+        out = np.zeros((ub.shape[0], g.ny, g.ld))
+        self.L.or_dealias2(self.h, _p(out), _p(ub), ub.shape[0])
+        return out
+
+    def test_filter(self, f, G):
+        f = np.array(f, np.float64, copy=True)
+        G = np.ascontiguousarray(G, np.float64)
+        n = f.shape[0] if f.ndim == 3 else 1
+        self.L.or_test_filter(self.h, _p(f), _p(G), n)
+        return f
+
+    def divstress_uv(self, tx, ty, tz):
+        out = np.zeros_like(np.asarray(tx, np.float64))
+        a, b, c = (np.ascontiguousarray(x, np.float64) for x in (tx, ty, tz))
+        self.L.or_divstress_uv(self.h, _p(out), _p(a), _p(b), _p(c))
+        return out
+
+    def divstress_w(self, tx, ty, tz):
+        out = np.zeros_like(np.asarray(tx, np.float64))
+        a, b, c = (np.ascontiguousarray(x, np.float64) for x in (tx, ty, tz))
+        self.L.or_divstress_w(self.h, _p(out), _p(a), _p(b), _p(c))
+        return out
+
+    def tridag_array(self, a, b, c, r, u0=None):
+        a, b, c = (np.ascontiguousarray(x, np.float64) for x in (a, b, c))
+        r = np.ascontiguousarray(r, np.complex128)
+        u = np.zeros_like(r) if u0 is None else np.array(u0, np.complex128, copy=True)
+        rc = self.L.or_tridag_array(self.h, _p(a), _p(b), _p(c), _p(r), _p(u))
+        self._check(rc, "tridag_array")
+        return u
+
+    def call(self, name):
+        rc = getattr(self.L, "or_" + name)(self.h)
+        self._check(rc, name)
+
+    def timestep_conditions(self):
+        a, b = C.c_double(), C.c_double()
+        self.L.or_timestep_conditions(self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def rmsdiv(self):
+        a = C.c_double()
+        self.L.or_rmsdiv(self.h, C.byref(a))
+        return a.value
+
+    def step(self, n=1):
+        self._check(self.L.or_step(self.h, int(n)), "step")

@@ -1,0 +1,1279 @@
+/*
+ * lesgo_oracle.c -- CPU binary64 restatement of the lesgo-lineage LES hot path of
+ * adiaiyer/Population_Balance (velocity RHS, SGS, pressure projection).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP implementation and the timed
+ * CPU baseline.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it.
+ *
+ * Each routine names the reference file:line it restates.  Reference traps kept on
+ * purpose: filt_da overwrites its input; lagrange_Sdep's REAL(4) conversions (switch
+ * fp32_casts); u_lag accumulates over the jt == 1 step that skips the dynamic model;
+ * BOGUS (-1234567890) on the invalid top levels.  FFTs are this file's own
+ * mixed-radix DFT with FFTW2's unnormalised r2c/c2r conventions (the reference
+ * links FFTW 2.1.x, which is not vendored; its DFT is the textbook one).
+ *
+ * Parity status of each function: see the test that pins it (tests/test_oracle_*.py).
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdint.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include "lesgo_oracle.h"
+
+typedef double complex cplx;
+#define BOGUS (-1234567890.0)
+#define PI_D 3.1415926535897932384626433
+
+static __thread char g_err[256];
+const char *or_last_error(void) { return g_err; }
+static int fail(int code, const char *msg) {
+  snprintf(g_err, sizeof g_err, "%s", msg);
+  return code;
+}
+
+int or_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
+/* ------------------------------------------------------------------------- */
+/* 1-D complex DFT: recursive mixed radix (4, 2, 3, 5, then any prime) with a   */
+/* table of exp(-2 pi i j / N).  Unnormalised; sign -1 forward, +1 backward.   */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  int n;
+  cplx *tw;
+} fplan;
+
+static void fplan_init(fplan *p, int n) {
+  p->n = n;
+  p->tw = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+  for (int j = 0; j < n; ++j) {
+    double a = 2.0 * PI_D * (double)j / (double)n;
+    p->tw[j] = cos(a) - I * sin(a);
+  }
+}
+static void fplan_free(fplan *p) { free(p->tw); p->tw = NULL; }
+
+static inline cplx twid(const fplan *P, long j, int sign) {
+  cplx w = P->tw[j % P->n];
+  return sign < 0 ? w : conj(w);
+}
+
+static void dft_rec(const fplan *P, int sign, cplx *out, const cplx *in, int n,
+                    int stride, int s) {
+  if (n == 1) { out[0] = in[0]; return; }
+  int p;
+  if (n % 4 == 0) p = 4;
+  else if (n % 2 == 0) p = 2;
+  else if (n % 3 == 0) p = 3;
+  else if (n % 5 == 0) p = 5;
+  else {
+    p = n;
+    for (int f = 7; f * f <= n; f += 2) if (n % f == 0) { p = f; break; }
+  }
+  int m = n / p;
+  for (int r = 0; r < p; ++r) dft_rec(P, sign, out + r * m, in + r * stride, m, stride * p, s * p);
+  cplx tbuf[64];
+  cplx *t = p <= 64 ? tbuf : (cplx *)malloc(sizeof(cplx) * (size_t)p);
+  cplx obuf[64];
+  cplx *o = p <= 64 ? obuf : (cplx *)malloc(sizeof(cplx) * (size_t)p);
+  for (int k = 0; k < m; ++k) {
+    for (int r = 0; r < p; ++r) {
+      cplx y = out[r * m + k];
+      t[r] = (r == 0 || k == 0) ? y : y * twid(P, (long)s * r * k, sign);
+    }
+    for (int q = 0; q < p; ++q) {
+      cplx acc = t[0];
+      for (int r = 1; r < p; ++r) acc += t[r] * twid(P, (long)s * m * ((long)r * q % p), sign);
+      o[q] = acc;
+    }
+    for (int q = 0; q < p; ++q) out[q * m + k] = o[q];
+  }
+  if (t != tbuf) free(t);
+  if (o != obuf) free(o);
+}
+
+static void dft(const fplan *P, int sign, cplx *out, const cplx *in) {
+  dft_rec(P, sign, out, in, P->n, 1, 1);
+}
+
+void or_cfft1(int n, int sign, const double *in_c, double *out_c) {
+  fplan P;
+  fplan_init(&P, n);
+  cplx *tmp = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+  memcpy(tmp, in_c, sizeof(cplx) * (size_t)n);
+  dft(&P, sign, (cplx *)out_c, tmp);
+  free(tmp);
+  fplan_free(&P);
+}
+
+/* ------------------------------------------------------------------------- */
+/* context                                                                    */
+/* ------------------------------------------------------------------------- */
+struct or_ctx {
+  lp_config cfg;
+  lp_grid g;
+  fplan px, py, px2, py2;
+  double *kx, *ky, *k2, *G_test, *G_test_test;
+  double *f[LP_NFIELDS];
+  size_t n[LP_NFIELDS];
+  int jt, jt_total;
+  int F_LM_MM_init, F_QN_NN_init;
+  /* persistent scratch */
+  double *S[6];
+  double *big[7];
+};
+
+static size_t plane(const or_ctx *c) { return (size_t)c->g.ld * c->g.ny; }
+static size_t vol(const or_ctx *c) { return plane(c) * c->g.nz; }
+
+or_ctx *or_create(const lp_config *cfg) {
+  or_ctx *c = (or_ctx *)calloc(1, sizeof(or_ctx));
+  c->cfg = *cfg;
+  c->g = cfg->grid;
+  const lp_grid *g = &c->g;
+  fplan_init(&c->px, g->nx);
+  fplan_init(&c->py, g->ny);
+  fplan_init(&c->px2, g->nx2);
+  fplan_init(&c->py2, g->ny2);
+  size_t ph = (size_t)g->lh * g->ny;
+  c->kx = (double *)calloc(ph, sizeof(double));
+  c->ky = (double *)calloc(ph, sizeof(double));
+  c->k2 = (double *)calloc(ph, sizeof(double));
+  c->G_test = (double *)calloc(ph, sizeof(double));
+  c->G_test_test = (double *)calloc(ph, sizeof(double));
+  or_init_wavenumber(c, c->kx, c->ky, c->k2);
+  /* main.f90:179-183 */
+  or_test_filter_init(c, 2.0 * cfg->filter_size, c->G_test);
+  or_test_filter_init(c, 4.0 * cfg->filter_size, c->G_test_test);
+  int npc = cfg->drop.npcon > 0 ? cfg->drop.npcon : 1;
+  for (int i = 0; i < LP_NFIELDS; ++i) {
+    size_t n = vol(c);
+    if (i == LP_P) n = plane(c) * (g->nz + 1);
+    if (i == LP_USTAR) n = plane(c);
+    if (i == LP_PCON || i == LP_RHS_PCON || i == LP_RHS_PCON_F || i == LP_RHS_PBE)
+      n = vol(c) * npc;
+    c->n[i] = n;
+    c->f[i] = (double *)calloc(n, sizeof(double));
+  }
+  c->jt = 1;
+  c->jt_total = 1;
+  return c;
+}
+
+void or_destroy(or_ctx *c) {
+  if (!c) return;
+  fplan_free(&c->px); fplan_free(&c->py); fplan_free(&c->px2); fplan_free(&c->py2);
+  free(c->kx); free(c->ky); free(c->k2); free(c->G_test); free(c->G_test_test);
+  for (int i = 0; i < LP_NFIELDS; ++i) free(c->f[i]);
+  for (int i = 0; i < 6; ++i) free(c->S[i]);
+  for (int i = 0; i < 7; ++i) free(c->big[i]);
+  free(c);
+}
+
+double *or_field(or_ctx *c, int32_t field) {
+  return (field >= 0 && field < LP_NFIELDS) ? c->f[field] : NULL;
+}
+size_t or_field_elems(or_ctx *c, int32_t field) {
+  return (field >= 0 && field < LP_NFIELDS) ? c->n[field] : 0;
+}
+void or_set_time(or_ctx *c, int32_t jt, int32_t jt_total) { c->jt = jt; c->jt_total = jt_total; }
+void or_get_time(or_ctx *c, int32_t *jt, int32_t *jt_total) { *jt = c->jt; *jt_total = c->jt_total; }
+
+/* ------------------------------------------------------------------------- */
+/* 2-D real plane transforms (FFTW2 rfftwnd_f77_one_* semantics, in-place     */
+/* layout: complex(lh,ny) overlaying real(ld,ny); unnormalised).              */
+/* Forward: r2c along x, then complex along y.  Inverse: complex along y,     */
+/* then c2r along x using the real parts of the kx = 0 and kx = nx/2 terms.   */
+/* ------------------------------------------------------------------------- */
+void or_rfft2_fwd(or_ctx *c, int big, const double *in, int pitch, double *out_c) {
+  const fplan *PX = big ? &c->px2 : &c->px, *PY = big ? &c->py2 : &c->py;
+  int nx = PX->n, ny = PY->n, lh = nx / 2 + 1;
+  cplx *out = (cplx *)out_c;
+  cplx *row = (cplx *)malloc(sizeof(cplx) * (size_t)(nx > ny ? nx : ny));
+  cplx *res = (cplx *)malloc(sizeof(cplx) * (size_t)(nx > ny ? nx : ny));
+  for (int y = 0; y < ny; ++y) {
+    for (int x = 0; x < nx; ++x) row[x] = in[(size_t)y * pitch + x];
+    dft(PX, -1, res, row);
+    for (int k = 0; k < lh; ++k) out[(size_t)y * lh + k] = res[k];
+  }
+  for (int k = 0; k < lh; ++k) {
+    for (int y = 0; y < ny; ++y) row[y] = out[(size_t)y * lh + k];
+    dft(PY, -1, res, row);
+    for (int y = 0; y < ny; ++y) out[(size_t)y * lh + k] = res[y];
+  }
+  free(row);
+  free(res);
+}
+
+void or_rfft2_inv(or_ctx *c, int big, const double *in_c, double *out, int pitch) {
+  const fplan *PX = big ? &c->px2 : &c->px, *PY = big ? &c->py2 : &c->py;
+  int nx = PX->n, ny = PY->n, lh = nx / 2 + 1;
+  const cplx *in = (const cplx *)in_c;
+  cplx *tmp = (cplx *)malloc(sizeof(cplx) * (size_t)lh * ny);
+  int nmax = nx > ny ? nx : ny;
+  cplx *row = (cplx *)malloc(sizeof(cplx) * (size_t)nmax);
+  cplx *res = (cplx *)malloc(sizeof(cplx) * (size_t)nmax);
+  for (int k = 0; k < lh; ++k) {
+    for (int y = 0; y < ny; ++y) row[y] = in[(size_t)y * lh + k];
+    dft(PY, +1, res, row);
+    for (int y = 0; y < ny; ++y) tmp[(size_t)y * lh + k] = res[y];
+  }
+  for (int y = 0; y < ny; ++y) {
+    const cplx *t = tmp + (size_t)y * lh;
+    row[0] = creal(t[0]);
+    row[nx / 2] = creal(t[nx / 2]);
+    for (int k = 1; k < nx / 2; ++k) {
+      row[k] = t[k];
+      row[nx - k] = conj(t[k]);
+    }
+    dft(PX, +1, res, row);
+    for (int x = 0; x < nx; ++x) out[(size_t)y * pitch + x] = creal(res[x]);
+    for (int x = nx; x < pitch; ++x) out[(size_t)y * pitch + x] = 0.0;
+  }
+  free(tmp);
+  free(row);
+  free(res);
+}
+
+/* ------------------------------------------------------------------------- */
+/* fft.f90:51-76  init_wavenumber                                             */
+/* ------------------------------------------------------------------------- */
+void or_init_wavenumber(or_ctx *c, double *kx, double *ky, double *k2) {
+  const lp_grid *g = &c->g;
+  int lh = g->lh, ny = g->ny;
+  for (int jy = 0; jy < ny; ++jy)
+    for (int jx = 0; jx < lh; ++jx) {
+      size_t i = (size_t)jy * lh + jx;
+      double vx = (jx < lh - 1) ? (double)jx : 0.0;                       /* :56-58 */
+      int m = ((jy + ny / 2) % ny) - ny / 2;                              /* :60-62 */
+      double vy = (double)m;
+      if (jx == lh - 1) { vx = 0.0; vy = 0.0; }                           /* :65-66 */
+      if (jy == ny / 2) { vx = 0.0; vy = 0.0; }                           /* :67-68 */
+      kx[i] = 2.0 * PI_D / g->L_x * vx;                                   /* :71-72 */
+      ky[i] = 2.0 * PI_D / g->L_y * vy;
+      k2[i] = kx[i] * kx[i] + ky[i] * ky[i];                              /* :75 */
+    }
+}
+
+/* test_filtermodule.f90:29-57 (ifilter = 1, sharp spectral cutoff) */
+void or_test_filter_init(or_ctx *c, double alpha, double *G) {
+  const lp_grid *g = &c->g;
+  int lh = g->lh, ny = g->ny;
+  double delta = alpha * sqrt(g->dx * g->dy);
+  double kc2 = (PI_D / delta) * (PI_D / delta);
+  for (int jy = 0; jy < ny; ++jy)
+    for (int jx = 0; jx < lh; ++jx) {
+      size_t i = (size_t)jy * lh + jx;
+      double v = 1.0 / ((double)g->nx * g->ny);
+      if (c->k2[i] >= kc2) v = 0.0;
+      if (jx == lh - 1 || jy == ny / 2) v = 0.0;
+      G[i] = v;
+    }
+}
+
+/* zero the Nyquist row/column of a (lh,ny) spectrum ("oddballs") */
+static void kill_oddballs(cplx *F, int lh, int ny) {
+  for (int y = 0; y < ny; ++y) F[(size_t)y * lh + lh - 1] = 0.0;
+  for (int x = 0; x < lh; ++x) F[(size_t)(ny / 2) * lh + x] = 0.0;
+}
+
+/* forward transform of const*f (a plane with pitch ld) into a fresh spectrum */
+static cplx *fwd_scaled(or_ctx *c, const double *f, double cst) {
+  const lp_grid *g = &c->g;
+  size_t np = plane(c);
+  double *tmp = (double *)malloc(sizeof(double) * np);
+  for (int y = 0; y < g->ny; ++y)
+    for (int x = 0; x < g->ld; ++x) tmp[(size_t)y * g->ld + x] = cst * f[(size_t)y * g->ld + x];
+  cplx *F = (cplx *)malloc(sizeof(cplx) * (size_t)g->lh * g->ny);
+  or_rfft2_fwd(c, 0, tmp, g->ld, (double *)F);
+  free(tmp);
+  return F;
+}
+
+/* filt_da.f90:3-36 */
+void or_filt_da(or_ctx *c, double *f, double *dfdx, double *dfdy, int nlev) {
+  const lp_grid *g = &c->g;
+  const double cst = 1.0 / ((double)g->nx * g->ny);
+  size_t np = plane(c), nh = (size_t)g->lh * g->ny;
+#pragma omp parallel for schedule(dynamic)
+  for (int k = 0; k < nlev; ++k) {
+    cplx *F = fwd_scaled(c, f + k * np, cst);
+    kill_oddballs(F, g->lh, g->ny);
+    cplx *Dx = (cplx *)malloc(sizeof(cplx) * nh), *Dy = (cplx *)malloc(sizeof(cplx) * nh);
+    for (size_t i = 0; i < nh; ++i) {
+      Dy[i] = I * c->ky[i] * F[i];
+      Dx[i] = I * c->kx[i] * F[i];
+    }
+    or_rfft2_inv(c, 0, (double *)F, f + k * np, g->ld);
+    or_rfft2_inv(c, 0, (double *)Dx, dfdx + k * np, g->ld);
+    or_rfft2_inv(c, 0, (double *)Dy, dfdy + k * np, g->ld);
+    free(F); free(Dx); free(Dy);
+  }
+}
+
+/* ddx.f90:1-31, ddy.f90:1-31, ddxy.f90:1-29 */
+static void ddxy_impl(or_ctx *c, double *dfdx, double *dfdy, const double *f, int nlev) {
+  const lp_grid *g = &c->g;
+  const double cst = 1.0 / ((double)g->nx * g->ny);
+  size_t np = plane(c), nh = (size_t)g->lh * g->ny;
+#pragma omp parallel for schedule(dynamic)
+  for (int k = 0; k < nlev; ++k) {
+    cplx *F = fwd_scaled(c, f + k * np, cst);
+    kill_oddballs(F, g->lh, g->ny);
+    cplx *D = (cplx *)malloc(sizeof(cplx) * nh);
+    if (dfdy) {
+      for (size_t i = 0; i < nh; ++i) D[i] = I * c->ky[i] * F[i];
+      or_rfft2_inv(c, 0, (double *)D, dfdy + k * np, g->ld);
+    }
+    if (dfdx) {
+      for (size_t i = 0; i < nh; ++i) D[i] = I * c->kx[i] * F[i];
+      or_rfft2_inv(c, 0, (double *)D, dfdx + k * np, g->ld);
+    }
+    free(F); free(D);
+  }
+}
+void or_ddx(or_ctx *c, double *dfdx, const double *f, int nlev) { ddxy_impl(c, dfdx, NULL, f, nlev); }
+void or_ddy(or_ctx *c, double *dfdy, const double *f, int nlev) { ddxy_impl(c, NULL, dfdy, f, nlev); }
+void or_ddxy(or_ctx *c, double *dfdx, double *dfdy, const double *f, int nlev) {
+  ddxy_impl(c, dfdx, dfdy, f, nlev);
+}
+
+/* ddz_uv.f90:5-60: dfdz(jz) = (f(jz)-f(jz-1))/dz, jz = 2..nz-1; dfdz(nz) = 0;
+ * level 1 untouched */
+void or_ddz_uv(or_ctx *c, double *dfdz, const double *f) {
+  const lp_grid *g = &c->g;
+  size_t np = plane(c);
+  const double cst = 1.0 / g->dz;
+#pragma omp parallel for
+  for (int k = 1; k < g->nz - 1; ++k)
+    for (int y = 0; y < g->ny; ++y)
+      for (int x = 0; x < g->nx; ++x) {
+        size_t i = k * np + (size_t)y * g->ld + x;
+        dfdz[i] = cst * (f[i] - f[i - np]);
+      }
+  for (size_t i = 0; i < np; ++i) dfdz[(g->nz - 1) * np + i] = 0.0;
+}
+
+/* ddz_w.f90:6-51: dfdz(jz) = (f(jz+1)-f(jz))/dz, jz = 1..nz-1; dfdz(nz) = 0 */
+void or_ddz_w(or_ctx *c, double *dfdz, const double *f) {
+  const lp_grid *g = &c->g;
+  size_t np = plane(c);
+  const double cst = 1.0 / g->dz;
+#pragma omp parallel for
+  for (int k = 0; k < g->nz - 1; ++k)
+    for (int y = 0; y < g->ny; ++y)
+      for (int x = 0; x < g->nx; ++x) {
+        size_t i = k * np + (size_t)y * g->ld + x;
+        dfdz[i] = cst * (f[i + np] - f[i]);
+      }
+  for (size_t i = 0; i < np; ++i) dfdz[(g->nz - 1) * np + i] = 0.0;
+}
+
+/* padd.f90:1-26 on one complex plane */
+static void padd_plane(const lp_grid *g, cplx *ub, const cplx *u) {
+  int lh = g->lh, lhb = g->lh_big, nx = g->nx, ny = g->ny, ny2 = g->ny2;
+  for (size_t i = 0; i < (size_t)lhb * ny2; ++i) ub[i] = 0.0;
+  for (int jy = 0; jy < ny / 2; ++jy)
+    for (int jx = 0; jx < nx / 2; ++jx) ub[(size_t)jy * lhb + jx] = u[(size_t)jy * lh + jx];
+  for (int jy = 1; jy <= ny / 2 - 1; ++jy)
+    for (int jx = 0; jx < nx / 2; ++jx)
+      ub[(size_t)(jy + ny2 - ny / 2) * lhb + jx] = u[(size_t)(jy + ny / 2) * lh + jx];
+}
+
+/* unpadd.f90:1-22 on one complex plane */
+static void unpadd_plane(const lp_grid *g, cplx *cc, const cplx *cb) {
+  int lh = g->lh, lhb = g->lh_big, nx = g->nx, ny = g->ny, ny2 = g->ny2;
+  for (int jy = 0; jy < ny / 2; ++jy)
+    for (int jx = 0; jx < nx / 2; ++jx) cc[(size_t)jy * lh + jx] = cb[(size_t)jy * lhb + jx];
+  for (int jy = 0; jy < ny; ++jy) cc[(size_t)jy * lh + nx / 2] = 0.0;
+  for (int jx = 0; jx < lh; ++jx) cc[(size_t)(ny / 2) * lh + jx] = 0.0;
+  for (int jy = 1; jy <= ny / 2 - 1; ++jy)
+    for (int jx = 0; jx < nx / 2; ++jx)
+      cc[(size_t)(jy + ny / 2) * lh + jx] = cb[(size_t)(jy + ny2 - ny / 2) * lhb + jx];
+}
+
+void or_padd(or_ctx *c, double *u_big, const double *u, int nplanes) {
+  const lp_grid *g = &c->g;
+  for (int k = 0; k < nplanes; ++k)
+    padd_plane(g, (cplx *)u_big + (size_t)k * g->lh_big * g->ny2,
+               (const cplx *)u + (size_t)k * g->lh * g->ny);
+}
+void or_unpadd(or_ctx *c, double *cc, const double *cc_big, int nplanes) {
+  const lp_grid *g = &c->g;
+  for (int k = 0; k < nplanes; ++k)
+    unpadd_plane(g, (cplx *)cc + (size_t)k * g->lh * g->ny,
+                 (const cplx *)cc_big + (size_t)k * g->lh_big * g->ny2);
+}
+
+/* dealias1.f90:1-31: u_big = c2r_big(padd(r2c(u/(nx ny)))) */
+void or_dealias1(or_ctx *c, const double *u, double *u_big, int nlev) {
+  const lp_grid *g = &c->g;
+  const double cst = 1.0 / ((double)g->nx * g->ny);
+  size_t np = plane(c), npb = (size_t)g->ld_big * g->ny2;
+#pragma omp parallel for schedule(dynamic)
+  for (int k = 0; k < nlev; ++k) {
+    cplx *F = fwd_scaled(c, u + k * np, cst);
+    cplx *B = (cplx *)malloc(sizeof(cplx) * (size_t)g->lh_big * g->ny2);
+    padd_plane(g, B, F);
+    or_rfft2_inv(c, 1, (double *)B, u_big + k * npb, g->ld_big);
+    free(F); free(B);
+  }
+}
+
+/* dealias2.f90:1-33: u_big *= 1/(nx2 ny2); u = c2r(unpadd(r2c_big(u_big)));
+ * u_big keeps its scaled spectrum (in-place transform, as in the reference). */
+void or_dealias2(or_ctx *c, double *u, double *u_big, int nlev) {
+  const lp_grid *g = &c->g;
+  const double cst = 1.0 / ((double)g->nx2 * g->ny2);
+  size_t np = plane(c), npb = (size_t)g->ld_big * g->ny2;
+  for (size_t i = 0; i < npb * nlev; ++i) u_big[i] *= cst;
+#pragma omp parallel for schedule(dynamic)
+  for (int k = 0; k < nlev; ++k) {
+    double *ub = u_big + k * npb;
+    or_rfft2_fwd(c, 1, ub, g->ld_big, ub);
+    cplx *F = (cplx *)malloc(sizeof(cplx) * (size_t)g->lh * g->ny);
+    unpadd_plane(g, F, (const cplx *)ub);
+    or_rfft2_inv(c, 0, (double *)F, u + k * np, g->ld);
+    free(F);
+  }
+}
+
+/* test_filtermodule.f90:12-27: in-place filter of one plane, normalisation in G */
+static void test_filter_plane(or_ctx *c, double *f, const double *G) {
+  const lp_grid *g = &c->g;
+  size_t nh = (size_t)g->lh * g->ny;
+  cplx *F = (cplx *)malloc(sizeof(cplx) * nh);
+  or_rfft2_fwd(c, 0, f, g->ld, (double *)F);
+  for (size_t i = 0; i < nh; ++i) F[i] = G[i] * F[i];
+  or_rfft2_inv(c, 0, (double *)F, f, g->ld);
+  free(F);
+}
+void or_test_filter(or_ctx *c, double *f, const double *G, int nplanes) {
+#pragma omp parallel for schedule(dynamic)
+  for (int k = 0; k < nplanes; ++k) test_filter_plane(c, f + (size_t)k * plane(c), G);
+}
+
+/* ------------------------------------------------------------------------- */
+/* convec.f90:9-316 (OCEAN/Stokes drift term zero: U_stokes = 0)             */
+/* ------------------------------------------------------------------------- */
+int or_convec(or_ctx *c) {
+  const lp_grid *g = &c->g;
+  int nz = g->nz;
+  size_t np = plane(c), npb = (size_t)g->ld_big * g->ny2;
+  for (int i = 0; i < 7; ++i)
+    if (!c->big[i]) c->big[i] = (double *)calloc(npb * nz, sizeof(double));
+  double *u1b = c->big[0], *u2b = c->big[1], *u3b = c->big[2];
+  double *w1b = c->big[3], *w2b = c->big[4], *w3b = c->big[5], *ccb = c->big[6];
+  double *u = c->f[LP_U], *v = c->f[LP_V], *w = c->f[LP_W];
+  double *dudy = c->f[LP_DUDY], *dudz = c->f[LP_DUDZ], *dvdx = c->f[LP_DVDX];
+  double *dvdz = c->f[LP_DVDZ], *dwdx = c->f[LP_DWDX], *dwdy = c->f[LP_DWDY];
+  double *cx = c->f[LP_RHSX], *cy = c->f[LP_RHSY], *cz = c->f[LP_RHSZ];
+  /* :76-98 velocities to the 3/2 grid */
+  or_dealias1(c, u, u1b, nz);
+  or_dealias1(c, v, u2b, nz);
+  or_dealias1(c, w, u3b, nz);
+  /* :100-133 vorticity (jz = 1 on uvp nodes) to the 3/2 grid */
+  double *vx = (double *)calloc(np * nz, sizeof(double));
+  double *vy = (double *)calloc(np * nz, sizeof(double));
+  double *vz = (double *)calloc(np * nz, sizeof(double));
+  for (int k = 0; k < nz; ++k)
+    for (size_t i = k * np; i < (k + 1) * np; ++i) {
+      if (k == 0) {
+        vx[i] = 0.5 * (dwdy[i] + dwdy[i + np]) - dvdz[i];
+        vy[i] = dudz[i] - 0.5 * (dwdx[i] + dwdx[i + np]);
+      } else {
+        vx[i] = dwdy[i] - dvdz[i];
+        vy[i] = dudz[i] - dwdx[i];
+      }
+      vz[i] = dvdx[i] - dudy[i];
+    }
+  or_dealias1(c, vx, w1b, nz);
+  or_dealias1(c, vy, w2b, nz);
+  or_dealias1(c, vz, w3b, nz);
+  free(vx); free(vy); free(vz);
+  const double cb = 1.0 / ((double)g->nx2 * g->ny2);
+  for (int comp = 0; comp < 3; ++comp) {
+    double *out = comp == 0 ? cx : comp == 1 ? cy : cz;
+    /* :140-158 (cx), :180-221 (cy), :241-279 (cz) products on the big grid */
+#pragma omp parallel for
+    for (int k = 0; k < nz - 1; ++k)
+      for (size_t i = 0; i < npb; ++i) {
+        size_t a = k * npb + i, b = a + npb, m = a - npb;
+        double val;
+        if (comp == 0) {
+          if (k == 0) val = cb * (u2b[a] * (-w3b[a]) + 0.5 * u3b[b] * (w2b[b]));
+          else val = cb * (u2b[a] * (-w3b[a]) + 0.5 * (u3b[b] * (w2b[b]) + u3b[a] * (w2b[a])));
+        } else if (comp == 1) {
+          if (k == 0) val = cb * (u1b[a] * (w3b[a]) + 0.5 * u3b[b] * (-w1b[b]));
+          else val = cb * (u1b[a] * (w3b[a]) + 0.5 * (u3b[b] * (-w1b[b]) + u3b[a] * (-w1b[a])));
+        } else {
+          if (k == 0) val = 0.0;
+          else val = cb * 0.5 * ((u1b[a] + u1b[m]) * (-w2b[a]) + (u2b[a] + u2b[m]) * (w1b[a]));
+        }
+        ccb[a] = val;
+      }
+    /* :161-170 etc: r2c big, unpadd, c2r small */
+#pragma omp parallel for schedule(dynamic)
+    for (int k = 0; k < nz - 1; ++k) {
+      double *pb = ccb + k * npb;
+      or_rfft2_fwd(c, 1, pb, g->ld_big, pb);
+      cplx *F = (cplx *)malloc(sizeof(cplx) * (size_t)g->lh * g->ny);
+      unpadd_plane(g, F, (const cplx *)pb);
+      or_rfft2_inv(c, 0, (double *)F, out + k * np, g->ld);
+      free(F);
+    }
+    for (size_t i = 0; i < np; ++i) out[(nz - 1) * np + i] = BOGUS; /* :306-308 */
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* wallstress.f90:5-215 (OCEAN_FLAG off, WALL_HOMOG_FLAG off, neutral:        */
+/* psi_m = 0, phi_m = 1 from obukhov, scalars_module.f90:1152-1163)           */
+/* ------------------------------------------------------------------------- */
+int or_wallstress(or_ctx *c) {
+  const lp_grid *g = &c->g;
+  const lp_config *cf = &c->cfg;
+  size_t np = plane(c);
+  double *u = c->f[LP_U], *v = c->f[LP_V];
+  double *txz = c->f[LP_TXZ], *tyz = c->f[LP_TYZ], *dudz = c->f[LP_DUDZ], *dvdz = c->f[LP_DVDZ];
+  double *ust = c->f[LP_USTAR];
+  if (cf->lbc_mom == LP_LBC_STRESS_FREE) {                                  /* :200-205 */
+    for (size_t i = 0; i < np; ++i) { txz[i] = 0; tyz[i] = 0; dudz[i] = 0; dvdz[i] = 0; }
+    return 0;
+  }
+  if (cf->lbc_mom != LP_LBC_WALL) return fail(LP_ERR_ARG, "invalid lbc_mom");
+  double *u1 = (double *)malloc(sizeof(double) * np), *v1 = (double *)malloc(sizeof(double) * np);
+  memcpy(u1, u, sizeof(double) * np);                                       /* :103-104 */
+  memcpy(v1, v, sizeof(double) * np);
+  test_filter_plane(c, u1, c->G_test);                                      /* :139-140 */
+  test_filter_plane(c, v1, c->G_test);
+  const double psi_m = 0.0, phi_m = 1.0;
+  double zl = 0.5 * g->dz - cf->d0;
+  double denom = log(zl / cf->z0) - psi_m;                                  /* :143 */
+  for (int y = 0; y < g->ny; ++y)
+    for (int x = 0; x < g->nx; ++x) {
+      size_t i = (size_t)y * g->ld + x;
+      double u_avg = sqrt(u1[i] * u1[i] + v1[i] * v1[i]);                   /* :145 */
+      double us = u_avg * cf->vonk / denom;                                 /* :162 */
+      double cst = -(us * us) / u_avg;                                      /* :167 */
+      txz[i] = cst * u1[i];
+      tyz[i] = cst * v1[i];
+      double du = us / (zl * cf->vonk) * u[i] / u_avg * phi_m;              /* :176-179 */
+      double dv = us / (zl * cf->vonk) * v[i] / u_avg * phi_m;
+      dudz[i] = (u[i] == 0.0) ? 0.0 : du;                                   /* :182-183 */
+      dvdz[i] = (v[i] == 0.0) ? 0.0 : dv;
+      ust[i] = us;
+    }
+  free(u1); free(v1);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* interpolag_Sdep.f90:2-373                                                  */
+/* ------------------------------------------------------------------------- */
+int or_interpolag_sdep(or_ctx *c) {
+  const lp_grid *g = &c->g;
+  const lp_config *cf = &c->cfg;
+  int nx = g->nx, ny = g->ny, nz = g->nz, ld = g->ld;
+  size_t np = plane(c);
+  size_t fn = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
+  double *FF[4];
+  const int fid[4] = {LP_F_LM, LP_F_MM, LP_F_QN, LP_F_NN};
+#define FFI(i, j, k) (((size_t)(k) * (ny + 2) + (j)) * (nx + 2) + (i))
+  for (int q = 0; q < 4; ++q) {
+    FF[q] = (double *)malloc(sizeof(double) * fn);
+    const double *F = c->f[fid[q]];
+    /* :62-71 interior; :75-105 periodic x, y wrap and z copy (incl. corners) */
+    for (int k = 0; k < nz + 2; ++k) {
+      int ks = k - 1; if (ks < 0) ks = 0; if (ks > nz - 1) ks = nz - 1;
+      for (int j = 0; j < ny + 2; ++j) {
+        int js = (j - 1 + ny) % ny;
+        for (int i = 0; i < nx + 2; ++i) {
+          int is = (i - 1 + nx) % nx;
+          FF[q][FFI(i, j, k)] = F[(size_t)ks * np + (size_t)js * ld + is];
+        }
+      }
+    }
+  }
+  double cs = (double)cf->cs_count;
+  double *ul = c->f[LP_U_LAG], *vl = c->f[LP_V_LAG], *wl = c->f[LP_W_LAG];
+  double *ut = (double *)malloc(sizeof(double) * np * nz), *vt = (double *)malloc(sizeof(double) * np * nz);
+  for (size_t i = 0; i < np * nz; ++i) { ut[i] = ul[i] / cs; vt[i] = vl[i] / cs; }  /* :204-205 */
+  for (size_t i = 0; i < np; ++i) { ul[i] = ut[i]; vl[i] = vt[i]; }                   /* :215,226 */
+  for (int k = 1; k < nz; ++k)                                                         /* :235-248 */
+    for (size_t i = 0; i < np; ++i) {
+      ul[k * np + i] = 0.5 * (ut[k * np + i] + ut[(k - 1) * np + i]);
+      vl[k * np + i] = 0.5 * (vt[k * np + i] + vt[(k - 1) * np + i]);
+    }
+  for (size_t i = 0; i < np * nz; ++i) wl[i] = wl[i] / cs;                             /* :250 */
+  for (size_t i = 0; i < np; ++i) wl[i] = 0.25 * wl[np + i];                          /* :258 */
+  free(ut); free(vt);
+  double *xp = (double *)malloc(sizeof(double) * np * nz);
+  double *yp = (double *)malloc(sizeof(double) * np * nz);
+  double *zp = (double *)malloc(sizeof(double) * np * nz);
+  for (size_t i = 0; i < np * nz; ++i) {                                               /* :263-265 */
+    xp[i] = -ul[i] * cf->dt * cs / g->dx;
+    yp[i] = -vl[i] * cf->dt * cs / g->dy;
+    zp[i] = -wl[i] * cf->dt * cs / g->dz;
+  }
+  for (size_t i = 0; i < np; ++i) {                                                    /* :281-291 */
+    double z2 = zp[np + i];
+    z2 = z2 + fmin(z2, 0.0);
+    z2 = fmin(1.0, z2);
+    z2 = fmax(-1.0, z2);
+    zp[np + i] = z2;
+    zp[i] = z2;
+  }
+  double *Fo[4];
+  for (int q = 0; q < 4; ++q) Fo[q] = c->f[fid[q]];
+#pragma omp parallel for
+  for (int k = 0; k < nz; ++k)                                                          /* :301-365 */
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        size_t s = (size_t)k * np + (size_t)j * ld + i;
+        int jjx = i + 1, jjy = j + 1, jjz = k + 1;
+        int addx = signbit(xp[s]) ? -1 : 1;
+        int addy = signbit(yp[s]) ? -1 : 1;
+        int addz = signbit(zp[s]) ? -1 : 1;
+        int jxa = jjx + addx, jya = jjy + addy, jza = jjz + addz;
+        double cx_ = fabs(xp[s]), cy_ = fabs(yp[s]), cz_ = fabs(zp[s]);
+        double fx = 1.0 - cx_, fy = 1.0 - cy_, fz = 1.0 - cz_;
+        for (int q = 0; q < 4; ++q) {
+          const double *F = FF[q];
+          Fo[q][s] = fx * fy * (F[FFI(jjx, jjy, jjz)] * fz + F[FFI(jjx, jjy, jza)] * cz_)
+                   + fx * cy_ * (F[FFI(jjx, jya, jjz)] * fz + F[FFI(jjx, jya, jza)] * cz_)
+                   + cx_ * fy * (F[FFI(jxa, jjy, jjz)] * fz + F[FFI(jxa, jjy, jza)] * cz_)
+                   + cx_ * cy_ * (F[FFI(jxa, jya, jjz)] * fz + F[FFI(jxa, jya, jza)] * cz_);
+        }
+      }
+#undef FFI
+  for (size_t i = 0; i < np * nz; ++i) { ul[i] = 0; vl[i] = 0; wl[i] = 0; }           /* :367-369 */
+  for (int q = 0; q < 4; ++q) free(FF[q]);
+  free(xp); free(yp); free(zp);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* lagrange_Sdep.f90:4-496 (use_bldg, inflow off)                             */
+/* ------------------------------------------------------------------------- */
+static inline double r4(double x) { return (double)(float)x; }
+
+int or_lagrange_sdep(or_ctx *c) {
+  const lp_grid *g = &c->g;
+  const lp_config *cf = &c->cfg;
+  int nx = g->nx, ny = g->ny, nz = g->nz, ld = g->ld;
+  size_t np = plane(c);
+  const int casts = cf->fp32_casts;
+  const double tiny = casts ? (double)1e-24f : 1e-24;
+  double delta = cf->filter_size * pow(g->dx * g->dy * g->dz, 1.0 / 3.0);  /* :75 */
+  const double tf1 = 2.0, tf2 = 4.0, tf1_2 = tf1 * tf1, tf2_2 = tf2 * tf2;
+  double opftdelta = cf->opftime * delta;
+  const double powcoeff = -1.0 / 8.0;
+  int rc = or_interpolag_sdep(c);                                            /* :120 */
+  if (rc) return rc;
+  double lagran_dt = cf->dt * (double)cf->cs_count;                          /* :123 */
+  const double cst = 2.0 * delta * delta;                                    /* :125 */
+  const int do_init_lm = cf->inilag && !c->F_LM_MM_init &&
+                         (c->jt == cf->cs_count || c->jt == cf->dyn_init);   /* :277-278 */
+  const int do_init_qn = cf->inilag && !c->F_QN_NN_init &&
+                         (c->jt == cf->cs_count || c->jt == cf->dyn_init);   /* :365-366 */
+  double *u = c->f[LP_U], *v = c->f[LP_V], *w = c->f[LP_W];
+  double *F_LM = c->f[LP_F_LM], *F_MM = c->f[LP_F_MM], *F_QN = c->f[LP_F_QN], *F_NN = c->f[LP_F_NN];
+  double *Cs = c->f[LP_CS_OPT2], *Beta = c->f[LP_BETA];
+  double **S = c->S;
+  const double bexp = log(tf1) / (log(tf2) - log(tf1));
+#pragma omp parallel for schedule(dynamic)
+  for (int k = 0; k < nz; ++k) {
+    double *buf = (double *)malloc(sizeof(double) * np * 45);
+    double *ub = buf, *vb = buf + np, *wb = buf + 2 * np;
+    double *uh = buf + 3 * np, *vh = buf + 4 * np, *wh = buf + 5 * np;
+    double *L[6], *Q[6], *Sb[6], *Sh[6], *SSb[6], *SSh[6];
+    for (int q = 0; q < 6; ++q) {
+      L[q] = buf + (6 + q) * np; Q[q] = buf + (12 + q) * np;
+      Sb[q] = buf + (18 + q) * np; Sh[q] = buf + (24 + q) * np;
+      SSb[q] = buf + (30 + q) * np; SSh[q] = buf + (36 + q) * np;
+    }
+    double *Sm = buf + 42 * np;
+    size_t o = (size_t)k * np;
+    for (size_t i = 0; i < np; ++i) {                                          /* :129-140 */
+      if (k == 0) { ub[i] = u[i]; vb[i] = v[i]; wb[i] = 0.25 * w[np + i]; }
+      else {
+        ub[i] = 0.5 * (u[o + i] + u[o - np + i]);
+        vb[i] = 0.5 * (v[o + i] + v[o - np + i]);
+        wb[i] = w[o + i];
+      }
+      uh[i] = ub[i]; vh[i] = vb[i]; wh[i] = wb[i];                               /* :141-143 */
+      L[0][i] = ub[i] * ub[i]; L[1][i] = ub[i] * vb[i]; L[2][i] = ub[i] * wb[i];  /* :144-149 */
+      L[3][i] = vb[i] * vb[i]; L[4][i] = vb[i] * wb[i]; L[5][i] = wb[i] * wb[i];
+      for (int q = 0; q < 6; ++q) Q[q][i] = L[q][i];                              /* :150-155 */
+    }
+    /* component order q: 0=11 1=12 2=13 3=22 4=23 5=33 */
+    test_filter_plane(c, ub, c->G_test); test_filter_plane(c, vb, c->G_test);
+    test_filter_plane(c, wb, c->G_test);
+    for (int q = 0; q < 6; ++q) test_filter_plane(c, L[q], c->G_test);
+    test_filter_plane(c, uh, c->G_test_test); test_filter_plane(c, vh, c->G_test_test);
+    test_filter_plane(c, wh, c->G_test_test);
+    for (int q = 0; q < 6; ++q) test_filter_plane(c, Q[q], c->G_test_test);
+    for (size_t i = 0; i < np; ++i) {                                          /* :162-189 */
+      L[0][i] -= ub[i] * ub[i]; L[1][i] -= ub[i] * vb[i]; L[2][i] -= ub[i] * wb[i];
+      L[3][i] -= vb[i] * vb[i]; L[4][i] -= vb[i] * wb[i]; L[5][i] -= wb[i] * wb[i];
+      Q[0][i] -= uh[i] * uh[i]; Q[1][i] -= uh[i] * vh[i]; Q[2][i] -= uh[i] * wh[i];
+      Q[3][i] -= vh[i] * vh[i]; Q[4][i] -= vh[i] * wh[i]; Q[5][i] -= wh[i] * wh[i];
+    }
+    for (size_t i = 0; i < np; ++i) {                                          /* :192-242 */
+      double s11 = S[0][o + i], s12 = S[1][o + i], s13 = S[2][o + i];
+      double s22 = S[3][o + i], s23 = S[4][o + i], s33 = S[5][o + i];
+      double sm = sqrt(2.0 * (s11 * s11 + s22 * s22 + s33 * s33 +
+                              2.0 * (s12 * s12 + s13 * s13 + s23 * s23)));
+      Sm[i] = sm;
+      for (int q = 0; q < 6; ++q) {
+        Sb[q][i] = S[q][o + i];
+        Sh[q][i] = S[q][o + i];
+        SSb[q][i] = sm * S[q][o + i];
+        SSh[q][i] = SSb[q][i];
+      }
+    }
+    for (int q = 0; q < 6; ++q) test_filter_plane(c, Sb[q], c->G_test);          /* :210-222 */
+    for (int q = 0; q < 6; ++q) test_filter_plane(c, Sh[q], c->G_test_test);
+    for (int q = 0; q < 6; ++q) test_filter_plane(c, SSb[q], c->G_test);        /* :244-256 */
+    for (int q = 0; q < 6; ++q) test_filter_plane(c, SSh[q], c->G_test_test);
+    double *FLM = F_LM + o, *FMM = F_MM + o, *FQN = F_QN + o, *FNN = F_NN + o;
+    double *cso = Cs + o, *bet = Beta + o;
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < ld; ++x) {
+        size_t i = (size_t)y * ld + x;
+        double sbar = sqrt(2.0 * (Sb[0][i] * Sb[0][i] + Sb[3][i] * Sb[3][i] + Sb[5][i] * Sb[5][i] +
+                                  2.0 * (Sb[1][i] * Sb[1][i] + Sb[2][i] * Sb[2][i] + Sb[4][i] * Sb[4][i])));
+        double shat = sqrt(2.0 * (Sh[0][i] * Sh[0][i] + Sh[3][i] * Sh[3][i] + Sh[5][i] * Sh[5][i] +
+                                  2.0 * (Sh[1][i] * Sh[1][i] + Sh[2][i] * Sh[2][i] + Sh[4][i] * Sh[4][i])));
+        double M[6], N[6];
+        for (int q = 0; q < 6; ++q) {                                          /* :258-270 */
+          M[q] = cst * (SSb[q][i] - tf1_2 * sbar * Sb[q][i]);
+          N[q] = cst * (SSh[q][i] - tf2_2 * shat * Sh[q][i]);
+        }
+        double LM = L[0][i] * M[0] + L[3][i] * M[3] + L[5][i] * M[5] +           /* :272-275 */
+                    2.0 * (L[1][i] * M[1] + L[2][i] * M[2] + L[4][i] * M[4]);
+        double MM = M[0] * M[0] + M[3] * M[3] + M[5] * M[5] +
+                    2.0 * (M[1] * M[1] + M[2] * M[2] + M[4] * M[4]);
+        double QN = Q[0][i] * N[0] + Q[3][i] * N[3] + Q[5][i] * N[5] +
+                    2.0 * (Q[1][i] * N[1] + Q[2][i] * N[2] + Q[4][i] * N[4]);
+        double NN = N[0] * N[0] + N[3] * N[3] + N[5] * N[5] +
+                    2.0 * (N[1] * N[1] + N[2] * N[2] + N[4] * N[4]);
+        int padc = (x >= ld - 2);
+        if (do_init_lm) {                                                       /* :279-290 */
+          FMM[i] = MM;
+          FLM[i] = 0.0;
+          if (padc) { FMM[i] = 1.0; FLM[i] = 1.0; }
+        }
+        double Tn, dum, epsi;
+        if (casts) {                                                            /* :311-315 */
+          Tn = (double)fmaxf((float)(FLM[i] * FMM[i]), 1e-24f);
+          Tn = opftdelta * pow(Tn, powcoeff);
+          Tn = (double)fmaxf(1e-24f, (float)Tn);
+        } else {
+          Tn = fmax(FLM[i] * FMM[i], tiny);
+          Tn = opftdelta * pow(Tn, powcoeff);
+          Tn = fmax(tiny, Tn);
+        }
+        dum = lagran_dt / Tn;
+        epsi = dum / (1.0 + dum);
+        FLM[i] = (epsi * LM + (1.0 - epsi) * FLM[i]);                          /* :324-325 */
+        FMM[i] = (epsi * MM + (1.0 - epsi) * FMM[i]);
+        FLM[i] = casts ? (double)fmaxf(1e-24f, (float)FLM[i]) : fmax(tiny, FLM[i]); /* :329 */
+        double cs2d = FLM[i] / FMM[i];                                           /* :358-363 */
+        if (padc) cs2d = tiny;
+        cs2d = casts ? r4(fmax(tiny, r4(cs2d))) : fmax(tiny, cs2d);
+        if (do_init_qn) {                                                       /* :366-379 */
+          FNN[i] = NN;
+          FQN[i] = 0.03 * NN;
+          if (padc) { FNN[i] = 1.0; FQN[i] = 1.0; }
+        }
+        if (casts) {                                                            /* :386-390 */
+          Tn = (double)fmaxf((float)(FQN[i] * FNN[i]), 1e-24f);
+          Tn = opftdelta * pow(Tn, powcoeff);
+          Tn = (double)fmaxf(1e-24f, (float)Tn);
+        } else {
+          Tn = fmax(FQN[i] * FNN[i], tiny);
+          Tn = opftdelta * pow(Tn, powcoeff);
+          Tn = fmax(tiny, Tn);
+        }
+        dum = lagran_dt / Tn;
+        epsi = dum / (1.0 + dum);
+        FQN[i] = (epsi * QN + (1.0 - epsi) * FQN[i]);                          /* :392-393 */
+        FNN[i] = (epsi * NN + (1.0 - epsi) * FNN[i]);
+        FQN[i] = casts ? (double)fmaxf(1e-24f, (float)FQN[i]) : fmax(tiny, FQN[i]); /* :397 */
+        double cs4d = FQN[i] / FNN[i];                                           /* :403-407 */
+        if (padc) cs4d = tiny;
+        cs4d = casts ? r4(fmax(tiny, r4(cs4d))) : fmax(tiny, cs4d);
+        double b = pow(cs4d / cs2d, bexp);                                       /* :409-410 */
+        if (k == nz - 1) b = 1.0;                                                /* :425-428 */
+        bet[i] = b;
+        double bclip = casts ? (double)fmaxf((float)b, (float)(1.0 / (tf1 * tf2)))  /* :430 */
+                             : fmax(b, 1.0 / (tf1 * tf2));
+        double csv = cs2d / bclip;                                               /* :431-434 */
+        if (padc) csv = tiny;
+        csv = casts ? (double)fmaxf(1e-24f, (float)csv) : fmax(tiny, csv);
+        cso[i] = csv;
+      }
+    free(buf);
+  }
+  if (do_init_lm) c->F_LM_MM_init = 1;                                        /* :292 */
+  if (do_init_qn) c->F_QN_NN_init = 1;                                        /* :380 */
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* sgs_stag.f90:3-401 (sgs on, molec off, OCEAN/SEABED off)                   */
+/* ------------------------------------------------------------------------- */
+int or_sgs_stag(or_ctx *c) {
+  const lp_grid *g = &c->g;
+  const lp_config *cf = &c->cfg;
+  int nx = g->nx, ny = g->ny, nz = g->nz, ld = g->ld;
+  size_t np = plane(c), nv = vol(c);
+  for (int q = 0; q < 6; ++q)
+    if (!c->S[q]) c->S[q] = (double *)calloc(nv, sizeof(double));
+  double **S = c->S;
+  double *u = c->f[LP_U], *v = c->f[LP_V], *w = c->f[LP_W];
+  double *dudx = c->f[LP_DUDX], *dudy = c->f[LP_DUDY], *dudz = c->f[LP_DUDZ];
+  double *dvdx = c->f[LP_DVDX], *dvdy = c->f[LP_DVDY], *dvdz = c->f[LP_DVDZ];
+  double *dwdx = c->f[LP_DWDX], *dwdy = c->f[LP_DWDY], *dwdz = c->f[LP_DWDZ];
+  double *txx = c->f[LP_TXX], *txy = c->f[LP_TXY], *txz = c->f[LP_TXZ];
+  double *tyy = c->f[LP_TYY], *tyz = c->f[LP_TYZ], *tzz = c->f[LP_TZZ];
+  double *Cs = c->f[LP_CS_OPT2], *Nu_t = c->f[LP_NU_T], *dissip = c->f[LP_DISSIP], *magS = c->f[LP_MAGS];
+  double delta = cf->filter_size * pow(g->dx * g->dy * g->dz, 1.0 / 3.0);    /* :46 */
+  /* :49-78 wall level on uvp nodes */
+  for (int y = 0; y < ny; ++y)
+    for (int x = 0; x < nx; ++x) {
+      size_t i = (size_t)y * ld + x;
+      double ux = dudx[i], uy = dudy[i], uz = dudz[i];
+      double vx = dvdx[i], vy = dvdy[i], vz = dvdz[i];
+      double wx = 0.5 * (dwdx[i] + dwdx[i + np]);
+      double wy = 0.5 * (dwdy[i] + dwdy[i + np]);
+      double wz = dwdz[i];
+      S[0][i] = ux; S[1][i] = 0.5 * (uy + vx); S[2][i] = 0.5 * (uz + wx);
+      S[3][i] = vy; S[4][i] = 0.5 * (vz + wy); S[5][i] = wz;
+    }
+  /* :105-125 w-nodes */
+#pragma omp parallel for
+  for (int k = 1; k < nz; ++k)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        size_t i = (size_t)k * np + (size_t)y * ld + x, m = i - np;
+        double ux = 0.5 * (dudx[i] + dudx[m]), uy = 0.5 * (dudy[i] + dudy[m]), uz = dudz[i];
+        double vx = 0.5 * (dvdx[i] + dvdx[m]), vy = 0.5 * (dvdy[i] + dvdy[m]), vz = dvdz[i];
+        double wx = dwdx[i], wy = dwdy[i], wz = 0.5 * (dwdz[i] + dwdz[m]);
+        S[0][i] = ux; S[1][i] = 0.5 * (uy + vx); S[2][i] = 0.5 * (uz + wx);
+        S[3][i] = vy; S[4][i] = 0.5 * (vz + wy); S[5][i] = wz;
+      }
+  double *lz = (double *)malloc(sizeof(double) * nz);
+  if (cf->model == 4 || cf->model == 5) {                                     /* :130-134 */
+    double *ul = c->f[LP_U_LAG], *vl = c->f[LP_V_LAG], *wl = c->f[LP_W_LAG];
+    for (size_t i = 0; i < nv; ++i) { ul[i] += u[i]; vl[i] += v[i]; wl[i] += w[i]; }
+  }
+  if (cf->model == 1) {                                                       /* :137-174 */
+    for (size_t i = 0; i < nv; ++i) Cs[i] = cf->Co * cf->Co;
+    for (int k = 0; k < nz; ++k) {
+      double zz = (k == 0) ? 0.5 * g->dz : (double)k * g->dz;
+      double n = (double)cf->nnn;
+      lz[k] = pow(pow(cf->Co, n) * pow(cf->vonk * zz, -n) + pow(delta, -n), -1.0 / n);
+    }
+  } else if (cf->model == 5) {
+    for (int k = 0; k < nz; ++k) lz[k] = delta;                               /* :177 */
+    if (c->jt == 1 && cf->inilag) {                                           /* :178-180 */
+      for (size_t i = 0; i < nv; ++i) Cs[i] = cf->cs2_init;
+    } else if ((c->jt >= cf->dyn_init || cf->initu) && (c->jt % cf->cs_count) == 0) {
+      int rc = or_lagrange_sdep(c);                                           /* :200-201 */
+      if (rc) { free(lz); return rc; }
+    }
+  } else {
+    free(lz);
+    return fail(LP_ERR_ARG, "unsupported SGS model (1 and 5 are implemented)");
+  }
+  /* :215-237 |S|, Nu_t, dissip, magS on 1..nz */
+#pragma omp parallel for
+  for (int k = 0; k < nz; ++k)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        size_t i = (size_t)k * np + (size_t)y * ld + x;
+        double s11 = S[0][i], s12 = S[1][i], s13 = S[2][i], s22 = S[3][i], s23 = S[4][i], s33 = S[5][i];
+        double s = sqrt(2.0 * (s11 * s11 + s22 * s22 + s33 * s33 + 2.0 * (s12 * s12 + s13 * s13 + s23 * s23)));
+        double l2 = lz[k] * lz[k];
+        Nu_t[i] = s * Cs[i] * l2;
+        dissip[i] = -(s * s * s) * Cs[i] * l2;
+        magS[i] = s;
+      }
+  /* :254-274 wall level */
+  for (int y = 0; y < ny; ++y)
+    for (int x = 0; x < nx; ++x) {
+      size_t i = (size_t)y * ld + x;
+      double cst = Nu_t[i];
+      txx[i] = -2.0 * cst * S[0][i];
+      txy[i] = -2.0 * cst * S[1][i];
+      tyy[i] = -2.0 * cst * S[3][i];
+      tzz[i] = -2.0 * cst * S[5][i];
+    }
+  /* :288-325 */
+#pragma omp parallel for
+  for (int k = 1; k < nz - 1; ++k)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        size_t i = (size_t)k * np + (size_t)y * ld + x, p = i + np;
+        double cst = 0.5 * (Nu_t[i] + Nu_t[p]);
+        txx[i] = -2.0 * cst * 0.5 * (S[0][i] + S[0][p]);
+        txy[i] = -2.0 * cst * 0.5 * (S[1][i] + S[1][p]);
+        tyy[i] = -2.0 * cst * 0.5 * (S[3][i] + S[3][p]);
+        tzz[i] = -2.0 * cst * 0.5 * (S[5][i] + S[5][p]);
+        double cn = Nu_t[i];
+        txz[i] = -2.0 * cn * S[2][i];
+        tyz[i] = -2.0 * cn * S[4][i];
+      }
+  /* :327-331 wall level txz, tyz were set by wallstress and are kept;
+     :391-394 stress-free lid */
+  for (size_t i = 0; i < np; ++i) { txz[(nz - 1) * np + i] = 0.0; tyz[(nz - 1) * np + i] = 0.0; }
+  free(lz);
+  return 0;
+}
+
+/* divstress_uv.f90:2-129 */
+void or_divstress_uv(or_ctx *c, double *divt, const double *tx, const double *ty,
+                     const double *tz) {
+  const lp_grid *g = &c->g;
+  size_t np = plane(c), nv = vol(c);
+  int nz = g->nz, ld = g->ld;
+  double *a = (double *)calloc(nv, sizeof(double)), *b = (double *)calloc(nv, sizeof(double));
+  double *d = (double *)calloc(nv, sizeof(double));
+  or_ddx(c, a, tx, nz);
+  or_ddy(c, b, ty, nz);
+  or_ddz_w(c, d, tz);
+  for (int k = 0; k < nz - 1; ++k)
+    for (int y = 0; y < g->ny; ++y)
+      for (int x = 0; x < ld; ++x) {
+        size_t i = (size_t)k * np + (size_t)y * ld + x;
+        divt[i] = (x >= ld - 2) ? 0.0 : a[i] + b[i] + d[i];
+      }
+  for (size_t i = 0; i < np; ++i) divt[(nz - 1) * np + i] = BOGUS;
+  free(a); free(b); free(d);
+}
+
+/* divstress_w.f90:4-85 */
+void or_divstress_w(or_ctx *c, double *divt, const double *tx, const double *ty,
+                    const double *tz) {
+  const lp_grid *g = &c->g;
+  size_t np = plane(c), nv = vol(c);
+  int nz = g->nz, ld = g->ld;
+  double *a = (double *)calloc(nv, sizeof(double)), *b = (double *)calloc(nv, sizeof(double));
+  double *d = (double *)calloc(nv, sizeof(double));
+  or_ddx(c, a, tx, nz);
+  or_ddy(c, b, ty, nz);
+  or_ddz_uv(c, d, tz);
+  for (int k = 0; k < nz; ++k)
+    for (int y = 0; y < g->ny; ++y)
+      for (int x = 0; x < ld; ++x) {
+        size_t i = (size_t)k * np + (size_t)y * ld + x;
+        double val;
+        if (x >= ld - 2) val = 0.0;
+        else if (k == 0) val = a[i] + b[i];
+        else val = a[i] + b[i] + d[i];
+        divt[i] = val;
+      }
+  free(a); free(b); free(d);
+}
+
+/* ------------------------------------------------------------------------- */
+/* tridag_array.f90:1-163 -- Thomas algorithm per (kx,ky) mode, real          */
+/* coefficients and complex right-hand side; rows j = 1..nz+1.               */
+/* ------------------------------------------------------------------------- */
+int or_tridag_array(or_ctx *c, const double *a, const double *b, const double *cc,
+                    const double *r_, double *u_) {
+  const lp_grid *g = &c->g;
+  int lh = g->lh, ny = g->ny, n = g->nz + 1;
+  size_t nh = (size_t)lh * ny;
+  const cplx *r = (const cplx *)r_;
+  cplx *u = (cplx *)u_;
+  for (int jy = 0; jy < ny; ++jy)                                            /* :32-41 */
+    for (int jx = 0; jx < lh - 1; ++jx)
+      if (b[(size_t)jy * lh + jx] == 0.0) return fail(LP_ERR_PIVOT, "tridag_array: zero pivot in row 1");
+  int err = 0;
+#pragma omp parallel for
+  for (size_t m = 0; m < nh; ++m) {
+    int jx = (int)(m % lh), jy = (int)(m / lh);
+    double bet = b[m];
+    u[m] = r[m] / bet;                                                       /* :43-44 */
+    if (jy == ny / 2 || jx == lh - 1 || (jx == 0 && jy == 0)) continue;      /* :61,65 */
+    double *gam = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    for (int j = 1; j < n; ++j) {                                            /* :57-97 */
+      size_t s = (size_t)j * nh + m;
+      gam[j] = cc[s - nh] / bet;
+      bet = b[s] - a[s] * gam[j];
+      if (bet == 0.0) {
+#pragma omp atomic write
+        err = 1;
+      }
+      u[s] = (r[s] - a[s] * u[s - nh]) / bet;
+    }
+    for (int j = n - 2; j >= 0; --j) {                                       /* :125-155 */
+      size_t s = (size_t)j * nh + m;
+      u[s] = u[s] - gam[j + 1] * u[s + nh];
+    }
+    free(gam);
+  }
+  if (err) return fail(LP_ERR_PIVOT, "tridag_array: zero pivot");
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* press_stag_array.f90:1-361 (non-MPI; lbc = 1 so no surface-temperature    */
+/* correction of the wall row)                                                */
+/* ------------------------------------------------------------------------- */
+int or_press_stag_array(or_ctx *c) {
+  const lp_grid *g = &c->g;
+  const lp_config *cf = &c->cfg;
+  int lh = g->lh, ny = g->ny, nz = g->nz, ld = g->ld;
+  size_t np = plane(c), nh = (size_t)lh * ny;
+  const double tadv1 = 1.5, dz = g->dz;
+  double *u = c->f[LP_U], *v = c->f[LP_V], *w = c->f[LP_W], *divtz = c->f[LP_DIVTZ];
+  double *p = c->f[LP_P], *dpdx = c->f[LP_DPDX], *dpdy = c->f[LP_DPDY];
+  cplx *H[3];
+  for (int q = 0; q < 3; ++q) H[q] = (cplx *)calloc(nh * (nz + 1), sizeof(cplx)); /* levels 0..nz */
+  const double cst = 1.0 / ((double)g->nx * g->ny);
+  double *src[3] = {u, v, w};
+#pragma omp parallel for schedule(dynamic)
+  for (int kk = 0; kk < 3 * (nz - 1); ++kk) {                              /* :75-87 */
+    int q = kk / (nz - 1), jz = kk % (nz - 1) + 1;
+    const double *f = src[q] + (size_t)(jz - 1) * np;
+    double *tmp = (double *)malloc(sizeof(double) * np);
+    for (size_t i = 0; i < np; ++i) tmp[i] = cst / tadv1 * (f[i] / cf->dt);
+    or_rfft2_fwd(c, 0, tmp, ld, (double *)(H[q] + (size_t)jz * nh));
+    free(tmp);
+  }
+  /* :97-103: H_z(nz) = 0 */
+  cplx *bottomw = (cplx *)malloc(sizeof(cplx) * nh), *topw = (cplx *)malloc(sizeof(cplx) * nh);
+  {                                                                         /* :105-122 */
+    double *tmp = (double *)malloc(sizeof(double) * np);
+    for (size_t i = 0; i < np; ++i) tmp[i] = cst * divtz[i];
+    or_rfft2_fwd(c, 0, tmp, ld, (double *)bottomw);
+    for (size_t i = 0; i < np; ++i) tmp[i] = cst * divtz[(size_t)(nz - 1) * np + i];
+    or_rfft2_fwd(c, 0, tmp, ld, (double *)topw);
+    free(tmp);
+  }
+  for (int jz = 1; jz <= nz - 1; ++jz)                                      /* :126-131 */
+    for (int q = 0; q < 3; ++q) kill_oddballs(H[q] + (size_t)jz * nh, lh, ny);
+  kill_oddballs(topw, lh, ny);                                              /* :133-136 */
+  kill_oddballs(bottomw, lh, ny);
+  size_t n1 = nh * (nz + 1);
+  double *a = (double *)calloc(n1, sizeof(double)), *b = (double *)calloc(n1, sizeof(double));
+  double *cc = (double *)calloc(n1, sizeof(double));
+  cplx *R = (cplx *)calloc(n1, sizeof(cplx));
+  for (size_t m = 0; m < nh; ++m) {                                          /* :151-154, :174-177 */
+    a[m] = BOGUS; b[m] = -1.0; cc[m] = 1.0; R[m] = -dz * bottomw[m];
+    size_t t = (size_t)nz * nh + m;
+    a[t] = -1.0; b[t] = 1.0; cc[t] = BOGUS; R[t] = -topw[m] * dz;
+  }
+  for (int jz = 2; jz <= nz; ++jz)                                           /* :229-258 */
+    for (int jy = 0; jy < ny; ++jy) {
+      if (jy == ny / 2) continue;
+      for (int jx = 0; jx < lh - 1; ++jx) {
+        if (jx == 0 && jy == 0) continue;
+        size_t m = (size_t)jy * lh + jx, s = (size_t)(jz - 1) * nh + m;
+        double kxv = c->kx[m], kyv = c->ky[m];
+        a[s] = 1.0 / (dz * dz);
+        b[s] = -(kxv * kxv + kyv * kyv + 2.0 / (dz * dz));
+        cc[s] = 1.0 / (dz * dz);
+        R[s] = I * (kxv * H[0][(size_t)(jz - 1) * nh + m] + kyv * H[1][(size_t)(jz - 1) * nh + m]) +
+               (H[2][(size_t)jz * nh + m] - H[2][(size_t)(jz - 1) * nh + m]) / dz;
+      }
+    }
+  cplx *ph = (cplx *)p;                                                       /* p_hat(lh,ny,0:nz) */
+  int rc = or_tridag_array(c, a, b, cc, (const double *)R, (double *)ph);    /* :290 */
+  if (rc == 0) {
+    ph[0] = 0.0;                                                              /* :303-312 */
+    ph[nh] = ph[0] - dz * bottomw[0];
+    for (int jz = 2; jz <= nz; ++jz) ph[(size_t)jz * nh] = ph[(size_t)(jz - 1) * nh] + H[2][(size_t)jz * nh] * dz;
+    for (int jz = 0; jz <= nz; ++jz) kill_oddballs(ph + (size_t)jz * nh, lh, ny); /* :328-329 */
+    /* :339-352 */
+    cplx *P0 = (cplx *)malloc(sizeof(cplx) * nh);
+    memcpy(P0, ph, sizeof(cplx) * nh);
+    or_rfft2_inv(c, 0, (double *)P0, p, ld);
+    free(P0);
+#pragma omp parallel for schedule(dynamic)
+    for (int jz = 1; jz <= nz - 1; ++jz) {
+      cplx *Ph = (cplx *)malloc(sizeof(cplx) * nh);
+      cplx *Dx = (cplx *)malloc(sizeof(cplx) * nh), *Dy = (cplx *)malloc(sizeof(cplx) * nh);
+      memcpy(Ph, ph + (size_t)jz * nh, sizeof(cplx) * nh);
+      for (size_t m = 0; m < nh; ++m) {
+        Dx[m] = I * c->kx[m] * Ph[m];
+        Dy[m] = I * c->ky[m] * Ph[m];
+      }
+      or_rfft2_inv(c, 0, (double *)Dx, dpdx + (size_t)(jz - 1) * np, ld);
+      or_rfft2_inv(c, 0, (double *)Dy, dpdy + (size_t)(jz - 1) * np, ld);
+      or_rfft2_inv(c, 0, (double *)Ph, p + (size_t)jz * np, ld);
+      free(Ph); free(Dx); free(Dy);
+    }
+    for (size_t i = 0; i < np; ++i) {                                          /* :355-357 */
+      dpdx[(size_t)(nz - 1) * np + i] = BOGUS;
+      dpdy[(size_t)(nz - 1) * np + i] = BOGUS;
+      p[(size_t)nz * np + i] = BOGUS;
+    }
+  }
+  for (int q = 0; q < 3; ++q) free(H[q]);
+  free(bottomw); free(topw); free(a); free(b); free(cc); free(R);
+  return rc;
+}
+
+/* forcing.f90:122-294 project (IBM force zero, inflow off, OCEAN off) */
+int or_project(or_ctx *c) {
+  const lp_grid *g = &c->g;
+  const lp_config *cf = &c->cfg;
+  int nx = g->nx, ny = g->ny, nz = g->nz, ld = g->ld;
+  size_t np = plane(c);
+  const double tadv1 = 1.5;
+  double *u = c->f[LP_U], *v = c->f[LP_V], *w = c->f[LP_W];
+  double *dpdx = c->f[LP_DPDX], *dpdy = c->f[LP_DPDY], *dpdz = c->f[LP_DPDZ];
+  for (int k = 0; k < nz - 1; ++k)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        size_t i = (size_t)k * np + (size_t)y * ld + x;
+        double R = -tadv1 * dpdx[i];
+        u[i] = (u[i] + cf->dt * (R + 0.0));
+        R = -tadv1 * dpdy[i];
+        v[i] = (v[i] + cf->dt * (R + 0.0));
+        if (k >= 1) {
+          R = -tadv1 * dpdz[i];
+          w[i] = (w[i] + cf->dt * (R + 0.0));
+        }
+      }
+  size_t top = (size_t)(nz - 1) * np, bel = (size_t)(nz - 2) * np;
+  if (cf->use_mean_p_force)
+    for (size_t i = 0; i < np; ++i) { u[top + i] = u[bel + i]; v[top + i] = v[bel + i]; }
+  for (size_t i = 0; i < np; ++i) w[top + i] = 0.0;
+  if (cf->lbc_mom == LP_LBC_STRESS_FREE)
+    for (size_t i = 0; i < np; ++i) { u[i] = u[np + i]; v[i] = v[np + i]; }
+  for (size_t i = 0; i < np; ++i) w[i] = 0.0;
+  return 0;
+}
+
+/* scalars_module2.f90:1336-1379 */
+void or_timestep_conditions(or_ctx *c, double *cfl, double *visc_stab) {
+  const lp_grid *g = &c->g;
+  size_t np = plane(c);
+  double *u = c->f[LP_U], *v = c->f[LP_V], *w = c->f[LP_W], *nu = c->f[LP_NU_T];
+  double umax = 0.0, numax = -INFINITY;
+  for (int k = 0; k < g->nz - 1; ++k)
+    for (int y = 0; y < g->ny; ++y)
+      for (int x = 0; x < g->nx; ++x) {
+        size_t i = (size_t)k * np + (size_t)y * g->ld + x;
+        double s = u[i] * u[i] + v[i] * v[i] + w[i] * w[i];
+        if (s > umax) umax = s;
+        if (nu[i] > numax) numax = nu[i];
+      }
+  double delta = fmin(g->dx, fmin(g->dy, g->dz));
+  *cfl = sqrt(umax) * c->cfg.dt / delta;
+  *visc_stab = c->cfg.dt * numax / (delta * delta);
+}
+
+/* rmsdiv.f90:1-44 */
+void or_rmsdiv(or_ctx *c, double *rms) {
+  const lp_grid *g = &c->g;
+  size_t np = plane(c);
+  double *du = c->f[LP_DUDX], *dv = c->f[LP_DVDY], *dw = c->f[LP_DWDZ];
+  double s = 0.0;
+  for (int k = 0; k < g->nz - 1; ++k)
+    for (int y = 0; y < g->ny; ++y)
+      for (int x = 0; x < g->nx; ++x) {
+        size_t i = (size_t)k * np + (size_t)y * g->ld + x;
+        s += fabs(du[i] + dv[i] + dw[i]);
+      }
+  *rms = s / ((double)g->nx * g->ny * (g->nz - 1));
+}
+
+/* ------------------------------------------------------------------------- */
+/* One time step: main.f90:207-826 (+:864-898 when droplets need dudt)       */
+/* ------------------------------------------------------------------------- */
+static int step_once(or_ctx *c) {
+  const lp_grid *g = &c->g;
+  const lp_config *cf = &c->cfg;
+  int nz = g->nz, nx = g->nx, ld = g->ld;
+  size_t np = plane(c), nv = vol(c);
+  double **f = c->f;
+  int rc;
+  memcpy(f[LP_RHSX_F], f[LP_RHSX], nv * sizeof(double));                   /* :252-254 */
+  memcpy(f[LP_RHSY_F], f[LP_RHSY], nv * sizeof(double));
+  memcpy(f[LP_RHSZ_F], f[LP_RHSZ], nv * sizeof(double));
+  or_filt_da(c, f[LP_U], f[LP_DUDX], f[LP_DUDY], nz);                      /* :277-279 */
+  or_filt_da(c, f[LP_V], f[LP_DVDX], f[LP_DVDY], nz);
+  or_filt_da(c, f[LP_W], f[LP_DWDX], f[LP_DWDY], nz);
+  or_ddz_uv(c, f[LP_DUDZ], f[LP_U]);                                        /* :284-288 */
+  or_ddz_uv(c, f[LP_DVDZ], f[LP_V]);
+  or_ddz_w(c, f[LP_DWDZ], f[LP_W]);
+  if ((rc = or_wallstress(c))) return rc;                                   /* :315 */
+  if ((rc = or_sgs_stag(c))) return rc;                                     /* :327 */
+  if (cf->drop.npcon > 0) return fail(LP_ERR_ARG, "droplet step not available in the oracle");
+  or_divstress_uv(c, f[LP_DIVTX], f[LP_TXX], f[LP_TXY], f[LP_TXZ]);         /* :394-397 */
+  or_divstress_uv(c, f[LP_DIVTY], f[LP_TXY], f[LP_TYY], f[LP_TYZ]);
+  or_divstress_w(c, f[LP_DIVTZ], f[LP_TXZ], f[LP_TYZ], f[LP_TZZ]);
+  or_convec(c);                                                             /* :419 */
+  for (size_t i = 0; i < (size_t)(nz - 1) * np; ++i) {                      /* :433-439 */
+    f[LP_RHSX][i] = -f[LP_RHSX][i] - f[LP_DIVTX][i];
+    f[LP_RHSY][i] = -f[LP_RHSY][i] - f[LP_DIVTY][i];
+    f[LP_RHSZ][i] = -f[LP_RHSZ][i] - f[LP_DIVTZ][i];
+    f[LP_DUDT][i] = -f[LP_DIVTX][i];
+    f[LP_DVDT][i] = -f[LP_DIVTY][i];
+    f[LP_DWDT][i] = -f[LP_DIVTZ][i];
+  }
+  double force = cf->use_mean_p_force ? cf->mean_p_force : 0.0;             /* :571-575 */
+  if (c->jt == 1 && !cf->initu) {                                           /* :577-584 */
+    memcpy(f[LP_RHSX_F], f[LP_RHSX], nv * sizeof(double));
+    memcpy(f[LP_RHSY_F], f[LP_RHSY], nv * sizeof(double));
+    memcpy(f[LP_RHSZ_F], f[LP_RHSZ], nv * sizeof(double));
+  }
+  const double tadv1 = 1.5, tadv2 = 1.0 - tadv1;
+  for (size_t i = 0; i < (size_t)(nz - 1) * np; ++i) {                      /* :716-728 */
+    f[LP_U][i] = f[LP_U][i] + cf->dt * (tadv1 * f[LP_RHSX][i] + tadv2 * f[LP_RHSX_F][i] + force);
+    f[LP_V][i] = f[LP_V][i] + cf->dt * (tadv1 * f[LP_RHSY][i] + tadv2 * f[LP_RHSY_F][i]);
+    f[LP_W][i] = f[LP_W][i] + cf->dt * (tadv1 * f[LP_RHSZ][i] + tadv2 * f[LP_RHSZ_F][i]);
+    f[LP_DUDT][i] += cf->mean_p_force;
+  }
+  if ((rc = or_press_stag_array(c))) return rc;                             /* :785 */
+  double *p = f[LP_P], *dpdz = f[LP_DPDZ];
+  for (int k = 0; k < nz - 1; ++k)                                          /* :789-791 */
+    for (int y = 0; y < g->ny; ++y)
+      for (int x = 0; x < ld; ++x) {
+        size_t i = (size_t)k * np + (size_t)y * ld + x;
+        dpdz[i] = (x < nx) ? (p[i + np] - p[i]) / g->dz : 0.0;
+      }
+  for (size_t i = 0; i < np; ++i) dpdz[(size_t)(nz - 1) * np + i] = BOGUS;
+  for (size_t i = 0; i < (size_t)(nz - 1) * np; ++i) {                      /* :795-803 */
+    f[LP_RHSX][i] -= f[LP_DPDX][i];
+    f[LP_RHSY][i] -= f[LP_DPDY][i];
+    f[LP_RHSZ][i] -= f[LP_DPDZ][i];
+    f[LP_DUDT][i] -= f[LP_DPDX][i];
+    f[LP_DVDT][i] -= f[LP_DPDY][i];
+    f[LP_DWDT][i] -= f[LP_DPDZ][i];
+  }
+  if ((rc = or_project(c))) return rc;                                      /* :826 */
+  return 0;
+}
+
+int or_step(or_ctx *c, int nsteps) {
+  for (int s = 0; s < nsteps; ++s) {
+    int rc = step_once(c);
+    if (rc) return rc;
+    c->jt += 1;
+    c->jt_total += 1;
+  }
+  return 0;
+}

@@ -1,0 +1,136 @@
+"""ctypes mirror of include/lesgo_amd.h (structures, enums, field table).
+
+The layout here must match the C header byte for byte; tests/test_abi.py checks the
+sizes against the compiled library.
+"""
+import ctypes as C
+import math
+
+LP_ABI_VERSION = 1
+
+# status codes
+LP_OK, LP_ERR_ARG, LP_ERR_HIP, LP_ERR_PIVOT, LP_ERR_NAN, LP_ERR_NOMEM, LP_ERR_NODEVICE = range(7)
+LP_FP64, LP_MIXED = 0, 1
+LP_LBC_WALL, LP_LBC_STRESS_FREE = 0, 1
+
+FIELD_NAMES = [
+    "u", "v", "w",
+    "dudx", "dudy", "dudz", "dvdx", "dvdy", "dvdz", "dwdx", "dwdy", "dwdz",
+    "RHSx", "RHSy", "RHSz", "RHSx_f", "RHSy_f", "RHSz_f",
+    "txx", "txy", "txz", "tyy", "tyz", "tzz",
+    "divtx", "divty", "divtz",
+    "p",
+    "dpdx", "dpdy", "dpdz",
+    "F_LM", "F_MM", "F_QN", "F_NN",
+    "Cs_opt2", "Beta",
+    "Nu_t", "dissip", "magS",
+    "u_lag", "v_lag", "w_lag",
+    "dudt", "dvdt", "dwdt",
+    "ustar",
+    "PCon", "RHS_PCon", "RHS_PCon_f", "Rhs_pbe",
+    "beta_pcon",
+]
+FIELD = {n: i for i, n in enumerate(FIELD_NAMES)}
+LP_NFIELDS = len(FIELD_NAMES)
+# fields that stay binary64 in an LP_MIXED context
+FIELDS_F64_IN_MIXED = {"p", "F_LM", "F_MM", "F_QN", "F_NN", "PCon", "RHS_PCon",
+                       "RHS_PCon_f", "Rhs_pbe"}
+BOGUS = -1234567890.0
+
+
+class lp_grid(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("ld", C.c_int32), ("lh", C.c_int32),
+        ("nx2", C.c_int32), ("ny2", C.c_int32),
+        ("lh_big", C.c_int32), ("ld_big", C.c_int32),
+        ("pad_", C.c_int32),
+        ("L_x", C.c_double), ("L_y", C.c_double), ("L_z", C.c_double),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+    ]
+
+
+class lp_droplet_config(C.Structure):
+    _fields_ = [
+        ("npcon", C.c_int32), ("pbe", C.c_int32), ("pbe_feedback", C.c_int32),
+        ("settling", C.c_int32), ("inertia", C.c_int32), ("active", C.c_int32),
+        ("inlet_x", C.c_int32), ("inlet_y", C.c_int32),
+        ("src_i", C.c_int32), ("src_j", C.c_int32), ("src_k", C.c_int32),
+        ("pad_", C.c_int32),
+        ("src_sigma", C.c_double), ("Sc", C.c_double),
+        ("rho_c", C.c_double), ("rho_w", C.c_double),
+        ("sigma_s", C.c_double), ("nu_c", C.c_double), ("nu_d", C.c_double),
+        ("k_b", C.c_double), ("d_min", C.c_double),
+        ("g", C.c_double), ("u_star", C.c_double),
+        ("diameter", C.POINTER(C.c_double)),
+        ("settling_vel", C.POINTER(C.c_double)),
+        ("Q_src", C.POINTER(C.c_double)),
+    ]
+
+
+class lp_config(C.Structure):
+    _fields_ = [
+        ("grid", lp_grid),
+        ("dt", C.c_double),
+        ("precision", C.c_int32), ("model", C.c_int32),
+        ("inilag", C.c_int32), ("initu", C.c_int32),
+        ("cs_count", C.c_int32), ("dyn_init", C.c_int32),
+        ("fp32_casts", C.c_int32), ("lbc_mom", C.c_int32),
+        ("z0", C.c_double), ("d0", C.c_double), ("vonk", C.c_double),
+        ("Co", C.c_double),
+        ("nnn", C.c_int32), ("use_mean_p_force", C.c_int32),
+        ("mean_p_force", C.c_double),
+        ("cs2_init", C.c_double), ("opftime", C.c_double), ("filter_size", C.c_double),
+        ("drop", lp_droplet_config),
+    ]
+
+
+def make_grid(nx, ny, nz, L_x=2 * math.pi, L_y=2 * math.pi, L_z=1.0):
+    """Derived sizes exactly as param.f90:57-72 (z_i = 1)."""
+    g = lp_grid()
+    g.nx, g.ny, g.nz = nx, ny, nz
+    g.lh = nx // 2 + 1
+    g.ld = 2 * g.lh
+    g.nx2, g.ny2 = 3 * nx // 2, 3 * ny // 2
+    g.lh_big = g.nx2 // 2 + 1
+    g.ld_big = 2 * g.lh_big
+    g.L_x, g.L_y, g.L_z = L_x, L_y, L_z
+    g.dx, g.dy = L_x / nx, L_y / ny
+    g.dz = L_z / (nz - 1)
+    return g
+
+
+def make_config(nx, ny, nz, L_x=2 * math.pi, L_y=2 * math.pi, L_z=1.0, dt=2e-4,
+                precision=LP_FP64, **kw):
+    """Reference-equivalent defaults: model 5, inilag, cs_count=1, DYN_init=2,
+    wall BC with z0=1e-4, d0=0, neutral; fp32_casts on."""
+    c = lp_config()
+    c.grid = make_grid(nx, ny, nz, L_x, L_y, L_z)
+    c.dt = dt
+    c.precision = precision
+    c.model = 5
+    c.inilag, c.initu = 1, 0
+    c.cs_count, c.dyn_init = 1, 2
+    c.fp32_casts = 1
+    c.lbc_mom = LP_LBC_WALL
+    c.z0, c.d0, c.vonk = 1e-4, 0.0, 0.4
+    c.Co, c.nnn = 0.16, 2
+    c.use_mean_p_force, c.mean_p_force = 0, 1.0
+    c.cs2_init, c.opftime, c.filter_size = 0.03, 1.5, 1.0
+    c.drop.npcon = 0
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise AttributeError(f"lp_config has no field {k!r}")
+        setattr(c, k, v)
+    return c
+
+
+def field_shape(g, name, npcon=0):
+    """numpy shape (C order) of a state field: (levels, ny, ld) etc."""
+    if name == "p":
+        return (g.nz + 1, g.ny, g.ld)
+    if name == "ustar":
+        return (1, g.ny, g.ld)
+    if name in ("PCon", "RHS_PCon", "RHS_PCon_f", "Rhs_pbe"):
+        return (max(npcon, 1), g.nz, g.ny, g.ld)
+    return (g.nz, g.ny, g.ld)

@@ -1,0 +1,57 @@
+// lp_internal.hpp -- host-side interface between the C ABI layer (lp_api.hip) and the
+// kernel launchers (lp_fft.hip, lp_physics.hip).  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace lp {
+
+enum Dt { F32 = 0, F64 = 1 };
+
+// twiddle tables exp(-2 pi i j / N), j < N, in both precisions
+struct Twiddle {
+  int n;
+  const void *d;  // double2[n]
+  const void *f;  // float2[n]
+};
+const void *tw_ptr(int n, Dt dt);  // device pointer; aborts the launch (nullptr) if missing
+
+// multiplier applied per spectral output of a column operation
+enum Mult { M_ONE = 0, M_IKX = 1, M_IKY = 2, M_G1 = 3, M_G2 = 4 };
+
+struct SpecTables {
+  const double *kx, *ky, *G1, *G2;  // (lh, ny) of the small grid
+  int lh, ny;                        // small-grid sizes
+};
+
+// Row pass: real rows of length 2M (scaled by `scale`) -> complex half spectra
+// (M+1 values, only the first ncols_out written).
+hipError_t row_r2c(Dt dt, int M, const void *in, long in_pitch, long in_plane,
+                   int rows, int nplanes, double scale, void *out, long out_pitch,
+                   long out_plane, int ncols_out, hipStream_t s);
+
+// Row pass: complex half spectra (ncols_in columns present, rest zero) -> real rows
+// of length 2M; columns [2M, out_pitch) set to zero.  Compute type dtc, store type dto.
+hipError_t row_c2r(Dt dtc, Dt dto, int M, const void *in, long in_pitch, long in_plane,
+                   int ncols_in, int rows, int nplanes, void *out, long out_pitch,
+                   long out_plane, hipStream_t s);
+
+struct ColOut {
+  void *ptr;
+  int mult;
+};
+
+// Column pass over ncols kx-columns of nplanes spectra:
+//  fwd: forward complex DFT of length nyi along y;
+//  map: padmode 0 (same grid, Nyquist row/column killed, multiplier applied),
+//       1 (zero-pad ny -> ny2 as padd.f90), 2 (truncate ny2 -> ny as unpadd.f90);
+//  inv: inverse DFT of length nyo.
+hipError_t col_op(Dt dt, int nyi, int nyo, int padmode, int fwd, int inv, const void *in,
+                  long in_pitch, long in_plane, int ncols, int nplanes, int nout,
+                  const ColOut *outs, long out_pitch, long out_plane, const SpecTables &tab,
+                  hipStream_t s);
+
+bool fft_len_supported(int n);
+
+}  // namespace lp

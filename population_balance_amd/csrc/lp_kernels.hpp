@@ -1,0 +1,672 @@
+// lp_kernels.hpp -- pointwise / stencil / gather / tridiagonal kernels of the LES step
+// (everything that is not an FFT pass).  Each kernel cites the reference lines it
+// implements.  T is the field type (double for LP_FP64, float for LP_MIXED); the
+// Lagrangian F_* state, the pressure solve and p are always double.
+#pragma once
+#include "lp_device.hpp"
+
+namespace lp {
+
+#define LP_BOGUS (-1234567890.0)
+
+struct Geo {
+  int nx, ny, nz, ld;
+  long np;  // ld*ny
+};
+
+__device__ __forceinline__ long gtid() { return (long)blockIdx.x * blockDim.x + threadIdx.x; }
+
+// ddz_uv.f90:39-57 ; level 1 untouched
+template <typename T>
+__global__ void k_ddz_uv(T *__restrict__ dfdz, const T *__restrict__ f, Geo g, T cst) {
+  long i = gtid();
+  if (i >= g.np * g.nz) return;
+  int k = (int)(i / g.np), x = (int)(i % g.ld);
+  if (k == g.nz - 1) dfdz[i] = T(0);
+  else if (k >= 1 && x < g.nx) dfdz[i] = cst * (f[i] - f[i - g.np]);
+}
+
+// ddz_w.f90:24-44
+template <typename T>
+__global__ void k_ddz_w(T *__restrict__ dfdz, const T *__restrict__ f, Geo g, T cst) {
+  long i = gtid();
+  if (i >= g.np * g.nz) return;
+  int k = (int)(i / g.np), x = (int)(i % g.ld);
+  if (k == g.nz - 1) dfdz[i] = T(0);
+  else if (x < g.nx) dfdz[i] = cst * (f[i + g.np] - f[i]);
+}
+
+// wallstress.f90:101-185 ('wall', neutral psi_m = 0, phi_m = 1, uniform z0, d0)
+template <typename T>
+__global__ void k_wallstress(const T *__restrict__ u1, const T *__restrict__ v1, const T *__restrict__ u,
+                             const T *__restrict__ v, T *__restrict__ txz, T *__restrict__ tyz,
+                             T *__restrict__ dudz, T *__restrict__ dvdz, T *__restrict__ ust, Geo g,
+                             double denom, double zl, double vonk) {
+  long i = gtid();
+  if (i >= g.np) return;
+  int x = (int)(i % g.ld);
+  if (x >= g.nx) return;
+  const double phi_m = 1.0;
+  double a = u1[i], b = v1[i];
+  double u_avg = sqrt(a * a + b * b);
+  double us = u_avg * vonk / denom;
+  double cst = -(us * us) / u_avg;
+  txz[i] = (T)(cst * a);
+  tyz[i] = (T)(cst * b);
+  double uu = u[i], vv = v[i];
+  double du = us / (zl * vonk) * uu / u_avg * phi_m;
+  double dv = us / (zl * vonk) * vv / u_avg * phi_m;
+  dudz[i] = (T)(uu == 0.0 ? 0.0 : du);
+  dvdz[i] = (T)(vv == 0.0 ? 0.0 : dv);
+  ust[i] = (T)us;
+}
+
+template <typename T>
+__global__ void k_fill(T *__restrict__ a, long n, T v) {
+  long i = gtid();
+  if (i < n) a[i] = v;
+}
+
+// sgs_stag.f90:49-125 strain rates: uvp nodes at jz = 1, w-nodes above
+template <typename T>
+__global__ void k_strain(const T *__restrict__ dudx, const T *__restrict__ dudy, const T *__restrict__ dudz,
+                         const T *__restrict__ dvdx, const T *__restrict__ dvdy, const T *__restrict__ dvdz,
+                         const T *__restrict__ dwdx, const T *__restrict__ dwdy, const T *__restrict__ dwdz,
+                         T *__restrict__ S11, T *__restrict__ S12, T *__restrict__ S13, T *__restrict__ S22,
+                         T *__restrict__ S23, T *__restrict__ S33, Geo g) {
+  long i = gtid();
+  if (i >= g.np * g.nz) return;
+  int k = (int)(i / g.np), x = (int)(i % g.ld);
+  if (x >= g.nx) return;
+  T ux, uy, uz, vx, vy, vz, wx, wy, wz;
+  const T h = T(0.5);
+  if (k == 0) {
+    ux = dudx[i]; uy = dudy[i]; uz = dudz[i];
+    vx = dvdx[i]; vy = dvdy[i]; vz = dvdz[i];
+    wx = h * (dwdx[i] + dwdx[i + g.np]);
+    wy = h * (dwdy[i] + dwdy[i + g.np]);
+    wz = dwdz[i];
+  } else {
+    long m = i - g.np;
+    ux = h * (dudx[i] + dudx[m]); uy = h * (dudy[i] + dudy[m]); uz = dudz[i];
+    vx = h * (dvdx[i] + dvdx[m]); vy = h * (dvdy[i] + dvdy[m]); vz = dvdz[i];
+    wx = dwdx[i]; wy = dwdy[i]; wz = h * (dwdz[i] + dwdz[m]);
+  }
+  S11[i] = ux; S12[i] = h * (uy + vx); S13[i] = h * (uz + wx);
+  S22[i] = vy; S23[i] = h * (vz + wy); S33[i] = wz;
+}
+
+// sgs_stag.f90:130-134  u_lag = u_lag + u (all columns)
+template <typename T>
+__global__ void k_lag_acc(T *__restrict__ ul, T *__restrict__ vl, T *__restrict__ wl, const T *__restrict__ u,
+                          const T *__restrict__ v, const T *__restrict__ w, long n) {
+  long i = gtid();
+  if (i >= n) return;
+  ul[i] += u[i];
+  vl[i] += v[i];
+  wl[i] += w[i];
+}
+
+template <typename T> __device__ __forceinline__ T smag(T s11, T s12, T s13, T s22, T s23, T s33) {
+  return sqrt(T(2) * (s11 * s11 + s22 * s22 + s33 * s33 + T(2) * (s12 * s12 + s13 * s13 + s23 * s23)));
+}
+
+// sgs_stag.f90:137-177 (mixing length) and :215-227 (|S|, Nu_t, dissip, magS)
+template <typename T>
+__global__ void k_nut(const T *__restrict__ S11, const T *__restrict__ S12, const T *__restrict__ S13,
+                      const T *__restrict__ S22, const T *__restrict__ S23, const T *__restrict__ S33,
+                      const T *__restrict__ Cs, T *__restrict__ Nu_t, T *__restrict__ dissip,
+                      T *__restrict__ magS, Geo g, int model, double delta, double Co, int nnn,
+                      double vonk, double dz) {
+  long i = gtid();
+  if (i >= g.np * g.nz) return;
+  int k = (int)(i / g.np), x = (int)(i % g.ld);
+  if (x >= g.nx) return;
+  double l;
+  if (model == 1) {
+    double zz = (k == 0) ? 0.5 * dz : (double)k * dz;
+    double n = (double)nnn;
+    l = pow(pow(Co, n) * pow(vonk * zz, -n) + pow(delta, -n), -1.0 / n);
+  } else {
+    l = delta;
+  }
+  T l2 = (T)(l * l);
+  T s = smag(S11[i], S12[i], S13[i], S22[i], S23[i], S33[i]);
+  T c = Cs[i];
+  Nu_t[i] = s * c * l2;
+  dissip[i] = -(s * s * s) * c * l2;
+  magS[i] = s;
+}
+
+// sgs_stag.f90:254-325, 391-394
+template <typename T>
+__global__ void k_tau(const T *__restrict__ S11, const T *__restrict__ S12, const T *__restrict__ S13,
+                      const T *__restrict__ S22, const T *__restrict__ S23, const T *__restrict__ S33,
+                      const T *__restrict__ Nu_t, T *__restrict__ txx, T *__restrict__ txy,
+                      T *__restrict__ txz, T *__restrict__ tyy, T *__restrict__ tyz, T *__restrict__ tzz,
+                      Geo g) {
+  long i = gtid();
+  if (i >= g.np * g.nz) return;
+  int k = (int)(i / g.np), x = (int)(i % g.ld);
+  if (k == g.nz - 1) {
+    txz[i] = T(0);
+    tyz[i] = T(0);
+    return;
+  }
+  if (x >= g.nx) return;
+  const T m2 = T(-2), h = T(0.5);
+  if (k == 0) {
+    T c = Nu_t[i];
+    txx[i] = m2 * c * S11[i];
+    txy[i] = m2 * c * S12[i];
+    tyy[i] = m2 * c * S22[i];
+    tzz[i] = m2 * c * S33[i];
+  } else {
+    long p = i + g.np;
+    T c = h * (Nu_t[i] + Nu_t[p]);
+    txx[i] = m2 * c * h * (S11[i] + S11[p]);
+    txy[i] = m2 * c * h * (S12[i] + S12[p]);
+    tyy[i] = m2 * c * h * (S22[i] + S22[p]);
+    tzz[i] = m2 * c * h * (S33[i] + S33[p]);
+    T cn = Nu_t[i];
+    txz[i] = m2 * cn * S13[i];
+    tyz[i] = m2 * cn * S23[i];
+  }
+}
+
+// interpolag_Sdep.f90:62-365: backtrace x - u dt and trilinear interpolation of the four
+// Lagrangian averages from the periodic (x, y) / clamped (z) copy of F.  Reads the old F,
+// writes the interpolated values to Fi; columns >= nx keep their old values.
+template <typename T>
+__global__ void k_interpolag(const double *__restrict__ F0, const double *__restrict__ F1,
+                             const double *__restrict__ F2, const double *__restrict__ F3,
+                             double *__restrict__ I0, double *__restrict__ I1, double *__restrict__ I2,
+                             double *__restrict__ I3, const T *__restrict__ ul, const T *__restrict__ vl,
+                             const T *__restrict__ wl, Geo g, double cs, double dt, double dx,
+                             double dy, double dz) {
+  long i = gtid();
+  if (i >= g.np * g.nz) return;
+  int k = (int)(i / g.np);
+  int r = (int)(i % g.np);
+  int y = r / g.ld, x = r % g.ld;
+  if (x >= g.nx) {
+    I0[i] = F0[i]; I1[i] = F1[i]; I2[i] = F2[i]; I3[i] = F3[i];
+    return;
+  }
+  // :204-258 lag velocities on w-nodes (u_lag at jz=1 stays on the uvp node)
+  double ut = (double)ul[i] / cs, vt = (double)vl[i] / cs;
+  double ulag, vlag, wlag;
+  if (k == 0) { ulag = ut; vlag = vt; }
+  else {
+    ulag = 0.5 * (ut + (double)ul[i - g.np] / cs);
+    vlag = 0.5 * (vt + (double)vl[i - g.np] / cs);
+  }
+  double xp = -ulag * dt * cs / dx;                    // :263-265
+  double yp = -vlag * dt * cs / dy;
+  double zp;
+  if (k <= 1) {                                        // :281-291
+    long i2 = (long)g.np + r;
+    double w2 = (double)wl[i2] / cs;
+    double z2 = -w2 * dt * cs / dz;
+    z2 = z2 + fmin(z2, 0.0);
+    z2 = fmin(1.0, z2);
+    z2 = fmax(-1.0, z2);
+    zp = z2;
+  } else {
+    wlag = (double)wl[i] / cs;
+    zp = -wlag * dt * cs / dz;
+  }
+  int addx = signbit(xp) ? -1 : 1, addy = signbit(yp) ? -1 : 1, addz = signbit(zp) ? -1 : 1;
+  double cxp = fabs(xp), cyp = fabs(yp), czp = fabs(zp);
+  double fx = 1.0 - cxp, fy = 1.0 - cyp, fz = 1.0 - czp;
+  // FF(i', j', k') = F(wrap(i'-1), wrap(j'-1), clamp(k'-1)); point (x,y,k) -> FF(x+1, y+1, k+1)
+  int xa = x + addx, ya = y + addy, ka = k + addz;
+  xa = (xa + g.nx) % g.nx;
+  ya = (ya + g.ny) % g.ny;
+  ka = ka < 0 ? 0 : (ka > g.nz - 1 ? g.nz - 1 : ka);
+  long o00 = (long)k * g.np + (long)y * g.ld;
+  long o0a = (long)ka * g.np + (long)y * g.ld;
+  long oa0 = (long)k * g.np + (long)ya * g.ld;
+  long oaa = (long)ka * g.np + (long)ya * g.ld;
+  const double *F[4] = {F0, F1, F2, F3};
+  double *O[4] = {I0, I1, I2, I3};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double *f = F[q];
+    double v = fx * fy * (f[o00 + x] * fz + f[o0a + x] * czp)
+             + fx * cyp * (f[oa0 + x] * fz + f[oaa + x] * czp)
+             + cxp * fy * (f[o00 + xa] * fz + f[o0a + xa] * czp)
+             + cxp * cyp * (f[oa0 + xa] * fz + f[oaa + xa] * czp);
+    O[q][i] = v;
+  }
+}
+
+// lagrange_Sdep.f90:129-155,192-242: the 21 fields that are test-filtered per level
+// order: 0-2 u,v,w (w-node values), 3-8 products 11,12,13,22,23,33, 9-14 S_ij, 15-20 |S| S_ij
+template <typename T>
+__global__ void k_lag_prep(const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+                           const T *__restrict__ S11, const T *__restrict__ S12, const T *__restrict__ S13,
+                           const T *__restrict__ S22, const T *__restrict__ S23, const T *__restrict__ S33,
+                           T *const *__restrict__ out, Geo g) {
+  long i = gtid();
+  if (i >= g.np * g.nz) return;
+  int k = (int)(i / g.np), x = (int)(i % g.ld);
+  if (x >= g.nx) return;
+  T ub, vb, wb;
+  if (k == 0) {
+    ub = u[i]; vb = v[i]; wb = T(0.25) * w[i + g.np];
+  } else {
+    ub = T(0.5) * (u[i] + u[i - g.np]);
+    vb = T(0.5) * (v[i] + v[i - g.np]);
+    wb = w[i];
+  }
+  out[0][i] = ub; out[1][i] = vb; out[2][i] = wb;
+  out[3][i] = ub * ub; out[4][i] = ub * vb; out[5][i] = ub * wb;
+  out[6][i] = vb * vb; out[7][i] = vb * wb; out[8][i] = wb * wb;
+  T s[6] = {S11[i], S12[i], S13[i], S22[i], S23[i], S33[i]};
+  T sm = smag(s[0], s[1], s[2], s[3], s[4], s[5]);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    out[9 + q][i] = s[q];
+    out[15 + q][i] = sm * s[q];
+  }
+}
+
+__device__ __forceinline__ double r4max(double a, float b) {  // (double) max(real(a), b)
+  return (double)fmaxf((float)a, b);
+}
+
+// lagrange_Sdep.f90:157-434 after the filters: Germano contractions, Lagrangian
+// relaxation (binary64) with the reference's REAL(4) conversions when casts != 0,
+// scale-dependence beta, Cs^2.  bar[q] = 2-delta filtered field q, hat[q] = 4-delta.
+struct LagParams {
+  double cst, tf1_2, tf2_2, opftdelta, powcoeff, lagran_dt, bexp, tiny, bclip_min;
+  int casts, init_lm, init_qn;
+};
+
+template <typename T>
+__global__ void k_lag_relax(T *const *__restrict__ bar, T *const *__restrict__ hat,
+                            const double *__restrict__ I0, const double *__restrict__ I1,
+                            const double *__restrict__ I2, const double *__restrict__ I3,
+                            double *__restrict__ FLM, double *__restrict__ FMM, double *__restrict__ FQN,
+                            double *__restrict__ FNN, T *__restrict__ Cs, T *__restrict__ Beta, Geo g,
+                            LagParams P) {
+  long i = gtid();
+  if (i >= g.np * g.nz) return;
+  int k = (int)(i / g.np), x = (int)(i % g.ld);
+  const bool padc = x >= g.ld - 2;
+  T ub = bar[0][i], vb = bar[1][i], wb = bar[2][i];
+  T uh = hat[0][i], vh = hat[1][i], wh = hat[2][i];
+  T L[6], Q[6];
+  L[0] = bar[3][i] - ub * ub; L[1] = bar[4][i] - ub * vb; L[2] = bar[5][i] - ub * wb;
+  L[3] = bar[6][i] - vb * vb; L[4] = bar[7][i] - vb * wb; L[5] = bar[8][i] - wb * wb;
+  Q[0] = hat[3][i] - uh * uh; Q[1] = hat[4][i] - uh * vh; Q[2] = hat[5][i] - uh * wh;
+  Q[3] = hat[6][i] - vh * vh; Q[4] = hat[7][i] - vh * wh; Q[5] = hat[8][i] - wh * wh;
+  T Sb[6], Sh[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) { Sb[q] = bar[9 + q][i]; Sh[q] = hat[9 + q][i]; }
+  T sbar = smag(Sb[0], Sb[1], Sb[2], Sb[3], Sb[4], Sb[5]);
+  T shat = smag(Sh[0], Sh[1], Sh[2], Sh[3], Sh[4], Sh[5]);
+  const T cst = (T)P.cst, t1 = (T)P.tf1_2, t2 = (T)P.tf2_2;
+  T M[6], N[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    M[q] = cst * (bar[15 + q][i] - t1 * sbar * Sb[q]);
+    N[q] = cst * (hat[15 + q][i] - t2 * shat * Sh[q]);
+  }
+  double LM = L[0] * M[0] + L[3] * M[3] + L[5] * M[5] + T(2) * (L[1] * M[1] + L[2] * M[2] + L[4] * M[4]);
+  double MM = M[0] * M[0] + M[3] * M[3] + M[5] * M[5] + T(2) * (M[1] * M[1] + M[2] * M[2] + M[4] * M[4]);
+  double QN = Q[0] * N[0] + Q[3] * N[3] + Q[5] * N[5] + T(2) * (Q[1] * N[1] + Q[2] * N[2] + Q[4] * N[4]);
+  double NN = N[0] * N[0] + N[3] * N[3] + N[5] * N[5] + T(2) * (N[1] * N[1] + N[2] * N[2] + N[4] * N[4]);
+  double flm = I0[i], fmm = I1[i], fqn = I2[i], fnn = I3[i];
+  const double tiny = P.tiny;
+  if (P.init_lm) { fmm = MM; flm = 0.0; if (padc) { fmm = 1.0; flm = 1.0; } }
+  double Tn, dum, epsi;
+  if (P.casts) {
+    Tn = (double)fmaxf((float)(flm * fmm), 1e-24f);
+    Tn = P.opftdelta * pow(Tn, P.powcoeff);
+    Tn = (double)fmaxf(1e-24f, (float)Tn);
+  } else {
+    Tn = fmax(flm * fmm, tiny);
+    Tn = P.opftdelta * pow(Tn, P.powcoeff);
+    Tn = fmax(tiny, Tn);
+  }
+  dum = P.lagran_dt / Tn;
+  epsi = dum / (1.0 + dum);
+  flm = (epsi * LM + (1.0 - epsi) * flm);
+  fmm = (epsi * MM + (1.0 - epsi) * fmm);
+  flm = P.casts ? r4max(flm, 1e-24f) : fmax(tiny, flm);
+  double cs2d = flm / fmm;
+  if (padc) cs2d = tiny;
+  cs2d = P.casts ? (double)(float)fmax(tiny, (double)(float)cs2d) : fmax(tiny, cs2d);
+  if (P.init_qn) { fnn = NN; fqn = 0.03 * NN; if (padc) { fnn = 1.0; fqn = 1.0; } }
+  if (P.casts) {
+    Tn = (double)fmaxf((float)(fqn * fnn), 1e-24f);
+    Tn = P.opftdelta * pow(Tn, P.powcoeff);
+    Tn = (double)fmaxf(1e-24f, (float)Tn);
+  } else {
+    Tn = fmax(fqn * fnn, tiny);
+    Tn = P.opftdelta * pow(Tn, P.powcoeff);
+    Tn = fmax(tiny, Tn);
+  }
+  dum = P.lagran_dt / Tn;
+  epsi = dum / (1.0 + dum);
+  fqn = (epsi * QN + (1.0 - epsi) * fqn);
+  fnn = (epsi * NN + (1.0 - epsi) * fnn);
+  fqn = P.casts ? r4max(fqn, 1e-24f) : fmax(tiny, fqn);
+  double cs4d = fqn / fnn;
+  if (padc) cs4d = tiny;
+  cs4d = P.casts ? (double)(float)fmax(tiny, (double)(float)cs4d) : fmax(tiny, cs4d);
+  double b = pow(cs4d / cs2d, P.bexp);
+  if (k == g.nz - 1) b = 1.0;
+  double bclip = P.casts ? (double)fmaxf((float)b, (float)P.bclip_min) : fmax(b, P.bclip_min);
+  double csv = cs2d / bclip;
+  if (padc) csv = tiny;
+  csv = P.casts ? (double)fmaxf(1e-24f, (float)csv) : fmax(tiny, csv);
+  FLM[i] = flm; FMM[i] = fmm; FQN[i] = fqn; FNN[i] = fnn;
+  Cs[i] = (T)csv;
+  Beta[i] = (T)b;
+}
+
+// divstress_uv.f90:116-125 and divstress_w.f90:173-211
+template <typename T>
+__global__ void k_div_combine(T *__restrict__ divt, const T *__restrict__ a, const T *__restrict__ b,
+                              const T *__restrict__ tz, Geo g, T cst, int is_w) {
+  long i = gtid();
+  if (i >= g.np * g.nz) return;
+  int k = (int)(i / g.np), x = (int)(i % g.ld);
+  if (!is_w) {
+    if (k == g.nz - 1) { divt[i] = (T)LP_BOGUS; return; }
+    if (x >= g.ld - 2) { divt[i] = T(0); return; }
+    // ddz_w(tz) at x < nx (padding of the derivative is irrelevant: set to 0 there)
+    T dz = (x < g.nx) ? cst * (tz[i + g.np] - tz[i]) : T(0);
+    divt[i] = a[i] + b[i] + dz;
+  } else {
+    if (x >= g.ld - 2) { divt[i] = T(0); return; }
+    T s = a[i] + b[i];
+    if (k >= 1 && k <= g.nz - 2) s += cst * (tz[i] - tz[i - g.np]);  // ddz_uv; 0 at nz
+    divt[i] = s;
+  }
+}
+
+// convec.f90:100-119 vorticity (jz = 1 special case on uvp nodes)
+template <typename T>
+__global__ void k_vort(const T *__restrict__ dudy, const T *__restrict__ dudz, const T *__restrict__ dvdx,
+                       const T *__restrict__ dvdz, const T *__restrict__ dwdx, const T *__restrict__ dwdy,
+                       T *__restrict__ vx, T *__restrict__ vy, T *__restrict__ vz, Geo g) {
+  long i = gtid();
+  if (i >= g.np * g.nz) return;
+  int k = (int)(i / g.np), x = (int)(i % g.ld);
+  if (x >= g.nx) return;
+  if (k == 0) {
+    vx[i] = T(0.5) * (dwdy[i] + dwdy[i + g.np]) - dvdz[i];
+    vy[i] = dudz[i] - T(0.5) * (dwdx[i] + dwdx[i + g.np]);
+  } else {
+    vx[i] = dwdy[i] - dvdz[i];
+    vy[i] = dudz[i] - dwdx[i];
+  }
+  vz[i] = dvdx[i] - dudy[i];
+}
+
+// convec.f90:138-158, 180-221, 241-279: products on the 3/2 grid, levels 1..nz-1
+template <typename T>
+__global__ void k_convec_prod(const T *__restrict__ u1, const T *__restrict__ u2, const T *__restrict__ u3,
+                              const T *__restrict__ w1, const T *__restrict__ w2, const T *__restrict__ w3,
+                              T *__restrict__ cx, T *__restrict__ cy, T *__restrict__ cz, long npb, int nz,
+                              T cb) {
+  long a = gtid();
+  if (a >= npb * (nz - 1)) return;
+  int k = (int)(a / npb);
+  long b = a + npb;
+  const T h = T(0.5);
+  if (k == 0) {
+    cx[a] = cb * (u2[a] * (-w3[a]) + h * u3[b] * (w2[b]));
+    cy[a] = cb * (u1[a] * (w3[a]) + h * u3[b] * (-w1[b]));
+    cz[a] = T(0);
+  } else {
+    long m = a - npb;
+    cx[a] = cb * (u2[a] * (-w3[a]) + h * (u3[b] * (w2[b]) + u3[a] * (w2[a])));
+    cy[a] = cb * (u1[a] * (w3[a]) + h * (u3[b] * (-w1[b]) + u3[a] * (-w1[a])));
+    cz[a] = cb * h * ((u1[a] + u1[m]) * (-w2[a]) + (u2[a] + u2[m]) * (w1[a]));
+  }
+}
+
+// main.f90:433-439 (+458-460): RHS = -conv - divt (+ beta_pcon); dudt = -divt
+template <typename T>
+__global__ void k_rhs_assemble(T *__restrict__ Rx, T *__restrict__ Ry, T *__restrict__ Rz,
+                               const T *__restrict__ dx_, const T *__restrict__ dy_, const T *__restrict__ dz_,
+                               T *__restrict__ ax, T *__restrict__ ay, T *__restrict__ az,
+                               const T *__restrict__ beta_pcon, long n) {
+  long i = gtid();
+  if (i >= n) return;
+  Rx[i] = -Rx[i] - dx_[i];
+  Ry[i] = -Ry[i] - dy_[i];
+  T rz = -Rz[i] - dz_[i];
+  if (beta_pcon) rz = rz + beta_pcon[i];
+  Rz[i] = rz;
+  ax[i] = -dx_[i];
+  ay[i] = -dy_[i];
+  az[i] = -dz_[i];
+}
+
+// main.f90:716-728 AB2 predictor (+ dudt += mean_p_force)
+template <typename T>
+__global__ void k_ab2(T *__restrict__ u, T *__restrict__ v, T *__restrict__ w, const T *__restrict__ Rx,
+                      const T *__restrict__ Ry, const T *__restrict__ Rz, const T *__restrict__ Fx,
+                      const T *__restrict__ Fy, const T *__restrict__ Fz, T *__restrict__ dudt, long n,
+                      T dt, T force, T mpf) {
+  long i = gtid();
+  if (i >= n) return;
+  const T t1 = T(1.5), t2 = T(1) - T(1.5);
+  u[i] = u[i] + dt * (t1 * Rx[i] + t2 * Fx[i] + force);
+  v[i] = v[i] + dt * (t1 * Ry[i] + t2 * Fy[i]);
+  w[i] = w[i] + dt * (t1 * Rz[i] + t2 * Fz[i]);
+  dudt[i] = dudt[i] + mpf;
+}
+
+// main.f90:789-803: dpdz from p (binary64), RHS -= grad p, dudt -= grad p
+template <typename T>
+__global__ void k_dpdz_rhs(const double *__restrict__ p, T *__restrict__ dpdz, const T *__restrict__ dpdx,
+                           const T *__restrict__ dpdy, T *__restrict__ Rx, T *__restrict__ Ry,
+                           T *__restrict__ Rz, T *__restrict__ ax, T *__restrict__ ay, T *__restrict__ az,
+                           Geo g, double dz) {
+  long i = gtid();
+  long nv = g.np * g.nz;
+  if (i >= nv) return;
+  int k = (int)(i / g.np), x = (int)(i % g.ld);
+  if (k == g.nz - 1) { dpdz[i] = (T)LP_BOGUS; return; }
+  T d = (x < g.nx) ? (T)((p[i + g.np] - p[i]) / dz) : T(0);
+  dpdz[i] = d;
+  Rx[i] -= dpdx[i];
+  Ry[i] -= dpdy[i];
+  Rz[i] -= d;
+  ax[i] -= dpdx[i];
+  ay[i] -= dpdy[i];
+  az[i] -= d;
+}
+
+// forcing.f90:145-191 velocity correction
+template <typename T>
+__global__ void k_project(T *__restrict__ u, T *__restrict__ v, T *__restrict__ w, const T *__restrict__ dpdx,
+                          const T *__restrict__ dpdy, const T *__restrict__ dpdz, Geo g, T dt) {
+  long i = gtid();
+  if (i >= g.np * (g.nz - 1)) return;
+  int k = (int)(i / g.np), x = (int)(i % g.ld);
+  if (x >= g.nx) return;
+  const T t1 = T(1.5);
+  T R = -t1 * dpdx[i];
+  u[i] = (u[i] + dt * (R + T(0)));
+  R = -t1 * dpdy[i];
+  v[i] = (v[i] + dt * (R + T(0)));
+  if (k >= 1) {
+    R = -t1 * dpdz[i];
+    w[i] = (w[i] + dt * (R + T(0)));
+  }
+}
+
+// forcing.f90:215-292 boundary values (all columns)
+template <typename T>
+__global__ void k_project_bc(T *__restrict__ u, T *__restrict__ v, T *__restrict__ w, Geo g, int mean_force,
+                             int stress_free) {
+  long i = gtid();
+  if (i >= g.np) return;
+  long top = (long)(g.nz - 1) * g.np + i, bel = (long)(g.nz - 2) * g.np + i;
+  if (mean_force) { u[top] = u[bel]; v[top] = v[bel]; }
+  w[top] = T(0);
+  if (stress_free) { u[i] = u[g.np + i]; v[i] = v[g.np + i]; }
+  w[i] = T(0);
+}
+
+// press_stag_array.f90:149-329 + tridag_array.f90:30-155, one thread per (kx,ky) mode:
+// assemble the wall row, interior rows and lid row on the fly, binary64 Thomas sweep,
+// gauge-fixed cumulative sum for the mean mode, zero on the Nyquist row and column.
+// H levels 1..nz-1 at planes 0..nz-2 (complex TH, pitch lh), bottomw/topw planes.
+template <typename TH>
+__global__ void k_press_tridiag(const cx<TH> *__restrict__ Hx, const cx<TH> *__restrict__ Hy,
+                                const cx<TH> *__restrict__ Hz, const cx<TH> *__restrict__ bw,
+                                const cx<TH> *__restrict__ tw_, const double *__restrict__ kxt,
+                                const double *__restrict__ kyt, cx<double> *__restrict__ ph,
+                                double *__restrict__ gam, int lh, int ny, int nz, double dz,
+                                int *__restrict__ err) {
+  long m = gtid();
+  long nh = (long)lh * ny;
+  if (m >= nh) return;
+  int jx = (int)(m % lh), jy = (int)(m / lh);
+  auto ld_h = [&](const cx<TH> *H, int level) -> cx<double> {  // H(level), level 1..nz
+    if (level >= nz) return mk<double>(0.0, 0.0);
+    cx<TH> v = H[(long)(level - 1) * nh + m];
+    return mk<double>((double)v.x, (double)v.y);
+  };
+  if (jy == ny / 2 || jx == lh - 1) {
+    for (int j = 0; j <= nz; ++j) ph[(long)j * nh + m] = mk<double>(0.0, 0.0);
+    return;
+  }
+  if (m == 0) {  // press_stag_array.f90:303-312
+    cx<double> acc = mk<double>(0.0, 0.0);
+    ph[0] = acc;
+    cx<TH> b0 = bw[0];
+    acc = mk<double>(acc.x - dz * (double)b0.x, acc.y - dz * (double)b0.y);
+    ph[nh] = acc;
+    for (int jz = 2; jz <= nz; ++jz) {
+      cx<double> hz = ld_h(Hz, jz);
+      acc = mk<double>(acc.x + hz.x * dz, acc.y + hz.y * dz);
+      ph[(long)jz * nh] = acc;
+    }
+    return;
+  }
+  const double kx = kxt[m], ky = kyt[m];
+  const double idz2 = 1.0 / (dz * dz);
+  const double bi = -(kx * kx + ky * ky + 2.0 / (dz * dz));
+  // row 0 (wall): b = -1, c = 1, r = -dz * bottomw
+  double bet = -1.0;
+  cx<TH> b0 = bw[m];
+  cx<double> r0 = mk<double>(-dz * (double)b0.x, -dz * (double)b0.y);
+  cx<double> uprev = mk<double>(r0.x / bet, r0.y / bet);
+  ph[m] = uprev;
+  double cprev = 1.0;
+  bool bad = false;
+  for (int j = 1; j <= nz; ++j) {
+    double a, b, c;
+    cx<double> r;
+    if (j < nz) {
+      a = idz2; b = bi; c = idz2;
+      cx<double> hx = ld_h(Hx, j), hy = ld_h(Hy, j), hz1 = ld_h(Hz, j + 1), hz0 = ld_h(Hz, j);
+      // i*(kx Hx + ky Hy) + (Hz(j+1) - Hz(j))/dz
+      double sx = kx * hx.x + ky * hy.x, sy = kx * hx.y + ky * hy.y;
+      r = mk<double>(-sy + (hz1.x - hz0.x) / dz, sx + (hz1.y - hz0.y) / dz);
+    } else {
+      cx<TH> t = tw_[m];
+      a = -1.0; b = 1.0; c = 0.0;
+      r = mk<double>(-(double)t.x * dz, -(double)t.y * dz);
+    }
+    double gm = cprev / bet;
+    gam[(long)j * nh + m] = gm;
+    bet = b - a * gm;
+    if (bet == 0.0) bad = true;
+    cx<double> uj = mk<double>((r.x - a * uprev.x) / bet, (r.y - a * uprev.y) / bet);
+    ph[(long)j * nh + m] = uj;
+    uprev = uj;
+    cprev = c;
+  }
+  cx<double> unext = uprev;
+  for (int j = nz - 1; j >= 0; --j) {
+    double gm = gam[(long)(j + 1) * nh + m];
+    cx<double> uj = ph[(long)j * nh + m];
+    uj = mk<double>(uj.x - gm * unext.x, uj.y - gm * unext.y);
+    ph[(long)j * nh + m] = uj;
+    unext = uj;
+  }
+  if (bad) atomicExch(err, 1);
+}
+
+// tridag_array.f90:30-155 with explicit a, b, c, r (all binary64); modes skipped as there
+__global__ void k_tridag_arrays(const double *__restrict__ a, const double *__restrict__ b,
+                                const double *__restrict__ c, const cx<double> *__restrict__ r,
+                                cx<double> *__restrict__ u, double *__restrict__ gam, int lh, int ny, int n,
+                                int *__restrict__ err) {
+  long m = gtid();
+  long nh = (long)lh * ny;
+  if (m >= nh) return;
+  int jx = (int)(m % lh), jy = (int)(m / lh);
+  double bet = b[m];
+  cx<double> r0 = r[m];
+  u[m] = mk<double>(r0.x / bet, r0.y / bet);
+  if (jx < lh - 1 && bet == 0.0) atomicExch(err, 1);
+  if (jy == ny / 2 || jx == lh - 1 || m == 0) return;
+  for (int j = 1; j < n; ++j) {
+    long s = (long)j * nh + m;
+    double gm = c[s - nh] / bet;
+    gam[s] = gm;
+    bet = b[s] - a[s] * gm;
+    if (bet == 0.0) atomicExch(err, 1);
+    cx<double> rr = r[s], up = u[s - nh];
+    u[s] = mk<double>((rr.x - a[s] * up.x) / bet, (rr.y - a[s] * up.y) / bet);
+  }
+  for (int j = n - 2; j >= 0; --j) {
+    long s = (long)j * nh + m;
+    cx<double> uj = u[s], un = u[s + nh];
+    double gm = gam[s + nh];
+    u[s] = mk<double>(uj.x - gm * un.x, uj.y - gm * un.y);
+  }
+}
+
+// scalars_module2.f90:1336-1379 and rmsdiv.f90:27-36 partial reductions (levels 1..nz-1)
+template <typename T>
+__global__ void k_reduce_diag(const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+                              const T *__restrict__ nut, const T *__restrict__ du, const T *__restrict__ dv,
+                              const T *__restrict__ dw, Geo g, double *__restrict__ part) {
+  __shared__ double s_u[256], s_n[256], s_d[256];
+  long n = g.np * (g.nz - 1);
+  double mu = 0.0, mn = -INFINITY, sd = 0.0;
+  for (long i = gtid(); i < n; i += (long)gridDim.x * blockDim.x) {
+    int x = (int)(i % g.ld);
+    if (x >= g.nx) continue;
+    double a = u[i], b = v[i], c = w[i];
+    mu = fmax(mu, a * a + b * b + c * c);
+    mn = fmax(mn, (double)nut[i]);
+    sd += fabs((double)du[i] + (double)dv[i] + (double)dw[i]);
+  }
+  s_u[threadIdx.x] = mu; s_n[threadIdx.x] = mn; s_d[threadIdx.x] = sd;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      s_u[threadIdx.x] = fmax(s_u[threadIdx.x], s_u[threadIdx.x + o]);
+      s_n[threadIdx.x] = fmax(s_n[threadIdx.x], s_n[threadIdx.x + o]);
+      s_d[threadIdx.x] += s_d[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x + 0] = s_u[0];
+    part[3 * blockIdx.x + 1] = s_n[0];
+    part[3 * blockIdx.x + 2] = s_d[0];
+  }
+}
+
+template <typename TI, typename TO>
+__global__ void k_convert(const TI *__restrict__ a, TO *__restrict__ b, long n) {
+  long i = gtid();
+  if (i < n) b[i] = (TO)a[i];
+}
+
+}  // namespace lp

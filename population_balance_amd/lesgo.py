@@ -1,0 +1,253 @@
+"""Host-side binding of liblesgo_hip.so (the HIP implementation) via ctypes.
+
+Mirrors the reference's subroutine interface: `filt_da(f, dfdx, dfdy)`,
+`ddx(dfdx, f)`, `press_stag_array()`, ... with the same argument meaning.  Arrays
+may be numpy arrays (host memory, staged by the library) or torch tensors on the
+GPU (device memory, used in place).  There is no CPU fallback: if the library is
+missing or no GPU is visible, every call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblesgo_hip.so")
+_lib = None
+
+
+class LesgoError(RuntimeError):
+    pass
+
+
+def load():
+    """Load liblesgo_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LesgoError(f"{LIB_PATH} is missing: build it with "
+                         "`python -m population_balance_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, i32 = C.c_void_p, C.c_int32
+    sig = {
+        "lp_abi_version": ([], C.c_int),
+        "lp_last_error": ([], C.c_char_p),
+        "lp_grid_init": ([C.POINTER(abi.lp_grid), i32, i32, i32, C.c_double, C.c_double, C.c_double], C.c_int),
+        "lp_config_default": ([C.POINTER(abi.lp_config)], None),
+        "lp_create": ([C.POINTER(abi.lp_config), C.POINTER(vp)], C.c_int),
+        "lp_destroy": ([vp], C.c_int),
+        "lp_sync": ([vp], C.c_int),
+        "lp_field_info": ([vp, i32, C.POINTER(vp), C.POINTER(C.c_size_t), C.POINTER(i32)], C.c_int),
+        "lp_upload": ([vp, i32, vp, C.c_size_t], C.c_int),
+        "lp_download": ([vp, i32, vp, C.c_size_t], C.c_int),
+        "lp_set_time": ([vp, i32, i32], C.c_int),
+        "lp_get_time": ([vp, C.POINTER(i32), C.POINTER(i32)], C.c_int),
+        "lp_init_wavenumber": ([vp, vp, vp, vp], C.c_int),
+        "lp_test_filter_init": ([vp, C.c_double, vp], C.c_int),
+        "lp_filt_da": ([vp, vp, vp, vp, i32], C.c_int),
+        "lp_ddx": ([vp, vp, vp, i32], C.c_int),
+        "lp_ddy": ([vp, vp, vp, i32], C.c_int),
+        "lp_ddxy": ([vp, vp, vp, vp, i32], C.c_int),
+        "lp_ddz_uv": ([vp, vp, vp], C.c_int),
+        "lp_ddz_w": ([vp, vp, vp], C.c_int),
+        "lp_padd": ([vp, vp, vp, i32], C.c_int),
+        "lp_unpadd": ([vp, vp, vp, i32], C.c_int),
+        "lp_dealias1": ([vp, vp, vp, i32], C.c_int),
+        "lp_dealias2": ([vp, vp, vp, i32], C.c_int),
+        "lp_test_filter": ([vp, vp, vp, i32], C.c_int),
+        "lp_divstress_uv": ([vp, vp, vp, vp, vp], C.c_int),
+        "lp_divstress_w": ([vp, vp, vp, vp, vp], C.c_int),
+        "lp_tridag_array": ([vp, vp, vp, vp, vp, vp], C.c_int),
+        "lp_wallstress": ([vp], C.c_int),
+        "lp_sgs_stag": ([vp], C.c_int),
+        "lp_lagrange_sdep": ([vp], C.c_int),
+        "lp_interpolag_sdep": ([vp], C.c_int),
+        "lp_convec": ([vp], C.c_int),
+        "lp_press_stag_array": ([vp], C.c_int),
+        "lp_project": ([vp], C.c_int),
+        "lp_timestep_conditions": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
+        "lp_rmsdiv": ([vp, C.POINTER(C.c_double)], C.c_int),
+        "lp_step": ([vp, i32], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    if L.lp_abi_version() != abi.LP_ABI_VERSION:
+        raise LesgoError("ABI version mismatch between liblesgo_hip.so and abi.py")
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    """device or host address of a numpy array / torch tensor."""
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        if not a.is_contiguous():
+            raise LesgoError("torch tensors must be contiguous")
+        return C.c_void_p(a.data_ptr())
+    if not a.flags["C_CONTIGUOUS"]:
+        raise LesgoError("numpy arrays must be C-contiguous")
+    return C.c_void_p(a.ctypes.data)
+
+
+class Lesgo:
+    """A context on the current HIP device (one stream, device-resident state)."""
+
+    def __init__(self, cfg):
+        self.L = load()
+        self.cfg = cfg
+        self.g = cfg.grid
+        h = C.c_void_p()
+        self._check(self.L.lp_create(C.byref(cfg), C.byref(h)), "lp_create")
+        self.h = h
+        self.fdtype = np.float32 if cfg.precision == abi.LP_MIXED else np.float64
+
+    def _check(self, rc, what):
+        if rc != abi.LP_OK:
+            raise LesgoError(f"{what} failed ({rc}): {self.L.lp_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.lp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state ----
+    def field_info(self, name):
+        p, b, e = C.c_void_p(), C.c_size_t(), C.c_int32()
+        self._check(self.L.lp_field_info(self.h, abi.FIELD[name], C.byref(p), C.byref(b), C.byref(e)),
+                    "lp_field_info")
+        return p.value, b.value, e.value
+
+    def field_dtype(self, name):
+        return np.float64 if self.field_info(name)[2] == 8 else np.float32
+
+    def upload(self, name, arr):
+        _, nbytes, esz = self.field_info(name)
+        a = np.ascontiguousarray(arr, dtype=np.float64 if esz == 8 else np.float32)
+        if a.nbytes != nbytes:
+            raise LesgoError(f"{name}: expected {nbytes} bytes, got {a.nbytes}")
+        self._check(self.L.lp_upload(self.h, abi.FIELD[name], _ptr(a), a.nbytes), "lp_upload")
+
+    def download(self, name):
+        _, nbytes, esz = self.field_info(name)
+        shape = abi.field_shape(self.g, name, self.cfg.drop.npcon)
+        dt = np.float64 if esz == 8 else np.float32
+        n = nbytes // esz
+        out = np.empty(n, dt)
+        self._check(self.L.lp_download(self.h, abi.FIELD[name], _ptr(out), nbytes), "lp_download")
+        return out.reshape(shape) if int(np.prod(shape)) == n else out
+
+    def set_time(self, jt, jt_total=None):
+        self._check(self.L.lp_set_time(self.h, jt, jt if jt_total is None else jt_total), "lp_set_time")
+
+    def time(self):
+        a, b = C.c_int32(), C.c_int32()
+        self._check(self.L.lp_get_time(self.h, C.byref(a), C.byref(b)), "lp_get_time")
+        return a.value, b.value
+
+    def sync(self):
+        self._check(self.L.lp_sync(self.h), "lp_sync")
+
+    # ---- spectral primitives (reference argument order) ----
+    def init_wavenumber(self):
+        g = self.g
+        kx, ky, k2 = (np.zeros((g.ny, g.lh)) for _ in range(3))
+        self._check(self.L.lp_init_wavenumber(self.h, _ptr(kx), _ptr(ky), _ptr(k2)), "init_wavenumber")
+        return kx, ky, k2
+
+    def test_filter_init(self, alpha):
+        G = np.zeros((self.g.ny, self.g.lh))
+        self._check(self.L.lp_test_filter_init(self.h, float(alpha), _ptr(G)), "test_filter_init")
+        return G
+
+    def _nlev(self, a):
+        return int(a.shape[0]) if len(a.shape) == 3 else 1
+
+    def filt_da(self, f, dfdx, dfdy):
+        """in place on f (filtered), writes dfdx, dfdy."""
+        self._check(self.L.lp_filt_da(self.h, _ptr(f), _ptr(dfdx), _ptr(dfdy), self._nlev(f)), "filt_da")
+
+    def ddx(self, dfdx, f):
+        self._check(self.L.lp_ddx(self.h, _ptr(dfdx), _ptr(f), self._nlev(f)), "ddx")
+
+    def ddy(self, dfdy, f):
+        self._check(self.L.lp_ddy(self.h, _ptr(dfdy), _ptr(f), self._nlev(f)), "ddy")
+
+    def ddxy(self, dfdx, dfdy, f):
+        self._check(self.L.lp_ddxy(self.h, _ptr(dfdx), _ptr(dfdy), _ptr(f), self._nlev(f)), "ddxy")
+
+    def ddz_uv(self, dfdz, f):
+        self._check(self.L.lp_ddz_uv(self.h, _ptr(dfdz), _ptr(f)), "ddz_uv")
+
+    def ddz_w(self, dfdz, f):
+        self._check(self.L.lp_ddz_w(self.h, _ptr(dfdz), _ptr(f)), "ddz_w")
+
+    def padd(self, u_big, u):
+        self._check(self.L.lp_padd(self.h, _ptr(u_big), _ptr(u), int(u.shape[0])), "padd")
+
+    def unpadd(self, cc, cc_big):
+        self._check(self.L.lp_unpadd(self.h, _ptr(cc), _ptr(cc_big), int(cc.shape[0])), "unpadd")
+
+    def dealias1(self, u, u_big):
+        self._check(self.L.lp_dealias1(self.h, _ptr(u), _ptr(u_big), self._nlev(u)), "dealias1")
+
+    def dealias2(self, u, u_big):
+        self._check(self.L.lp_dealias2(self.h, _ptr(u), _ptr(u_big), self._nlev(u)), "dealias2")
+
+    def test_filter(self, f, G):
+        self._check(self.L.lp_test_filter(self.h, _ptr(f), _ptr(G), self._nlev(f)), "test_filter")
+
+    def divstress_uv(self, divt, tx, ty, tz):
+        self._check(self.L.lp_divstress_uv(self.h, _ptr(divt), _ptr(tx), _ptr(ty), _ptr(tz)), "divstress_uv")
+
+    def divstress_w(self, divt, tx, ty, tz):
+        self._check(self.L.lp_divstress_w(self.h, _ptr(divt), _ptr(tx), _ptr(ty), _ptr(tz)), "divstress_w")
+
+    def tridag_array(self, a, b, c, r, u):
+        self._check(self.L.lp_tridag_array(self.h, _ptr(a), _ptr(b), _ptr(c), _ptr(r), _ptr(u)), "tridag_array")
+
+    # ---- routines on the state ----
+    def wallstress(self):
+        self._check(self.L.lp_wallstress(self.h), "wallstress")
+
+    def sgs_stag(self):
+        self._check(self.L.lp_sgs_stag(self.h), "sgs_stag")
+
+    def lagrange_sdep(self):
+        self._check(self.L.lp_lagrange_sdep(self.h), "lagrange_sdep")
+
+    def interpolag_sdep(self):
+        self._check(self.L.lp_interpolag_sdep(self.h), "interpolag_sdep")
+
+    def convec(self):
+        self._check(self.L.lp_convec(self.h), "convec")
+
+    def press_stag_array(self):
+        self._check(self.L.lp_press_stag_array(self.h), "press_stag_array")
+
+    def project(self):
+        self._check(self.L.lp_project(self.h), "project")
+
+    def timestep_conditions(self):
+        a, b = C.c_double(), C.c_double()
+        self._check(self.L.lp_timestep_conditions(self.h, C.byref(a), C.byref(b)), "timestep_conditions")
+        return a.value, b.value
+
+    def rmsdiv(self):
+        a = C.c_double()
+        self._check(self.L.lp_rmsdiv(self.h, C.byref(a)), "rmsdiv")
+        return a.value
+
+    def step(self, n=1):
+        self._check(self.L.lp_step(self.h, int(n)), "step")
