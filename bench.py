@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""Throughput of the LES time step (grid-point-steps per second) on one MI355X.
+
+Workload: 256 x 256 x 257 staggered levels, domain 2pi x 2pi x 1, dt = 2e-4, mean
+pressure gradient 1, seeded divergence-free k^-5/3 field + log law, Lagrangian
+scale-dependent SGS (inilag, cs_count = 1, DYN_init = 2), wall-stress bottom,
+stress-free lid.  The headline `value` is the FP32/FP64-mixed path; the FP64 path is
+timed beside it on the same initial state, and the mixed-vs-FP64 relative L2 of u, v,
+w, p after the run is reported.  The CPU baseline is the binary64 oracle (a C
+restatement of the reference) on the host cores, on a 256 x 256 x 33 slab sample.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from population_balance_amd import abi, ic  # noqa: E402
+
+HBM_PEAK_GBPS = 8000.0
+# algorithmic bytes per grid-point-step of the velocity step (each reference routine
+# reads each distinct 3-D input once and writes each 3-D output once)
+B_PT = {"mixed": 636.0, "fp64": 1192.0}
+METRIC = "Mpt-steps/s at 256³ (FP32-mixed vs FP64, ≤1e-5 L2); adjoint step time"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_gpu(cfg, u, v, w, steps, warmup, prof_class=None):
+    from population_balance_amd.lesgo import Lesgo
+    lg = Lesgo(cfg)
+    for k, a in (("u", u), ("v", v), ("w", w)):
+        lg.upload(k, a)
+    lg.step(warmup)
+    lg.sync()
+    if prof_class is not None:
+        lg.prof_enable(prof_class)
+    t0 = time.perf_counter()
+    lg.step(steps)            # returns after the stream has drained
+    t1 = time.perf_counter()
+    prof = None
+    if prof_class is not None:
+        prof = lg.prof_read()
+        lg.prof_enable(-1)
+    return lg, (t1 - t0), prof
+
+
+def class_breakdown(lg, nsteps=2):
+    """untimed: per-class kernel time per step (diagnostic, printed to stderr)."""
+    out = {}
+    for name, cls in abi.KERNEL_CLASSES.items():
+        lg.prof_enable(cls)
+        lg.step(nsteps)
+        ms, n, b = lg.prof_read()
+        out[name] = {"ms_per_step": ms / nsteps, "launches_per_step": n / nsteps,
+                     "GBps": (b / (ms * 1e-3) / 1e9) if ms > 0 else 0.0}
+    lg.prof_enable(-1)
+    return out
+
+
+def cpu_baseline(nx, seed):
+    from oracle import Oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    nz = 33
+    cfg = abi.make_config(nx, nx, nz, use_mean_p_force=1, mean_p_force=1.0)
+    u, v, w = ic.turbulent_ic(cfg.grid, seed=seed)
+    o = Oracle(cfg, threads=threads)
+    o.set_field("u", u); o.set_field("v", v); o.set_field("w", w)
+    o.step(1)                      # jt = 1 (constant Cs^2)
+    t0 = time.perf_counter()
+    o.step(2)                      # jt = 2, 3 (Lagrangian model active)
+    dt = time.perf_counter() - t0
+    pts = nx * nx * (nz - 1) * 2
+    o.close()
+    return {"value": pts / dt / 1e6, "unit": "Mpt-steps/s", "cores": threads, "kind": "port",
+            "sample": f"binary64 oracle (C restatement of the reference path), {nx}x{nx}x{nz} "
+                      f"levels, steps 2-3 (Lagrangian SGS active), {dt:.1f} s wall, "
+                      f"{threads} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nx", type=int, default=256)
+    ap.add_argument("--nz", type=int, default=257)
+    ap.add_argument("--seed", type=int, default=ic.DEFAULT_SEED)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--breakdown", action="store_true", help="per-kernel-class timing to stderr")
+    ap.add_argument("--roofline-class", default="col_op", choices=list(abi.KERNEL_CLASSES))
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(the multi-GPU z-slab design is documented, not built)", file=sys.stderr)
+        sys.exit(2)
+    nx, nz = a.nx, a.nz
+    kw = dict(use_mean_p_force=1, mean_p_force=1.0)
+    cfg_m = abi.make_config(nx, nx, nz, precision=abi.LP_MIXED, **kw)
+    cfg_d = abi.make_config(nx, nx, nz, precision=abi.LP_FP64, **kw)
+    t = time.perf_counter()
+    u, v, w = ic.turbulent_ic(cfg_m.grid, seed=a.seed)
+    log(f"[bench] initial condition {nx}x{nx}x{nz} generated in {time.perf_counter() - t:.1f} s")
+    pts = nx * nx * (nz - 1)
+    cls = abi.KERNEL_CLASSES[a.roofline_class]
+
+    lg_m, t_m, prof = run_gpu(cfg_m, u, v, w, a.steps, a.warmup, prof_class=cls)
+    ms_m = t_m / a.steps * 1e3
+    val_m = pts * a.steps / t_m / 1e6
+    log(f"[bench] mixed: {ms_m:.3f} ms/step, {val_m:.1f} Mpt-steps/s")
+    # fields for the mixed-vs-FP64 comparison, taken before any extra (untimed) steps
+    fields_m = {k: lg_m.download(k) for k in ("u", "v", "w", "p")} if not a.no_fp64 else None
+    kms, klaunch, kbytes = prof
+    k_avg_ms = kms / max(klaunch, 1)
+    k_bytes = kbytes / max(klaunch, 1)
+    achieved = k_bytes / (k_avg_ms * 1e-3) / 1e9 if k_avg_ms > 0 else 0.0
+    breakdown = class_breakdown(lg_m) if a.breakdown else None
+    if breakdown:
+        for k, v_ in breakdown.items():
+            log(f"[bench]   {k:14s} {v_['ms_per_step']:8.3f} ms/step  {v_['launches_per_step']:6.0f} "
+                f"launches  {v_['GBps']:7.0f} GB/s")
+
+    result = {
+        "metric": METRIC, "value": round(val_m, 2), "unit": "Mpt-steps/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_m, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"LES velocity step {nx}x{nx}x{nz} (staggered levels), 2pi x 2pi x 1, "
+                               "dt=2e-4, mean pressure gradient 1, Lagrangian scale-dependent SGS; "
+                               "FFT/convection/SGS binary32, pressure tridiagonal + mean mode + "
+                               "F_LM/MM/QN/NN binary64",
+                   "grid": [nx, nx, nz], "seed": a.seed},
+        "roofline": {"bound": "hbm", "kernel": a.roofline_class, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": None, "avg_launch_ms": round(k_avg_ms, 5),
+                     "bytes_per_launch": k_bytes, "launches": klaunch},
+        "step_roofline": {"bytes_per_pt": B_PT["mixed"],
+                          "achieved_GBps": round(B_PT["mixed"] * pts / (ms_m * 1e-3) / 1e9, 1),
+                          "frac": round(B_PT["mixed"] * pts / (ms_m * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)},
+    }
+    if breakdown:
+        result["kernel_classes"] = breakdown
+    if not a.no_fp64:
+        lg_m.close()
+        lg_d, t_d, _ = run_gpu(cfg_d, u, v, w, a.steps, a.warmup)
+        ms_d = t_d / a.steps * 1e3
+        val_d = pts * a.steps / t_d / 1e6
+        log(f"[bench] fp64: {ms_d:.3f} ms/step, {val_d:.1f} Mpt-steps/s")
+        errs = {}
+        for k, am in fields_m.items():
+            ad = lg_d.download(k)
+            lev = slice(0, nz) if k == "p" else slice(0, nz - 1)
+            x = am[lev, :, :nx].astype(np.float64)
+            y = ad[lev, :, :nx]
+            errs[k] = float(np.linalg.norm(x - y) / np.linalg.norm(y))
+        lg_d.close()
+        result["fp64"] = {"value": round(val_d, 2), "ms_per_step": round(ms_d, 4),
+                          "step_roofline_frac": round(B_PT["fp64"] * pts / (ms_d * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}
+        result["rel_l2_mixed_vs_fp64"] = {"after_steps": a.steps + a.warmup, **errs}
+    else:
+        lg_m.close()
+    if not a.no_cpu:
+        log("[bench] CPU baseline (oracle) ...")
+        cb = cpu_baseline(nx, a.seed)
+        log(f"[bench] cpu: {cb['value']:.2f} Mpt-steps/s on {cb['cores']} threads")
+        result["cpu_baseline"] = cb
+        result["speedup_vs_cpu"] = round(val_m / cb["value"], 1)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -211,6 +211,23 @@ int lp_rmsdiv(lp_ctx *ctx, double *rmsdiv);     /* rmsdiv.f90:1-44              
 /* One or more device-resident time steps of the hot loop (main.f90:207-1004). */
 int lp_step(lp_ctx *ctx, int32_t nsteps);
 
+/* ---------------- measurement ----------------
+ * Kernel classes whose launches can be bracketed by HIP events on the context's
+ * stream while lp_step runs (no counterpart in the reference; used by bench.py). */
+enum lp_kernel_class {
+  LP_K_ROW_R2C = 0,        /* row pass real -> half spectrum                        */
+  LP_K_COL_OP = 1,         /* column FFT + spectral map + inverse column FFT          */
+  LP_K_ROW_C2R = 2,        /* row pass half spectrum -> real                        */
+  LP_K_LAG_RELAX = 3,      /* Germano contractions + Lagrangian relaxation           */
+  LP_K_PRESS_TRIDIAG = 4,  /* per-mode binary64 Thomas solve                        */
+  LP_K_NCLASSES = 5
+};
+/* Start (kernel_class >= 0) or stop (-1) event timing; starting clears the totals. */
+int lp_prof_enable(lp_ctx *ctx, int32_t kernel_class);
+/* Totals since lp_prof_enable: summed kernel time, launches, and the kernels'
+ * compulsory bytes (each input read once, each output written once). */
+int lp_prof_read(lp_ctx *ctx, double *total_ms, int64_t *launches, double *bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -117,7 +117,41 @@ struct lp_ctx {
   int nred = 0;
   int jt = 1, jt_total = 1;
   int F_LM_MM_init = 0, F_QN_NN_init = 0;
+  // event timing of one kernel class (lp_prof_enable)
+  int prof_class = -1;
+  std::vector<hipEvent_t> ev;
+  size_t ev_used = 0;
+  double prof_bytes = 0.0, prof_ms = 0.0;
+  long prof_launches = 0;
 };
+
+// bracket a launch of kernel class `cls` with events when that class is profiled
+static void prof_begin(lp_ctx *c, int cls) {
+  if (c->prof_class != cls) return;
+  if (c->ev_used + 2 > c->ev.size()) {
+    for (int i = 0; i < 256; ++i) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      c->ev.push_back(e);
+    }
+  }
+  (void)hipEventRecord(c->ev[c->ev_used], c->s);
+}
+static void prof_end(lp_ctx *c, int cls, double bytes) {
+  if (c->prof_class != cls || c->ev_used + 2 > c->ev.size()) return;
+  (void)hipEventRecord(c->ev[c->ev_used + 1], c->s);
+  c->ev_used += 2;
+  c->prof_bytes += bytes;
+  c->prof_launches += 1;
+}
+// fold recorded event pairs into the running total (call after a stream sync)
+static void prof_collect(lp_ctx *c) {
+  for (size_t i = 0; i + 1 < c->ev_used; i += 2) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]) == hipSuccess) c->prof_ms += ms;
+  }
+  c->ev_used = 0;
+}
 
 static size_t esz_of(Dt d) { return d == F64 ? 8 : 4; }
 static long plane_el(const lp_ctx *c) { return (long)c->g.ld * c->g.ny; }
@@ -242,6 +276,7 @@ extern "C" int lp_destroy(lp_ctx *c) {
   fr(c->bw); fr(c->tw); fr(c->gam);
   for (auto p : c->pspec) fr(p);
   fr(c->d_err); fr(c->d_part);
+  for (auto e : c->ev) (void)hipEventDestroy(e);
   if (c->s) (void)hipStreamDestroy(c->s);
   delete c;
   return LP_OK;
@@ -404,23 +439,40 @@ extern "C" int lp_get_time(lp_ctx *c, int32_t *jt, int32_t *jt_total) {
 // ------------------------------------------------------------------------------------
 // spectral pipelines (all asynchronous on c->s)
 // ------------------------------------------------------------------------------------
+// compulsory bytes of the FFT passes (input read once, output written once)
+static double bytes_r2c(int nx, int rows, int nlev, int ncols_out, size_t e) {
+  return (double)rows * nlev * ((double)nx * e + (double)ncols_out * 2 * e);
+}
+static double bytes_c2r(int nx, int rows, int nlev, int ncols_in, size_t ei, size_t eo) {
+  return (double)rows * nlev * ((double)ncols_in * 2 * ei + (double)nx * eo);
+}
+static double bytes_col(int nyi, int nyo, int ncols, int nlev, int nout, size_t e) {
+  return (double)ncols * nlev * 2 * e * ((double)nyi + (double)nout * nyo);
+}
+
 static int r2c_small(lp_ctx *c, const void *in, int nlev, double scale, void *spec) {
   const lp_grid &g = c->g;
+  prof_begin(c, LP_K_ROW_R2C);
   HIPCHK(row_r2c(c->fdt, g.nx / 2, in, g.ld, plane_el(c), g.ny, nlev, scale, spec, g.lh,
                  (long)g.lh * g.ny, g.lh, c->s));
+  prof_end(c, LP_K_ROW_R2C, bytes_r2c(g.nx, g.ny, nlev, g.lh, esz_of(c->fdt)));
   return LP_OK;
 }
 static int c2r_small(lp_ctx *c, const void *spec, int ncols, int nlev, void *out, Dt dto) {
   const lp_grid &g = c->g;
+  prof_begin(c, LP_K_ROW_C2R);
   HIPCHK(row_c2r(c->fdt, dto, g.nx / 2, spec, g.lh, (long)g.lh * g.ny, ncols, g.ny, nlev, out, g.ld,
                  plane_el(c), c->s));
+  prof_end(c, LP_K_ROW_C2R, bytes_c2r(g.nx, g.ny, nlev, ncols, esz_of(c->fdt), esz_of(dto)));
   return LP_OK;
 }
 // column map on small spectra: in -> nout outputs (same grid)
 static int col_small(lp_ctx *c, const void *in, int nlev, int fwd, int inv, int nout, ColOut *o) {
   const lp_grid &g = c->g;
   long ps = (long)g.lh * g.ny;
+  prof_begin(c, LP_K_COL_OP);
   HIPCHK(col_op(c->fdt, g.ny, g.ny, 0, fwd, inv, in, g.lh, ps, g.lh, nlev, nout, o, g.lh, ps, c->tab, c->s));
+  prof_end(c, LP_K_COL_OP, bytes_col(g.ny, g.ny, g.lh, nlev, nout, esz_of(c->fdt)));
   return LP_OK;
 }
 
@@ -574,12 +626,15 @@ static int do_lagrange(lp_ctx *c) {
   P.bclip_min = 1.0 / (2.0 * 4.0);
   P.init_lm = cf.inilag && !c->F_LM_MM_init && (c->jt == cf.cs_count || c->jt == cf.dyn_init);
   P.init_qn = cf.inilag && !c->F_QN_NN_init && (c->jt == cf.cs_count || c->jt == cf.dyn_init);
+  prof_begin(c, LP_K_LAG_RELAX);
   FDISPATCH(c, T,
             hipLaunchKernelGGL(k_lag_relax<T>, grid1(g.np * g.nz), dim3(256), 0, c->s, (T *const *)c->d_lagbar,
                                (T *const *)c->d_laghat, c->Fi[0], c->Fi[1], c->Fi[2], c->Fi[3],
                                (double *)c->f[LP_F_LM], (double *)c->f[LP_F_MM], (double *)c->f[LP_F_QN],
                                (double *)c->f[LP_F_NN], (T *)c->f[LP_CS_OPT2], (T *)c->f[LP_BETA], g, P));
   HIPCHK(hipGetLastError());
+  prof_end(c, LP_K_LAG_RELAX,
+           (double)g.np * g.nz * (2.0 * kNumLag * esz_of(c->fdt) + 8.0 * 8 + 2.0 * esz_of(c->fdt)));
   if (P.init_lm) c->F_LM_MM_init = 1;
   if (P.init_qn) c->F_QN_NN_init = 1;
   return LP_OK;
@@ -715,6 +770,7 @@ static int do_press(lp_ctx *c) {
   // :149-329 + tridag_array
   HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(int), c->s));
   cx<double> *ph = (cx<double> *)c->f[LP_P];
+  prof_begin(c, LP_K_PRESS_TRIDIAG);
   if (c->fdt == F64)
     hipLaunchKernelGGL(k_press_tridiag<double>, grid1(ps), dim3(256), 0, c->s, (const cx<double> *)c->Hs[0],
                        (const cx<double> *)c->Hs[1], (const cx<double> *)c->Hs[2], (const cx<double> *)c->bw,
@@ -724,6 +780,8 @@ static int do_press(lp_ctx *c) {
                        (const cx<float> *)c->Hs[1], (const cx<float> *)c->Hs[2], (const cx<float> *)c->bw,
                        (const cx<float> *)c->tw, c->kx, c->ky, ph, c->gam, gg.lh, gg.ny, gg.nz, gg.dz, c->d_err);
   HIPCHK(hipGetLastError());
+  // H_x, H_y, H_z read once, p_hat written once (binary64 complex)
+  prof_end(c, LP_K_PRESS_TRIDIAG, (double)ps * (gg.nz - 1) * 3 * 2 * esz_of(c->fdt) + (double)ps * (gg.nz + 1) * 16);
   // :339-352 inverse transforms in binary64: p(0..nz-1), dpdx, dpdy (1..nz-1)
   long psd = ps;  // complex elements per plane
   {
@@ -830,7 +888,33 @@ extern "C" int lp_step(lp_ctx *c, int32_t nsteps) {
     c->jt += 1;
     c->jt_total += 1;
   }
-  return check_err_flag(c);
+  int rc = check_err_flag(c);
+  prof_collect(c);
+  return rc;
+}
+
+extern "C" int lp_prof_enable(lp_ctx *c, int32_t kernel_class) {
+  if (!c || kernel_class < -1 || kernel_class >= LP_K_NCLASSES)
+    return set_err(LP_ERR_ARG, "lp_prof_enable: bad argument");
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->s));
+  prof_collect(c);
+  c->prof_class = kernel_class;
+  c->prof_ms = 0.0;
+  c->prof_bytes = 0.0;
+  c->prof_launches = 0;
+  return LP_OK;
+}
+
+extern "C" int lp_prof_read(lp_ctx *c, double *total_ms, int64_t *launches, double *bytes) {
+  if (!c) return set_err(LP_ERR_ARG, "lp_prof_read: null context");
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->s));
+  prof_collect(c);
+  if (total_ms) *total_ms = c->prof_ms;
+  if (launches) *launches = c->prof_launches;
+  if (bytes) *bytes = c->prof_bytes;
+  return LP_OK;
 }
 
 // ------------------------------------------------------------------------------------

@@ -55,6 +55,14 @@ def normals(seed, n, offset=0):
     return out[:n]
 
 
+def _zsynth(B, coef):
+    """sum_m B[j, m] coef[m, ...] with real B (two real GEMMs)."""
+    flat = coef.reshape(coef.shape[0], -1)
+    re = B @ np.ascontiguousarray(flat.real)
+    im = B @ np.ascontiguousarray(flat.imag)
+    return (re + 1j * im).reshape((B.shape[0],) + coef.shape[1:])
+
+
 def _irfft2_levels(spec, nx, ny, ld):
     """(nl, ny, nx//2+1) complex spectra -> (nl, ny, ld) real planes (padding 0)."""
     phys = np.fft.irfft2(spec, s=(ny, nx), axes=(-2, -1))
@@ -97,9 +105,9 @@ def turbulent_ic(grid, seed=DEFAULT_SEED, z0=1e-4, vonk=0.4, u_rms=1.0, log_law=
     zu = (np.arange(nz) + 0.5) * dz
     Sw = np.sin(np.outer(zw, kz))                        # (nz, nmz)
     Cu = np.cos(np.outer(zu, kz))
-    Ax = np.einsum("jm,mab->jab", Sw, Ax_c)               # spectra on w-nodes (nz, ny, lh)
-    Ay = np.einsum("jm,mab->jab", Sw, Ay_c)
-    Az = np.einsum("jm,mab->jab", Cu, Az_c)               # on uvp-nodes
+    Ax = _zsynth(Sw, Ax_c)                                # spectra on w-nodes (nz, ny, lh)
+    Ay = _zsynth(Sw, Ay_c)
+    Az = _zsynth(Cu, Az_c)                                # on uvp-nodes
     ikx = 1j * MX * (2 * np.pi / Lx)
     iky = 1j * MY * (2 * np.pi / Ly)
     U = np.zeros((nz, ny, lh), complex)

@@ -71,6 +71,8 @@ def load():
         "lp_timestep_conditions": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
         "lp_rmsdiv": ([vp, C.POINTER(C.c_double)], C.c_int),
         "lp_step": ([vp, i32], C.c_int),
+        "lp_prof_enable": ([vp, i32], C.c_int),
+        "lp_prof_read": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -251,3 +253,12 @@ class Lesgo:
 
     def step(self, n=1):
         self._check(self.L.lp_step(self.h, int(n)), "step")
+
+    # ---- measurement ----
+    def prof_enable(self, kernel_class):
+        self._check(self.L.lp_prof_enable(self.h, int(kernel_class)), "prof_enable")
+
+    def prof_read(self):
+        ms, n, b = C.c_double(), C.c_int64(), C.c_double()
+        self._check(self.L.lp_prof_read(self.h, C.byref(ms), C.byref(n), C.byref(b)), "prof_read")
+        return ms.value, n.value, b.value

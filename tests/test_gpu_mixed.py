@@ -1,0 +1,111 @@
+"""FP32/FP64-mixed path against the binary64 oracle, routine by routine and over whole
+steps.  Target of the mixed path: <= 1e-5 relative L2 per field."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from harness import load_gpu, load_oracle, random_fields, seeded_state
+from population_balance_amd import abi
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _pair(cfg_d):
+    from oracle import Oracle
+    from population_balance_amd.lesgo import Lesgo
+    cfg_m = abi.make_config(cfg_d.grid.nx, cfg_d.grid.ny, cfg_d.grid.nz, precision=abi.LP_MIXED,
+                            use_mean_p_force=cfg_d.use_mean_p_force, model=cfg_d.model,
+                            fp32_casts=cfg_d.fp32_casts)
+    return Oracle(cfg_d), Lesgo(cfg_m)
+
+
+def _errs(o, lg, names, levels=None):
+    out = {}
+    for n in names:
+        a, b = lg.download(n).astype(np.float64), o.field(n)
+        if levels is not None:
+            a, b = a[levels], b[levels]
+        out[n] = rel_l2(a, b, o.g.nx)
+    return out
+
+
+def test_mixed_spectral_primitives():
+    cfg = abi.make_config(64, 64, 9)
+    o, lg = _pair(cfg)
+    g = cfg.grid
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal((g.nz, g.ny, g.ld)).astype(np.float32)
+    ref = o.filt_da(f.astype(np.float64))
+    got = [f.copy(), np.zeros_like(f), np.zeros_like(f)]
+    lg.filt_da(*got)
+    e = [rel_l2(a, b, g.nx) for a, b in zip(got, ref)]
+    print("filt_da mixed", e)
+    assert max(e) < 1e-6
+
+
+def test_mixed_routines():
+    cfg = abi.make_config(64, 64, 33)
+    o, lg = _pair(cfg)
+    st = seeded_state(cfg)
+    load_oracle(o, st); load_gpu(lg, st)
+    g = cfg.grid
+    lev = slice(0, g.nz - 1)
+    # derivatives as main.f90:277-288 on both sides
+    for comp in "uvw":
+        f = o.field(comp).copy()
+        f2, dx, dy = o.filt_da(f)
+        o.set_field(comp, f2); o.set_field(f"d{comp}dx", dx); o.set_field(f"d{comp}dy", dy)
+    o.set_field("dudz", o.ddz_uv(o.field("u"), dfdz=o.field("dudz")))
+    o.set_field("dvdz", o.ddz_uv(o.field("v"), dfdz=o.field("dvdz")))
+    o.set_field("dwdz", o.ddz_w(o.field("w")))
+    names = ["u", "v", "w", "dudx", "dudy", "dudz", "dvdx", "dvdy", "dvdz", "dwdx", "dwdy", "dwdz"]
+    for n in names:
+        lg.upload(n, o.field(n))
+    o.call("wallstress"); lg.wallstress()
+    e = _errs(o, lg, ["txz", "tyz", "dudz", "dvdz"], levels=slice(0, 1))
+    print("wallstress", e)
+    for jt in (1, 2):
+        o.set_time(jt); lg.set_time(jt)
+        o.call("sgs_stag"); lg.sgs_stag()
+        e = _errs(o, lg, ["Cs_opt2", "Nu_t", "txx", "txy", "txz", "tyy", "tyz", "tzz"], levels=lev)
+        print(f"sgs_stag jt={jt}", {k: f"{v:.2e}" for k, v in e.items()})
+        assert max(e.values()) < 1e-3
+    o.call("convec"); lg.convec()
+    e = _errs(o, lg, ["RHSx", "RHSy", "RHSz"], levels=lev)
+    print("convec", {k: f"{v:.2e}" for k, v in e.items()})
+    assert max(e.values()) < 1e-5
+
+
+def test_mixed_press():
+    cfg = abi.make_config(64, 64, 33)
+    o, lg = _pair(cfg)
+    st = seeded_state(cfg)
+    noise = random_fields(cfg, ["divtz"], seed=3)
+    st["divtz"] = noise["divtz"]
+    load_oracle(o, st); load_gpu(lg, st)
+    o.call("press_stag_array"); lg.press_stag_array()
+    g = cfg.grid
+    e = _errs(o, lg, ["p"], levels=slice(0, g.nz))
+    e.update(_errs(o, lg, ["dpdx", "dpdy"], levels=slice(0, g.nz - 1)))
+    print("press mixed", {k: f"{v:.2e}" for k, v in e.items()})
+
+
+@pytest.mark.parametrize("nx,nz,nsteps,mpf", [(64, 33, 8, 0), (64, 33, 8, 1), (128, 65, 4, 1)])
+def test_mixed_steps(nx, nz, nsteps, mpf):
+    print(f"\n[case] mixed steps {nx}x{nx}x{nz} n={nsteps} mean_p_force={mpf}", flush=True)
+    cfg = abi.make_config(nx, nx, nz, use_mean_p_force=mpf)
+    o, lg = _pair(cfg)
+    st = seeded_state(cfg)
+    load_oracle(o, st); load_gpu(lg, st)
+    worst = {}
+    for s in range(nsteps):
+        o.step(1); lg.step(1)
+        e = _errs(o, lg, ["u", "v", "w", "RHSx", "RHSy", "RHSz", "Cs_opt2"], levels=slice(0, nz - 1))
+        e.update(_errs(o, lg, ["p"], levels=slice(0, nz)))
+        print(f"  step {s + 1}: " + " ".join(f"{k}={v:.2e}" for k, v in e.items()), flush=True)
+        for k, v in e.items():
+            worst[k] = max(worst.get(k, 0.0), v)
+    for k in ("u", "v", "w", "p"):
+        assert worst[k] < TOL, (k, worst[k])
