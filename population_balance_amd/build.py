@@ -33,6 +33,29 @@ def _compile(src):
     return out
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FC = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+
+
+def build_fortran(verbose=False):
+    """Compile the ISO_C_BINDING module and the drop-in demo host with amdflang (the
+    reference's language).  Skipped (returns None) when no Fortran compiler exists."""
+    if not os.path.exists(FC):
+        return None
+    out = os.path.join(FORTRAN, "drop_in_demo")
+    mod_dir = os.path.join(OBJ, "fortran")
+    os.makedirs(mod_dir, exist_ok=True)
+    cmd = [FC, "-O2", f"-J{mod_dir}", os.path.join(FORTRAN, "lesgo_amd_iface.f90"),
+           os.path.join(FORTRAN, "drop_in_demo.f90"), "-L" + HERE, "-llesgo_hip",
+           "-Wl,-rpath," + HERE, "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"amdflang failed:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {out}")
+    return out
+
+
 def build(jobs=None, verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     with ThreadPoolExecutor(max_workers=jobs or len(SOURCES)) as ex:
@@ -45,6 +68,7 @@ def build(jobs=None, verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_fortran(verbose)
     return LIB
 
 

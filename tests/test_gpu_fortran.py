@@ -1,0 +1,19 @@
+"""A Fortran host (the reference's language) drives the HIP path through the
+ISO_C_BINDING module with reference-shaped arrays (drop-in check)."""
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_fortran_drop_in_filt_da():
+    exe = os.path.join(ROOT, "population_balance_amd", "fortran", "drop_in_demo")
+    assert os.path.exists(exe), "build() compiles the Fortran demo with amdflang"
+    print("\n[case] fortran drop-in demo", flush=True)
+    r = subprocess.run([exe, "--run"], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    assert "drop-in demo ok" in r.stdout
