@@ -65,7 +65,10 @@ typedef struct lp_grid {
   double dx, dy, dz;             /* L_x/nx, L_y/ny, L_z/(nz-1)                          */
 } lp_grid;
 
-/* Droplet (PCon) parameters: param.f90:303-440,560-682 re-expressed at run time. */
+/* Droplet (PCon) parameters: param.f90:303-440,560-682 re-expressed at run time.
+ * Boundary modes for bc_x / bc_y: 0 periodic (periodicbcx/y), 1 inlet/outlet
+ * (PCON_INLET_X: zero inflow at x = 0, zero gradient at x = L_x; PCON_INLET_Y: zero
+ * gradient at y = 0, zero inflow at y = L_y), 2 upwind-sign ghosts (neither). */
 typedef struct lp_droplet_config {
   int32_t npcon;                 /* number of size bins (0 = droplets off)             */
   int32_t pbe;                   /* compute the breakup source Rhs_pbe (PBE_FLAG)      */
@@ -73,16 +76,22 @@ typedef struct lp_droplet_config {
                                     reference, scalars_module_crossflow_mod.f90:4470)  */
   int32_t settling, inertia;     /* settling, PCon_acc                                 */
   int32_t active;                /* active_pcon: droplet buoyancy into RHSz            */
-  int32_t inlet_x, inlet_y;      /* PCON_INLET_X / PCON_INLET_Y                        */
-  int32_t src_i, src_j, src_k;   /* point-source centre (1-based node indices)         */
-  int32_t pad_;
-  double src_sigma;              /* Gaussian source width in cells                     */
+  int32_t bc_x, bc_y;            /* see above                                          */
+  int32_t src_i, src_j, src_k;   /* source centre xps, yps and level zps_local (1-based);
+                                    src_frac goes to src_k, 1-src_frac to src_k+1      */
+  int32_t src_ngau;              /* footprint exponent N_GAU (2 = Gaussian)            */
+  int32_t src_jt0, src_jt1;      /* source on for src_jt0 <= jt_total < src_jt1       */
+  int32_t pcon_init;             /* PCon_init: first jt_total with a droplet step      */
+  int32_t init_pcon;             /* initPCon: RHS_PCon_f comes with the state          */
+  double src_sigma;              /* footprint width in cells (2*b0_plume/dx)           */
+  double src_frac;               /* share of the lower source level (0.7)              */
   double Sc;                     /* SGS Schmidt number (model_sc = 1)                  */
+  double sfc_flux;               /* prescribed surface flux (lbcp = 1)                 */
   double rho_c, rho_w;           /* droplet / sea-water density                        */
   double sigma_s, nu_c, nu_d, k_b, d_min; /* surface tension, viscosities, breakup
                                     constant, minimum daughter size (param.f90:401)    */
-  double g;                      /* gravity (dimensional, nondimensionalised inside)   */
-  double u_star;                 /* velocity scale                                     */
+  double g;                      /* gravity                                            */
+  double u_star, z_i;            /* velocity and length scales                         */
   const double *diameter;        /* [npcon] bin diameters (m)                          */
   const double *settling_vel;    /* [npcon] rise/settling velocity (nondimensional)    */
   const double *Q_src;           /* [npcon] source strength (nondimensional)           */
@@ -208,6 +217,13 @@ int lp_project(lp_ctx *ctx);                    /* forcing.f90:122-294          
 int lp_timestep_conditions(lp_ctx *ctx, double *cfl, double *visc_stab);
                                                 /* scalars_module2.f90:1336-1379   */
 int lp_rmsdiv(lp_ctx *ctx, double *rmsdiv);     /* rmsdiv.f90:1-44                 */
+/* scalars_module_crossflow_mod.f90:1494-1786 (pollen_all_in_one2 with
+ * pollen_RHS_calc2, calcbeta_pcon, step_pollen2) on the context state; needs npcon > 0. */
+int lp_pollen_all_in_one2(lp_ctx *ctx);
+/* scalars_module_crossflow_mod.f90:4530-4597: beta(npcon,npcon) binary64, beta[i*n+j]
+ * = probability that a drop of bin j breaks into bin i (host routine, no device). */
+int lp_calculate_probability(const double *diameter, int32_t npcon, double sigma_s,
+                             double d_min, double *beta);
 /* One or more device-resident time steps of the hot loop (main.f90:207-1004). */
 int lp_step(lp_ctx *ctx, int32_t nsteps);
 

@@ -63,16 +63,27 @@ def lib():
         L.or_divstress_w.argtypes = [vp, vp, vp, vp, vp]
         L.or_tridag_array.argtypes = [vp, vp, vp, vp, vp, vp]
         for n in ("or_wallstress", "or_sgs_stag", "or_lagrange_sdep", "or_interpolag_sdep",
-                  "or_convec", "or_press_stag_array", "or_project"):
+                  "or_convec", "or_press_stag_array", "or_project", "or_pollen_all_in_one2"):
             getattr(L, n).argtypes = [vp]
             getattr(L, n).restype = C.c_int
         L.or_tridag_array.restype = C.c_int
         L.or_timestep_conditions.argtypes = [vp, dp, dp]
         L.or_rmsdiv.argtypes = [vp, dp]
+        L.or_ke_dudt.argtypes = [vp]
+        L.or_calculate_probability.argtypes = [vp, C.c_int, C.c_double, C.c_double, vp]
         L.or_step.argtypes = [vp, C.c_int]
         L.or_step.restype = C.c_int
         _lib = L
     return _lib
+
+
+def calculate_probability(d, sigma=15.5e-3, d_min=1e-7):
+    """scalars_module_crossflow_mod.f90:4530-4597, beta[i, j] (bin j breaks into bin i)."""
+    d = np.ascontiguousarray(d, np.float64)
+    n = len(d)
+    beta = np.zeros((n, n))
+    lib().or_calculate_probability(C.c_void_p(d.ctypes.data), n, sigma, d_min, C.c_void_p(beta.ctypes.data))
+    return beta
 
 
 def _p(a):
@@ -270,6 +281,9 @@ class Oracle:
         a = C.c_double()
         self.L.or_rmsdiv(self.h, C.byref(a))
         return a.value
+
+    def ke_dudt(self):
+        self.L.or_ke_dudt(self.h)
 
     def step(self, n=1):
         self._check(self.L.or_step(self.h, int(n)), "step")

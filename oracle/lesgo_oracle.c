@@ -134,6 +134,8 @@ struct or_ctx {
   /* persistent scratch */
   double *S[6];
   double *big[7];
+  /* droplet bins (copied from the configuration) */
+  double *diam, *ws, *qsrc;
 };
 
 static size_t plane(const or_ctx *c) { return (size_t)c->g.ld * c->g.ny; }
@@ -170,6 +172,21 @@ or_ctx *or_create(const lp_config *cfg) {
   }
   c->jt = 1;
   c->jt_total = 1;
+  int np_ = cfg->drop.npcon;
+  if (np_ > 0) {
+    c->diam = (double *)malloc(sizeof(double) * np_);
+    c->ws = (double *)malloc(sizeof(double) * np_);
+    c->qsrc = (double *)malloc(sizeof(double) * np_);
+    for (int i = 0; i < np_; ++i) {
+      c->diam[i] = cfg->drop.diameter[i];
+      c->ws[i] = cfg->drop.settling_vel[i];
+      c->qsrc[i] = cfg->drop.Q_src[i];
+    }
+    /* the configuration keeps pointers the caller may free */
+    c->cfg.drop.diameter = c->diam;
+    c->cfg.drop.settling_vel = c->ws;
+    c->cfg.drop.Q_src = c->qsrc;
+  }
   return c;
 }
 
@@ -180,6 +197,7 @@ void or_destroy(or_ctx *c) {
   for (int i = 0; i < LP_NFIELDS; ++i) free(c->f[i]);
   for (int i = 0; i < 6; ++i) free(c->S[i]);
   for (int i = 0; i < 7; ++i) free(c->big[i]);
+  free(c->diam); free(c->ws); free(c->qsrc);
   free(c);
 }
 
@@ -1201,6 +1219,477 @@ void or_rmsdiv(or_ctx *c, double *rms) {
 }
 
 /* ------------------------------------------------------------------------- */
+/* main.f90:864-898: ke = energy(); filt_da(ke); ddz_uv(ke); dudt += grad ke      */
+/* (the acceleration the droplet inertia term reads at the next step)          */
+/* ------------------------------------------------------------------------- */
+void or_ke_dudt(or_ctx *c) {
+  const lp_grid *g = &c->g;
+  int nz = g->nz, ld = g->ld;
+  size_t np = plane(c), nv = vol(c);
+  double *u = c->f[LP_U], *v = c->f[LP_V], *w = c->f[LP_W];
+  double *ke = (double *)calloc(nv, sizeof(double));
+  for (int k = 0; k < nz - 1; ++k)                                              /* energy.f90:23-31 */
+    for (int y = 0; y < g->ny; ++y)
+      for (int x = 0; x < g->nx; ++x) {
+        size_t i = (size_t)k * np + (size_t)y * ld + x;
+        double tw = 0.5 * (w[i] + w[i + np]);
+        ke[i] = 0.5 * (u[i] * u[i] + v[i] * v[i] + tw * tw);
+      }
+  double *kt = (double *)malloc(nv * sizeof(double));
+  memcpy(kt, ke, nv * sizeof(double));
+  double *kx = (double *)calloc(nv, sizeof(double)), *ky = (double *)calloc(nv, sizeof(double));
+  double *kz = (double *)calloc(nv, sizeof(double));
+  or_filt_da(c, kt, kx, ky, nz);
+  or_ddz_uv(c, kz, ke);
+  for (size_t i = 0; i < (size_t)(nz - 1) * np; ++i) {
+    c->f[LP_DUDT][i] += kx[i];
+    c->f[LP_DVDT][i] += ky[i];
+    c->f[LP_DWDT][i] += kz[i];
+  }
+  free(ke); free(kt); free(kx); free(ky); free(kz);
+}
+
+/* scalars_module_crossflow_mod.f90:4530-4597 calculate_probability: beta[i*n+j] for
+ * a drop of bin j breaking into bin i (i < j) */
+void or_calculate_probability(const double *d, int n, double sigma, double d_min, double *beta) {
+  double *dc = (double *)malloc(sizeof(double) * n), *ds = (double *)malloc(sizeof(double) * n);
+  double *emax = (double *)malloc(sizeof(double) * n), *emin = (double *)malloc(sizeof(double) * n);
+  double *ef = (double *)calloc((size_t)n * n, sizeof(double));
+  for (int i = 0; i < n; ++i) { dc[i] = pow(d[i], 3.0); ds[i] = pow(d[i], 2.0); }
+  for (int i = 0; i < n; ++i) {
+    emax[i] = PI_D * sigma * 0.25992104 * ds[i];
+    emin[i] = PI_D * sigma * (pow(dc[i] - d_min * d_min * d_min, 2.0 / 3.0) + d_min * d_min - ds[i]);
+    for (int j = i + 1; j < n; ++j)
+      ef[i * n + j] = PI_D * sigma * (pow(dc[j] - dc[i], 2.0 / 3.0) + ds[i] - ds[j]);
+  }
+  for (int i = 0; i < n * n; ++i) beta[i] = 0.0;
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      double num = emin[j] + emax[j] - ef[i * n + j];
+      double den = 0.0;
+      for (int k = 0; k < j; ++k) den += emin[j] + emax[j] - ef[k * n + j];
+      beta[i * n + j] = num / den;
+    }
+  free(dc); free(ds); free(emax); free(emin); free(ef);
+}
+
+/* scalars_module_crossflow_mod.f90:2167-2230 breakage frequency of bin ip at a cell with
+ * SGS dissipation `dissip` (fit parameters from the Ohnesorge number of the bin) */
+static double breakage_freq(const lp_droplet_config *D, double dip, double dissip) {
+  double Oh = pow(D->rho_c * D->nu_d * D->nu_d / D->sigma_s / dip, 0.5);
+  double a = 2.392 * exp(-26.76 * Oh) + 2.877 * exp(-0.1244 * Oh);
+  double b = 0.45;
+  double cc = -pow(10.0, (0.5446 * pow(Oh, -0.3776) / (1 + 12.67 * pow(Oh, 1.462))));
+  double dd = -(4.172 * pow(Oh, -0.5492)) / (1 + 0.5079 * pow(Oh, -0.4879));
+  /* -0.7182 is a default-kind (binary32) literal in the reference */
+  double e = pow(10.0, 0.4113 * exp(-55.94 * log10(Oh + 1)) + 0.5125 * exp(-(double)0.7182f * log10(Oh + 1)));
+  double ds = dissip * pow(D->u_star, 3.0) / D->z_i;
+  double tau_c = pow(fabs(dissip), 0.33333333) * pow(dip, -0.6666667);
+  double Re = pow(fabs(ds), 1.0 / 3.0) * pow(dip, 4.0 / 3.0) / D->nu_c;
+  double Re1 = log10(Re);
+  double fb = 0.5 * ((signbit(Re1) ? -1.0 : 1.0) + 1.0);
+  double al = a * fb * pow(Re1, fb * b);
+  double bl = fb * cc * pow(Re1, fb * dd) - e;
+  double fit = pow(10.0, al + bl);
+  return D->k_b * pow(2.1, 0.5) * PI_D / 4.0 * 0.84 * tau_c * fit;
+}
+
+static inline double smart_r(double r) {
+  return fmax(0.0, fmin(fmin(2.0 * r, 0.75 * r + 0.25), 4.0));
+}
+/* SMART face value given the upwind cell, the cell upwind of it, and the downwind
+ * cell (:2877-3475): zero-denominator guard takes the upwind value */
+static inline double smart_face(double cup, double cupup, double cdn) {
+  double den = cup - cupup;
+  if (den == 0.0) return cup;
+  double r = smart_r((cdn - cup) / den);
+  return cup + r * den / 2.0;
+}
+
+/* scalars_module_crossflow_mod.f90:1494-1786 pollen_all_in_one2 with pollen_RHS_calc2
+ * (:1894-4472), calcbeta_pcon (:1795-1843) and step_pollen2 (:4481-4508).
+ * PCon_scheme = 3 (SMART), model_sc = 1, lbcp = 1, no Stokes drift, no SGS
+ * acceleration flux, single Gaussian point source. */
+int or_pollen_all_in_one2(or_ctx *c) {
+  const lp_grid *g = &c->g;
+  const lp_config *cf = &c->cfg;
+  const lp_droplet_config *D = &cf->drop;
+  const int nx = g->nx, ny = g->ny, nz = g->nz, ld = g->ld, npc = D->npcon;
+  const size_t np = plane(c), nv = vol(c);
+  const double dx = g->dx, dy = g->dy, dz = g->dz, dt = cf->dt;
+  double *PC = c->f[LP_PCON], *RH = c->f[LP_RHS_PCON], *RF = c->f[LP_RHS_PCON_F], *PB = c->f[LP_RHS_PBE];
+  double *u = c->f[LP_U], *v = c->f[LP_V], *w = c->f[LP_W];
+  double *dudx = c->f[LP_DUDX], *dvdy = c->f[LP_DVDY], *nut = c->f[LP_NU_T], *magS = c->f[LP_MAGS];
+  double *dudt = c->f[LP_DUDT], *dvdt = c->f[LP_DVDT], *dwdt = c->f[LP_DWDT];
+  double *dissip = c->f[LP_DISSIP];
+  /* :1521 */
+  memcpy(RF, RH, nv * npc * sizeof(double));
+  const int src_on = c->jt_total >= D->src_jt0 && c->jt_total < D->src_jt1;
+  /* ---- pollen_RHS_calc2 ---- */
+  /* face velocity arrays (ld, ny+1, nz) */
+  size_t fp = (size_t)ld * (ny + 1);
+#define FI(i, j, k) ((size_t)(k) * fp + (size_t)(j) * ld + (i))
+  double *ui0 = (double *)calloc(fp * nz, sizeof(double)), *vi0 = (double *)calloc(fp * nz, sizeof(double));
+  double *ui = (double *)calloc(fp * nz, sizeof(double)), *vi = (double *)calloc(fp * nz, sizeof(double));
+  double *wi = (double *)calloc(fp * nz, sizeof(double));
+  /* :2054-2081 u: dense nx x nx matrix times (dx dudx(1:nx-1), sum u) */
+  double *dvec = (double *)malloc(sizeof(double) * (nx > ny ? nx : ny));
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j) {
+      const double *du = dudx + (size_t)k * np + (size_t)j * ld, *uu = u + (size_t)k * np + (size_t)j * ld;
+      for (int m = 0; m < nx - 1; ++m) dvec[m] = dx * du[m];
+      double su = 0.0;
+      for (int m = 0; m < nx; ++m) su += uu[m];
+      dvec[nx - 1] = su;
+      for (int i = 0; i < nx; ++i) {
+        double acc = 0.0;
+        for (int m = 0; m < nx; ++m) {     /* matrix_x(m,i) */
+          double M;
+          if (m == nx - 1) M = 1.0 / nx;
+          else if (i <= m) M = -(nx - (m + 1)) * (1.0 / nx);
+          else M = (m + 1) * (1.0 / nx);
+          acc += M * dvec[m];
+        }
+        ui0[FI(i, j, k)] = acc;
+      }
+      ui0[FI(nx, j, k)] = ui0[FI(0, j, k)];                                  /* :2081 */
+    }
+  /* :2088-2122 v */
+  for (int k = 0; k < nz; ++k)
+    for (int i = 0; i < nx; ++i) {
+      for (int m = 0; m < ny - 1; ++m) dvec[m] = dy * dvdy[(size_t)k * np + (size_t)m * ld + i];
+      double sv = 0.0;
+      for (int m = 0; m < ny; ++m) sv += v[(size_t)k * np + (size_t)m * ld + i];
+      dvec[ny - 1] = sv;
+      for (int j = 0; j < ny; ++j) {
+        double acc = 0.0;
+        for (int m = 0; m < ny; ++m) {
+          double M;
+          if (m == ny - 1) M = 1.0 / ny;
+          else if (j <= m) M = -(ny - (m + 1)) * (1.0 / ny);
+          else M = (m + 1) * (1.0 / ny);
+          acc += M * dvec[m];
+        }
+        vi0[FI(i, j, k)] = acc;
+      }
+      vi0[FI(i, ny, k)] = vi0[FI(i, 0, k)];
+    }
+  free(dvec);
+  /* :2130 w is already on the faces; :2135-2138 magS(nz) = magS(nz-1) */
+  for (size_t i = 0; i < np; ++i) magS[(size_t)(nz - 1) * np + i] = magS[(size_t)(nz - 2) * np + i];
+  /* PBE set-up :2005-2021, :2167-2230 */
+  double *beta = NULL, *gfreq = NULL;
+  if (D->pbe) {
+    memset(PB, 0, nv * npc * sizeof(double));
+    beta = (double *)malloc(sizeof(double) * npc * npc);
+    or_calculate_probability(D->diameter, npc, D->sigma_s, D->d_min, beta);
+    gfreq = (double *)calloc(nv * npc, sizeof(double));
+#pragma omp parallel for
+    for (long q = 0; q < (long)nv * npc; ++q) {
+      int ip = (int)(q / nv);
+      size_t i = (size_t)(q % nv);
+      gfreq[q] = breakage_freq(D, D->diameter[ip], dissip[i]);
+    }
+  }
+  /* Kc_t on cell centres (:3482-3496) */
+  double *Kc = (double *)calloc(nv, sizeof(double));
+  for (int k = 0; k < nz - 1; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        size_t s = (size_t)k * np + (size_t)j * ld + i;
+        double kv;
+        if (k == 0) kv = nut[s];
+        else if (k == nz - 2) kv = nut[s];
+        else kv = (nut[s] + nut[s + np]) / 2.0;
+        Kc[s] = kv / D->Sc;
+      }
+  const double ghat = D->g / (D->u_star * D->u_star / D->z_i);
+  double *cx_ = (double *)malloc(sizeof(double) * fp * nz), *cy_ = (double *)malloc(sizeof(double) * fp * nz);
+  double *cz_ = (double *)malloc(sizeof(double) * fp * nz);
+  double *gx0 = (double *)malloc(sizeof(double) * ny * nz), *gxL = (double *)malloc(sizeof(double) * ny * nz);
+  double *gy0 = (double *)malloc(sizeof(double) * nx * nz), *gyL = (double *)malloc(sizeof(double) * nx * nz);
+  /* source footprint (:4222-4256): sum of weights; the 2 pi (width/dx)^2 cancels */
+  double *wsrc = NULL, wsum = 0.0;
+  if (src_on) {
+    wsrc = (double *)malloc(sizeof(double) * nx * ny);
+    double width = D->src_sigma * dx;
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        double rx = ((i + 1) - D->src_i) * dx, ry = ((j + 1) - D->src_j) * dy;
+        double r2 = rx * rx + ry * ry;
+        double wv = exp(-pow(r2, D->src_ngau / 2) / (2.0 * pow(width, D->src_ngau)));
+        wsrc[j * nx + i] = wv;
+        wsum += wv / (2.0 * PI_D * pow(width / dx, 2.0));
+      }
+  }
+  for (int ipc = 0; ipc < npc; ++ipc) {
+    double *C = PC + (size_t)ipc * nv, *R = RH + (size_t)ipc * nv;
+#define CC(i, j, k) C[(size_t)(k) * np + (size_t)(j) * ld + (i)]
+    const double ws = D->settling_vel[ipc];
+    /* :2243-2276 */
+    memcpy(ui, ui0, sizeof(double) * fp * nz);
+    memcpy(vi, vi0, sizeof(double) * fp * nz);
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) wi[FI(i, j, k)] = w[(size_t)k * np + (size_t)j * ld + i];
+    if (D->settling) {
+      for (int k = 1; k < nz - 1; ++k)
+        for (int j = 0; j < ny; ++j)
+          for (int i = 0; i < nx; ++i) wi[FI(i, j, k)] -= ws;
+      if (D->inertia) {
+        double f = ws / ghat;
+        for (int k = 0; k < nz; ++k)
+          for (int j = 0; j < ny; ++j) {
+            const double *a = dudt + (size_t)k * np + (size_t)j * ld;
+            for (int i = 1; i < nx; ++i) ui[FI(i, j, k)] -= 0.5 * (a[i - 1] + a[i]) * f;
+            ui[FI(0, j, k)] -= 0.5 * (a[nx - 1] + a[0]) * f;
+            ui[FI(nx, j, k)] -= 0.5 * (a[nx - 1] + a[0]) * f;
+          }
+        for (int k = 0; k < nz; ++k)
+          for (int i = 0; i < nx; ++i) {
+            const double *a = dvdt + (size_t)k * np + i;
+            for (int j = 1; j < ny; ++j) vi[FI(i, j, k)] -= 0.5 * (a[(size_t)(j - 1) * ld] + a[(size_t)j * ld]) * f;
+            vi[FI(i, 0, k)] -= 0.5 * (a[(size_t)(ny - 1) * ld] + a[0]) * f;
+            vi[FI(i, ny, k)] -= 0.5 * (a[(size_t)(ny - 1) * ld] + a[0]) * f;
+          }
+        for (int k = 1; k < nz - 1; ++k)
+          for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) wi[FI(i, j, k)] -= dwdt[(size_t)k * np + (size_t)j * ld + i] * f;
+      }
+    }
+    /* :2287-2467 surface flux, lbcp = 1 (pointsource: only negative fluxes kept) */
+    double psf = D->sfc_flux;
+    if (psf > 0) psf = 0.0;
+    /* :2478-2590 ghosts and boundary faces */
+    for (int k = 0; k < nz - 1; ++k) {
+      for (int j = 0; j < ny; ++j) {
+        size_t t = (size_t)k * ny + j;
+        if (D->bc_x == 0) { gx0[t] = CC(nx - 1, j, k); gxL[t] = CC(0, j, k); }
+        else if (D->bc_x == 1) {
+          gxL[t] = CC(nx - 1, j, k); cx_[FI(0, j, k)] = 0.0; gx0[t] = 0.0; cx_[FI(nx, j, k)] = CC(nx - 1, j, k);
+        } else {
+          double s0 = signbit(ui[FI(0, j, k)]) ? -1.0 : 1.0;
+          gx0[t] = CC(0, j, k) * (1 - s0) / 2; cx_[FI(0, j, k)] = gx0[t];
+          double s1 = signbit(-ui[FI(0, j, k)]) ? -1.0 : 1.0;
+          gxL[t] = CC(nx - 1, j, k) * (1 - s1) / 2; cx_[FI(nx, j, k)] = gxL[t];
+        }
+      }
+      for (int i = 0; i < nx; ++i) {
+        size_t t = (size_t)k * nx + i;
+        if (D->bc_y == 0) { gy0[t] = CC(i, ny - 1, k); gyL[t] = CC(i, 0, k); }
+        else if (D->bc_y == 1) {
+          gy0[t] = CC(i, 0, k); cy_[FI(i, 0, k)] = CC(i, 0, k); cy_[FI(i, ny, k)] = 0.0; gyL[t] = 0.0;
+        } else {
+          double s0 = signbit(vi[FI(i, 0, k)]) ? -1.0 : 1.0;
+          gy0[t] = CC(i, 0, k) * (1 - s0) / 2; cy_[FI(i, 0, k)] = gy0[t];
+          double s1 = signbit(-vi[FI(i, 0, k)]) ? -1.0 : 1.0;
+          gyL[t] = CC(i, ny - 1, k) * (1 - s1) / 2; cy_[FI(i, ny, k)] = gyL[t];
+        }
+      }
+    }
+    /* :2877-3475 SMART faces (face i lies between cells i-1 and i) */
+    for (int k = 0; k < nz - 1; ++k)
+      for (int j = 0; j < ny; ++j) {
+        size_t t = (size_t)k * ny + j;
+        for (int i = 1; i < nx; ++i) {
+          double cm2 = (i >= 2) ? CC(i - 2, j, k) : gx0[t];
+          double cp1 = (i + 1 <= nx - 1) ? CC(i + 1, j, k) : gxL[t];
+          if (ui[FI(i, j, k)] >= 0.0) cx_[FI(i, j, k)] = smart_face(CC(i - 1, j, k), cm2, CC(i, j, k));
+          else cx_[FI(i, j, k)] = smart_face(CC(i, j, k), cp1, CC(i - 1, j, k));
+        }
+        if (D->bc_x == 0) {   /* :3370-3410 periodic faces 1 and nx+1 */
+          double f0;
+          if (ui[FI(0, j, k)] >= 0.0) f0 = smart_face(CC(nx - 1, j, k), CC(nx - 2, j, k), CC(0, j, k));
+          else f0 = smart_face(CC(0, j, k), CC(1, j, k), CC(nx - 1, j, k));
+          cx_[FI(0, j, k)] = f0;
+          cx_[FI(nx, j, k)] = f0;
+        }
+      }
+    for (int k = 0; k < nz - 1; ++k)
+      for (int i = 0; i < nx; ++i) {
+        size_t t = (size_t)k * nx + i;
+        for (int j = 1; j < ny; ++j) {
+          double cm2 = (j >= 2) ? CC(i, j - 2, k) : gy0[t];
+          double cp1 = (j + 1 <= ny - 1) ? CC(i, j + 1, k) : gyL[t];
+          if (vi[FI(i, j, k)] >= 0.0) cy_[FI(i, j, k)] = smart_face(CC(i, j - 1, k), cm2, CC(i, j, k));
+          else cy_[FI(i, j, k)] = smart_face(CC(i, j, k), cp1, CC(i, j - 1, k));
+        }
+        if (D->bc_y == 0) {
+          double f0;
+          if (vi[FI(i, 0, k)] >= 0.0) f0 = smart_face(CC(i, ny - 1, k), CC(i, ny - 2, k), CC(i, 0, k));
+          else f0 = smart_face(CC(i, 0, k), CC(i, 1, k), CC(i, ny - 1, k));
+          cy_[FI(i, 0, k)] = f0;
+          cy_[FI(i, ny, k)] = f0;
+        }
+      }
+    /* z faces: face k (0-based) between cells k-1 and k */
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        for (int k = 2; k <= nz - 3; ++k) {
+          if (wi[FI(i, j, k)] >= 0.0) cz_[FI(i, j, k)] = smart_face(CC(i, j, k - 1), CC(i, j, k - 2), CC(i, j, k));
+          else cz_[FI(i, j, k)] = smart_face(CC(i, j, k), CC(i, j, k + 1), CC(i, j, k - 1));
+        }
+        /* face 2 (0-based 1): central when w >= 0 at the wall side */
+        if (wi[FI(i, j, 1)] >= 0.0) cz_[FI(i, j, 1)] = (CC(i, j, 1) + CC(i, j, 0)) / 2.0;
+        else cz_[FI(i, j, 1)] = smart_face(CC(i, j, 1), CC(i, j, 2), CC(i, j, 0));
+        /* face nz-1 (0-based nz-2): uses C(nz) (= C(nz-1) after step_pollen2) */
+        if (wi[FI(i, j, nz - 2)] >= 0.0)
+          cz_[FI(i, j, nz - 2)] = smart_face(CC(i, j, nz - 3), CC(i, j, nz - 4), CC(i, j, nz - 2));
+        else cz_[FI(i, j, nz - 2)] = smart_face(CC(i, j, nz - 2), CC(i, j, nz - 1), CC(i, j, nz - 3));
+        cz_[FI(i, j, 0)] = 0.0;                                                /* :3340 */
+        cz_[FI(i, j, nz - 1)] = cz_[FI(i, j, nz - 2)];                         /* :3356 */
+      }
+    /* :3559-3677 RHS = -div(u C) + div(Kc grad C) */
+    for (int k = 0; k < nz - 1; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          size_t s = (size_t)k * np + (size_t)j * ld + i;
+          double r = -(1.0 / dx) * (ui[FI(i + 1, j, k)] * cx_[FI(i + 1, j, k)] - ui[FI(i, j, k)] * cx_[FI(i, j, k)]);
+          r = r - (1.0 / dy) * (vi[FI(i, j + 1, k)] * cy_[FI(i, j + 1, k)] - vi[FI(i, j, k)] * cy_[FI(i, j, k)]);
+          r = r - (1.0 / dz) * (wi[FI(i, j, k + 1)] * cz_[FI(i, j, k + 1)] - wi[FI(i, j, k)] * cz_[FI(i, j, k)]);
+          R[s] = r;
+        }
+    for (int k = 0; k < nz - 1; ++k)
+      for (int j = 0; j < ny; ++j) {
+        size_t t = (size_t)k * ny + j;
+        const double *Kr = Kc + (size_t)k * np + (size_t)j * ld;
+        double *Rr = R + (size_t)k * np + (size_t)j * ld;
+        const double *Cr = C + (size_t)k * np + (size_t)j * ld;
+        for (int i = 1; i < nx - 1; ++i)
+          Rr[i] = Rr[i] + (1.0 / (dx * dx)) * (((Kr[i] + Kr[i + 1]) / 2.0) * (Cr[i + 1] - Cr[i]) -
+                                              ((Kr[i] + Kr[i - 1]) / 2.0) * (Cr[i] - Cr[i - 1]));
+        Rr[0] = Rr[0] + (1.0 / (dx * dx)) * (((Kr[0] + Kr[1]) / 2.0) * (Cr[1] - Cr[0]) -
+                                            ((Kr[0] + Kr[nx - 1]) / 2.0) * (Cr[0] - gx0[t]));
+        Rr[nx - 1] = Rr[nx - 1] + (1.0 / (dx * dx)) * (((Kr[nx - 1] + Kr[0]) / 2.0) * (gxL[t] - Cr[nx - 1]) -
+                                                      ((Kr[nx - 1] + Kr[nx - 2]) / 2.0) * (Cr[nx - 1] - Cr[nx - 2]));
+      }
+    for (int k = 0; k < nz - 1; ++k)
+      for (int i = 0; i < nx; ++i) {
+        size_t t = (size_t)k * nx + i;
+#define KK(j) Kc[(size_t)k * np + (size_t)(j) * ld + i]
+#define CY(j) C[(size_t)k * np + (size_t)(j) * ld + i]
+#define RY(j) R[(size_t)k * np + (size_t)(j) * ld + i]
+        for (int j = 1; j < ny - 1; ++j)
+          RY(j) = RY(j) + (1.0 / (dy * dy)) * (((KK(j) + KK(j + 1)) / 2.0) * (CY(j + 1) - CY(j)) -
+                                              ((KK(j) + KK(j - 1)) / 2.0) * (CY(j) - CY(j - 1)));
+        RY(0) = RY(0) + (1.0 / (dy * dy)) * (((KK(0) + KK(1)) / 2.0) * (CY(1) - CY(0)) -
+                                            ((KK(0) + KK(ny - 1)) / 2.0) * (CY(0) - gy0[t]));
+        RY(ny - 1) = RY(ny - 1) + (1.0 / (dy * dy)) * (((KK(ny - 1) + KK(0)) / 2.0) * (gyL[t] - CY(ny - 1)) -
+                                                      ((KK(ny - 1) + KK(ny - 2)) / 2.0) * (CY(ny - 1) - CY(ny - 2)));
+#undef KK
+#undef CY
+#undef RY
+      }
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+#define KZ(k) Kc[(size_t)(k) * np + (size_t)j * ld + i]
+#define CZ(k) C[(size_t)(k) * np + (size_t)j * ld + i]
+#define RZ(k) R[(size_t)(k) * np + (size_t)j * ld + i]
+        for (int k = 1; k < nz - 2; ++k)
+          RZ(k) = RZ(k) + (1.0 / (dz * dz)) * (((KZ(k + 1) + KZ(k)) / 2.0) * (CZ(k + 1) - CZ(k)) -
+                                              ((KZ(k) + KZ(k - 1)) / 2.0) * (CZ(k) - CZ(k - 1)));
+        RZ(0) = RZ(0) + (1.0 / (dz * dz)) * (((KZ(1) + KZ(0)) / 2.0) * (CZ(1) - CZ(0))) + (1.0 / dz) * (psf);
+        RZ(nz - 2) = RZ(nz - 2) + (1.0 / (dz * dz)) * (0.0 - ((KZ(nz - 2) + KZ(nz - 3)) / 2.0) * (CZ(nz - 2) - CZ(nz - 3)));
+#undef KZ
+#undef CZ
+#undef RZ
+      }
+    /* :4222-4256 point source over two levels */
+    if (src_on) {
+      double width = D->src_sigma * dx;
+      double q = D->Q_src[ipc];
+      int k0 = D->src_k - 1;
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          double fw = wsrc[j * nx + i] / (2.0 * PI_D * pow(width / dx, 2.0)) / wsum;
+          R[(size_t)k0 * np + (size_t)j * ld + i] += D->src_frac * q * fw;
+          R[(size_t)(k0 + 1) * np + (size_t)j * ld + i] += (1.0 - D->src_frac) * q * fw;
+        }
+    }
+    /* :4356-4448 breakup transfer from every larger bin */
+    if (D->pbe) {
+      for (int jp = ipc + 1; jp < npc; ++jp) {
+        double bij = beta[ipc * npc + jp];
+        const double di = D->diameter[ipc], dj = D->diameter[jp];
+        double dc_i = di * di * di, dc_j = dj * dj * dj;                      /* diameter**3 */
+        double d0 = pow(dc_j - dc_i, 1.0 / 3.0);
+        int cnt = -1;
+        double wlo = 0.0, whi = 0.0;
+        if (d0 > D->diameter[0]) {
+          for (int cn = 1; cn < npc; ++cn)
+            if (!(d0 > D->diameter[cn])) {
+              double a1 = fabs(D->diameter[cn] - d0), b1 = fabs(D->diameter[cn - 1] - d0);
+              double ab = a1 + b1;
+              cnt = cn;
+              double dl = D->diameter[cn - 1], dh = D->diameter[cn];
+              wlo = (a1 / ab) * ((d0 * d0 * d0) / (dl * dl * dl));
+              whi = (b1 / ab) * ((d0 * d0 * d0) / (dh * dh * dh));
+              break;
+            }
+        }
+        for (int k = 0; k < nz - 1; ++k)
+          for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+              size_t s = (size_t)k * np + (size_t)j * ld + i;
+              double T = gfreq[(size_t)jp * nv + s] * bij * PC[(size_t)jp * nv + s];
+              PB[(size_t)ipc * nv + s] += T;
+              PB[(size_t)jp * nv + s] -= T;
+              if (d0 > D->diameter[0]) {
+                if (cnt > 0) {
+                  PB[(size_t)(cnt - 1) * nv + s] += T * wlo;
+                  PB[(size_t)cnt * nv + s] += T * whi;
+                }
+              } else {
+                PB[s] += T * ((d0 * d0 * d0) / (D->diameter[0] * D->diameter[0] * D->diameter[0]));
+              }
+            }
+      }
+    }
+#undef CC
+  }
+  /* :1657 (commented out in the reference): optional feedback of the breakup source */
+  if (D->pbe && D->pbe_feedback)
+    for (size_t i = 0; i < nv * npc; ++i) RH[i] += PB[i];
+  /* :1660-1665 first droplet step: Euler */
+  if ((c->jt_total == D->pcon_init || (c->jt_total == D->src_jt0)) && !D->init_pcon)
+    memcpy(RF, RH, nv * npc * sizeof(double));
+  /* :1673-1675 -> :1795-1843 calcbeta_pcon */
+  double *bp = c->f[LP_BETA_PCON];
+  memset(bp, 0, nv * sizeof(double));
+  if (D->active) {
+    double gh = D->g * (D->z_i / (D->u_star * D->u_star));
+    for (int k = 0; k < nz - 1; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          size_t s = (size_t)k * np + (size_t)j * ld + i;
+          double t = 0.0;
+          for (int ip = 0; ip < npc; ++ip) {
+            double dd = D->diameter[ip];
+            double V = PI_D / 6 * (dd * dd * dd);                             /* param.f90:665 */
+            t = t + gh * (D->rho_c / D->rho_w - 1.0) * V * PC[(size_t)ip * nv + s];
+          }
+          bp[s] = t;
+        }
+  }
+  /* :4481-4508 step_pollen2 */
+  for (int ip = 0; ip < npc; ++ip) {
+    double *C = PC + (size_t)ip * nv, *R = RH + (size_t)ip * nv, *F = RF + (size_t)ip * nv;
+    for (int k = 0; k < nz - 1; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          size_t s = (size_t)k * np + (size_t)j * ld + i;
+          C[s] = C[s] + dt * (1.5 * R[s] - 0.5 * F[s]);
+        }
+    memcpy(C + (size_t)(nz - 1) * np, C + (size_t)(nz - 2) * np, np * sizeof(double));
+  }
+#undef FI
+  free(ui0); free(vi0); free(ui); free(vi); free(wi); free(Kc);
+  free(cx_); free(cy_); free(cz_); free(gx0); free(gxL); free(gy0); free(gyL);
+  free(wsrc); free(beta); free(gfreq);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
 /* One time step: main.f90:207-826 (+:864-898 when droplets need dudt)       */
 /* ------------------------------------------------------------------------- */
 static int step_once(or_ctx *c) {
@@ -1221,7 +1710,11 @@ static int step_once(or_ctx *c) {
   or_ddz_w(c, f[LP_DWDZ], f[LP_W]);
   if ((rc = or_wallstress(c))) return rc;                                   /* :315 */
   if ((rc = or_sgs_stag(c))) return rc;                                     /* :327 */
-  if (cf->drop.npcon > 0) return fail(LP_ERR_ARG, "droplet step not available in the oracle");
+  if (cf->drop.npcon > 0 && c->jt_total >= cf->drop.pcon_init) {                 /* :360-372 */
+    if ((rc = or_pollen_all_in_one2(c))) return rc;
+  } else {
+    memset(f[LP_BETA_PCON], 0, nv * sizeof(double));                         /* :352 */
+  }
   or_divstress_uv(c, f[LP_DIVTX], f[LP_TXX], f[LP_TXY], f[LP_TXZ]);         /* :394-397 */
   or_divstress_uv(c, f[LP_DIVTY], f[LP_TXY], f[LP_TYY], f[LP_TYZ]);
   or_divstress_w(c, f[LP_DIVTZ], f[LP_TXZ], f[LP_TYZ], f[LP_TZZ]);
@@ -1233,6 +1726,7 @@ static int step_once(or_ctx *c) {
     f[LP_DUDT][i] = -f[LP_DIVTX][i];
     f[LP_DVDT][i] = -f[LP_DIVTY][i];
     f[LP_DWDT][i] = -f[LP_DIVTZ][i];
+    if (cf->drop.npcon > 0 && cf->drop.active) f[LP_RHSZ][i] += f[LP_BETA_PCON][i];  /* :458-460 */
   }
   double force = cf->use_mean_p_force ? cf->mean_p_force : 0.0;             /* :571-575 */
   if (c->jt == 1 && !cf->initu) {                                           /* :577-584 */
@@ -1265,6 +1759,7 @@ static int step_once(or_ctx *c) {
     f[LP_DWDT][i] -= f[LP_DPDZ][i];
   }
   if ((rc = or_project(c))) return rc;                                      /* :826 */
+  if (cf->drop.npcon > 0) or_ke_dudt(c);                                    /* :864-898 */
   return 0;
 }
 

@@ -55,14 +55,16 @@ class lp_droplet_config(C.Structure):
     _fields_ = [
         ("npcon", C.c_int32), ("pbe", C.c_int32), ("pbe_feedback", C.c_int32),
         ("settling", C.c_int32), ("inertia", C.c_int32), ("active", C.c_int32),
-        ("inlet_x", C.c_int32), ("inlet_y", C.c_int32),
+        ("bc_x", C.c_int32), ("bc_y", C.c_int32),
         ("src_i", C.c_int32), ("src_j", C.c_int32), ("src_k", C.c_int32),
-        ("pad_", C.c_int32),
-        ("src_sigma", C.c_double), ("Sc", C.c_double),
+        ("src_ngau", C.c_int32), ("src_jt0", C.c_int32), ("src_jt1", C.c_int32),
+        ("pcon_init", C.c_int32), ("init_pcon", C.c_int32),
+        ("src_sigma", C.c_double), ("src_frac", C.c_double),
+        ("Sc", C.c_double), ("sfc_flux", C.c_double),
         ("rho_c", C.c_double), ("rho_w", C.c_double),
         ("sigma_s", C.c_double), ("nu_c", C.c_double), ("nu_d", C.c_double),
         ("k_b", C.c_double), ("d_min", C.c_double),
-        ("g", C.c_double), ("u_star", C.c_double),
+        ("g", C.c_double), ("u_star", C.c_double), ("z_i", C.c_double),
         ("diameter", C.POINTER(C.c_double)),
         ("settling_vel", C.POINTER(C.c_double)),
         ("Q_src", C.POINTER(C.c_double)),
