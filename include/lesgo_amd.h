@@ -236,7 +236,9 @@ enum lp_kernel_class {
   LP_K_ROW_C2R = 2,        /* row pass half spectrum -> real                        */
   LP_K_LAG_RELAX = 3,      /* Germano contractions + Lagrangian relaxation           */
   LP_K_PRESS_TRIDIAG = 4,  /* per-mode binary64 Thomas solve                        */
-  LP_K_NCLASSES = 5
+  LP_K_POLLEN_RHS = 5,     /* droplet transport RHS of all bins (SMART + diffusion)   */
+  LP_K_PBE = 6,            /* droplet breakup source of all bins                     */
+  LP_K_NCLASSES = 7
 };
 /* Start (kernel_class >= 0) or stop (-1) event timing; starting clears the totals. */
 int lp_prof_enable(lp_ctx *ctx, int32_t kernel_class);

@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -13,6 +14,7 @@
 #include "../../include/lesgo_amd.h"
 #include "lp_internal.hpp"
 #include "lp_kernels.hpp"
+#include "lp_droplets.hpp"
 
 using namespace lp;
 
@@ -117,6 +119,15 @@ struct lp_ctx {
   int nred = 0;
   int jt = 1, jt_total = 1;
   int F_LM_MM_init = 0, F_QN_NN_init = 0;
+  // droplet bins (npcon > 0)
+  double *d_ws = nullptr, *d_qsrc = nullptr, *d_vfac = nullptr;  // per bin
+  double *d_foot = nullptr;                    // normalised source footprint (nx*ny)
+  PbeBin *d_bins = nullptr;
+  PbePair *d_pairs = nullptr;
+  int npairs = 0;
+  double *ui0 = nullptr, *vi0 = nullptr;        // face velocities (ld, ny+1, nz)
+  double *kcell = nullptr;                      // Kc_t on cell centres
+  double kpref = 0.0, ustar3 = 0.0;
   // event timing of one kernel class (lp_prof_enable)
   int prof_class = -1;
   std::vector<hipEvent_t> ev;
@@ -255,6 +266,157 @@ static void host_filter(const lp_grid &g, const std::vector<double> &k2, double 
     }
 }
 
+// scalars_module_crossflow_mod.f90:4530-4597: daughter-size probability from the
+// surface-energy increase of each split (Tsouris-Tavlarides style).  beta[i*n+j] is the
+// probability that a drop of bin j (> i) yields a drop of bin i.
+static void host_breakup_probability(const double *d, int n, double sigma, double d_min, double *beta) {
+  std::vector<double> cube(n), sq(n), e_max(n), e_min(n), e_split((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    cube[i] = std::pow(d[i], 3.0);
+    sq[i] = std::pow(d[i], 2.0);
+  }
+  for (int i = 0; i < n; ++i) {
+    e_max[i] = M_PI * sigma * 0.25992104 * sq[i];
+    e_min[i] = M_PI * sigma * (std::pow(cube[i] - d_min * d_min * d_min, 2.0 / 3.0) + d_min * d_min - sq[i]);
+    for (int j = i + 1; j < n; ++j)
+      e_split[(size_t)i * n + j] = M_PI * sigma * (std::pow(cube[j] - cube[i], 2.0 / 3.0) + sq[i] - sq[j]);
+  }
+  for (int q = 0; q < n * n; ++q) beta[q] = 0.0;
+  for (int j = 1; j < n; ++j) {
+    double norm = 0.0;
+    for (int m = 0; m < j; ++m) norm += e_min[j] + e_max[j] - e_split[(size_t)m * n + j];
+    for (int i = 0; i < j; ++i) beta[(size_t)i * n + j] = (e_min[j] + e_max[j] - e_split[(size_t)i * n + j]) / norm;
+  }
+}
+
+// :2167-2230 Ohnesorge-number fit coefficients of the breakage frequency of one bin
+static PbeBin host_bin_fit(const lp_droplet_config &D, double dip) {
+  PbeBin B;
+  const double Oh = std::pow(D.rho_c * D.nu_d * D.nu_d / D.sigma_s / dip, 0.5);
+  B.a = 2.392 * std::exp(-26.76 * Oh) + 2.877 * std::exp(-0.1244 * Oh);
+  B.b = 0.45;
+  B.c = -std::pow(10.0, (0.5446 * std::pow(Oh, -0.3776) / (1 + 12.67 * std::pow(Oh, 1.462))));
+  B.d = -(4.172 * std::pow(Oh, -0.5492)) / (1 + 0.5079 * std::pow(Oh, -0.4879));
+  // 0.7182 is a default-kind (binary32) literal in the reference
+  const double l1 = std::log10(Oh + 1);
+  B.e = std::pow(10.0, 0.4113 * std::exp(-55.94 * l1) + 0.5125 * std::exp(-(double)0.7182f * l1));
+  B.dpow_tau = std::pow(dip, -0.6666667);
+  B.dpow_re = std::pow(dip, 4.0 / 3.0);
+  return B;
+}
+
+static int droplet_check(const lp_config &cf) {
+  const lp_droplet_config &D = cf.drop;
+  const lp_grid &g = cf.grid;
+  if (D.npcon < 0 || D.npcon > 32) return set_err(LP_ERR_ARG, "npcon must be in 0..32 (got %d)", D.npcon);
+  if (D.npcon == 0) return LP_OK;
+  if (!D.diameter || !D.settling_vel || !D.Q_src)
+    return set_err(LP_ERR_ARG, "droplets: diameter, settling_vel and Q_src must be given");
+  for (int b = 0; b < D.npcon; ++b) {
+    if (!(D.diameter[b] > 0.0)) return set_err(LP_ERR_ARG, "droplets: diameter[%d] must be > 0", b);
+    if (b > 0 && !(D.diameter[b] > D.diameter[b - 1]))
+      return set_err(LP_ERR_ARG, "droplets: diameters must increase with the bin index");
+  }
+  if (D.bc_x < 0 || D.bc_x > 2 || D.bc_y < 0 || D.bc_y > 2)
+    return set_err(LP_ERR_ARG, "droplets: bc_x/bc_y must be 0 (periodic), 1 (inlet) or 2 (upwind ghosts)");
+  if (!(D.Sc > 0.0) || !(D.u_star > 0.0) || !(D.z_i > 0.0) || !(D.g > 0.0) || !(D.nu_c > 0.0))
+    return set_err(LP_ERR_ARG, "droplets: Sc, u_star, z_i, g and nu_c must be > 0");
+  if (D.src_jt0 < D.src_jt1 && (D.src_k < 1 || D.src_k > g.nz - 2))
+    return set_err(LP_ERR_ARG, "droplets: src_k must lie in 1..nz-2 (got %d)", D.src_k);
+  if (D.src_jt0 < D.src_jt1 && !(D.src_sigma > 0.0)) return set_err(LP_ERR_ARG, "droplets: src_sigma must be > 0");
+  return LP_OK;
+}
+
+static int upload_vec(void **dst, const void *src, size_t bytes) {
+  RCHK(dalloc(dst, bytes));
+  hipError_t e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return set_err(LP_ERR_HIP, "table upload failed: %s", hipGetErrorString(e));
+  return LP_OK;
+}
+
+// per-bin tables, source footprint and droplet scratch
+static int droplet_setup(lp_ctx *c) {
+  const lp_droplet_config &D = c->cfg.drop;
+  const lp_grid &g = c->g;
+  const int n = D.npcon;
+  const long nv = vol_el(c);
+  std::vector<double> vfac(n);
+  const double gh = D.g * (D.z_i / (D.u_star * D.u_star));
+  for (int b = 0; b < n; ++b) {  // :1795-1843 buoyancy weight of one bin (param.f90:665 volume)
+    const double dd = D.diameter[b];
+    const double V = M_PI / 6 * (dd * dd * dd);
+    vfac[b] = gh * (D.rho_c / D.rho_w - 1.0) * V;
+  }
+  RCHK(upload_vec((void **)&c->d_ws, D.settling_vel, sizeof(double) * n));
+  RCHK(upload_vec((void **)&c->d_qsrc, D.Q_src, sizeof(double) * n));
+  RCHK(upload_vec((void **)&c->d_vfac, vfac.data(), sizeof(double) * n));
+  // :4222-4256 footprint normalised to unit sum
+  std::vector<double> foot((size_t)g.nx * g.ny, 0.0);
+  if (D.src_jt0 < D.src_jt1) {
+    const double width = D.src_sigma * g.dx;
+    const double area = 2.0 * M_PI * std::pow(width / g.dx, 2.0);
+    double wsum = 0.0;
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i) {
+        const double rx = ((i + 1) - D.src_i) * g.dx, ry = ((j + 1) - D.src_j) * g.dy;
+        const double r2 = rx * rx + ry * ry;
+        const double wv = std::exp(-std::pow(r2, D.src_ngau / 2) / (2.0 * std::pow(width, D.src_ngau)));
+        foot[(size_t)j * g.nx + i] = wv;
+        wsum += wv / area;
+      }
+    for (auto &x : foot) x = x / area / wsum;
+  }
+  RCHK(upload_vec((void **)&c->d_foot, foot.data(), sizeof(double) * foot.size()));
+  // :2167-2230, :4356-4448 breakup tables
+  std::vector<PbeBin> bins(n);
+  for (int b = 0; b < n; ++b) bins[b] = host_bin_fit(D, D.diameter[b]);
+  std::vector<double> beta((size_t)n * n);
+  host_breakup_probability(D.diameter, n, D.sigma_s, D.d_min, beta.data());
+  std::vector<PbePair> pairs;
+  const double *d = D.diameter;
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      PbePair q;
+      q.i = i; q.j = j; q.lo = -1; q.hi = -1;
+      q.beta = beta[(size_t)i * n + j];
+      q.wlo = q.whi = 0.0;
+      const double d0 = std::pow(d[j] * d[j] * d[j] - d[i] * d[i] * d[i], 1.0 / 3.0);
+      if (d0 > d[0]) {  // split the complementary drop between the two nearest bins
+        for (int cn = 1; cn < n; ++cn)
+          if (!(d0 > d[cn])) {
+            const double a1 = std::fabs(d[cn] - d0), b1 = std::fabs(d[cn - 1] - d0);
+            const double ab = a1 + b1;
+            q.lo = cn - 1;
+            q.hi = cn;
+            q.wlo = (a1 / ab) * ((d0 * d0 * d0) / (d[cn - 1] * d[cn - 1] * d[cn - 1]));
+            q.whi = (b1 / ab) * ((d0 * d0 * d0) / (d[cn] * d[cn] * d[cn]));
+            break;
+          }
+      } else {  // smaller than the first bin: its volume goes to bin 1
+        q.lo = 0;
+        q.wlo = (d0 * d0 * d0) / (d[0] * d[0] * d[0]);
+      }
+      pairs.push_back(q);
+    }
+  c->npairs = (int)pairs.size();
+  RCHK(upload_vec((void **)&c->d_bins, bins.data(), sizeof(PbeBin) * n));
+  RCHK(upload_vec((void **)&c->d_pairs, pairs.data(), sizeof(PbePair) * std::max<size_t>(pairs.size(), 1)));
+  c->kpref = D.k_b * std::pow(2.1, 0.5) * M_PI / 4.0 * 0.84;
+  c->ustar3 = std::pow(D.u_star, 3.0);
+  const size_t fb = sizeof(double) * (size_t)g.ld * (g.ny + 1) * g.nz;
+  RCHK(dalloc((void **)&c->ui0, fb));
+  RCHK(dalloc((void **)&c->vi0, fb));
+  RCHK(dalloc((void **)&c->kcell, sizeof(double) * nv));
+  return LP_OK;
+}
+
+extern "C" int lp_calculate_probability(const double *diameter, int32_t npcon, double sigma_s, double d_min,
+                                        double *beta) {
+  if (!diameter || !beta || npcon < 1) return set_err(LP_ERR_ARG, "lp_calculate_probability: bad argument");
+  host_breakup_probability(diameter, npcon, sigma_s, d_min, beta);
+  return LP_OK;
+}
+
 extern "C" int lp_destroy(lp_ctx *c) {
   if (!c) return LP_OK;
   (void)hipSetDevice(c->dev);
@@ -276,6 +438,8 @@ extern "C" int lp_destroy(lp_ctx *c) {
   fr(c->bw); fr(c->tw); fr(c->gam);
   for (auto p : c->pspec) fr(p);
   fr(c->d_err); fr(c->d_part);
+  fr(c->d_ws); fr(c->d_qsrc); fr(c->d_vfac); fr(c->d_foot); fr(c->d_bins); fr(c->d_pairs);
+  fr(c->ui0); fr(c->vi0); fr(c->kcell);
   for (auto e : c->ev) (void)hipEventDestroy(e);
   if (c->s) (void)hipStreamDestroy(c->s);
   delete c;
@@ -299,7 +463,7 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
   if (cfg->lbc_mom != LP_LBC_WALL && cfg->lbc_mom != LP_LBC_STRESS_FREE)
     return set_err(LP_ERR_ARG, "invalid lbc_mom");
   if (cfg->cs_count < 1) return set_err(LP_ERR_ARG, "cs_count must be >= 1");
-  if (cfg->drop.npcon > 0) return set_err(LP_ERR_ARG, "lp_create: droplet bins are not available yet");
+  RCHK(droplet_check(*cfg));
   lp_ctx *c = new lp_ctx();
   c->cfg = *cfg;
   c->g = g;
@@ -325,8 +489,9 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
     long n = nv;
     if (i == LP_P) n = np * (g.nz + 1);
     if (i == LP_USTAR) n = np;
-    if (i == LP_PCON || i == LP_RHS_PCON || i == LP_RHS_PCON_F || i == LP_RHS_PBE || i == LP_BETA_PCON)
-      n = 1;  // droplets not allocated in this build
+    if (i == LP_PCON || i == LP_RHS_PCON || i == LP_RHS_PCON_F || i == LP_RHS_PBE)
+      n = cfg->drop.npcon > 0 ? nv * cfg->drop.npcon : 1;
+    if (i == LP_BETA_PCON && cfg->drop.npcon == 0) n = 1;
     c->esz[i] = field_is_f64(c, i) ? 8 : 4;
     c->bytes[i] = (size_t)n * c->esz[i];
     CK(dalloc(&c->f[i], c->bytes[i]));
@@ -383,6 +548,14 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
   CK(dalloc((void **)&c->d_err, sizeof(int) * 4));
   c->nred = 512;
   CK(dalloc((void **)&c->d_part, sizeof(double) * 3 * c->nred));
+  if (cfg->drop.npcon > 0) {
+    rc = droplet_setup(c);  // reads the caller's bin arrays, which are not kept
+    c->cfg.drop.diameter = c->cfg.drop.settling_vel = c->cfg.drop.Q_src = nullptr;
+    if (rc != LP_OK) {
+      lp_destroy(c);
+      return rc;
+    }
+  }
 #undef CK
   *out = c;
   return LP_OK;
@@ -808,6 +981,92 @@ static int do_press(lp_ctx *c) {
   return LP_OK;
 }
 
+static DropGeo drop_geo(const lp_ctx *c) {
+  DropGeo d;
+  d.nx = c->g.nx; d.ny = c->g.ny; d.nz = c->g.nz; d.ld = c->g.ld;
+  d.npc = c->cfg.drop.npcon;
+  d.np = plane_el(c);
+  d.nv = vol_el(c);
+  d.fp = (long)c->g.ld * (c->g.ny + 1);
+  return d;
+}
+
+// scalars_module_crossflow_mod.f90:1494-1786 pollen_all_in_one2: RHS_PCon_f = RHS_PCon,
+// pollen_RHS_calc2, optional breakup feedback, Euler start, calcbeta_pcon, step_pollen2
+static int do_pollen(lp_ctx *c) {
+  const lp_config &cf = c->cfg;
+  const lp_droplet_config &D = cf.drop;
+  const lp_grid &gg = c->g;
+  DropGeo dg = drop_geo(c);
+  // :1521 -- RHS_PCon_f takes the old RHS by exchanging the buffers (k_pollen_rhs
+  // rewrites every point of the new RHS_PCon)
+  std::swap(c->f[LP_RHS_PCON], c->f[LP_RHS_PCON_F]);
+  double *PC = (double *)c->f[LP_PCON], *RH = (double *)c->f[LP_RHS_PCON], *RF = (double *)c->f[LP_RHS_PCON_F];
+  FDISPATCH(c, T, {
+    hipLaunchKernelGGL(k_face_u<T>, dim3((unsigned)(((long)gg.ny * gg.nz + 3) / 4)), dim3(256), 0, c->s,
+                       (const T *)c->f[LP_U], (const T *)c->f[LP_DUDX], c->ui0, dg, gg.dx);
+    hipLaunchKernelGGL(k_face_v<T>, grid1((long)gg.nx * gg.nz), dim3(256), 0, c->s, (const T *)c->f[LP_V],
+                       (const T *)c->f[LP_DVDY], c->vi0, dg, gg.dy);
+    hipLaunchKernelGGL(k_kc<T>, grid1(dg.nv), dim3(256), 0, c->s, (const T *)c->f[LP_NU_T], (T *)c->f[LP_MAGS],
+                       c->kcell, dg, D.Sc);
+  });
+  HIPCHK(hipGetLastError());
+  PollenParams P;
+  P.idx = 1.0 / gg.dx; P.idy = 1.0 / gg.dy; P.idz = 1.0 / gg.dz;
+  P.idx2 = 1.0 / (gg.dx * gg.dx); P.idy2 = 1.0 / (gg.dy * gg.dy); P.idz2 = 1.0 / (gg.dz * gg.dz);
+  P.ghat = D.g / (D.u_star * D.u_star / D.z_i);
+  P.psf = D.sfc_flux > 0 ? 0.0 : D.sfc_flux;  // :2287-2467, lbcp = 1
+  P.frac = D.src_frac;
+  P.frac1 = 1.0 - D.src_frac;
+  P.bc_x = D.bc_x; P.bc_y = D.bc_y;
+  P.settling = D.settling; P.inertia = D.inertia;
+  P.src_on = c->jt_total >= D.src_jt0 && c->jt_total < D.src_jt1;
+  P.src_k0 = D.src_k - 1;
+  const double nb = (double)dg.nv * dg.npc * 8.0;
+  prof_begin(c, LP_K_POLLEN_RHS);
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_pollen_rhs<T>, grid1(dg.nv), dim3(256), 0, c->s, PC, RH, RF, c->ui0, c->vi0,
+                                     (const T *)c->f[LP_W], (const T *)c->f[LP_DUDT], (const T *)c->f[LP_DVDT],
+                                     (const T *)c->f[LP_DWDT], c->kcell, c->d_ws, c->d_qsrc, c->d_foot, dg, P));
+  HIPCHK(hipGetLastError());
+  // PCon read once, RHS_PCon written once, plus the bin-independent inputs
+  prof_end(c, LP_K_POLLEN_RHS, 2.0 * nb + (double)dg.nv * (8.0 + 3.0 * esz_of(c->fdt)) + 4.0 * dg.fp * gg.nz * 8.0);
+  if (D.pbe) {
+    prof_begin(c, LP_K_PBE);
+    size_t lds = sizeof(double) * 2 * 64 * (size_t)dg.npc;
+    FDISPATCH(c, T, hipLaunchKernelGGL(k_pbe<T>, dim3((unsigned)((dg.nv + 63) / 64)), dim3(64), lds, c->s, PC,
+                                       (double *)c->f[LP_RHS_PBE], RH, (const T *)c->f[LP_DISSIP], c->d_bins,
+                                       c->d_pairs, c->npairs, dg, c->kpref, c->ustar3, D.z_i, D.nu_c,
+                                       D.pbe_feedback));
+    HIPCHK(hipGetLastError());
+    prof_end(c, LP_K_PBE, (D.pbe_feedback ? 4.0 : 2.0) * nb + (double)dg.nv * esz_of(c->fdt));
+  }
+  if ((c->jt_total == D.pcon_init || c->jt_total == D.src_jt0) && !D.init_pcon)  // :1660-1665
+    HIPCHK(hipMemcpyAsync(RF, RH, c->bytes[LP_RHS_PCON], hipMemcpyDeviceToDevice, c->s));
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_beta_step<T>, grid1(dg.nv), dim3(256), 0, c->s, PC, (const double *)RH,
+                                     (const double *)RF, (T *)c->f[LP_BETA_PCON], c->d_vfac, D.active, dg, cf.dt));
+  HIPCHK(hipGetLastError());
+  return LP_OK;
+}
+
+// main.f90:864-898: dudt += grad(ke) with ke = energy(); the x/y gradients come from
+// filt_da of a copy of ke (only its derivative outputs are used), the z gradient from
+// ddz_uv of ke itself
+static int do_ke_dudt(lp_ctx *c) {
+  const lp_grid &gg = c->g;
+  Geo g = geo(c);
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_energy<T>, grid1(g.np * g.nz), dim3(256), 0, c->s, (const T *)c->f[LP_U],
+                                     (const T *)c->f[LP_V], (const T *)c->f[LP_W], (T *)c->tmp[0], g));
+  HIPCHK(hipGetLastError());
+  const int m[2] = {M_IKX, M_IKY};
+  void *outs[2] = {c->tmp[1], c->tmp[2]};
+  RCHK(spectral_apply(c, c->tmp[0], gg.nz, 1.0 / ((double)gg.nx * gg.ny), 2, m, outs));
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_ke_add<T>, grid1(g.np * (g.nz - 1)), dim3(256), 0, c->s, (T *)c->f[LP_DUDT],
+                                     (T *)c->f[LP_DVDT], (T *)c->f[LP_DWDT], (const T *)c->tmp[1],
+                                     (const T *)c->tmp[2], (const T *)c->tmp[0], g, (T)(1.0 / gg.dz)));
+  HIPCHK(hipGetLastError());
+  return LP_OK;
+}
+
 static int check_err_flag(lp_ctx *c) {
   int h = 0;
   HIPCHK(hipMemcpyAsync(&h, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->s));
@@ -848,6 +1107,16 @@ static int step_once(lp_ctx *c) {
   RCHK(do_ddz_w(c, c->f[LP_DWDZ], c->f[LP_W]));
   RCHK(do_wallstress(c));  // :315
   RCHK(do_sgs_stag(c));    // :327
+  const bool drops = cf.drop.npcon > 0;
+  if (drops) {
+    if (c->jt_total >= cf.drop.pcon_init) {  // :360-372
+      RCHK(do_pollen(c));
+    } else {  // :352
+      FDISPATCH(c, T, hipLaunchKernelGGL(k_fill<T>, grid1(g.np * g.nz), dim3(256), 0, c->s,
+                                         (T *)c->f[LP_BETA_PCON], g.np * g.nz, T(0)));
+      HIPCHK(hipGetLastError());
+    }
+  }
   RCHK(do_divstress(c, c->f[LP_DIVTX], c->f[LP_TXX], c->f[LP_TXY], c->f[LP_TXZ], 0));  // :394-397
   RCHK(do_divstress(c, c->f[LP_DIVTY], c->f[LP_TXY], c->f[LP_TYY], c->f[LP_TYZ], 0));
   RCHK(do_divstress(c, c->f[LP_DIVTZ], c->f[LP_TXZ], c->f[LP_TYZ], c->f[LP_TZZ], 1));
@@ -855,7 +1124,8 @@ static int step_once(lp_ctx *c) {
   FDISPATCH(c, T, hipLaunchKernelGGL(k_rhs_assemble<T>, grid1(n1), dim3(256), 0, c->s, (T *)c->f[LP_RHSX],
                                      (T *)c->f[LP_RHSY], (T *)c->f[LP_RHSZ], (const T *)c->f[LP_DIVTX],
                                      (const T *)c->f[LP_DIVTY], (const T *)c->f[LP_DIVTZ], (T *)c->f[LP_DUDT],
-                                     (T *)c->f[LP_DVDT], (T *)c->f[LP_DWDT], (const T *)nullptr, n1));
+                                     (T *)c->f[LP_DVDT], (T *)c->f[LP_DWDT],
+                                     (const T *)(drops && cf.drop.active ? c->f[LP_BETA_PCON] : nullptr), n1));
   HIPCHK(hipGetLastError());
   if (c->jt == 1 && !cf.initu) {  // :577-584
     HIPCHK(hipMemcpyAsync(c->f[LP_RHSX_F], c->f[LP_RHSX], vb, hipMemcpyDeviceToDevice, c->s));
@@ -877,6 +1147,7 @@ static int step_once(lp_ctx *c) {
                                      (T *)c->f[LP_DWDT], g, gg.dz));
   HIPCHK(hipGetLastError());
   RCHK(do_project(c));  // :826
+  if (drops) RCHK(do_ke_dudt(c));  // :864-898
   return LP_OK;
 }
 
@@ -948,6 +1219,15 @@ extern "C" int lp_lagrange_sdep(lp_ctx *c) {
                                      (T *)c->S[1], (T *)c->S[2], (T *)c->S[3], (T *)c->S[4], (T *)c->S[5], g));
   HIPCHK(hipGetLastError());
   RCHK(do_lagrange(c));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return LP_OK;
+}
+
+extern "C" int lp_pollen_all_in_one2(lp_ctx *c) {
+  if (!c) return set_err(LP_ERR_ARG, "lp_pollen_all_in_one2: null context");
+  if (c->cfg.drop.npcon < 1) return set_err(LP_ERR_ARG, "lp_pollen_all_in_one2 needs npcon > 0");
+  HIPCHK(hipSetDevice(c->dev));
+  RCHK(do_pollen(c));
   HIPCHK(hipStreamSynchronize(c->s));
   return LP_OK;
 }

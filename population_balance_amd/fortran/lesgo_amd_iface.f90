@@ -13,6 +13,7 @@ module lesgo_amd
   public :: lp_divstress_uv, lp_divstress_w, lp_tridag_array
   public :: lp_wallstress, lp_sgs_stag, lp_lagrange_sdep, lp_interpolag_sdep, lp_convec
   public :: lp_press_stag_array, lp_project, lp_timestep_conditions, lp_rmsdiv, lp_step
+  public :: lp_pollen_all_in_one2, lp_calculate_probability
   public :: LP_OK, LP_FP64, LP_MIXED, LP_LBC_WALL, LP_LBC_STRESS_FREE
   public :: LP_U, LP_V, LP_W, LP_P, LP_RHSX, LP_RHSY, LP_RHSZ
 
@@ -30,8 +31,10 @@ module lesgo_amd
 
   type, bind(C) :: lp_droplet_config
     integer(c_int32_t) :: npcon, pbe, pbe_feedback, settling, inertia, active
-    integer(c_int32_t) :: inlet_x, inlet_y, src_i, src_j, src_k, pad_
-    real(c_double) :: src_sigma, Sc, rho_c, rho_w, sigma_s, nu_c, nu_d, k_b, d_min, g, u_star
+    integer(c_int32_t) :: bc_x, bc_y, src_i, src_j, src_k, src_ngau, src_jt0, src_jt1
+    integer(c_int32_t) :: pcon_init, init_pcon
+    real(c_double) :: src_sigma, src_frac, Sc, sfc_flux, rho_c, rho_w
+    real(c_double) :: sigma_s, nu_c, nu_d, k_b, d_min, g, u_star, z_i
     type(c_ptr) :: diameter, settling_vel, Q_src
   end type lp_droplet_config
 
@@ -232,6 +235,18 @@ module lesgo_amd
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: rms
+    end function
+    integer(c_int) function lp_pollen_all_in_one2(ctx) bind(C, name="lp_pollen_all_in_one2")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function lp_calculate_probability(diameter, npcon, sigma_s, d_min, beta) &
+        bind(C, name="lp_calculate_probability")
+      import :: c_int, c_int32_t, c_double
+      real(c_double), intent(in) :: diameter(*)
+      integer(c_int32_t), value :: npcon
+      real(c_double), value :: sigma_s, d_min
+      real(c_double), intent(out) :: beta(*)
     end function
     integer(c_int) function lp_step(ctx, nsteps) bind(C, name="lp_step")
       import :: c_int, c_ptr, c_int32_t

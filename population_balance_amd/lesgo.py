@@ -71,6 +71,8 @@ def load():
         "lp_timestep_conditions": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
         "lp_rmsdiv": ([vp, C.POINTER(C.c_double)], C.c_int),
         "lp_step": ([vp, i32], C.c_int),
+        "lp_pollen_all_in_one2": ([vp], C.c_int),
+        "lp_calculate_probability": ([vp, i32, C.c_double, C.c_double, vp], C.c_int),
         "lp_prof_enable": ([vp, i32], C.c_int),
         "lp_prof_read": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)], C.c_int),
     }
@@ -95,6 +97,18 @@ def _ptr(a):
     if not a.flags["C_CONTIGUOUS"]:
         raise LesgoError("numpy arrays must be C-contiguous")
     return C.c_void_p(a.ctypes.data)
+
+
+def calculate_probability(diameter, sigma_s=15.5e-3, d_min=1e-7):
+    """scalars_module_crossflow_mod.f90:4530-4597 on the host: beta[i, j] is the
+    probability that a drop of bin j breaks into bin i."""
+    L = load()
+    d = np.ascontiguousarray(diameter, np.float64)
+    beta = np.zeros((len(d), len(d)))
+    rc = L.lp_calculate_probability(_ptr(d), len(d), float(sigma_s), float(d_min), _ptr(beta))
+    if rc != abi.LP_OK:
+        raise LesgoError(f"lp_calculate_probability failed ({rc}): {L.lp_last_error().decode()}")
+    return beta
 
 
 class Lesgo:
@@ -250,6 +264,9 @@ class Lesgo:
         a = C.c_double()
         self._check(self.L.lp_rmsdiv(self.h, C.byref(a)), "rmsdiv")
         return a.value
+
+    def pollen_all_in_one2(self):
+        self._check(self.L.lp_pollen_all_in_one2(self.h), "pollen_all_in_one2")
 
     def step(self, n=1):
         self._check(self.L.lp_step(self.h, int(n)), "step")

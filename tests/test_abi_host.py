@@ -107,3 +107,14 @@ def test_fortran_binding_layout():
     size = int(re.search(r"sizeof\(lp_config\) = (\d+)", out).group(1))
     assert size == C.sizeof(abi.lp_config)
     assert "ld = 34" in out
+
+
+def test_host_breakup_probability_matches_oracle():
+    """lp_calculate_probability is a host routine (no device needed): identical to the
+    oracle's restatement of scalars_module_crossflow_mod.f90:4530-4597"""
+    from oracle import calculate_probability as oracle_prob
+    from population_balance_amd import droplets
+    from population_balance_amd.lesgo import calculate_probability
+    for n in (2, 6, 15):
+        d = droplets.diameters(n)
+        assert np.array_equal(calculate_probability(d), oracle_prob(d))

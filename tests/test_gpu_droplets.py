@@ -1,0 +1,164 @@
+"""GPU parity of the multi-bin droplet step (scalars_module_crossflow_mod.f90:1494-4597:
+pollen_all_in_one2 with SMART transport, SGS diffusion, point source, PBE breakup,
+calcbeta_pcon, step_pollen2) and of whole LES steps with droplets (main.f90:360-372,
+458-460, 864-898) against the binary64 oracle.  FP64: 1e-12 relative L2; mixed
+(binary32 velocities, binary64 droplet state): 1e-5."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from population_balance_amd import abi, droplets, ic
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+TOL_MIXED = 1e-5
+DROP_OUT = ["PCon", "RHS_PCon", "RHS_PCon_f", "Rhs_pbe", "beta_pcon", "magS"]
+
+
+def _cfg(nx=32, nz=17, npcon=6, precision=abi.LP_FP64, **kw):
+    cfg = abi.make_config(nx, nx, nz, precision=precision)
+    ds = droplets.DropletSet(npcon)
+    ds.fill(cfg, **kw)
+    return cfg, ds
+
+
+def _pair(cfg):
+    from oracle import Oracle
+    from population_balance_amd.lesgo import Lesgo
+    return Oracle(cfg), Lesgo(cfg)
+
+
+def _routine_state(cfg, ds, seed=5):
+    """every input of pollen_all_in_one2: turbulent velocities with their x/y
+    derivatives, an SGS viscosity and dissipation of realistic sign and size, the
+    accelerations of the inertia term, and a source blob plus positive noise."""
+    from oracle import Oracle
+    g = cfg.grid
+    rng = np.random.default_rng(seed)
+    o = Oracle(cfg)
+    u, v, w = ic.turbulent_ic(g, seed=seed)
+    st = {"u": u, "v": v, "w": w}
+    _, st["dudx"], _ = o.filt_da(u)
+    _, _, st["dvdy"] = o.filt_da(v)
+    shp = abi.field_shape(g, "u")
+
+    def pad0(a):
+        a[..., g.nx:] = 0.0
+        return a
+    st["Nu_t"] = pad0(0.01 * (1.0 + rng.random(shp)))
+    st["dissip"] = pad0(-(0.5 + rng.random(shp)) * 10.0 ** rng.uniform(-3, 1, shp))
+    st["magS"] = pad0(rng.random(shp))
+    for n in ("dudt", "dvdt", "dwdt"):
+        st[n] = pad0(rng.standard_normal(shp))
+    C = droplets.initial_concentration(cfg, ds) + 1e-4 * rng.random(abi.field_shape(g, "PCon", ds.diameter.size))
+    C[..., g.nx:] = 0.0
+    C[:, -1] = C[:, -2]
+    st["PCon"] = C
+    st["RHS_PCon"] = pad0(1e-3 * rng.standard_normal(C.shape))
+    o.close()
+    return st
+
+
+def _load(o, lg, st, jt_total=1):
+    for k, a in st.items():
+        o.set_field(k, a)
+        lg.upload(k, a)
+    o.set_time(1, jt_total)
+    lg.set_time(1, jt_total)
+
+
+def _cmp(o, lg, names, tol, nx):
+    errs = {n: rel_l2(lg.download(n), o.field(n), nx) for n in names}
+    bad = {k: v for k, v in errs.items() if not v < tol}
+    assert not bad, f"relative L2 above {tol}: {bad} (all: {errs})"
+    return errs
+
+
+@pytest.mark.parametrize("bc", [(0, 0), (1, 1), (2, 2)], ids=["periodic", "inlet", "ghost"])
+@pytest.mark.parametrize("phys", ["transport", "settling", "full"])
+def test_pollen_routine(bc, phys):
+    """two consecutive pollen_all_in_one2 calls (Euler start, then AB2 with the rotated
+    RHS) on every boundary treatment"""
+    kw = dict(bc=bc)
+    if phys == "transport":
+        kw.update(pbe=False, settling=False, inertia=False, active=False)
+    elif phys == "settling":
+        kw.update(pbe=False, settling=True, inertia=True, active=True, sfc_flux=-0.5)
+    else:
+        kw.update(pbe=True, settling=True, inertia=True, active=True)
+    cfg, ds = _cfg(**kw)
+    if phys == "full":
+        cfg.drop.pbe_feedback = 1
+    o, lg = _pair(cfg)
+    st = _routine_state(cfg, ds)
+    _load(o, lg, st, jt_total=1)
+    for jt in (1, 2):
+        o.set_time(jt, jt)
+        lg.set_time(jt, jt)
+        o.call("pollen_all_in_one2")
+        lg.pollen_all_in_one2()
+        _cmp(o, lg, DROP_OUT if cfg.drop.pbe else [n for n in DROP_OUT if n != "Rhs_pbe"], TOL, cfg.grid.nx)
+
+
+def test_pollen_source_window_and_init_pcon():
+    """source switched on at jt_total = 3 with initPCon: no Euler restart, AB2 with the
+    uploaded RHS_PCon_f"""
+    cfg, ds = _cfg(bc=(1, 1), jt0=3, jt1=4)
+    cfg.drop.init_pcon = 1
+    o, lg = _pair(cfg)
+    st = _routine_state(cfg, ds, seed=9)
+    st["RHS_PCon_f"] = 0.5 * st["RHS_PCon"]
+    _load(o, lg, st, jt_total=2)
+    for jt in (2, 3, 4):
+        o.set_time(jt, jt)
+        lg.set_time(jt, jt)
+        o.call("pollen_all_in_one2")
+        lg.pollen_all_in_one2()
+        _cmp(o, lg, DROP_OUT, TOL, cfg.grid.nx)
+
+
+@pytest.mark.parametrize("shape,nb,nsteps", [((32, 17), 6, 3), ((64, 17), 15, 2)])
+def test_step_with_droplets_fp64(shape, nb, nsteps):
+    nx, nz = shape
+    cfg, ds = _cfg(nx=nx, nz=nz, npcon=nb, bc=(1, 1))
+    cfg.drop.pbe_feedback = 1
+    o, lg = _pair(cfg)
+    g = cfg.grid
+    u, v, w = ic.turbulent_ic(g, seed=11)
+    C = droplets.initial_concentration(cfg, ds)
+    for k, a in (("u", u), ("v", v), ("w", w), ("PCon", C)):
+        o.set_field(k, a)
+        lg.upload(k, a)
+    o.step(nsteps)
+    lg.step(nsteps)
+    _cmp(o, lg, ["u", "v", "w", "p", "dudt", "dvdt", "dwdt"] + DROP_OUT, TOL, g.nx)
+
+
+def test_step_with_droplets_mixed():
+    cfg, ds = _cfg(nx=32, nz=17, npcon=6, bc=(2, 2), precision=abi.LP_MIXED)
+    from oracle import Oracle
+    from population_balance_amd.lesgo import Lesgo
+    cfg64, _ = _cfg(nx=32, nz=17, npcon=6, bc=(2, 2))
+    cfg64.drop = cfg.drop
+    o, lg = Oracle(cfg64), Lesgo(cfg)
+    g = cfg.grid
+    u, v, w = ic.turbulent_ic(g, seed=13)
+    C = droplets.initial_concentration(cfg, ds)
+    for k, a in (("u", u), ("v", v), ("w", w), ("PCon", C)):
+        o.set_field(k, a)
+        lg.upload(k, a)
+    o.step(3)
+    lg.step(3)
+    _cmp(o, lg, ["u", "v", "w", "PCon", "RHS_PCon", "Rhs_pbe", "beta_pcon"], TOL_MIXED, g.nx)
+
+
+def test_droplet_config_rejected():
+    from population_balance_amd.lesgo import Lesgo, LesgoError
+    cfg, ds = _cfg(bc=(3, 0))
+    with pytest.raises(LesgoError, match="bc_x"):
+        Lesgo(cfg)
+    cfg, ds = _cfg()
+    cfg.drop.src_k = cfg.grid.nz - 1
+    with pytest.raises(LesgoError, match="src_k"):
+        Lesgo(cfg)
