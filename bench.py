@@ -67,6 +67,39 @@ def class_breakdown(lg, nsteps=2):
     return out
 
 
+def droplet_leg(steps, warmup, seed, nx=192, nz=128, nbins=15):
+    """the oil-plume workload: velocity step plus 15 droplet-size bins (binary64
+    concentrations, binary32 velocities), log-spaced 10 um - 2 mm, Gaussian point source
+    (sigma 3 cells), settling + inertia, PBE breakup, buoyancy feedback."""
+    from population_balance_amd import droplets
+    from population_balance_amd.lesgo import Lesgo
+    cfg = abi.make_config(nx, nx, nz, precision=abi.LP_MIXED, use_mean_p_force=1, mean_p_force=1.0)
+    ds = droplets.DropletSet(nbins)
+    ds.fill(cfg)
+    u, v, w = ic.turbulent_ic(cfg.grid, seed=seed)
+    lg = Lesgo(cfg)
+    for k, a in (("u", u), ("v", v), ("w", w), ("PCon", droplets.initial_concentration(cfg, ds))):
+        lg.upload(k, a)
+    lg.step(warmup)
+    lg.sync()
+    t0 = time.perf_counter()
+    lg.step(steps)
+    dt = time.perf_counter() - t0
+    pts = nx * nx * (nz - 1)
+    out = {"workload": f"LES step + {nbins} droplet bins {nx}x{nx}x{nz}, mixed",
+           "value": round(pts * steps / dt / 1e6, 2), "unit": "Mpt-steps/s",
+           "ms_per_step": round(dt / steps * 1e3, 4)}
+    for name in ("pollen_rhs", "pbe"):
+        lg.prof_enable(abi.KERNEL_CLASSES[name])
+        lg.step(2)
+        ms, n, b = lg.prof_read()
+        out[name] = {"ms_per_step": round(ms / 2, 4),
+                     "GBps": round(b / (ms * 1e-3) / 1e9, 1) if ms > 0 else 0.0}
+    lg.prof_enable(-1)
+    lg.close()
+    return out
+
+
 def cpu_baseline(nx, seed):
     from oracle import Oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
@@ -97,6 +130,7 @@ def main():
     ap.add_argument("--seed", type=int, default=ic.DEFAULT_SEED)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-droplets", action="store_true", help="skip the 192x192x128 droplet workload")
     ap.add_argument("--breakdown", action="store_true", help="per-kernel-class timing to stderr")
     ap.add_argument("--roofline-class", default="col_op", choices=list(abi.KERNEL_CLASSES))
     a = ap.parse_args()
@@ -169,6 +203,10 @@ def main():
         result["rel_l2_mixed_vs_fp64"] = {"after_steps": a.steps + a.warmup, **errs}
     else:
         lg_m.close()
+    if not a.no_droplets:
+        dl = droplet_leg(a.steps, a.warmup, a.seed)
+        log(f"[bench] droplets: {dl['ms_per_step']:.3f} ms/step, {dl['value']:.1f} Mpt-steps/s")
+        result["droplet_step"] = dl
     if not a.no_cpu:
         log("[bench] CPU baseline (oracle) ...")
         cb = cpu_baseline(nx, a.seed)

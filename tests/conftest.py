@@ -16,8 +16,9 @@ def pytest_configure(config):
 
 def rel_l2(a, b, nx=None):
     """relative L2 error of a against b over the physical columns (x < nx)."""
-    a = np.asarray(a, np.float64)
-    b = np.asarray(b, np.float64)
+    dt = np.complex128 if (np.iscomplexobj(a) or np.iscomplexobj(b)) else np.float64
+    a = np.asarray(a, dt)
+    b = np.asarray(b, dt)
     if nx is not None:
         a = a[..., :nx]
         b = b[..., :nx]
