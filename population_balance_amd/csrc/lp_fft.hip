@@ -6,203 +6,267 @@
 //   col_op  : per kx column, complex FFT along y, a spectral map, inverse FFT along y
 //   row_c2r : M+1 complex -> 2M reals (merge + half-length inverse FFT), using only the
 //             real parts of the kx = 0 and kx = M terms (FFTW2 halfcomplex semantics)
-// Rows and columns are staged in LDS; each wave64 group handles whole transforms.
-#include "lp_device.hpp"
+// Every transform is a register-resident four-step FFT (lp_fft4.hpp) with one LDS
+// exchange; loads and stores go straight between HBM and registers.
+#include "lp_fft4.hpp"
 #include "lp_internal.hpp"
 
 namespace lp {
-
-static constexpr int kBlock = 256;
 
 template <typename T> struct V2;
 template <> struct V2<double> { using t = double2; };
 template <> struct V2<float> { using t = float2; };
 
-__host__ __device__ constexpr int row_tpr(int M) {  // threads per row
-  return M >= 128 ? 64 : M >= 64 ? 32 : M >= 32 ? 16 : 8;
+// lanes per row transform of M complex points (P) -- M = P*Q
+__host__ __device__ constexpr int row_lanes(int M) {
+  return M == 16 ? 4 : M == 24 ? 4 : M == 32 ? 4 : M == 48 ? 4 : M == 64 ? 8 : M == 72 ? 6
+       : M == 96 ? 8 : M == 128 ? 8 : M == 144 ? 12 : M == 192 ? 8 : 0;
 }
+static constexpr int kRowBlock = 256;
+
+// LDS column stride (elements) of a transform slot holding n elements: rounded so that
+// consecutive slots start 4 elements apart modulo 32
+__host__ __device__ constexpr int slot_stride(int n) { return ((n + 27) / 32) * 32 + 4; }
 
 // ---------------------------------------------------------------------------------
-// row_r2c
+// row_r2c: real rows of 2M values -> M+1 complex (half-length complex FFT + split)
 // ---------------------------------------------------------------------------------
 template <typename T, int M>
-__global__ __launch_bounds__(kBlock) void k_row_r2c(const T *__restrict__ in, long in_pitch, long in_plane,
-                                                    int rows, int nplanes, T scale, cx<T> *__restrict__ out,
-                                                    long out_pitch, long out_plane, int ncols_out,
-                                                    const cx<T> *__restrict__ twM,
-                                                    const cx<T> *__restrict__ twN) {
-  constexpr int TPR = row_tpr(M);
-  constexpr int RPB = kBlock / TPR;
-  __shared__ cx<T> sA[RPB][M];
-  __shared__ cx<T> sB[RPB][M];
-  const int r = threadIdx.x / TPR, t = threadIdx.x % TPR;
-  const long grow = (long)blockIdx.x * RPB + r;
-  const long total = (long)rows * nplanes;
-  const bool valid = grow < total;
+__global__ __launch_bounds__(kRowBlock) void k_row_r2c(const T *__restrict__ in, long in_pitch, long in_plane,
+                                                       int rows, int nplanes, T scale, cx<T> *__restrict__ out,
+                                                       long out_pitch, long out_plane, int ncols_out,
+                                                       const cx<T> *__restrict__ twM,
+                                                       const cx<T> *__restrict__ twN) {
+  constexpr int P = row_lanes(M);
+  using F = Fft4<T, M, P, -1>;
+  constexpr int Q = F::Q;
+  constexpr int RB = kRowBlock / P;
+  constexpr int SS = slot_stride(F::SCR > M ? F::SCR : M);
+  __shared__ cx<T> lds[RB * SS];
+  const int t = threadIdx.x % P, r = threadIdx.x / P;
+  const long grow = (long)blockIdx.x * RB + r;
+  const bool valid = grow < (long)rows * nplanes;
   const long plane = valid ? grow / rows : 0, y = valid ? grow % rows : 0;
-  const T *src = in + plane * in_plane + y * in_pitch;
+  cx<T> *sc = lds + r * SS;
   using v2 = typename V2<T>::t;
-  if (valid) {
-    for (int n = t; n < M; n += TPR) {
-      v2 p = *reinterpret_cast<const v2 *>(src + 2 * n);
-      sA[r][n] = mk<T>(p.x * scale, p.y * scale);
-    }
+  cx<T> v[Q];
+  const T *src = in + plane * in_plane + y * in_pitch;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    v2 p = valid ? *reinterpret_cast<const v2 *>(src + 2 * (t + P * q)) : v2{0, 0};
+    v[q] = mk<T>(p.x * scale, p.y * scale);
   }
+  F::stage1(v, t, sc, twM);
   __syncthreads();
-  cx<T> *Z = fft_lds<T, M, -1>(&sA[r][0], &sB[r][0], t, TPR, twM);
-  // split: X[k] = (Z[k]+conj(Z[M-k]))/2 - i/2 w^k (Z[k]-conj(Z[M-k])), w = exp(-2 pi i/2M)
+  cx<T> a[F::G][P];
+  bool ok[F::G];
+#pragma unroll
+  for (int j = 0; j < F::G; ++j) ok[j] = F::stage2(j, t, sc, a[j]);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < F::G; ++j)
+    if (ok[j]) {
+#pragma unroll
+      for (int k1 = 0; k1 < P; ++k1) sc[t + P * j + Q * k1] = a[j][k1];
+    }
+  __syncthreads();
+  // split: X[k] = (Z[k] + conj Z[M-k])/2 - i/2 w^k (Z[k] - conj Z[M-k]), w = exp(-2 pi i/2M)
   if (valid) {
     cx<T> *dst = out + plane * out_plane + y * out_pitch;
-    for (int k = t; k <= M && k < ncols_out; k += TPR) {
-      cx<T> a = Z[k % M], b = conjg(Z[(M - k) % M]);
-      cx<T> e = a + b, o = a - b;
-      cx<T> w = twN[k % (2 * M)];
-      cx<T> wo = cmul(w, o);
-      // -i * wo
-      cx<T> X = mk<T>(T(0.5) * (e.x + wo.y), T(0.5) * (e.y - wo.x));
-      dst[k] = X;
+    for (int k = t; k <= M && k < ncols_out; k += P) {
+      cx<T> za = sc[k % M], zb = conjg(sc[(M - k) % M]);
+      cx<T> e = za + zb, o = za - zb;
+      cx<T> wo = cmul(twN[k % (2 * M)], o);
+      dst[k] = mk<T>(T(0.5) * (e.x + wo.y), T(0.5) * (e.y - wo.x));
     }
   }
 }
 
 // ---------------------------------------------------------------------------------
-// row_c2r
+// row_c2r: M+1 complex (first ncols_in present; imaginary parts of kx = 0 and kx = M
+// ignored, as FFTW2's halfcomplex c2r) -> 2M reals, columns [2M, out_pitch) zeroed
 // ---------------------------------------------------------------------------------
 template <typename T, typename TO, int M>
-__global__ __launch_bounds__(kBlock) void k_row_c2r(const cx<T> *__restrict__ in, long in_pitch, long in_plane,
-                                                    int ncols_in, int rows, int nplanes,
-                                                    TO *__restrict__ out, long out_pitch, long out_plane,
-                                                    const cx<T> *__restrict__ twM,
-                                                    const cx<T> *__restrict__ twN) {
-  constexpr int TPR = row_tpr(M);
-  constexpr int RPB = kBlock / TPR;
-  __shared__ cx<T> sX[RPB][M + 1];
-  __shared__ cx<T> sA[RPB][M];
-  __shared__ cx<T> sB[RPB][M];
-  const int r = threadIdx.x / TPR, t = threadIdx.x % TPR;
-  const long grow = (long)blockIdx.x * RPB + r;
-  const long total = (long)rows * nplanes;
-  const bool valid = grow < total;
+__global__ __launch_bounds__(kRowBlock) void k_row_c2r(const cx<T> *__restrict__ in, long in_pitch, long in_plane,
+                                                       int ncols_in, int rows, int nplanes,
+                                                       TO *__restrict__ out, long out_pitch, long out_plane,
+                                                       const cx<T> *__restrict__ twM,
+                                                       const cx<T> *__restrict__ twN) {
+  constexpr int P = row_lanes(M);
+  using F = Fft4<T, M, P, +1>;
+  constexpr int Q = F::Q;
+  constexpr int RB = kRowBlock / P;
+  constexpr int SS = slot_stride(F::SCR);
+  __shared__ cx<T> lds[RB * SS];
+  const int t = threadIdx.x % P, r = threadIdx.x / P;
+  const long grow = (long)blockIdx.x * RB + r;
+  const bool valid = grow < (long)rows * nplanes;
   const long plane = valid ? grow / rows : 0, y = valid ? grow % rows : 0;
-  if (valid) {
-    const cx<T> *src = in + plane * in_plane + y * in_pitch;
-    for (int k = t; k <= M; k += TPR) {
-      cx<T> v = k < ncols_in ? src[k] : mk<T>(0, 0);
-      if (k == 0 || k == M) v.y = 0;
-      sX[r][k] = v;
+  cx<T> *sc = lds + r * SS;
+  const cx<T> *src = in + plane * in_plane + y * in_pitch;
+  auto ld = [&](int k) -> cx<T> {
+    cx<T> x = (valid && k < ncols_in) ? src[k] : mk<T>(0, 0);
+    if (k == 0 || k == M) x.y = 0;
+    return x;
+  };
+  // merge: Y[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k])
+  cx<T> v[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int k = t + P * q;
+    cx<T> xa = ld(k), xb = conjg(ld(M - k));
+    cx<T> e = xa + xb, o = xa - xb;
+    cx<T> wo = cmul(conjg(twN[k]), o);
+    v[q] = mk<T>(e.x - wo.y, e.y + wo.x);
+  }
+  F::stage1(v, t, sc, twM);
+  __syncthreads();
+  TO *dst = out + plane * out_plane + y * out_pitch;
+#pragma unroll
+  for (int j = 0; j < F::G; ++j) {
+    cx<T> a[P];
+    if (F::stage2(j, t, sc, a) && valid) {
+#pragma unroll
+      for (int k1 = 0; k1 < P; ++k1) {
+        const int n = t + P * j + Q * k1;
+        dst[2 * n] = (TO)a[k1].x;
+        dst[2 * n + 1] = (TO)a[k1].y;
+      }
     }
   }
-  __syncthreads();
-  // merge: Y[k] = (X[k]+conj(X[M-k])) + i w^-k (X[k]-conj(X[M-k]))
-  for (int k = t; k < M; k += TPR) {
-    cx<T> a = sX[r][k], b = conjg(sX[r][M - k]);
-    cx<T> e = a + b, o = a - b;
-    cx<T> w = conjg(twN[k]);
-    cx<T> wo = cmul(w, o);
-    sA[r][k] = mk<T>(e.x - wo.y, e.y + wo.x);
-  }
-  __syncthreads();
-  cx<T> *Yv = fft_lds<T, M, +1>(&sA[r][0], &sB[r][0], t, TPR, twM);
-  if (valid) {
-    TO *dst = out + plane * out_plane + y * out_pitch;
-    for (int n = t; n < M; n += TPR) {
-      cx<T> v = Yv[n];
-      dst[2 * n] = (TO)v.x;
-      dst[2 * n + 1] = (TO)v.y;
-    }
-    for (long n = 2 * M + t; n < out_pitch; n += TPR) dst[n] = (TO)0;
-  }
+  if (valid)
+    for (long n = 2 * M + t; n < out_pitch; n += P) dst[n] = (TO)0;
 }
 
 // ---------------------------------------------------------------------------------
-// col_op
+// col_op: per kx column, forward DFT along y, spectral map, inverse DFT along y.
+// A block owns TX adjacent columns (one 128-byte segment per y) of one plane; 16 lanes
+// per column.  The spectrum between the transforms stays in registers in the
+// "D16" layout: lane t holds entries t + 16*m, which is also what the padding,
+// truncation and multiplier maps need, so they are lane-local.
 // ---------------------------------------------------------------------------------
+struct ColSlot {
+  void *ptr;
+  long pitch;  // complex elements between y rows of this output
+  int mult;
+  int tpitch;  // row length of the multiplier tables (lh of the small grid)
+};
 struct ColParams {
-  ColOut outs[3];
+  ColSlot outs[3];
   int nout;
 };
-
-__host__ __device__ constexpr int col_tx(int nmax, int bytes_cx) {
-  // columns per block: 3 buffers of nmax complex per column within 96 KiB
-  return (3 * nmax * bytes_cx * 16 <= 98304) ? 16
-         : (3 * nmax * bytes_cx * 8 <= 98304) ? 8
-         : (3 * nmax * bytes_cx * 4 <= 98304) ? 4
-         : (3 * nmax * bytes_cx * 2 <= 98304) ? 2 : 1;
-}
+static constexpr int kColLanes = 16;
+template <typename T> __host__ __device__ constexpr int col_tx() { return 128 / (int)sizeof(cx<T>); }
 
 template <typename T, int NYI, int NYO>
-__global__ __launch_bounds__(kBlock) void k_col_op(const cx<T> *__restrict__ in, long in_pitch, long in_plane,
-                                                   int ncols, int nplanes, int padmode, int fwd, int inv,
-                                                   ColParams P, long out_pitch, long out_plane,
-                                                   const double *__restrict__ kxt, const double *__restrict__ kyt,
-                                                   const double *__restrict__ G1, const double *__restrict__ G2,
-                                                   int lh_s, int ny_s,
-                                                   const cx<T> *__restrict__ twI, const cx<T> *__restrict__ twO) {
-  constexpr int NMAX = NYI > NYO ? NYI : NYO;
-  constexpr int TX = col_tx(NMAX, (int)sizeof(cx<T>));
-  constexpr int TPC = kBlock / TX;
-  __shared__ cx<T> sbuf[3][TX][NMAX];
-  const int c = threadIdx.x / TPC, t = threadIdx.x % TPC;
+__global__ __launch_bounds__(col_tx<T>() * kColLanes) void k_col_op(
+    const cx<T> *__restrict__ in, long in_pitch, long in_plane, int ncols, int fwd, int inv, ColParams Pr,
+    long out_pitch, long out_plane, const double *__restrict__ kxt, const double *__restrict__ kyt,
+    const double *__restrict__ G1, const double *__restrict__ G2, int lh_s, int ny_s,
+    const cx<T> *__restrict__ twI, const cx<T> *__restrict__ twO) {
+  constexpr int TX = col_tx<T>();
+  constexpr int PL = kColLanes;
+  using FI = Fft4<T, NYI, PL, -1>;
+  using FO = Fft4<T, NYO, PL, +1>;
+  constexpr int QI = FI::Q, QO = FO::Q;
+  constexpr int PM = NYI == NYO ? 0 : (NYO > NYI ? 1 : 2);  // same grid / pad / truncate
+  constexpr int DQ = (NYI > NYO ? NYI - NYO : NYO - NYI) / PL;
+  constexpr int NS = FI::SCR > FO::SCR ? (FI::SCR > NYI ? FI::SCR : NYI) : (FO::SCR > NYI ? FO::SCR : NYI);
+  constexpr int SS = slot_stride(NS);
+  __shared__ cx<T> lds[TX * SS];
+  const int c = threadIdx.x % TX, t = threadIdx.x / TX;
+  const int kx = blockIdx.x * TX + c;
+  const bool cv = kx < ncols;
   const int plane = blockIdx.y;
-  const int kx0 = blockIdx.x * TX;
-  const cx<T> *src = in + (long)plane * in_plane;
-  // coalesced load: element i -> (y = i / TX, col = i % TX)
-  for (int i = threadIdx.x; i < NYI * TX; i += kBlock) {
-    int yy = i / TX, cc = i % TX;
-    int kx = kx0 + cc;
-    cx<T> v = mk<T>(0, 0);
-    if (kx < ncols) v = src[(long)yy * in_pitch + kx];
-    sbuf[0][cc][yy] = v;
+  cx<T> *sc = lds + c * SS;
+  // invalid columns load column ncols-1 and discard it (no divergent loads)
+  const cx<T> *src = in + (long)plane * in_plane + (cv ? kx : ncols - 1) + (long)t * in_pitch;
+  const long istep = (long)PL * in_pitch;
+  cx<T> sp[QI];  // D16: sp[m] = spectrum[t + 16 m]
+#pragma unroll
+  for (int m = 0; m < QI; ++m) {
+    const cx<T> x = src[m * istep];
+    sp[m] = cv ? x : mk<T>(0, 0);
   }
-  __syncthreads();
-  int ib = 0;  // buffer holding the input spectrum
   if (fwd) {
-    cx<T> *res = fft_lds<T, NYI, -1>(&sbuf[0][c][0], &sbuf[1][c][0], t, TPC, twI);
-    ib = (res == &sbuf[0][c][0]) ? 0 : 1;
-  }
-  // ib is identical for every column (same stage count), so it is block-uniform
-  const int b1 = (ib == 0) ? 1 : 0, b2 = 2;
-  const int kx = kx0 + c;
-  for (int j = 0; j < P.nout; ++j) {
+    FI::stage1(sp, t, sc, twI);
     __syncthreads();
-    const int mult = P.outs[j].mult;
-    for (int yo = t; yo < NYO; yo += TPC) {
-      cx<T> v = mk<T>(0, 0);
-      int yi;
-      if (padmode == 0) {
-        yi = yo;
-        if (kx < ncols && yo != ny_s / 2 && kx != lh_s - 1) {
-          v = sbuf[ib][c][yi];
-          long ti = (long)yo * lh_s + kx;
-          if (mult == M_IKX) v = mul_i(v, (T)kxt[ti]);
-          else if (mult == M_IKY) v = mul_i(v, (T)kyt[ti]);
-          else if (mult == M_G1) v = scal(v, (T)G1[ti]);
-          else if (mult == M_G2) v = scal(v, (T)G2[ti]);
+    if constexpr (QI == PL) {  // stage 2 already yields the D16 layout
+      cx<T> a[PL];
+      FI::stage2(0, t, sc, a);
+#pragma unroll
+      for (int k1 = 0; k1 < PL; ++k1) sp[k1] = a[k1];
+      __syncthreads();
+    } else {  // regroup through LDS in natural order
+      cx<T> a[FI::G][PL];
+      bool ok[FI::G];
+#pragma unroll
+      for (int j = 0; j < FI::G; ++j) ok[j] = FI::stage2(j, t, sc, a[j]);
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < FI::G; ++j)
+        if (ok[j]) {
+#pragma unroll
+          for (int k1 = 0; k1 < PL; ++k1) sc[t + PL * j + QI * k1] = a[j][k1];
         }
-      } else if (padmode == 1) {  // small (NYI = ny) -> big (NYO = ny2)
-        if (yo < NYI / 2) yi = yo;
-        else if (yo >= NYO - NYI / 2 + 1) yi = yo - NYO + NYI;
-        else yi = -1;
-        if (yi >= 0 && kx < ncols) v = sbuf[ib][c][yi];
-      } else {  // big (NYI = ny2) -> small (NYO = ny)
-        if (yo < NYO / 2) yi = yo;
-        else if (yo > NYO / 2) yi = yo + NYI - NYO;
-        else yi = -1;
-        if (yi >= 0 && kx < ncols) v = sbuf[ib][c][yi];
-      }
-      sbuf[b1][c][yo] = v;
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < QI; ++m) sp[m] = sc[t + PL * m];
+      __syncthreads();
     }
-    __syncthreads();
-    cx<T> *res = &sbuf[b1][c][0];
-    if (inv) res = fft_lds<T, NYO, +1>(&sbuf[b1][c][0], &sbuf[b2][c][0], t, TPC, twO);
-    __syncthreads();
-    // coalesced store
-    cx<T> *dst = reinterpret_cast<cx<T> *>(P.outs[j].ptr) + (long)plane * out_plane;
-    const int rb = (res == &sbuf[b1][c][0]) ? b1 : b2;
-    for (int i = threadIdx.x; i < NYO * TX; i += kBlock) {
-      int yy = i / TX, cc = i % TX;
-      int kxx = kx0 + cc;
-      if (kxx < ncols) dst[(long)yy * out_pitch + kxx] = sbuf[rb][cc][yy];
+  }
+  for (int jo = 0; jo < Pr.nout; ++jo) {
+    const int mult = Pr.outs[jo].mult;
+    // one multiplier table per output: i*kx, i*ky (imaginary) or a filter (real)
+    const double *mt = mult == M_IKX ? kxt : mult == M_IKY ? kyt : mult == M_G1 ? G1 : G2;
+    const bool imag = mult == M_IKX || mult == M_IKY;
+    const bool keep = cv && kx != lh_s - 1;
+    const int tp = Pr.outs[jo].tpitch;
+    const double *mrow = mt + (long)t * tp + (cv ? kx : 0);
+    cx<T> v[QO];
+#pragma unroll
+    for (int q = 0; q < QO; ++q) {
+      const int yo = t + PL * q;
+      cx<T> x = mk<T>(0, 0);
+      if constexpr (PM == 0) {
+        x = sp[q];
+        if (mult != M_ONE) {  // block-uniform branch; the loads below are unconditional
+          const T f = (T)mrow[(long)q * PL * tp];
+          x = mk<T>((imag ? -x.y : x.x) * f, (imag ? x.x : x.y) * f);
+        }
+        if (!keep || yo == ny_s / 2) x = mk<T>(0, 0);
+      } else if constexpr (PM == 1) {  // padd.f90: ny -> ny2, middle rows zero
+        if (q < QI && yo < NYI / 2) x = sp[q < QI ? q : 0];
+        else if (q >= DQ && yo >= NYO - NYI / 2 + 1) x = sp[q >= DQ ? q - DQ : 0];
+      } else {  // unpadd.f90: ny2 -> ny, Nyquist row zero
+        if (yo < NYO / 2) x = sp[q];
+        else if (yo > NYO / 2) x = sp[q + DQ];
+      }
+      v[q] = x;
+    }
+    // addresses are formed from per-output values so that they are not hoisted out of
+    // the output loop (which would keep 16 64-bit addresses live)
+    const long pitch = Pr.outs[jo].pitch;
+    cx<T> *dst = reinterpret_cast<cx<T> *>(Pr.outs[jo].ptr) + (long)plane * out_plane + kx;
+    if (inv) {
+      FO::stage1(v, t, sc, twO);
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < FO::G; ++j) {
+        cx<T> a[PL];
+        if (FO::stage2(j, t, sc, a) && cv) {
+          cx<T> *d = dst + (long)(t + PL * j) * pitch;
+          const long ostep = (long)QO * pitch;
+#pragma unroll
+          for (int k1 = 0; k1 < PL; ++k1) d[k1 * ostep] = a[k1];
+        }
+      }
+      __syncthreads();
+    } else if (cv) {
+      cx<T> *d = dst + (long)t * pitch;
+      const long ostep = (long)PL * pitch;
+#pragma unroll
+      for (int q = 0; q < QO; ++q) d[q * ostep] = v[q];
     }
   }
 }
@@ -226,10 +290,10 @@ static hipError_t launch_r2c(const void *in, long in_pitch, long in_plane, int r
                              hipStream_t s, Dt dt) {
   const void *twM = tw_ptr(M, dt), *twN = tw_ptr(2 * M, dt);
   if (!twM || !twN) return hipErrorInvalidValue;
-  constexpr int RPB = kBlock / row_tpr(M);
+  constexpr int RPB = kRowBlock / row_lanes(M);
   long total = (long)rows * nplanes;
   dim3 grid((unsigned)((total + RPB - 1) / RPB));
-  hipLaunchKernelGGL((k_row_r2c<T, M>), grid, dim3(kBlock), 0, s, (const T *)in, in_pitch, in_plane, rows,
+  hipLaunchKernelGGL((k_row_r2c<T, M>), grid, dim3(kRowBlock), 0, s, (const T *)in, in_pitch, in_plane, rows,
                      nplanes, (T)scale, (cx<T> *)out, out_pitch, out_plane, ncols_out,
                      (const cx<T> *)twM, (const cx<T> *)twN);
   return hipGetLastError();
@@ -255,10 +319,10 @@ static hipError_t launch_c2r(const void *in, long in_pitch, long in_plane, int n
                              int nplanes, void *out, long out_pitch, long out_plane, hipStream_t s, Dt dt) {
   const void *twM = tw_ptr(M, dt), *twN = tw_ptr(2 * M, dt);
   if (!twM || !twN) return hipErrorInvalidValue;
-  constexpr int RPB = kBlock / row_tpr(M);
+  constexpr int RPB = kRowBlock / row_lanes(M);
   long total = (long)rows * nplanes;
   dim3 grid((unsigned)((total + RPB - 1) / RPB));
-  hipLaunchKernelGGL((k_row_c2r<T, TO, M>), grid, dim3(kBlock), 0, s, (const cx<T> *)in, in_pitch, in_plane,
+  hipLaunchKernelGGL((k_row_c2r<T, TO, M>), grid, dim3(kRowBlock), 0, s, (const cx<T> *)in, in_pitch, in_plane,
                      ncols_in, rows, nplanes, (TO *)out, out_pitch, out_plane, (const cx<T> *)twM,
                      (const cx<T> *)twN);
   return hipGetLastError();
@@ -291,15 +355,17 @@ static hipError_t launch_col(int padmode, int fwd, int inv, const void *in, long
                              long out_plane, const SpecTables &tab, hipStream_t s, Dt dt) {
   const void *twI = tw_ptr(NYI, dt), *twO = tw_ptr(NYO, dt);
   if (!twI || !twO) return hipErrorInvalidValue;
-  constexpr int NMAX = NYI > NYO ? NYI : NYO;
-  constexpr int TX = col_tx(NMAX, (int)sizeof(cx<T>));
+  // the map is fixed by the lengths: same grid (0), padd (1), unpadd (2)
+  if (padmode != (NYI == NYO ? 0 : (NYO > NYI ? 1 : 2))) return hipErrorInvalidValue;
+  constexpr int TX = col_tx<T>();
   ColParams P;
   P.nout = nout;
-  for (int j = 0; j < 3; ++j) P.outs[j] = j < nout ? outs[j] : ColOut{nullptr, 0};
+  for (int j = 0; j < 3; ++j) P.outs[j] = j < nout ? ColSlot{outs[j].ptr, out_pitch, outs[j].mult, tab.lh}
+                                     : ColSlot{nullptr, 0, 0, 0};
   dim3 grid((unsigned)((ncols + TX - 1) / TX), (unsigned)nplanes);
-  hipLaunchKernelGGL((k_col_op<T, NYI, NYO>), grid, dim3(kBlock), 0, s, (const cx<T> *)in, in_pitch, in_plane,
-                     ncols, nplanes, padmode, fwd, inv, P, out_pitch, out_plane, tab.kx, tab.ky, tab.G1,
-                     tab.G2, tab.lh, tab.ny, (const cx<T> *)twI, (const cx<T> *)twO);
+  hipLaunchKernelGGL((k_col_op<T, NYI, NYO>), grid, dim3(TX * kColLanes), 0, s, (const cx<T> *)in, in_pitch,
+                     in_plane, ncols, fwd, inv, P, out_pitch, out_plane, tab.kx, tab.ky, tab.G1, tab.G2, tab.lh,
+                     tab.ny, (const cx<T> *)twI, (const cx<T> *)twO);
   return hipGetLastError();
 }
 
