@@ -119,6 +119,7 @@ struct lp_ctx {
   int nred = 0;
   int jt = 1, jt_total = 1;
   int F_LM_MM_init = 0, F_QN_NN_init = 0;
+  int ncut1 = 0, ncut2 = 0;  // leading kx columns where G_test / G_test_test are nonzero
   // droplet bins (npcon > 0)
   double *d_ws = nullptr, *d_qsrc = nullptr, *d_vfac = nullptr;  // per bin
   double *d_foot = nullptr;                    // normalised source footprint (nx*ny)
@@ -512,6 +513,15 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
     return set_err(LP_ERR_HIP, "table upload failed");
   }
   c->tab.kx = c->kx; c->tab.ky = c->ky; c->tab.G1 = c->G1; c->tab.G2 = c->G2;
+  auto cut = [&](const std::vector<double> &G) {
+    int n = 1;
+    for (int jy = 0; jy < g.ny; ++jy)
+      for (int jx = 0; jx < g.lh; ++jx)
+        if (G[(size_t)jy * g.lh + jx] != 0.0 && jx + 1 > n) n = jx + 1;
+    return n;
+  };
+  c->ncut1 = cut(G1);
+  c->ncut2 = cut(G2);
   c->tab.lh = g.lh; c->tab.ny = g.ny;
   size_t fe = esz_of(c->fdt);
   size_t spec_b = (size_t)g.lh * g.ny * g.nz * 2 * fe;  // == volume bytes
@@ -623,12 +633,15 @@ static double bytes_col(int nyi, int nyo, int ncols, int nlev, int nout, size_t 
   return (double)ncols * nlev * 2 * e * ((double)nyi + (double)nout * nyo);
 }
 
-static int r2c_small(lp_ctx *c, const void *in, int nlev, double scale, void *spec) {
+// ncols_out < lh writes only the leading kx columns (a spectrum that the caller knows to
+// be zero beyond them)
+static int r2c_small(lp_ctx *c, const void *in, int nlev, double scale, void *spec, int ncols_out = 0) {
   const lp_grid &g = c->g;
+  if (ncols_out <= 0) ncols_out = g.lh;
   prof_begin(c, LP_K_ROW_R2C);
   HIPCHK(row_r2c(c->fdt, g.nx / 2, in, g.ld, plane_el(c), g.ny, nlev, scale, spec, g.lh,
-                 (long)g.lh * g.ny, g.lh, c->s));
-  prof_end(c, LP_K_ROW_R2C, bytes_r2c(g.nx, g.ny, nlev, g.lh, esz_of(c->fdt)));
+                 (long)g.lh * g.ny, ncols_out, c->s));
+  prof_end(c, LP_K_ROW_R2C, bytes_r2c(g.nx, g.ny, nlev, ncols_out, esz_of(c->fdt)));
   return LP_OK;
 }
 static int c2r_small(lp_ctx *c, const void *spec, int ncols, int nlev, void *out, Dt dto) {
@@ -640,12 +653,13 @@ static int c2r_small(lp_ctx *c, const void *spec, int ncols, int nlev, void *out
   return LP_OK;
 }
 // column map on small spectra: in -> nout outputs (same grid)
-static int col_small(lp_ctx *c, const void *in, int nlev, int fwd, int inv, int nout, ColOut *o) {
+static int col_small(lp_ctx *c, const void *in, int nlev, int fwd, int inv, int nout, ColOut *o, int ncols = 0) {
   const lp_grid &g = c->g;
   long ps = (long)g.lh * g.ny;
+  if (ncols <= 0) ncols = g.lh;
   prof_begin(c, LP_K_COL_OP);
-  HIPCHK(col_op(c->fdt, g.ny, g.ny, 0, fwd, inv, in, g.lh, ps, g.lh, nlev, nout, o, g.lh, ps, c->tab, c->s));
-  prof_end(c, LP_K_COL_OP, bytes_col(g.ny, g.ny, g.lh, nlev, nout, esz_of(c->fdt)));
+  HIPCHK(col_op(c->fdt, g.ny, g.ny, 0, fwd, inv, in, g.lh, ps, ncols, nlev, nout, o, g.lh, ps, c->tab, c->s));
+  prof_end(c, LP_K_COL_OP, bytes_col(g.ny, g.ny, ncols, nlev, nout, esz_of(c->fdt)));
   return LP_OK;
 }
 
@@ -779,12 +793,15 @@ static int do_lagrange(lp_ctx *c) {
                                      (T *const *)c->d_lagin, g));
   HIPCHK(hipGetLastError());
   // lagrange_Sdep.f90:157-256: every field filtered at 2*delta and 4*delta
+  // the filters are sharp cutoffs: kx columns >= ncut1 (G_test) / ncut2 (G_test_test)
+  // are zero after filtering, so only the leading columns are transformed
+  const int n1 = c->ncut1, n2 = c->ncut2, nc = n1 > n2 ? n1 : n2;
   for (int q = 0; q < kNumLag; ++q) {
-    RCHK(r2c_small(c, c->lagin[q], gg.nz, 1.0, c->spec[0]));
+    RCHK(r2c_small(c, c->lagin[q], gg.nz, 1.0, c->spec[0], nc));
     ColOut o[2] = {{c->spec[1], M_G1}, {c->spec[2], M_G2}};
-    RCHK(col_small(c, c->spec[0], gg.nz, 1, 1, 2, o));
-    RCHK(c2r_small(c, c->spec[1], gg.lh, gg.nz, c->lagbar[q], c->fdt));
-    RCHK(c2r_small(c, c->spec[2], gg.lh, gg.nz, c->laghat[q], c->fdt));
+    RCHK(col_small(c, c->spec[0], gg.nz, 1, 1, 2, o, nc));
+    RCHK(c2r_small(c, c->spec[1], n1, gg.nz, c->lagbar[q], c->fdt));
+    RCHK(c2r_small(c, c->spec[2], n2, gg.nz, c->laghat[q], c->fdt));
   }
   double delta = cf.filter_size * std::pow(gg.dx * gg.dy * gg.dz, 1.0 / 3.0);
   LagParams P;

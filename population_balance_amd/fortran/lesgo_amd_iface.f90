@@ -16,6 +16,7 @@ module lesgo_amd
   public :: lp_pollen_all_in_one2, lp_calculate_probability
   public :: LP_OK, LP_FP64, LP_MIXED, LP_LBC_WALL, LP_LBC_STRESS_FREE
   public :: LP_U, LP_V, LP_W, LP_P, LP_RHSX, LP_RHSY, LP_RHSZ
+  public :: LP_PCON, LP_RHS_PCON, LP_RHS_PCON_F, LP_RHS_PBE, LP_BETA_PCON
 
   integer(c_int), parameter :: LP_OK = 0
   integer(c_int32_t), parameter :: LP_FP64 = 0, LP_MIXED = 1
@@ -23,6 +24,8 @@ module lesgo_amd
   ! field ids (enum lp_field)
   integer(c_int32_t), parameter :: LP_U = 0, LP_V = 1, LP_W = 2
   integer(c_int32_t), parameter :: LP_RHSX = 12, LP_RHSY = 13, LP_RHSZ = 14, LP_P = 27
+  integer(c_int32_t), parameter :: LP_PCON = 47, LP_RHS_PCON = 48, LP_RHS_PCON_F = 49
+  integer(c_int32_t), parameter :: LP_RHS_PBE = 50, LP_BETA_PCON = 51
 
   type, bind(C) :: lp_grid
     integer(c_int32_t) :: nx, ny, nz, ld, lh, nx2, ny2, lh_big, ld_big, pad_

@@ -154,3 +154,24 @@ def test_steps_fp64(nx, nz, nsteps):
         errs.update(_cmp(o, lg, ["RHSx", "RHSy", "RHSz"], levels=slice(0, nz - 1)))
         print(f"  step {s + 1}: " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
     assert lg.time() == o.time()
+
+
+@pytest.mark.parametrize("prec", [abi.LP_FP64, abi.LP_MIXED])
+def test_timestep_conditions_and_rmsdiv(prec):
+    """scalars_module2.f90:1336-1379 and rmsdiv.f90:1-44 as device reductions"""
+    cfg = abi.make_config(64, 64, 17, precision=prec)
+    cfg64 = abi.make_config(64, 64, 17)
+    from oracle import Oracle
+    from population_balance_amd.lesgo import Lesgo
+    o, lg = Oracle(cfg64), Lesgo(cfg)
+    st = seeded_state(cfg)
+    st.update(random_fields(cfg, ["Nu_t", "dudx", "dvdy", "dwdz"], seed=4))
+    if prec == abi.LP_MIXED:  # identical inputs on both sides
+        st = {k: v.astype(np.float32).astype(np.float64) for k, v in st.items()}
+    load_oracle(o, st); load_gpu(lg, st)
+    cfl_o, visc_o = o.timestep_conditions()
+    cfl_g, visc_g = lg.timestep_conditions()
+    assert abs(cfl_g - cfl_o) <= 1e-14 * cfl_o
+    assert abs(visc_g - visc_o) <= 1e-14 * abs(visc_o)
+    r_o, r_g = o.rmsdiv(), lg.rmsdiv()
+    assert abs(r_g - r_o) <= 1e-12 * r_o
