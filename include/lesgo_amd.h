@@ -220,6 +220,12 @@ int lp_rmsdiv(lp_ctx *ctx, double *rmsdiv);     /* rmsdiv.f90:1-44              
 /* scalars_module_crossflow_mod.f90:1494-1786 (pollen_all_in_one2 with
  * pollen_RHS_calc2, calcbeta_pcon, step_pollen2) on the context state; needs npcon > 0. */
 int lp_pollen_all_in_one2(lp_ctx *ctx);
+/* The three parts of that step on their own (scalars_module_crossflow_mod.f90):
+ * pollen_RHS_calc2 (:1894-4472; writes RHS_PCon, Rhs_pbe, magS(nz)), calcbeta_pcon
+ * (:1795-1843; writes beta_pcon) and step_pollen2 (:4481-4508; advances PCon). */
+int lp_pollen_rhs_calc2(lp_ctx *ctx);
+int lp_calcbeta_pcon(lp_ctx *ctx);
+int lp_step_pollen2(lp_ctx *ctx);
 /* scalars_module_crossflow_mod.f90:4530-4597: beta(npcon,npcon) binary64, beta[i*n+j]
  * = probability that a drop of bin j breaks into bin i (host routine, no device). */
 int lp_calculate_probability(const double *diameter, int32_t npcon, double sigma_s,

@@ -63,7 +63,8 @@ def lib():
         L.or_divstress_w.argtypes = [vp, vp, vp, vp, vp]
         L.or_tridag_array.argtypes = [vp, vp, vp, vp, vp, vp]
         for n in ("or_wallstress", "or_sgs_stag", "or_lagrange_sdep", "or_interpolag_sdep",
-                  "or_convec", "or_press_stag_array", "or_project", "or_pollen_all_in_one2"):
+                  "or_convec", "or_press_stag_array", "or_project", "or_pollen_all_in_one2",
+                  "or_pollen_rhs_calc2", "or_calcbeta_pcon", "or_step_pollen2"):
             getattr(L, n).argtypes = [vp]
             getattr(L, n).restype = C.c_int
         L.or_tridag_array.restype = C.c_int

@@ -1306,24 +1306,22 @@ static inline double smart_face(double cup, double cupup, double cdn) {
   return cup + r * den / 2.0;
 }
 
-/* scalars_module_crossflow_mod.f90:1494-1786 pollen_all_in_one2 with pollen_RHS_calc2
- * (:1894-4472), calcbeta_pcon (:1795-1843) and step_pollen2 (:4481-4508).
- * PCon_scheme = 3 (SMART), model_sc = 1, lbcp = 1, no Stokes drift, no SGS
- * acceleration flux, single Gaussian point source. */
-int or_pollen_all_in_one2(or_ctx *c) {
+/* scalars_module_crossflow_mod.f90:1894-4472 pollen_RHS_calc2 (with the optional
+ * breakup feedback into RHS_PCon).  PCon_scheme = 3 (SMART), model_sc = 1, lbcp = 1, no
+ * Stokes drift, no SGS acceleration flux, single Gaussian point source. */
+int or_pollen_rhs_calc2(or_ctx *c) {
   const lp_grid *g = &c->g;
   const lp_config *cf = &c->cfg;
   const lp_droplet_config *D = &cf->drop;
   const int nx = g->nx, ny = g->ny, nz = g->nz, ld = g->ld, npc = D->npcon;
   const size_t np = plane(c), nv = vol(c);
-  const double dx = g->dx, dy = g->dy, dz = g->dz, dt = cf->dt;
-  double *PC = c->f[LP_PCON], *RH = c->f[LP_RHS_PCON], *RF = c->f[LP_RHS_PCON_F], *PB = c->f[LP_RHS_PBE];
+  const double dx = g->dx, dy = g->dy, dz = g->dz;
+  double *PC = c->f[LP_PCON], *RH = c->f[LP_RHS_PCON], *PB = c->f[LP_RHS_PBE];
   double *u = c->f[LP_U], *v = c->f[LP_V], *w = c->f[LP_W];
   double *dudx = c->f[LP_DUDX], *dvdy = c->f[LP_DVDY], *nut = c->f[LP_NU_T], *magS = c->f[LP_MAGS];
   double *dudt = c->f[LP_DUDT], *dvdt = c->f[LP_DVDT], *dwdt = c->f[LP_DWDT];
   double *dissip = c->f[LP_DISSIP];
-  /* :1521 */
-  memcpy(RF, RH, nv * npc * sizeof(double));
+  if (npc < 1) return fail(LP_ERR_ARG, "pollen_RHS_calc2 needs npcon > 0");
   const int src_on = c->jt_total >= D->src_jt0 && c->jt_total < D->src_jt1;
   /* ---- pollen_RHS_calc2 ---- */
   /* face velocity arrays (ld, ny+1, nz) */
@@ -1647,13 +1645,24 @@ int or_pollen_all_in_one2(or_ctx *c) {
     }
 #undef CC
   }
-  /* :1657 (commented out in the reference): optional feedback of the breakup source */
+  /* :4470 (commented out in the reference): optional feedback of the breakup source */
   if (D->pbe && D->pbe_feedback)
     for (size_t i = 0; i < nv * npc; ++i) RH[i] += PB[i];
-  /* :1660-1665 first droplet step: Euler */
-  if ((c->jt_total == D->pcon_init || (c->jt_total == D->src_jt0)) && !D->init_pcon)
-    memcpy(RF, RH, nv * npc * sizeof(double));
-  /* :1673-1675 -> :1795-1843 calcbeta_pcon */
+#undef FI
+  free(ui0); free(vi0); free(ui); free(vi); free(wi); free(Kc);
+  free(cx_); free(cy_); free(cz_); free(gx0); free(gxL); free(gy0); free(gyL);
+  free(wsrc); free(beta); free(gfreq);
+  return 0;
+}
+
+/* scalars_module_crossflow_mod.f90:1795-1843 calcbeta_pcon */
+int or_calcbeta_pcon(or_ctx *c) {
+  const lp_grid *g = &c->g;
+  const lp_droplet_config *D = &c->cfg.drop;
+  const int nx = g->nx, ny = g->ny, nz = g->nz, ld = g->ld, npc = D->npcon;
+  const size_t np = plane(c), nv = vol(c);
+  double *PC = c->f[LP_PCON];
+  if (npc < 1) return fail(LP_ERR_ARG, "calcbeta_pcon needs npcon > 0");
   double *bp = c->f[LP_BETA_PCON];
   memset(bp, 0, nv * sizeof(double));
   if (D->active) {
@@ -1671,7 +1680,17 @@ int or_pollen_all_in_one2(or_ctx *c) {
           bp[s] = t;
         }
   }
-  /* :4481-4508 step_pollen2 */
+  return 0;
+}
+
+/* scalars_module_crossflow_mod.f90:4481-4508 step_pollen2 */
+int or_step_pollen2(or_ctx *c) {
+  const lp_grid *g = &c->g;
+  const int nx = g->nx, ny = g->ny, nz = g->nz, ld = g->ld, npc = c->cfg.drop.npcon;
+  const size_t np = plane(c), nv = vol(c);
+  const double dt = c->cfg.dt;
+  double *PC = c->f[LP_PCON], *RH = c->f[LP_RHS_PCON], *RF = c->f[LP_RHS_PCON_F];
+  if (npc < 1) return fail(LP_ERR_ARG, "step_pollen2 needs npcon > 0");
   for (int ip = 0; ip < npc; ++ip) {
     double *C = PC + (size_t)ip * nv, *R = RH + (size_t)ip * nv, *F = RF + (size_t)ip * nv;
     for (int k = 0; k < nz - 1; ++k)
@@ -1682,11 +1701,24 @@ int or_pollen_all_in_one2(or_ctx *c) {
         }
     memcpy(C + (size_t)(nz - 1) * np, C + (size_t)(nz - 2) * np, np * sizeof(double));
   }
-#undef FI
-  free(ui0); free(vi0); free(ui); free(vi); free(wi); free(Kc);
-  free(cx_); free(cy_); free(cz_); free(gx0); free(gxL); free(gy0); free(gyL);
-  free(wsrc); free(beta); free(gfreq);
   return 0;
+}
+
+/* scalars_module_crossflow_mod.f90:1494-1786 pollen_all_in_one2: history, RHS, Euler
+ * start, buoyancy, step (the mass-balance diagnostics at :1714-1781 are output only) */
+int or_pollen_all_in_one2(or_ctx *c) {
+  const lp_droplet_config *D = &c->cfg.drop;
+  const size_t nv = vol(c);
+  const int npc = D->npcon;
+  int rc;
+  if (npc < 1) return fail(LP_ERR_ARG, "pollen_all_in_one2 needs npcon > 0");
+  double *RH = c->f[LP_RHS_PCON], *RF = c->f[LP_RHS_PCON_F];
+  memcpy(RF, RH, nv * npc * sizeof(double));                                  /* :1521 */
+  if ((rc = or_pollen_rhs_calc2(c))) return rc;
+  if ((c->jt_total == D->pcon_init || (c->jt_total == D->src_jt0)) && !D->init_pcon)  /* :1660-1665 */
+    memcpy(RF, RH, nv * npc * sizeof(double));
+  if ((rc = or_calcbeta_pcon(c))) return rc;                                   /* :1673-1675 */
+  return or_step_pollen2(c);                                                   /* :1700 */
 }
 
 /* ------------------------------------------------------------------------- */

@@ -64,6 +64,9 @@ void or_timestep_conditions(or_ctx *c, double *cfl, double *visc_stab);
 void or_rmsdiv(or_ctx *c, double *rms);
 int or_step(or_ctx *c, int nsteps);
 int or_pollen_all_in_one2(or_ctx *c);
+int or_pollen_rhs_calc2(or_ctx *c);
+int or_calcbeta_pcon(or_ctx *c);
+int or_step_pollen2(or_ctx *c);
 void or_ke_dudt(or_ctx *c);
 void or_calculate_probability(const double *d, int n, double sigma, double d_min, double *beta);
 

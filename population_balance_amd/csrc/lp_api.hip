@@ -1008,17 +1008,14 @@ static DropGeo drop_geo(const lp_ctx *c) {
   return d;
 }
 
-// scalars_module_crossflow_mod.f90:1494-1786 pollen_all_in_one2: RHS_PCon_f = RHS_PCon,
-// pollen_RHS_calc2, optional breakup feedback, Euler start, calcbeta_pcon, step_pollen2
-static int do_pollen(lp_ctx *c) {
-  const lp_config &cf = c->cfg;
-  const lp_droplet_config &D = cf.drop;
+// scalars_module_crossflow_mod.f90:1894-4472 pollen_RHS_calc2 (+ the optional breakup
+// feedback).  prev: the previous RHS after the buffer exchange of pollen_all_in_one2, or
+// null when called on its own.
+static int do_pollen_rhs(lp_ctx *c, const double *prev) {
+  const lp_droplet_config &D = c->cfg.drop;
   const lp_grid &gg = c->g;
   DropGeo dg = drop_geo(c);
-  // :1521 -- RHS_PCon_f takes the old RHS by exchanging the buffers (k_pollen_rhs
-  // rewrites every point of the new RHS_PCon)
-  std::swap(c->f[LP_RHS_PCON], c->f[LP_RHS_PCON_F]);
-  double *PC = (double *)c->f[LP_PCON], *RH = (double *)c->f[LP_RHS_PCON], *RF = (double *)c->f[LP_RHS_PCON_F];
+  double *PC = (double *)c->f[LP_PCON], *RH = (double *)c->f[LP_RHS_PCON];
   FDISPATCH(c, T, {
     hipLaunchKernelGGL(k_face_u<T>, dim3((unsigned)(((long)gg.ny * gg.nz + 3) / 4)), dim3(256), 0, c->s,
                        (const T *)c->f[LP_U], (const T *)c->f[LP_DUDX], c->ui0, dg, gg.dx);
@@ -1041,9 +1038,10 @@ static int do_pollen(lp_ctx *c) {
   P.src_k0 = D.src_k - 1;
   const double nb = (double)dg.nv * dg.npc * 8.0;
   prof_begin(c, LP_K_POLLEN_RHS);
-  FDISPATCH(c, T, hipLaunchKernelGGL(k_pollen_rhs<T>, grid1(dg.nv), dim3(256), 0, c->s, PC, RH, RF, c->ui0, c->vi0,
-                                     (const T *)c->f[LP_W], (const T *)c->f[LP_DUDT], (const T *)c->f[LP_DVDT],
-                                     (const T *)c->f[LP_DWDT], c->kcell, c->d_ws, c->d_qsrc, c->d_foot, dg, P));
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_pollen_rhs<T>, grid1(dg.nv), dim3(256), 0, c->s, PC, RH, prev, c->ui0,
+                                     c->vi0, (const T *)c->f[LP_W], (const T *)c->f[LP_DUDT],
+                                     (const T *)c->f[LP_DVDT], (const T *)c->f[LP_DWDT], c->kcell, c->d_ws,
+                                     c->d_qsrc, c->d_foot, dg, P));
   HIPCHK(hipGetLastError());
   // PCon read once, RHS_PCon written once, plus the bin-independent inputs
   prof_end(c, LP_K_POLLEN_RHS, 2.0 * nb + (double)dg.nv * (8.0 + 3.0 * esz_of(c->fdt)) + 4.0 * dg.fp * gg.nz * 8.0);
@@ -1057,12 +1055,32 @@ static int do_pollen(lp_ctx *c) {
     HIPCHK(hipGetLastError());
     prof_end(c, LP_K_PBE, (D.pbe_feedback ? 4.0 : 2.0) * nb + (double)dg.nv * esz_of(c->fdt));
   }
-  if ((c->jt_total == D.pcon_init || c->jt_total == D.src_jt0) && !D.init_pcon)  // :1660-1665
-    HIPCHK(hipMemcpyAsync(RF, RH, c->bytes[LP_RHS_PCON], hipMemcpyDeviceToDevice, c->s));
-  FDISPATCH(c, T, hipLaunchKernelGGL(k_beta_step<T>, grid1(dg.nv), dim3(256), 0, c->s, PC, (const double *)RH,
-                                     (const double *)RF, (T *)c->f[LP_BETA_PCON], c->d_vfac, D.active, dg, cf.dt));
+  return LP_OK;
+}
+
+// :1795-1843 calcbeta_pcon and/or :4481-4508 step_pollen2
+static int do_beta_step(lp_ctx *c, int do_beta, int do_step) {
+  DropGeo dg = drop_geo(c);
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_beta_step<T>, grid1(dg.nv), dim3(256), 0, c->s, (double *)c->f[LP_PCON],
+                                     (const double *)c->f[LP_RHS_PCON], (const double *)c->f[LP_RHS_PCON_F],
+                                     (T *)c->f[LP_BETA_PCON], c->d_vfac, c->cfg.drop.active, dg, c->cfg.dt, do_beta,
+                                     do_step));
   HIPCHK(hipGetLastError());
   return LP_OK;
+}
+
+// scalars_module_crossflow_mod.f90:1494-1786 pollen_all_in_one2: RHS_PCon_f = RHS_PCon,
+// pollen_RHS_calc2, Euler start, calcbeta_pcon + step_pollen2 (fused)
+static int do_pollen(lp_ctx *c) {
+  const lp_droplet_config &D = c->cfg.drop;
+  // :1521 -- RHS_PCon_f takes the old RHS by exchanging the buffers (k_pollen_rhs
+  // rewrites every point of the new RHS_PCon)
+  std::swap(c->f[LP_RHS_PCON], c->f[LP_RHS_PCON_F]);
+  RCHK(do_pollen_rhs(c, (const double *)c->f[LP_RHS_PCON_F]));
+  if ((c->jt_total == D.pcon_init || c->jt_total == D.src_jt0) && !D.init_pcon)  // :1660-1665
+    HIPCHK(hipMemcpyAsync(c->f[LP_RHS_PCON_F], c->f[LP_RHS_PCON], c->bytes[LP_RHS_PCON], hipMemcpyDeviceToDevice,
+                          c->s));
+  return do_beta_step(c, 1, 1);
 }
 
 // main.f90:864-898: dudt += grad(ke) with ke = energy(); the x/y gradients come from
@@ -1240,14 +1258,19 @@ extern "C" int lp_lagrange_sdep(lp_ctx *c) {
   return LP_OK;
 }
 
-extern "C" int lp_pollen_all_in_one2(lp_ctx *c) {
-  if (!c) return set_err(LP_ERR_ARG, "lp_pollen_all_in_one2: null context");
-  if (c->cfg.drop.npcon < 1) return set_err(LP_ERR_ARG, "lp_pollen_all_in_one2 needs npcon > 0");
-  HIPCHK(hipSetDevice(c->dev));
-  RCHK(do_pollen(c));
-  HIPCHK(hipStreamSynchronize(c->s));
-  return LP_OK;
-}
+#define DROP_ENTRY(name, body)                                                      \
+  extern "C" int name(lp_ctx *c) {                                                  \
+    if (!c) return set_err(LP_ERR_ARG, #name ": null context");                     \
+    if (c->cfg.drop.npcon < 1) return set_err(LP_ERR_ARG, #name " needs npcon > 0"); \
+    HIPCHK(hipSetDevice(c->dev));                                                   \
+    RCHK(body);                                                                     \
+    HIPCHK(hipStreamSynchronize(c->s));                                             \
+    return LP_OK;                                                                   \
+  }
+DROP_ENTRY(lp_pollen_all_in_one2, do_pollen(c))
+DROP_ENTRY(lp_pollen_rhs_calc2, do_pollen_rhs(c, nullptr))
+DROP_ENTRY(lp_calcbeta_pcon, do_beta_step(c, 1, 0))
+DROP_ENTRY(lp_step_pollen2, do_beta_step(c, 0, 1))
 
 extern "C" int lp_press_stag_array(lp_ctx *c) {
   if (!c) return set_err(LP_ERR_ARG, "lp_press_stag_array: null context");

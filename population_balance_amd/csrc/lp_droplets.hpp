@@ -124,8 +124,9 @@ struct PollenParams {
 // pollen_RHS_calc2 for every bin at one cell: face velocities of the bin (:2243-2276),
 // ghosts (:2478-2590), SMART faces (:2877-3475), flux divergence and SGS diffusion
 // (:3559-3677) and the point source (:4222-4256).  One thread per point of the volume;
-// points outside the cells keep the values of the previous RHS, which the caller has
-// moved to RF (RHS_PCon_f = RHS_PCon, :1521, done by exchanging the two buffers).
+// points outside the cells keep the values of the previous RHS: when the caller has
+// moved it to RF (RHS_PCon_f = RHS_PCon, :1521, done by exchanging the two buffers) they
+// are copied back from RF, otherwise (RF null) they are left alone.
 template <typename T>
 __global__ __launch_bounds__(256) void k_pollen_rhs(const double *__restrict__ PC, double *__restrict__ RH,
                                                     const double *__restrict__ RF, const double *__restrict__ ui0,
@@ -142,7 +143,8 @@ __global__ __launch_bounds__(256) void k_pollen_rhs(const double *__restrict__ P
   const int j = (int)((s % np) / ld);
   const int i = (int)(s % ld);
   if (k >= nz - 1 || i >= nx) {
-    for (int b = 0; b < g.npc; ++b) RH[(long)b * g.nv + s] = RF[(long)b * g.nv + s];
+    if (RF)
+      for (int b = 0; b < g.npc; ++b) RH[(long)b * g.nv + s] = RF[(long)b * g.nv + s];
     return;
   }
   const int im = (i == 0) ? nx - 1 : i - 1, ip = (i == nx - 1) ? 0 : i + 1;
@@ -336,19 +338,21 @@ __global__ __launch_bounds__(64) void k_pbe(const double *__restrict__ PC, doubl
 // calcbeta_pcon (:1795-1843; buoyancy of all bins into beta_pcon, zero outside the cells
 // or when the droplets are passive) and step_pollen2 (:4481-4508; AB2 in the cells, the
 // top level copies level nz-1).  One thread per point of the volume.
+// do_beta / do_step select either routine or both (fused: one pass over PCon).
 template <typename T>
 __global__ void k_beta_step(double *__restrict__ PC, const double *__restrict__ RH, const double *__restrict__ RF,
-                            T *__restrict__ bp, const double *__restrict__ vfac, int active, DropGeo g, double dt) {
+                            T *__restrict__ bp, const double *__restrict__ vfac, int active, DropGeo g, double dt,
+                            int do_beta, int do_step) {
   const long s = gtid();
   if (s >= g.nv) return;
   const int k = (int)(s / g.np), x = (int)(s % g.ld);
   if (k == g.nz - 1) {
-    bp[s] = T(0);
+    if (do_beta) bp[s] = T(0);
     return;
   }
   if (x >= g.nx) {
-    bp[s] = T(0);
-    if (k == g.nz - 2)
+    if (do_beta) bp[s] = T(0);
+    if (do_step && k == g.nz - 2)
       for (int b = 0; b < g.npc; ++b) PC[(long)b * g.nv + s + g.np] = PC[(long)b * g.nv + s];
     return;
   }
@@ -356,12 +360,14 @@ __global__ void k_beta_step(double *__restrict__ PC, const double *__restrict__ 
   for (int b = 0; b < g.npc; ++b) {
     const long o = (long)b * g.nv + s;
     const double c = PC[o];
-    tb = tb + vfac[b] * c;
-    const double cn = c + dt * (1.5 * RH[o] - 0.5 * RF[o]);
-    PC[o] = cn;
-    if (k == g.nz - 2) PC[o + g.np] = cn;
+    if (do_beta) tb = tb + vfac[b] * c;
+    if (do_step) {
+      const double cn = c + dt * (1.5 * RH[o] - 0.5 * RF[o]);
+      PC[o] = cn;
+      if (k == g.nz - 2) PC[o + g.np] = cn;
+    }
   }
-  bp[s] = active ? (T)tb : T(0);
+  if (do_beta) bp[s] = active ? (T)tb : T(0);
 }
 
 // energy.f90:23-31 kinetic energy on the uvp nodes (zero outside the cells)

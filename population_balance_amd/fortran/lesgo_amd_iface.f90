@@ -14,6 +14,7 @@ module lesgo_amd
   public :: lp_wallstress, lp_sgs_stag, lp_lagrange_sdep, lp_interpolag_sdep, lp_convec
   public :: lp_press_stag_array, lp_project, lp_timestep_conditions, lp_rmsdiv, lp_step
   public :: lp_pollen_all_in_one2, lp_calculate_probability
+  public :: lp_pollen_rhs_calc2, lp_calcbeta_pcon, lp_step_pollen2
   public :: LP_OK, LP_FP64, LP_MIXED, LP_LBC_WALL, LP_LBC_STRESS_FREE
   public :: LP_U, LP_V, LP_W, LP_P, LP_RHSX, LP_RHSY, LP_RHSZ
   public :: LP_PCON, LP_RHS_PCON, LP_RHS_PCON_F, LP_RHS_PBE, LP_BETA_PCON
@@ -240,6 +241,18 @@ module lesgo_amd
       real(c_double), intent(out) :: rms
     end function
     integer(c_int) function lp_pollen_all_in_one2(ctx) bind(C, name="lp_pollen_all_in_one2")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function lp_pollen_rhs_calc2(ctx) bind(C, name="lp_pollen_rhs_calc2")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function lp_calcbeta_pcon(ctx) bind(C, name="lp_calcbeta_pcon")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function lp_step_pollen2(ctx) bind(C, name="lp_step_pollen2")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
     end function

@@ -72,6 +72,9 @@ def load():
         "lp_rmsdiv": ([vp, C.POINTER(C.c_double)], C.c_int),
         "lp_step": ([vp, i32], C.c_int),
         "lp_pollen_all_in_one2": ([vp], C.c_int),
+        "lp_pollen_rhs_calc2": ([vp], C.c_int),
+        "lp_calcbeta_pcon": ([vp], C.c_int),
+        "lp_step_pollen2": ([vp], C.c_int),
         "lp_calculate_probability": ([vp, i32, C.c_double, C.c_double, vp], C.c_int),
         "lp_prof_enable": ([vp, i32], C.c_int),
         "lp_prof_read": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)], C.c_int),
@@ -267,6 +270,15 @@ class Lesgo:
 
     def pollen_all_in_one2(self):
         self._check(self.L.lp_pollen_all_in_one2(self.h), "pollen_all_in_one2")
+
+    def pollen_rhs_calc2(self):
+        self._check(self.L.lp_pollen_rhs_calc2(self.h), "pollen_rhs_calc2")
+
+    def calcbeta_pcon(self):
+        self._check(self.L.lp_calcbeta_pcon(self.h), "calcbeta_pcon")
+
+    def step_pollen2(self):
+        self._check(self.L.lp_step_pollen2(self.h), "step_pollen2")
 
     def step(self, n=1):
         self._check(self.L.lp_step(self.h, int(n)), "step")

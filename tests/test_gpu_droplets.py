@@ -218,3 +218,17 @@ def test_droplet_config_rejected():
     cfg.drop.src_k = cfg.grid.nz - 1
     with pytest.raises(LesgoError, match="src_k"):
         Lesgo(cfg)
+
+
+@pytest.mark.parametrize("bc", [(1, 1), (2, 2)], ids=["inlet", "ghost"])
+def test_pollen_parts(bc):
+    """pollen_RHS_calc2, calcbeta_pcon and step_pollen2 called one by one"""
+    cfg, ds = _cfg(bc=bc, pbe=True, settling=True, inertia=True, active=True)
+    o, lg = _pair(cfg)
+    st = _routine_state(cfg, ds, seed=17)
+    st["RHS_PCon_f"] = 0.3 * st["RHS_PCon"]
+    _load(o, lg, st, jt_total=4)
+    for name in ("pollen_rhs_calc2", "calcbeta_pcon", "step_pollen2"):
+        o.call(name)
+        getattr(lg, name)()
+        _cmp(o, lg, DROP_OUT, TOL, cfg.grid.nx)
