@@ -15,6 +15,7 @@
 #include "lp_internal.hpp"
 #include "lp_kernels.hpp"
 #include "lp_droplets.hpp"
+#include "lp_lag.hpp"
 
 using namespace lp;
 
@@ -106,10 +107,10 @@ struct lp_ctx {
   void *big[9] = {};          // 3/2-grid physical fields (ld_big x ny2 x nz)
   void *tmp[4] = {};          // field-type volumes
   void *S[6] = {};            // strain rates
-  void *lagin[kNumLag] = {};  // fields to be test-filtered
-  void *lagbar[kNumLag] = {}, *laghat[kNumLag] = {};
-  void **d_lagin = nullptr, **d_lagbar = nullptr, **d_laghat = nullptr;
-  double *Fi[4] = {};         // interpolated Lagrangian averages
+  void *lspec = nullptr;      // x spectra of the 21 test-filter inputs [21][nz][ny][nc]
+  void *lbar = nullptr, *lhat = nullptr;  // their y-filtered spectra [21][nz][ny][ncut]
+  void *lbarv = nullptr, *lhatv = nullptr;  // the 42 filtered fields as volumes [21][nz][ny][ld]
+  double *Fi[4] = {};         // second buffer of the Lagrangian averages (exchanged with F)
   void *Hs[3] = {};           // pressure RHS spectra
   void *bw = nullptr, *tw = nullptr;  // wall / lid divergence spectra
   double *gam = nullptr;      // Thomas scratch (nz+1) x lh x ny
@@ -430,10 +431,7 @@ extern "C" int lp_destroy(lp_ctx *c) {
   for (auto p : c->big) fr(p);
   for (auto p : c->tmp) fr(p);
   for (auto p : c->S) fr(p);
-  for (auto p : c->lagin) fr(p);
-  for (auto p : c->lagbar) fr(p);
-  for (auto p : c->laghat) fr(p);
-  fr(c->d_lagin); fr(c->d_lagbar); fr(c->d_laghat);
+  fr(c->lspec); fr(c->lbar); fr(c->lhat); fr(c->lbarv); fr(c->lhatv);
   for (auto p : c->Fi) fr(p);
   for (auto p : c->Hs) fr(p);
   fr(c->bw); fr(c->tw); fr(c->gam);
@@ -534,20 +532,13 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
   for (auto &p : c->tmp) CK(dalloc(&p, vol_b));
   for (auto &p : c->S) CK(dalloc(&p, vol_b));
   if (cfg->model == 5) {
-    for (int q = 0; q < kNumLag; ++q) {
-      CK(dalloc(&c->lagin[q], vol_b));
-      CK(dalloc(&c->lagbar[q], vol_b));
-      CK(dalloc(&c->laghat[q], vol_b));
-    }
-    CK(dalloc((void **)&c->d_lagin, sizeof(void *) * kNumLag));
-    CK(dalloc((void **)&c->d_lagbar, sizeof(void *) * kNumLag));
-    CK(dalloc((void **)&c->d_laghat, sizeof(void *) * kNumLag));
-    if (hipMemcpy(c->d_lagin, c->lagin, sizeof(void *) * kNumLag, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->d_lagbar, c->lagbar, sizeof(void *) * kNumLag, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->d_laghat, c->laghat, sizeof(void *) * kNumLag, hipMemcpyHostToDevice) != hipSuccess) {
-      lp_destroy(c);
-      return set_err(LP_ERR_HIP, "pointer table upload failed");
-    }
+    const size_t pl = (size_t)kNumLag * g.ny * g.nz * 2 * fe;  // one kx column of all 21 fields
+    const int nc = c->ncut1 > c->ncut2 ? c->ncut1 : c->ncut2;
+    CK(dalloc(&c->lspec, pl * nc));
+    CK(dalloc(&c->lbar, pl * c->ncut1));
+    CK(dalloc(&c->lhat, pl * c->ncut2));
+    CK(dalloc(&c->lbarv, (size_t)kNumLag * vol_b));
+    CK(dalloc(&c->lhatv, (size_t)kNumLag * vol_b));
     for (auto &p : c->Fi) CK(dalloc((void **)&p, sizeof(double) * nv));
   }
   for (auto &p : c->Hs) CK(dalloc(&p, spec_b));
@@ -763,7 +754,26 @@ static int do_wallstress(lp_ctx *c) {
   return LP_OK;
 }
 
+static void swap_F(lp_ctx *c) {
+  const int fid[4] = {LP_F_LM, LP_F_MM, LP_F_QN, LP_F_NN};
+  for (int q = 0; q < 4; ++q) {
+    void *t = c->f[fid[q]];
+    c->f[fid[q]] = c->Fi[q];
+    c->Fi[q] = (double *)t;
+  }
+}
+
+static int reset_lag_velocities(lp_ctx *c) {  // interpolag_Sdep.f90:367-369
+  size_t vb = c->bytes[LP_U_LAG];
+  HIPCHK(hipMemsetAsync(c->f[LP_U_LAG], 0, vb, c->s));
+  HIPCHK(hipMemsetAsync(c->f[LP_V_LAG], 0, vb, c->s));
+  HIPCHK(hipMemsetAsync(c->f[LP_W_LAG], 0, vb, c->s));
+  return LP_OK;
+}
+
+// interpolag_Sdep.f90:62-369 on its own: F_LM/MM/QN/NN <- interpolated averages
 static int do_interpolag(lp_ctx *c) {
+  if (!c->Fi[0]) return set_err(LP_ERR_ARG, "interpolag_Sdep needs model 5");
   Geo g = geo(c);
   const lp_config &cf = c->cfg;
   FDISPATCH(c, T,
@@ -774,34 +784,45 @@ static int do_interpolag(lp_ctx *c) {
                                (const T *)c->f[LP_W_LAG], g, (double)cf.cs_count, cf.dt, c->g.dx, c->g.dy,
                                c->g.dz));
   HIPCHK(hipGetLastError());
-  size_t vb = c->bytes[LP_U_LAG];
-  HIPCHK(hipMemsetAsync(c->f[LP_U_LAG], 0, vb, c->s));  // interpolag_Sdep.f90:367-369
-  HIPCHK(hipMemsetAsync(c->f[LP_V_LAG], 0, vb, c->s));
-  HIPCHK(hipMemsetAsync(c->f[LP_W_LAG], 0, vb, c->s));
+  swap_F(c);
+  return reset_lag_velocities(c);
+}
+
+// x back-transform of the 21 filtered spectra [21][nz][ny][ncols] into 21 volumes
+static int c2r_batch(lp_ctx *c, const void *spec, int ncols, void *out) {
+  const lp_grid &g = c->g;
+  prof_begin(c, LP_K_ROW_C2R);
+  HIPCHK(row_c2r(c->fdt, c->fdt, g.nx / 2, spec, ncols, (long)ncols * g.ny, ncols, g.ny, kNumLag * g.nz, out, g.ld,
+                 plane_el(c), c->s));
+  prof_end(c, LP_K_ROW_C2R, bytes_c2r(g.nx, g.ny, kNumLag * g.nz, ncols, esz_of(c->fdt), esz_of(c->fdt)));
   return LP_OK;
 }
 
+// lagrange_Sdep.f90:4-496 (with interpolag_Sdep at :120).  Needs the strain rates in
+// c->S and |S| in c->tmp[3] (k_strain).
 static int do_lagrange(lp_ctx *c) {
   const lp_config &cf = c->cfg;
   const lp_grid &gg = c->g;
   Geo g = geo(c);
-  RCHK(do_interpolag(c));  // lagrange_Sdep.f90:120
-  FDISPATCH(c, T, hipLaunchKernelGGL(k_lag_prep<T>, grid1(g.np * g.nz), dim3(256), 0, c->s,
-                                     (const T *)c->f[LP_U], (const T *)c->f[LP_V], (const T *)c->f[LP_W],
-                                     (const T *)c->S[0], (const T *)c->S[1], (const T *)c->S[2],
-                                     (const T *)c->S[3], (const T *)c->S[4], (const T *)c->S[5],
-                                     (T *const *)c->d_lagin, g));
-  HIPCHK(hipGetLastError());
-  // lagrange_Sdep.f90:157-256: every field filtered at 2*delta and 4*delta
-  // the filters are sharp cutoffs: kx columns >= ncut1 (G_test) / ncut2 (G_test_test)
-  // are zero after filtering, so only the leading columns are transformed
   const int n1 = c->ncut1, n2 = c->ncut2, nc = n1 > n2 ? n1 : n2;
-  for (int q = 0; q < kNumLag; ++q) {
-    RCHK(r2c_small(c, c->lagin[q], gg.nz, 1.0, c->spec[0], nc));
-    ColOut o[2] = {{c->spec[1], M_G1}, {c->spec[2], M_G2}};
-    RCHK(col_small(c, c->spec[0], gg.nz, 1, 1, 2, o, nc));
-    RCHK(c2r_small(c, c->spec[1], n1, gg.nz, c->lagbar[q], c->fdt));
-    RCHK(c2r_small(c, c->spec[2], n2, gg.nz, c->laghat[q], c->fdt));
+  const size_t fe = esz_of(c->fdt);
+  const double vol = (double)g.np * g.nz;
+  // lagrange_Sdep.f90:129-155,192-242: the 21 filter inputs, transformed along x
+  LagInputs in;
+  in.u = c->f[LP_U]; in.v = c->f[LP_V]; in.w = c->f[LP_W];
+  for (int q = 0; q < 6; ++q) in.S[q] = c->S[q];
+  in.magS = c->tmp[3];
+  prof_begin(c, LP_K_ROW_R2C);
+  HIPCHK(lag_r2c(c->fdt, in, g, c->lspec, nc, c->s));
+  prof_end(c, LP_K_ROW_R2C, vol * 10.0 * fe + (double)kNumLag * gg.ny * gg.nz * nc * 2 * fe);
+  // :157-256 the y part of both test filters, all 21 fields in one pass
+  {
+    const long ps = (long)nc * gg.ny;
+    ColOut o[2] = {{c->lbar, M_G1, n1, (long)n1 * gg.ny, n1}, {c->lhat, M_G2, n2, (long)n2 * gg.ny, n2}};
+    prof_begin(c, LP_K_COL_OP);
+    HIPCHK(col_op(c->fdt, gg.ny, gg.ny, 0, 1, 1, c->lspec, nc, ps, nc, kNumLag * gg.nz, 2, o, nc, ps, c->tab,
+                  c->s));
+    prof_end(c, LP_K_COL_OP, (double)kNumLag * gg.nz * gg.ny * (nc + n1 + n2) * 2 * fe);
   }
   double delta = cf.filter_size * std::pow(gg.dx * gg.dy * gg.dz, 1.0 / 3.0);
   LagParams P;
@@ -816,15 +837,20 @@ static int do_lagrange(lp_ctx *c) {
   P.bclip_min = 1.0 / (2.0 * 4.0);
   P.init_lm = cf.inilag && !c->F_LM_MM_init && (c->jt == cf.cs_count || c->jt == cf.dyn_init);
   P.init_qn = cf.inilag && !c->F_QN_NN_init && (c->jt == cf.cs_count || c->jt == cf.dyn_init);
+  LagInterp I{(double)cf.cs_count, cf.dt, gg.dx, gg.dy, gg.dz};
+  const double *Fold[4] = {(const double *)c->f[LP_F_LM], (const double *)c->f[LP_F_MM],
+                           (const double *)c->f[LP_F_QN], (const double *)c->f[LP_F_NN]};
+  double *Fnew[4] = {c->Fi[0], c->Fi[1], c->Fi[2], c->Fi[3]};
+  // back along x: the 42 filtered fields as volumes
+  RCHK(c2r_batch(c, c->lbar, n1, c->lbarv));
+  RCHK(c2r_batch(c, c->lhat, n2, c->lhatv));
+  // :257-434 (+ interpolag_Sdep.f90:62-365): interpolation, contractions, relaxation
   prof_begin(c, LP_K_LAG_RELAX);
-  FDISPATCH(c, T,
-            hipLaunchKernelGGL(k_lag_relax<T>, grid1(g.np * g.nz), dim3(256), 0, c->s, (T *const *)c->d_lagbar,
-                               (T *const *)c->d_laghat, c->Fi[0], c->Fi[1], c->Fi[2], c->Fi[3],
-                               (double *)c->f[LP_F_LM], (double *)c->f[LP_F_MM], (double *)c->f[LP_F_QN],
-                               (double *)c->f[LP_F_NN], (T *)c->f[LP_CS_OPT2], (T *)c->f[LP_BETA], g, P));
-  HIPCHK(hipGetLastError());
-  prof_end(c, LP_K_LAG_RELAX,
-           (double)g.np * g.nz * (2.0 * kNumLag * esz_of(c->fdt) + 8.0 * 8 + 2.0 * esz_of(c->fdt)));
+  HIPCHK(lag_relax(c->fdt, c->lbarv, c->lhatv, Fold, Fnew, c->f[LP_U_LAG], c->f[LP_V_LAG], c->f[LP_W_LAG],
+                   c->f[LP_CS_OPT2], c->f[LP_BETA], g, P, I, c->s));
+  prof_end(c, LP_K_LAG_RELAX, vol * (2.0 * kNumLag * fe + 8.0 * 8 + 5.0 * fe));
+  swap_F(c);
+  RCHK(reset_lag_velocities(c));
   if (P.init_lm) c->F_LM_MM_init = 1;
   if (P.init_qn) c->F_QN_NN_init = 1;
   return LP_OK;
@@ -839,7 +865,8 @@ static int do_sgs_stag(lp_ctx *c) {
                                      (const T *)c->f[LP_DUDY], (const T *)c->f[LP_DUDZ], (const T *)c->f[LP_DVDX],
                                      (const T *)c->f[LP_DVDY], (const T *)c->f[LP_DVDZ], (const T *)c->f[LP_DWDX],
                                      (const T *)c->f[LP_DWDY], (const T *)c->f[LP_DWDZ], (T *)c->S[0],
-                                     (T *)c->S[1], (T *)c->S[2], (T *)c->S[3], (T *)c->S[4], (T *)c->S[5], g));
+                                     (T *)c->S[1], (T *)c->S[2], (T *)c->S[3], (T *)c->S[4], (T *)c->S[5],
+                                     (T *)c->tmp[3], g));
   HIPCHK(hipGetLastError());
   double delta = cf.filter_size * std::pow(gg.dx * gg.dy * gg.dz, 1.0 / 3.0);
   if (cf.model == 5) {
@@ -1251,7 +1278,8 @@ extern "C" int lp_lagrange_sdep(lp_ctx *c) {
                                      (const T *)c->f[LP_DUDY], (const T *)c->f[LP_DUDZ], (const T *)c->f[LP_DVDX],
                                      (const T *)c->f[LP_DVDY], (const T *)c->f[LP_DVDZ], (const T *)c->f[LP_DWDX],
                                      (const T *)c->f[LP_DWDY], (const T *)c->f[LP_DWDZ], (T *)c->S[0],
-                                     (T *)c->S[1], (T *)c->S[2], (T *)c->S[3], (T *)c->S[4], (T *)c->S[5], g));
+                                     (T *)c->S[1], (T *)c->S[2], (T *)c->S[3], (T *)c->S[4], (T *)c->S[5],
+                                     (T *)c->tmp[3], g));
   HIPCHK(hipGetLastError());
   RCHK(do_lagrange(c));
   HIPCHK(hipStreamSynchronize(c->s));

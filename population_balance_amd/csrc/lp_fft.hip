@@ -13,20 +13,7 @@
 
 namespace lp {
 
-template <typename T> struct V2;
-template <> struct V2<double> { using t = double2; };
-template <> struct V2<float> { using t = float2; };
-
-// lanes per row transform of M complex points (P) -- M = P*Q
-__host__ __device__ constexpr int row_lanes(int M) {
-  return M == 16 ? 4 : M == 24 ? 4 : M == 32 ? 4 : M == 48 ? 4 : M == 64 ? 8 : M == 72 ? 6
-       : M == 96 ? 8 : M == 128 ? 8 : M == 144 ? 12 : M == 192 ? 8 : 0;
-}
 static constexpr int kRowBlock = 256;
-
-// LDS column stride (elements) of a transform slot holding n elements: rounded so that
-// consecutive slots start 4 elements apart modulo 32
-__host__ __device__ constexpr int slot_stride(int n) { return ((n + 27) / 32) * 32 + 4; }
 
 // ---------------------------------------------------------------------------------
 // row_r2c: real rows of 2M values -> M+1 complex (half-length complex FFT + split)
@@ -57,19 +44,19 @@ __global__ __launch_bounds__(kRowBlock) void k_row_r2c(const T *__restrict__ in,
     v[q] = mk<T>(p.x * scale, p.y * scale);
   }
   F::stage1(v, t, sc, twM);
-  __syncthreads();
+  slot_sync<P>();
   cx<T> a[F::G][P];
   bool ok[F::G];
 #pragma unroll
   for (int j = 0; j < F::G; ++j) ok[j] = F::stage2(j, t, sc, a[j]);
-  __syncthreads();
+  slot_sync<P>();
 #pragma unroll
   for (int j = 0; j < F::G; ++j)
     if (ok[j]) {
 #pragma unroll
       for (int k1 = 0; k1 < P; ++k1) sc[t + P * j + Q * k1] = a[j][k1];
     }
-  __syncthreads();
+  slot_sync<P>();
   // split: X[k] = (Z[k] + conj Z[M-k])/2 - i/2 w^k (Z[k] - conj Z[M-k]), w = exp(-2 pi i/2M)
   if (valid) {
     cx<T> *dst = out + plane * out_plane + y * out_pitch;
@@ -120,7 +107,7 @@ __global__ __launch_bounds__(kRowBlock) void k_row_c2r(const cx<T> *__restrict__
     v[q] = mk<T>(e.x - wo.y, e.y + wo.x);
   }
   F::stage1(v, t, sc, twM);
-  __syncthreads();
+  slot_sync<P>();
   TO *dst = out + plane * out_plane + y * out_pitch;
 #pragma unroll
   for (int j = 0; j < F::G; ++j) {
@@ -148,8 +135,10 @@ __global__ __launch_bounds__(kRowBlock) void k_row_c2r(const cx<T> *__restrict__
 struct ColSlot {
   void *ptr;
   long pitch;  // complex elements between y rows of this output
+  long plane;  // complex elements between planes of this output
   int mult;
   int tpitch;  // row length of the multiplier tables (lh of the small grid)
+  int ncols;   // columns stored (kx < ncols)
 };
 struct ColParams {
   ColSlot outs[3];
@@ -247,14 +236,15 @@ __global__ __launch_bounds__(col_tx<T>() * kColLanes) void k_col_op(
     // addresses are formed from per-output values so that they are not hoisted out of
     // the output loop (which would keep 16 64-bit addresses live)
     const long pitch = Pr.outs[jo].pitch;
-    cx<T> *dst = reinterpret_cast<cx<T> *>(Pr.outs[jo].ptr) + (long)plane * out_plane + kx;
+    const bool st = cv && kx < Pr.outs[jo].ncols;
+    cx<T> *dst = reinterpret_cast<cx<T> *>(Pr.outs[jo].ptr) + (long)plane * Pr.outs[jo].plane + kx;
     if (inv) {
       FO::stage1(v, t, sc, twO);
       __syncthreads();
 #pragma unroll
       for (int j = 0; j < FO::G; ++j) {
         cx<T> a[PL];
-        if (FO::stage2(j, t, sc, a) && cv) {
+        if (FO::stage2(j, t, sc, a) && st) {
           cx<T> *d = dst + (long)(t + PL * j) * pitch;
           const long ostep = (long)QO * pitch;
 #pragma unroll
@@ -262,7 +252,7 @@ __global__ __launch_bounds__(col_tx<T>() * kColLanes) void k_col_op(
         }
       }
       __syncthreads();
-    } else if (cv) {
+    } else if (st) {
       cx<T> *d = dst + (long)t * pitch;
       const long ostep = (long)PL * pitch;
 #pragma unroll
@@ -360,8 +350,15 @@ static hipError_t launch_col(int padmode, int fwd, int inv, const void *in, long
   constexpr int TX = col_tx<T>();
   ColParams P;
   P.nout = nout;
-  for (int j = 0; j < 3; ++j) P.outs[j] = j < nout ? ColSlot{outs[j].ptr, out_pitch, outs[j].mult, tab.lh}
-                                     : ColSlot{nullptr, 0, 0, 0};
+  for (int j = 0; j < 3; ++j) {
+    if (j >= nout) {
+      P.outs[j] = ColSlot{nullptr, 0, 0, 0, 0, 0};
+      continue;
+    }
+    const ColOut &o = outs[j];
+    P.outs[j] = ColSlot{o.ptr, o.pitch ? o.pitch : out_pitch, o.plane ? o.plane : out_plane, o.mult, tab.lh,
+                        o.ncols ? o.ncols : ncols};
+  }
   dim3 grid((unsigned)((ncols + TX - 1) / TX), (unsigned)nplanes);
   hipLaunchKernelGGL((k_col_op<T, NYI, NYO>), grid, dim3(TX * kColLanes), 0, s, (const cx<T> *)in, in_pitch,
                      in_plane, ncols, fwd, inv, P, out_pitch, out_plane, tab.kx, tab.ky, tab.G1, tab.G2, tab.lh,

@@ -23,6 +23,34 @@ __device__ constexpr double kW144[144][2] = {
 #include "lp_w144.inc"
 };
 
+template <typename T> struct V2;
+template <> struct V2<double> { using t = double2; };
+template <> struct V2<float> { using t = float2; };
+
+// lanes per row transform of M complex points (P) -- M = P*Q
+__host__ __device__ constexpr int row_lanes(int M) {
+  return M == 16 ? 4 : M == 24 ? 4 : M == 32 ? 4 : M == 48 ? 4 : M == 64 ? 8 : M == 72 ? 6
+       : M == 96 ? 8 : M == 128 ? 8 : M == 144 ? 12 : M == 192 ? 8 : 0;
+}
+
+// LDS stride (elements) of a transform slot holding n elements: rounded so that
+// consecutive slots start 4 elements apart modulo 32
+__host__ __device__ constexpr int slot_stride(int n) { return ((n + 27) / 32) * 32 + 4; }
+
+// Barrier for the LDS slot of one transform done by P adjacent lanes: when P divides the
+// wavefront the slot belongs to one wavefront and a wavefront-scope barrier orders its
+// LDS accesses; otherwise the whole block synchronises.
+template <int P>
+__device__ __forceinline__ void slot_sync() {
+  if constexpr (64 % P == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 template <typename T, int SIGN>
 __device__ __forceinline__ cx<T> tw_const(int m, int R) {
   const int idx = (m % R) * (144 / R);

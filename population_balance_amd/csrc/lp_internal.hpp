@@ -40,6 +40,9 @@ hipError_t row_c2r(Dt dtc, Dt dto, int M, const void *in, long in_pitch, long in
 struct ColOut {
   void *ptr;
   int mult;
+  long pitch = 0;  // overrides of the call's out_pitch / out_plane / ncols (0: keep)
+  long plane = 0;
+  int ncols = 0;
 };
 
 // Column pass over ncols kx-columns of nplanes spectra:

@@ -67,13 +67,17 @@ __global__ void k_fill(T *__restrict__ a, long n, T v) {
   if (i < n) a[i] = v;
 }
 
+template <typename T> __device__ __forceinline__ T smag(T s11, T s12, T s13, T s22, T s23, T s33) {
+  return sqrt(T(2) * (s11 * s11 + s22 * s22 + s33 * s33 + T(2) * (s12 * s12 + s13 * s13 + s23 * s23)));
+}
+
 // sgs_stag.f90:49-125 strain rates: uvp nodes at jz = 1, w-nodes above
 template <typename T>
 __global__ void k_strain(const T *__restrict__ dudx, const T *__restrict__ dudy, const T *__restrict__ dudz,
                          const T *__restrict__ dvdx, const T *__restrict__ dvdy, const T *__restrict__ dvdz,
                          const T *__restrict__ dwdx, const T *__restrict__ dwdy, const T *__restrict__ dwdz,
                          T *__restrict__ S11, T *__restrict__ S12, T *__restrict__ S13, T *__restrict__ S22,
-                         T *__restrict__ S23, T *__restrict__ S33, Geo g) {
+                         T *__restrict__ S23, T *__restrict__ S33, T *__restrict__ Smag, Geo g) {
   long i = gtid();
   if (i >= g.np * g.nz) return;
   int k = (int)(i / g.np), x = (int)(i % g.ld);
@@ -92,8 +96,10 @@ __global__ void k_strain(const T *__restrict__ dudx, const T *__restrict__ dudy,
     vx = h * (dvdx[i] + dvdx[m]); vy = h * (dvdy[i] + dvdy[m]); vz = dvdz[i];
     wx = dwdx[i]; wy = dwdy[i]; wz = h * (dwdz[i] + dwdz[m]);
   }
-  S11[i] = ux; S12[i] = h * (uy + vx); S13[i] = h * (uz + wx);
-  S22[i] = vy; S23[i] = h * (vz + wy); S33[i] = wz;
+  const T s11 = ux, s12 = h * (uy + vx), s13 = h * (uz + wx), s22 = vy, s23 = h * (vz + wy), s33 = wz;
+  S11[i] = s11; S12[i] = s12; S13[i] = s13;
+  S22[i] = s22; S23[i] = s23; S33[i] = s33;
+  if (Smag) Smag[i] = smag(s11, s12, s13, s22, s23, s33);  // sgs_stag.f90:215 |S|
 }
 
 // sgs_stag.f90:130-134  u_lag = u_lag + u (all columns)
@@ -107,9 +113,6 @@ __global__ void k_lag_acc(T *__restrict__ ul, T *__restrict__ vl, T *__restrict_
   wl[i] += w[i];
 }
 
-template <typename T> __device__ __forceinline__ T smag(T s11, T s12, T s13, T s22, T s23, T s33) {
-  return sqrt(T(2) * (s11 * s11 + s22 * s22 + s33 * s33 + T(2) * (s12 * s12 + s13 * s13 + s23 * s23)));
-}
 
 // sgs_stag.f90:137-177 (mixing length) and :215-227 (|S|, Nu_t, dissip, magS)
 template <typename T>
@@ -175,24 +178,14 @@ __global__ void k_tau(const T *__restrict__ S11, const T *__restrict__ S12, cons
 }
 
 // interpolag_Sdep.f90:62-365: backtrace x - u dt and trilinear interpolation of the four
-// Lagrangian averages from the periodic (x, y) / clamped (z) copy of F.  Reads the old F,
-// writes the interpolated values to Fi; columns >= nx keep their old values.
+// Lagrangian averages from the periodic (x, y) / clamped (z) copy of F at point
+// (x < nx, y, k); i = its flat index.
 template <typename T>
-__global__ void k_interpolag(const double *__restrict__ F0, const double *__restrict__ F1,
-                             const double *__restrict__ F2, const double *__restrict__ F3,
-                             double *__restrict__ I0, double *__restrict__ I1, double *__restrict__ I2,
-                             double *__restrict__ I3, const T *__restrict__ ul, const T *__restrict__ vl,
-                             const T *__restrict__ wl, Geo g, double cs, double dt, double dx,
-                             double dy, double dz) {
-  long i = gtid();
-  if (i >= g.np * g.nz) return;
-  int k = (int)(i / g.np);
-  int r = (int)(i % g.np);
-  int y = r / g.ld, x = r % g.ld;
-  if (x >= g.nx) {
-    I0[i] = F0[i]; I1[i] = F1[i]; I2[i] = F2[i]; I3[i] = F3[i];
-    return;
-  }
+__device__ __forceinline__ void interp_point(const double *const (&F)[4], double (&out)[4], const T *__restrict__ ul,
+                                             const T *__restrict__ vl, const T *__restrict__ wl, Geo g, long i,
+                                             int x, int y, int k, double cs, double dt, double dx, double dy,
+                                             double dz) {
+  const int r = (int)(i - (long)k * g.np);
   // :204-258 lag velocities on w-nodes (u_lag at jz=1 stays on the uvp node)
   double ut = (double)ul[i] / cs, vt = (double)vl[i] / cs;
   double ulag, vlag, wlag;
@@ -228,48 +221,38 @@ __global__ void k_interpolag(const double *__restrict__ F0, const double *__rest
   long o0a = (long)ka * g.np + (long)y * g.ld;
   long oa0 = (long)k * g.np + (long)ya * g.ld;
   long oaa = (long)ka * g.np + (long)ya * g.ld;
-  const double *F[4] = {F0, F1, F2, F3};
-  double *O[4] = {I0, I1, I2, I3};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const double *f = F[q];
-    double v = fx * fy * (f[o00 + x] * fz + f[o0a + x] * czp)
-             + fx * cyp * (f[oa0 + x] * fz + f[oaa + x] * czp)
-             + cxp * fy * (f[o00 + xa] * fz + f[o0a + xa] * czp)
-             + cxp * cyp * (f[oa0 + xa] * fz + f[oaa + xa] * czp);
-    O[q][i] = v;
+    out[q] = fx * fy * (f[o00 + x] * fz + f[o0a + x] * czp)
+           + fx * cyp * (f[oa0 + x] * fz + f[oaa + x] * czp)
+           + cxp * fy * (f[o00 + xa] * fz + f[o0a + xa] * czp)
+           + cxp * cyp * (f[oa0 + xa] * fz + f[oaa + xa] * czp);
   }
 }
 
-// lagrange_Sdep.f90:129-155,192-242: the 21 fields that are test-filtered per level
-// order: 0-2 u,v,w (w-node values), 3-8 products 11,12,13,22,23,33, 9-14 S_ij, 15-20 |S| S_ij
+// the whole volume: reads the old F, writes the interpolated values to O; columns
+// >= nx keep their old values
 template <typename T>
-__global__ void k_lag_prep(const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
-                           const T *__restrict__ S11, const T *__restrict__ S12, const T *__restrict__ S13,
-                           const T *__restrict__ S22, const T *__restrict__ S23, const T *__restrict__ S33,
-                           T *const *__restrict__ out, Geo g) {
+__global__ void k_interpolag(const double *__restrict__ F0, const double *__restrict__ F1,
+                             const double *__restrict__ F2, const double *__restrict__ F3,
+                             double *__restrict__ I0, double *__restrict__ I1, double *__restrict__ I2,
+                             double *__restrict__ I3, const T *__restrict__ ul, const T *__restrict__ vl,
+                             const T *__restrict__ wl, Geo g, double cs, double dt, double dx,
+                             double dy, double dz) {
   long i = gtid();
   if (i >= g.np * g.nz) return;
-  int k = (int)(i / g.np), x = (int)(i % g.ld);
-  if (x >= g.nx) return;
-  T ub, vb, wb;
-  if (k == 0) {
-    ub = u[i]; vb = v[i]; wb = T(0.25) * w[i + g.np];
-  } else {
-    ub = T(0.5) * (u[i] + u[i - g.np]);
-    vb = T(0.5) * (v[i] + v[i - g.np]);
-    wb = w[i];
+  int k = (int)(i / g.np);
+  int r = (int)(i % g.np);
+  int y = r / g.ld, x = r % g.ld;
+  if (x >= g.nx) {
+    I0[i] = F0[i]; I1[i] = F1[i]; I2[i] = F2[i]; I3[i] = F3[i];
+    return;
   }
-  out[0][i] = ub; out[1][i] = vb; out[2][i] = wb;
-  out[3][i] = ub * ub; out[4][i] = ub * vb; out[5][i] = ub * wb;
-  out[6][i] = vb * vb; out[7][i] = vb * wb; out[8][i] = wb * wb;
-  T s[6] = {S11[i], S12[i], S13[i], S22[i], S23[i], S33[i]};
-  T sm = smag(s[0], s[1], s[2], s[3], s[4], s[5]);
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    out[9 + q][i] = s[q];
-    out[15 + q][i] = sm * s[q];
-  }
+  const double *const F[4] = {F0, F1, F2, F3};
+  double v[4];
+  interp_point<T>(F, v, ul, vl, wl, g, i, x, y, k, cs, dt, dx, dy, dz);
+  I0[i] = v[0]; I1[i] = v[1]; I2[i] = v[2]; I3[i] = v[3];
 }
 
 __device__ __forceinline__ double r4max(double a, float b) {  // (double) max(real(a), b)
@@ -284,41 +267,34 @@ struct LagParams {
   int casts, init_lm, init_qn;
 };
 
-template <typename T>
-__global__ void k_lag_relax(T *const *__restrict__ bar, T *const *__restrict__ hat,
-                            const double *__restrict__ I0, const double *__restrict__ I1,
-                            const double *__restrict__ I2, const double *__restrict__ I3,
-                            double *__restrict__ FLM, double *__restrict__ FMM, double *__restrict__ FQN,
-                            double *__restrict__ FNN, T *__restrict__ Cs, T *__restrict__ Beta, Geo g,
-                            LagParams P) {
-  long i = gtid();
-  if (i >= g.np * g.nz) return;
-  int k = (int)(i / g.np), x = (int)(i % g.ld);
-  const bool padc = x >= g.ld - 2;
-  T ub = bar[0][i], vb = bar[1][i], wb = bar[2][i];
-  T uh = hat[0][i], vh = hat[1][i], wh = hat[2][i];
-  T L[6], Q[6];
-  L[0] = bar[3][i] - ub * ub; L[1] = bar[4][i] - ub * vb; L[2] = bar[5][i] - ub * wb;
-  L[3] = bar[6][i] - vb * vb; L[4] = bar[7][i] - vb * wb; L[5] = bar[8][i] - wb * wb;
-  Q[0] = hat[3][i] - uh * uh; Q[1] = hat[4][i] - uh * vh; Q[2] = hat[5][i] - uh * wh;
-  Q[3] = hat[6][i] - vh * vh; Q[4] = hat[7][i] - vh * wh; Q[5] = hat[8][i] - wh * wh;
-  T Sb[6], Sh[6];
+// Germano contractions at one point from the 21 fields filtered at one width
+// (lagrange_Sdep.f90:257-299 for 2 delta, :300-326 for 4 delta): V(q) is filtered field
+// q, tf2 the squared filter-width ratio.  Returns L.M (or Q.N) and M.M (or N.N).
+template <typename T, typename Vf>
+__device__ __forceinline__ void contract_point(Vf V, double cst_d, double tf2, double &lm, double &mm) {
+  T ub = V(0), vb = V(1), wb = V(2);
+  T L[6];
+  L[0] = V(3) - ub * ub; L[1] = V(4) - ub * vb; L[2] = V(5) - ub * wb;
+  L[3] = V(6) - vb * vb; L[4] = V(7) - vb * wb; L[5] = V(8) - wb * wb;
+  T Sb[6];
 #pragma unroll
-  for (int q = 0; q < 6; ++q) { Sb[q] = bar[9 + q][i]; Sh[q] = hat[9 + q][i]; }
+  for (int q = 0; q < 6; ++q) Sb[q] = V(9 + q);
   T sbar = smag(Sb[0], Sb[1], Sb[2], Sb[3], Sb[4], Sb[5]);
-  T shat = smag(Sh[0], Sh[1], Sh[2], Sh[3], Sh[4], Sh[5]);
-  const T cst = (T)P.cst, t1 = (T)P.tf1_2, t2 = (T)P.tf2_2;
-  T M[6], N[6];
+  const T cst = (T)cst_d, t1 = (T)tf2;
+  T M[6];
 #pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    M[q] = cst * (bar[15 + q][i] - t1 * sbar * Sb[q]);
-    N[q] = cst * (hat[15 + q][i] - t2 * shat * Sh[q]);
-  }
-  double LM = L[0] * M[0] + L[3] * M[3] + L[5] * M[5] + T(2) * (L[1] * M[1] + L[2] * M[2] + L[4] * M[4]);
-  double MM = M[0] * M[0] + M[3] * M[3] + M[5] * M[5] + T(2) * (M[1] * M[1] + M[2] * M[2] + M[4] * M[4]);
-  double QN = Q[0] * N[0] + Q[3] * N[3] + Q[5] * N[5] + T(2) * (Q[1] * N[1] + Q[2] * N[2] + Q[4] * N[4]);
-  double NN = N[0] * N[0] + N[3] * N[3] + N[5] * N[5] + T(2) * (N[1] * N[1] + N[2] * N[2] + N[4] * N[4]);
-  double flm = I0[i], fmm = I1[i], fqn = I2[i], fnn = I3[i];
+  for (int q = 0; q < 6; ++q) M[q] = cst * (V(15 + q) - t1 * sbar * Sb[q]);
+  lm = L[0] * M[0] + L[3] * M[3] + L[5] * M[5] + T(2) * (L[1] * M[1] + L[2] * M[2] + L[4] * M[4]);
+  mm = M[0] * M[0] + M[3] * M[3] + M[5] * M[5] + T(2) * (M[1] * M[1] + M[2] * M[2] + M[4] * M[4]);
+}
+
+// lagrange_Sdep.f90:327-434: relaxation of the four averages (binary64, with the
+// reference's REAL(4) conversions when casts != 0), Cs^2 and beta.  fi: the interpolated
+// (or, in the padding columns, old) averages on entry, the new ones on return.
+template <typename T>
+__device__ __forceinline__ void relax_point(double LM, double MM, double QN, double NN, double (&fi)[4], bool padc,
+                                            bool top, const LagParams &P, T &cs_out, T &beta_out) {
+  double flm = fi[0], fmm = fi[1], fqn = fi[2], fnn = fi[3];
   const double tiny = P.tiny;
   if (P.init_lm) { fmm = MM; flm = 0.0; if (padc) { fmm = 1.0; flm = 1.0; } }
   double Tn, dum, epsi;
@@ -358,14 +334,14 @@ __global__ void k_lag_relax(T *const *__restrict__ bar, T *const *__restrict__ h
   if (padc) cs4d = tiny;
   cs4d = P.casts ? (double)(float)fmax(tiny, (double)(float)cs4d) : fmax(tiny, cs4d);
   double b = pow(cs4d / cs2d, P.bexp);
-  if (k == g.nz - 1) b = 1.0;
+  if (top) b = 1.0;
   double bclip = P.casts ? (double)fmaxf((float)b, (float)P.bclip_min) : fmax(b, P.bclip_min);
   double csv = cs2d / bclip;
   if (padc) csv = tiny;
   csv = P.casts ? (double)fmaxf(1e-24f, (float)csv) : fmax(tiny, csv);
-  FLM[i] = flm; FMM[i] = fmm; FQN[i] = fqn; FNN[i] = fnn;
-  Cs[i] = (T)csv;
-  Beta[i] = (T)b;
+  fi[0] = flm; fi[1] = fmm; fi[2] = fqn; fi[3] = fnn;
+  cs_out = (T)csv;
+  beta_out = (T)b;
 }
 
 // divstress_uv.f90:116-125 and divstress_w.f90:173-211
@@ -565,42 +541,83 @@ __global__ void k_press_tridiag(const cx<TH> *__restrict__ Hx, const cx<TH> *__r
   ph[m] = uprev;
   double cprev = 1.0;
   bool bad = false;
-  for (int j = 1; j <= nz; ++j) {
-    double a, b, c;
-    cx<double> r;
-    if (j < nz) {
-      a = idz2; b = bi; c = idz2;
-      cx<double> hx = ld_h(Hx, j), hy = ld_h(Hy, j), hz1 = ld_h(Hz, j + 1), hz0 = ld_h(Hz, j);
-      // i*(kx Hx + ky Hy) + (Hz(j+1) - Hz(j))/dz
-      double sx = kx * hx.x + ky * hy.x, sy = kx * hx.y + ky * hy.y;
-      r = mk<double>(-sy + (hz1.x - hz0.x) / dz, sx + (hz1.y - hz0.y) / dz);
-    } else {
-      cx<TH> t = tw_[m];
-      a = -1.0; b = 1.0; c = 0.0;
-      r = mk<double>(-(double)t.x * dz, -(double)t.y * dz);
+  // The right-hand sides do not depend on the recurrence: they are loaded kCh levels at a
+  // time so that the loads of a chunk are in flight together.
+  constexpr int kCh = 8;
+  cx<double> hz_lo = ld_h(Hz, 1);  // Hz(j) for the next row j
+  for (int j0 = 1; j0 <= nz; j0 += kCh) {
+    // unconditional loads from clamped levels (1..nz-1), zeroed afterwards where the
+    // level is nz or beyond (H has no level nz)
+    cx<TH> rx[kCh], ry[kCh], rz[kCh];
+#pragma unroll
+    for (int q = 0; q < kCh; ++q) {
+      const int j = j0 + q;
+      const int lx = j < nz ? j : nz - 1, lz = j + 1 < nz ? j + 1 : nz - 1;
+      rx[q] = Hx[(long)(lx - 1) * nh + m];
+      ry[q] = Hy[(long)(lx - 1) * nh + m];
+      rz[q] = Hz[(long)(lz - 1) * nh + m];
     }
-    double gm = cprev / bet;
-    gam[(long)j * nh + m] = gm;
-    bet = b - a * gm;
-    if (bet == 0.0) bad = true;
-    cx<double> uj = mk<double>((r.x - a * uprev.x) / bet, (r.y - a * uprev.y) / bet);
-    ph[(long)j * nh + m] = uj;
-    uprev = uj;
-    cprev = c;
+    cx<double> hx[kCh], hy[kCh], hz[kCh];
+#pragma unroll
+    for (int q = 0; q < kCh; ++q) {
+      const int j = j0 + q;
+      const bool okx = j < nz, okz = j + 1 < nz;
+      hx[q] = mk<double>(okx ? (double)rx[q].x : 0.0, okx ? (double)rx[q].y : 0.0);
+      hy[q] = mk<double>(okx ? (double)ry[q].x : 0.0, okx ? (double)ry[q].y : 0.0);
+      hz[q] = mk<double>(okz ? (double)rz[q].x : 0.0, okz ? (double)rz[q].y : 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < kCh; ++q) {
+      const int j = j0 + q;
+      if (j > nz) break;
+      double a, b, c;
+      cx<double> r;
+      if (j < nz) {
+        a = idz2; b = bi; c = idz2;
+        const cx<double> hz1 = hz[q], hz0 = hz_lo;
+        // i*(kx Hx + ky Hy) + (Hz(j+1) - Hz(j))/dz
+        double sx = kx * hx[q].x + ky * hy[q].x, sy = kx * hx[q].y + ky * hy[q].y;
+        r = mk<double>(-sy + (hz1.x - hz0.x) / dz, sx + (hz1.y - hz0.y) / dz);
+        hz_lo = hz1;
+      } else {
+        cx<TH> t = tw_[m];
+        a = -1.0; b = 1.0; c = 0.0;
+        r = mk<double>(-(double)t.x * dz, -(double)t.y * dz);
+      }
+      double gm = cprev / bet;
+      gam[(long)j * nh + m] = gm;
+      bet = b - a * gm;
+      if (bet == 0.0) bad = true;
+      cx<double> uj = mk<double>((r.x - a * uprev.x) / bet, (r.y - a * uprev.y) / bet);
+      ph[(long)j * nh + m] = uj;
+      uprev = uj;
+      cprev = c;
+    }
   }
   cx<double> unext = uprev;
-  for (int j = nz - 1; j >= 0; --j) {
-    double gm = gam[(long)(j + 1) * nh + m];
-    cx<double> uj = ph[(long)j * nh + m];
-    uj = mk<double>(uj.x - gm * unext.x, uj.y - gm * unext.y);
-    ph[(long)j * nh + m] = uj;
-    unext = uj;
+  for (int j0 = nz - 1; j0 >= 0; j0 -= kCh) {
+    double gq[kCh];
+    cx<double> uq[kCh];
+#pragma unroll
+    for (int q = 0; q < kCh; ++q) {
+      const int j = j0 - q >= 0 ? j0 - q : 0;
+      gq[q] = gam[(long)(j + 1) * nh + m];
+      uq[q] = ph[(long)j * nh + m];
+    }
+#pragma unroll
+    for (int q = 0; q < kCh; ++q) {
+      const int j = j0 - q;
+      if (j < 0) break;
+      cx<double> uj = mk<double>(uq[q].x - gq[q] * unext.x, uq[q].y - gq[q] * unext.y);
+      ph[(long)j * nh + m] = uj;
+      unext = uj;
+    }
   }
   if (bad) atomicExch(err, 1);
 }
 
 // tridag_array.f90:30-155 with explicit a, b, c, r (all binary64); modes skipped as there
-__global__ void k_tridag_arrays(const double *__restrict__ a, const double *__restrict__ b,
+static __global__ void k_tridag_arrays(const double *__restrict__ a, const double *__restrict__ b,
                                 const double *__restrict__ c, const cx<double> *__restrict__ r,
                                 cx<double> *__restrict__ u, double *__restrict__ gam, int lh, int ny, int n,
                                 int *__restrict__ err) {

@@ -8,8 +8,8 @@ STATE_FIELDS = ["u", "v", "w", "RHSx", "RHSy", "RHSz", "F_LM", "F_MM", "F_QN", "
                 "Cs_opt2", "u_lag", "v_lag", "w_lag"]
 
 
-def seeded_state(cfg, seed=ic.DEFAULT_SEED):
-    u, v, w = ic.turbulent_ic(cfg.grid, seed=seed, z0=cfg.z0, vonk=cfg.vonk)
+def seeded_state(cfg, seed=ic.DEFAULT_SEED, log_law=True):
+    u, v, w = ic.turbulent_ic(cfg.grid, seed=seed, z0=cfg.z0, vonk=cfg.vonk, log_law=log_law)
     return {"u": u, "v": v, "w": w}
 
 

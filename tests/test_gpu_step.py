@@ -74,10 +74,15 @@ def _ulp32_close(a, b, nx, max_frac=0.01, ulps=4):
 
 
 @pytest.mark.parametrize("model,casts", [(1, 1), (5, 0), (5, 1)])
-def test_sgs_stag_lagrangian(model, casts):
-    cfg = abi.make_config(32, 32, 17, model=model, fp32_casts=casts)
+@pytest.mark.parametrize("nx,nz", [(32, 17), (128, 9)])
+def test_sgs_stag_lagrangian(model, casts, nx, nz):
+    """At 128^2 the log-law mean flow makes L_ij = bar(u_i u_j) - bar(u_i) bar(u_j) cancel so
+    strongly that a 1e-15 relative change of u moves the oracle's own Cs^2 by 3e-12; the
+    larger grid therefore uses the turbulent field without the mean profile (its Cs^2
+    moves by 8e-14 for the same change)."""
+    cfg = abi.make_config(nx, nx, nz, model=model, fp32_casts=casts)
     o, lg = _pair(cfg)
-    st = seeded_state(cfg)
+    st = seeded_state(cfg, log_law=nx < 128)
     _derivs(o, lg, cfg, st)
     # jt = 1 (Cs^2 = 0.03 everywhere) then jt = 2 (Lagrangian model initialised), jt = 3
     lagstate = random_fields(cfg, ["u_lag", "v_lag", "w_lag"], seed=4)
@@ -103,6 +108,8 @@ def test_sgs_stag_lagrangian(model, casts):
                  levels=lev)
             if model == 5:
                 _cmp(o, lg, ["F_LM", "F_MM", "F_QN", "F_NN", "u_lag", "v_lag", "w_lag"])
+                if jt > 1:
+                    _cmp(o, lg, ["Beta"], levels=lev)
 
 
 def test_convec():
@@ -175,3 +182,17 @@ def test_timestep_conditions_and_rmsdiv(prec):
     assert abs(visc_g - visc_o) <= 1e-14 * abs(visc_o)
     r_o, r_g = o.rmsdiv(), lg.rmsdiv()
     assert abs(r_g - r_o) <= 1e-12 * r_o
+
+
+@pytest.mark.parametrize("nx", [32, 128])
+def test_interpolag_entry(nx):
+    """interpolag_Sdep.f90:62-369 on its own: F_* replaced by the interpolated averages,
+    lag velocities reset"""
+    cfg = abi.make_config(nx, nx, 9)
+    o, lg = _pair(cfg)
+    st = random_fields(cfg, ["u_lag", "v_lag", "w_lag"], seed=5)
+    for j, n in enumerate(("F_LM", "F_MM", "F_QN", "F_NN")):
+        st[n] = np.abs(random_fields(cfg, [n], seed=6 + j)[n]) + 0.1
+    load_oracle(o, st, 5); load_gpu(lg, st, 5)
+    o.call("interpolag_sdep"); lg.interpolag_sdep()
+    _cmp(o, lg, ["F_LM", "F_MM", "F_QN", "F_NN", "u_lag", "v_lag", "w_lag"])
