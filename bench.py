@@ -120,6 +120,19 @@ def cpu_baseline(nx, seed):
                       f"{threads} OpenMP threads"}
 
 
+def copy_probe(mib=512, reps=3):
+    """a plain device copy of known size (torch), the reference reading for the
+    FETCH_SIZE / WRITE_SIZE counters"""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,7 +146,11 @@ def main():
     ap.add_argument("--no-droplets", action="store_true", help="skip the 192x192x128 droplet workload")
     ap.add_argument("--breakdown", action="store_true", help="per-kernel-class timing to stderr")
     ap.add_argument("--roofline-class", default="col_op", choices=list(abi.KERNEL_CLASSES))
+    ap.add_argument("--copy-probe", action="store_true", help="only run a plain copy of known size")
     a = ap.parse_args()
+    if a.copy_probe:
+        copy_probe()
+        return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(the multi-GPU z-slab design is documented, not built)", file=sys.stderr)

@@ -104,11 +104,15 @@ struct lp_ctx {
   // scratch
   void *spec[4] = {};         // small spectra (lh x ny x nz) of the field type
   void *specb[2] = {};        // big spectra (lh_big x ny2 x nz)
+  // Internal spectra keep rows of sp (small) / sb (big) complex values: lh and lh_big
+  // rounded up to 16, so that every row starts on a 128-byte boundary.
+  long sp = 0, sb = 0;
+  void *phat = nullptr;       // binary64 p-hat (0..nz) in the sp layout
   void *big[9] = {};          // 3/2-grid physical fields (ld_big x ny2 x nz)
   void *tmp[4] = {};          // field-type volumes
   void *S[6] = {};            // strain rates
-  void *lspec = nullptr;      // x spectra of the 21 test-filter inputs [21][nz][ny][nc]
-  void *lbar = nullptr, *lhat = nullptr;  // their y-filtered spectra [21][nz][ny][ncut]
+  void *lspec = nullptr;      // x spectra of the 21 test-filter inputs [21][nz][ny][pad16(nc)]
+  void *lbar = nullptr, *lhat = nullptr;  // their y-filtered spectra [21][nz][ny][pad16(ncut)]
   void *lbarv = nullptr, *lhatv = nullptr;  // the 42 filtered fields as volumes [21][nz][ny][ld]
   double *Fi[4] = {};         // second buffer of the Lagrangian averages (exchanged with F)
   void *Hs[3] = {};           // pressure RHS spectra
@@ -167,6 +171,8 @@ static void prof_collect(lp_ctx *c) {
 }
 
 static size_t esz_of(Dt d) { return d == F64 ? 8 : 4; }
+// a spectral row length rounded up so that rows start on 128-byte boundaries
+static long pad16(long n) { return (n + 15) / 16 * 16; }
 static long plane_el(const lp_ctx *c) { return (long)c->g.ld * c->g.ny; }
 static long vol_el(const lp_ctx *c) { return plane_el(c) * c->g.nz; }
 static Geo geo(const lp_ctx *c) {
@@ -434,7 +440,7 @@ extern "C" int lp_destroy(lp_ctx *c) {
   fr(c->lspec); fr(c->lbar); fr(c->lhat); fr(c->lbarv); fr(c->lhatv);
   for (auto p : c->Fi) fr(p);
   for (auto p : c->Hs) fr(p);
-  fr(c->bw); fr(c->tw); fr(c->gam);
+  fr(c->bw); fr(c->tw); fr(c->gam); fr(c->phat);
   for (auto p : c->pspec) fr(p);
   fr(c->d_err); fr(c->d_part);
   fr(c->d_ws); fr(c->d_qsrc); fr(c->d_vfac); fr(c->d_foot); fr(c->d_bins); fr(c->d_pairs);
@@ -522,8 +528,10 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
   c->ncut2 = cut(G2);
   c->tab.lh = g.lh; c->tab.ny = g.ny;
   size_t fe = esz_of(c->fdt);
-  size_t spec_b = (size_t)g.lh * g.ny * g.nz * 2 * fe;  // == volume bytes
-  size_t specb_b = (size_t)g.lh_big * g.ny2 * g.nz * 2 * fe;
+  c->sp = (g.lh + 15) / 16 * 16;
+  c->sb = (g.lh_big + 15) / 16 * 16;
+  size_t spec_b = (size_t)c->sp * g.ny * g.nz * 2 * fe;
+  size_t specb_b = (size_t)c->sb * g.ny2 * g.nz * 2 * fe;
   size_t big_b = (size_t)g.ld_big * g.ny2 * g.nz * fe;
   size_t vol_b = (size_t)nv * fe;
   for (auto &p : c->spec) CK(dalloc(&p, spec_b));
@@ -534,18 +542,19 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
   if (cfg->model == 5) {
     const size_t pl = (size_t)kNumLag * g.ny * g.nz * 2 * fe;  // one kx column of all 21 fields
     const int nc = c->ncut1 > c->ncut2 ? c->ncut1 : c->ncut2;
-    CK(dalloc(&c->lspec, pl * nc));
-    CK(dalloc(&c->lbar, pl * c->ncut1));
-    CK(dalloc(&c->lhat, pl * c->ncut2));
+    CK(dalloc(&c->lspec, pl * pad16(nc)));
+    CK(dalloc(&c->lbar, pl * pad16(c->ncut1)));
+    CK(dalloc(&c->lhat, pl * pad16(c->ncut2)));
     CK(dalloc(&c->lbarv, (size_t)kNumLag * vol_b));
     CK(dalloc(&c->lhatv, (size_t)kNumLag * vol_b));
     for (auto &p : c->Fi) CK(dalloc((void **)&p, sizeof(double) * nv));
   }
   for (auto &p : c->Hs) CK(dalloc(&p, spec_b));
-  CK(dalloc(&c->bw, (size_t)g.lh * g.ny * 2 * fe));
-  CK(dalloc(&c->tw, (size_t)g.lh * g.ny * 2 * fe));
+  CK(dalloc(&c->bw, (size_t)c->sp * g.ny * 2 * fe));
+  CK(dalloc(&c->tw, (size_t)c->sp * g.ny * 2 * fe));
   CK(dalloc((void **)&c->gam, sizeof(double) * (size_t)g.lh * g.ny * (g.nz + 1)));
-  for (auto &p : c->pspec) CK(dalloc(&p, sizeof(double) * 2 * (size_t)g.lh * g.ny * (g.nz + 1)));
+  for (auto &p : c->pspec) CK(dalloc(&p, sizeof(double) * 2 * (size_t)c->sp * g.ny * (g.nz + 1)));
+  CK(dalloc(&c->phat, sizeof(double) * 2 * (size_t)c->sp * g.ny * (g.nz + 1)));
   CK(dalloc((void **)&c->d_err, sizeof(int) * 4));
   c->nred = 512;
   CK(dalloc((void **)&c->d_part, sizeof(double) * 3 * c->nred));
@@ -630,15 +639,15 @@ static int r2c_small(lp_ctx *c, const void *in, int nlev, double scale, void *sp
   const lp_grid &g = c->g;
   if (ncols_out <= 0) ncols_out = g.lh;
   prof_begin(c, LP_K_ROW_R2C);
-  HIPCHK(row_r2c(c->fdt, g.nx / 2, in, g.ld, plane_el(c), g.ny, nlev, scale, spec, g.lh,
-                 (long)g.lh * g.ny, ncols_out, c->s));
+  HIPCHK(row_r2c(c->fdt, g.nx / 2, in, g.ld, plane_el(c), g.ny, nlev, scale, spec, c->sp,
+                 c->sp * g.ny, ncols_out, c->s));
   prof_end(c, LP_K_ROW_R2C, bytes_r2c(g.nx, g.ny, nlev, ncols_out, esz_of(c->fdt)));
   return LP_OK;
 }
 static int c2r_small(lp_ctx *c, const void *spec, int ncols, int nlev, void *out, Dt dto) {
   const lp_grid &g = c->g;
   prof_begin(c, LP_K_ROW_C2R);
-  HIPCHK(row_c2r(c->fdt, dto, g.nx / 2, spec, g.lh, (long)g.lh * g.ny, ncols, g.ny, nlev, out, g.ld,
+  HIPCHK(row_c2r(c->fdt, dto, g.nx / 2, spec, c->sp, c->sp * g.ny, ncols, g.ny, nlev, out, g.ld,
                  plane_el(c), c->s));
   prof_end(c, LP_K_ROW_C2R, bytes_c2r(g.nx, g.ny, nlev, ncols, esz_of(c->fdt), esz_of(dto)));
   return LP_OK;
@@ -646,10 +655,10 @@ static int c2r_small(lp_ctx *c, const void *spec, int ncols, int nlev, void *out
 // column map on small spectra: in -> nout outputs (same grid)
 static int col_small(lp_ctx *c, const void *in, int nlev, int fwd, int inv, int nout, ColOut *o, int ncols = 0) {
   const lp_grid &g = c->g;
-  long ps = (long)g.lh * g.ny;
+  const long ps = c->sp * g.ny;
   if (ncols <= 0) ncols = g.lh;
   prof_begin(c, LP_K_COL_OP);
-  HIPCHK(col_op(c->fdt, g.ny, g.ny, 0, fwd, inv, in, g.lh, ps, ncols, nlev, nout, o, g.lh, ps, c->tab, c->s));
+  HIPCHK(col_op(c->fdt, g.ny, g.ny, 0, fwd, inv, in, c->sp, ps, ncols, nlev, nout, o, c->sp, ps, c->tab, c->s));
   prof_end(c, LP_K_COL_OP, bytes_col(g.ny, g.ny, ncols, nlev, nout, esz_of(c->fdt)));
   return LP_OK;
 }
@@ -671,10 +680,10 @@ static int to_big(lp_ctx *c, const void *f, int nlev, void *out_big) {
   double cst = 1.0 / ((double)g.nx * g.ny);
   RCHK(r2c_small(c, f, nlev, cst, c->spec[0]));
   ColOut o{c->specb[0], M_ONE};
-  long ps = (long)g.lh * g.ny, pb = (long)g.lh_big * g.ny2;
-  HIPCHK(col_op(c->fdt, g.ny, g.ny2, 1, 1, 1, c->spec[0], g.lh, ps, g.nx / 2, nlev, 1, &o, g.lh_big, pb,
+  const long ps = c->sp * g.ny, pb = c->sb * g.ny2;
+  HIPCHK(col_op(c->fdt, g.ny, g.ny2, 1, 1, 1, c->spec[0], c->sp, ps, g.nx / 2, nlev, 1, &o, c->sb, pb,
                 c->tab, c->s));
-  HIPCHK(row_c2r(c->fdt, c->fdt, g.nx2 / 2, c->specb[0], g.lh_big, pb, g.nx / 2, g.ny2, nlev, out_big,
+  HIPCHK(row_c2r(c->fdt, c->fdt, g.nx2 / 2, c->specb[0], c->sb, pb, g.nx / 2, g.ny2, nlev, out_big,
                  g.ld_big, (long)g.ld_big * g.ny2, c->s));
   return LP_OK;
 }
@@ -682,11 +691,11 @@ static int to_big(lp_ctx *c, const void *f, int nlev, void *out_big) {
 // dealias2.f90 core / convec.f90:161-170: 3/2-grid field (already scaled) -> small field
 static int from_big(lp_ctx *c, const void *fb, int nlev, double scale, void *out) {
   const lp_grid &g = c->g;
-  long pb = (long)g.lh_big * g.ny2, ps = (long)g.lh * g.ny;
+  const long pb = c->sb * g.ny2, ps = c->sp * g.ny;
   HIPCHK(row_r2c(c->fdt, g.nx2 / 2, fb, g.ld_big, (long)g.ld_big * g.ny2, g.ny2, nlev, scale, c->specb[1],
-                 g.lh_big, pb, g.nx / 2, c->s));
+                 c->sb, pb, g.nx / 2, c->s));
   ColOut o{c->spec[1], M_ONE};
-  HIPCHK(col_op(c->fdt, g.ny2, g.ny, 2, 1, 1, c->specb[1], g.lh_big, pb, g.nx / 2, nlev, 1, &o, g.lh, ps,
+  HIPCHK(col_op(c->fdt, g.ny2, g.ny, 2, 1, 1, c->specb[1], c->sb, pb, g.nx / 2, nlev, 1, &o, c->sp, ps,
                 c->tab, c->s));
   RCHK(c2r_small(c, c->spec[1], g.nx / 2, nlev, out, c->fdt));
   return LP_OK;
@@ -788,11 +797,12 @@ static int do_interpolag(lp_ctx *c) {
   return reset_lag_velocities(c);
 }
 
-// x back-transform of the 21 filtered spectra [21][nz][ny][ncols] into 21 volumes
-static int c2r_batch(lp_ctx *c, const void *spec, int ncols, void *out) {
+// x back-transform of the 21 filtered spectra [21][nz][ny][pitch] (ncols columns used)
+// into 21 volumes
+static int c2r_batch(lp_ctx *c, const void *spec, int ncols, long pitch, void *out) {
   const lp_grid &g = c->g;
   prof_begin(c, LP_K_ROW_C2R);
-  HIPCHK(row_c2r(c->fdt, c->fdt, g.nx / 2, spec, ncols, (long)ncols * g.ny, ncols, g.ny, kNumLag * g.nz, out, g.ld,
+  HIPCHK(row_c2r(c->fdt, c->fdt, g.nx / 2, spec, pitch, pitch * g.ny, ncols, g.ny, kNumLag * g.nz, out, g.ld,
                  plane_el(c), c->s));
   prof_end(c, LP_K_ROW_C2R, bytes_c2r(g.nx, g.ny, kNumLag * g.nz, ncols, esz_of(c->fdt), esz_of(c->fdt)));
   return LP_OK;
@@ -813,14 +823,15 @@ static int do_lagrange(lp_ctx *c) {
   for (int q = 0; q < 6; ++q) in.S[q] = c->S[q];
   in.magS = c->tmp[3];
   prof_begin(c, LP_K_ROW_R2C);
-  HIPCHK(lag_r2c(c->fdt, in, g, c->lspec, nc, c->s));
+  const long pc = pad16(nc), p1 = pad16(n1), p2 = pad16(n2);
+  HIPCHK(lag_r2c(c->fdt, in, g, c->lspec, nc, pc, c->s));
   prof_end(c, LP_K_ROW_R2C, vol * 10.0 * fe + (double)kNumLag * gg.ny * gg.nz * nc * 2 * fe);
   // :157-256 the y part of both test filters, all 21 fields in one pass
   {
-    const long ps = (long)nc * gg.ny;
-    ColOut o[2] = {{c->lbar, M_G1, n1, (long)n1 * gg.ny, n1}, {c->lhat, M_G2, n2, (long)n2 * gg.ny, n2}};
+    const long ps = pc * gg.ny;
+    ColOut o[2] = {{c->lbar, M_G1, p1, p1 * gg.ny, n1}, {c->lhat, M_G2, p2, p2 * gg.ny, n2}};
     prof_begin(c, LP_K_COL_OP);
-    HIPCHK(col_op(c->fdt, gg.ny, gg.ny, 0, 1, 1, c->lspec, nc, ps, nc, kNumLag * gg.nz, 2, o, nc, ps, c->tab,
+    HIPCHK(col_op(c->fdt, gg.ny, gg.ny, 0, 1, 1, c->lspec, pc, ps, nc, kNumLag * gg.nz, 2, o, pc, ps, c->tab,
                   c->s));
     prof_end(c, LP_K_COL_OP, (double)kNumLag * gg.nz * gg.ny * (nc + n1 + n2) * 2 * fe);
   }
@@ -842,8 +853,8 @@ static int do_lagrange(lp_ctx *c) {
                            (const double *)c->f[LP_F_QN], (const double *)c->f[LP_F_NN]};
   double *Fnew[4] = {c->Fi[0], c->Fi[1], c->Fi[2], c->Fi[3]};
   // back along x: the 42 filtered fields as volumes
-  RCHK(c2r_batch(c, c->lbar, n1, c->lbarv));
-  RCHK(c2r_batch(c, c->lhat, n2, c->lhatv));
+  RCHK(c2r_batch(c, c->lbar, n1, p1, c->lbarv));
+  RCHK(c2r_batch(c, c->lhat, n2, p2, c->lhatv));
   // :257-434 (+ interpolag_Sdep.f90:62-365): interpolation, contractions, relaxation
   prof_begin(c, LP_K_LAG_RELAX);
   HIPCHK(lag_relax(c->fdt, c->lbarv, c->lhatv, Fold, Fnew, c->f[LP_U_LAG], c->f[LP_V_LAG], c->f[LP_W_LAG],
@@ -961,7 +972,8 @@ static int do_press(lp_ctx *c) {
   const lp_grid &gg = c->g;
   const lp_config &cf = c->cfg;
   Geo g = geo(c);
-  long ps = (long)gg.lh * gg.ny, np = g.np;
+  const long nmodes = (long)gg.lh * gg.ny, np = g.np;
+  const long ps = c->sp * gg.ny;  // complex elements per spectral plane
   const double tadv1 = 1.5;
   double cst = 1.0 / ((double)gg.nx * gg.ny);
   // press_stag_array.f90:75-87,126-131: H = FFT(u*/(1.5 dt))/(nx ny), oddballs zeroed
@@ -986,31 +998,33 @@ static int do_press(lp_ctx *c) {
   }
   // :149-329 + tridag_array
   HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(int), c->s));
-  cx<double> *ph = (cx<double> *)c->f[LP_P];
+  cx<double> *ph = (cx<double> *)c->phat;
   prof_begin(c, LP_K_PRESS_TRIDIAG);
   if (c->fdt == F64)
-    hipLaunchKernelGGL(k_press_tridiag<double>, grid1(ps), dim3(256), 0, c->s, (const cx<double> *)c->Hs[0],
+    hipLaunchKernelGGL(k_press_tridiag<double>, grid1(nmodes), dim3(256), 0, c->s, (const cx<double> *)c->Hs[0],
                        (const cx<double> *)c->Hs[1], (const cx<double> *)c->Hs[2], (const cx<double> *)c->bw,
-                       (const cx<double> *)c->tw, c->kx, c->ky, ph, c->gam, gg.lh, gg.ny, gg.nz, gg.dz, c->d_err);
+                       (const cx<double> *)c->tw, c->kx, c->ky, ph, c->gam, gg.lh, gg.ny, gg.nz, c->sp, gg.dz,
+                       c->d_err);
   else
-    hipLaunchKernelGGL(k_press_tridiag<float>, grid1(ps), dim3(256), 0, c->s, (const cx<float> *)c->Hs[0],
+    hipLaunchKernelGGL(k_press_tridiag<float>, grid1(nmodes), dim3(256), 0, c->s, (const cx<float> *)c->Hs[0],
                        (const cx<float> *)c->Hs[1], (const cx<float> *)c->Hs[2], (const cx<float> *)c->bw,
-                       (const cx<float> *)c->tw, c->kx, c->ky, ph, c->gam, gg.lh, gg.ny, gg.nz, gg.dz, c->d_err);
+                       (const cx<float> *)c->tw, c->kx, c->ky, ph, c->gam, gg.lh, gg.ny, gg.nz, c->sp, gg.dz,
+                       c->d_err);
   HIPCHK(hipGetLastError());
   // H_x, H_y, H_z read once, p_hat written once (binary64 complex)
-  prof_end(c, LP_K_PRESS_TRIDIAG, (double)ps * (gg.nz - 1) * 3 * 2 * esz_of(c->fdt) + (double)ps * (gg.nz + 1) * 16);
+  prof_end(c, LP_K_PRESS_TRIDIAG,
+           (double)nmodes * (gg.nz - 1) * 3 * 2 * esz_of(c->fdt) + (double)nmodes * (gg.nz + 1) * 16);
   // :339-352 inverse transforms in binary64: p(0..nz-1), dpdx, dpdy (1..nz-1)
-  long psd = ps;  // complex elements per plane
   {
     ColOut o[3] = {{c->pspec[0], M_ONE}, {c->pspec[1], M_IKX}, {c->pspec[2], M_IKY}};
-    HIPCHK(col_op(F64, gg.ny, gg.ny, 0, 0, 1, ph, gg.lh, psd, gg.lh, gg.nz, 3, o, gg.lh, psd, c->tab, c->s));
+    HIPCHK(col_op(F64, gg.ny, gg.ny, 0, 0, 1, ph, c->sp, ps, gg.lh, gg.nz, 3, o, c->sp, ps, c->tab, c->s));
   }
-  HIPCHK(row_c2r(F64, F64, gg.nx / 2, c->pspec[0], gg.lh, psd, gg.lh, gg.ny, gg.nz, c->f[LP_P], gg.ld, np,
+  HIPCHK(row_c2r(F64, F64, gg.nx / 2, c->pspec[0], c->sp, ps, gg.lh, gg.ny, gg.nz, c->f[LP_P], gg.ld, np,
                  c->s));
   Dt dto = c->fdt;
-  HIPCHK(row_c2r(F64, dto, gg.nx / 2, (const cx<double> *)c->pspec[1] + psd, gg.lh, psd, gg.lh, gg.ny,
+  HIPCHK(row_c2r(F64, dto, gg.nx / 2, (const cx<double> *)c->pspec[1] + ps, c->sp, ps, gg.lh, gg.ny,
                  gg.nz - 1, c->f[LP_DPDX], gg.ld, np, c->s));
-  HIPCHK(row_c2r(F64, dto, gg.nx / 2, (const cx<double> *)c->pspec[2] + psd, gg.lh, psd, gg.lh, gg.ny,
+  HIPCHK(row_c2r(F64, dto, gg.nx / 2, (const cx<double> *)c->pspec[2] + ps, c->sp, ps, gg.lh, gg.ny,
                  gg.nz - 1, c->f[LP_DPDY], gg.ld, np, c->s));
   // :355-357
   hipLaunchKernelGGL(k_fill<double>, grid1(np), dim3(256), 0, c->s, (double *)c->f[LP_P] + (long)gg.nz * np, np,
@@ -1565,8 +1579,8 @@ extern "C" int lp_test_filter(lp_ctx *c, void *f, const double *G, int32_t nplan
   t.G1 = (const double *)gg.dev;
   RCHK(r2c_small(c, a.dev, nplanes, 1.0, c->spec[0]));
   ColOut o{c->spec[0], M_G1};
-  long ps = (long)g.lh * g.ny;
-  HIPCHK(col_op(c->fdt, g.ny, g.ny, 0, 1, 1, c->spec[0], g.lh, ps, g.lh, nplanes, 1, &o, g.lh, ps, t, c->s));
+  const long ps = c->sp * g.ny;
+  HIPCHK(col_op(c->fdt, g.ny, g.ny, 0, 1, 1, c->spec[0], c->sp, ps, g.lh, nplanes, 1, &o, c->sp, ps, t, c->s));
   RCHK(c2r_small(c, c->spec[0], g.lh, nplanes, a.dev, c->fdt));
   return finish_all(c, {&a, &gg});
 }

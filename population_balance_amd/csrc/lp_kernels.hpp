@@ -502,19 +502,21 @@ __global__ void k_press_tridiag(const cx<TH> *__restrict__ Hx, const cx<TH> *__r
                                 const cx<TH> *__restrict__ Hz, const cx<TH> *__restrict__ bw,
                                 const cx<TH> *__restrict__ tw_, const double *__restrict__ kxt,
                                 const double *__restrict__ kyt, cx<double> *__restrict__ ph,
-                                double *__restrict__ gam, int lh, int ny, int nz, double dz,
+                                double *__restrict__ gam, int lh, int ny, int nz, long pitch, double dz,
                                 int *__restrict__ err) {
-  long m = gtid();
-  long nh = (long)lh * ny;
+  const long m = gtid();  // mode (kx, ky) = (m % lh, m / lh)
+  const long nh = (long)lh * ny;
   if (m >= nh) return;
-  int jx = (int)(m % lh), jy = (int)(m / lh);
+  const int jx = (int)(m % lh), jy = (int)(m / lh);
+  const long o = (long)jy * pitch + jx;  // offset of the mode in a spectral plane
+  const long ps = pitch * ny;            // spectral plane (H, bw, tw, ph)
   auto ld_h = [&](const cx<TH> *H, int level) -> cx<double> {  // H(level), level 1..nz
     if (level >= nz) return mk<double>(0.0, 0.0);
-    cx<TH> v = H[(long)(level - 1) * nh + m];
+    cx<TH> v = H[(long)(level - 1) * ps + o];
     return mk<double>((double)v.x, (double)v.y);
   };
   if (jy == ny / 2 || jx == lh - 1) {
-    for (int j = 0; j <= nz; ++j) ph[(long)j * nh + m] = mk<double>(0.0, 0.0);
+    for (int j = 0; j <= nz; ++j) ph[(long)j * ps + o] = mk<double>(0.0, 0.0);
     return;
   }
   if (m == 0) {  // press_stag_array.f90:303-312
@@ -522,11 +524,11 @@ __global__ void k_press_tridiag(const cx<TH> *__restrict__ Hx, const cx<TH> *__r
     ph[0] = acc;
     cx<TH> b0 = bw[0];
     acc = mk<double>(acc.x - dz * (double)b0.x, acc.y - dz * (double)b0.y);
-    ph[nh] = acc;
+    ph[ps] = acc;
     for (int jz = 2; jz <= nz; ++jz) {
       cx<double> hz = ld_h(Hz, jz);
       acc = mk<double>(acc.x + hz.x * dz, acc.y + hz.y * dz);
-      ph[(long)jz * nh] = acc;
+      ph[(long)jz * ps] = acc;
     }
     return;
   }
@@ -535,10 +537,10 @@ __global__ void k_press_tridiag(const cx<TH> *__restrict__ Hx, const cx<TH> *__r
   const double bi = -(kx * kx + ky * ky + 2.0 / (dz * dz));
   // row 0 (wall): b = -1, c = 1, r = -dz * bottomw
   double bet = -1.0;
-  cx<TH> b0 = bw[m];
+  cx<TH> b0 = bw[o];
   cx<double> r0 = mk<double>(-dz * (double)b0.x, -dz * (double)b0.y);
   cx<double> uprev = mk<double>(r0.x / bet, r0.y / bet);
-  ph[m] = uprev;
+  ph[o] = uprev;
   double cprev = 1.0;
   bool bad = false;
   // The right-hand sides do not depend on the recurrence: they are loaded kCh levels at a
@@ -553,9 +555,9 @@ __global__ void k_press_tridiag(const cx<TH> *__restrict__ Hx, const cx<TH> *__r
     for (int q = 0; q < kCh; ++q) {
       const int j = j0 + q;
       const int lx = j < nz ? j : nz - 1, lz = j + 1 < nz ? j + 1 : nz - 1;
-      rx[q] = Hx[(long)(lx - 1) * nh + m];
-      ry[q] = Hy[(long)(lx - 1) * nh + m];
-      rz[q] = Hz[(long)(lz - 1) * nh + m];
+      rx[q] = Hx[(long)(lx - 1) * ps + o];
+      ry[q] = Hy[(long)(lx - 1) * ps + o];
+      rz[q] = Hz[(long)(lz - 1) * ps + o];
     }
     cx<double> hx[kCh], hy[kCh], hz[kCh];
 #pragma unroll
@@ -580,7 +582,7 @@ __global__ void k_press_tridiag(const cx<TH> *__restrict__ Hx, const cx<TH> *__r
         r = mk<double>(-sy + (hz1.x - hz0.x) / dz, sx + (hz1.y - hz0.y) / dz);
         hz_lo = hz1;
       } else {
-        cx<TH> t = tw_[m];
+        cx<TH> t = tw_[o];
         a = -1.0; b = 1.0; c = 0.0;
         r = mk<double>(-(double)t.x * dz, -(double)t.y * dz);
       }
@@ -589,7 +591,7 @@ __global__ void k_press_tridiag(const cx<TH> *__restrict__ Hx, const cx<TH> *__r
       bet = b - a * gm;
       if (bet == 0.0) bad = true;
       cx<double> uj = mk<double>((r.x - a * uprev.x) / bet, (r.y - a * uprev.y) / bet);
-      ph[(long)j * nh + m] = uj;
+      ph[(long)j * ps + o] = uj;
       uprev = uj;
       cprev = c;
     }
@@ -602,14 +604,14 @@ __global__ void k_press_tridiag(const cx<TH> *__restrict__ Hx, const cx<TH> *__r
     for (int q = 0; q < kCh; ++q) {
       const int j = j0 - q >= 0 ? j0 - q : 0;
       gq[q] = gam[(long)(j + 1) * nh + m];
-      uq[q] = ph[(long)j * nh + m];
+      uq[q] = ph[(long)j * ps + o];
     }
 #pragma unroll
     for (int q = 0; q < kCh; ++q) {
       const int j = j0 - q;
       if (j < 0) break;
       cx<double> uj = mk<double>(uq[q].x - gq[q] * unext.x, uq[q].y - gq[q] * unext.y);
-      ph[(long)j * nh + m] = uj;
+      ph[(long)j * ps + o] = uj;
       unext = uj;
     }
   }

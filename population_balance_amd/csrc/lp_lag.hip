@@ -66,7 +66,7 @@ __device__ __forceinline__ typename V2<T>::t lag_in_pair(int q, const LagIn<T> &
 // One wavefront transforms one field for 64/P rows; the four wavefronts of a block take
 // four fields at a time, six rounds cover the 21.
 template <typename T, int M>
-__global__ __launch_bounds__(256) void k_lag_r2c(LagIn<T> in, Geo g, cx<T> *__restrict__ spec, int nc,
+__global__ __launch_bounds__(256) void k_lag_r2c(LagIn<T> in, Geo g, cx<T> *__restrict__ spec, int nc, long pitch,
                                                  const cx<T> *__restrict__ twM, const cx<T> *__restrict__ twN) {
   constexpr int P = row_lanes(M);
   using F = Fft4<T, M, P, -1>;
@@ -107,7 +107,7 @@ __global__ __launch_bounds__(256) void k_lag_r2c(LagIn<T> in, Geo g, cx<T> *__re
       }
     slot_sync<P>();
     if (act) {  // split (as k_row_r2c), leading nc columns
-      cx<T> *dst = spec + (((long)q * g.nz + k) * g.ny + y) * nc;
+      cx<T> *dst = spec + (((long)q * g.nz + k) * g.ny + y) * pitch;
       for (int kk = t; kk < nc; kk += P) {
         cx<T> za = sc[kk % M], zb = conjg(sc[(M - kk) % M]);
         cx<T> e = za + zb, o = za - zb;
@@ -158,9 +158,10 @@ __global__ __launch_bounds__(256) void k_lag_relax(const T *__restrict__ bar, co
 #define LP_SMALL_ROWS(X) X(16) X(32) X(48) X(64) X(96) X(128)
 
 template <typename T, int M>
-static hipError_t launch_lag_r2c(const LagInputs &in, Geo g, void *spec, int nc, hipStream_t s, Dt dt) {
+static hipError_t launch_lag_r2c(const LagInputs &in, Geo g, void *spec, int nc, long pitch, hipStream_t s,
+                                 Dt dt) {
   const void *twM = tw_ptr(M, dt), *twN = tw_ptr(2 * M, dt);
-  if (!twM || !twN || nc < 1 || nc > M + 1) return hipErrorInvalidValue;
+  if (!twM || !twN || nc < 1 || nc > M + 1 || pitch < nc) return hipErrorInvalidValue;
   LagIn<T> a;
   a.u = (const T *)in.u; a.v = (const T *)in.v; a.w = (const T *)in.w;
   for (int q = 0; q < 6; ++q) a.S[q] = (const T *)in.S[q];
@@ -168,15 +169,15 @@ static hipError_t launch_lag_r2c(const LagInputs &in, Geo g, void *spec, int nc,
   constexpr int RB = 64 / row_lanes(M);
   const long rows = (long)g.ny * g.nz;
   hipLaunchKernelGGL((k_lag_r2c<T, M>), dim3((unsigned)((rows + RB - 1) / RB)), dim3(256), 0, s, a, g,
-                     (cx<T> *)spec, nc, (const cx<T> *)twM, (const cx<T> *)twN);
+                     (cx<T> *)spec, nc, pitch, (const cx<T> *)twM, (const cx<T> *)twN);
   return hipGetLastError();
 }
 
-hipError_t lag_r2c(Dt dt, const LagInputs &in, Geo g, void *spec, int nc, hipStream_t s) {
+hipError_t lag_r2c(Dt dt, const LagInputs &in, Geo g, void *spec, int nc, long pitch, hipStream_t s) {
 #define CASE(m)                                                                          \
   case m:                                                                                \
-    return dt == F64 ? launch_lag_r2c<double, m>(in, g, spec, nc, s, dt)                 \
-                     : launch_lag_r2c<float, m>(in, g, spec, nc, s, dt);
+    return dt == F64 ? launch_lag_r2c<double, m>(in, g, spec, nc, pitch, s, dt)          \
+                     : launch_lag_r2c<float, m>(in, g, spec, nc, pitch, s, dt);
   switch (g.nx / 2) {
     LP_SMALL_ROWS(CASE)
     default: return hipErrorInvalidValue;

@@ -22,8 +22,8 @@ struct LagInterp {
   double cs, dt, dx, dy, dz;
 };
 
-// spec: [21][nz][ny][nc] complex of the field type
-hipError_t lag_r2c(Dt dt, const LagInputs &in, Geo g, void *spec, int nc, hipStream_t s);
+// spec: [21][nz][ny][pitch] complex of the field type, the leading nc columns written
+hipError_t lag_r2c(Dt dt, const LagInputs &in, Geo g, void *spec, int nc, long pitch, hipStream_t s);
 
 // bar, hat: the 21 filtered fields as volumes (field q at q*ld*ny*nz); Fold are read
 // (gathered), Fnew written
