@@ -213,25 +213,35 @@ __global__ __launch_bounds__(col_tx<T>() * kColLanes) void k_col_op(
     const int tp = Pr.outs[jo].tpitch;
     const double *mrow = mt + (long)t * tp + (cv ? kx : 0);
     cx<T> v[QO];
+    if constexpr (PM == 0) {
+      // the multipliers of all q are loaded together (block-uniform branch)
+      T fq[QO];
+      if (mult != M_ONE) {
 #pragma unroll
-    for (int q = 0; q < QO; ++q) {
-      const int yo = t + PL * q;
-      cx<T> x = mk<T>(0, 0);
-      if constexpr (PM == 0) {
-        x = sp[q];
-        if (mult != M_ONE) {  // block-uniform branch; the loads below are unconditional
-          const T f = (T)mrow[(long)q * PL * tp];
-          x = mk<T>((imag ? -x.y : x.x) * f, (imag ? x.x : x.y) * f);
-        }
-        if (!keep || yo == ny_s / 2) x = mk<T>(0, 0);
-      } else if constexpr (PM == 1) {  // padd.f90: ny -> ny2, middle rows zero
-        if (q < QI && yo < NYI / 2) x = sp[q < QI ? q : 0];
-        else if (q >= DQ && yo >= NYO - NYI / 2 + 1) x = sp[q >= DQ ? q - DQ : 0];
-      } else {  // unpadd.f90: ny2 -> ny, Nyquist row zero
-        if (yo < NYO / 2) x = sp[q];
-        else if (yo > NYO / 2) x = sp[q + DQ];
+        for (int q = 0; q < QO; ++q) fq[q] = (T)mrow[(long)q * PL * tp];
       }
-      v[q] = x;
+#pragma unroll
+      for (int q = 0; q < QO; ++q) {
+        const int yo = t + PL * q;
+        cx<T> x = sp[q];
+        if (mult != M_ONE) x = mk<T>((imag ? -x.y : x.x) * fq[q], (imag ? x.x : x.y) * fq[q]);
+        v[q] = (!keep || yo == ny_s / 2) ? mk<T>(0, 0) : x;
+      }
+    }
+    if constexpr (PM != 0) {
+#pragma unroll
+      for (int q = 0; q < QO; ++q) {
+        const int yo = t + PL * q;
+        cx<T> x = mk<T>(0, 0);
+        if constexpr (PM == 1) {  // padd.f90: ny -> ny2, middle rows zero
+          if (q < QI && yo < NYI / 2) x = sp[q < QI ? q : 0];
+          else if (q >= DQ && yo >= NYO - NYI / 2 + 1) x = sp[q >= DQ ? q - DQ : 0];
+        } else {  // unpadd.f90: ny2 -> ny, Nyquist row zero
+          if (yo < NYO / 2) x = sp[q];
+          else if (yo > NYO / 2) x = sp[q + DQ];
+        }
+        v[q] = x;
+      }
     }
     // addresses are formed from per-output values so that they are not hoisted out of
     // the output loop (which would keep 16 64-bit addresses live)

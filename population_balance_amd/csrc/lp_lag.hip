@@ -26,40 +26,65 @@ struct F4 {
   double *p[4];
 };
 
-// lagrange_Sdep.f90:129-155,192-242: values of input field q at the two points i, i+1
-// (i even).  Order: 0-2 u, v, w on w-nodes, 3-8 the products 11,12,13,22,23,33,
-// 9-14 S_ij, 15-20 |S| S_ij.  q is uniform over the wavefront.
-template <typename T>
-__device__ __forceinline__ typename V2<T>::t lag_in_pair(int q, const LagIn<T> &in, const Geo &g, long i, int k) {
+// lagrange_Sdep.f90:129-155,192-242: the 21 filter inputs.  Order: 0-2 u, v, w on
+// w-nodes, 3-8 the products 11,12,13,22,23,33, 9-14 S_ij, 15-20 |S| S_ij.  Each helper
+// returns the values at the two points i, i+1 (i even) of level k.
+template <typename T> struct LagPair {
   using v2 = typename V2<T>::t;
-  auto ld2 = [](const T *f, long o) { return *reinterpret_cast<const v2 *>(f + o); };
-  auto hv = [&](const T *f) -> v2 {  // u, v averaged to the w-node (jz = 1 stays)
-    v2 a = ld2(f, i);
-    if (k == 0) return a;
-    v2 b = ld2(f, i - g.np);
-    return v2{T(0.5) * (a.x + b.x), T(0.5) * (a.y + b.y)};
-  };
-  auto wn = [&]() -> v2 {  // w; at jz = 1 a quarter of w(2)
-    if (k == 0) {
-      v2 a = ld2(in.w, i + g.np);
-      return v2{T(0.25) * a.x, T(0.25) * a.y};
-    }
-    return ld2(in.w, i);
-  };
-  auto mul = [](v2 a, v2 b) { return v2{a.x * b.x, a.y * b.y}; };
-  switch (q) {
-    case 0: return hv(in.u);
-    case 1: return hv(in.v);
-    case 2: return wn();
-    case 3: { v2 a = hv(in.u); return mul(a, a); }
-    case 4: return mul(hv(in.u), hv(in.v));
-    case 5: return mul(hv(in.u), wn());
-    case 6: { v2 a = hv(in.v); return mul(a, a); }
-    case 7: return mul(hv(in.v), wn());
-    case 8: { v2 a = wn(); return mul(a, a); }
+  const LagIn<T> &in;
+  const long np;
+  const int k;
+  __device__ __forceinline__ static v2 ld2(const T *f, long o) { return *reinterpret_cast<const v2 *>(f + o); }
+  __device__ __forceinline__ static v2 mul(v2 a, v2 b) { return v2{a.x * b.x, a.y * b.y}; }
+  // branch-free (loads from clamped offsets, then a select) so that the loads of a row
+  // are issued together
+  __device__ __forceinline__ v2 hv(const T *f, long i) const {  // u, v on the w-node (jz = 1 stays)
+    const v2 a = ld2(f, i);
+    const v2 b = ld2(f, i - (k > 0 ? np : 0));
+    const v2 h{T(0.5) * (a.x + b.x), T(0.5) * (a.y + b.y)};
+    return k == 0 ? a : h;
+  }
+  __device__ __forceinline__ v2 wn(long i) const {  // w; at jz = 1 a quarter of w(2)
+    const v2 a = ld2(in.w, i + (k == 0 ? np : 0));
+    const v2 q{T(0.25) * a.x, T(0.25) * a.y};
+    return k == 0 ? q : a;
+  }
+};
+
+// z[m] = field values at points 2(t + P m), 2(t + P m) + 1 of the row starting at base;
+// the loads of all m are independent, so they are issued together
+template <typename T, int Q, int P, typename Fn>
+__device__ __forceinline__ void row_pairs(Fn f, cx<T> (&z)[Q], long base, int t) {
+#pragma unroll
+  for (int m = 0; m < Q; ++m) {
+    const auto a = f(base + 2 * (t + P * m));
+    z[m] = mk<T>(a.x, a.y);
+  }
+}
+
+template <typename T, int Q, int P>
+__device__ __forceinline__ void lag_row(int q, const LagIn<T> &in, const Geo &g, int k, long base, int t,
+                                        cx<T> (&z)[Q]) {
+  const LagPair<T> L{in, g.np, k};
+  using LP = LagPair<T>;
+  switch (q) {  // q is uniform over the wavefront
+    case 0: row_pairs<T, Q, P>([&](long i) { return L.hv(in.u, i); }, z, base, t); break;
+    case 1: row_pairs<T, Q, P>([&](long i) { return L.hv(in.v, i); }, z, base, t); break;
+    case 2: row_pairs<T, Q, P>([&](long i) { return L.wn(i); }, z, base, t); break;
+    case 3: row_pairs<T, Q, P>([&](long i) { auto a = L.hv(in.u, i); return LP::mul(a, a); }, z, base, t); break;
+    case 4: row_pairs<T, Q, P>([&](long i) { return LP::mul(L.hv(in.u, i), L.hv(in.v, i)); }, z, base, t); break;
+    case 5: row_pairs<T, Q, P>([&](long i) { return LP::mul(L.hv(in.u, i), L.wn(i)); }, z, base, t); break;
+    case 6: row_pairs<T, Q, P>([&](long i) { auto a = L.hv(in.v, i); return LP::mul(a, a); }, z, base, t); break;
+    case 7: row_pairs<T, Q, P>([&](long i) { return LP::mul(L.hv(in.v, i), L.wn(i)); }, z, base, t); break;
+    case 8: row_pairs<T, Q, P>([&](long i) { auto a = L.wn(i); return LP::mul(a, a); }, z, base, t); break;
     default:
-      if (q < 15) return ld2(in.S[q - 9], i);
-      return mul(ld2(in.magS, i), ld2(in.S[q - 15], i));
+      if (q < 15) {
+        const T *S = in.S[q - 9];
+        row_pairs<T, Q, P>([&](long i) { return LP::ld2(S, i); }, z, base, t);
+      } else {
+        const T *S = in.S[q - 15];
+        row_pairs<T, Q, P>([&](long i) { return LP::mul(LP::ld2(in.magS, i), LP::ld2(S, i)); }, z, base, t);
+      }
   }
 }
 
@@ -86,11 +111,11 @@ __global__ __launch_bounds__(256) void k_lag_r2c(LagIn<T> in, Geo g, cx<T> *__re
     const int q = __builtin_amdgcn_readfirstlane(4 * it + wv);
     const bool act = valid && q < 21;
     cx<T> z[Q];
+    if (act) {
+      lag_row<T, Q, P>(q, in, g, k, base, t, z);
+    } else {
 #pragma unroll
-    for (int m = 0; m < Q; ++m) {
-      typename V2<T>::t a{T(0), T(0)};
-      if (act) a = lag_in_pair<T>(q, in, g, base + 2 * (t + P * m), k);
-      z[m] = mk<T>(a.x, a.y);
+      for (int m = 0; m < Q; ++m) z[m] = mk<T>(0, 0);
     }
     F::stage1(z, t, sc, twM);
     slot_sync<P>();
