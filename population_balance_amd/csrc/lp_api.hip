@@ -113,7 +113,7 @@ struct lp_ctx {
   void *S[6] = {};            // strain rates
   void *lspec = nullptr;      // x spectra of the 21 test-filter inputs [21][nz][ny][pad16(nc)]
   void *lbar = nullptr, *lhat = nullptr;  // their y-filtered spectra [21][nz][ny][pad16(ncut)]
-  void *lbarv = nullptr, *lhatv = nullptr;  // the 42 filtered fields as volumes [21][nz][ny][ld]
+  double *lcon[4] = {};       // Germano contractions L.M, M.M, Q.N, N.N (binary64 volumes)
   double *Fi[4] = {};         // second buffer of the Lagrangian averages (exchanged with F)
   void *Hs[3] = {};           // pressure RHS spectra
   void *bw = nullptr, *tw = nullptr;  // wall / lid divergence spectra
@@ -437,7 +437,8 @@ extern "C" int lp_destroy(lp_ctx *c) {
   for (auto p : c->big) fr(p);
   for (auto p : c->tmp) fr(p);
   for (auto p : c->S) fr(p);
-  fr(c->lspec); fr(c->lbar); fr(c->lhat); fr(c->lbarv); fr(c->lhatv);
+  fr(c->lspec); fr(c->lbar); fr(c->lhat);
+  for (auto p : c->lcon) fr(p);
   for (auto p : c->Fi) fr(p);
   for (auto p : c->Hs) fr(p);
   fr(c->bw); fr(c->tw); fr(c->gam); fr(c->phat);
@@ -545,8 +546,7 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
     CK(dalloc(&c->lspec, pl * pad16(nc)));
     CK(dalloc(&c->lbar, pl * pad16(c->ncut1)));
     CK(dalloc(&c->lhat, pl * pad16(c->ncut2)));
-    CK(dalloc(&c->lbarv, (size_t)kNumLag * vol_b));
-    CK(dalloc(&c->lhatv, (size_t)kNumLag * vol_b));
+    for (auto &p : c->lcon) CK(dalloc((void **)&p, sizeof(double) * nv));
     for (auto &p : c->Fi) CK(dalloc((void **)&p, sizeof(double) * nv));
   }
   for (auto &p : c->Hs) CK(dalloc(&p, spec_b));
@@ -797,17 +797,6 @@ static int do_interpolag(lp_ctx *c) {
   return reset_lag_velocities(c);
 }
 
-// x back-transform of the 21 filtered spectra [21][nz][ny][pitch] (ncols columns used)
-// into 21 volumes
-static int c2r_batch(lp_ctx *c, const void *spec, int ncols, long pitch, void *out) {
-  const lp_grid &g = c->g;
-  prof_begin(c, LP_K_ROW_C2R);
-  HIPCHK(row_c2r(c->fdt, c->fdt, g.nx / 2, spec, pitch, pitch * g.ny, ncols, g.ny, kNumLag * g.nz, out, g.ld,
-                 plane_el(c), c->s));
-  prof_end(c, LP_K_ROW_C2R, bytes_c2r(g.nx, g.ny, kNumLag * g.nz, ncols, esz_of(c->fdt), esz_of(c->fdt)));
-  return LP_OK;
-}
-
 // lagrange_Sdep.f90:4-496 (with interpolag_Sdep at :120).  Needs the strain rates in
 // c->S and |S| in c->tmp[3] (k_strain).
 static int do_lagrange(lp_ctx *c) {
@@ -852,14 +841,15 @@ static int do_lagrange(lp_ctx *c) {
   const double *Fold[4] = {(const double *)c->f[LP_F_LM], (const double *)c->f[LP_F_MM],
                            (const double *)c->f[LP_F_QN], (const double *)c->f[LP_F_NN]};
   double *Fnew[4] = {c->Fi[0], c->Fi[1], c->Fi[2], c->Fi[3]};
-  // back along x: the 42 filtered fields as volumes
-  RCHK(c2r_batch(c, c->lbar, n1, p1, c->lbarv));
-  RCHK(c2r_batch(c, c->lhat, n2, p2, c->lhatv));
-  // :257-434 (+ interpolag_Sdep.f90:62-365): interpolation, contractions, relaxation
+  // :257-326 back along x and contracted, row by row
+  prof_begin(c, LP_K_ROW_C2R);
+  HIPCHK(lag_c2r_contract(c->fdt, c->lbar, c->lhat, n1, n2, p1, p2, c->lcon, g, P, c->s));
+  prof_end(c, LP_K_ROW_C2R, (double)kNumLag * gg.ny * gg.nz * (n1 + n2) * 2 * fe + vol * 4 * 8);
+  // :327-434 (+ interpolag_Sdep.f90:62-365): interpolation and relaxation
   prof_begin(c, LP_K_LAG_RELAX);
-  HIPCHK(lag_relax(c->fdt, c->lbarv, c->lhatv, Fold, Fnew, c->f[LP_U_LAG], c->f[LP_V_LAG], c->f[LP_W_LAG],
-                   c->f[LP_CS_OPT2], c->f[LP_BETA], g, P, I, c->s));
-  prof_end(c, LP_K_LAG_RELAX, vol * (2.0 * kNumLag * fe + 8.0 * 8 + 5.0 * fe));
+  HIPCHK(lag_relax(c->fdt, c->lcon, Fold, Fnew, c->f[LP_U_LAG], c->f[LP_V_LAG], c->f[LP_W_LAG], c->f[LP_CS_OPT2],
+                   c->f[LP_BETA], g, P, I, c->s));
+  prof_end(c, LP_K_LAG_RELAX, vol * (4.0 * 8 + 8.0 * 8 + 5.0 * fe));
   swap_F(c);
   RCHK(reset_lag_velocities(c));
   if (P.init_lm) c->F_LM_MM_init = 1;

@@ -6,9 +6,10 @@
 //                x transform (four-step FFT + real split), writing only the kx columns
 //                below the filters' cutoff (the 21 inputs are never stored as volumes);
 //   col_op       (lp_fft.hip) filters all 21 fields along y at both widths in one pass;
-//   row_c2r      (lp_fft.hip) transforms them back along x: 42 filtered volumes;
-//   k_lag_relax  interpolates the old averages at the backtraced point, contracts the
-//                42 filtered values and relaxes the averages, one thread per point.
+//   k_lag_c2r_contract  transforms the 21 fields of one width back along x for one
+//                (y, z) row in LDS and writes their two Germano contractions per point;
+//   k_lag_relax  interpolates the old averages at the backtraced point and relaxes them
+//                with the four contractions, one thread per point.
 #include "lp_fft4.hpp"
 #include "lp_lag.hpp"
 
@@ -144,14 +145,92 @@ __global__ __launch_bounds__(256) void k_lag_r2c(LagIn<T> in, Geo g, cx<T> *__re
   }
 }
 
-// lagrange_Sdep.f90:257-434 with interpolag_Sdep.f90:62-365, one thread per point:
-// bar / hat hold the 21 fields filtered at 2 and 4 delta as volumes (field q at q*nv);
-// the old averages are gathered at the backtraced point, relaxed and written to Fnew.
+// lagrange_Sdep.f90:257-326: x back-transform of the 21 fields of one width for one
+// (y, z) row and their Germano contractions.  blockIdx.x = row, blockIdx.y = width
+// (0: 2 delta -> L.M, M.M; 1: 4 delta -> Q.N, N.N).  21 transforms of P lanes each; the
+// LDS of the transforms is reused for the 21 physical rows once every transform has read
+// it.  c0/c1: the two contractions (binary64 volumes) of each width.
+template <typename T, int M>
+__global__ __launch_bounds__(256) void k_lag_c2r_contract(const cx<T> *__restrict__ bar, const cx<T> *__restrict__ hat,
+                                                          int n1, int n2, long p1, long p2, double *__restrict__ lm,
+                                                          double *__restrict__ mm, double *__restrict__ qn,
+                                                          double *__restrict__ nn, Geo g, double cst, double tf1_2,
+                                                          double tf2_2, const cx<T> *__restrict__ twM,
+                                                          const cx<T> *__restrict__ twN) {
+  constexpr int PL = row_lanes(M);
+  using F = Fft4<T, M, PL, +1>;
+  constexpr int Q = F::Q;
+  constexpr int SS = slot_stride(F::SCR);
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<T> *scr = reinterpret_cast<cx<T> *>(smem);  // [21][SS] during the transforms
+  T *vals = reinterpret_cast<T *>(smem);         // [21][ld] afterwards
+  const int ldp = g.ld;
+  const int row = blockIdx.x, half = blockIdx.y;
+  const int k = row / g.ny, y = row % g.ny;
+  const int slot = threadIdx.x / PL, t = threadIdx.x % PL;
+  const bool act = slot < 21;
+  const int q = act ? slot : 0;
+  const cx<T> *spec = half == 0 ? bar : hat;
+  const int nin = half == 0 ? n1 : n2;
+  const long pitch = half == 0 ? p1 : p2;
+  const cx<T> *src = spec + (((long)q * g.nz + k) * g.ny + y) * pitch;
+  auto ldx = [&](int kk) -> cx<T> {
+    cx<T> x = (act && kk < nin) ? src[kk] : mk<T>(0, 0);
+    if (kk == 0 || kk == M) x.y = 0;
+    return x;
+  };
+  // merge (as k_row_c2r): Y[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k])
+  cx<T> v[Q];
+#pragma unroll
+  for (int q2 = 0; q2 < Q; ++q2) {
+    const int kk = t + PL * q2;
+    cx<T> xa = ldx(kk), xb = conjg(ldx(M - kk));
+    cx<T> e = xa + xb, o = xa - xb;
+    cx<T> wo = cmul(conjg(twN[kk]), o);
+    v[q2] = mk<T>(e.x - wo.y, e.y + wo.x);
+  }
+  cx<T> *sc = scr + (act ? slot : 0) * SS;
+  if (act) F::stage1(v, t, sc, twM);
+  slot_sync<PL>();
+  cx<T> a[F::G][PL];
+  bool ok[F::G];
+#pragma unroll
+  for (int j = 0; j < F::G; ++j) ok[j] = act && F::stage2(j, t, sc, a[j]);
+  __syncthreads();  // every transform has read its slot: the LDS now holds rows
+#pragma unroll
+  for (int j = 0; j < F::G; ++j)
+    if (ok[j]) {
+#pragma unroll
+      for (int k1 = 0; k1 < PL; ++k1) {
+        const int n = t + PL * j + Q * k1;
+        vals[q * ldp + 2 * n] = a[j][k1].x;
+        vals[q * ldp + 2 * n + 1] = a[j][k1].y;
+      }
+    }
+  if (act)
+    for (int x = 2 * M + t; x < ldp; x += PL) vals[q * ldp + x] = T(0);
+  __syncthreads();
+  double *o0 = half == 0 ? lm : qn, *o1 = half == 0 ? mm : nn;
+  const double tf = half == 0 ? tf1_2 : tf2_2;
+  const long base = (long)k * g.np + (long)y * g.ld;
+  for (int x = threadIdx.x; x < ldp; x += blockDim.x) {
+    double c0, c1;
+    contract_point<T>([&](int qq) -> T { return vals[qq * ldp + x]; }, cst, tf, c0, c1);
+    o0[base + x] = c0;
+    o1[base + x] = c1;
+  }
+}
+
+// lagrange_Sdep.f90:327-434 with interpolag_Sdep.f90:62-365, one thread per point: the
+// old averages are gathered at the backtraced point, relaxed with the contractions and
+// written to Fnew.
 template <typename T>
-__global__ __launch_bounds__(256) void k_lag_relax(const T *__restrict__ bar, const T *__restrict__ hat, F4c Fold,
-                                                   F4 Fnew, const T *__restrict__ ul, const T *__restrict__ vl,
-                                                   const T *__restrict__ wl, T *__restrict__ Cs,
-                                                   T *__restrict__ Beta, Geo g, LagParams P, LagInterp I) {
+__global__ __launch_bounds__(256) void k_lag_relax(const double *__restrict__ lm, const double *__restrict__ mm,
+                                                   const double *__restrict__ qn, const double *__restrict__ nn,
+                                                   F4c Fold, F4 Fnew, const T *__restrict__ ul,
+                                                   const T *__restrict__ vl, const T *__restrict__ wl,
+                                                   T *__restrict__ Cs, T *__restrict__ Beta, Geo g, LagParams P,
+                                                   LagInterp I) {
   const long nv = g.np * g.nz;
   const long i = gtid();
   if (i >= nv) return;
@@ -166,11 +245,8 @@ __global__ __launch_bounds__(256) void k_lag_relax(const T *__restrict__ bar, co
 #pragma unroll
     for (int q = 0; q < 4; ++q) fi[q] = Fo[q][i];
   }
-  double lm, mm, qn, nn;
-  contract_point<T>([&](int q) -> T { return bar[q * nv + i]; }, P.cst, P.tf1_2, lm, mm);
-  contract_point<T>([&](int q) -> T { return hat[q * nv + i]; }, P.cst, P.tf2_2, qn, nn);
   T cs, b;
-  relax_point<T>(lm, mm, qn, nn, fi, x >= g.ld - 2, k == g.nz - 1, P, cs, b);
+  relax_point<T>(lm[i], mm[i], qn[i], nn[i], fi, x >= g.ld - 2, k == g.nz - 1, P, cs, b);
 #pragma unroll
   for (int q = 0; q < 4; ++q) Fnew.p[q][i] = fi[q];
   Cs[i] = cs;
@@ -210,26 +286,59 @@ hipError_t lag_r2c(Dt dt, const LagInputs &in, Geo g, void *spec, int nc, long p
 #undef CASE
 }
 
+template <typename T, int M>
+static hipError_t launch_lag_contract(const void *bar, const void *hat, int n1, int n2, long p1, long p2,
+                                      double *const C[4], Geo g, const LagParams &P, hipStream_t s, Dt dt) {
+  const void *twM = tw_ptr(M, dt), *twN = tw_ptr(2 * M, dt);
+  if (!twM || !twN || n1 < 1 || n2 < 1 || n1 > M + 1 || n2 > M + 1) return hipErrorInvalidValue;
+  constexpr int PL = row_lanes(M);
+  using F = Fft4<T, M, PL, +1>;
+  const size_t lds_fft = (size_t)21 * slot_stride(F::SCR) * sizeof(cx<T>);
+  const size_t lds_rows = (size_t)21 * g.ld * sizeof(T);
+  const size_t lds = lds_fft > lds_rows ? lds_fft : lds_rows;
+  const int threads = (21 * PL + 63) / 64 * 64;
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_lag_c2r_contract<T, M>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL((k_lag_c2r_contract<T, M>), dim3((unsigned)(g.ny * g.nz), 2), dim3(threads), lds, s,
+                     (const cx<T> *)bar, (const cx<T> *)hat, n1, n2, p1, p2, C[0], C[1], C[2], C[3], g, P.cst,
+                     P.tf1_2, P.tf2_2, (const cx<T> *)twM, (const cx<T> *)twN);
+  return hipGetLastError();
+}
+
+hipError_t lag_c2r_contract(Dt dt, const void *bar, const void *hat, int n1, int n2, long p1, long p2,
+                            double *const C[4], Geo g, const LagParams &P, hipStream_t s) {
+#define CASE(m)                                                                                       \
+  case m:                                                                                             \
+    return dt == F64 ? launch_lag_contract<double, m>(bar, hat, n1, n2, p1, p2, C, g, P, s, dt)       \
+                     : launch_lag_contract<float, m>(bar, hat, n1, n2, p1, p2, C, g, P, s, dt);
+  switch (g.nx / 2) {
+    LP_SMALL_ROWS(CASE)
+    default: return hipErrorInvalidValue;
+  }
+#undef CASE
+}
+
 template <typename T>
-static hipError_t launch_lag_relax(const void *bar, const void *hat, const double *const Fold[4],
-                                   double *const Fnew[4], const void *ul, const void *vl, const void *wl,
-                                   void *Cs, void *Beta, Geo g, const LagParams &P, const LagInterp &I,
-                                   hipStream_t s) {
+static hipError_t launch_lag_relax(double *const C[4], const double *const Fold[4], double *const Fnew[4],
+                                   const void *ul, const void *vl, const void *wl, void *Cs, void *Beta, Geo g,
+                                   const LagParams &P, const LagInterp &I, hipStream_t s) {
   F4c fo;
   F4 fn;
   for (int q = 0; q < 4; ++q) { fo.p[q] = Fold[q]; fn.p[q] = Fnew[q]; }
   const long n = g.np * g.nz;
-  hipLaunchKernelGGL((k_lag_relax<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (const T *)bar,
-                     (const T *)hat, fo, fn, (const T *)ul, (const T *)vl, (const T *)wl, (T *)Cs, (T *)Beta, g,
-                     P, I);
+  hipLaunchKernelGGL((k_lag_relax<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, C[0], C[1], C[2], C[3],
+                     fo, fn, (const T *)ul, (const T *)vl, (const T *)wl, (T *)Cs, (T *)Beta, g, P, I);
   return hipGetLastError();
 }
 
-hipError_t lag_relax(Dt dt, const void *bar, const void *hat, const double *const Fold[4], double *const Fnew[4],
-                     const void *ul, const void *vl, const void *wl, void *Cs, void *Beta, Geo g,
-                     const LagParams &P, const LagInterp &I, hipStream_t s) {
-  return dt == F64 ? launch_lag_relax<double>(bar, hat, Fold, Fnew, ul, vl, wl, Cs, Beta, g, P, I, s)
-                   : launch_lag_relax<float>(bar, hat, Fold, Fnew, ul, vl, wl, Cs, Beta, g, P, I, s);
+hipError_t lag_relax(Dt dt, double *const C[4], const double *const Fold[4], double *const Fnew[4], const void *ul,
+                     const void *vl, const void *wl, void *Cs, void *Beta, Geo g, const LagParams &P,
+                     const LagInterp &I, hipStream_t s) {
+  return dt == F64 ? launch_lag_relax<double>(C, Fold, Fnew, ul, vl, wl, Cs, Beta, g, P, I, s)
+                   : launch_lag_relax<float>(C, Fold, Fnew, ul, vl, wl, Cs, Beta, g, P, I, s);
 }
 
 }  // namespace lp
