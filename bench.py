@@ -120,6 +120,23 @@ def cpu_baseline(nx, seed):
                       f"{threads} OpenMP threads"}
 
 
+def measured_traffic(kernel_class, nx, nz, precision):
+    """HBM bytes per launch of the roofline kernel from the committed FETCH_SIZE /
+    WRITE_SIZE summary (rocprofv3 counter passes, calibrated on a plain copy), or None
+    when no summary covers this kernel and workload."""
+    if (kernel_class, nx, nz, precision) != ("col_op", 256, 257, "mixed"):
+        return None
+    path = os.path.join(ROOT, "profiles", "col_op_256_mixed_hbm_traffic.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def copy_probe(mib=512, reps=3):
     """a plain device copy of known size (torch), the reference reading for the
     FETCH_SIZE / WRITE_SIZE counters"""
@@ -175,6 +192,7 @@ def main():
     k_avg_ms = kms / max(klaunch, 1)
     k_bytes = kbytes / max(klaunch, 1)
     achieved = k_bytes / (k_avg_ms * 1e-3) / 1e9 if k_avg_ms > 0 else 0.0
+    traffic = measured_traffic(a.roofline_class, nx, nz, "mixed")
     breakdown = class_breakdown(lg_m) if a.breakdown else None
     if breakdown:
         for k, v_ in breakdown.items():
@@ -193,7 +211,9 @@ def main():
                    "grid": [nx, nx, nz], "seed": a.seed},
         "roofline": {"bound": "hbm", "kernel": a.roofline_class, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                     "traffic": None, "avg_launch_ms": round(k_avg_ms, 5),
+                     "traffic": traffic,
+                     "traffic_source": "profiles/col_op_256_mixed_hbm_traffic.md" if traffic else None,
+                     "avg_launch_ms": round(k_avg_ms, 5),
                      "bytes_per_launch": k_bytes, "launches": klaunch},
         "step_roofline": {"bytes_per_pt": B_PT["mixed"],
                           "achieved_GBps": round(B_PT["mixed"] * pts / (ms_m * 1e-3) / 1e9, 1),
