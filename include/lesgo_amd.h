@@ -244,7 +244,9 @@ enum lp_kernel_class {
   LP_K_PRESS_TRIDIAG = 4,  /* per-mode binary64 Thomas solve                        */
   LP_K_POLLEN_RHS = 5,     /* droplet transport RHS of all bins (SMART + diffusion)   */
   LP_K_PBE = 6,            /* droplet breakup source of all bins                     */
-  LP_K_NCLASSES = 7
+  LP_K_CONVEC = 7,         /* 3/2-grid x back-transforms + cross products + x forward */
+  LP_K_COL_PAD = 8,        /* column passes padding to / truncating from the 3/2 grid */
+  LP_K_NCLASSES = 9
 };
 /* Start (kernel_class >= 0) or stop (-1) event timing; starting clears the totals. */
 int lp_prof_enable(lp_ctx *ctx, int32_t kernel_class);

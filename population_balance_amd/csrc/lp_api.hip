@@ -16,6 +16,7 @@
 #include "lp_kernels.hpp"
 #include "lp_droplets.hpp"
 #include "lp_lag.hpp"
+#include "lp_convec.hpp"
 
 using namespace lp;
 
@@ -108,7 +109,9 @@ struct lp_ctx {
   // rounded up to 16, so that every row starts on a 128-byte boundary.
   long sp = 0, sb = 0;
   void *phat = nullptr;       // binary64 p-hat (0..nz) in the sp layout
-  void *big[9] = {};          // 3/2-grid physical fields (ld_big x ny2 x nz)
+  void *cvb = nullptr;        // convec: y-padded half spectra of u1 u2 u3 w1 w2 w3 [6][nz][ny2][cvp_pitch]
+  void *cvp = nullptr;        // convec: half spectra of the 3/2-grid products [3][nz][ny2][cvp_pitch]
+  long cv_pitch = 0;          // kx columns kept on the 3/2 grid (nx/2) rounded up to 16
   void *tmp[4] = {};          // field-type volumes
   void *S[6] = {};            // strain rates
   void *lspec = nullptr;      // x spectra of the 21 test-filter inputs [21][nz][ny][pad16(nc)]
@@ -434,7 +437,7 @@ extern "C" int lp_destroy(lp_ctx *c) {
   fr(c->kx); fr(c->ky); fr(c->k2); fr(c->G1); fr(c->G2);
   for (auto p : c->spec) fr(p);
   for (auto p : c->specb) fr(p);
-  for (auto p : c->big) fr(p);
+  fr(c->cvb); fr(c->cvp);
   for (auto p : c->tmp) fr(p);
   for (auto p : c->S) fr(p);
   fr(c->lspec); fr(c->lbar); fr(c->lhat);
@@ -533,11 +536,13 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
   c->sb = (g.lh_big + 15) / 16 * 16;
   size_t spec_b = (size_t)c->sp * g.ny * g.nz * 2 * fe;
   size_t specb_b = (size_t)c->sb * g.ny2 * g.nz * 2 * fe;
-  size_t big_b = (size_t)g.ld_big * g.ny2 * g.nz * fe;
   size_t vol_b = (size_t)nv * fe;
   for (auto &p : c->spec) CK(dalloc(&p, spec_b));
   for (auto &p : c->specb) CK(dalloc(&p, specb_b));
-  for (auto &p : c->big) CK(dalloc(&p, big_b));
+  c->cv_pitch = pad16(g.nx / 2);
+  const size_t cv_field = (size_t)c->cv_pitch * g.ny2 * g.nz * 2 * fe;
+  CK(dalloc(&c->cvb, 6 * cv_field));
+  CK(dalloc(&c->cvp, 3 * cv_field));
   for (auto &p : c->tmp) CK(dalloc(&p, vol_b));
   for (auto &p : c->S) CK(dalloc(&p, vol_b));
   if (cfg->model == 5) {
@@ -917,38 +922,59 @@ static int do_divstress(lp_ctx *c, void *divt, const void *tx, const void *ty, c
   return LP_OK;
 }
 
+// dealias1.f90 up to the x back-transform: r2c of a small field, then the column pass
+// that pads it to ny2 and transforms it back along y, into field q of c->cvb
+static int pad_y(lp_ctx *c, const void *f, int q) {
+  const lp_grid &g = c->g;
+  const double cst = 1.0 / ((double)g.nx * g.ny);
+  RCHK(r2c_small(c, f, g.nz, cst, c->spec[0]));
+  const long ps = c->sp * g.ny, pb = c->cv_pitch * g.ny2;
+  const size_t fe = esz_of(c->fdt);
+  ColOut o{(char *)c->cvb + (size_t)q * pb * g.nz * 2 * fe, M_ONE};
+  prof_begin(c, LP_K_COL_PAD);
+  HIPCHK(col_op(c->fdt, g.ny, g.ny2, 1, 1, 1, c->spec[0], c->sp, ps, g.nx / 2, g.nz, 1, &o, c->cv_pitch, pb,
+                c->tab, c->s));
+  prof_end(c, LP_K_COL_PAD, bytes_col(g.ny, g.ny2, g.nx / 2, g.nz, 1, fe));
+  return LP_OK;
+}
+
 static int do_convec(lp_ctx *c) {
   const lp_grid &gg = c->g;
   Geo g = geo(c);
   long nv = g.np * g.nz;
+  const size_t fe = esz_of(c->fdt);
   // convec.f90:76-98
-  RCHK(to_big(c, c->f[LP_U], gg.nz, c->big[0]));
-  RCHK(to_big(c, c->f[LP_V], gg.nz, c->big[1]));
-  RCHK(to_big(c, c->f[LP_W], gg.nz, c->big[2]));
+  RCHK(pad_y(c, c->f[LP_U], 0));
+  RCHK(pad_y(c, c->f[LP_V], 1));
+  RCHK(pad_y(c, c->f[LP_W], 2));
   // convec.f90:100-133
   FDISPATCH(c, T, hipLaunchKernelGGL(k_vort<T>, grid1(nv), dim3(256), 0, c->s, (const T *)c->f[LP_DUDY],
                                      (const T *)c->f[LP_DUDZ], (const T *)c->f[LP_DVDX], (const T *)c->f[LP_DVDZ],
                                      (const T *)c->f[LP_DWDX], (const T *)c->f[LP_DWDY], (T *)c->tmp[0],
                                      (T *)c->tmp[1], (T *)c->tmp[2], g));
   HIPCHK(hipGetLastError());
-  RCHK(to_big(c, c->tmp[0], gg.nz, c->big[3]));
-  RCHK(to_big(c, c->tmp[1], gg.nz, c->big[4]));
-  RCHK(to_big(c, c->tmp[2], gg.nz, c->big[5]));
-  long npb = (long)gg.ld_big * gg.ny2;
-  double cb = 1.0 / ((double)gg.nx2 * gg.ny2);
-  // convec.f90:140-158, 180-221, 241-279 products (normalisation folded in)
+  RCHK(pad_y(c, c->tmp[0], 3));
+  RCHK(pad_y(c, c->tmp[1], 4));
+  RCHK(pad_y(c, c->tmp[2], 5));
+  // convec.f90:140-158, 180-221, 241-279 products (normalisation folded in) between the
+  // x back- and forward transforms of the 3/2 grid, kx < nx/2 kept (dealias2)
+  const double cb = 1.0 / ((double)gg.nx2 * gg.ny2);
+  const long pb = c->cv_pitch * gg.ny2;
+  prof_begin(c, LP_K_CONVEC);
+  HIPCHK(convec_rows(c->fdt, gg.nx2 / 2, c->cvb, c->cvp, c->cv_pitch, gg.ny2, gg.nz, gg.nx / 2, cb, c->s));
+  // six padded spectra read, three product spectra written (kx < nx/2)
+  prof_end(c, LP_K_CONVEC, (double)(6 * gg.nz + 3 * (gg.nz - 1)) * gg.ny2 * (gg.nx / 2) * 2 * fe);
+  // convec.f90:161-170, 227-236, 286-296: unpadd along y, c2r small
   void *outs[3] = {c->f[LP_RHSX], c->f[LP_RHSY], c->f[LP_RHSZ]};
-  FDISPATCH(c, T, {
-    hipLaunchKernelGGL(k_convec_prod<T>, grid1(npb * (gg.nz - 1)), dim3(256), 0, c->s, (const T *)c->big[0],
-                       (const T *)c->big[1], (const T *)c->big[2], (const T *)c->big[3], (const T *)c->big[4],
-                       (const T *)c->big[5], (T *)c->big[6], (T *)c->big[7], (T *)c->big[8], npb, gg.nz,
-                       (T)cb);
-  });
-  HIPCHK(hipGetLastError());
-  // convec.f90:161-170, 227-236, 286-296: r2c big, unpadd, c2r small
-  RCHK(from_big(c, c->big[6], gg.nz - 1, 1.0, outs[0]));
-  RCHK(from_big(c, c->big[7], gg.nz - 1, 1.0, outs[1]));
-  RCHK(from_big(c, c->big[8], gg.nz - 1, 1.0, outs[2]));
+  const long ps = c->sp * gg.ny;
+  for (int q = 0; q < 3; ++q) {
+    ColOut o{c->spec[1], M_ONE};
+    prof_begin(c, LP_K_COL_PAD);
+    HIPCHK(col_op(c->fdt, gg.ny2, gg.ny, 2, 1, 1, (const char *)c->cvp + (size_t)q * pb * gg.nz * 2 * fe,
+                  c->cv_pitch, pb, gg.nx / 2, gg.nz - 1, 1, &o, c->sp, ps, c->tab, c->s));
+    prof_end(c, LP_K_COL_PAD, bytes_col(gg.ny2, gg.ny, gg.nx / 2, gg.nz - 1, 1, fe));
+    RCHK(c2r_small(c, c->spec[1], gg.nx / 2, gg.nz - 1, outs[q], c->fdt));
+  }
   // convec.f90:306-308 top level BOGUS
   for (int q = 0; q < 3; ++q) {
     FDISPATCH(c, T, hipLaunchKernelGGL(k_fill<T>, grid1(g.np), dim3(256), 0, c->s,
