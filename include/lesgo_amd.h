@@ -246,7 +246,8 @@ enum lp_kernel_class {
   LP_K_PBE = 6,            /* droplet breakup source of all bins                     */
   LP_K_CONVEC = 7,         /* 3/2-grid x back-transforms + cross products + x forward */
   LP_K_COL_PAD = 8,        /* column passes padding to / truncating from the 3/2 grid */
-  LP_K_NCLASSES = 9
+  LP_K_COL_DIV = 9,        /* column pass summing the x and y stress derivatives      */
+  LP_K_NCLASSES = 10
 };
 /* Start (kernel_class >= 0) or stop (-1) event timing; starting clears the totals. */
 int lp_prof_enable(lp_ctx *ctx, int32_t kernel_class);

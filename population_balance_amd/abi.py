@@ -37,7 +37,8 @@ FIELDS_F64_IN_MIXED = {"p", "F_LM", "F_MM", "F_QN", "F_NN", "PCon", "RHS_PCon",
                        "RHS_PCon_f", "Rhs_pbe"}
 BOGUS = -1234567890.0
 KERNEL_CLASSES = {"row_r2c": 0, "col_op": 1, "row_c2r": 2, "lag_relax": 3, "press_tridiag": 4,
-                  "pollen_rhs": 5, "pbe": 6, "convec_rows": 7, "col_pad": 8}
+                  "pollen_rhs": 5, "pbe": 6, "convec_rows": 7, "col_pad": 8,
+                  "col_div": 9}
 
 
 class lp_grid(C.Structure):

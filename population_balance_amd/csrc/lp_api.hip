@@ -907,18 +907,46 @@ static int do_sgs_stag(lp_ctx *c) {
   return LP_OK;
 }
 
+// divstress_uv.f90:31-127 / divstress_w.f90:31-82 from the x-spectra of tx and ty: the
+// x- and y-derivatives summed in spectral space (col_div), the z-derivative of tz added
+// in the back-transform
+static int divstress_from_spectra(lp_ctx *c, void *divt, const void *sx, const void *sy, const void *tz,
+                                  int is_w, void *work) {
+  const lp_grid &g = c->g;
+  const long ps = c->sp * g.ny;
+  const size_t fe = esz_of(c->fdt);
+  prof_begin(c, LP_K_COL_DIV);
+  HIPCHK(col_div(c->fdt, g.ny, sx, sy, c->sp, ps, g.lh, g.nz, work, c->tab, c->s));
+  prof_end(c, LP_K_COL_DIV, 3.0 * g.lh * g.ny * g.nz * 2 * fe);
+  prof_begin(c, LP_K_ROW_C2R);
+  HIPCHK(row_c2r_div(c->fdt, g.nx / 2, work, c->sp, ps, g.lh, g.ny, g.nz, divt, g.ld, plane_el(c), tz,
+                     1.0 / g.dz, is_w, c->s));
+  prof_end(c, LP_K_ROW_C2R, bytes_c2r(g.nx, g.ny, g.nz, g.lh, fe, fe) + (double)g.ld * g.ny * g.nz * fe);
+  return LP_OK;
+}
+
 static int do_divstress(lp_ctx *c, void *divt, const void *tx, const void *ty, const void *tz, int is_w) {
   const lp_grid &gg = c->g;
-  Geo g = geo(c);
-  double cst = 1.0 / ((double)gg.nx * gg.ny);
-  const int mx[1] = {M_IKX}, my[1] = {M_IKY};
-  void *oa[1] = {c->tmp[2]}, *ob[1] = {c->tmp[3]};
-  RCHK(spectral_apply(c, tx, gg.nz, cst, 1, mx, oa));
-  RCHK(spectral_apply(c, ty, gg.nz, cst, 1, my, ob));
-  FDISPATCH(c, T, hipLaunchKernelGGL(k_div_combine<T>, grid1(g.np * g.nz), dim3(256), 0, c->s, (T *)divt,
-                                     (const T *)c->tmp[2], (const T *)c->tmp[3], (const T *)tz, g,
-                                     (T)(1.0 / gg.dz), is_w));
-  HIPCHK(hipGetLastError());
+  const double cst = 1.0 / ((double)gg.nx * gg.ny);
+  RCHK(r2c_small(c, tx, gg.nz, cst, c->spec[0]));
+  RCHK(r2c_small(c, ty, gg.nz, cst, c->spec[1]));
+  return divstress_from_spectra(c, divt, c->spec[0], c->spec[1], tz, is_w, c->spec[2]);
+}
+
+// main.f90:394-397: the three divergences; txy, txz and tyz are transformed once
+static int do_divstress_all(lp_ctx *c) {
+  const lp_grid &gg = c->g;
+  const double cst = 1.0 / ((double)gg.nx * gg.ny);
+  void *sxx = c->spec[0], *sxy = c->spec[1], *syy = c->spec[2], *sxz = c->spec[3], *syz = c->Hs[0];
+  void *work = c->Hs[1];
+  RCHK(r2c_small(c, c->f[LP_TXX], gg.nz, cst, sxx));
+  RCHK(r2c_small(c, c->f[LP_TXY], gg.nz, cst, sxy));
+  RCHK(r2c_small(c, c->f[LP_TYY], gg.nz, cst, syy));
+  RCHK(r2c_small(c, c->f[LP_TXZ], gg.nz, cst, sxz));
+  RCHK(r2c_small(c, c->f[LP_TYZ], gg.nz, cst, syz));
+  RCHK(divstress_from_spectra(c, c->f[LP_DIVTX], sxx, sxy, c->f[LP_TXZ], 0, work));
+  RCHK(divstress_from_spectra(c, c->f[LP_DIVTY], sxy, syy, c->f[LP_TYZ], 0, work));
+  RCHK(divstress_from_spectra(c, c->f[LP_DIVTZ], sxz, syz, c->f[LP_TZZ], 1, work));
   return LP_OK;
 }
 
@@ -1209,9 +1237,7 @@ static int step_once(lp_ctx *c) {
       HIPCHK(hipGetLastError());
     }
   }
-  RCHK(do_divstress(c, c->f[LP_DIVTX], c->f[LP_TXX], c->f[LP_TXY], c->f[LP_TXZ], 0));  // :394-397
-  RCHK(do_divstress(c, c->f[LP_DIVTY], c->f[LP_TXY], c->f[LP_TYY], c->f[LP_TYZ], 0));
-  RCHK(do_divstress(c, c->f[LP_DIVTZ], c->f[LP_TXZ], c->f[LP_TYZ], c->f[LP_TZZ], 1));
+  RCHK(do_divstress_all(c));  // :394-397
   RCHK(do_convec(c));  // :419
   FDISPATCH(c, T, hipLaunchKernelGGL(k_rhs_assemble<T>, grid1(n1), dim3(256), 0, c->s, (T *)c->f[LP_RHSX],
                                      (T *)c->f[LP_RHSY], (T *)c->f[LP_RHSZ], (const T *)c->f[LP_DIVTX],

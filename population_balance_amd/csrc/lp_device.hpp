@@ -8,6 +8,9 @@
 // 4, then 2, then 3 (every length used here is 2^a 3^b).  Twiddles come from a table
 // tw[j] = exp(-2*pi*i*j/N) computed on the host in binary64.
 #pragma once
+
+// the reference's BOGUS marker (param.f90) for levels a routine leaves undefined
+#define LP_BOGUS (-1234567890.0)
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 

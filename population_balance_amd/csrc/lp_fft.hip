@@ -73,12 +73,18 @@ __global__ __launch_bounds__(kRowBlock) void k_row_r2c(const T *__restrict__ in,
 // row_c2r: M+1 complex (first ncols_in present; imaginary parts of kx = 0 and kx = M
 // ignored, as FFTW2's halfcomplex c2r) -> 2M reals, columns [2M, out_pitch) zeroed
 // ---------------------------------------------------------------------------------
-template <typename T, typename TO, int M>
+// EPI selects what is stored: 0 the transform; 1 (divstress_uv.f90:88-89,116-125) the
+// transform plus dz^-1 (tz(k+1) - tz(k)), BOGUS on the top level; 2 (divstress_w.f90:35,
+// 46-81 with the wall at level 1 and no lid term) the transform plus dz^-1 (tz(k) - tz(k-1))
+// on levels 1..nz-2.  The padding columns are zero (BOGUS on the uv top level).  tz has the
+// layout of out.
+template <typename T, typename TO, int M, int EPI>
 __global__ __launch_bounds__(kRowBlock) void k_row_c2r(const cx<T> *__restrict__ in, long in_pitch, long in_plane,
                                                        int ncols_in, int rows, int nplanes,
                                                        TO *__restrict__ out, long out_pitch, long out_plane,
                                                        const cx<T> *__restrict__ twM,
-                                                       const cx<T> *__restrict__ twN) {
+                                                       const cx<T> *__restrict__ twN,
+                                                       const TO *__restrict__ tz, TO cst) {
   constexpr int P = row_lanes(M);
   using F = Fft4<T, M, P, +1>;
   constexpr int Q = F::Q;
@@ -109,6 +115,23 @@ __global__ __launch_bounds__(kRowBlock) void k_row_c2r(const cx<T> *__restrict__
   F::stage1(v, t, sc, twM);
   slot_sync<P>();
   TO *dst = out + plane * out_plane + y * out_pitch;
+  const TO *tzr = tz + plane * out_plane + y * out_pitch;
+  const bool top = plane == nplanes - 1;
+  const bool mid = plane >= 1 && plane <= nplanes - 2;
+  const long up = top ? 0 : out_plane, dn = mid ? out_plane : 0;  // neighbour levels, clamped
+  auto fin = [&](T v, int x) -> TO {
+    if constexpr (EPI == 1) {
+      const TO d = cst * (tzr[x + up] - tzr[x]);
+      return top ? (TO)LP_BOGUS : (TO)v + d;
+    } else if constexpr (EPI == 2) {
+      TO r = (TO)v;
+      const TO d = cst * (tzr[x] - tzr[x - dn]);
+      if (mid) r += d;
+      return r;
+    } else {
+      return (TO)v;
+    }
+  };
 #pragma unroll
   for (int j = 0; j < F::G; ++j) {
     cx<T> a[P];
@@ -116,13 +139,15 @@ __global__ __launch_bounds__(kRowBlock) void k_row_c2r(const cx<T> *__restrict__
 #pragma unroll
       for (int k1 = 0; k1 < P; ++k1) {
         const int n = t + P * j + Q * k1;
-        dst[2 * n] = (TO)a[k1].x;
-        dst[2 * n + 1] = (TO)a[k1].y;
+        dst[2 * n] = fin(a[k1].x, 2 * n);
+        dst[2 * n + 1] = fin(a[k1].y, 2 * n + 1);
       }
     }
   }
-  if (valid)
-    for (long n = 2 * M + t; n < out_pitch; n += P) dst[n] = (TO)0;
+  if (valid) {
+    const TO pad = (EPI == 1 && top) ? (TO)LP_BOGUS : (TO)0;
+    for (long n = 2 * M + t; n < out_pitch; n += P) dst[n] = pad;
+  }
 }
 
 // ---------------------------------------------------------------------------------
@@ -146,6 +171,41 @@ struct ColParams {
 };
 static constexpr int kColLanes = 16;
 template <typename T> __host__ __device__ constexpr int col_tx() { return 128 / (int)sizeof(cx<T>); }
+
+// forward DFT of length NY along a column held in the D16 layout (sp[m] = x[t + 16 m]),
+// result in the same layout; sc is the column's LDS slot (block-wide barriers)
+template <typename T, int NY>
+__device__ __forceinline__ void col_forward(cx<T> (&sp)[NY / kColLanes], int t, cx<T> *sc,
+                                            const cx<T> *__restrict__ twI) {
+  constexpr int PL = kColLanes;
+  using FI = Fft4<T, NY, PL, -1>;
+  constexpr int QI = FI::Q;
+  FI::stage1(sp, t, sc, twI);
+  __syncthreads();
+  if constexpr (QI == PL) {  // stage 2 already yields the D16 layout
+    cx<T> a[PL];
+    FI::stage2(0, t, sc, a);
+#pragma unroll
+    for (int k1 = 0; k1 < PL; ++k1) sp[k1] = a[k1];
+    __syncthreads();
+  } else {  // regroup through LDS in natural order
+    cx<T> a[FI::G][PL];
+    bool ok[FI::G];
+#pragma unroll
+    for (int j = 0; j < FI::G; ++j) ok[j] = FI::stage2(j, t, sc, a[j]);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < FI::G; ++j)
+      if (ok[j]) {
+#pragma unroll
+        for (int k1 = 0; k1 < PL; ++k1) sc[t + PL * j + QI * k1] = a[j][k1];
+      }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < QI; ++m) sp[m] = sc[t + PL * m];
+    __syncthreads();
+  }
+}
 
 template <typename T, int NYI, int NYO>
 __global__ __launch_bounds__(col_tx<T>() * kColLanes) void k_col_op(
@@ -177,33 +237,7 @@ __global__ __launch_bounds__(col_tx<T>() * kColLanes) void k_col_op(
     const cx<T> x = src[m * istep];
     sp[m] = cv ? x : mk<T>(0, 0);
   }
-  if (fwd) {
-    FI::stage1(sp, t, sc, twI);
-    __syncthreads();
-    if constexpr (QI == PL) {  // stage 2 already yields the D16 layout
-      cx<T> a[PL];
-      FI::stage2(0, t, sc, a);
-#pragma unroll
-      for (int k1 = 0; k1 < PL; ++k1) sp[k1] = a[k1];
-      __syncthreads();
-    } else {  // regroup through LDS in natural order
-      cx<T> a[FI::G][PL];
-      bool ok[FI::G];
-#pragma unroll
-      for (int j = 0; j < FI::G; ++j) ok[j] = FI::stage2(j, t, sc, a[j]);
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < FI::G; ++j)
-        if (ok[j]) {
-#pragma unroll
-          for (int k1 = 0; k1 < PL; ++k1) sc[t + PL * j + QI * k1] = a[j][k1];
-        }
-      __syncthreads();
-#pragma unroll
-      for (int m = 0; m < QI; ++m) sp[m] = sc[t + PL * m];
-      __syncthreads();
-    }
-  }
+  if (fwd) col_forward<T, NYI>(sp, t, sc, twI);
   for (int jo = 0; jo < Pr.nout; ++jo) {
     const int mult = Pr.outs[jo].mult;
     // one multiplier table per output: i*kx, i*ky (imaginary) or a filter (real)
@@ -272,9 +306,98 @@ __global__ __launch_bounds__(col_tx<T>() * kColLanes) void k_col_op(
 }
 
 // ---------------------------------------------------------------------------------
+// col_div: the x- and y-derivative terms of divstress_uv.f90 / divstress_w.f90 summed in
+// spectral space.  Per kx column: forward y DFTs of the x-spectra A and B, then
+// i kx A + i ky B with the Nyquist row and column zeroed (the ddx / ddy multipliers),
+// inverse y DFT into out.
+// ---------------------------------------------------------------------------------
+template <typename T, int NY>
+__global__ __launch_bounds__(col_tx<T>() * kColLanes) void k_col_div(
+    const cx<T> *__restrict__ A, const cx<T> *__restrict__ B, long pitch, long plane_el, int ncols,
+    cx<T> *__restrict__ out, const double *__restrict__ kxt, const double *__restrict__ kyt, int lh_s, int ny_s,
+    const cx<T> *__restrict__ tw) {
+  constexpr int TX = col_tx<T>();
+  constexpr int PL = kColLanes;
+  using FO = Fft4<T, NY, PL, +1>;
+  constexpr int Q = NY / PL;
+  constexpr int SS = slot_stride(FO::SCR > NY ? FO::SCR : NY);
+  __shared__ cx<T> lds[TX * SS];
+  const int c = threadIdx.x % TX, t = threadIdx.x / TX;
+  const int kx = blockIdx.x * TX + c;
+  const bool cv = kx < ncols;
+  const long plane = blockIdx.y;
+  cx<T> *sc = lds + c * SS;
+  const long off = plane * plane_el + (cv ? kx : ncols - 1) + (long)t * pitch;
+  const long istep = (long)PL * pitch;
+  cx<T> sa[Q], sb[Q];  // D16
+#pragma unroll
+  for (int m = 0; m < Q; ++m) {
+    sa[m] = A[off + m * istep];
+    sb[m] = B[off + m * istep];
+  }
+  col_forward<T, NY>(sa, t, sc, tw);
+  col_forward<T, NY>(sb, t, sc, tw);
+  const bool keep = cv && kx != lh_s - 1;
+  const long mo = (long)t * lh_s + (cv ? kx : 0);
+  T fx[Q], fy[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    fx[q] = (T)kxt[mo + (long)q * PL * lh_s];
+    fy[q] = (T)kyt[mo + (long)q * PL * lh_s];
+  }
+  cx<T> v[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int yo = t + PL * q;
+    const cx<T> a = sa[q], b = sb[q];
+    const cx<T> x = mk<T>(-a.y * fx[q] + -b.y * fy[q], a.x * fx[q] + b.x * fy[q]);
+    v[q] = (!keep || yo == ny_s / 2) ? mk<T>(0, 0) : x;
+  }
+  FO::stage1(v, t, sc, tw);
+  __syncthreads();
+  cx<T> *dst = out + plane * plane_el + kx;
+#pragma unroll
+  for (int j = 0; j < FO::G; ++j) {
+    cx<T> a[PL];
+    if (FO::stage2(j, t, sc, a) && cv) {
+      cx<T> *d = dst + (long)(t + PL * j) * pitch;
+      const long ostep = (long)Q * pitch;
+#pragma unroll
+      for (int k1 = 0; k1 < PL; ++k1) d[k1 * ostep] = a[k1];
+    }
+  }
+}
+
+template <typename T, int NY>
+static hipError_t launch_col_div(const void *A, const void *B, long pitch, long plane, int ncols, int nplanes,
+                                 void *out, const SpecTables &tab, hipStream_t s, Dt dt) {
+  const void *tw = tw_ptr(NY, dt);
+  if (!tw || ncols < 1 || ncols > tab.lh || pitch < ncols || tab.ny != NY) return hipErrorInvalidValue;
+  constexpr int TX = col_tx<T>();
+  dim3 grid((unsigned)((ncols + TX - 1) / TX), (unsigned)nplanes);
+  hipLaunchKernelGGL((k_col_div<T, NY>), grid, dim3(TX * kColLanes), 0, s, (const cx<T> *)A, (const cx<T> *)B,
+                     pitch, plane, ncols, (cx<T> *)out, tab.kx, tab.ky, tab.lh, tab.ny, (const cx<T> *)tw);
+  return hipGetLastError();
+}
+
+hipError_t col_div(Dt dt, int ny, const void *A, const void *B, long pitch, long plane, int ncols, int nplanes,
+                   void *out, const SpecTables &tab, hipStream_t s) {
+#define CASE(n)                                                                                      \
+  case n:                                                                                            \
+    return dt == F64 ? launch_col_div<double, n>(A, B, pitch, plane, ncols, nplanes, out, tab, s, dt) \
+                     : launch_col_div<float, n>(A, B, pitch, plane, ncols, nplanes, out, tab, s, dt);
+  switch (ny) {
+    CASE(32) CASE(48) CASE(64) CASE(96) CASE(128) CASE(144) CASE(192) CASE(256)
+    default: return hipErrorInvalidValue;
+  }
+#undef CASE
+}
+
+// ---------------------------------------------------------------------------------
 // dispatch
 // ---------------------------------------------------------------------------------
 #define LP_ROW_LENGTHS(X) X(16) X(24) X(32) X(48) X(64) X(72) X(96) X(128) X(144) X(192)
+#define LP_SMALL_ROW_LENGTHS(X) X(16) X(32) X(48) X(64) X(96) X(128)
 
 bool fft_len_supported(int n) {
   switch (n) {
@@ -314,18 +437,41 @@ hipError_t row_r2c(Dt dt, int M, const void *in, long in_pitch, long in_plane, i
 #undef CASE
 }
 
-template <typename T, typename TO, int M>
+template <typename T, typename TO, int M, int EPI = 0>
 static hipError_t launch_c2r(const void *in, long in_pitch, long in_plane, int ncols_in, int rows,
-                             int nplanes, void *out, long out_pitch, long out_plane, hipStream_t s, Dt dt) {
+                             int nplanes, void *out, long out_pitch, long out_plane, hipStream_t s, Dt dt,
+                             const void *tz = nullptr, double cst = 0.0) {
   const void *twM = tw_ptr(M, dt), *twN = tw_ptr(2 * M, dt);
   if (!twM || !twN) return hipErrorInvalidValue;
   constexpr int RPB = kRowBlock / row_lanes(M);
   long total = (long)rows * nplanes;
   dim3 grid((unsigned)((total + RPB - 1) / RPB));
-  hipLaunchKernelGGL((k_row_c2r<T, TO, M>), grid, dim3(kRowBlock), 0, s, (const cx<T> *)in, in_pitch, in_plane,
-                     ncols_in, rows, nplanes, (TO *)out, out_pitch, out_plane, (const cx<T> *)twM,
-                     (const cx<T> *)twN);
+  hipLaunchKernelGGL((k_row_c2r<T, TO, M, EPI>), grid, dim3(kRowBlock), 0, s, (const cx<T> *)in, in_pitch,
+                     in_plane, ncols_in, rows, nplanes, (TO *)out, out_pitch, out_plane, (const cx<T> *)twM,
+                     (const cx<T> *)twN, (const TO *)(EPI ? tz : out), (TO)cst);
   return hipGetLastError();
+}
+
+hipError_t row_c2r_div(Dt dt, int M, const void *in, long in_pitch, long in_plane, int ncols_in, int rows,
+                       int nplanes, void *out, long out_pitch, long out_plane, const void *tz, double cst,
+                       int is_w, hipStream_t s) {
+  if (!tz || nplanes < 2 || out_pitch < 2 * M) return hipErrorInvalidValue;
+#define CASE(m)                                                                                          \
+  case m:                                                                                                \
+    if (dt == F64)                                                                                       \
+      return is_w ? launch_c2r<double, double, m, 2>(in, in_pitch, in_plane, ncols_in, rows, nplanes,    \
+                                                     out, out_pitch, out_plane, s, dt, tz, cst)          \
+                  : launch_c2r<double, double, m, 1>(in, in_pitch, in_plane, ncols_in, rows, nplanes,    \
+                                                     out, out_pitch, out_plane, s, dt, tz, cst);         \
+    return is_w ? launch_c2r<float, float, m, 2>(in, in_pitch, in_plane, ncols_in, rows, nplanes, out,   \
+                                                 out_pitch, out_plane, s, dt, tz, cst)                   \
+                : launch_c2r<float, float, m, 1>(in, in_pitch, in_plane, ncols_in, rows, nplanes, out,   \
+                                                 out_pitch, out_plane, s, dt, tz, cst);
+  switch (M) {
+    LP_SMALL_ROW_LENGTHS(CASE)
+    default: return hipErrorInvalidValue;
+  }
+#undef CASE
 }
 
 hipError_t row_c2r(Dt dtc, Dt dto, int M, const void *in, long in_pitch, long in_plane, int ncols_in,

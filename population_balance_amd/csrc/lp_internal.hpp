@@ -55,6 +55,19 @@ hipError_t col_op(Dt dt, int nyi, int nyo, int padmode, int fwd, int inv, const 
                   const ColOut *outs, long out_pitch, long out_plane, const SpecTables &tab,
                   hipStream_t s);
 
+// Column pass of the divergence terms: out = inverse_y( i kx fwd_y(A) + i ky fwd_y(B) )
+// with the Nyquist row and column zeroed; A, B, out share pitch / plane; ny == tab.ny.
+hipError_t col_div(Dt dt, int ny, const void *A, const void *B, long pitch, long plane, int ncols, int nplanes,
+                   void *out, const SpecTables &tab, hipStream_t s);
+
+// row_c2r of small spectra followed by the z-derivative term of divstress: is_w = 0
+// (divstress_uv) adds cst (tz(k+1) - tz(k)) and writes BOGUS on the top level; is_w = 1
+// (divstress_w) adds cst (tz(k) - tz(k-1)) on levels 1..nplanes-2.  Compute and store
+// type dt; tz has the layout of out.
+hipError_t row_c2r_div(Dt dt, int M, const void *in, long in_pitch, long in_plane, int ncols_in, int rows,
+                       int nplanes, void *out, long out_pitch, long out_plane, const void *tz, double cst,
+                       int is_w, hipStream_t s);
+
 bool fft_len_supported(int n);
 
 }  // namespace lp

@@ -7,8 +7,6 @@
 
 namespace lp {
 
-#define LP_BOGUS (-1234567890.0)
-
 struct Geo {
   int nx, ny, nz, ld;
   long np;  // ld*ny
@@ -342,27 +340,6 @@ __device__ __forceinline__ void relax_point(double LM, double MM, double QN, dou
   fi[0] = flm; fi[1] = fmm; fi[2] = fqn; fi[3] = fnn;
   cs_out = (T)csv;
   beta_out = (T)b;
-}
-
-// divstress_uv.f90:116-125 and divstress_w.f90:173-211
-template <typename T>
-__global__ void k_div_combine(T *__restrict__ divt, const T *__restrict__ a, const T *__restrict__ b,
-                              const T *__restrict__ tz, Geo g, T cst, int is_w) {
-  long i = gtid();
-  if (i >= g.np * g.nz) return;
-  int k = (int)(i / g.np), x = (int)(i % g.ld);
-  if (!is_w) {
-    if (k == g.nz - 1) { divt[i] = (T)LP_BOGUS; return; }
-    if (x >= g.ld - 2) { divt[i] = T(0); return; }
-    // ddz_w(tz) at x < nx (padding of the derivative is irrelevant: set to 0 there)
-    T dz = (x < g.nx) ? cst * (tz[i + g.np] - tz[i]) : T(0);
-    divt[i] = a[i] + b[i] + dz;
-  } else {
-    if (x >= g.ld - 2) { divt[i] = T(0); return; }
-    T s = a[i] + b[i];
-    if (k >= 1 && k <= g.nz - 2) s += cst * (tz[i] - tz[i - g.np]);  // ddz_uv; 0 at nz
-    divt[i] = s;
-  }
 }
 
 // convec.f90:100-119 vorticity (jz = 1 special case on uvp nodes)
