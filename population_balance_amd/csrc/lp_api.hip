@@ -892,17 +892,12 @@ static int do_sgs_stag(lp_ctx *c) {
                                        (T)(cf.Co * cf.Co)));
     HIPCHK(hipGetLastError());
   }
-  FDISPATCH(c, T, hipLaunchKernelGGL(k_nut<T>, grid1(nv), dim3(256), 0, c->s, (const T *)c->S[0],
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_nut_tau<T>, grid1(nv), dim3(256), 0, c->s, (const T *)c->S[0],
                                      (const T *)c->S[1], (const T *)c->S[2], (const T *)c->S[3], (const T *)c->S[4],
                                      (const T *)c->S[5], (const T *)c->f[LP_CS_OPT2], (T *)c->f[LP_NU_T],
-                                     (T *)c->f[LP_DISSIP], (T *)c->f[LP_MAGS], g, cf.model, delta, cf.Co, cf.nnn,
-                                     cf.vonk, gg.dz));
-  HIPCHK(hipGetLastError());
-  FDISPATCH(c, T, hipLaunchKernelGGL(k_tau<T>, grid1(nv), dim3(256), 0, c->s, (const T *)c->S[0],
-                                     (const T *)c->S[1], (const T *)c->S[2], (const T *)c->S[3], (const T *)c->S[4],
-                                     (const T *)c->S[5], (const T *)c->f[LP_NU_T], (T *)c->f[LP_TXX],
-                                     (T *)c->f[LP_TXY], (T *)c->f[LP_TXZ], (T *)c->f[LP_TYY], (T *)c->f[LP_TYZ],
-                                     (T *)c->f[LP_TZZ], g));
+                                     (T *)c->f[LP_DISSIP], (T *)c->f[LP_MAGS], (T *)c->f[LP_TXX], (T *)c->f[LP_TXY],
+                                     (T *)c->f[LP_TXZ], (T *)c->f[LP_TYY], (T *)c->f[LP_TYZ], (T *)c->f[LP_TZZ], g,
+                                     cf.model, delta, cf.Co, cf.nnn, cf.vonk, gg.dz));
   HIPCHK(hipGetLastError());
   return LP_OK;
 }
@@ -1195,18 +1190,25 @@ static int check_err_flag(lp_ctx *c) {
   return LP_OK;
 }
 
+// forcing.f90:215-292 boundary values
+static int project_bc(lp_ctx *c) {
+  const lp_config &cf = c->cfg;
+  Geo g = geo(c);
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_project_bc<T>, grid1(g.np), dim3(256), 0, c->s, (T *)c->f[LP_U],
+                                     (T *)c->f[LP_V], (T *)c->f[LP_W], g, cf.use_mean_p_force,
+                                     cf.lbc_mom == LP_LBC_STRESS_FREE ? 1 : 0));
+  HIPCHK(hipGetLastError());
+  return LP_OK;
+}
+
 static int do_project(lp_ctx *c) {
   const lp_config &cf = c->cfg;
   Geo g = geo(c);
-  FDISPATCH(c, T, {
-    hipLaunchKernelGGL(k_project<T>, grid1(g.np * (g.nz - 1)), dim3(256), 0, c->s, (T *)c->f[LP_U],
-                       (T *)c->f[LP_V], (T *)c->f[LP_W], (const T *)c->f[LP_DPDX], (const T *)c->f[LP_DPDY],
-                       (const T *)c->f[LP_DPDZ], g, (T)cf.dt);
-    hipLaunchKernelGGL(k_project_bc<T>, grid1(g.np), dim3(256), 0, c->s, (T *)c->f[LP_U], (T *)c->f[LP_V],
-                       (T *)c->f[LP_W], g, cf.use_mean_p_force, cf.lbc_mom == LP_LBC_STRESS_FREE ? 1 : 0);
-  });
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_project<T>, grid1(g.np * (g.nz - 1)), dim3(256), 0, c->s, (T *)c->f[LP_U],
+                                     (T *)c->f[LP_V], (T *)c->f[LP_W], (const T *)c->f[LP_DPDX],
+                                     (const T *)c->f[LP_DPDY], (const T *)c->f[LP_DPDZ], g, (T)cf.dt));
   HIPCHK(hipGetLastError());
-  return LP_OK;
+  return project_bc(c);
 }
 
 // main.f90:252-826
@@ -1215,10 +1217,8 @@ static int step_once(lp_ctx *c) {
   const lp_grid &gg = c->g;
   Geo g = geo(c);
   long n1 = g.np * (g.nz - 1);
-  size_t vb = c->bytes[LP_RHSX];
-  HIPCHK(hipMemcpyAsync(c->f[LP_RHSX_F], c->f[LP_RHSX], vb, hipMemcpyDeviceToDevice, c->s));  // :252-254
-  HIPCHK(hipMemcpyAsync(c->f[LP_RHSY_F], c->f[LP_RHSY], vb, hipMemcpyDeviceToDevice, c->s));
-  HIPCHK(hipMemcpyAsync(c->f[LP_RHSZ_F], c->f[LP_RHSZ], vb, hipMemcpyDeviceToDevice, c->s));
+  // :252-254 RHS_f = RHS: the buffers exchange roles; convec rewrites every level of RHS
+  for (int q = 0; q < 3; ++q) std::swap(c->f[LP_RHSX + q], c->f[LP_RHSX_F + q]);
   RCHK(do_filt_da(c, c->f[LP_U], c->f[LP_DUDX], c->f[LP_DUDY], gg.nz));  // :277-279
   RCHK(do_filt_da(c, c->f[LP_V], c->f[LP_DVDX], c->f[LP_DVDY], gg.nz));
   RCHK(do_filt_da(c, c->f[LP_W], c->f[LP_DWDX], c->f[LP_DWDY], gg.nz));
@@ -1239,32 +1239,34 @@ static int step_once(lp_ctx *c) {
   }
   RCHK(do_divstress_all(c));  // :394-397
   RCHK(do_convec(c));  // :419
-  FDISPATCH(c, T, hipLaunchKernelGGL(k_rhs_assemble<T>, grid1(n1), dim3(256), 0, c->s, (T *)c->f[LP_RHSX],
+  // :433-439, 458-460, 577-584, 716-728
+  const bool first = c->jt == 1 && !cf.initu;
+  double force = cf.use_mean_p_force ? cf.mean_p_force : 0.0;
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_rhs_ab2<T>, grid1(n1), dim3(256), 0, c->s, (T *)c->f[LP_RHSX],
                                      (T *)c->f[LP_RHSY], (T *)c->f[LP_RHSZ], (const T *)c->f[LP_DIVTX],
                                      (const T *)c->f[LP_DIVTY], (const T *)c->f[LP_DIVTZ], (T *)c->f[LP_DUDT],
                                      (T *)c->f[LP_DVDT], (T *)c->f[LP_DWDT],
-                                     (const T *)(drops && cf.drop.active ? c->f[LP_BETA_PCON] : nullptr), n1));
+                                     (const T *)(drops && cf.drop.active ? c->f[LP_BETA_PCON] : nullptr),
+                                     (T *)c->f[LP_RHSX_F], (T *)c->f[LP_RHSY_F], (T *)c->f[LP_RHSZ_F], first ? 1 : 0,
+                                     (T *)c->f[LP_U], (T *)c->f[LP_V], (T *)c->f[LP_W], n1, (T)cf.dt, (T)force,
+                                     (T)cf.mean_p_force));
   HIPCHK(hipGetLastError());
-  if (c->jt == 1 && !cf.initu) {  // :577-584
-    HIPCHK(hipMemcpyAsync(c->f[LP_RHSX_F], c->f[LP_RHSX], vb, hipMemcpyDeviceToDevice, c->s));
-    HIPCHK(hipMemcpyAsync(c->f[LP_RHSY_F], c->f[LP_RHSY], vb, hipMemcpyDeviceToDevice, c->s));
-    HIPCHK(hipMemcpyAsync(c->f[LP_RHSZ_F], c->f[LP_RHSZ], vb, hipMemcpyDeviceToDevice, c->s));
+  if (first) {  // :577-584 also copies the (BOGUS) top level
+    const size_t fe = esz_of(c->fdt), off = (size_t)n1 * fe, pb = (size_t)g.np * fe;
+    for (int q = 0; q < 3; ++q)
+      HIPCHK(hipMemcpyAsync((char *)c->f[LP_RHSX_F + q] + off, (const char *)c->f[LP_RHSX + q] + off, pb,
+                            hipMemcpyDeviceToDevice, c->s));
   }
-  double force = cf.use_mean_p_force ? cf.mean_p_force : 0.0;
-  FDISPATCH(c, T, hipLaunchKernelGGL(k_ab2<T>, grid1(n1), dim3(256), 0, c->s, (T *)c->f[LP_U], (T *)c->f[LP_V],
-                                     (T *)c->f[LP_W], (const T *)c->f[LP_RHSX], (const T *)c->f[LP_RHSY],
-                                     (const T *)c->f[LP_RHSZ], (const T *)c->f[LP_RHSX_F],
-                                     (const T *)c->f[LP_RHSY_F], (const T *)c->f[LP_RHSZ_F], (T *)c->f[LP_DUDT],
-                                     n1, (T)cf.dt, (T)force, (T)cf.mean_p_force));
-  HIPCHK(hipGetLastError());
   RCHK(do_press(c));  // :785
-  FDISPATCH(c, T, hipLaunchKernelGGL(k_dpdz_rhs<T>, grid1(g.np * g.nz), dim3(256), 0, c->s,
+  // :789-803, 826 (forcing.f90:145-292)
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_dpdz_project<T>, grid1(g.np * g.nz), dim3(256), 0, c->s,
                                      (const double *)c->f[LP_P], (T *)c->f[LP_DPDZ], (const T *)c->f[LP_DPDX],
                                      (const T *)c->f[LP_DPDY], (T *)c->f[LP_RHSX], (T *)c->f[LP_RHSY],
                                      (T *)c->f[LP_RHSZ], (T *)c->f[LP_DUDT], (T *)c->f[LP_DVDT],
-                                     (T *)c->f[LP_DWDT], g, gg.dz));
+                                     (T *)c->f[LP_DWDT], (T *)c->f[LP_U], (T *)c->f[LP_V], (T *)c->f[LP_W], g,
+                                     gg.dz, (T)cf.dt));
   HIPCHK(hipGetLastError());
-  RCHK(do_project(c));  // :826
+  RCHK(project_bc(c));
   if (drops) RCHK(do_ke_dudt(c));  // :864-898
   return LP_OK;
 }

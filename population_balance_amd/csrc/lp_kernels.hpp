@@ -112,17 +112,13 @@ __global__ void k_lag_acc(T *__restrict__ ul, T *__restrict__ vl, T *__restrict_
 }
 
 
-// sgs_stag.f90:137-177 (mixing length) and :215-227 (|S|, Nu_t, dissip, magS)
+// sgs_stag.f90:137-177 (mixing length) and :215-227 (|S|, Nu_t, dissip, magS) of level k
 template <typename T>
-__global__ void k_nut(const T *__restrict__ S11, const T *__restrict__ S12, const T *__restrict__ S13,
-                      const T *__restrict__ S22, const T *__restrict__ S23, const T *__restrict__ S33,
-                      const T *__restrict__ Cs, T *__restrict__ Nu_t, T *__restrict__ dissip,
-                      T *__restrict__ magS, Geo g, int model, double delta, double Co, int nnn,
-                      double vonk, double dz) {
-  long i = gtid();
-  if (i >= g.np * g.nz) return;
-  int k = (int)(i / g.np), x = (int)(i % g.ld);
-  if (x >= g.nx) return;
+__device__ __forceinline__ T nut_point(const T *__restrict__ S11, const T *__restrict__ S12,
+                                       const T *__restrict__ S13, const T *__restrict__ S22,
+                                       const T *__restrict__ S23, const T *__restrict__ S33,
+                                       const T *__restrict__ Cs, long i, int k, int model, double delta, double Co,
+                                       int nnn, double vonk, double dz, T &s, T &c, T &l2) {
   double l;
   if (model == 1) {
     double zz = (k == 0) ? 0.5 * dz : (double)k * dz;
@@ -131,24 +127,33 @@ __global__ void k_nut(const T *__restrict__ S11, const T *__restrict__ S12, cons
   } else {
     l = delta;
   }
-  T l2 = (T)(l * l);
-  T s = smag(S11[i], S12[i], S13[i], S22[i], S23[i], S33[i]);
-  T c = Cs[i];
-  Nu_t[i] = s * c * l2;
-  dissip[i] = -(s * s * s) * c * l2;
-  magS[i] = s;
+  l2 = (T)(l * l);
+  s = smag(S11[i], S12[i], S13[i], S22[i], S23[i], S33[i]);
+  c = Cs[i];
+  return s * c * l2;
 }
 
-// sgs_stag.f90:254-325, 391-394
+// sgs_stag.f90:137-177, 215-227 (Nu_t, dissip, magS) and :254-325, 391-394 (the six
+// stresses) in one pass; the w-node stresses average Nu_t with the level above, which is
+// recomputed here from that level's strain rates
 template <typename T>
-__global__ void k_tau(const T *__restrict__ S11, const T *__restrict__ S12, const T *__restrict__ S13,
-                      const T *__restrict__ S22, const T *__restrict__ S23, const T *__restrict__ S33,
-                      const T *__restrict__ Nu_t, T *__restrict__ txx, T *__restrict__ txy,
-                      T *__restrict__ txz, T *__restrict__ tyy, T *__restrict__ tyz, T *__restrict__ tzz,
-                      Geo g) {
+__global__ void k_nut_tau(const T *__restrict__ S11, const T *__restrict__ S12, const T *__restrict__ S13,
+                          const T *__restrict__ S22, const T *__restrict__ S23, const T *__restrict__ S33,
+                          const T *__restrict__ Cs, T *__restrict__ Nu_t, T *__restrict__ dissip,
+                          T *__restrict__ magS, T *__restrict__ txx, T *__restrict__ txy, T *__restrict__ txz,
+                          T *__restrict__ tyy, T *__restrict__ tyz, T *__restrict__ tzz, Geo g, int model,
+                          double delta, double Co, int nnn, double vonk, double dz) {
   long i = gtid();
   if (i >= g.np * g.nz) return;
   int k = (int)(i / g.np), x = (int)(i % g.ld);
+  T nu = T(0);
+  if (x < g.nx) {
+    T s, c, l2;
+    nu = nut_point<T>(S11, S12, S13, S22, S23, S33, Cs, i, k, model, delta, Co, nnn, vonk, dz, s, c, l2);
+    Nu_t[i] = nu;
+    dissip[i] = -(s * s * s) * c * l2;
+    magS[i] = s;
+  }
   if (k == g.nz - 1) {
     txz[i] = T(0);
     tyz[i] = T(0);
@@ -157,21 +162,22 @@ __global__ void k_tau(const T *__restrict__ S11, const T *__restrict__ S12, cons
   if (x >= g.nx) return;
   const T m2 = T(-2), h = T(0.5);
   if (k == 0) {
-    T c = Nu_t[i];
-    txx[i] = m2 * c * S11[i];
-    txy[i] = m2 * c * S12[i];
-    tyy[i] = m2 * c * S22[i];
-    tzz[i] = m2 * c * S33[i];
+    txx[i] = m2 * nu * S11[i];
+    txy[i] = m2 * nu * S12[i];
+    tyy[i] = m2 * nu * S22[i];
+    tzz[i] = m2 * nu * S33[i];
   } else {
     long p = i + g.np;
-    T c = h * (Nu_t[i] + Nu_t[p]);
+    T s1, c1, l21;
+    const T nu1 = nut_point<T>(S11, S12, S13, S22, S23, S33, Cs, p, k + 1, model, delta, Co, nnn, vonk, dz, s1,
+                               c1, l21);
+    T c = h * (nu + nu1);
     txx[i] = m2 * c * h * (S11[i] + S11[p]);
     txy[i] = m2 * c * h * (S12[i] + S12[p]);
     tyy[i] = m2 * c * h * (S22[i] + S22[p]);
     tzz[i] = m2 * c * h * (S33[i] + S33[p]);
-    T cn = Nu_t[i];
-    txz[i] = m2 * cn * S13[i];
-    tyz[i] = m2 * cn * S23[i];
+    txz[i] = m2 * nu * S13[i];
+    tyz[i] = m2 * nu * S23[i];
   }
 }
 
@@ -384,58 +390,66 @@ __global__ void k_convec_prod(const T *__restrict__ u1, const T *__restrict__ u2
   }
 }
 
-// main.f90:433-439 (+458-460): RHS = -conv - divt (+ beta_pcon); dudt = -divt
+// main.f90:433-439 (+458-460) and 716-728 in one pass: RHS = -conv - divt (+ beta_pcon),
+// dudt = -divt (+ mean_p_force on x), then the AB2 predictor.  first: RHS_f = RHS
+// (main.f90:577-584) for the levels written here.
 template <typename T>
-__global__ void k_rhs_assemble(T *__restrict__ Rx, T *__restrict__ Ry, T *__restrict__ Rz,
-                               const T *__restrict__ dx_, const T *__restrict__ dy_, const T *__restrict__ dz_,
-                               T *__restrict__ ax, T *__restrict__ ay, T *__restrict__ az,
-                               const T *__restrict__ beta_pcon, long n) {
+__global__ void k_rhs_ab2(T *__restrict__ Rx, T *__restrict__ Ry, T *__restrict__ Rz, const T *__restrict__ dx_,
+                          const T *__restrict__ dy_, const T *__restrict__ dz_, T *__restrict__ ax,
+                          T *__restrict__ ay, T *__restrict__ az, const T *__restrict__ beta_pcon,
+                          T *__restrict__ Fx, T *__restrict__ Fy, T *__restrict__ Fz, int first, T *__restrict__ u,
+                          T *__restrict__ v, T *__restrict__ w, long n, T dt, T force, T mpf) {
   long i = gtid();
   if (i >= n) return;
-  Rx[i] = -Rx[i] - dx_[i];
-  Ry[i] = -Ry[i] - dy_[i];
+  const T rx = -Rx[i] - dx_[i];
+  const T ry = -Ry[i] - dy_[i];
   T rz = -Rz[i] - dz_[i];
   if (beta_pcon) rz = rz + beta_pcon[i];
+  Rx[i] = rx;
+  Ry[i] = ry;
   Rz[i] = rz;
-  ax[i] = -dx_[i];
+  ax[i] = -dx_[i] + mpf;
   ay[i] = -dy_[i];
   az[i] = -dz_[i];
-}
-
-// main.f90:716-728 AB2 predictor (+ dudt += mean_p_force)
-template <typename T>
-__global__ void k_ab2(T *__restrict__ u, T *__restrict__ v, T *__restrict__ w, const T *__restrict__ Rx,
-                      const T *__restrict__ Ry, const T *__restrict__ Rz, const T *__restrict__ Fx,
-                      const T *__restrict__ Fy, const T *__restrict__ Fz, T *__restrict__ dudt, long n,
-                      T dt, T force, T mpf) {
-  long i = gtid();
-  if (i >= n) return;
+  T fx, fy, fz;
+  if (first) {
+    fx = rx; fy = ry; fz = rz;
+    Fx[i] = rx; Fy[i] = ry; Fz[i] = rz;
+  } else {
+    fx = Fx[i]; fy = Fy[i]; fz = Fz[i];
+  }
   const T t1 = T(1.5), t2 = T(1) - T(1.5);
-  u[i] = u[i] + dt * (t1 * Rx[i] + t2 * Fx[i] + force);
-  v[i] = v[i] + dt * (t1 * Ry[i] + t2 * Fy[i]);
-  w[i] = w[i] + dt * (t1 * Rz[i] + t2 * Fz[i]);
-  dudt[i] = dudt[i] + mpf;
+  u[i] = u[i] + dt * (t1 * rx + t2 * fx + force);
+  v[i] = v[i] + dt * (t1 * ry + t2 * fy);
+  w[i] = w[i] + dt * (t1 * rz + t2 * fz);
 }
 
-// main.f90:789-803: dpdz from p (binary64), RHS -= grad p, dudt -= grad p
+// main.f90:789-803 (dpdz from p in binary64, RHS -= grad p, dudt -= grad p) and the
+// velocity correction of forcing.f90:145-191 in one pass
 template <typename T>
-__global__ void k_dpdz_rhs(const double *__restrict__ p, T *__restrict__ dpdz, const T *__restrict__ dpdx,
-                           const T *__restrict__ dpdy, T *__restrict__ Rx, T *__restrict__ Ry,
-                           T *__restrict__ Rz, T *__restrict__ ax, T *__restrict__ ay, T *__restrict__ az,
-                           Geo g, double dz) {
+__global__ void k_dpdz_project(const double *__restrict__ p, T *__restrict__ dpdz, const T *__restrict__ dpdx,
+                               const T *__restrict__ dpdy, T *__restrict__ Rx, T *__restrict__ Ry,
+                               T *__restrict__ Rz, T *__restrict__ ax, T *__restrict__ ay, T *__restrict__ az,
+                               T *__restrict__ u, T *__restrict__ v, T *__restrict__ w, Geo g, double dz, T dt) {
   long i = gtid();
   long nv = g.np * g.nz;
   if (i >= nv) return;
   int k = (int)(i / g.np), x = (int)(i % g.ld);
   if (k == g.nz - 1) { dpdz[i] = (T)LP_BOGUS; return; }
-  T d = (x < g.nx) ? (T)((p[i + g.np] - p[i]) / dz) : T(0);
+  const T d = (x < g.nx) ? (T)((p[i + g.np] - p[i]) / dz) : T(0);
+  const T px = dpdx[i], py = dpdy[i];
   dpdz[i] = d;
-  Rx[i] -= dpdx[i];
-  Ry[i] -= dpdy[i];
+  Rx[i] -= px;
+  Ry[i] -= py;
   Rz[i] -= d;
-  ax[i] -= dpdx[i];
-  ay[i] -= dpdy[i];
+  ax[i] -= px;
+  ay[i] -= py;
   az[i] -= d;
+  if (x >= g.nx) return;
+  const T t1 = T(1.5);
+  u[i] = (u[i] + dt * (-t1 * px + T(0)));
+  v[i] = (v[i] + dt * (-t1 * py + T(0)));
+  if (k >= 1) w[i] = (w[i] + dt * (-t1 * d + T(0)));
 }
 
 // forcing.f90:145-191 velocity correction
