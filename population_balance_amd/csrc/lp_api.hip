@@ -109,6 +109,7 @@ struct lp_ctx {
   // rounded up to 16, so that every row starts on a 128-byte boundary.
   long sp = 0, sb = 0;
   void *phat = nullptr;       // binary64 p-hat (0..nz) in the sp layout
+  void *uvw_spec[3] = {};     // x-spectra of the filtered u, v, w left by the step's filt_da
   void *cvb = nullptr;        // convec: y-padded half spectra of u1 u2 u3 w1 w2 w3 [6][nz][ny2][cvp_pitch]
   void *cvp = nullptr;        // convec: half spectra of the 3/2-grid products [3][nz][ny2][cvp_pitch]
   long cv_pitch = 0;          // kx columns kept on the 3/2 grid (nx/2) rounded up to 16
@@ -438,6 +439,7 @@ extern "C" int lp_destroy(lp_ctx *c) {
   for (auto p : c->spec) fr(p);
   for (auto p : c->specb) fr(p);
   fr(c->cvb); fr(c->cvp);
+  for (auto p : c->uvw_spec) fr(p);
   for (auto p : c->tmp) fr(p);
   for (auto p : c->S) fr(p);
   fr(c->lspec); fr(c->lbar); fr(c->lhat);
@@ -542,6 +544,7 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
   c->cv_pitch = pad16(g.nx / 2);
   const size_t cv_field = (size_t)c->cv_pitch * g.ny2 * g.nz * 2 * fe;
   CK(dalloc(&c->cvb, 6 * cv_field));
+  for (auto &p : c->uvw_spec) CK(dalloc(&p, spec_b));
   CK(dalloc(&c->cvp, 3 * cv_field));
   for (auto &p : c->tmp) CK(dalloc(&p, vol_b));
   for (auto &p : c->S) CK(dalloc(&p, vol_b));
@@ -720,11 +723,17 @@ static int from_big(lp_ctx *c, const void *fb, int nlev, double scale, void *out
 // ------------------------------------------------------------------------------------
 // routines on the context state
 // ------------------------------------------------------------------------------------
-static int do_filt_da(lp_ctx *c, void *f, void *dfdx, void *dfdy, int nlev) {
+// keep (optional): receives the x-spectrum of the filtered f (kx < lh), which the
+// convection's 3/2 padding of the same field starts from
+static int do_filt_da(lp_ctx *c, void *f, void *dfdx, void *dfdy, int nlev, void *keep = nullptr) {
   double cst = 1.0 / ((double)c->g.nx * c->g.ny);
-  const int m[3] = {M_ONE, M_IKX, M_IKY};
-  void *outs[3] = {f, dfdx, dfdy};
-  return spectral_apply(c, f, nlev, cst, 3, m, outs);
+  RCHK(r2c_small(c, f, nlev, cst, c->spec[0]));
+  void *s0 = keep ? keep : c->spec[0];
+  ColOut o[3] = {{s0, M_ONE}, {c->spec[1], M_IKX}, {c->spec[2], M_IKY}};
+  RCHK(col_small(c, c->spec[0], nlev, 1, 1, 3, o));
+  RCHK(c2r_small(c, s0, c->g.lh, nlev, f, c->fdt));
+  RCHK(c2r_small(c, c->spec[1], c->g.lh, nlev, dfdx, c->fdt));
+  return c2r_small(c, c->spec[2], c->g.lh, nlev, dfdy, c->fdt);
 }
 
 static int do_ddz_uv(lp_ctx *c, void *dfdz, const void *f) {
@@ -945,40 +954,44 @@ static int do_divstress_all(lp_ctx *c) {
   return LP_OK;
 }
 
-// dealias1.f90 up to the x back-transform: r2c of a small field, then the column pass
-// that pads it to ny2 and transforms it back along y, into field q of c->cvb
-static int pad_y(lp_ctx *c, const void *f, int q) {
+// dealias1.f90 from the x-spectrum on: the column pass that pads it to ny2 and transforms
+// it back along y, into field q of c->cvb; scale multiplies the padded spectrum
+static int pad_y(lp_ctx *c, const void *spec, int q, double scale = 1.0) {
   const lp_grid &g = c->g;
-  const double cst = 1.0 / ((double)g.nx * g.ny);
-  RCHK(r2c_small(c, f, g.nz, cst, c->spec[0]));
   const long ps = c->sp * g.ny, pb = c->cv_pitch * g.ny2;
   const size_t fe = esz_of(c->fdt);
   ColOut o{(char *)c->cvb + (size_t)q * pb * g.nz * 2 * fe, M_ONE};
+  o.scale = scale;
   prof_begin(c, LP_K_COL_PAD);
-  HIPCHK(col_op(c->fdt, g.ny, g.ny2, 1, 1, 1, c->spec[0], c->sp, ps, g.nx / 2, g.nz, 1, &o, c->cv_pitch, pb,
-                c->tab, c->s));
+  HIPCHK(col_op(c->fdt, g.ny, g.ny2, 1, 1, 1, spec, c->sp, ps, g.nx / 2, g.nz, 1, &o, c->cv_pitch, pb, c->tab,
+                c->s));
   prof_end(c, LP_K_COL_PAD, bytes_col(g.ny, g.ny2, g.nx / 2, g.nz, 1, fe));
   return LP_OK;
 }
 
-static int do_convec(lp_ctx *c) {
+// kept: c->uvw_spec hold the x-spectra of the current (filtered) u, v, w, as the step's
+// filt_da leaves them; otherwise they are transformed here
+static int do_convec(lp_ctx *c, bool kept) {
   const lp_grid &gg = c->g;
   Geo g = geo(c);
-  long nv = g.np * g.nz;
   const size_t fe = esz_of(c->fdt);
+  const double cst = 1.0 / ((double)gg.nx * gg.ny);
   // convec.f90:76-98
-  RCHK(pad_y(c, c->f[LP_U], 0));
-  RCHK(pad_y(c, c->f[LP_V], 1));
-  RCHK(pad_y(c, c->f[LP_W], 2));
-  // convec.f90:100-133
-  FDISPATCH(c, T, hipLaunchKernelGGL(k_vort<T>, grid1(nv), dim3(256), 0, c->s, (const T *)c->f[LP_DUDY],
-                                     (const T *)c->f[LP_DUDZ], (const T *)c->f[LP_DVDX], (const T *)c->f[LP_DVDZ],
-                                     (const T *)c->f[LP_DWDX], (const T *)c->f[LP_DWDY], (T *)c->tmp[0],
-                                     (T *)c->tmp[1], (T *)c->tmp[2], g));
-  HIPCHK(hipGetLastError());
-  RCHK(pad_y(c, c->tmp[0], 3));
-  RCHK(pad_y(c, c->tmp[1], 4));
-  RCHK(pad_y(c, c->tmp[2], 5));
+  const int vel[3] = {LP_U, LP_V, LP_W};
+  for (int q = 0; q < 3; ++q) {
+    if (!kept) RCHK(r2c_small(c, c->f[vel[q]], gg.nz, cst, c->uvw_spec[q]));
+    // filt_da's spectrum went through an unnormalised y round trip: ny times the r2c
+    RCHK(pad_y(c, c->uvw_spec[q], q, kept ? 1.0 / gg.ny : 1.0));
+  }
+  // convec.f90:100-133: each vorticity component formed inside its x transform
+  const int va[3] = {LP_DWDY, LP_DWDX, LP_DVDX}, vb[3] = {LP_DVDZ, LP_DUDZ, LP_DUDY};
+  for (int q = 0; q < 3; ++q) {
+    prof_begin(c, LP_K_ROW_R2C);
+    HIPCHK(row_r2c_vort(c->fdt, gg.nx / 2, q, c->f[va[q]], c->f[vb[q]], gg.ld, plane_el(c), gg.ny, gg.nz, cst,
+                        c->spec[0], c->sp, c->sp * gg.ny, gg.nx / 2, c->s));
+    prof_end(c, LP_K_ROW_R2C, bytes_r2c(gg.nx, gg.ny, gg.nz, gg.nx / 2, fe) + (double)vol_el(c) * fe);
+    RCHK(pad_y(c, c->spec[0], 3 + q));
+  }
   // convec.f90:140-158, 180-221, 241-279 products (normalisation folded in) between the
   // x back- and forward transforms of the 3/2 grid, kx < nx/2 kept (dealias2)
   const double cb = 1.0 / ((double)gg.nx2 * gg.ny2);
@@ -1219,9 +1232,9 @@ static int step_once(lp_ctx *c) {
   long n1 = g.np * (g.nz - 1);
   // :252-254 RHS_f = RHS: the buffers exchange roles; convec rewrites every level of RHS
   for (int q = 0; q < 3; ++q) std::swap(c->f[LP_RHSX + q], c->f[LP_RHSX_F + q]);
-  RCHK(do_filt_da(c, c->f[LP_U], c->f[LP_DUDX], c->f[LP_DUDY], gg.nz));  // :277-279
-  RCHK(do_filt_da(c, c->f[LP_V], c->f[LP_DVDX], c->f[LP_DVDY], gg.nz));
-  RCHK(do_filt_da(c, c->f[LP_W], c->f[LP_DWDX], c->f[LP_DWDY], gg.nz));
+  RCHK(do_filt_da(c, c->f[LP_U], c->f[LP_DUDX], c->f[LP_DUDY], gg.nz, c->uvw_spec[0]));  // :277-279
+  RCHK(do_filt_da(c, c->f[LP_V], c->f[LP_DVDX], c->f[LP_DVDY], gg.nz, c->uvw_spec[1]));
+  RCHK(do_filt_da(c, c->f[LP_W], c->f[LP_DWDX], c->f[LP_DWDY], gg.nz, c->uvw_spec[2]));
   RCHK(do_ddz_uv(c, c->f[LP_DUDZ], c->f[LP_U]));  // :284-288
   RCHK(do_ddz_uv(c, c->f[LP_DVDZ], c->f[LP_V]));
   RCHK(do_ddz_w(c, c->f[LP_DWDZ], c->f[LP_W]));
@@ -1238,7 +1251,7 @@ static int step_once(lp_ctx *c) {
     }
   }
   RCHK(do_divstress_all(c));  // :394-397
-  RCHK(do_convec(c));  // :419
+  RCHK(do_convec(c, true));  // :419 (filt_da above left the u, v, w spectra)
   // :433-439, 458-460, 577-584, 716-728
   const bool first = c->jt == 1 && !cf.initu;
   double force = cf.use_mean_p_force ? cf.mean_p_force : 0.0;
@@ -1322,7 +1335,7 @@ extern "C" int lp_prof_read(lp_ctx *c, double *total_ms, int64_t *launches, doub
 STATE_ENTRY(lp_wallstress, do_wallstress(c))
 STATE_ENTRY(lp_sgs_stag, do_sgs_stag(c))
 STATE_ENTRY(lp_interpolag_sdep, do_interpolag(c))
-STATE_ENTRY(lp_convec, do_convec(c))
+STATE_ENTRY(lp_convec, do_convec(c, false))
 STATE_ENTRY(lp_project, do_project(c))
 
 extern "C" int lp_lagrange_sdep(lp_ctx *c) {

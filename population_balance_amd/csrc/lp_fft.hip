@@ -18,12 +18,18 @@ static constexpr int kRowBlock = 256;
 // ---------------------------------------------------------------------------------
 // row_r2c: real rows of 2M values -> M+1 complex (half-length complex FFT + split)
 // ---------------------------------------------------------------------------------
-template <typename T, int M>
+// PRO selects the input: 0 the rows of `in`; 1..3 a vorticity component formed from two
+// derivative volumes a, b (convec.f90:100-133): 1 w_y - v_z (a = dwdy, b = dvdz), 2
+// u_z - w_x (a = dwdx, b = dudz), 3 v_x - u_y (a = dvdx, b = dudy); on plane 0 (jz = 1)
+// w_y and w_x are averaged with plane 1 onto the uvp node.  a, b have in's layout.
+template <typename T, int M, int PRO = 0>
 __global__ __launch_bounds__(kRowBlock) void k_row_r2c(const T *__restrict__ in, long in_pitch, long in_plane,
                                                        int rows, int nplanes, T scale, cx<T> *__restrict__ out,
                                                        long out_pitch, long out_plane, int ncols_out,
                                                        const cx<T> *__restrict__ twM,
-                                                       const cx<T> *__restrict__ twN) {
+                                                       const cx<T> *__restrict__ twN,
+                                                       const T *__restrict__ pa = nullptr,
+                                                       const T *__restrict__ pb = nullptr) {
   constexpr int P = row_lanes(M);
   using F = Fft4<T, M, P, -1>;
   constexpr int Q = F::Q;
@@ -37,11 +43,34 @@ __global__ __launch_bounds__(kRowBlock) void k_row_r2c(const T *__restrict__ in,
   cx<T> *sc = lds + r * SS;
   using v2 = typename V2<T>::t;
   cx<T> v[Q];
-  const T *src = in + plane * in_plane + y * in_pitch;
+  if constexpr (PRO == 0) {
+    const T *src = in + plane * in_plane + y * in_pitch;
 #pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    v2 p = valid ? *reinterpret_cast<const v2 *>(src + 2 * (t + P * q)) : v2{0, 0};
-    v[q] = mk<T>(p.x * scale, p.y * scale);
+    for (int q = 0; q < Q; ++q) {
+      v2 p = valid ? *reinterpret_cast<const v2 *>(src + 2 * (t + P * q)) : v2{0, 0};
+      v[q] = mk<T>(p.x * scale, p.y * scale);
+    }
+  } else {
+    const long o = plane * in_plane + y * in_pitch;
+    const long up = (PRO != 3 && plane == 0) ? in_plane : 0;  // clamped: same row above jz = 1
+    const T h = T(0.5);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const long x = o + 2 * (t + P * q);
+      const v2 a0 = *reinterpret_cast<const v2 *>(pa + x);
+      const v2 b0 = *reinterpret_cast<const v2 *>(pb + x);
+      v2 r;
+      if constexpr (PRO == 3) {
+        r = v2{a0.x - b0.x, a0.y - b0.y};
+      } else {
+        const v2 a1 = *reinterpret_cast<const v2 *>(pa + x + up);
+        const v2 av{h * (a0.x + a1.x), h * (a0.y + a1.y)};
+        const v2 aa = plane == 0 ? av : a0;
+        r = PRO == 1 ? v2{aa.x - b0.x, aa.y - b0.y} : v2{b0.x - aa.x, b0.y - aa.y};
+      }
+      r = valid ? r : v2{0, 0};
+      v[q] = mk<T>(r.x * scale, r.y * scale);
+    }
   }
   F::stage1(v, t, sc, twM);
   slot_sync<P>();
@@ -164,6 +193,7 @@ struct ColSlot {
   int mult;
   int tpitch;  // row length of the multiplier tables (lh of the small grid)
   int ncols;   // columns stored (kx < ncols)
+  double scale;  // factor on the padded / truncated spectrum (padmode 1, 2)
 };
 struct ColParams {
   ColSlot outs[3];
@@ -263,6 +293,8 @@ __global__ __launch_bounds__(col_tx<T>() * kColLanes) void k_col_op(
       }
     }
     if constexpr (PM != 0) {
+      const T osc = (T)Pr.outs[jo].scale;
+      const bool scaled = Pr.outs[jo].scale != 1.0;
 #pragma unroll
       for (int q = 0; q < QO; ++q) {
         const int yo = t + PL * q;
@@ -274,7 +306,7 @@ __global__ __launch_bounds__(col_tx<T>() * kColLanes) void k_col_op(
           if (yo < NYO / 2) x = sp[q];
           else if (yo > NYO / 2) x = sp[q + DQ];
         }
-        v[q] = x;
+        v[q] = scaled ? mk<T>(x.x * osc, x.y * osc) : x;
       }
     }
     // addresses are formed from per-output values so that they are not hoisted out of
@@ -407,19 +439,46 @@ bool fft_len_supported(int n) {
   }
 }
 
-template <typename T, int M>
+template <typename T, int M, int PRO = 0>
 static hipError_t launch_r2c(const void *in, long in_pitch, long in_plane, int rows, int nplanes,
                              double scale, void *out, long out_pitch, long out_plane, int ncols_out,
-                             hipStream_t s, Dt dt) {
+                             hipStream_t s, Dt dt, const void *pa = nullptr, const void *pb = nullptr) {
   const void *twM = tw_ptr(M, dt), *twN = tw_ptr(2 * M, dt);
   if (!twM || !twN) return hipErrorInvalidValue;
   constexpr int RPB = kRowBlock / row_lanes(M);
   long total = (long)rows * nplanes;
   dim3 grid((unsigned)((total + RPB - 1) / RPB));
-  hipLaunchKernelGGL((k_row_r2c<T, M>), grid, dim3(kRowBlock), 0, s, (const T *)in, in_pitch, in_plane, rows,
+  hipLaunchKernelGGL((k_row_r2c<T, M, PRO>), grid, dim3(kRowBlock), 0, s, (const T *)in, in_pitch, in_plane, rows,
                      nplanes, (T)scale, (cx<T> *)out, out_pitch, out_plane, ncols_out,
-                     (const cx<T> *)twM, (const cx<T> *)twN);
+                     (const cx<T> *)twM, (const cx<T> *)twN, (const T *)pa, (const T *)pb);
   return hipGetLastError();
+}
+
+hipError_t row_r2c_vort(Dt dt, int M, int comp, const void *a, const void *b, long in_pitch, long in_plane,
+                        int rows, int nplanes, double scale, void *out, long out_pitch, long out_plane,
+                        int ncols_out, hipStream_t s) {
+  if (!a || !b || comp < 0 || comp > 2 || nplanes < 2) return hipErrorInvalidValue;
+#define CASE(m)                                                                                           \
+  case m:                                                                                                 \
+    if (dt == F64) {                                                                                      \
+      if (comp == 0) return launch_r2c<double, m, 1>(nullptr, in_pitch, in_plane, rows, nplanes, scale,   \
+                                                     out, out_pitch, out_plane, ncols_out, s, dt, a, b);  \
+      if (comp == 1) return launch_r2c<double, m, 2>(nullptr, in_pitch, in_plane, rows, nplanes, scale,   \
+                                                     out, out_pitch, out_plane, ncols_out, s, dt, a, b);  \
+      return launch_r2c<double, m, 3>(nullptr, in_pitch, in_plane, rows, nplanes, scale, out, out_pitch,  \
+                                      out_plane, ncols_out, s, dt, a, b);                                 \
+    }                                                                                                     \
+    if (comp == 0) return launch_r2c<float, m, 1>(nullptr, in_pitch, in_plane, rows, nplanes, scale, out, \
+                                                  out_pitch, out_plane, ncols_out, s, dt, a, b);          \
+    if (comp == 1) return launch_r2c<float, m, 2>(nullptr, in_pitch, in_plane, rows, nplanes, scale, out, \
+                                                  out_pitch, out_plane, ncols_out, s, dt, a, b);          \
+    return launch_r2c<float, m, 3>(nullptr, in_pitch, in_plane, rows, nplanes, scale, out, out_pitch,     \
+                                   out_plane, ncols_out, s, dt, a, b);
+  switch (M) {
+    LP_SMALL_ROW_LENGTHS(CASE)
+    default: return hipErrorInvalidValue;
+  }
+#undef CASE
 }
 
 hipError_t row_r2c(Dt dt, int M, const void *in, long in_pitch, long in_plane, int rows, int nplanes,
@@ -508,12 +567,13 @@ static hipError_t launch_col(int padmode, int fwd, int inv, const void *in, long
   P.nout = nout;
   for (int j = 0; j < 3; ++j) {
     if (j >= nout) {
-      P.outs[j] = ColSlot{nullptr, 0, 0, 0, 0, 0};
+      P.outs[j] = ColSlot{nullptr, 0, 0, 0, 0, 0, 1.0};
       continue;
     }
     const ColOut &o = outs[j];
+    if (padmode == 0 && o.scale != 1.0) return hipErrorInvalidValue;
     P.outs[j] = ColSlot{o.ptr, o.pitch ? o.pitch : out_pitch, o.plane ? o.plane : out_plane, o.mult, tab.lh,
-                        o.ncols ? o.ncols : ncols};
+                        o.ncols ? o.ncols : ncols, o.scale};
   }
   dim3 grid((unsigned)((ncols + TX - 1) / TX), (unsigned)nplanes);
   hipLaunchKernelGGL((k_col_op<T, NYI, NYO>), grid, dim3(TX * kColLanes), 0, s, (const cx<T> *)in, in_pitch,

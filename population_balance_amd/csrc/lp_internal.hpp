@@ -37,12 +37,20 @@ hipError_t row_c2r(Dt dtc, Dt dto, int M, const void *in, long in_pitch, long in
                    int ncols_in, int rows, int nplanes, void *out, long out_pitch,
                    long out_plane, hipStream_t s);
 
+// row_r2c of a vorticity component formed from two derivative volumes (comp 0: w_y - v_z
+// from a = dwdy, b = dvdz; 1: u_z - w_x from a = dwdx, b = dudz; 2: v_x - u_y from
+// a = dvdx, b = dudy), the x-derivatives of w averaged onto the uvp node on plane 0.
+hipError_t row_r2c_vort(Dt dt, int M, int comp, const void *a, const void *b, long in_pitch, long in_plane,
+                        int rows, int nplanes, double scale, void *out, long out_pitch, long out_plane,
+                        int ncols_out, hipStream_t s);
+
 struct ColOut {
   void *ptr;
   int mult;
   long pitch = 0;  // overrides of the call's out_pitch / out_plane / ncols (0: keep)
   long plane = 0;
   int ncols = 0;
+  double scale = 1.0;  // padmode 1 and 2 only: factor on the output spectrum
 };
 
 // Column pass over ncols kx-columns of nplanes spectra:

@@ -348,25 +348,6 @@ __device__ __forceinline__ void relax_point(double LM, double MM, double QN, dou
   beta_out = (T)b;
 }
 
-// convec.f90:100-119 vorticity (jz = 1 special case on uvp nodes)
-template <typename T>
-__global__ void k_vort(const T *__restrict__ dudy, const T *__restrict__ dudz, const T *__restrict__ dvdx,
-                       const T *__restrict__ dvdz, const T *__restrict__ dwdx, const T *__restrict__ dwdy,
-                       T *__restrict__ vx, T *__restrict__ vy, T *__restrict__ vz, Geo g) {
-  long i = gtid();
-  if (i >= g.np * g.nz) return;
-  int k = (int)(i / g.np), x = (int)(i % g.ld);
-  if (x >= g.nx) return;
-  if (k == 0) {
-    vx[i] = T(0.5) * (dwdy[i] + dwdy[i + g.np]) - dvdz[i];
-    vy[i] = dudz[i] - T(0.5) * (dwdx[i] + dwdx[i + g.np]);
-  } else {
-    vx[i] = dwdy[i] - dvdz[i];
-    vy[i] = dudz[i] - dwdx[i];
-  }
-  vz[i] = dvdx[i] - dudy[i];
-}
-
 // convec.f90:138-158, 180-221, 241-279: products on the 3/2 grid, levels 1..nz-1
 template <typename T>
 __global__ void k_convec_prod(const T *__restrict__ u1, const T *__restrict__ u2, const T *__restrict__ u3,
