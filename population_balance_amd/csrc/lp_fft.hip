@@ -14,6 +14,26 @@
 namespace lp {
 
 static constexpr int kRowBlock = 256;
+static constexpr int kR2cBlock = 128;  // r2c: persistent blocks of 128 threads
+
+template <typename T> __host__ __device__ constexpr bool r2c_persistent() { return sizeof(T) == 4; }
+
+// grid of the r2c pass: persistent (8 blocks per compute unit, at most one per group) or
+// one block per group
+static unsigned r2c_grid(long ngroups, bool persistent) {
+  if (!persistent) return (unsigned)ngroups;
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+      cus = n;
+    else
+      cus = 256;
+  }
+  const long g = 8L * cus;
+  return (unsigned)(ngroups < g ? ngroups : g);
+}
 
 // ---------------------------------------------------------------------------------
 // row_r2c: real rows of 2M values -> M+1 complex (half-length complex FFT + split)
@@ -23,7 +43,7 @@ static constexpr int kRowBlock = 256;
 // u_z - w_x (a = dwdx, b = dudz), 3 v_x - u_y (a = dvdx, b = dudy); on plane 0 (jz = 1)
 // w_y and w_x are averaged with plane 1 onto the uvp node.  a, b have in's layout.
 template <typename T, int M, int PRO = 0>
-__global__ __launch_bounds__(kRowBlock) void k_row_r2c(const T *__restrict__ in, long in_pitch, long in_plane,
+__global__ __launch_bounds__(kR2cBlock) void k_row_r2c(const T *__restrict__ in, long in_pitch, long in_plane,
                                                        int rows, int nplanes, T scale, cx<T> *__restrict__ out,
                                                        long out_pitch, long out_plane, int ncols_out,
                                                        const cx<T> *__restrict__ twM,
@@ -33,68 +53,89 @@ __global__ __launch_bounds__(kRowBlock) void k_row_r2c(const T *__restrict__ in,
   constexpr int P = row_lanes(M);
   using F = Fft4<T, M, P, -1>;
   constexpr int Q = F::Q;
-  constexpr int RB = kRowBlock / P;
+  constexpr int RB = kR2cBlock / P;
   constexpr int SS = slot_stride(F::SCR > M ? F::SCR : M);
   __shared__ cx<T> lds[RB * SS];
   const int t = threadIdx.x % P, r = threadIdx.x / P;
-  const long grow = (long)blockIdx.x * RB + r;
-  const bool valid = grow < (long)rows * nplanes;
-  const long plane = valid ? grow / rows : 0, y = valid ? grow % rows : 0;
+  const long total = (long)rows * nplanes;
+  const long ngroups = (total + RB - 1) / RB;
   cx<T> *sc = lds + r * SS;
   using v2 = typename V2<T>::t;
-  cx<T> v[Q];
-  if constexpr (PRO == 0) {
-    const T *src = in + plane * in_plane + y * in_pitch;
+  // input of row group g (rows beyond the last read row 0 and are not stored)
+  auto load = [&](long g, cx<T> (&v)[Q]) {
+    const long grow = g * RB + r;
+    const bool valid = grow < total;
+    const long plane = valid ? grow / rows : 0, y = valid ? grow % rows : 0;
+    if constexpr (PRO == 0) {
+      const T *src = in + plane * in_plane + y * in_pitch;
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      v2 p = valid ? *reinterpret_cast<const v2 *>(src + 2 * (t + P * q)) : v2{0, 0};
-      v[q] = mk<T>(p.x * scale, p.y * scale);
-    }
-  } else {
-    const long o = plane * in_plane + y * in_pitch;
-    const long up = (PRO != 3 && plane == 0) ? in_plane : 0;  // clamped: same row above jz = 1
-    const T h = T(0.5);
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const long x = o + 2 * (t + P * q);
-      const v2 a0 = *reinterpret_cast<const v2 *>(pa + x);
-      const v2 b0 = *reinterpret_cast<const v2 *>(pb + x);
-      v2 r;
-      if constexpr (PRO == 3) {
-        r = v2{a0.x - b0.x, a0.y - b0.y};
-      } else {
-        const v2 a1 = *reinterpret_cast<const v2 *>(pa + x + up);
-        const v2 av{h * (a0.x + a1.x), h * (a0.y + a1.y)};
-        const v2 aa = plane == 0 ? av : a0;
-        r = PRO == 1 ? v2{aa.x - b0.x, aa.y - b0.y} : v2{b0.x - aa.x, b0.y - aa.y};
+      for (int q = 0; q < Q; ++q) {
+        const v2 p = *reinterpret_cast<const v2 *>(src + 2 * (t + P * q));
+        v[q] = mk<T>(p.x * scale, p.y * scale);
       }
-      r = valid ? r : v2{0, 0};
-      v[q] = mk<T>(r.x * scale, r.y * scale);
-    }
-  }
-  F::stage1(v, t, sc, twM);
-  slot_sync<P>();
-  cx<T> a[F::G][P];
-  bool ok[F::G];
+    } else {
+      const long o = plane * in_plane + y * in_pitch;
+      const long up = (PRO != 3 && plane == 0) ? in_plane : 0;  // clamped: same row above jz = 1
+      const T h = T(0.5);
 #pragma unroll
-  for (int j = 0; j < F::G; ++j) ok[j] = F::stage2(j, t, sc, a[j]);
-  slot_sync<P>();
-#pragma unroll
-  for (int j = 0; j < F::G; ++j)
-    if (ok[j]) {
-#pragma unroll
-      for (int k1 = 0; k1 < P; ++k1) sc[t + P * j + Q * k1] = a[j][k1];
+      for (int q = 0; q < Q; ++q) {
+        const long x = o + 2 * (t + P * q);
+        const v2 a0 = *reinterpret_cast<const v2 *>(pa + x);
+        const v2 b0 = *reinterpret_cast<const v2 *>(pb + x);
+        v2 rr;
+        if constexpr (PRO == 3) {
+          rr = v2{a0.x - b0.x, a0.y - b0.y};
+        } else {
+          const v2 a1 = *reinterpret_cast<const v2 *>(pa + x + up);
+          const v2 av{h * (a0.x + a1.x), h * (a0.y + a1.y)};
+          const v2 aa = plane == 0 ? av : a0;
+          rr = PRO == 1 ? v2{aa.x - b0.x, aa.y - b0.y} : v2{b0.x - aa.x, b0.y - aa.y};
+        }
+        v[q] = mk<T>(rr.x * scale, rr.y * scale);
+      }
     }
-  slot_sync<P>();
-  // split: X[k] = (Z[k] + conj Z[M-k])/2 - i/2 w^k (Z[k] - conj Z[M-k]), w = exp(-2 pi i/2M)
-  if (valid) {
-    cx<T> *dst = out + plane * out_plane + y * out_pitch;
-    for (int k = t; k <= M && k < ncols_out; k += P) {
-      cx<T> za = sc[k % M], zb = conjg(sc[(M - k) % M]);
-      cx<T> e = za + zb, o = za - zb;
-      cx<T> wo = cmul(twN[k % (2 * M)], o);
-      dst[k] = mk<T>(T(0.5) * (e.x + wo.y), T(0.5) * (e.y - wo.x));
+  };
+  // binary32: persistent blocks, the next group's loads issued before this group's
+  // transform; binary64 (twice the registers per value): one group per block
+  constexpr bool PF = r2c_persistent<T>();
+  long g = blockIdx.x;
+  cx<T> v[Q];
+  if (PF && g < ngroups) load(g, v);
+  for (; g < ngroups; g += gridDim.x) {
+    cx<T> vn[Q];
+    if (!PF) load(g, v);
+    else if (g + gridDim.x < ngroups) load(g + gridDim.x, vn);
+    const long grow = g * RB + r;
+    const bool valid = grow < total;
+    const long plane = valid ? grow / rows : 0, y = valid ? grow % rows : 0;
+    F::stage1(v, t, sc, twM);
+    slot_sync<P>();
+    cx<T> a[F::G][P];
+    bool ok[F::G];
+#pragma unroll
+    for (int j = 0; j < F::G; ++j) ok[j] = F::stage2(j, t, sc, a[j]);
+    slot_sync<P>();
+#pragma unroll
+    for (int j = 0; j < F::G; ++j)
+      if (ok[j]) {
+#pragma unroll
+        for (int k1 = 0; k1 < P; ++k1) sc[t + P * j + Q * k1] = a[j][k1];
+      }
+    slot_sync<P>();
+    // split: X[k] = (Z[k] + conj Z[M-k])/2 - i/2 w^k (Z[k] - conj Z[M-k]), w = exp(-2 pi i/2M)
+    if (valid) {
+      cx<T> *dst = out + plane * out_plane + y * out_pitch;
+      for (int k = t; k <= M && k < ncols_out; k += P) {
+        cx<T> za = sc[k % M], zb = conjg(sc[(M - k) % M]);
+        cx<T> e = za + zb, o = za - zb;
+        cx<T> wo = cmul(twN[k % (2 * M)], o);
+        dst[k] = mk<T>(T(0.5) * (e.x + wo.y), T(0.5) * (e.y - wo.x));
+      }
     }
+    if (!PF) break;
+    slot_sync<P>();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) v[q] = vn[q];
   }
 }
 
@@ -201,6 +242,9 @@ struct ColParams {
 };
 static constexpr int kColLanes = 16;
 template <typename T> __host__ __device__ constexpr int col_tx() { return 128 / (int)sizeof(cx<T>); }
+// LDS stride of a column slot.  The 16 lanes of a quarter wavefront are 16 (8 in binary64)
+// different columns at the same row offset, so an odd stride puts them on different banks.
+__host__ __device__ constexpr int col_slot_stride(int n) { return n | 1; }
 
 // forward DFT of length NY along a column held in the D16 layout (sp[m] = x[t + 16 m]),
 // result in the same layout; sc is the column's LDS slot (block-wide barriers)
@@ -251,7 +295,7 @@ __global__ __launch_bounds__(col_tx<T>() * kColLanes) void k_col_op(
   constexpr int PM = NYI == NYO ? 0 : (NYO > NYI ? 1 : 2);  // same grid / pad / truncate
   constexpr int DQ = (NYI > NYO ? NYI - NYO : NYO - NYI) / PL;
   constexpr int NS = FI::SCR > FO::SCR ? (FI::SCR > NYI ? FI::SCR : NYI) : (FO::SCR > NYI ? FO::SCR : NYI);
-  constexpr int SS = slot_stride(NS);
+  constexpr int SS = col_slot_stride(NS);
   __shared__ cx<T> lds[TX * SS];
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const int kx = blockIdx.x * TX + c;
@@ -352,7 +396,7 @@ __global__ __launch_bounds__(col_tx<T>() * kColLanes) void k_col_div(
   constexpr int PL = kColLanes;
   using FO = Fft4<T, NY, PL, +1>;
   constexpr int Q = NY / PL;
-  constexpr int SS = slot_stride(FO::SCR > NY ? FO::SCR : NY);
+  constexpr int SS = col_slot_stride(FO::SCR > NY ? FO::SCR : NY);
   __shared__ cx<T> lds[TX * SS];
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const int kx = blockIdx.x * TX + c;
@@ -445,10 +489,11 @@ static hipError_t launch_r2c(const void *in, long in_pitch, long in_plane, int r
                              hipStream_t s, Dt dt, const void *pa = nullptr, const void *pb = nullptr) {
   const void *twM = tw_ptr(M, dt), *twN = tw_ptr(2 * M, dt);
   if (!twM || !twN) return hipErrorInvalidValue;
-  constexpr int RPB = kRowBlock / row_lanes(M);
+  constexpr int RPB = kR2cBlock / row_lanes(M);
   long total = (long)rows * nplanes;
-  dim3 grid((unsigned)((total + RPB - 1) / RPB));
-  hipLaunchKernelGGL((k_row_r2c<T, M, PRO>), grid, dim3(kRowBlock), 0, s, (const T *)in, in_pitch, in_plane, rows,
+  if (total < 1) return hipErrorInvalidValue;
+  dim3 grid(r2c_grid((total + RPB - 1) / RPB, r2c_persistent<T>()));
+  hipLaunchKernelGGL((k_row_r2c<T, M, PRO>), grid, dim3(kR2cBlock), 0, s, (const T *)in, in_pitch, in_plane, rows,
                      nplanes, (T)scale, (cx<T> *)out, out_pitch, out_plane, ncols_out,
                      (const cx<T> *)twM, (const cx<T> *)twN, (const T *)pa, (const T *)pb);
   return hipGetLastError();
