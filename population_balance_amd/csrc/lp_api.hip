@@ -128,6 +128,9 @@ struct lp_ctx {
   int nred = 0;
   int jt = 1, jt_total = 1;
   int F_LM_MM_init = 0, F_QN_NN_init = 0;
+  // u_lag, v_lag, w_lag known to hold zeros (after their reset, or as allocated); cleared
+  // by an accumulation, an upload, or handing out their device pointers
+  bool lag_vel_zero = true;
   int ncut1 = 0, ncut2 = 0;  // leading kx columns where G_test / G_test_test are nonzero
   // droplet bins (npcon > 0)
   double *d_ws = nullptr, *d_qsrc = nullptr, *d_vfac = nullptr;  // per bin
@@ -588,6 +591,7 @@ extern "C" int lp_sync(lp_ctx *c) {
 
 extern "C" int lp_field_info(lp_ctx *c, int32_t field, void **dev_ptr, size_t *bytes, int32_t *elem_size) {
   if (!c || field < 0 || field >= LP_NFIELDS) return set_err(LP_ERR_ARG, "lp_field_info: bad field %d", field);
+  if (dev_ptr && (field == LP_U_LAG || field == LP_V_LAG || field == LP_W_LAG)) c->lag_vel_zero = false;
   if (dev_ptr) *dev_ptr = c->f[field];
   if (bytes) *bytes = c->bytes[field];
   if (elem_size) *elem_size = c->esz[field];
@@ -599,6 +603,7 @@ extern "C" int lp_upload(lp_ctx *c, int32_t field, const void *host, size_t byte
   if (bytes != c->bytes[field])
     return set_err(LP_ERR_ARG, "lp_upload: field %d expects %zu bytes, got %zu", field, c->bytes[field], bytes);
   HIPCHK(hipSetDevice(c->dev));
+  if (field == LP_U_LAG || field == LP_V_LAG || field == LP_W_LAG) c->lag_vel_zero = false;
   HIPCHK(hipMemcpyAsync(c->f[field], host, bytes, hipMemcpyHostToDevice, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
   return LP_OK;
@@ -791,6 +796,7 @@ static int reset_lag_velocities(lp_ctx *c) {  // interpolag_Sdep.f90:367-369
   HIPCHK(hipMemsetAsync(c->f[LP_U_LAG], 0, vb, c->s));
   HIPCHK(hipMemsetAsync(c->f[LP_V_LAG], 0, vb, c->s));
   HIPCHK(hipMemsetAsync(c->f[LP_W_LAG], 0, vb, c->s));
+  c->lag_vel_zero = true;
   return LP_OK;
 }
 
@@ -813,7 +819,9 @@ static int do_interpolag(lp_ctx *c) {
 
 // lagrange_Sdep.f90:4-496 (with interpolag_Sdep at :120).  Needs the strain rates in
 // c->S and |S| in c->tmp[3] (k_strain).
-static int do_lagrange(lp_ctx *c) {
+// use_u: u_lag is zero and this step's accumulation is skipped, so u_lag + u == u and the
+// backtrace reads u, v, w directly; u_lag then stays zero, as the reset leaves it
+static int do_lagrange(lp_ctx *c, bool use_u = false) {
   const lp_config &cf = c->cfg;
   const lp_grid &gg = c->g;
   Geo g = geo(c);
@@ -861,11 +869,12 @@ static int do_lagrange(lp_ctx *c) {
   prof_end(c, LP_K_ROW_C2R, (double)kNumLag * gg.ny * gg.nz * (n1 + n2) * 2 * fe + vol * 4 * 8);
   // :327-434 (+ interpolag_Sdep.f90:62-365): interpolation and relaxation
   prof_begin(c, LP_K_LAG_RELAX);
-  HIPCHK(lag_relax(c->fdt, c->lcon, Fold, Fnew, c->f[LP_U_LAG], c->f[LP_V_LAG], c->f[LP_W_LAG], c->f[LP_CS_OPT2],
+  const int lv[3] = {use_u ? LP_U : LP_U_LAG, use_u ? LP_V : LP_V_LAG, use_u ? LP_W : LP_W_LAG};
+  HIPCHK(lag_relax(c->fdt, c->lcon, Fold, Fnew, c->f[lv[0]], c->f[lv[1]], c->f[lv[2]], c->f[LP_CS_OPT2],
                    c->f[LP_BETA], g, P, I, c->s));
   prof_end(c, LP_K_LAG_RELAX, vol * (4.0 * 8 + 8.0 * 8 + 5.0 * fe));
   swap_F(c);
-  RCHK(reset_lag_velocities(c));
+  if (!use_u) RCHK(reset_lag_velocities(c));
   if (P.init_lm) c->F_LM_MM_init = 1;
   if (P.init_qn) c->F_QN_NN_init = 1;
   return LP_OK;
@@ -885,16 +894,24 @@ static int do_sgs_stag(lp_ctx *c) {
   HIPCHK(hipGetLastError());
   double delta = cf.filter_size * std::pow(gg.dx * gg.dy * gg.dz, 1.0 / 3.0);
   if (cf.model == 5) {
-    FDISPATCH(c, T, hipLaunchKernelGGL(k_lag_acc<T>, grid1(nv), dim3(256), 0, c->s, (T *)c->f[LP_U_LAG],
-                                       (T *)c->f[LP_V_LAG], (T *)c->f[LP_W_LAG], (const T *)c->f[LP_U],
-                                       (const T *)c->f[LP_V], (const T *)c->f[LP_W], nv));
-    HIPCHK(hipGetLastError());
-    if (c->jt == 1 && cf.inilag) {  // sgs_stag.f90:178-180
-      FDISPATCH(c, T, hipLaunchKernelGGL(k_fill<T>, grid1(nv), dim3(256), 0, c->s, (T *)c->f[LP_CS_OPT2], nv,
-                                         (T)cf.cs2_init));
+    const bool init_cs = c->jt == 1 && cf.inilag;  // sgs_stag.f90:178-180
+    const bool lag_now = !init_cs && (c->jt >= cf.dyn_init || cf.initu) && (c->jt % cf.cs_count) == 0;
+    if (lag_now && c->lag_vel_zero) {
+      // sgs_stag.f90:130-134 then :200-201 with u_lag = 0 + u: the model reads u directly
+      RCHK(do_lagrange(c, true));
+    } else {
+      FDISPATCH(c, T, hipLaunchKernelGGL(k_lag_acc<T>, grid1(nv), dim3(256), 0, c->s, (T *)c->f[LP_U_LAG],
+                                         (T *)c->f[LP_V_LAG], (T *)c->f[LP_W_LAG], (const T *)c->f[LP_U],
+                                         (const T *)c->f[LP_V], (const T *)c->f[LP_W], nv));
       HIPCHK(hipGetLastError());
-    } else if ((c->jt >= cf.dyn_init || cf.initu) && (c->jt % cf.cs_count) == 0) {
-      RCHK(do_lagrange(c));  // sgs_stag.f90:200-201
+      c->lag_vel_zero = false;
+      if (init_cs) {
+        FDISPATCH(c, T, hipLaunchKernelGGL(k_fill<T>, grid1(nv), dim3(256), 0, c->s, (T *)c->f[LP_CS_OPT2], nv,
+                                           (T)cf.cs2_init));
+        HIPCHK(hipGetLastError());
+      } else if (lag_now) {
+        RCHK(do_lagrange(c));  // sgs_stag.f90:200-201
+      }
     }
   } else {  // model 1: Cs_opt2 = Co^2 (sgs_stag.f90:148)
     FDISPATCH(c, T, hipLaunchKernelGGL(k_fill<T>, grid1(nv), dim3(256), 0, c->s, (T *)c->f[LP_CS_OPT2], nv,

@@ -184,33 +184,35 @@ __global__ void k_nut_tau(const T *__restrict__ S11, const T *__restrict__ S12, 
 // interpolag_Sdep.f90:62-365: backtrace x - u dt and trilinear interpolation of the four
 // Lagrangian averages from the periodic (x, y) / clamped (z) copy of F at point
 // (x < nx, y, k); i = its flat index.
-template <typename T>
+// UNIT: cs == 1 (cs_count = 1), where the divisions by cs are the identity and are skipped
+template <typename T, bool UNIT = false>
 __device__ __forceinline__ void interp_point(const double *const (&F)[4], double (&out)[4], const T *__restrict__ ul,
                                              const T *__restrict__ vl, const T *__restrict__ wl, Geo g, long i,
                                              int x, int y, int k, double cs, double dt, double dx, double dy,
                                              double dz) {
   const int r = (int)(i - (long)k * g.np);
+  auto dcs = [&](double v) { return UNIT ? v : v / cs; };
   // :204-258 lag velocities on w-nodes (u_lag at jz=1 stays on the uvp node)
-  double ut = (double)ul[i] / cs, vt = (double)vl[i] / cs;
+  double ut = dcs((double)ul[i]), vt = dcs((double)vl[i]);
   double ulag, vlag, wlag;
   if (k == 0) { ulag = ut; vlag = vt; }
   else {
-    ulag = 0.5 * (ut + (double)ul[i - g.np] / cs);
-    vlag = 0.5 * (vt + (double)vl[i - g.np] / cs);
+    ulag = 0.5 * (ut + dcs((double)ul[i - g.np]));
+    vlag = 0.5 * (vt + dcs((double)vl[i - g.np]));
   }
   double xp = -ulag * dt * cs / dx;                    // :263-265
   double yp = -vlag * dt * cs / dy;
   double zp;
   if (k <= 1) {                                        // :281-291
     long i2 = (long)g.np + r;
-    double w2 = (double)wl[i2] / cs;
+    double w2 = dcs((double)wl[i2]);
     double z2 = -w2 * dt * cs / dz;
     z2 = z2 + fmin(z2, 0.0);
     z2 = fmin(1.0, z2);
     z2 = fmax(-1.0, z2);
     zp = z2;
   } else {
-    wlag = (double)wl[i] / cs;
+    wlag = dcs((double)wl[i]);
     zp = -wlag * dt * cs / dz;
   }
   int addx = signbit(xp) ? -1 : 1, addy = signbit(yp) ? -1 : 1, addz = signbit(zp) ? -1 : 1;

@@ -224,7 +224,7 @@ __global__ __launch_bounds__(256) void k_lag_c2r_contract(const cx<T> *__restric
 // lagrange_Sdep.f90:327-434 with interpolag_Sdep.f90:62-365, one thread per point: the
 // old averages are gathered at the backtraced point, relaxed with the contractions and
 // written to Fnew.
-template <typename T>
+template <typename T, bool UNIT>
 __global__ __launch_bounds__(256) void k_lag_relax(const double *__restrict__ lm, const double *__restrict__ mm,
                                                    const double *__restrict__ qn, const double *__restrict__ nn,
                                                    F4c Fold, F4 Fnew, const T *__restrict__ ul,
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(256) void k_lag_relax(const double *__restrict__ lm
   const double *const Fo[4] = {Fold.p[0], Fold.p[1], Fold.p[2], Fold.p[3]};
   double fi[4];
   if (x < g.nx) {
-    interp_point<T>(Fo, fi, ul, vl, wl, g, i, x, y, k, I.cs, I.dt, I.dx, I.dy, I.dz);
+    interp_point<T, UNIT>(Fo, fi, ul, vl, wl, g, i, x, y, k, I.cs, I.dt, I.dx, I.dy, I.dz);
   } else {  // interpolag_Sdep.f90 leaves the padding columns alone
 #pragma unroll
     for (int q = 0; q < 4; ++q) fi[q] = Fo[q][i];
@@ -329,8 +329,12 @@ static hipError_t launch_lag_relax(double *const C[4], const double *const Fold[
   F4 fn;
   for (int q = 0; q < 4; ++q) { fo.p[q] = Fold[q]; fn.p[q] = Fnew[q]; }
   const long n = g.np * g.nz;
-  hipLaunchKernelGGL((k_lag_relax<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, C[0], C[1], C[2], C[3],
-                     fo, fn, (const T *)ul, (const T *)vl, (const T *)wl, (T *)Cs, (T *)Beta, g, P, I);
+  if (I.cs == 1.0)
+    hipLaunchKernelGGL((k_lag_relax<T, true>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, C[0], C[1], C[2],
+                       C[3], fo, fn, (const T *)ul, (const T *)vl, (const T *)wl, (T *)Cs, (T *)Beta, g, P, I);
+  else
+    hipLaunchKernelGGL((k_lag_relax<T, false>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, C[0], C[1],
+                       C[2], C[3], fo, fn, (const T *)ul, (const T *)vl, (const T *)wl, (T *)Cs, (T *)Beta, g, P, I);
   return hipGetLastError();
 }
 
