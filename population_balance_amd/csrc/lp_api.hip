@@ -96,6 +96,12 @@ struct lp_ctx {
   lp_grid g;
   int dev = 0;
   hipStream_t s = nullptr;
+  // second stream of the step: the convective term runs there, beside the SGS model,
+  // between ev_fork (after wallstress) and ev_join (before the RHS assembly)
+  hipStream_t s2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  void *cv_spec[2] = {};  // convec's own spectral scratch (divstress runs meanwhile)
+  bool fork_pending = false;  // the step's convec is still to be started on s2
   Dt fdt = F64;  // field type
   void *f[LP_NFIELDS] = {};
   size_t bytes[LP_NFIELDS] = {};
@@ -455,6 +461,10 @@ extern "C" int lp_destroy(lp_ctx *c) {
   fr(c->d_ws); fr(c->d_qsrc); fr(c->d_vfac); fr(c->d_foot); fr(c->d_bins); fr(c->d_pairs);
   fr(c->ui0); fr(c->vi0); fr(c->kcell);
   for (auto e : c->ev) (void)hipEventDestroy(e);
+  for (auto p : c->cv_spec) fr(p);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->s2) (void)hipStreamDestroy(c->s2);
   if (c->s) (void)hipStreamDestroy(c->s);
   delete c;
   return LP_OK;
@@ -495,6 +505,12 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
   if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return set_err(LP_ERR_HIP, "hipStreamCreate failed");
+  }
+  if (hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+    lp_destroy(c);
+    return set_err(LP_ERR_HIP, "stream/event creation failed");
   }
   int lens[] = {g.nx / 2, g.nx, g.ny, g.nx2 / 2, g.nx2, g.ny2};
   for (int n : lens) CK(ensure_twiddle(n));
@@ -548,6 +564,7 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
   const size_t cv_field = (size_t)c->cv_pitch * g.ny2 * g.nz * 2 * fe;
   CK(dalloc(&c->cvb, 6 * cv_field));
   for (auto &p : c->uvw_spec) CK(dalloc(&p, spec_b));
+  for (auto &p : c->cv_spec) CK(dalloc(&p, spec_b));
   CK(dalloc(&c->cvp, 3 * cv_field));
   for (auto &p : c->tmp) CK(dalloc(&p, vol_b));
   for (auto &p : c->S) CK(dalloc(&p, vol_b));
@@ -819,6 +836,25 @@ static int do_interpolag(lp_ctx *c) {
 
 // lagrange_Sdep.f90:4-496 (with interpolag_Sdep at :120).  Needs the strain rates in
 // c->S and |S| in c->tmp[3] (k_strain).
+static int do_convec(lp_ctx *c, bool kept);
+
+// main.f90:419 on the second stream: convec needs u, v, w, their derivatives and the
+// wall-stress dudz/dvdz only, so it runs beside the rest of the SGS model, the droplets
+// and divstress (its scratch is its own) and is joined before the RHS assembly.  Started
+// once per step, after the last column pass of the SGS model.
+static int fork_convec(lp_ctx *c) {
+  if (!c->fork_pending) return LP_OK;
+  c->fork_pending = false;
+  HIPCHK(hipEventRecord(c->ev_fork, c->s));
+  HIPCHK(hipStreamWaitEvent(c->s2, c->ev_fork, 0));
+  std::swap(c->s, c->s2);
+  const int rc = do_convec(c, true);  // filt_da left the u, v, w spectra
+  std::swap(c->s, c->s2);
+  RCHK(rc);
+  HIPCHK(hipEventRecord(c->ev_join, c->s2));
+  return LP_OK;
+}
+
 // use_u: u_lag is zero and this step's accumulation is skipped, so u_lag + u == u and the
 // backtrace reads u, v, w directly; u_lag then stays zero, as the reset leaves it
 static int do_lagrange(lp_ctx *c, bool use_u = false) {
@@ -846,6 +882,7 @@ static int do_lagrange(lp_ctx *c, bool use_u = false) {
                   c->s));
     prof_end(c, LP_K_COL_OP, (double)kNumLag * gg.nz * gg.ny * (nc + n1 + n2) * 2 * fe);
   }
+  RCHK(fork_convec(c));  // in a step: convec overlaps the rest of the model
   double delta = cf.filter_size * std::pow(gg.dx * gg.dy * gg.dz, 1.0 / 3.0);
   LagParams P;
   P.cst = 2.0 * delta * delta;
@@ -1005,9 +1042,9 @@ static int do_convec(lp_ctx *c, bool kept) {
   for (int q = 0; q < 3; ++q) {
     prof_begin(c, LP_K_ROW_R2C);
     HIPCHK(row_r2c_vort(c->fdt, gg.nx / 2, q, c->f[va[q]], c->f[vb[q]], gg.ld, plane_el(c), gg.ny, gg.nz, cst,
-                        c->spec[0], c->sp, c->sp * gg.ny, gg.nx / 2, c->s));
+                        c->cv_spec[0], c->sp, c->sp * gg.ny, gg.nx / 2, c->s));
     prof_end(c, LP_K_ROW_R2C, bytes_r2c(gg.nx, gg.ny, gg.nz, gg.nx / 2, fe) + (double)vol_el(c) * fe);
-    RCHK(pad_y(c, c->spec[0], 3 + q));
+    RCHK(pad_y(c, c->cv_spec[0], 3 + q));
   }
   // convec.f90:140-158, 180-221, 241-279 products (normalisation folded in) between the
   // x back- and forward transforms of the 3/2 grid, kx < nx/2 kept (dealias2)
@@ -1021,12 +1058,12 @@ static int do_convec(lp_ctx *c, bool kept) {
   void *outs[3] = {c->f[LP_RHSX], c->f[LP_RHSY], c->f[LP_RHSZ]};
   const long ps = c->sp * gg.ny;
   for (int q = 0; q < 3; ++q) {
-    ColOut o{c->spec[1], M_ONE};
+    ColOut o{c->cv_spec[1], M_ONE};
     prof_begin(c, LP_K_COL_PAD);
     HIPCHK(col_op(c->fdt, gg.ny2, gg.ny, 2, 1, 1, (const char *)c->cvp + (size_t)q * pb * gg.nz * 2 * fe,
                   c->cv_pitch, pb, gg.nx / 2, gg.nz - 1, 1, &o, c->sp, ps, c->tab, c->s));
     prof_end(c, LP_K_COL_PAD, bytes_col(gg.ny2, gg.ny, gg.nx / 2, gg.nz - 1, 1, fe));
-    RCHK(c2r_small(c, c->spec[1], gg.nx / 2, gg.nz - 1, outs[q], c->fdt));
+    RCHK(c2r_small(c, c->cv_spec[1], gg.nx / 2, gg.nz - 1, outs[q], c->fdt));
   }
   // convec.f90:306-308 top level BOGUS
   for (int q = 0; q < 3; ++q) {
@@ -1256,7 +1293,13 @@ static int step_once(lp_ctx *c) {
   RCHK(do_ddz_uv(c, c->f[LP_DVDZ], c->f[LP_V]));
   RCHK(do_ddz_w(c, c->f[LP_DWDZ], c->f[LP_W]));
   RCHK(do_wallstress(c));  // :315
-  RCHK(do_sgs_stag(c));    // :327
+  c->fork_pending = true;  // convec (:419) starts on the second stream inside sgs_stag
+  const int rc_sgs = do_sgs_stag(c);  // :327
+  if (rc_sgs != LP_OK) {
+    c->fork_pending = false;
+    return rc_sgs;
+  }
+  RCHK(fork_convec(c));  // no Lagrangian step: start it here
   const bool drops = cf.drop.npcon > 0;
   if (drops) {
     if (c->jt_total >= cf.drop.pcon_init) {  // :360-372
@@ -1268,7 +1311,7 @@ static int step_once(lp_ctx *c) {
     }
   }
   RCHK(do_divstress_all(c));  // :394-397
-  RCHK(do_convec(c, true));  // :419 (filt_da above left the u, v, w spectra)
+  HIPCHK(hipStreamWaitEvent(c->s, c->ev_join, 0));
   // :433-439, 458-460, 577-584, 716-728
   const bool first = c->jt == 1 && !cf.initu;
   double force = cf.use_mean_p_force ? cf.mean_p_force : 0.0;
