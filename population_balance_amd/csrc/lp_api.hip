@@ -142,8 +142,7 @@ struct lp_ctx {
   double *d_ws = nullptr, *d_qsrc = nullptr, *d_vfac = nullptr;  // per bin
   double *d_foot = nullptr;                    // normalised source footprint (nx*ny)
   PbeBin *d_bins = nullptr;
-  PbePair *d_pairs = nullptr;
-  int npairs = 0;
+  double *d_pbeA = nullptr;  // [npc][npc] pair-transfer matrix of the breakup source
   double *ui0 = nullptr, *vi0 = nullptr;        // face velocities (ld, ny+1, nz)
   double *kcell = nullptr;                      // Kc_t on cell centres
   double kpref = 0.0, ustar3 = 0.0;
@@ -419,9 +418,17 @@ static int droplet_setup(lp_ctx *c) {
       }
       pairs.push_back(q);
     }
-  c->npairs = (int)pairs.size();
+  // the pair loop (:4356-4448) as a matrix: tr = gf(j) beta PCon(j) goes +1 into bin i,
+  // -1 out of bin j and w_lo, w_hi into the complementary drop's bins
+  std::vector<double> A((size_t)n * n, 0.0);
+  for (const PbePair &q : pairs) {
+    A[(size_t)q.i * n + q.j] += q.beta;
+    A[(size_t)q.j * n + q.j] -= q.beta;
+    if (q.lo >= 0) A[(size_t)q.lo * n + q.j] += q.beta * q.wlo;
+    if (q.hi >= 0) A[(size_t)q.hi * n + q.j] += q.beta * q.whi;
+  }
   RCHK(upload_vec((void **)&c->d_bins, bins.data(), sizeof(PbeBin) * n));
-  RCHK(upload_vec((void **)&c->d_pairs, pairs.data(), sizeof(PbePair) * std::max<size_t>(pairs.size(), 1)));
+  RCHK(upload_vec((void **)&c->d_pbeA, A.data(), sizeof(double) * A.size()));
   c->kpref = D.k_b * std::pow(2.1, 0.5) * M_PI / 4.0 * 0.84;
   c->ustar3 = std::pow(D.u_star, 3.0);
   const size_t fb = sizeof(double) * (size_t)g.ld * (g.ny + 1) * g.nz;
@@ -458,7 +465,7 @@ extern "C" int lp_destroy(lp_ctx *c) {
   fr(c->bw); fr(c->tw); fr(c->gam); fr(c->phat);
   for (auto p : c->pspec) fr(p);
   fr(c->d_err); fr(c->d_part);
-  fr(c->d_ws); fr(c->d_qsrc); fr(c->d_vfac); fr(c->d_foot); fr(c->d_bins); fr(c->d_pairs);
+  fr(c->d_ws); fr(c->d_qsrc); fr(c->d_vfac); fr(c->d_foot); fr(c->d_bins); fr(c->d_pbeA);
   fr(c->ui0); fr(c->vi0); fr(c->kcell);
   for (auto e : c->ev) (void)hipEventDestroy(e);
   for (auto p : c->cv_spec) fr(p);
@@ -1185,7 +1192,9 @@ static int do_pollen_rhs(lp_ctx *c, const double *prev) {
   P.src_k0 = D.src_k - 1;
   const double nb = (double)dg.nv * dg.npc * 8.0;
   prof_begin(c, LP_K_POLLEN_RHS);
-  FDISPATCH(c, T, hipLaunchKernelGGL(k_pollen_rhs<T>, grid1(dg.nv), dim3(256), 0, c->s, PC, RH, prev, c->ui0,
+  const int nbb = dg.npc < kPollenBinsPerBlock ? dg.npc : kPollenBinsPerBlock;
+  const dim3 pgrid((unsigned)((dg.nv + 63) / 64), (unsigned)((dg.npc + nbb - 1) / nbb));
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_pollen_rhs<T>, pgrid, dim3(64 * nbb), 0, c->s, PC, RH, prev, c->ui0,
                                      c->vi0, (const T *)c->f[LP_W], (const T *)c->f[LP_DUDT],
                                      (const T *)c->f[LP_DVDT], (const T *)c->f[LP_DWDT], c->kcell, c->d_ws,
                                      c->d_qsrc, c->d_foot, dg, P));
@@ -1194,11 +1203,14 @@ static int do_pollen_rhs(lp_ctx *c, const double *prev) {
   prof_end(c, LP_K_POLLEN_RHS, 2.0 * nb + (double)dg.nv * (8.0 + 3.0 * esz_of(c->fdt)) + 4.0 * dg.fp * gg.nz * 8.0);
   if (D.pbe) {
     prof_begin(c, LP_K_PBE);
-    size_t lds = sizeof(double) * 2 * 64 * (size_t)dg.npc;
-    FDISPATCH(c, T, hipLaunchKernelGGL(k_pbe<T>, dim3((unsigned)((dg.nv + 63) / 64)), dim3(64), lds, c->s, PC,
-                                       (double *)c->f[LP_RHS_PBE], RH, (const T *)c->f[LP_DISSIP], c->d_bins,
-                                       c->d_pairs, c->npairs, dg, c->kpref, c->ustar3, D.z_i, D.nu_c,
-                                       D.pbe_feedback));
+#define PBE_LAUNCH(NB)                                                                                     \
+  FDISPATCH(c, T, hipLaunchKernelGGL((k_pbe<T, NB>), grid1(dg.nv), dim3(256), 0, c->s, PC,                 \
+                                     (double *)c->f[LP_RHS_PBE], RH, (const T *)c->f[LP_DISSIP], c->d_bins, \
+                                     c->d_pbeA, dg, c->kpref, c->ustar3, D.z_i, D.nu_c, D.pbe_feedback))
+    if (dg.npc <= 8) PBE_LAUNCH(8);
+    else if (dg.npc <= 16) PBE_LAUNCH(16);
+    else PBE_LAUNCH(32);
+#undef PBE_LAUNCH
     HIPCHK(hipGetLastError());
     prof_end(c, LP_K_PBE, (D.pbe_feedback ? 4.0 : 2.0) * nb + (double)dg.nv * esz_of(c->fdt));
   }

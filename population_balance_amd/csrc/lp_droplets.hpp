@@ -121,37 +121,61 @@ struct PollenParams {
   int src_on, src_k0;       // source active this step; its lower level (0-based)
 };
 
-// pollen_RHS_calc2 for every bin at one cell: face velocities of the bin (:2243-2276),
+// pollen_RHS_calc2 for one bin at one cell: face velocities of the bin (:2243-2276),
 // ghosts (:2478-2590), SMART faces (:2877-3475), flux divergence and SGS diffusion
-// (:3559-3677) and the point source (:4222-4256).  One thread per point of the volume;
-// points outside the cells keep the values of the previous RHS: when the caller has
-// moved it to RF (RHS_PCon_f = RHS_PCon, :1521, done by exchanging the two buffers) they
-// are copied back from RF, otherwise (RF null) they are left alone.
+// (:3559-3677) and the point source (:4222-4256).  A wavefront takes 64 consecutive points
+// of one bin and the wavefronts of a block take the same points for successive bins, so
+// the bin-independent inputs (face velocities, accelerations, Kc_t) are shared through
+// L1.  Every load of a bin is issued unconditionally from a clamped index and the stencil
+// logic selects among the loaded values.  Points outside the cells keep the values of the
+// previous RHS: when the caller has moved it to RF (RHS_PCon_f = RHS_PCon, :1521, done by
+// exchanging the two buffers) they are copied back from RF, otherwise (RF null) they are
+// left alone.
+static constexpr int kPollenBinsPerBlock = 8;
 template <typename T>
-__global__ __launch_bounds__(256) void k_pollen_rhs(const double *__restrict__ PC, double *__restrict__ RH,
-                                                    const double *__restrict__ RF, const double *__restrict__ ui0,
-                                                    const double *__restrict__ vi0, const T *__restrict__ w,
-                                                    const T *__restrict__ dudt, const T *__restrict__ dvdt,
-                                                    const T *__restrict__ dwdt, const double *__restrict__ kc,
-                                                    const double *__restrict__ wsb, const double *__restrict__ qsrc,
-                                                    const double *__restrict__ foot, DropGeo g, PollenParams P) {
-  const long s = gtid();
-  if (s >= g.nv) return;
+__global__ __launch_bounds__(64 * kPollenBinsPerBlock) void k_pollen_rhs(
+    const double *__restrict__ PC, double *__restrict__ RH, const double *__restrict__ RF,
+    const double *__restrict__ ui0, const double *__restrict__ vi0, const T *__restrict__ w,
+    const T *__restrict__ dudt, const T *__restrict__ dvdt, const T *__restrict__ dwdt, const double *__restrict__ kc,
+    const double *__restrict__ wsb, const double *__restrict__ qsrc, const double *__restrict__ foot, DropGeo g,
+    PollenParams P) {
+  const long s = (long)blockIdx.x * 64 + (threadIdx.x & 63);
+  const int b = blockIdx.y * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);  // wavefront-uniform
+  if (s >= g.nv || b >= g.npc) return;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const long np = g.np, ld = g.ld;
   const int k = (int)(s / np);
   const int j = (int)((s % np) / ld);
   const int i = (int)(s % ld);
+  const long ob = (long)b * g.nv + s;
   if (k >= nz - 1 || i >= nx) {
-    if (RF)
-      for (int b = 0; b < g.npc; ++b) RH[(long)b * g.nv + s] = RF[(long)b * g.nv + s];
+    if (RF) RH[ob] = RF[ob];
     return;
   }
   const int im = (i == 0) ? nx - 1 : i - 1, ip = (i == nx - 1) ? 0 : i + 1;
   const int jm = (j == 0) ? ny - 1 : j - 1, jp = (j == ny - 1) ? 0 : j + 1;
   const long row = (long)k * np + (long)j * ld;  // start of this x row
   const long col = (long)k * np + i;             // start of this y column
-  // bin-independent face velocities: faces i, i+1 and 1 of the row / column
+  // clamped neighbour indices (values outside the row / column are selected away below)
+  const int i_m2 = i >= 2 ? i - 2 : 0, i_m1 = i >= 1 ? i - 1 : 0;
+  const int i_p1 = i + 1 <= nx - 1 ? i + 1 : nx - 1, i_p2 = i + 2 <= nx - 1 ? i + 2 : nx - 1;
+  const int j_m2 = j >= 2 ? j - 2 : 0, j_m1 = j >= 1 ? j - 1 : 0;
+  const int j_p1 = j + 1 <= ny - 1 ? j + 1 : ny - 1, j_p2 = j + 2 <= ny - 1 ? j + 2 : ny - 1;
+  const int k_m2 = k >= 2 ? k - 2 : 0, k_m1 = k >= 1 ? k - 1 : 0;
+  const int k_p2 = k + 2 <= nz - 1 ? k + 2 : nz - 1;  // k + 1 <= nz - 1 always
+  const long pl = (long)j * ld + i;                   // offset of this point in a level
+  // this bin's concentrations
+  const double *C = PC + (long)b * g.nv;
+  const double ci = C[s];
+  const double cx_m2 = C[row + i_m2], cx_m1 = C[row + i_m1], cx_p1 = C[row + i_p1], cx_p2 = C[row + i_p2];
+  const double cx_0 = C[row], cx_1 = C[row + 1], cx_n2 = C[row + nx - 2], cx_n1 = C[row + nx - 1];
+  const double cy_m2 = C[col + (long)j_m2 * ld], cy_m1 = C[col + (long)j_m1 * ld];
+  const double cy_p1 = C[col + (long)j_p1 * ld], cy_p2 = C[col + (long)j_p2 * ld];
+  const double cy_0 = C[col], cy_1 = C[col + ld], cy_n2 = C[col + (long)(ny - 2) * ld],
+               cy_n1 = C[col + (long)(ny - 1) * ld];
+  const double cz_m2 = C[(long)k_m2 * np + pl], cz_m1 = C[(long)k_m1 * np + pl];
+  const double cz_p1 = C[(long)(k + 1) * np + pl], cz_p2 = C[(long)k_p2 * np + pl];
+  // bin-independent inputs: faces i, i+1 and 1 of the row / column
   const double *ur = ui0 + (long)k * g.fp + (long)j * ld;
   const double *vc = vi0 + (long)k * g.fp + i;
   const double u0 = ur[i], u1 = ur[i + 1], uF = ur[0];
@@ -161,111 +185,113 @@ __global__ __launch_bounds__(256) void k_pollen_rhs(const double *__restrict__ P
   const bool inert = P.settling && P.inertia;
   if (inert) {  // accelerations on the faces (:2255-2270)
     const T *a = dudt + row;
-    axF = 0.5 * ((double)a[nx - 1] + (double)a[0]);
-    ax0 = (i == 0) ? axF : 0.5 * ((double)a[i - 1] + (double)a[i]);
-    ax1 = (i == nx - 1) ? axF : 0.5 * ((double)a[i] + (double)a[i + 1]);
+    const double am = (double)a[i_m1], a0 = (double)a[i], ap = (double)a[i_p1];
+    const double aL = (double)a[nx - 1], aR = (double)a[0];
+    axF = 0.5 * (aL + aR);
+    ax0 = (i == 0) ? axF : 0.5 * (am + a0);
+    ax1 = (i == nx - 1) ? axF : 0.5 * (a0 + ap);
     const T *bcol = dvdt + col;
-    ayF = 0.5 * ((double)bcol[(long)(ny - 1) * ld] + (double)bcol[0]);
-    ay0 = (j == 0) ? ayF : 0.5 * ((double)bcol[(long)(j - 1) * ld] + (double)bcol[(long)j * ld]);
-    ay1 = (j == ny - 1) ? ayF : 0.5 * ((double)bcol[(long)j * ld] + (double)bcol[(long)(j + 1) * ld]);
+    const double bm = (double)bcol[(long)j_m1 * ld], b0 = (double)bcol[(long)j * ld];
+    const double bp = (double)bcol[(long)j_p1 * ld];
+    const double bL = (double)bcol[(long)(ny - 1) * ld], bR = (double)bcol[0];
+    ayF = 0.5 * (bL + bR);
+    ay0 = (j == 0) ? ayF : 0.5 * (bm + b0);
+    ay1 = (j == ny - 1) ? ayF : 0.5 * (b0 + bp);
     az0 = (double)dwdt[s];
     az1 = (double)dwdt[s + np];
   }
-  // Kc_t stencil (same for every bin)
+  // Kc_t stencil
   const double K0 = kc[s];
   const double Kxm = kc[row + im], Kxp = kc[row + ip];
   const double Kym = kc[col + (long)jm * ld], Kyp = kc[col + (long)jp * ld];
-  const double Kzm = (k > 0) ? kc[s - np] : 0.0;
-  const double Kzp = (k < nz - 2) ? kc[s + np] : 0.0;
-  double fsrc = 0.0;
-  bool src_here = false;
-  if (P.src_on && (k == P.src_k0 || k == P.src_k0 + 1)) {
-    src_here = true;
-    fsrc = foot[(long)j * nx + i];
-  }
+  const double kzm_v = kc[s - (k > 0 ? np : 0)], kzp_v = kc[s + np];
+  const double Kzm = (k > 0) ? kzm_v : 0.0;
+  const double Kzp = (k < nz - 2) ? kzp_v : 0.0;
+  const bool src_here = P.src_on && (k == P.src_k0 || k == P.src_k0 + 1);
+  const double fsrc = src_here ? foot[(long)j * nx + i] : 0.0;
   const double fr = (k == P.src_k0) ? P.frac : P.frac1;
 
-  for (int b = 0; b < g.npc; ++b) {
-    const double *C = PC + (long)b * g.nv;
-    const double ws = wsb[b];
-    double uf0 = u0, uf1 = u1, ufF = uF, vf0 = v0, vf1 = v1, vfF = vF, wf0 = w0, wf1 = w1;
-    if (P.settling) {  // w faces 2..nz-1 of the reference carry the settling velocity
-      if (k >= 1) wf0 -= ws;
-      if (k + 1 <= nz - 2) wf1 -= ws;
-      if (P.inertia) {
-        const double f = ws / P.ghat;
-        uf0 -= ax0 * f; uf1 -= ax1 * f; ufF -= axF * f;
-        vf0 -= ay0 * f; vf1 -= ay1 * f; vfF -= ayF * f;
-        if (k >= 1) wf0 -= az0 * f;
-        if (k + 1 <= nz - 2) wf1 -= az1 * f;
-      }
+  // face velocities of this bin
+  const double ws = wsb[b];
+  double uf0 = u0, uf1 = u1, ufF = uF, vf0 = v0, vf1 = v1, vfF = vF, wf0 = w0, wf1 = w1;
+  if (P.settling) {  // w faces 2..nz-1 of the reference carry the settling velocity
+    if (k >= 1) wf0 -= ws;
+    if (k + 1 <= nz - 2) wf1 -= ws;
+    if (P.inertia) {
+      const double f = ws / P.ghat;
+      uf0 -= ax0 * f; uf1 -= ax1 * f; ufF -= axF * f;
+      vf0 -= ay0 * f; vf1 -= ay1 * f; vfF -= ayF * f;
+      if (k >= 1) wf0 -= az0 * f;
+      if (k + 1 <= nz - 2) wf1 -= az1 * f;
     }
-    auto Cx = [&](int ii) -> double { return C[row + ii]; };
-    auto Cy = [&](int jj) -> double { return C[col + (long)jj * ld]; };
-    auto Cz = [&](int kk) -> double { return C[(long)kk * np + (long)j * ld + i]; };
-    const double ci = C[s];
-    // ghost cells -1 and n of the row / column
-    double gx0, gxL, gy0, gyL;
-    if (P.bc_x == 0) { gx0 = Cx(nx - 1); gxL = Cx(0); }
-    else if (P.bc_x == 1) { gx0 = 0.0; gxL = Cx(nx - 1); }
-    else { gx0 = Cx(0) * (1 - sign1(ufF)) / 2; gxL = Cx(nx - 1) * (1 - sign1(-ufF)) / 2; }
-    if (P.bc_y == 0) { gy0 = Cy(ny - 1); gyL = Cy(0); }
-    else if (P.bc_y == 1) { gy0 = Cy(0); gyL = 0.0; }
-    else { gy0 = Cy(0) * (1 - sign1(vfF)) / 2; gyL = Cy(ny - 1) * (1 - sign1(-vfF)) / 2; }
-    // face f lies between cells f-1 and f; end faces of a periodic row use the face-1 sign
-    auto facex = [&](int f, double uf) -> double {
-      if (f == 0 || f == nx) {
-        if (P.bc_x == 0)
-          return (ufF >= 0.0) ? smart_face(Cx(nx - 1), Cx(nx - 2), Cx(0)) : smart_face(Cx(0), Cx(1), Cx(nx - 1));
-        return (f == 0) ? gx0 : gxL;
-      }
-      const double cm2 = (f >= 2) ? Cx(f - 2) : gx0;
-      const double cp1 = (f + 1 <= nx - 1) ? Cx(f + 1) : gxL;
-      return (uf >= 0.0) ? smart_face(Cx(f - 1), cm2, Cx(f)) : smart_face(Cx(f), cp1, Cx(f - 1));
-    };
-    auto facey = [&](int f, double vf) -> double {
-      if (f == 0 || f == ny) {
-        if (P.bc_y == 0)
-          return (vfF >= 0.0) ? smart_face(Cy(ny - 1), Cy(ny - 2), Cy(0)) : smart_face(Cy(0), Cy(1), Cy(ny - 1));
-        return (f == 0) ? gy0 : gyL;
-      }
-      const double cm2 = (f >= 2) ? Cy(f - 2) : gy0;
-      const double cp1 = (f + 1 <= ny - 1) ? Cy(f + 1) : gyL;
-      return (vf >= 0.0) ? smart_face(Cy(f - 1), cm2, Cy(f)) : smart_face(Cy(f), cp1, Cy(f - 1));
-    };
-    // z faces 1..nz-2 (0-based); face 0 carries no flux, face 1 is central for upward
-    // flow, and the top face nz-1 repeats face nz-2 (:3340, :3356)
-    auto facez = [&](int f, double wf) -> double {
-      if (f == 0) return 0.0;
-      if (f == 1) return (wf >= 0.0) ? (Cz(1) + Cz(0)) / 2.0 : smart_face(Cz(1), Cz(2), Cz(0));
-      return (wf >= 0.0) ? smart_face(Cz(f - 1), Cz(f - 2), Cz(f)) : smart_face(Cz(f), Cz(f + 1), Cz(f - 1));
-    };
-    const double cxb = facex(i, uf0), cxt = facex(i + 1, uf1);
-    const double cyb = facey(j, vf0), cyt = facey(j + 1, vf1);
-    const double czb = facez(k, wf0);
-    const double czt = (k + 1 == nz - 1) ? czb : facez(k + 1, wf1);
-    double r = -P.idx * (uf1 * cxt - uf0 * cxb);
-    r = r - P.idy * (vf1 * cyt - vf0 * cyb);
-    r = r - P.idz * (wf1 * czt - wf0 * czb);
-    {
-      const double cm = (i == 0) ? gx0 : Cx(i - 1);
-      const double cp = (i == nx - 1) ? gxL : Cx(i + 1);
-      r = r + P.idx2 * (((K0 + Kxp) / 2.0) * (cp - ci) - ((K0 + Kxm) / 2.0) * (ci - cm));
-    }
-    {
-      const double cm = (j == 0) ? gy0 : Cy(j - 1);
-      const double cp = (j == ny - 1) ? gyL : Cy(j + 1);
-      r = r + P.idy2 * (((K0 + Kyp) / 2.0) * (cp - ci) - ((K0 + Kym) / 2.0) * (ci - cm));
-    }
-    if (k == 0)
-      r = r + P.idz2 * (((Kzp + K0) / 2.0) * (Cz(1) - ci)) + P.idz * P.psf;
-    else if (k == nz - 2)
-      r = r + P.idz2 * (0.0 - ((K0 + Kzm) / 2.0) * (ci - Cz(k - 1)));
-    else
-      r = r + P.idz2 * (((Kzp + K0) / 2.0) * (Cz(k + 1) - ci) - ((K0 + Kzm) / 2.0) * (ci - Cz(k - 1)));
-    if (src_here) r += fr * qsrc[b] * fsrc;
-    RH[(long)b * g.nv + s] = r;
   }
+  // ghost cells -1 and n of the row / column
+  double gx0, gxL, gy0, gyL;
+  if (P.bc_x == 0) { gx0 = cx_n1; gxL = cx_0; }
+  else if (P.bc_x == 1) { gx0 = 0.0; gxL = cx_n1; }
+  else { gx0 = cx_0 * (1 - sign1(ufF)) / 2; gxL = cx_n1 * (1 - sign1(-ufF)) / 2; }
+  if (P.bc_y == 0) { gy0 = cy_n1; gyL = cy_0; }
+  else if (P.bc_y == 1) { gy0 = cy_0; gyL = 0.0; }
+  else { gy0 = cy_0 * (1 - sign1(vfF)) / 2; gyL = cy_n1 * (1 - sign1(-vfF)) / 2; }
+  // the end faces of a periodic row use the face-1 sign; face f lies between cells f-1, f
+  const double xend = (ufF >= 0.0) ? smart_face(cx_n1, cx_n2, cx_0) : smart_face(cx_0, cx_1, cx_n1);
+  const double yend = (vfF >= 0.0) ? smart_face(cy_n1, cy_n2, cy_0) : smart_face(cy_0, cy_1, cy_n1);
+  double cxb, cxt, cyb, cyt;
+  if (i == 0) {
+    cxb = (P.bc_x == 0) ? xend : gx0;
+  } else {
+    const double cm2 = (i >= 2) ? cx_m2 : gx0, cp1 = (i + 1 <= nx - 1) ? cx_p1 : gxL;
+    cxb = (uf0 >= 0.0) ? smart_face(cx_m1, cm2, ci) : smart_face(ci, cp1, cx_m1);
+  }
+  if (i + 1 == nx) {
+    cxt = (P.bc_x == 0) ? xend : gxL;
+  } else {
+    const double cm2 = (i + 1 >= 2) ? cx_m1 : gx0, cp1 = (i + 2 <= nx - 1) ? cx_p2 : gxL;
+    cxt = (uf1 >= 0.0) ? smart_face(ci, cm2, cx_p1) : smart_face(cx_p1, cp1, ci);
+  }
+  if (j == 0) {
+    cyb = (P.bc_y == 0) ? yend : gy0;
+  } else {
+    const double cm2 = (j >= 2) ? cy_m2 : gy0, cp1 = (j + 1 <= ny - 1) ? cy_p1 : gyL;
+    cyb = (vf0 >= 0.0) ? smart_face(cy_m1, cm2, ci) : smart_face(ci, cp1, cy_m1);
+  }
+  if (j + 1 == ny) {
+    cyt = (P.bc_y == 0) ? yend : gyL;
+  } else {
+    const double cm2 = (j + 1 >= 2) ? cy_m1 : gy0, cp1 = (j + 2 <= ny - 1) ? cy_p2 : gyL;
+    cyt = (vf1 >= 0.0) ? smart_face(ci, cm2, cy_p1) : smart_face(cy_p1, cp1, ci);
+  }
+  // z faces 1..nz-2 (0-based): face 0 carries no flux, face 1 is central for upward flow,
+  // and the top face nz-1 repeats face nz-2 (:3340, :3356)
+  double czb;
+  if (k == 0) czb = 0.0;
+  else if (k == 1) czb = (wf0 >= 0.0) ? (ci + cz_m1) / 2.0 : smart_face(ci, cz_p1, cz_m1);
+  else czb = (wf0 >= 0.0) ? smart_face(cz_m1, cz_m2, ci) : smart_face(ci, cz_p1, cz_m1);
+  double czt;
+  if (k + 1 == nz - 1) czt = czb;
+  else if (k == 0) czt = (wf1 >= 0.0) ? (cz_p1 + ci) / 2.0 : smart_face(cz_p1, cz_p2, ci);
+  else czt = (wf1 >= 0.0) ? smart_face(ci, cz_m1, cz_p1) : smart_face(cz_p1, cz_p2, ci);
+  double r = -P.idx * (uf1 * cxt - uf0 * cxb);
+  r = r - P.idy * (vf1 * cyt - vf0 * cyb);
+  r = r - P.idz * (wf1 * czt - wf0 * czb);
+  {
+    const double cm = (i == 0) ? gx0 : cx_m1;
+    const double cp = (i == nx - 1) ? gxL : cx_p1;
+    r = r + P.idx2 * (((K0 + Kxp) / 2.0) * (cp - ci) - ((K0 + Kxm) / 2.0) * (ci - cm));
+  }
+  {
+    const double cm = (j == 0) ? gy0 : cy_m1;
+    const double cp = (j == ny - 1) ? gyL : cy_p1;
+    r = r + P.idy2 * (((K0 + Kyp) / 2.0) * (cp - ci) - ((K0 + Kym) / 2.0) * (ci - cm));
+  }
+  if (k == 0)
+    r = r + P.idz2 * (((Kzp + K0) / 2.0) * (cz_p1 - ci)) + P.idz * P.psf;
+  else if (k == nz - 2)
+    r = r + P.idz2 * (0.0 - ((K0 + Kzm) / 2.0) * (ci - cz_m1));
+  else
+    r = r + P.idz2 * (((Kzp + K0) / 2.0) * (cz_p1 - ci) - ((K0 + Kzm) / 2.0) * (ci - cz_m1));
+  if (src_here) r += fr * qsrc[b] * fsrc;
+  RH[ob] = r;
 }
 
 // breakage-frequency fit of one bin (:2167-2230): the Ohnesorge-number coefficients and
@@ -284,54 +310,66 @@ struct PbePair {
 };
 
 // Rhs_pbe for every bin at one cell; zero outside the cells (the reference clears the
-// whole array, :2005-2021).  With feedback, RHS_PCon += Rhs_pbe.  The per-thread bin
-// arrays live in LDS, laid out [bin][thread].
-template <typename T>
-__global__ __launch_bounds__(64) void k_pbe(const double *__restrict__ PC, double *__restrict__ PB,
-                                            double *__restrict__ RH, const T *__restrict__ dissip,
-                                            const PbeBin *__restrict__ bins, const PbePair *__restrict__ pairs,
-                                            int npairs, DropGeo g, double kpref, double ustar3, double z_i,
-                                            double nu_c, int feedback) {
-  extern __shared__ double lds[];
-  double *gf = lds;                     // [npc][64]
-  double *acc = lds + 64 * g.npc;       // [npc][64]
-  const int t = threadIdx.x;
+// whole array, :2005-2021).  With feedback, RHS_PCon += Rhs_pbe.  The pair loop of
+// :4356-4448 is linear in g_j = gf(j) PCon(j): Rhs_pbe(b) = sum_j A(b, j) g_j with
+// A(b, j) = sum over the pairs (i, j) of beta_ij (+1 into i, -1 out of j, + w_lo / w_hi into
+// the complementary drop's bins), built on the host.  NB >= npc bins are held in
+// registers (loops unrolled, bins >= npc skipped); A is read through the scalar cache.
+// pow(Re1, b) and pow(Re1, d) share one logarithm (exp(b ln Re1)) and pow(10, x) is
+// exp10(x); both agree with pow to a few ulp.
+template <typename T, int NB>
+__global__ __launch_bounds__(256) void k_pbe(const double *__restrict__ PC, double *__restrict__ PB,
+                                             double *__restrict__ RH, const T *__restrict__ dissip,
+                                             const PbeBin *__restrict__ bins, const double *__restrict__ A,
+                                             DropGeo g, double kpref, double ustar3, double z_i, double nu_c,
+                                             int feedback) {
   const long s = gtid();
   if (s >= g.nv) return;
+  const int npc = g.npc;
   const int k = (int)(s / g.np), x = (int)(s % g.ld);
   if (k >= g.nz - 1 || x >= g.nx) {
-    for (int b = 0; b < g.npc; ++b) PB[(long)b * g.nv + s] = 0.0;
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+      if (b < npc) PB[(long)b * g.nv + s] = 0.0;
     return;
   }
   const double eps = (double)dissip[s];
   const double ds = eps * ustar3 / z_i;
   const double et = pow(fabs(eps), 0.33333333);
   const double er = pow(fabs(ds), 1.0 / 3.0);
-  for (int b = 0; b < g.npc; ++b) {
-    const PbeBin B = bins[b];
-    const double tau_c = et * B.dpow_tau;
-    const double Re = er * B.dpow_re / nu_c;
-    const double Re1 = log10(Re);
-    const double fb = 0.5 * (sign1(Re1) + 1.0);
-    const double al = B.a * fb * pow(Re1, fb * B.b);
-    const double bl = fb * B.c * pow(Re1, fb * B.d) - B.e;
-    const double fit = pow(10.0, al + bl);
-    gf[b * 64 + t] = kpref * tau_c * fit;
-    acc[b * 64 + t] = 0.0;
+  double gj[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    gj[b] = 0.0;
+    if (b < npc) {
+      const PbeBin B = bins[b];
+      const double tau_c = et * B.dpow_tau;
+      const double Re = er * B.dpow_re / nu_c;
+      const double Re1 = log10(Re);
+      const double fb = 0.5 * (sign1(Re1) + 1.0);
+      double pb = 1.0, pd = 1.0;  // pow(Re1, fb b), pow(Re1, fb d): 1 when fb = 0
+      if (fb != 0.0) {
+        const double lr = log(Re1);
+        pb = exp(B.b * lr);
+        pd = exp(B.d * lr);
+      }
+      const double al = B.a * fb * pb;
+      const double bl = fb * B.c * pd - B.e;
+      const double fit = exp10(al + bl);
+      gj[b] = kpref * tau_c * fit * PC[(long)b * g.nv + s];
+    }
   }
-  for (int q = 0; q < npairs; ++q) {
-    const PbePair Q = pairs[q];
-    const double tr = gf[Q.j * 64 + t] * Q.beta * PC[(long)Q.j * g.nv + s];
-    acc[Q.i * 64 + t] += tr;
-    acc[Q.j * 64 + t] -= tr;
-    if (Q.lo >= 0) acc[Q.lo * 64 + t] += tr * Q.wlo;
-    if (Q.hi >= 0) acc[Q.hi * 64 + t] += tr * Q.whi;
-  }
-  for (int b = 0; b < g.npc; ++b) {
-    const long o = (long)b * g.nv + s;
-    const double pb = acc[b * 64 + t];
-    PB[o] = pb;
-    if (feedback) RH[o] = RH[o] + pb;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    if (b < npc) {
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+        if (j < npc) acc = fma(A[b * npc + j], gj[j], acc);
+      const long o = (long)b * g.nv + s;
+      PB[o] = acc;
+      if (feedback) RH[o] = RH[o] + acc;
+    }
   }
 }
 
