@@ -1192,9 +1192,8 @@ static int do_pollen_rhs(lp_ctx *c, const double *prev) {
   P.src_k0 = D.src_k - 1;
   const double nb = (double)dg.nv * dg.npc * 8.0;
   prof_begin(c, LP_K_POLLEN_RHS);
-  const int nbb = dg.npc < kPollenBinsPerBlock ? dg.npc : kPollenBinsPerBlock;
-  const dim3 pgrid((unsigned)((dg.nv + 63) / 64), (unsigned)((dg.npc + nbb - 1) / nbb));
-  FDISPATCH(c, T, hipLaunchKernelGGL(k_pollen_rhs<T>, pgrid, dim3(64 * nbb), 0, c->s, PC, RH, prev, c->ui0,
+  const dim3 pgrid((unsigned)((dg.nv + 255) / 256), (unsigned)((dg.npc + kPollenBins - 1) / kPollenBins));
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_pollen_rhs<T>, pgrid, dim3(256), 0, c->s, PC, RH, prev, c->ui0,
                                      c->vi0, (const T *)c->f[LP_W], (const T *)c->f[LP_DUDT],
                                      (const T *)c->f[LP_DVDT], (const T *)c->f[LP_DWDT], c->kcell, c->d_ws,
                                      c->d_qsrc, c->d_foot, dg, P));

@@ -121,35 +121,38 @@ struct PollenParams {
   int src_on, src_k0;       // source active this step; its lower level (0-based)
 };
 
-// pollen_RHS_calc2 for one bin at one cell: face velocities of the bin (:2243-2276),
-// ghosts (:2478-2590), SMART faces (:2877-3475), flux divergence and SGS diffusion
-// (:3559-3677) and the point source (:4222-4256).  A wavefront takes 64 consecutive points
-// of one bin and the wavefronts of a block take the same points for successive bins, so
-// the bin-independent inputs (face velocities, accelerations, Kc_t) are shared through
-// L1.  Every load of a bin is issued unconditionally from a clamped index and the stencil
-// logic selects among the loaded values.  Points outside the cells keep the values of the
+// pollen_RHS_calc2 at one cell for a group of G bins: face velocities of the bin
+// (:2243-2276), ghosts (:2478-2590), SMART faces (:2877-3475), flux divergence and SGS
+// diffusion (:3559-3677) and the point source (:4222-4256).  A thread takes one point and
+// G consecutive bins (blockIdx.y = bin group), so the bin-independent inputs (face
+// velocities, accelerations, Kc_t) are loaded once per G bins.  Every load of a bin is
+// issued unconditionally from a clamped index and the stencil logic selects among the
+// loaded values; the row and column end values are loaded only where a face or ghost of
+// the row or column end is formed.  Points outside the cells keep the values of the
 // previous RHS: when the caller has moved it to RF (RHS_PCon_f = RHS_PCon, :1521, done by
 // exchanging the two buffers) they are copied back from RF, otherwise (RF null) they are
 // left alone.
-static constexpr int kPollenBinsPerBlock = 8;
+static constexpr int kPollenBins = 8;  // bins per thread
 template <typename T>
-__global__ __launch_bounds__(64 * kPollenBinsPerBlock) void k_pollen_rhs(
+__global__ __launch_bounds__(256) void k_pollen_rhs(
     const double *__restrict__ PC, double *__restrict__ RH, const double *__restrict__ RF,
     const double *__restrict__ ui0, const double *__restrict__ vi0, const T *__restrict__ w,
     const T *__restrict__ dudt, const T *__restrict__ dvdt, const T *__restrict__ dwdt, const double *__restrict__ kc,
     const double *__restrict__ wsb, const double *__restrict__ qsrc, const double *__restrict__ foot, DropGeo g,
     PollenParams P) {
-  const long s = (long)blockIdx.x * 64 + (threadIdx.x & 63);
-  const int b = blockIdx.y * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);  // wavefront-uniform
-  if (s >= g.nv || b >= g.npc) return;
+  const long s = gtid();
+  const int b0 = blockIdx.y * kPollenBins;  // first bin of the group (uniform)
+  if (s >= g.nv) return;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const long np = g.np, ld = g.ld;
   const int k = (int)(s / np);
   const int j = (int)((s % np) / ld);
   const int i = (int)(s % ld);
-  const long ob = (long)b * g.nv + s;
   if (k >= nz - 1 || i >= nx) {
-    if (RF) RH[ob] = RF[ob];
+    if (RF)
+#pragma unroll
+      for (int bb = 0; bb < kPollenBins; ++bb)
+        if (b0 + bb < g.npc) RH[(long)(b0 + bb) * g.nv + s] = RF[(long)(b0 + bb) * g.nv + s];
     return;
   }
   const int im = (i == 0) ? nx - 1 : i - 1, ip = (i == nx - 1) ? 0 : i + 1;
@@ -164,17 +167,8 @@ __global__ __launch_bounds__(64 * kPollenBinsPerBlock) void k_pollen_rhs(
   const int k_m2 = k >= 2 ? k - 2 : 0, k_m1 = k >= 1 ? k - 1 : 0;
   const int k_p2 = k + 2 <= nz - 1 ? k + 2 : nz - 1;  // k + 1 <= nz - 1 always
   const long pl = (long)j * ld + i;                   // offset of this point in a level
-  // this bin's concentrations
-  const double *C = PC + (long)b * g.nv;
-  const double ci = C[s];
-  const double cx_m2 = C[row + i_m2], cx_m1 = C[row + i_m1], cx_p1 = C[row + i_p1], cx_p2 = C[row + i_p2];
-  const double cx_0 = C[row], cx_1 = C[row + 1], cx_n2 = C[row + nx - 2], cx_n1 = C[row + nx - 1];
-  const double cy_m2 = C[col + (long)j_m2 * ld], cy_m1 = C[col + (long)j_m1 * ld];
-  const double cy_p1 = C[col + (long)j_p1 * ld], cy_p2 = C[col + (long)j_p2 * ld];
-  const double cy_0 = C[col], cy_1 = C[col + ld], cy_n2 = C[col + (long)(ny - 2) * ld],
-               cy_n1 = C[col + (long)(ny - 1) * ld];
-  const double cz_m2 = C[(long)k_m2 * np + pl], cz_m1 = C[(long)k_m1 * np + pl];
-  const double cz_p1 = C[(long)(k + 1) * np + pl], cz_p2 = C[(long)k_p2 * np + pl];
+  const bool xends = i <= 1 || i >= nx - 2;           // faces or ghosts of the row ends used
+  const bool yends = j <= 1 || j >= ny - 2;
   // bin-independent inputs: faces i, i+1 and 1 of the row / column
   const double *ur = ui0 + (long)k * g.fp + (long)j * ld;
   const double *vc = vi0 + (long)k * g.fp + i;
@@ -191,12 +185,12 @@ __global__ __launch_bounds__(64 * kPollenBinsPerBlock) void k_pollen_rhs(
     ax0 = (i == 0) ? axF : 0.5 * (am + a0);
     ax1 = (i == nx - 1) ? axF : 0.5 * (a0 + ap);
     const T *bcol = dvdt + col;
-    const double bm = (double)bcol[(long)j_m1 * ld], b0 = (double)bcol[(long)j * ld];
+    const double bm = (double)bcol[(long)j_m1 * ld], bc0 = (double)bcol[(long)j * ld];
     const double bp = (double)bcol[(long)j_p1 * ld];
     const double bL = (double)bcol[(long)(ny - 1) * ld], bR = (double)bcol[0];
     ayF = 0.5 * (bL + bR);
-    ay0 = (j == 0) ? ayF : 0.5 * (bm + b0);
-    ay1 = (j == ny - 1) ? ayF : 0.5 * (b0 + bp);
+    ay0 = (j == 0) ? ayF : 0.5 * (bm + bc0);
+    ay1 = (j == ny - 1) ? ayF : 0.5 * (bc0 + bp);
     az0 = (double)dwdt[s];
     az1 = (double)dwdt[s + np];
   }
@@ -211,87 +205,105 @@ __global__ __launch_bounds__(64 * kPollenBinsPerBlock) void k_pollen_rhs(
   const double fsrc = src_here ? foot[(long)j * nx + i] : 0.0;
   const double fr = (k == P.src_k0) ? P.frac : P.frac1;
 
-  // face velocities of this bin
-  const double ws = wsb[b];
-  double uf0 = u0, uf1 = u1, ufF = uF, vf0 = v0, vf1 = v1, vfF = vF, wf0 = w0, wf1 = w1;
-  if (P.settling) {  // w faces 2..nz-1 of the reference carry the settling velocity
-    if (k >= 1) wf0 -= ws;
-    if (k + 1 <= nz - 2) wf1 -= ws;
-    if (P.inertia) {
-      const double f = ws / P.ghat;
-      uf0 -= ax0 * f; uf1 -= ax1 * f; ufF -= axF * f;
-      vf0 -= ay0 * f; vf1 -= ay1 * f; vfF -= ayF * f;
-      if (k >= 1) wf0 -= az0 * f;
-      if (k + 1 <= nz - 2) wf1 -= az1 * f;
+#pragma unroll
+  for (int bb = 0; bb < kPollenBins; ++bb) {
+    const int b = b0 + bb;
+    if (b >= g.npc) break;
+    // this bin's concentrations
+    const double *C = PC + (long)b * g.nv;
+    const double ci = C[s];
+    const double cx_m2 = C[row + i_m2], cx_m1 = C[row + i_m1], cx_p1 = C[row + i_p1], cx_p2 = C[row + i_p2];
+    const double cy_m2 = C[col + (long)j_m2 * ld], cy_m1 = C[col + (long)j_m1 * ld];
+    const double cy_p1 = C[col + (long)j_p1 * ld], cy_p2 = C[col + (long)j_p2 * ld];
+    const double cz_m2 = C[(long)k_m2 * np + pl], cz_m1 = C[(long)k_m1 * np + pl];
+    const double cz_p1 = C[(long)(k + 1) * np + pl], cz_p2 = C[(long)k_p2 * np + pl];
+    double cx_0 = 0.0, cx_1 = 0.0, cx_n2 = 0.0, cx_n1 = 0.0, cy_0 = 0.0, cy_1 = 0.0, cy_n2 = 0.0, cy_n1 = 0.0;
+    if (xends) { cx_0 = C[row]; cx_1 = C[row + 1]; cx_n2 = C[row + nx - 2]; cx_n1 = C[row + nx - 1]; }
+    if (yends) {
+      cy_0 = C[col]; cy_1 = C[col + ld];
+      cy_n2 = C[col + (long)(ny - 2) * ld]; cy_n1 = C[col + (long)(ny - 1) * ld];
     }
+    // face velocities of this bin
+    const double ws = wsb[b];
+    double uf0 = u0, uf1 = u1, ufF = uF, vf0 = v0, vf1 = v1, vfF = vF, wf0 = w0, wf1 = w1;
+    if (P.settling) {  // w faces 2..nz-1 of the reference carry the settling velocity
+      if (k >= 1) wf0 -= ws;
+      if (k + 1 <= nz - 2) wf1 -= ws;
+      if (P.inertia) {
+        const double f = ws / P.ghat;
+        uf0 -= ax0 * f; uf1 -= ax1 * f; ufF -= axF * f;
+        vf0 -= ay0 * f; vf1 -= ay1 * f; vfF -= ayF * f;
+        if (k >= 1) wf0 -= az0 * f;
+        if (k + 1 <= nz - 2) wf1 -= az1 * f;
+      }
+    }
+    // ghost cells -1 and n of the row / column (used only where xends / yends)
+    double gx0, gxL, gy0, gyL;
+    if (P.bc_x == 0) { gx0 = cx_n1; gxL = cx_0; }
+    else if (P.bc_x == 1) { gx0 = 0.0; gxL = cx_n1; }
+    else { gx0 = cx_0 * (1 - sign1(ufF)) / 2; gxL = cx_n1 * (1 - sign1(-ufF)) / 2; }
+    if (P.bc_y == 0) { gy0 = cy_n1; gyL = cy_0; }
+    else if (P.bc_y == 1) { gy0 = cy_0; gyL = 0.0; }
+    else { gy0 = cy_0 * (1 - sign1(vfF)) / 2; gyL = cy_n1 * (1 - sign1(-vfF)) / 2; }
+    // face f lies between cells f-1 and f; the end faces of a periodic row use the face-1
+    // sign
+    double cxb, cxt, cyb, cyt;
+    if (i == 0) {
+      cxb = (P.bc_x == 0) ? ((ufF >= 0.0) ? smart_face(cx_n1, cx_n2, cx_0) : smart_face(cx_0, cx_1, cx_n1)) : gx0;
+    } else {
+      const double cm2 = (i >= 2) ? cx_m2 : gx0, cp1 = (i + 1 <= nx - 1) ? cx_p1 : gxL;
+      cxb = (uf0 >= 0.0) ? smart_face(cx_m1, cm2, ci) : smart_face(ci, cp1, cx_m1);
+    }
+    if (i + 1 == nx) {
+      cxt = (P.bc_x == 0) ? ((ufF >= 0.0) ? smart_face(cx_n1, cx_n2, cx_0) : smart_face(cx_0, cx_1, cx_n1)) : gxL;
+    } else {
+      const double cm2 = (i + 1 >= 2) ? cx_m1 : gx0, cp1 = (i + 2 <= nx - 1) ? cx_p2 : gxL;
+      cxt = (uf1 >= 0.0) ? smart_face(ci, cm2, cx_p1) : smart_face(cx_p1, cp1, ci);
+    }
+    if (j == 0) {
+      cyb = (P.bc_y == 0) ? ((vfF >= 0.0) ? smart_face(cy_n1, cy_n2, cy_0) : smart_face(cy_0, cy_1, cy_n1)) : gy0;
+    } else {
+      const double cm2 = (j >= 2) ? cy_m2 : gy0, cp1 = (j + 1 <= ny - 1) ? cy_p1 : gyL;
+      cyb = (vf0 >= 0.0) ? smart_face(cy_m1, cm2, ci) : smart_face(ci, cp1, cy_m1);
+    }
+    if (j + 1 == ny) {
+      cyt = (P.bc_y == 0) ? ((vfF >= 0.0) ? smart_face(cy_n1, cy_n2, cy_0) : smart_face(cy_0, cy_1, cy_n1)) : gyL;
+    } else {
+      const double cm2 = (j + 1 >= 2) ? cy_m1 : gy0, cp1 = (j + 2 <= ny - 1) ? cy_p2 : gyL;
+      cyt = (vf1 >= 0.0) ? smart_face(ci, cm2, cy_p1) : smart_face(cy_p1, cp1, ci);
+    }
+    // z faces 1..nz-2 (0-based): face 0 carries no flux, face 1 is central for upward flow,
+    // and the top face nz-1 repeats face nz-2 (:3340, :3356)
+    double czb;
+    if (k == 0) czb = 0.0;
+    else if (k == 1) czb = (wf0 >= 0.0) ? (ci + cz_m1) / 2.0 : smart_face(ci, cz_p1, cz_m1);
+    else czb = (wf0 >= 0.0) ? smart_face(cz_m1, cz_m2, ci) : smart_face(ci, cz_p1, cz_m1);
+    double czt;
+    if (k + 1 == nz - 1) czt = czb;
+    else if (k == 0) czt = (wf1 >= 0.0) ? (cz_p1 + ci) / 2.0 : smart_face(cz_p1, cz_p2, ci);
+    else czt = (wf1 >= 0.0) ? smart_face(ci, cz_m1, cz_p1) : smart_face(cz_p1, cz_p2, ci);
+    double r = -P.idx * (uf1 * cxt - uf0 * cxb);
+    r = r - P.idy * (vf1 * cyt - vf0 * cyb);
+    r = r - P.idz * (wf1 * czt - wf0 * czb);
+    {
+      const double cm = (i == 0) ? gx0 : cx_m1;
+      const double cp = (i == nx - 1) ? gxL : cx_p1;
+      r = r + P.idx2 * (((K0 + Kxp) / 2.0) * (cp - ci) - ((K0 + Kxm) / 2.0) * (ci - cm));
+    }
+    {
+      const double cm = (j == 0) ? gy0 : cy_m1;
+      const double cp = (j == ny - 1) ? gyL : cy_p1;
+      r = r + P.idy2 * (((K0 + Kyp) / 2.0) * (cp - ci) - ((K0 + Kym) / 2.0) * (ci - cm));
+    }
+    if (k == 0)
+      r = r + P.idz2 * (((Kzp + K0) / 2.0) * (cz_p1 - ci)) + P.idz * P.psf;
+    else if (k == nz - 2)
+      r = r + P.idz2 * (0.0 - ((K0 + Kzm) / 2.0) * (ci - cz_m1));
+    else
+      r = r + P.idz2 * (((Kzp + K0) / 2.0) * (cz_p1 - ci) - ((K0 + Kzm) / 2.0) * (ci - cz_m1));
+    if (src_here) r += fr * qsrc[b] * fsrc;
+    RH[(long)b * g.nv + s] = r;
   }
-  // ghost cells -1 and n of the row / column
-  double gx0, gxL, gy0, gyL;
-  if (P.bc_x == 0) { gx0 = cx_n1; gxL = cx_0; }
-  else if (P.bc_x == 1) { gx0 = 0.0; gxL = cx_n1; }
-  else { gx0 = cx_0 * (1 - sign1(ufF)) / 2; gxL = cx_n1 * (1 - sign1(-ufF)) / 2; }
-  if (P.bc_y == 0) { gy0 = cy_n1; gyL = cy_0; }
-  else if (P.bc_y == 1) { gy0 = cy_0; gyL = 0.0; }
-  else { gy0 = cy_0 * (1 - sign1(vfF)) / 2; gyL = cy_n1 * (1 - sign1(-vfF)) / 2; }
-  // the end faces of a periodic row use the face-1 sign; face f lies between cells f-1, f
-  const double xend = (ufF >= 0.0) ? smart_face(cx_n1, cx_n2, cx_0) : smart_face(cx_0, cx_1, cx_n1);
-  const double yend = (vfF >= 0.0) ? smart_face(cy_n1, cy_n2, cy_0) : smart_face(cy_0, cy_1, cy_n1);
-  double cxb, cxt, cyb, cyt;
-  if (i == 0) {
-    cxb = (P.bc_x == 0) ? xend : gx0;
-  } else {
-    const double cm2 = (i >= 2) ? cx_m2 : gx0, cp1 = (i + 1 <= nx - 1) ? cx_p1 : gxL;
-    cxb = (uf0 >= 0.0) ? smart_face(cx_m1, cm2, ci) : smart_face(ci, cp1, cx_m1);
-  }
-  if (i + 1 == nx) {
-    cxt = (P.bc_x == 0) ? xend : gxL;
-  } else {
-    const double cm2 = (i + 1 >= 2) ? cx_m1 : gx0, cp1 = (i + 2 <= nx - 1) ? cx_p2 : gxL;
-    cxt = (uf1 >= 0.0) ? smart_face(ci, cm2, cx_p1) : smart_face(cx_p1, cp1, ci);
-  }
-  if (j == 0) {
-    cyb = (P.bc_y == 0) ? yend : gy0;
-  } else {
-    const double cm2 = (j >= 2) ? cy_m2 : gy0, cp1 = (j + 1 <= ny - 1) ? cy_p1 : gyL;
-    cyb = (vf0 >= 0.0) ? smart_face(cy_m1, cm2, ci) : smart_face(ci, cp1, cy_m1);
-  }
-  if (j + 1 == ny) {
-    cyt = (P.bc_y == 0) ? yend : gyL;
-  } else {
-    const double cm2 = (j + 1 >= 2) ? cy_m1 : gy0, cp1 = (j + 2 <= ny - 1) ? cy_p2 : gyL;
-    cyt = (vf1 >= 0.0) ? smart_face(ci, cm2, cy_p1) : smart_face(cy_p1, cp1, ci);
-  }
-  // z faces 1..nz-2 (0-based): face 0 carries no flux, face 1 is central for upward flow,
-  // and the top face nz-1 repeats face nz-2 (:3340, :3356)
-  double czb;
-  if (k == 0) czb = 0.0;
-  else if (k == 1) czb = (wf0 >= 0.0) ? (ci + cz_m1) / 2.0 : smart_face(ci, cz_p1, cz_m1);
-  else czb = (wf0 >= 0.0) ? smart_face(cz_m1, cz_m2, ci) : smart_face(ci, cz_p1, cz_m1);
-  double czt;
-  if (k + 1 == nz - 1) czt = czb;
-  else if (k == 0) czt = (wf1 >= 0.0) ? (cz_p1 + ci) / 2.0 : smart_face(cz_p1, cz_p2, ci);
-  else czt = (wf1 >= 0.0) ? smart_face(ci, cz_m1, cz_p1) : smart_face(cz_p1, cz_p2, ci);
-  double r = -P.idx * (uf1 * cxt - uf0 * cxb);
-  r = r - P.idy * (vf1 * cyt - vf0 * cyb);
-  r = r - P.idz * (wf1 * czt - wf0 * czb);
-  {
-    const double cm = (i == 0) ? gx0 : cx_m1;
-    const double cp = (i == nx - 1) ? gxL : cx_p1;
-    r = r + P.idx2 * (((K0 + Kxp) / 2.0) * (cp - ci) - ((K0 + Kxm) / 2.0) * (ci - cm));
-  }
-  {
-    const double cm = (j == 0) ? gy0 : cy_m1;
-    const double cp = (j == ny - 1) ? gyL : cy_p1;
-    r = r + P.idy2 * (((K0 + Kyp) / 2.0) * (cp - ci) - ((K0 + Kym) / 2.0) * (ci - cm));
-  }
-  if (k == 0)
-    r = r + P.idz2 * (((Kzp + K0) / 2.0) * (cz_p1 - ci)) + P.idz * P.psf;
-  else if (k == nz - 2)
-    r = r + P.idz2 * (0.0 - ((K0 + Kzm) / 2.0) * (ci - cz_m1));
-  else
-    r = r + P.idz2 * (((Kzp + K0) / 2.0) * (cz_p1 - ci) - ((K0 + Kzm) / 2.0) * (ci - cz_m1));
-  if (src_here) r += fr * qsrc[b] * fsrc;
-  RH[ob] = r;
 }
 
 // breakage-frequency fit of one bin (:2167-2230): the Ohnesorge-number coefficients and
