@@ -1173,8 +1173,8 @@ static int do_pollen_rhs(lp_ctx *c, const double *prev) {
   FDISPATCH(c, T, {
     hipLaunchKernelGGL(k_face_u<T>, dim3((unsigned)(((long)gg.ny * gg.nz + 3) / 4)), dim3(256), 0, c->s,
                        (const T *)c->f[LP_U], (const T *)c->f[LP_DUDX], c->ui0, dg, gg.dx);
-    hipLaunchKernelGGL(k_face_v<T>, grid1((long)gg.nx * gg.nz), dim3(256), 0, c->s, (const T *)c->f[LP_V],
-                       (const T *)c->f[LP_DVDY], c->vi0, dg, gg.dy);
+    hipLaunchKernelGGL(k_face_v<T>, dim3((unsigned)((gg.nx + 63) / 64), (unsigned)gg.nz), dim3(64 * kFaceSeg), 0,
+                       c->s, (const T *)c->f[LP_V], (const T *)c->f[LP_DVDY], c->vi0, dg, gg.dy);
     hipLaunchKernelGGL(k_kc<T>, grid1(dg.nv), dim3(256), 0, c->s, (const T *)c->f[LP_NU_T], (T *)c->f[LP_MAGS],
                        c->kcell, dg, D.Sc);
   });

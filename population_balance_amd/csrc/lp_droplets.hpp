@@ -59,26 +59,50 @@ __global__ __launch_bounds__(256) void k_face_u(const T *__restrict__ u, const T
   if (lane == 0) dst[g.nx] = A + 0.0;
 }
 
-// the same reconstruction along y; one thread per (x, z) column (coalesced over x)
+// the same reconstruction along y.  A block takes 64 x-columns of one level (coalesced
+// over x); its kFaceSeg wavefronts scan consecutive y segments, and the segment sums
+// are combined through LDS.
+static constexpr int kFaceSeg = 8;
 template <typename T>
-__global__ void k_face_v(const T *__restrict__ v, const T *__restrict__ dvdy, double *__restrict__ vi0, DropGeo g,
-                         double dy) {
-  const long t = gtid();
-  if (t >= (long)g.nx * g.nz) return;
-  const int k = (int)(t / g.nx), i = (int)(t % g.nx);
-  const T *vc = v + (long)k * g.np + i;
-  const T *ec = dvdy + (long)k * g.np + i;
-  double *dst = vi0 + (long)k * g.fp + i;
-  double su = 0.0, sp = 0.0, P = 0.0;
-  for (int m = 0; m < g.ny; ++m) {
-    dst[(long)m * g.ld] = P;
+__global__ __launch_bounds__(64 * kFaceSeg) void k_face_v(const T *__restrict__ v, const T *__restrict__ dvdy,
+                                                         double *__restrict__ vi0, DropGeo g, double dy) {
+  __shared__ double E[kFaceSeg][64], SU[kFaceSeg][64], SP[kFaceSeg][64];
+  const int x = threadIdx.x & 63, seg = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + x, k = blockIdx.y;
+  const bool on = i < g.nx;
+  const int L = (g.ny + kFaceSeg - 1) / kFaceSeg;
+  const int m0 = seg * L, m1 = m0 + L < g.ny ? m0 + L : g.ny;
+  const T *vc = v + (long)k * g.np + (on ? i : 0);
+  const T *ec = dvdy + (long)k * g.np + (on ? i : 0);
+  auto e_at = [&](int m) -> double { return (m < g.ny - 1) ? dy * (double)ec[(long)m * g.ld] : 0.0; };
+  double P = 0.0, su = 0.0, sp = 0.0;  // segment-local exclusive prefix and sums
+  for (int m = m0; m < m1; ++m) {
     sp += P;
     su += (double)vc[(long)m * g.ld];
-    if (m < g.ny - 1) P += dy * (double)ec[(long)m * g.ld];
+    P += e_at(m);
   }
-  const double A = su / g.ny - sp / g.ny;
-  for (int m = 0; m < g.ny; ++m) dst[(long)m * g.ld] = A + dst[(long)m * g.ld];
-  dst[(long)g.ny * g.ld] = A + 0.0;
+  E[seg][x] = P;
+  SU[seg][x] = su;
+  SP[seg][x] = sp;
+  __syncthreads();
+  double off = 0.0, tsu = 0.0, tsp = 0.0, run = 0.0;
+  for (int q = 0; q < kFaceSeg; ++q) {
+    const int n0 = q * L, n1 = n0 + L < g.ny ? n0 + L : g.ny;
+    const int n = n1 > n0 ? n1 - n0 : 0;
+    if (q == seg) off = run;
+    tsu += SU[q][x];
+    tsp += SP[q][x] + (double)n * run;
+    run += E[q][x];
+  }
+  const double A = tsu / g.ny - tsp / g.ny;
+  if (!on) return;
+  double *dst = vi0 + (long)k * g.fp + i;
+  P = off;
+  for (int m = m0; m < m1; ++m) {
+    dst[(long)m * g.ld] = A + P;
+    P += e_at(m);
+  }
+  if (seg == 0) dst[(long)g.ny * g.ld] = A + 0.0;
 }
 
 // :3482-3496 Kc_t = Nu_t on cell centres / Sc (zero outside the cells);
