@@ -1165,7 +1165,10 @@ static DropGeo drop_geo(const lp_ctx *c) {
 // scalars_module_crossflow_mod.f90:1894-4472 pollen_RHS_calc2 (+ the optional breakup
 // feedback).  prev: the previous RHS after the buffer exchange of pollen_all_in_one2, or
 // null when called on its own.
-static int do_pollen_rhs(lp_ctx *c, const double *prev) {
+// fuse_step: pollen_all_in_one2 with the breakup source on -- calcbeta_pcon and
+// step_pollen2 (and the Euler start's RHS_PCon_f = RHS_PCon) are done by the breakup
+// pass; returns 1 in *stepped when it did them
+static int do_pollen_rhs(lp_ctx *c, const double *prev, bool fuse_step = false, bool *stepped = nullptr) {
   const lp_droplet_config &D = c->cfg.drop;
   const lp_grid &gg = c->g;
   DropGeo dg = drop_geo(c);
@@ -1202,13 +1205,22 @@ static int do_pollen_rhs(lp_ctx *c, const double *prev) {
   prof_end(c, LP_K_POLLEN_RHS, 2.0 * nb + (double)dg.nv * (8.0 + 3.0 * esz_of(c->fdt)) + 4.0 * dg.fp * gg.nz * 8.0);
   if (D.pbe) {
     prof_begin(c, LP_K_PBE);
-#define PBE_LAUNCH(NB)                                                                                     \
-  FDISPATCH(c, T, hipLaunchKernelGGL((k_pbe<T, NB>), grid1(dg.nv), dim3(256), 0, c->s, PC,                 \
-                                     (double *)c->f[LP_RHS_PBE], RH, (const T *)c->f[LP_DISSIP], c->d_bins, \
-                                     c->d_pbeA, dg, c->kpref, c->ustar3, D.z_i, D.nu_c, D.pbe_feedback))
-    if (dg.npc <= 8) PBE_LAUNCH(8);
-    else if (dg.npc <= 16) PBE_LAUNCH(16);
-    else PBE_LAUNCH(32);
+    PbeStep St{PC, (double *)c->f[LP_RHS_PCON_F], c->f[LP_BETA_PCON], c->d_vfac, D.active,
+               ((c->jt_total == D.pcon_init || c->jt_total == D.src_jt0) && !D.init_pcon) ? 1 : 0, c->cfg.dt};
+#define PBE_LAUNCH(NB, STEP)                                                                                      \
+  FDISPATCH(c, T, hipLaunchKernelGGL((k_pbe<T, NB, STEP>), grid1(dg.nv), dim3(256), 0, c->s, PC,                \
+                                     (double *)c->f[LP_RHS_PBE], RH, (const T *)c->f[LP_DISSIP], c->d_bins,      \
+                                     c->d_pbeA, dg, c->kpref, c->ustar3, D.z_i, D.nu_c, D.pbe_feedback, St))
+    if (fuse_step) {
+      if (dg.npc <= 8) PBE_LAUNCH(8, true);
+      else if (dg.npc <= 16) PBE_LAUNCH(16, true);
+      else PBE_LAUNCH(32, true);
+      if (stepped) *stepped = true;
+    } else {
+      if (dg.npc <= 8) PBE_LAUNCH(8, false);
+      else if (dg.npc <= 16) PBE_LAUNCH(16, false);
+      else PBE_LAUNCH(32, false);
+    }
 #undef PBE_LAUNCH
     HIPCHK(hipGetLastError());
     prof_end(c, LP_K_PBE, (D.pbe_feedback ? 4.0 : 2.0) * nb + (double)dg.nv * esz_of(c->fdt));
@@ -1234,7 +1246,9 @@ static int do_pollen(lp_ctx *c) {
   // :1521 -- RHS_PCon_f takes the old RHS by exchanging the buffers (k_pollen_rhs
   // rewrites every point of the new RHS_PCon)
   std::swap(c->f[LP_RHS_PCON], c->f[LP_RHS_PCON_F]);
-  RCHK(do_pollen_rhs(c, (const double *)c->f[LP_RHS_PCON_F]));
+  bool stepped = false;
+  RCHK(do_pollen_rhs(c, (const double *)c->f[LP_RHS_PCON_F], true, &stepped));
+  if (stepped) return LP_OK;  // the breakup pass did :1660-1665 and the step
   if ((c->jt_total == D.pcon_init || c->jt_total == D.src_jt0) && !D.init_pcon)  // :1660-1665
     HIPCHK(hipMemcpyAsync(c->f[LP_RHS_PCON_F], c->f[LP_RHS_PCON], c->bytes[LP_RHS_PCON], hipMemcpyDeviceToDevice,
                           c->s));

@@ -353,12 +353,24 @@ struct PbePair {
 // registers (loops unrolled, bins >= npc skipped); A is read through the scalar cache.
 // pow(Re1, b) and pow(Re1, d) share one logarithm (exp(b ln Re1)) and pow(10, x) is
 // exp10(x); both agree with pow to a few ulp.
-template <typename T, int NB>
-__global__ __launch_bounds__(256) void k_pbe(const double *__restrict__ PC, double *__restrict__ PB,
+//
+// STEP (pollen_all_in_one2 with the breakup source on): the same pass then does
+// calcbeta_pcon and step_pollen2 at the point (as k_beta_step, from the old PCon), and on
+// an Euler start first sets RHS_PCon_f = RHS_PCon (:1660-1665).
+struct PbeStep {
+  double *PCw;           // PCon (the same array as PC), advanced in place at this point
+  double *RF;            // RHS_PCon_f
+  void *bp;              // beta_pcon (field type)
+  const double *vfac;    // buoyancy factor of each bin
+  int active, euler;
+  double dt;
+};
+template <typename T, int NB, bool STEP>
+__global__ __launch_bounds__(256) void k_pbe(const double *PC, double *__restrict__ PB,
                                              double *__restrict__ RH, const T *__restrict__ dissip,
                                              const PbeBin *__restrict__ bins, const double *__restrict__ A,
                                              DropGeo g, double kpref, double ustar3, double z_i, double nu_c,
-                                             int feedback) {
+                                             int feedback, PbeStep St) {
   const long s = gtid();
   if (s >= g.nv) return;
   const int npc = g.npc;
@@ -367,16 +379,25 @@ __global__ __launch_bounds__(256) void k_pbe(const double *__restrict__ PC, doub
 #pragma unroll
     for (int b = 0; b < NB; ++b)
       if (b < npc) PB[(long)b * g.nv + s] = 0.0;
+    if constexpr (STEP) {  // as k_beta_step outside the cells
+      T *bp = (T *)St.bp;
+      bp[s] = T(0);
+      if (k == g.nz - 2)
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+          if (b < npc) St.PCw[(long)b * g.nv + s + g.np] = St.PCw[(long)b * g.nv + s];
+    }
     return;
   }
   const double eps = (double)dissip[s];
   const double ds = eps * ustar3 / z_i;
   const double et = pow(fabs(eps), 0.33333333);
   const double er = pow(fabs(ds), 1.0 / 3.0);
-  double gj[NB];
+  double gj[NB], pc[NB];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     gj[b] = 0.0;
+    pc[b] = 0.0;
     if (b < npc) {
       const PbeBin B = bins[b];
       const double tau_c = et * B.dpow_tau;
@@ -392,9 +413,11 @@ __global__ __launch_bounds__(256) void k_pbe(const double *__restrict__ PC, doub
       const double al = B.a * fb * pb;
       const double bl = fb * B.c * pd - B.e;
       const double fit = exp10(al + bl);
-      gj[b] = kpref * tau_c * fit * PC[(long)b * g.nv + s];
+      pc[b] = PC[(long)b * g.nv + s];
+      gj[b] = kpref * tau_c * fit * pc[b];
     }
   }
+  double tb = 0.0;
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     if (b < npc) {
@@ -404,8 +427,30 @@ __global__ __launch_bounds__(256) void k_pbe(const double *__restrict__ PC, doub
         if (j < npc) acc = fma(A[b * npc + j], gj[j], acc);
       const long o = (long)b * g.nv + s;
       PB[o] = acc;
-      if (feedback) RH[o] = RH[o] + acc;
+      double rh = RH[o];
+      if (feedback) {
+        rh = rh + acc;
+        RH[o] = rh;
+      }
+      if constexpr (STEP) {  // calcbeta_pcon from the old PCon, then step_pollen2
+        double rf;
+        if (St.euler) {
+          St.RF[o] = rh;
+          rf = rh;
+        } else {
+          rf = St.RF[o];
+        }
+        const double c = pc[b];
+        tb = tb + St.vfac[b] * c;
+        const double cn = c + St.dt * (1.5 * rh - 0.5 * rf);
+        St.PCw[o] = cn;
+        if (k == g.nz - 2) St.PCw[o + g.np] = cn;
+      }
     }
+  }
+  if constexpr (STEP) {
+    T *bp = (T *)St.bp;
+    bp[s] = St.active ? (T)tb : T(0);
   }
 }
 
