@@ -163,6 +163,26 @@ def test_steps_fp64(nx, nz, nsteps):
     assert lg.time() == o.time()
 
 
+@pytest.mark.parametrize("cs_count", [1, 2])
+def test_steps_lagrangian_state(cs_count):
+    """the Lagrangian model's state through whole steps: F_LM/MM/QN/NN, Cs_opt2, Beta and
+    the accumulated lag velocities.  cs_count = 1 updates the model every step from jt = 2
+    on (the lag velocities are zero between updates); cs_count = 2 alternates an
+    accumulation step with an update step.  REAL(4) casts off, so that every field is
+    compared at 1e-12."""
+    nx, nz, nsteps = 32, 17, 6
+    print(f"\n[case] lagrangian state cs_count={cs_count} {nx}x{nx}x{nz} n={nsteps}", flush=True)
+    cfg = abi.make_config(nx, nx, nz, cs_count=cs_count, dyn_init=2, fp32_casts=0)
+    o, lg = _pair(cfg)
+    st = seeded_state(cfg, log_law=False)
+    load_oracle(o, st); load_gpu(lg, st)
+    names = ["u", "v", "w", "F_LM", "F_MM", "F_QN", "F_NN", "Cs_opt2", "u_lag", "v_lag", "w_lag"]
+    for s in range(nsteps):
+        o.step(1); lg.step(1)
+        errs = _cmp(o, lg, names, levels=slice(0, nz - 1))
+        print(f"  step {s + 1}: " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
+
+
 @pytest.mark.parametrize("prec", [abi.LP_FP64, abi.LP_MIXED])
 def test_timestep_conditions_and_rmsdiv(prec):
     """scalars_module2.f90:1336-1379 and rmsdiv.f90:1-44 as device reductions"""
