@@ -884,10 +884,22 @@ static int do_lagrange(lp_ctx *c, bool use_u = false) {
   {
     const long ps = pc * gg.ny;
     ColOut o[2] = {{c->lbar, M_G1, p1, p1 * gg.ny, n1}, {c->lhat, M_G2, p2, p2 * gg.ny, n2}};
+    // full groups of TX columns in the block-wide pass; one or two trailing columns (nx =
+    // 256: the kx = 64 column that G_test keeps by rounding) one column per block
+    const int tx = c->fdt == F64 ? 8 : 16;
+    const int nmain = (nc % tx <= 2 && nc > tx) ? nc / tx * tx : nc;
     prof_begin(c, LP_K_COL_OP);
-    HIPCHK(col_op(c->fdt, gg.ny, gg.ny, 0, 1, 1, c->lspec, pc, ps, nc, kNumLag * gg.nz, 2, o, pc, ps, c->tab,
+    HIPCHK(col_op(c->fdt, gg.ny, gg.ny, 0, 1, 1, c->lspec, pc, ps, nmain, kNumLag * gg.nz, 2, o, pc, ps, c->tab,
                   c->s));
-    prof_end(c, LP_K_COL_OP, (double)kNumLag * gg.nz * gg.ny * (nc + n1 + n2) * 2 * fe);
+    prof_end(c, LP_K_COL_OP, (double)kNumLag * gg.nz * gg.ny * (nmain + std::min(nmain, n1) + std::min(nmain, n2)) * 2 * fe);
+    if (nmain < nc) {
+      ColOut ot[2];
+      int nt = 0;
+      if (nmain < n1) ot[nt++] = o[0];
+      if (nmain < n2) ot[nt++] = o[1];
+      HIPCHK(col_op_tail(c->fdt, gg.ny, c->lspec, pc, ps, nmain, nc, kNumLag * gg.nz, nt, ot, pc, ps, c->tab,
+                         c->s));
+    }
   }
   RCHK(fork_convec(c));  // in a step: convec overlaps the rest of the model
   double delta = cf.filter_size * std::pow(gg.dx * gg.dy * gg.dz, 1.0 / 3.0);

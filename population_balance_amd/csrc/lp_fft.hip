@@ -281,13 +281,14 @@ __device__ __forceinline__ void col_forward(cx<T> (&sp)[NY / kColLanes], int t, 
   }
 }
 
-template <typename T, int NYI, int NYO>
-__global__ __launch_bounds__(col_tx<T>() * kColLanes) void k_col_op(
+// TX columns per block (16 for binary32, 8 for binary64: one 128-byte run per y); TX = 1
+// serves a single trailing column.  Block column c handles kx = col0 + blockIdx.x*TX + c.
+template <typename T, int NYI, int NYO, int TX = col_tx<T>()>
+__global__ __launch_bounds__(TX * kColLanes) void k_col_op(
     const cx<T> *__restrict__ in, long in_pitch, long in_plane, int ncols, int fwd, int inv, ColParams Pr,
     long out_pitch, long out_plane, const double *__restrict__ kxt, const double *__restrict__ kyt,
     const double *__restrict__ G1, const double *__restrict__ G2, int lh_s, int ny_s,
-    const cx<T> *__restrict__ twI, const cx<T> *__restrict__ twO) {
-  constexpr int TX = col_tx<T>();
+    const cx<T> *__restrict__ twI, const cx<T> *__restrict__ twO, int col0) {
   constexpr int PL = kColLanes;
   using FI = Fft4<T, NYI, PL, -1>;
   using FO = Fft4<T, NYO, PL, +1>;
@@ -298,7 +299,7 @@ __global__ __launch_bounds__(col_tx<T>() * kColLanes) void k_col_op(
   constexpr int SS = col_slot_stride(NS);
   __shared__ cx<T> lds[TX * SS];
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
-  const int kx = blockIdx.x * TX + c;
+  const int kx = col0 + blockIdx.x * TX + c;
   const bool cv = kx < ncols;
   const int plane = blockIdx.y;
   cx<T> *sc = lds + c * SS;
@@ -599,12 +600,13 @@ hipError_t row_c2r(Dt dtc, Dt dto, int M, const void *in, long in_pitch, long in
 #undef CASE
 }
 
+// col0 > 0: the columns col0..ncols-1 only (same grid, one column per block)
 template <typename T, int NYI, int NYO>
 static hipError_t launch_col(int padmode, int fwd, int inv, const void *in, long in_pitch, long in_plane,
                              int ncols, int nplanes, int nout, const ColOut *outs, long out_pitch,
-                             long out_plane, const SpecTables &tab, hipStream_t s, Dt dt) {
+                             long out_plane, const SpecTables &tab, hipStream_t s, Dt dt, int col0 = 0) {
   const void *twI = tw_ptr(NYI, dt), *twO = tw_ptr(NYO, dt);
-  if (!twI || !twO) return hipErrorInvalidValue;
+  if (!twI || !twO || col0 < 0 || col0 >= ncols) return hipErrorInvalidValue;
   // the map is fixed by the lengths: same grid (0), padd (1), unpadd (2)
   if (padmode != (NYI == NYO ? 0 : (NYO > NYI ? 1 : 2))) return hipErrorInvalidValue;
   constexpr int TX = col_tx<T>();
@@ -620,11 +622,39 @@ static hipError_t launch_col(int padmode, int fwd, int inv, const void *in, long
     P.outs[j] = ColSlot{o.ptr, o.pitch ? o.pitch : out_pitch, o.plane ? o.plane : out_plane, o.mult, tab.lh,
                         o.ncols ? o.ncols : ncols, o.scale};
   }
+  if (col0 > 0) {
+    if constexpr (NYI == NYO) {
+      dim3 grid((unsigned)(ncols - col0), (unsigned)nplanes);
+      hipLaunchKernelGGL((k_col_op<T, NYI, NYO, 1>), grid, dim3(kColLanes), 0, s, (const cx<T> *)in, in_pitch,
+                         in_plane, ncols, fwd, inv, P, out_pitch, out_plane, tab.kx, tab.ky, tab.G1, tab.G2,
+                         tab.lh, tab.ny, (const cx<T> *)twI, (const cx<T> *)twO, col0);
+      return hipGetLastError();
+    } else {
+      return hipErrorInvalidValue;
+    }
+  }
   dim3 grid((unsigned)((ncols + TX - 1) / TX), (unsigned)nplanes);
   hipLaunchKernelGGL((k_col_op<T, NYI, NYO>), grid, dim3(TX * kColLanes), 0, s, (const cx<T> *)in, in_pitch,
                      in_plane, ncols, fwd, inv, P, out_pitch, out_plane, tab.kx, tab.ky, tab.G1, tab.G2, tab.lh,
-                     tab.ny, (const cx<T> *)twI, (const cx<T> *)twO);
+                     tab.ny, (const cx<T> *)twI, (const cx<T> *)twO, 0);
   return hipGetLastError();
+}
+
+hipError_t col_op_tail(Dt dt, int ny, const void *in, long in_pitch, long in_plane, int col0, int ncols,
+                       int nplanes, int nout, const ColOut *outs, long out_pitch, long out_plane,
+                       const SpecTables &tab, hipStream_t s) {
+  if (nout < 1 || nout > 3) return hipErrorInvalidValue;
+#define CASE(n)                                                                                          \
+  case n:                                                                                                \
+    return dt == F64 ? launch_col<double, n, n>(0, 1, 1, in, in_pitch, in_plane, ncols, nplanes, nout,  \
+                                                outs, out_pitch, out_plane, tab, s, dt, col0)           \
+                     : launch_col<float, n, n>(0, 1, 1, in, in_pitch, in_plane, ncols, nplanes, nout,   \
+                                               outs, out_pitch, out_plane, tab, s, dt, col0);
+  switch (ny) {
+    CASE(32) CASE(48) CASE(64) CASE(96) CASE(128) CASE(144) CASE(192) CASE(256) CASE(288) CASE(384)
+    default: return hipErrorInvalidValue;
+  }
+#undef CASE
 }
 
 hipError_t col_op(Dt dt, int nyi, int nyo, int padmode, int fwd, int inv, const void *in, long in_pitch,

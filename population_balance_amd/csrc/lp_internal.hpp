@@ -63,6 +63,12 @@ hipError_t col_op(Dt dt, int nyi, int nyo, int padmode, int fwd, int inv, const 
                   const ColOut *outs, long out_pitch, long out_plane, const SpecTables &tab,
                   hipStream_t s);
 
+// col_op (same grid, forward + inverse) on the columns col0..ncols-1 only, one column per
+// block: the few columns past the last full group of a block-wide launch.
+hipError_t col_op_tail(Dt dt, int ny, const void *in, long in_pitch, long in_plane, int col0, int ncols,
+                       int nplanes, int nout, const ColOut *outs, long out_pitch, long out_plane,
+                       const SpecTables &tab, hipStream_t s);
+
 // Column pass of the divergence terms: out = inverse_y( i kx fwd_y(A) + i ky fwd_y(B) )
 // with the Nyquist row and column zeroed; A, B, out share pitch / plane; ny == tab.ny.
 hipError_t col_div(Dt dt, int ny, const void *A, const void *B, long pitch, long plane, int ncols, int nplanes,
