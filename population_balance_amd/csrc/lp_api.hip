@@ -915,7 +915,8 @@ static int do_lagrange(lp_ctx *c, bool use_u = false) {
   P.bclip_min = 1.0 / (2.0 * 4.0);
   P.init_lm = cf.inilag && !c->F_LM_MM_init && (c->jt == cf.cs_count || c->jt == cf.dyn_init);
   P.init_qn = cf.inilag && !c->F_QN_NN_init && (c->jt == cf.cs_count || c->jt == cf.dyn_init);
-  LagInterp I{(double)cf.cs_count, cf.dt, gg.dx, gg.dy, gg.dz};
+  const double csd = (double)cf.cs_count;
+  LagInterp I{csd, cf.dt * csd / gg.dx, cf.dt * csd / gg.dy, cf.dt * csd / gg.dz};
   const double *Fold[4] = {(const double *)c->f[LP_F_LM], (const double *)c->f[LP_F_MM],
                            (const double *)c->f[LP_F_QN], (const double *)c->f[LP_F_NN]};
   double *Fnew[4] = {c->Fi[0], c->Fi[1], c->Fi[2], c->Fi[3]};

@@ -184,12 +184,12 @@ __global__ void k_nut_tau(const T *__restrict__ S11, const T *__restrict__ S12, 
 // interpolag_Sdep.f90:62-365: backtrace x - u dt and trilinear interpolation of the four
 // Lagrangian averages from the periodic (x, y) / clamped (z) copy of F at point
 // (x < nx, y, k); i = its flat index.
-// UNIT: cs == 1 (cs_count = 1), where the divisions by cs are the identity and are skipped
+// UNIT: cs == 1 (cs_count = 1), where the divisions by cs are the identity and are skipped.
+// sx, sy, sz = dt cs / dx, dt cs / dy, dt cs / dz (the backtrace scale factors).
 template <typename T, bool UNIT = false>
 __device__ __forceinline__ void interp_point(const double *const (&F)[4], double (&out)[4], const T *__restrict__ ul,
                                              const T *__restrict__ vl, const T *__restrict__ wl, Geo g, long i,
-                                             int x, int y, int k, double cs, double dt, double dx, double dy,
-                                             double dz) {
+                                             int x, int y, int k, double cs, double sx, double sy, double sz) {
   const int r = (int)(i - (long)k * g.np);
   auto dcs = [&](double v) { return UNIT ? v : v / cs; };
   // :204-258 lag velocities on w-nodes (u_lag at jz=1 stays on the uvp node)
@@ -200,20 +200,20 @@ __device__ __forceinline__ void interp_point(const double *const (&F)[4], double
     ulag = 0.5 * (ut + dcs((double)ul[i - g.np]));
     vlag = 0.5 * (vt + dcs((double)vl[i - g.np]));
   }
-  double xp = -ulag * dt * cs / dx;                    // :263-265
-  double yp = -vlag * dt * cs / dy;
+  double xp = -ulag * sx;                             // :263-265
+  double yp = -vlag * sy;
   double zp;
   if (k <= 1) {                                        // :281-291
     long i2 = (long)g.np + r;
     double w2 = dcs((double)wl[i2]);
-    double z2 = -w2 * dt * cs / dz;
+    double z2 = -w2 * sz;
     z2 = z2 + fmin(z2, 0.0);
     z2 = fmin(1.0, z2);
     z2 = fmax(-1.0, z2);
     zp = z2;
   } else {
     wlag = dcs((double)wl[i]);
-    zp = -wlag * dt * cs / dz;
+    zp = -wlag * sz;
   }
   int addx = signbit(xp) ? -1 : 1, addy = signbit(yp) ? -1 : 1, addz = signbit(zp) ? -1 : 1;
   double cxp = fabs(xp), cyp = fabs(yp), czp = fabs(zp);
@@ -257,8 +257,13 @@ __global__ void k_interpolag(const double *__restrict__ F0, const double *__rest
   }
   const double *const F[4] = {F0, F1, F2, F3};
   double v[4];
-  interp_point<T>(F, v, ul, vl, wl, g, i, x, y, k, cs, dt, dx, dy, dz);
+  interp_point<T>(F, v, ul, vl, wl, g, i, x, y, k, cs, dt * cs / dx, dt * cs / dy, dt * cs / dz);
   I0[i] = v[0]; I1[i] = v[1]; I2[i] = v[2]; I3[i] = v[3];
+}
+
+// x^p for the relaxation time scale; the reference's p = -1/8 as three square roots
+__device__ __forceinline__ double pow_coeff(double x, double p) {
+  return p == -0.125 ? 1.0 / sqrt(sqrt(sqrt(x))) : pow(x, p);
 }
 
 __device__ __forceinline__ double r4max(double a, float b) {  // (double) max(real(a), b)
@@ -303,18 +308,17 @@ __device__ __forceinline__ void relax_point(double LM, double MM, double QN, dou
   double flm = fi[0], fmm = fi[1], fqn = fi[2], fnn = fi[3];
   const double tiny = P.tiny;
   if (P.init_lm) { fmm = MM; flm = 0.0; if (padc) { fmm = 1.0; flm = 1.0; } }
-  double Tn, dum, epsi;
+  double Tn, epsi;
   if (P.casts) {
     Tn = (double)fmaxf((float)(flm * fmm), 1e-24f);
-    Tn = P.opftdelta * pow(Tn, P.powcoeff);
+    Tn = P.opftdelta * pow_coeff(Tn, P.powcoeff);
     Tn = (double)fmaxf(1e-24f, (float)Tn);
   } else {
     Tn = fmax(flm * fmm, tiny);
-    Tn = P.opftdelta * pow(Tn, P.powcoeff);
+    Tn = P.opftdelta * pow_coeff(Tn, P.powcoeff);
     Tn = fmax(tiny, Tn);
   }
-  dum = P.lagran_dt / Tn;
-  epsi = dum / (1.0 + dum);
+  epsi = P.lagran_dt / (Tn + P.lagran_dt);  // dum / (1 + dum), dum = lagran_dt / Tn
   flm = (epsi * LM + (1.0 - epsi) * flm);
   fmm = (epsi * MM + (1.0 - epsi) * fmm);
   flm = P.casts ? r4max(flm, 1e-24f) : fmax(tiny, flm);
@@ -324,15 +328,14 @@ __device__ __forceinline__ void relax_point(double LM, double MM, double QN, dou
   if (P.init_qn) { fnn = NN; fqn = 0.03 * NN; if (padc) { fnn = 1.0; fqn = 1.0; } }
   if (P.casts) {
     Tn = (double)fmaxf((float)(fqn * fnn), 1e-24f);
-    Tn = P.opftdelta * pow(Tn, P.powcoeff);
+    Tn = P.opftdelta * pow_coeff(Tn, P.powcoeff);
     Tn = (double)fmaxf(1e-24f, (float)Tn);
   } else {
     Tn = fmax(fqn * fnn, tiny);
-    Tn = P.opftdelta * pow(Tn, P.powcoeff);
+    Tn = P.opftdelta * pow_coeff(Tn, P.powcoeff);
     Tn = fmax(tiny, Tn);
   }
-  dum = P.lagran_dt / Tn;
-  epsi = dum / (1.0 + dum);
+  epsi = P.lagran_dt / (Tn + P.lagran_dt);  // dum / (1 + dum), dum = lagran_dt / Tn
   fqn = (epsi * QN + (1.0 - epsi) * fqn);
   fnn = (epsi * NN + (1.0 - epsi) * fnn);
   fqn = P.casts ? r4max(fqn, 1e-24f) : fmax(tiny, fqn);

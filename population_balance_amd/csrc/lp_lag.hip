@@ -240,7 +240,7 @@ __global__ __launch_bounds__(256) void k_lag_relax(const double *__restrict__ lm
   const double *const Fo[4] = {Fold.p[0], Fold.p[1], Fold.p[2], Fold.p[3]};
   double fi[4];
   if (x < g.nx) {
-    interp_point<T, UNIT>(Fo, fi, ul, vl, wl, g, i, x, y, k, I.cs, I.dt, I.dx, I.dy, I.dz);
+    interp_point<T, UNIT>(Fo, fi, ul, vl, wl, g, i, x, y, k, I.cs, I.sx, I.sy, I.sz);
   } else {  // interpolag_Sdep.f90 leaves the padding columns alone
 #pragma unroll
     for (int q = 0; q < 4; ++q) fi[q] = Fo[q][i];

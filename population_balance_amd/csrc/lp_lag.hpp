@@ -21,7 +21,8 @@ struct LagInputs {
 };
 
 struct LagInterp {
-  double cs, dt, dx, dy, dz;
+  double cs;          // cs_count
+  double sx, sy, sz;  // dt cs / dx, dt cs / dy, dt cs / dz
 };
 
 // spec: [21][nz][ny][pitch] complex of the field type, the leading nc columns written
