@@ -18,8 +18,12 @@
 
 namespace lp {
 
+// lanes per transform: 16 for the long transforms (Q = 12 for M = 192, 9 for M = 144),
+// which keeps the per-lane values of a transform within the register budget
+__host__ __device__ constexpr int cv_lanes(int M) { return (M == 192 || M == 144) ? 16 : row_lanes(M); }
+
 template <typename T, int M> struct CvCfg {
-  static constexpr int P = row_lanes(M);
+  static constexpr int P = cv_lanes(M);
   static constexpr int L = sizeof(T) == 4 ? 4 : 2;  // product levels per block
   static constexpr int NU = L + 1;                  // levels held of u1, u2, u3, w1, w2
   static constexpr int NS = 6 * L + 5;              // x back-transforms per block
