@@ -123,7 +123,7 @@ struct lp_ctx {
   void *S[6] = {};            // strain rates
   void *lspec = nullptr;      // x spectra of the 21 test-filter inputs [21][nz][ny][pad16(nc)]
   void *lbar = nullptr, *lhat = nullptr;  // their y-filtered spectra [21][nz][ny][pad16(ncut)]
-  double *lcon[4] = {};       // Germano contractions L.M, M.M, Q.N, N.N (binary64 volumes)
+  void *lcon[4] = {};         // Germano contractions L.M, M.M, Q.N, N.N (field-type volumes)
   double *Fi[4] = {};         // second buffer of the Lagrangian averages (exchanged with F)
   void *Hs[3] = {};           // pressure RHS spectra
   void *bw = nullptr, *tw = nullptr;  // wall / lid divergence spectra
@@ -581,7 +581,7 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
     CK(dalloc(&c->lspec, pl * pad16(nc)));
     CK(dalloc(&c->lbar, pl * pad16(c->ncut1)));
     CK(dalloc(&c->lhat, pl * pad16(c->ncut2)));
-    for (auto &p : c->lcon) CK(dalloc((void **)&p, sizeof(double) * nv));
+    for (auto &p : c->lcon) CK(dalloc(&p, fe * nv));
     for (auto &p : c->Fi) CK(dalloc((void **)&p, sizeof(double) * nv));
   }
   for (auto &p : c->Hs) CK(dalloc(&p, spec_b));
@@ -681,11 +681,13 @@ static int r2c_small(lp_ctx *c, const void *in, int nlev, double scale, void *sp
   prof_end(c, LP_K_ROW_R2C, bytes_r2c(g.nx, g.ny, nlev, ncols_out, esz_of(c->fdt)));
   return LP_OK;
 }
-static int c2r_small(lp_ctx *c, const void *spec, int ncols, int nlev, void *out, Dt dto) {
+// ikx: c->kx (its first row is kx of every column) to store the x-derivative instead
+static int c2r_small(lp_ctx *c, const void *spec, int ncols, int nlev, void *out, Dt dto,
+                     const double *ikx = nullptr) {
   const lp_grid &g = c->g;
   prof_begin(c, LP_K_ROW_C2R);
   HIPCHK(row_c2r(c->fdt, dto, g.nx / 2, spec, c->sp, c->sp * g.ny, ncols, g.ny, nlev, out, g.ld,
-                 plane_el(c), c->s));
+                 plane_el(c), c->s, ikx));
   prof_end(c, LP_K_ROW_C2R, bytes_c2r(g.nx, g.ny, nlev, ncols, esz_of(c->fdt), esz_of(dto)));
   return LP_OK;
 }
@@ -753,15 +755,17 @@ static int from_big(lp_ctx *c, const void *fb, int nlev, double scale, void *out
 // routines on the context state
 // ------------------------------------------------------------------------------------
 // keep (optional): receives the x-spectrum of the filtered f (kx < lh), which the
-// convection's 3/2 padding of the same field starts from
+// convection's 3/2 padding of the same field starts from.  kx is uniform along a column,
+// so the (kx, y) spectrum of df/dx is i kx times that of f: the column pass produces f
+// and df/dy only, and df/dx is the back-transform of f's spectrum times i kx.
 static int do_filt_da(lp_ctx *c, void *f, void *dfdx, void *dfdy, int nlev, void *keep = nullptr) {
   double cst = 1.0 / ((double)c->g.nx * c->g.ny);
   RCHK(r2c_small(c, f, nlev, cst, c->spec[0]));
   void *s0 = keep ? keep : c->spec[0];
-  ColOut o[3] = {{s0, M_ONE}, {c->spec[1], M_IKX}, {c->spec[2], M_IKY}};
-  RCHK(col_small(c, c->spec[0], nlev, 1, 1, 3, o));
+  ColOut o[2] = {{s0, M_ONE}, {c->spec[2], M_IKY}};
+  RCHK(col_small(c, c->spec[0], nlev, 1, 1, 2, o));
   RCHK(c2r_small(c, s0, c->g.lh, nlev, f, c->fdt));
-  RCHK(c2r_small(c, c->spec[1], c->g.lh, nlev, dfdx, c->fdt));
+  RCHK(c2r_small(c, s0, c->g.lh, nlev, dfdx, c->fdt, c->kx));
   return c2r_small(c, c->spec[2], c->g.lh, nlev, dfdy, c->fdt);
 }
 
@@ -923,13 +927,13 @@ static int do_lagrange(lp_ctx *c, bool use_u = false) {
   // :257-326 back along x and contracted, row by row
   prof_begin(c, LP_K_ROW_C2R);
   HIPCHK(lag_c2r_contract(c->fdt, c->lbar, c->lhat, n1, n2, p1, p2, c->lcon, g, P, c->s));
-  prof_end(c, LP_K_ROW_C2R, (double)kNumLag * gg.ny * gg.nz * (n1 + n2) * 2 * fe + vol * 4 * 8);
+  prof_end(c, LP_K_ROW_C2R, (double)kNumLag * gg.ny * gg.nz * (n1 + n2) * 2 * fe + vol * 4 * fe);
   // :327-434 (+ interpolag_Sdep.f90:62-365): interpolation and relaxation
   prof_begin(c, LP_K_LAG_RELAX);
   const int lv[3] = {use_u ? LP_U : LP_U_LAG, use_u ? LP_V : LP_V_LAG, use_u ? LP_W : LP_W_LAG};
   HIPCHK(lag_relax(c->fdt, c->lcon, Fold, Fnew, c->f[lv[0]], c->f[lv[1]], c->f[lv[2]], c->f[LP_CS_OPT2],
                    c->f[LP_BETA], g, P, I, c->s));
-  prof_end(c, LP_K_LAG_RELAX, vol * (4.0 * 8 + 8.0 * 8 + 5.0 * fe));
+  prof_end(c, LP_K_LAG_RELAX, vol * (8.0 * 8 + 9.0 * fe));
   swap_F(c);
   if (!use_u) RCHK(reset_lag_velocities(c));
   if (P.init_lm) c->F_LM_MM_init = 1;
@@ -1140,16 +1144,17 @@ static int do_press(lp_ctx *c) {
   // H_x, H_y, H_z read once, p_hat written once (binary64 complex)
   prof_end(c, LP_K_PRESS_TRIDIAG,
            (double)nmodes * (gg.nz - 1) * 3 * 2 * esz_of(c->fdt) + (double)nmodes * (gg.nz + 1) * 16);
-  // :339-352 inverse transforms in binary64: p(0..nz-1), dpdx, dpdy (1..nz-1)
+  // :339-352 inverse transforms in binary64: p(0..nz-1), dpdx, dpdy (1..nz-1); dpdx is
+  // the back-transform of p's (kx, y) spectrum times i kx (kx is uniform along a column)
   {
-    ColOut o[3] = {{c->pspec[0], M_ONE}, {c->pspec[1], M_IKX}, {c->pspec[2], M_IKY}};
-    HIPCHK(col_op(F64, gg.ny, gg.ny, 0, 0, 1, ph, c->sp, ps, gg.lh, gg.nz, 3, o, c->sp, ps, c->tab, c->s));
+    ColOut o[2] = {{c->pspec[0], M_ONE}, {c->pspec[2], M_IKY}};
+    HIPCHK(col_op(F64, gg.ny, gg.ny, 0, 0, 1, ph, c->sp, ps, gg.lh, gg.nz, 2, o, c->sp, ps, c->tab, c->s));
   }
   HIPCHK(row_c2r(F64, F64, gg.nx / 2, c->pspec[0], c->sp, ps, gg.lh, gg.ny, gg.nz, c->f[LP_P], gg.ld, np,
                  c->s));
   Dt dto = c->fdt;
-  HIPCHK(row_c2r(F64, dto, gg.nx / 2, (const cx<double> *)c->pspec[1] + ps, c->sp, ps, gg.lh, gg.ny,
-                 gg.nz - 1, c->f[LP_DPDX], gg.ld, np, c->s));
+  HIPCHK(row_c2r(F64, dto, gg.nx / 2, (const cx<double> *)c->pspec[0] + ps, c->sp, ps, gg.lh, gg.ny,
+                 gg.nz - 1, c->f[LP_DPDX], gg.ld, np, c->s, c->kx));
   HIPCHK(row_c2r(F64, dto, gg.nx / 2, (const cx<double> *)c->pspec[2] + ps, c->sp, ps, gg.lh, gg.ny,
                  gg.nz - 1, c->f[LP_DPDY], gg.ld, np, c->s));
   // :355-357

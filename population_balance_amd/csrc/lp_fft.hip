@@ -139,6 +139,15 @@ __global__ __launch_bounds__(kR2cBlock) void k_row_r2c(const T *__restrict__ in,
   }
 }
 
+// merge of the half spectrum into the M-point complex input of a 2M-point real inverse
+// transform: Y[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k]), w = exp(-2 pi i/2M)
+template <typename T> __device__ __forceinline__ cx<T> merge_c2r(cx<T> xa, cx<T> xm, cx<T> w) {
+  const cx<T> xb = conjg(xm);
+  const cx<T> e = xa + xb, o = xa - xb;
+  const cx<T> wo = cmul(conjg(w), o);
+  return mk<T>(e.x - wo.y, e.y + wo.x);
+}
+
 // ---------------------------------------------------------------------------------
 // row_c2r: M+1 complex (first ncols_in present; imaginary parts of kx = 0 and kx = M
 // ignored, as FFTW2's halfcomplex c2r) -> 2M reals, columns [2M, out_pitch) zeroed
@@ -147,14 +156,16 @@ __global__ __launch_bounds__(kR2cBlock) void k_row_r2c(const T *__restrict__ in,
 // transform plus dz^-1 (tz(k+1) - tz(k)), BOGUS on the top level; 2 (divstress_w.f90:35,
 // 46-81 with the wall at level 1 and no lid term) the transform plus dz^-1 (tz(k) - tz(k-1))
 // on levels 1..nz-2.  The padding columns are zero (BOGUS on the uv top level).  tz has the
-// layout of out.
+// layout of out.  ikx (optional): the input is multiplied by i*ikx[kx] first, which turns
+// the (kx, y) spectrum of f into that of df/dx (ddx.f90:15-24; kx is uniform along y).
 template <typename T, typename TO, int M, int EPI>
 __global__ __launch_bounds__(kRowBlock) void k_row_c2r(const cx<T> *__restrict__ in, long in_pitch, long in_plane,
                                                        int ncols_in, int rows, int nplanes,
                                                        TO *__restrict__ out, long out_pitch, long out_plane,
                                                        const cx<T> *__restrict__ twM,
                                                        const cx<T> *__restrict__ twN,
-                                                       const TO *__restrict__ tz, TO cst) {
+                                                       const TO *__restrict__ tz, TO cst,
+                                                       const double *__restrict__ ikx) {
   constexpr int P = row_lanes(M);
   using F = Fft4<T, M, P, +1>;
   constexpr int Q = F::Q;
@@ -172,15 +183,21 @@ __global__ __launch_bounds__(kRowBlock) void k_row_c2r(const cx<T> *__restrict__
     if (k == 0 || k == M) x.y = 0;
     return x;
   };
-  // merge: Y[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k])
   cx<T> v[Q];
+  if (ikx) {  // i kx X[k]: the kx = 0 and kx = M entries of the table are zero
 #pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const int k = t + P * q;
-    cx<T> xa = ld(k), xb = conjg(ld(M - k));
-    cx<T> e = xa + xb, o = xa - xb;
-    cx<T> wo = cmul(conjg(twN[k]), o);
-    v[q] = mk<T>(e.x - wo.y, e.y + wo.x);
+    for (int q = 0; q < Q; ++q) {
+      const int k = t + P * q;
+      const T sa = (T)ikx[k], sb = (T)ikx[M - k];
+      const cx<T> ra = ld(k), rb = ld(M - k);
+      v[q] = merge_c2r(mk<T>(-ra.y * sa, ra.x * sa), mk<T>(-rb.y * sb, rb.x * sb), twN[k]);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int k = t + P * q;
+      v[q] = merge_c2r(ld(k), ld(M - k), twN[k]);
+    }
   }
   F::stage1(v, t, sc, twM);
   slot_sync<P>();
@@ -545,7 +562,7 @@ hipError_t row_r2c(Dt dt, int M, const void *in, long in_pitch, long in_plane, i
 template <typename T, typename TO, int M, int EPI = 0>
 static hipError_t launch_c2r(const void *in, long in_pitch, long in_plane, int ncols_in, int rows,
                              int nplanes, void *out, long out_pitch, long out_plane, hipStream_t s, Dt dt,
-                             const void *tz = nullptr, double cst = 0.0) {
+                             const void *tz = nullptr, double cst = 0.0, const double *ikx = nullptr) {
   const void *twM = tw_ptr(M, dt), *twN = tw_ptr(2 * M, dt);
   if (!twM || !twN) return hipErrorInvalidValue;
   constexpr int RPB = kRowBlock / row_lanes(M);
@@ -553,7 +570,7 @@ static hipError_t launch_c2r(const void *in, long in_pitch, long in_plane, int n
   dim3 grid((unsigned)((total + RPB - 1) / RPB));
   hipLaunchKernelGGL((k_row_c2r<T, TO, M, EPI>), grid, dim3(kRowBlock), 0, s, (const cx<T> *)in, in_pitch,
                      in_plane, ncols_in, rows, nplanes, (TO *)out, out_pitch, out_plane, (const cx<T> *)twM,
-                     (const cx<T> *)twN, (const TO *)(EPI ? tz : out), (TO)cst);
+                     (const cx<T> *)twN, (const TO *)(EPI ? tz : out), (TO)cst, ikx);
   return hipGetLastError();
 }
 
@@ -580,18 +597,19 @@ hipError_t row_c2r_div(Dt dt, int M, const void *in, long in_pitch, long in_plan
 }
 
 hipError_t row_c2r(Dt dtc, Dt dto, int M, const void *in, long in_pitch, long in_plane, int ncols_in,
-                   int rows, int nplanes, void *out, long out_pitch, long out_plane, hipStream_t s) {
-#define CASE(m)                                                                                        \
-  case m:                                                                                              \
-    if (dtc == F64 && dto == F64)                                                                      \
-      return launch_c2r<double, double, m>(in, in_pitch, in_plane, ncols_in, rows, nplanes, out,       \
-                                           out_pitch, out_plane, s, dtc);                              \
-    if (dtc == F64 && dto == F32)                                                                      \
-      return launch_c2r<double, float, m>(in, in_pitch, in_plane, ncols_in, rows, nplanes, out,        \
-                                          out_pitch, out_plane, s, dtc);                               \
-    if (dtc == F32 && dto == F32)                                                                      \
-      return launch_c2r<float, float, m>(in, in_pitch, in_plane, ncols_in, rows, nplanes, out,         \
-                                         out_pitch, out_plane, s, dtc);                                \
+                   int rows, int nplanes, void *out, long out_pitch, long out_plane, hipStream_t s,
+                   const double *ikx) {
+#define CASE(m)                                                                                         \
+  case m:                                                                                               \
+    if (dtc == F64 && dto == F64)                                                                       \
+      return launch_c2r<double, double, m>(in, in_pitch, in_plane, ncols_in, rows, nplanes, out,        \
+                                           out_pitch, out_plane, s, dtc, nullptr, 0.0, ikx);            \
+    if (dtc == F64 && dto == F32)                                                                       \
+      return launch_c2r<double, float, m>(in, in_pitch, in_plane, ncols_in, rows, nplanes, out,         \
+                                          out_pitch, out_plane, s, dtc, nullptr, 0.0, ikx);             \
+    if (dtc == F32 && dto == F32)                                                                       \
+      return launch_c2r<float, float, m>(in, in_pitch, in_plane, ncols_in, rows, nplanes, out,          \
+                                         out_pitch, out_plane, s, dtc, nullptr, 0.0, ikx);              \
     return hipErrorInvalidValue;
   switch (M) {
     LP_ROW_LENGTHS(CASE)

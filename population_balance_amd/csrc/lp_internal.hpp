@@ -33,9 +33,11 @@ hipError_t row_r2c(Dt dt, int M, const void *in, long in_pitch, long in_plane,
 
 // Row pass: complex half spectra (ncols_in columns present, rest zero) -> real rows
 // of length 2M; columns [2M, out_pitch) set to zero.  Compute type dtc, store type dto.
+// ikx (optional, M+1 values, zero at kx = 0 and kx = M): the spectra are multiplied by
+// i*ikx[kx] first, so the rows are the x-derivative of what they would be without it.
 hipError_t row_c2r(Dt dtc, Dt dto, int M, const void *in, long in_pitch, long in_plane,
                    int ncols_in, int rows, int nplanes, void *out, long out_pitch,
-                   long out_plane, hipStream_t s);
+                   long out_plane, hipStream_t s, const double *ikx = nullptr);
 
 // row_r2c of a vorticity component formed from two derivative volumes (comp 0: w_y - v_z
 // from a = dwdy, b = dvdz; 1: u_z - w_x from a = dwdx, b = dudz; 2: v_x - u_y from

@@ -145,6 +145,11 @@ __global__ __launch_bounds__(256) void k_lag_r2c(LagIn<T> in, Geo g, cx<T> *__re
   }
 }
 
+// LDS row stride (elements) of the physical rows in k_lag_c2r_contract: at least ld, and
+// = 16 (mod 64), so the 8-byte row writes of the slots of a wavefront (8 lanes per slot,
+// consecutive pairs) fall on distinct banks
+__host__ __device__ constexpr int lag_row_stride(int ld) { return (ld - 16 + 63) / 64 * 64 + 16; }
+
 // lagrange_Sdep.f90:257-326: x back-transform of the 21 fields of one width for one
 // (y, z) row and their Germano contractions.  blockIdx.x = row, blockIdx.y = width
 // (0: 2 delta -> L.M, M.M; 1: 4 delta -> Q.N, N.N).  21 transforms of P lanes each; the
@@ -152,19 +157,21 @@ __global__ __launch_bounds__(256) void k_lag_r2c(LagIn<T> in, Geo g, cx<T> *__re
 // it.  c0/c1: the two contractions (binary64 volumes) of each width.
 template <typename T, int M>
 __global__ __launch_bounds__(256) void k_lag_c2r_contract(const cx<T> *__restrict__ bar, const cx<T> *__restrict__ hat,
-                                                          int n1, int n2, long p1, long p2, double *__restrict__ lm,
-                                                          double *__restrict__ mm, double *__restrict__ qn,
-                                                          double *__restrict__ nn, Geo g, double cst, double tf1_2,
+                                                          int n1, int n2, long p1, long p2, T *__restrict__ lm,
+                                                          T *__restrict__ mm, T *__restrict__ qn,
+                                                          T *__restrict__ nn, Geo g, double cst, double tf1_2,
                                                           double tf2_2, const cx<T> *__restrict__ twM,
                                                           const cx<T> *__restrict__ twN) {
   constexpr int PL = row_lanes(M);
   using F = Fft4<T, M, PL, +1>;
   constexpr int Q = F::Q;
   constexpr int SS = slot_stride(F::SCR);
+  using v2 = typename V2<T>::t;
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T> *scr = reinterpret_cast<cx<T> *>(smem);  // [21][SS] during the transforms
-  T *vals = reinterpret_cast<T *>(smem);         // [21][ld] afterwards
+  T *vals = reinterpret_cast<T *>(smem);         // [21][rs] afterwards
   const int ldp = g.ld;
+  const int rs = lag_row_stride(ldp);
   const int row = blockIdx.x, half = blockIdx.y;
   const int k = row / g.ny, y = row % g.ny;
   const int slot = threadIdx.x / PL, t = threadIdx.x % PL;
@@ -203,54 +210,109 @@ __global__ __launch_bounds__(256) void k_lag_c2r_contract(const cx<T> *__restric
 #pragma unroll
       for (int k1 = 0; k1 < PL; ++k1) {
         const int n = t + PL * j + Q * k1;
-        vals[q * ldp + 2 * n] = a[j][k1].x;
-        vals[q * ldp + 2 * n + 1] = a[j][k1].y;
+        *reinterpret_cast<v2 *>(vals + q * rs + 2 * n) = v2{a[j][k1].x, a[j][k1].y};
       }
     }
   if (act)
-    for (int x = 2 * M + t; x < ldp; x += PL) vals[q * ldp + x] = T(0);
+    for (int x = 2 * M + t; x < ldp; x += PL) vals[q * rs + x] = T(0);
   __syncthreads();
-  double *o0 = half == 0 ? lm : qn, *o1 = half == 0 ? mm : nn;
+  T *o0 = half == 0 ? lm : qn, *o1 = half == 0 ? mm : nn;
   const double tf = half == 0 ? tf1_2 : tf2_2;
   const long base = (long)k * g.np + (long)y * g.ld;
   for (int x = threadIdx.x; x < ldp; x += blockDim.x) {
     double c0, c1;
-    contract_point<T>([&](int qq) -> T { return vals[qq * ldp + x]; }, cst, tf, c0, c1);
-    o0[base + x] = c0;
-    o1[base + x] = c1;
+    contract_point<T>([&](int qq) -> T { return vals[qq * rs + x]; }, cst, tf, c0, c1);
+    o0[base + x] = (T)c0;  // c0, c1 are values of T: exact
+    o1[base + x] = (T)c1;
   }
+}
+
+// element i of p through a 32-bit byte offset (a volume level is far below 4 GiB), so a
+// load or store is one VGPR offset from a wave-uniform base
+template <typename V> __device__ __forceinline__ const V &at(const V *p, unsigned i) {
+  return *reinterpret_cast<const V *>(reinterpret_cast<const char *>(p) + i * (unsigned)sizeof(V));
+}
+template <typename V> __device__ __forceinline__ V &at(V *p, unsigned i) {
+  return *reinterpret_cast<V *>(reinterpret_cast<char *>(p) + i * (unsigned)sizeof(V));
 }
 
 // lagrange_Sdep.f90:327-434 with interpolag_Sdep.f90:62-365, one thread per point: the
 // old averages are gathered at the backtraced point, relaxed with the contractions and
-// written to Fnew.
+// written to Fnew.  The level k is uniform per block, so the level bases are wave-uniform
+// and every gather of the backtrace (levels k-1 .. k+1) is a 32-bit offset from level
+// kb = max(k-1, 0).  The interpolation is interp_point's, term for term.
 template <typename T, bool UNIT>
-__global__ __launch_bounds__(256) void k_lag_relax(const double *__restrict__ lm, const double *__restrict__ mm,
-                                                   const double *__restrict__ qn, const double *__restrict__ nn,
+__global__ __launch_bounds__(256) void k_lag_relax(const T *__restrict__ lm, const T *__restrict__ mm,
+                                                   const T *__restrict__ qn, const T *__restrict__ nn,
                                                    F4c Fold, F4 Fnew, const T *__restrict__ ul,
                                                    const T *__restrict__ vl, const T *__restrict__ wl,
                                                    T *__restrict__ Cs, T *__restrict__ Beta, Geo g, LagParams P,
                                                    LagInterp I) {
-  const long nv = g.np * g.nz;
-  const long i = gtid();
-  if (i >= nv) return;
-  const int k = (int)(i / g.np);
-  const int r = (int)(i % g.np);
-  const int y = r / g.ld, x = r % g.ld;
-  const double *const Fo[4] = {Fold.p[0], Fold.p[1], Fold.p[2], Fold.p[3]};
+  int k, ri, y, x;  // level-chunk order (lc_point): the backtrace reads levels k-1 .. k+1
+  if (!lc_point(g.np, g.nz, k, ri)) return;
+  lc_split(ri, g.ld, y, x);
+  const unsigned r = (unsigned)ri;  // point within the level
+  const long pk = (long)k * g.np;
   double fi[4];
   if (x < g.nx) {
-    interp_point<T, UNIT>(Fo, fi, ul, vl, wl, g, i, x, y, k, I.cs, I.sx, I.sy, I.sz);
+    auto dcs = [&](double v) { return UNIT ? v : v / I.cs; };
+    // interpolag_Sdep.f90:204-258 lag velocities on w-nodes (u_lag at jz=1 stays on the uvp node)
+    const double ut = dcs((double)at(ul + pk, r)), vt = dcs((double)at(vl + pk, r));
+    double ulag, vlag;
+    if (k == 0) {
+      ulag = ut;
+      vlag = vt;
+    } else {
+      ulag = 0.5 * (ut + dcs((double)at(ul + (pk - g.np), r)));
+      vlag = 0.5 * (vt + dcs((double)at(vl + (pk - g.np), r)));
+    }
+    const double xp = -ulag * I.sx;  // :263-265
+    const double yp = -vlag * I.sy;
+    double zp;
+    if (k <= 1) {  // :281-291
+      const double w2 = dcs((double)at(wl + g.np, r));
+      double z2 = -w2 * I.sz;
+      z2 = z2 + fmin(z2, 0.0);
+      z2 = fmin(1.0, z2);
+      z2 = fmax(-1.0, z2);
+      zp = z2;
+    } else {
+      const double wlag = dcs((double)at(wl + pk, r));
+      zp = -wlag * I.sz;
+    }
+    const int addx = signbit(xp) ? -1 : 1, addy = signbit(yp) ? -1 : 1, addz = signbit(zp) ? -1 : 1;
+    const double cxp = fabs(xp), cyp = fabs(yp), czp = fabs(zp);
+    const double fx = 1.0 - cxp, fy = 1.0 - cyp, fz = 1.0 - czp;
+    // periodic in x and y (x + addx in [-1, nx], y + addy in [-1, ny]), clamped in z
+    int xa = x + addx, ya = y + addy, ka = k + addz;
+    xa = xa < 0 ? xa + g.nx : (xa >= g.nx ? xa - g.nx : xa);
+    ya = ya < 0 ? ya + g.ny : (ya >= g.ny ? ya - g.ny : ya);
+    ka = ka < 0 ? 0 : (ka > g.nz - 1 ? g.nz - 1 : ka);
+    const int kb = k > 0 ? k - 1 : 0;
+    const long pb = (long)kb * g.np;
+    const unsigned dk = (unsigned)((k - kb) * g.np), da = (unsigned)((ka - kb) * g.np);
+    const unsigned o00 = dk + (unsigned)(y * g.ld), o0a = da + (unsigned)(y * g.ld);
+    const unsigned oa0 = dk + (unsigned)(ya * g.ld), oaa = da + (unsigned)(ya * g.ld);
+    const unsigned ux = (unsigned)x, uxa = (unsigned)xa;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double *f = Fold.p[q] + pb;
+      fi[q] = fx * fy * (at(f, o00 + ux) * fz + at(f, o0a + ux) * czp)
+            + fx * cyp * (at(f, oa0 + ux) * fz + at(f, oaa + ux) * czp)
+            + cxp * fy * (at(f, o00 + uxa) * fz + at(f, o0a + uxa) * czp)
+            + cxp * cyp * (at(f, oa0 + uxa) * fz + at(f, oaa + uxa) * czp);
+    }
   } else {  // interpolag_Sdep.f90 leaves the padding columns alone
 #pragma unroll
-    for (int q = 0; q < 4; ++q) fi[q] = Fo[q][i];
+    for (int q = 0; q < 4; ++q) fi[q] = at(Fold.p[q] + pk, r);
   }
   T cs, b;
-  relax_point<T>(lm[i], mm[i], qn[i], nn[i], fi, x >= g.ld - 2, k == g.nz - 1, P, cs, b);
+  relax_point<T>((double)at(lm + pk, r), (double)at(mm + pk, r), (double)at(qn + pk, r), (double)at(nn + pk, r), fi,
+                 x >= g.ld - 2, k == g.nz - 1, P, cs, b);
 #pragma unroll
-  for (int q = 0; q < 4; ++q) Fnew.p[q][i] = fi[q];
-  Cs[i] = cs;
-  Beta[i] = b;
+  for (int q = 0; q < 4; ++q) at(Fnew.p[q] + pk, r) = fi[q];
+  at(Cs + pk, r) = cs;
+  at(Beta + pk, r) = b;
 }
 
 // ------------------------------------------------------------------------------------
@@ -288,13 +350,13 @@ hipError_t lag_r2c(Dt dt, const LagInputs &in, Geo g, void *spec, int nc, long p
 
 template <typename T, int M>
 static hipError_t launch_lag_contract(const void *bar, const void *hat, int n1, int n2, long p1, long p2,
-                                      double *const C[4], Geo g, const LagParams &P, hipStream_t s, Dt dt) {
+                                      void *const C[4], Geo g, const LagParams &P, hipStream_t s, Dt dt) {
   const void *twM = tw_ptr(M, dt), *twN = tw_ptr(2 * M, dt);
   if (!twM || !twN || n1 < 1 || n2 < 1 || n1 > M + 1 || n2 > M + 1) return hipErrorInvalidValue;
   constexpr int PL = row_lanes(M);
   using F = Fft4<T, M, PL, +1>;
   const size_t lds_fft = (size_t)21 * slot_stride(F::SCR) * sizeof(cx<T>);
-  const size_t lds_rows = (size_t)21 * g.ld * sizeof(T);
+  const size_t lds_rows = (size_t)21 * lag_row_stride(g.ld) * sizeof(T);
   const size_t lds = lds_fft > lds_rows ? lds_fft : lds_rows;
   const int threads = (21 * PL + 63) / 64 * 64;
   if (lds > 65536) {
@@ -303,13 +365,14 @@ static hipError_t launch_lag_contract(const void *bar, const void *hat, int n1, 
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL((k_lag_c2r_contract<T, M>), dim3((unsigned)(g.ny * g.nz), 2), dim3(threads), lds, s,
-                     (const cx<T> *)bar, (const cx<T> *)hat, n1, n2, p1, p2, C[0], C[1], C[2], C[3], g, P.cst,
+                     (const cx<T> *)bar, (const cx<T> *)hat, n1, n2, p1, p2, (T *)C[0], (T *)C[1], (T *)C[2],
+                     (T *)C[3], g, P.cst,
                      P.tf1_2, P.tf2_2, (const cx<T> *)twM, (const cx<T> *)twN);
   return hipGetLastError();
 }
 
 hipError_t lag_c2r_contract(Dt dt, const void *bar, const void *hat, int n1, int n2, long p1, long p2,
-                            double *const C[4], Geo g, const LagParams &P, hipStream_t s) {
+                            void *const C[4], Geo g, const LagParams &P, hipStream_t s) {
 #define CASE(m)                                                                                       \
   case m:                                                                                             \
     return dt == F64 ? launch_lag_contract<double, m>(bar, hat, n1, n2, p1, p2, C, g, P, s, dt)       \
@@ -322,23 +385,26 @@ hipError_t lag_c2r_contract(Dt dt, const void *bar, const void *hat, int n1, int
 }
 
 template <typename T>
-static hipError_t launch_lag_relax(double *const C[4], const double *const Fold[4], double *const Fnew[4],
+static hipError_t launch_lag_relax(void *const C[4], const double *const Fold[4], double *const Fnew[4],
                                    const void *ul, const void *vl, const void *wl, void *Cs, void *Beta, Geo g,
                                    const LagParams &P, const LagInterp &I, hipStream_t s) {
   F4c fo;
   F4 fn;
   for (int q = 0; q < 4; ++q) { fo.p[q] = Fold[q]; fn.p[q] = Fnew[q]; }
-  const long n = g.np * g.nz;
+  if (g.np >= (1L << 24) || g.np * 3 * (long)sizeof(double) >= (1L << 32)) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)lc_blocks(g.np, g.nz));
   if (I.cs == 1.0)
-    hipLaunchKernelGGL((k_lag_relax<T, true>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, C[0], C[1], C[2],
-                       C[3], fo, fn, (const T *)ul, (const T *)vl, (const T *)wl, (T *)Cs, (T *)Beta, g, P, I);
+    hipLaunchKernelGGL((k_lag_relax<T, true>), grid, dim3(256), 0, s, (const T *)C[0], (const T *)C[1],
+                       (const T *)C[2], (const T *)C[3], fo, fn, (const T *)ul, (const T *)vl, (const T *)wl, (T *)Cs,
+                       (T *)Beta, g, P, I);
   else
-    hipLaunchKernelGGL((k_lag_relax<T, false>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, C[0], C[1],
-                       C[2], C[3], fo, fn, (const T *)ul, (const T *)vl, (const T *)wl, (T *)Cs, (T *)Beta, g, P, I);
+    hipLaunchKernelGGL((k_lag_relax<T, false>), grid, dim3(256), 0, s, (const T *)C[0], (const T *)C[1],
+                       (const T *)C[2], (const T *)C[3], fo, fn, (const T *)ul, (const T *)vl, (const T *)wl, (T *)Cs,
+                       (T *)Beta, g, P, I);
   return hipGetLastError();
 }
 
-hipError_t lag_relax(Dt dt, double *const C[4], const double *const Fold[4], double *const Fnew[4], const void *ul,
+hipError_t lag_relax(Dt dt, void *const C[4], const double *const Fold[4], double *const Fnew[4], const void *ul,
                      const void *vl, const void *wl, void *Cs, void *Beta, Geo g, const LagParams &P,
                      const LagInterp &I, hipStream_t s) {
   return dt == F64 ? launch_lag_relax<double>(C, Fold, Fnew, ul, vl, wl, Cs, Beta, g, P, I, s)
