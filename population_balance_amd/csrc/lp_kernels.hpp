@@ -14,35 +14,6 @@ struct Geo {
 
 __device__ __forceinline__ long gtid() { return (long)blockIdx.x * blockDim.x + threadIdx.x; }
 
-// Level-chunk order for kernels that gather from the levels next to their own.
-// Workgroup ids are dealt round-robin to the 8 XCDs (ids j, j+8, j+16, ... run on XCD j,
-// which has its own L2).  A level of np points is cut into chunks of 256; XCD j takes
-// chunks [j*cg, (j+1)*cg) of every level, level after level, so that the neighbouring
-// levels a block reads were brought into the same L2 by the blocks just before it.
-// Blocks of 256 threads, lc_blocks(np, nz) of them.
-__host__ __device__ __forceinline__ int lc_cg(long np) { return ((int)((np + 255) / 256) + 7) / 8; }
-__host__ __forceinline__ long lc_blocks(long np, int nz) { return 8L * lc_cg(np) * nz; }
-// level k of this thread and its point r within the level; false when it has none
-__device__ __forceinline__ bool lc_point(long np, int nz, int &k, int &r) {
-  const int cg = lc_cg(np);
-  const int j = (int)(blockIdx.x & 7u), s = (int)(blockIdx.x >> 3);
-  k = s / cg;
-  r = (j * cg + s % cg) * 256 + (int)threadIdx.x;
-  return k < nz && r < np;
-}
-// y = r / ld and x = r % ld for 0 <= r < 2^24 (the binary32 quotient is within one of y)
-__device__ __forceinline__ void lc_split(int r, int ld, int &y, int &x) {
-  y = (int)((float)r * (1.0f / (float)ld));
-  x = r - y * ld;
-  if (x < 0) {
-    --y;
-    x += ld;
-  } else if (x >= ld) {
-    ++y;
-    x -= ld;
-  }
-}
-
 // ddz_uv.f90:39-57 ; level 1 untouched
 template <typename T>
 __global__ void k_ddz_uv(T *__restrict__ dfdz, const T *__restrict__ f, Geo g, T cst) {

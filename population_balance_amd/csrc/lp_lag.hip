@@ -227,92 +227,133 @@ __global__ __launch_bounds__(256) void k_lag_c2r_contract(const cx<T> *__restric
   }
 }
 
-// element i of p through a 32-bit byte offset (a volume level is far below 4 GiB), so a
-// load or store is one VGPR offset from a wave-uniform base
-template <typename V> __device__ __forceinline__ const V &at(const V *p, unsigned i) {
-  return *reinterpret_cast<const V *>(reinterpret_cast<const char *>(p) + i * (unsigned)sizeof(V));
-}
-template <typename V> __device__ __forceinline__ V &at(V *p, unsigned i) {
-  return *reinterpret_cast<V *>(reinterpret_cast<char *>(p) + i * (unsigned)sizeof(V));
-}
+// k_lag_relax works on tiles of kRx x kRy points of one level.  The old averages it
+// gathers (4 fields, the point's x/y neighbours on both sides, levels k-1..k+1) are
+// staged in LDS first with coalesced loads: slot (sx, sy, sk) holds the value at
+// x = wrap(x0-1+sx), y = wrap(y0-1+sy), level clamp(k-1+sk), so the backtrace corner
+// x + addx of local point lx is slot lx+1+addx, also across the periodic seam and in a
+// partial last tile.
+constexpr int kRx = 64, kRy = 4;  // kRx = one wavefront, kRy wavefronts per block
+constexpr int kHx = kRx + 2, kHy = kRy + 2;
 
 // lagrange_Sdep.f90:327-434 with interpolag_Sdep.f90:62-365, one thread per point: the
-// old averages are gathered at the backtraced point, relaxed with the contractions and
-// written to Fnew.  The level k is uniform per block, so the level bases are wave-uniform
-// and every gather of the backtrace (levels k-1 .. k+1) is a 32-bit offset from level
-// kb = max(k-1, 0).  The interpolation is interp_point's, term for term.
+// old averages are gathered at the backtraced point (from the LDS tile), relaxed with the
+// contractions and written to Fnew.  Blocks take the tiles of a level in XCD-aware order
+// (workgroup ids are dealt round-robin to the 8 XCDs; XCD j takes tiles [j*cg, (j+1)*cg)
+// of every level, level after level, so a tile's neighbouring levels are in its L2).
+// The tile of the last x column also relaxes the two padding columns of its rows, which
+// interpolag_Sdep.f90 leaves alone (their old values enter the relaxation).  The
+// interpolation is interp_point's, term for term.
 template <typename T, bool UNIT>
 __global__ __launch_bounds__(256) void k_lag_relax(const T *__restrict__ lm, const T *__restrict__ mm,
                                                    const T *__restrict__ qn, const T *__restrict__ nn,
                                                    F4c Fold, F4 Fnew, const T *__restrict__ ul,
                                                    const T *__restrict__ vl, const T *__restrict__ wl,
                                                    T *__restrict__ Cs, T *__restrict__ Beta, Geo g, LagParams P,
-                                                   LagInterp I) {
-  int k, ri, y, x;  // level-chunk order (lc_point): the backtrace reads levels k-1 .. k+1
-  if (!lc_point(g.np, g.nz, k, ri)) return;
-  lc_split(ri, g.ld, y, x);
-  const unsigned r = (unsigned)ri;  // point within the level
+                                                   LagInterp I, int ntx, int cg) {
+  __shared__ double sF[4][3][kHy][kHx];
+  const int ntile = ntx * (g.ny / kRy);
+  const int j = (int)(blockIdx.x & 7u), s = (int)(blockIdx.x >> 3);
+  const int k = s / cg, tt = j * cg + s % cg;
+  if (k >= g.nz || tt >= ntile) return;  // uniform over the block
+  const int x0 = (tt % ntx) * kRx, y0 = (tt / ntx) * kRy;
   const long pk = (long)k * g.np;
-  double fi[4];
+  // stage the tile: 4 fields x 3 levels x kHy rows of kHx slots.  Wavefront wv loads rows
+  // wv, wv+4, ... (row index uniform over the wavefront), lane l slot l; slots 64, 65 of
+  // all rows follow.  x0 - 1 <= x <= x0 + 64 < 3 nx; the slots past a partial tile only
+  // need to stay in bounds (no point reads them).
+  auto wrapx = [&](int xg) {
+    xg = xg < 0 ? xg + g.nx : xg;
+    xg = xg >= g.nx ? xg - g.nx : xg;
+    return xg >= g.nx ? xg - g.nx : xg;
+  };
+  auto row_src = [&](int r) {  // r = (q*3 + sk)*kHy + sy
+    const int q = r / (3 * kHy), sk = (r / kHy) % 3, sy = r % kHy;
+    int yg = y0 - 1 + sy, kg = k - 1 + sk;
+    yg = yg < 0 ? yg + g.ny : (yg >= g.ny ? yg - g.ny : yg);
+    kg = kg < 0 ? 0 : (kg > g.nz - 1 ? g.nz - 1 : kg);
+    return Fold.p[q] + (long)kg * g.np + (long)yg * g.ld;
+  };
+  constexpr int kRows = 4 * 3 * kHy;
+  double *sflat = &sF[0][0][0][0];
+  {
+    const int lane = threadIdx.x % 64, wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    const int xs = wrapx(x0 - 1 + lane);
+#pragma unroll
+    for (int it = 0; it < kRows / 4; ++it) {
+      const int r = 4 * it + wv;
+      sflat[r * kHx + lane] = row_src(r)[xs];
+    }
+    if (threadIdx.x < 2 * kRows) {
+      const int r = threadIdx.x >> 1, sx = 64 + (threadIdx.x & 1);
+      sflat[r * kHx + sx] = row_src(r)[wrapx(x0 - 1 + sx)];
+    }
+  }
+  __syncthreads();
+  const int lx = threadIdx.x % kRx, ly = threadIdx.x / kRx;
+  const int x = x0 + lx, y = y0 + ly;
+  auto relax_store = [&](long i, int xx, double (&fi)[4]) {
+    T cs, b;
+    relax_point<T>((double)lm[i], (double)mm[i], (double)qn[i], (double)nn[i], fi, xx >= g.ld - 2, k == g.nz - 1,
+                   P, cs, b);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) Fnew.p[q][i] = fi[q];
+    Cs[i] = cs;
+    Beta[i] = b;
+  };
   if (x < g.nx) {
+    const long i = pk + (long)y * g.ld + x;
     auto dcs = [&](double v) { return UNIT ? v : v / I.cs; };
     // interpolag_Sdep.f90:204-258 lag velocities on w-nodes (u_lag at jz=1 stays on the uvp node)
-    const double ut = dcs((double)at(ul + pk, r)), vt = dcs((double)at(vl + pk, r));
+    const double ut = dcs((double)ul[i]), vt = dcs((double)vl[i]);
     double ulag, vlag;
     if (k == 0) {
       ulag = ut;
       vlag = vt;
     } else {
-      ulag = 0.5 * (ut + dcs((double)at(ul + (pk - g.np), r)));
-      vlag = 0.5 * (vt + dcs((double)at(vl + (pk - g.np), r)));
+      ulag = 0.5 * (ut + dcs((double)ul[i - g.np]));
+      vlag = 0.5 * (vt + dcs((double)vl[i - g.np]));
     }
     const double xp = -ulag * I.sx;  // :263-265
     const double yp = -vlag * I.sy;
     double zp;
     if (k <= 1) {  // :281-291
-      const double w2 = dcs((double)at(wl + g.np, r));
+      const double w2 = dcs((double)wl[(long)g.np + (i - pk)]);
       double z2 = -w2 * I.sz;
       z2 = z2 + fmin(z2, 0.0);
       z2 = fmin(1.0, z2);
       z2 = fmax(-1.0, z2);
       zp = z2;
     } else {
-      const double wlag = dcs((double)at(wl + pk, r));
+      const double wlag = dcs((double)wl[i]);
       zp = -wlag * I.sz;
     }
     const int addx = signbit(xp) ? -1 : 1, addy = signbit(yp) ? -1 : 1, addz = signbit(zp) ? -1 : 1;
     const double cxp = fabs(xp), cyp = fabs(yp), czp = fabs(zp);
     const double fx = 1.0 - cxp, fy = 1.0 - cyp, fz = 1.0 - czp;
-    // periodic in x and y (x + addx in [-1, nx], y + addy in [-1, ny]), clamped in z
-    int xa = x + addx, ya = y + addy, ka = k + addz;
-    xa = xa < 0 ? xa + g.nx : (xa >= g.nx ? xa - g.nx : xa);
-    ya = ya < 0 ? ya + g.ny : (ya >= g.ny ? ya - g.ny : ya);
-    ka = ka < 0 ? 0 : (ka > g.nz - 1 ? g.nz - 1 : ka);
-    const int kb = k > 0 ? k - 1 : 0;
-    const long pb = (long)kb * g.np;
-    const unsigned dk = (unsigned)((k - kb) * g.np), da = (unsigned)((ka - kb) * g.np);
-    const unsigned o00 = dk + (unsigned)(y * g.ld), o0a = da + (unsigned)(y * g.ld);
-    const unsigned oa0 = dk + (unsigned)(ya * g.ld), oaa = da + (unsigned)(ya * g.ld);
-    const unsigned ux = (unsigned)x, uxa = (unsigned)xa;
+    // slots of the corners: x, x + addx (periodic); y, y + addy (periodic); k, k + addz
+    // (clamped: slot 1 + addz holds level clamp(k + addz))
+    const int sx0 = lx + 1, sxa = lx + 1 + addx, sy0 = ly + 1, sya = ly + 1 + addy, ska = 1 + addz;
+    double fi[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const double *f = Fold.p[q] + pb;
-      fi[q] = fx * fy * (at(f, o00 + ux) * fz + at(f, o0a + ux) * czp)
-            + fx * cyp * (at(f, oa0 + ux) * fz + at(f, oaa + ux) * czp)
-            + cxp * fy * (at(f, o00 + uxa) * fz + at(f, o0a + uxa) * czp)
-            + cxp * cyp * (at(f, oa0 + uxa) * fz + at(f, oaa + uxa) * czp);
+      fi[q] = fx * fy * (sF[q][1][sy0][sx0] * fz + sF[q][ska][sy0][sx0] * czp)
+            + fx * cyp * (sF[q][1][sya][sx0] * fz + sF[q][ska][sya][sx0] * czp)
+            + cxp * fy * (sF[q][1][sy0][sxa] * fz + sF[q][ska][sy0][sxa] * czp)
+            + cxp * cyp * (sF[q][1][sya][sxa] * fz + sF[q][ska][sya][sxa] * czp);
     }
-  } else {  // interpolag_Sdep.f90 leaves the padding columns alone
-#pragma unroll
-    for (int q = 0; q < 4; ++q) fi[q] = at(Fold.p[q] + pk, r);
+    relax_store(i, x, fi);
   }
-  T cs, b;
-  relax_point<T>((double)at(lm + pk, r), (double)at(mm + pk, r), (double)at(qn + pk, r), (double)at(nn + pk, r), fi,
-                 x >= g.ld - 2, k == g.nz - 1, P, cs, b);
+  // the padding columns x = nx .. ld-1 of the tile's rows (old values, no interpolation)
+  const int npad = g.ld - g.nx;
+  if (x0 + kRx >= g.nx && threadIdx.x < npad * kRy) {
+    const int px = g.nx + (int)threadIdx.x % npad, py = y0 + (int)threadIdx.x / npad;
+    const long i = pk + (long)py * g.ld + px;
+    double fi[4];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) at(Fnew.p[q] + pk, r) = fi[q];
-  at(Cs + pk, r) = cs;
-  at(Beta + pk, r) = b;
+    for (int q = 0; q < 4; ++q) fi[q] = Fold.p[q][i];
+    relax_store(i, px, fi);
+  }
 }
 
 // ------------------------------------------------------------------------------------
@@ -391,16 +432,17 @@ static hipError_t launch_lag_relax(void *const C[4], const double *const Fold[4]
   F4c fo;
   F4 fn;
   for (int q = 0; q < 4; ++q) { fo.p[q] = Fold[q]; fn.p[q] = Fnew[q]; }
-  if (g.np >= (1L << 24) || g.np * 3 * (long)sizeof(double) >= (1L << 32)) return hipErrorInvalidValue;
-  const dim3 grid((unsigned)lc_blocks(g.np, g.nz));
+  if (g.ny % kRy != 0 || g.ld - g.nx > 256 / kRy) return hipErrorInvalidValue;
+  const int ntx = (g.nx + kRx - 1) / kRx, cg = (ntx * (g.ny / kRy) + 7) / 8;
+  const dim3 grid((unsigned)(8L * cg * g.nz));
   if (I.cs == 1.0)
     hipLaunchKernelGGL((k_lag_relax<T, true>), grid, dim3(256), 0, s, (const T *)C[0], (const T *)C[1],
                        (const T *)C[2], (const T *)C[3], fo, fn, (const T *)ul, (const T *)vl, (const T *)wl, (T *)Cs,
-                       (T *)Beta, g, P, I);
+                       (T *)Beta, g, P, I, ntx, cg);
   else
     hipLaunchKernelGGL((k_lag_relax<T, false>), grid, dim3(256), 0, s, (const T *)C[0], (const T *)C[1],
                        (const T *)C[2], (const T *)C[3], fo, fn, (const T *)ul, (const T *)vl, (const T *)wl, (T *)Cs,
-                       (T *)Beta, g, P, I);
+                       (T *)Beta, g, P, I, ntx, cg);
   return hipGetLastError();
 }
 
