@@ -1131,12 +1131,12 @@ static int do_press(lp_ctx *c) {
   cx<double> *ph = (cx<double> *)c->phat;
   prof_begin(c, LP_K_PRESS_TRIDIAG);
   if (c->fdt == F64)
-    hipLaunchKernelGGL(k_press_tridiag<double>, grid1(nmodes), dim3(256), 0, c->s, (const cx<double> *)c->Hs[0],
+    hipLaunchKernelGGL(k_press_tridiag<double>, dim3((unsigned)((nmodes + 63) / 64)), dim3(64), 0, c->s, (const cx<double> *)c->Hs[0],
                        (const cx<double> *)c->Hs[1], (const cx<double> *)c->Hs[2], (const cx<double> *)c->bw,
                        (const cx<double> *)c->tw, c->kx, c->ky, ph, c->gam, gg.lh, gg.ny, gg.nz, c->sp, gg.dz,
                        c->d_err);
   else
-    hipLaunchKernelGGL(k_press_tridiag<float>, grid1(nmodes), dim3(256), 0, c->s, (const cx<float> *)c->Hs[0],
+    hipLaunchKernelGGL(k_press_tridiag<float>, dim3((unsigned)((nmodes + 63) / 64)), dim3(64), 0, c->s, (const cx<float> *)c->Hs[0],
                        (const cx<float> *)c->Hs[1], (const cx<float> *)c->Hs[2], (const cx<float> *)c->bw,
                        (const cx<float> *)c->tw, c->kx, c->ky, ph, c->gam, gg.lh, gg.ny, gg.nz, c->sp, gg.dz,
                        c->d_err);

@@ -453,7 +453,7 @@ __global__ void k_project_bc(T *__restrict__ u, T *__restrict__ v, T *__restrict
 // gauge-fixed cumulative sum for the mean mode, zero on the Nyquist row and column.
 // H levels 1..nz-1 at planes 0..nz-2 (complex TH, pitch lh), bottomw/topw planes.
 template <typename TH>
-__global__ void k_press_tridiag(const cx<TH> *__restrict__ Hx, const cx<TH> *__restrict__ Hy,
+__global__ __launch_bounds__(64) void k_press_tridiag(const cx<TH> *__restrict__ Hx, const cx<TH> *__restrict__ Hy,
                                 const cx<TH> *__restrict__ Hz, const cx<TH> *__restrict__ bw,
                                 const cx<TH> *__restrict__ tw_, const double *__restrict__ kxt,
                                 const double *__restrict__ kyt, cx<double> *__restrict__ ph,
@@ -498,47 +498,47 @@ __global__ void k_press_tridiag(const cx<TH> *__restrict__ Hx, const cx<TH> *__r
   ph[o] = uprev;
   double cprev = 1.0;
   bool bad = false;
-  // The right-hand sides do not depend on the recurrence: they are loaded kCh levels at a
-  // time so that the loads of a chunk are in flight together.
-  constexpr int kCh = 8;
-  cx<double> hz_lo = ld_h(Hz, 1);  // Hz(j) for the next row j
-  for (int j0 = 1; j0 <= nz; j0 += kCh) {
-    // unconditional loads from clamped levels (1..nz-1), zeroed afterwards where the
-    // level is nz or beyond (H has no level nz)
-    cx<TH> rx[kCh], ry[kCh], rz[kCh];
+  // The right-hand sides and the back-substitution inputs do not depend on the
+  // recurrence: they are loaded kCh levels at a time, and the loads of the next chunk are
+  // issued before the current chunk is swept (two register buffers), so that the latency
+  // of one chunk overlaps the recurrence of the other.  Levels beyond the system are
+  // loaded from clamped levels and not used.
+  constexpr int kCh = sizeof(TH) == 4 ? 16 : 8;
+  struct HChunk {
+    cx<TH> x[kCh], y[kCh], z[kCh];
+  };
+  auto load_h = [&](int j0, HChunk &c) {
 #pragma unroll
     for (int q = 0; q < kCh; ++q) {
       const int j = j0 + q;
       const int lx = j < nz ? j : nz - 1, lz = j + 1 < nz ? j + 1 : nz - 1;
-      rx[q] = Hx[(long)(lx - 1) * ps + o];
-      ry[q] = Hy[(long)(lx - 1) * ps + o];
-      rz[q] = Hz[(long)(lz - 1) * ps + o];
+      c.x[q] = Hx[(long)(lx - 1) * ps + o];
+      c.y[q] = Hy[(long)(lx - 1) * ps + o];
+      c.z[q] = Hz[(long)(lz - 1) * ps + o];
     }
-    cx<double> hx[kCh], hy[kCh], hz[kCh];
-#pragma unroll
-    for (int q = 0; q < kCh; ++q) {
-      const int j = j0 + q;
-      const bool okx = j < nz, okz = j + 1 < nz;
-      hx[q] = mk<double>(okx ? (double)rx[q].x : 0.0, okx ? (double)rx[q].y : 0.0);
-      hy[q] = mk<double>(okx ? (double)ry[q].x : 0.0, okx ? (double)ry[q].y : 0.0);
-      hz[q] = mk<double>(okz ? (double)rz[q].x : 0.0, okz ? (double)rz[q].y : 0.0);
-    }
+  };
+  cx<double> hz_lo = ld_h(Hz, 1);  // Hz(j) for the next row j
+  auto sweep_h = [&](int j0, const HChunk &c) {
 #pragma unroll
     for (int q = 0; q < kCh; ++q) {
       const int j = j0 + q;
       if (j > nz) break;
-      double a, b, c;
+      double a, b, cc;
       cx<double> r;
       if (j < nz) {
-        a = idz2; b = bi; c = idz2;
-        const cx<double> hz1 = hz[q], hz0 = hz_lo;
+        a = idz2; b = bi; cc = idz2;
+        const bool okz = j + 1 < nz;
+        const cx<double> hx = mk<double>((double)c.x[q].x, (double)c.x[q].y);
+        const cx<double> hy = mk<double>((double)c.y[q].x, (double)c.y[q].y);
+        const cx<double> hz1 = mk<double>(okz ? (double)c.z[q].x : 0.0, okz ? (double)c.z[q].y : 0.0);
+        const cx<double> hz0 = hz_lo;
         // i*(kx Hx + ky Hy) + (Hz(j+1) - Hz(j))/dz
-        double sx = kx * hx[q].x + ky * hy[q].x, sy = kx * hx[q].y + ky * hy[q].y;
+        double sx = kx * hx.x + ky * hy.x, sy = kx * hx.y + ky * hy.y;
         r = mk<double>(-sy + (hz1.x - hz0.x) / dz, sx + (hz1.y - hz0.y) / dz);
         hz_lo = hz1;
       } else {
         cx<TH> t = tw_[o];
-        a = -1.0; b = 1.0; c = 0.0;
+        a = -1.0; b = 1.0; cc = 0.0;
         r = mk<double>(-(double)t.x * dz, -(double)t.y * dz);
       }
       double gm = cprev / bet;
@@ -548,26 +548,51 @@ __global__ void k_press_tridiag(const cx<TH> *__restrict__ Hx, const cx<TH> *__r
       cx<double> uj = mk<double>((r.x - a * uprev.x) / bet, (r.y - a * uprev.y) / bet);
       ph[(long)j * ps + o] = uj;
       uprev = uj;
-      cprev = c;
+      cprev = cc;
+    }
+  };
+  {
+    HChunk ca, cb;
+    load_h(1, ca);
+    for (int j0 = 1; j0 <= nz; j0 += 2 * kCh) {
+      load_h(j0 + kCh, cb);
+      sweep_h(j0, ca);
+      load_h(j0 + 2 * kCh, ca);
+      sweep_h(j0 + kCh, cb);
     }
   }
-  cx<double> unext = uprev;
-  for (int j0 = nz - 1; j0 >= 0; j0 -= kCh) {
-    double gq[kCh];
-    cx<double> uq[kCh];
+  // back substitution, from level nz-1 down to 0, with the same two-buffer prefetch
+  struct BChunk {
+    double g[kCh];
+    cx<double> u[kCh];
+  };
+  auto load_b = [&](int j0, BChunk &c) {
 #pragma unroll
     for (int q = 0; q < kCh; ++q) {
       const int j = j0 - q >= 0 ? j0 - q : 0;
-      gq[q] = gam[(long)(j + 1) * nh + m];
-      uq[q] = ph[(long)j * ps + o];
+      c.g[q] = gam[(long)(j + 1) * nh + m];
+      c.u[q] = ph[(long)j * ps + o];
     }
+  };
+  cx<double> unext = uprev;
+  auto sweep_b = [&](int j0, const BChunk &c) {
 #pragma unroll
     for (int q = 0; q < kCh; ++q) {
       const int j = j0 - q;
       if (j < 0) break;
-      cx<double> uj = mk<double>(uq[q].x - gq[q] * unext.x, uq[q].y - gq[q] * unext.y);
+      cx<double> uj = mk<double>(c.u[q].x - c.g[q] * unext.x, c.u[q].y - c.g[q] * unext.y);
       ph[(long)j * ps + o] = uj;
       unext = uj;
+    }
+  };
+  {
+    BChunk ca, cb;
+    load_b(nz - 1, ca);
+    for (int j0 = nz - 1; j0 >= 0; j0 -= 2 * kCh) {
+      load_b(j0 - kCh, cb);
+      sweep_b(j0, ca);
+      load_b(j0 - 2 * kCh, ca);
+      sweep_b(j0 - kCh, cb);
     }
   }
   if (bad) atomicExch(err, 1);
