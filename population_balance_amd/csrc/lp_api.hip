@@ -96,10 +96,13 @@ struct lp_ctx {
   lp_grid g;
   int dev = 0;
   hipStream_t s = nullptr;
-  // second stream of the step: the convective term runs there, beside the SGS model,
-  // between ev_fork (after wallstress) and ev_join (before the RHS assembly)
+  // second stream of the step: the single-plane wall-stress passes beside the
+  // z-derivatives and the pressure's wall/lid rows beside its H transforms (each between
+  // ev_aux_fork and ev_aux), the convective term beside the SGS model (ev_fork after the
+  // Lagrangian column pass .. ev_join before the RHS assembly)
   hipStream_t s2 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_aux_fork = nullptr, ev_aux = nullptr;
+  bool prof_off = false;  // work on s2 that overlaps other kernels is not profiled
   void *cv_spec[2] = {};  // convec's own spectral scratch (divstress runs meanwhile)
   bool fork_pending = false;  // the step's convec is still to be started on s2
   Dt fdt = F64;  // field type
@@ -156,7 +159,7 @@ struct lp_ctx {
 
 // bracket a launch of kernel class `cls` with events when that class is profiled
 static void prof_begin(lp_ctx *c, int cls) {
-  if (c->prof_class != cls) return;
+  if (c->prof_class != cls || c->prof_off) return;
   if (c->ev_used + 2 > c->ev.size()) {
     for (int i = 0; i < 256; ++i) {
       hipEvent_t e;
@@ -167,7 +170,7 @@ static void prof_begin(lp_ctx *c, int cls) {
   (void)hipEventRecord(c->ev[c->ev_used], c->s);
 }
 static void prof_end(lp_ctx *c, int cls, double bytes) {
-  if (c->prof_class != cls || c->ev_used + 2 > c->ev.size()) return;
+  if (c->prof_class != cls || c->prof_off || c->ev_used + 2 > c->ev.size()) return;
   (void)hipEventRecord(c->ev[c->ev_used + 1], c->s);
   c->ev_used += 2;
   c->prof_bytes += bytes;
@@ -471,6 +474,8 @@ extern "C" int lp_destroy(lp_ctx *c) {
   for (auto p : c->cv_spec) fr(p);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->ev_aux_fork) (void)hipEventDestroy(c->ev_aux_fork);
+  if (c->ev_aux) (void)hipEventDestroy(c->ev_aux);
   if (c->s2) (void)hipStreamDestroy(c->s2);
   if (c->s) (void)hipStreamDestroy(c->s);
   delete c;
@@ -515,7 +520,9 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
   }
   if (hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_aux_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_aux, hipEventDisableTiming) != hipSuccess) {
     lp_destroy(c);
     return set_err(LP_ERR_HIP, "stream/event creation failed");
   }
@@ -1106,6 +1113,38 @@ static int do_press(lp_ctx *c) {
   const long ps = c->sp * gg.ny;  // complex elements per spectral plane
   const double tadv1 = 1.5;
   double cst = 1.0 / ((double)gg.nx * gg.ny);
+  // :105-122,133-136 wall and lid rows from divtz(1), divtz(nz) (single-plane passes),
+  // the pivot flag and the BOGUS top levels of p, dpdx, dpdy (:355-357) on the second
+  // stream, beside the H transforms
+  auto side = [&]() -> int {
+    RCHK(r2c_small(c, c->f[LP_DIVTZ], 1, cst, c->bw));
+    ColOut ob{c->bw, M_ONE};
+    RCHK(col_small(c, c->bw, 1, 1, 0, 1, &ob));
+    const char *divtz_top = (const char *)c->f[LP_DIVTZ] + (size_t)(gg.nz - 1) * np * esz_of(c->fdt);
+    RCHK(r2c_small(c, divtz_top, 1, cst, c->tw));
+    ColOut ot{c->tw, M_ONE};
+    RCHK(col_small(c, c->tw, 1, 1, 0, 1, &ot));
+    HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(int), c->s));
+    hipLaunchKernelGGL(k_fill<double>, grid1(np), dim3(256), 0, c->s, (double *)c->f[LP_P] + (long)gg.nz * np, np,
+                       LP_BOGUS);
+    FDISPATCH(c, T, {
+      hipLaunchKernelGGL(k_fill<T>, grid1(np), dim3(256), 0, c->s, (T *)c->f[LP_DPDX] + (long)(gg.nz - 1) * np,
+                         np, (T)LP_BOGUS);
+      hipLaunchKernelGGL(k_fill<T>, grid1(np), dim3(256), 0, c->s, (T *)c->f[LP_DPDY] + (long)(gg.nz - 1) * np,
+                         np, (T)LP_BOGUS);
+    });
+    HIPCHK(hipGetLastError());
+    return LP_OK;
+  };
+  HIPCHK(hipEventRecord(c->ev_aux_fork, c->s));
+  HIPCHK(hipStreamWaitEvent(c->s2, c->ev_aux_fork, 0));
+  std::swap(c->s, c->s2);
+  c->prof_off = true;
+  const int rc_side = side();
+  c->prof_off = false;
+  std::swap(c->s, c->s2);
+  RCHK(rc_side);
+  HIPCHK(hipEventRecord(c->ev_aux, c->s2));
   // press_stag_array.f90:75-87,126-131: H = FFT(u*/(1.5 dt))/(nx ny), oddballs zeroed
   const int fid[3] = {LP_U, LP_V, LP_W};
   for (int q = 0; q < 3; ++q) {
@@ -1113,30 +1152,18 @@ static int do_press(lp_ctx *c) {
     ColOut o{c->Hs[q], M_ONE};
     RCHK(col_small(c, c->Hs[q], gg.nz - 1, 1, 0, 1, &o));
   }
-  // :105-122,133-136 wall and lid rows from divtz(1), divtz(nz)
-  RCHK(r2c_small(c, c->f[LP_DIVTZ], 1, cst, c->bw));
-  {
-    ColOut o{c->bw, M_ONE};
-    RCHK(col_small(c, c->bw, 1, 1, 0, 1, &o));
-  }
-  size_t fe = esz_of(c->fdt);
-  const char *divtz_top = (const char *)c->f[LP_DIVTZ] + (size_t)(gg.nz - 1) * np * fe;
-  RCHK(r2c_small(c, divtz_top, 1, cst, c->tw));
-  {
-    ColOut o{c->tw, M_ONE};
-    RCHK(col_small(c, c->tw, 1, 1, 0, 1, &o));
-  }
   // :149-329 + tridag_array
-  HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(int), c->s));
+  HIPCHK(hipStreamWaitEvent(c->s, c->ev_aux, 0));
   cx<double> *ph = (cx<double> *)c->phat;
+  const dim3 grid_modes((unsigned)((nmodes + 63) / 64));  // 64 modes per block: every CU gets some
   prof_begin(c, LP_K_PRESS_TRIDIAG);
   if (c->fdt == F64)
-    hipLaunchKernelGGL(k_press_tridiag<double>, dim3((unsigned)((nmodes + 63) / 64)), dim3(64), 0, c->s, (const cx<double> *)c->Hs[0],
+    hipLaunchKernelGGL(k_press_tridiag<double>, grid_modes, dim3(64), 0, c->s, (const cx<double> *)c->Hs[0],
                        (const cx<double> *)c->Hs[1], (const cx<double> *)c->Hs[2], (const cx<double> *)c->bw,
                        (const cx<double> *)c->tw, c->kx, c->ky, ph, c->gam, gg.lh, gg.ny, gg.nz, c->sp, gg.dz,
                        c->d_err);
   else
-    hipLaunchKernelGGL(k_press_tridiag<float>, dim3((unsigned)((nmodes + 63) / 64)), dim3(64), 0, c->s, (const cx<float> *)c->Hs[0],
+    hipLaunchKernelGGL(k_press_tridiag<float>, grid_modes, dim3(64), 0, c->s, (const cx<float> *)c->Hs[0],
                        (const cx<float> *)c->Hs[1], (const cx<float> *)c->Hs[2], (const cx<float> *)c->bw,
                        (const cx<float> *)c->tw, c->kx, c->ky, ph, c->gam, gg.lh, gg.ny, gg.nz, c->sp, gg.dz,
                        c->d_err);
@@ -1157,16 +1184,6 @@ static int do_press(lp_ctx *c) {
                  gg.nz - 1, c->f[LP_DPDX], gg.ld, np, c->s, c->kx));
   HIPCHK(row_c2r(F64, dto, gg.nx / 2, (const cx<double> *)c->pspec[2] + ps, c->sp, ps, gg.lh, gg.ny,
                  gg.nz - 1, c->f[LP_DPDY], gg.ld, np, c->s));
-  // :355-357
-  hipLaunchKernelGGL(k_fill<double>, grid1(np), dim3(256), 0, c->s, (double *)c->f[LP_P] + (long)gg.nz * np, np,
-                     LP_BOGUS);
-  FDISPATCH(c, T, {
-    hipLaunchKernelGGL(k_fill<T>, grid1(np), dim3(256), 0, c->s, (T *)c->f[LP_DPDX] + (long)(gg.nz - 1) * np, np,
-                       (T)LP_BOGUS);
-    hipLaunchKernelGGL(k_fill<T>, grid1(np), dim3(256), 0, c->s, (T *)c->f[LP_DPDY] + (long)(gg.nz - 1) * np, np,
-                       (T)LP_BOGUS);
-  });
-  HIPCHK(hipGetLastError());
   return LP_OK;
 }
 
@@ -1332,10 +1349,21 @@ static int step_once(lp_ctx *c) {
   RCHK(do_filt_da(c, c->f[LP_U], c->f[LP_DUDX], c->f[LP_DUDY], gg.nz, c->uvw_spec[0]));  // :277-279
   RCHK(do_filt_da(c, c->f[LP_V], c->f[LP_DVDX], c->f[LP_DVDY], gg.nz, c->uvw_spec[1]));
   RCHK(do_filt_da(c, c->f[LP_W], c->f[LP_DWDX], c->f[LP_DWDY], gg.nz, c->uvw_spec[2]));
-  RCHK(do_ddz_uv(c, c->f[LP_DUDZ], c->f[LP_U]));  // :284-288
+  // :315 wallstress (level 1 of txz, tyz, dudz, dvdz from the test-filtered u(1), v(1):
+  // single-plane passes) on the second stream, beside the z-derivatives of the other levels
+  HIPCHK(hipEventRecord(c->ev_aux_fork, c->s));
+  HIPCHK(hipStreamWaitEvent(c->s2, c->ev_aux_fork, 0));
+  std::swap(c->s, c->s2);
+  c->prof_off = true;
+  const int rc_wall = do_wallstress(c);
+  c->prof_off = false;
+  std::swap(c->s, c->s2);
+  RCHK(rc_wall);
+  HIPCHK(hipEventRecord(c->ev_aux, c->s2));
+  RCHK(do_ddz_uv(c, c->f[LP_DUDZ], c->f[LP_U]));  // :284-288 (levels 2..nz-1)
   RCHK(do_ddz_uv(c, c->f[LP_DVDZ], c->f[LP_V]));
   RCHK(do_ddz_w(c, c->f[LP_DWDZ], c->f[LP_W]));
-  RCHK(do_wallstress(c));  // :315
+  HIPCHK(hipStreamWaitEvent(c->s, c->ev_aux, 0));  // sgs_stag reads level 1
   c->fork_pending = true;  // convec (:419) starts on the second stream inside sgs_stag
   const int rc_sgs = do_sgs_stag(c);  // :327
   if (rc_sgs != LP_OK) {
