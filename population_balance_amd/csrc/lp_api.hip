@@ -158,20 +158,36 @@ struct lp_ctx {
 };
 
 // bracket a launch of kernel class `cls` with events when that class is profiled
-static void prof_begin(lp_ctx *c, int cls) {
-  if (c->prof_class != cls || c->prof_off) return;
+// a free pair of timing events for class cls, when it is being profiled
+static bool prof_reserve(lp_ctx *c, int cls) {
+  if (c->prof_class != cls || c->prof_off) return false;
   if (c->ev_used + 2 > c->ev.size()) {
     for (int i = 0; i < 256; ++i) {
       hipEvent_t e;
-      if (hipEventCreate(&e) != hipSuccess) return;
+      if (hipEventCreate(&e) != hipSuccess) return false;
       c->ev.push_back(e);
     }
   }
-  (void)hipEventRecord(c->ev[c->ev_used], c->s);
+  return true;
+}
+static void prof_begin(lp_ctx *c, int cls) {
+  if (prof_reserve(c, cls)) (void)hipEventRecord(c->ev[c->ev_used], c->s);
 }
 static void prof_end(lp_ctx *c, int cls, double bytes) {
   if (c->prof_class != cls || c->prof_off || c->ev_used + 2 > c->ev.size()) return;
   (void)hipEventRecord(c->ev[c->ev_used + 1], c->s);
+  c->ev_used += 2;
+  c->prof_bytes += bytes;
+  c->prof_launches += 1;
+}
+// the pair for one launch that takes the events itself (hipExtLaunchKernel: no marker
+// packets around it, so the measurement adds no idle time to the stream), then its account
+static LaunchEv prof_attach(lp_ctx *c, int cls) {
+  if (!prof_reserve(c, cls)) return LaunchEv{};
+  return LaunchEv{c->ev[c->ev_used], c->ev[c->ev_used + 1]};
+}
+static void prof_attached(lp_ctx *c, const LaunchEv &ev, double bytes) {
+  if (!ev.start) return;
   c->ev_used += 2;
   c->prof_bytes += bytes;
   c->prof_launches += 1;
@@ -703,9 +719,10 @@ static int col_small(lp_ctx *c, const void *in, int nlev, int fwd, int inv, int 
   const lp_grid &g = c->g;
   const long ps = c->sp * g.ny;
   if (ncols <= 0) ncols = g.lh;
-  prof_begin(c, LP_K_COL_OP);
-  HIPCHK(col_op(c->fdt, g.ny, g.ny, 0, fwd, inv, in, c->sp, ps, ncols, nlev, nout, o, c->sp, ps, c->tab, c->s));
-  prof_end(c, LP_K_COL_OP, bytes_col(g.ny, g.ny, ncols, nlev, nout, esz_of(c->fdt)));
+  const LaunchEv ev = prof_attach(c, LP_K_COL_OP);
+  HIPCHK(col_op(c->fdt, g.ny, g.ny, 0, fwd, inv, in, c->sp, ps, ncols, nlev, nout, o, c->sp, ps, c->tab, c->s,
+                ev));
+  prof_attached(c, ev, bytes_col(g.ny, g.ny, ncols, nlev, nout, esz_of(c->fdt)));
   return LP_OK;
 }
 
@@ -794,12 +811,13 @@ static int do_ddz_w(lp_ctx *c, void *dfdz, const void *f) {
 static int do_wallstress(lp_ctx *c) {
   const lp_config &cf = c->cfg;
   Geo g = geo(c);
-  size_t pb = (size_t)g.np * esz_of(c->fdt);
   if (cf.lbc_mom == LP_LBC_STRESS_FREE) {  // wallstress.f90:200-205
-    HIPCHK(hipMemsetAsync(c->f[LP_TXZ], 0, pb, c->s));
-    HIPCHK(hipMemsetAsync(c->f[LP_TYZ], 0, pb, c->s));
-    HIPCHK(hipMemsetAsync(c->f[LP_DUDZ], 0, pb, c->s));
-    HIPCHK(hipMemsetAsync(c->f[LP_DVDZ], 0, pb, c->s));
+    // fill kernels rather than hipMemsetAsync: this may run on the second stream, and the
+    // runtime's memset there was seen to serialise the two streams
+    const int zf[4] = {LP_TXZ, LP_TYZ, LP_DUDZ, LP_DVDZ};
+    for (int q = 0; q < 4; ++q)
+      FDISPATCH(c, T, hipLaunchKernelGGL(k_fill<T>, grid1(g.np), dim3(256), 0, c->s, (T *)c->f[zf[q]], g.np, T(0)));
+    HIPCHK(hipGetLastError());
     return LP_OK;
   }
   // wallstress.f90:103-104,139-140: test-filtered copies of u(1), v(1)
@@ -827,10 +845,10 @@ static void swap_F(lp_ctx *c) {
 }
 
 static int reset_lag_velocities(lp_ctx *c) {  // interpolag_Sdep.f90:367-369
-  size_t vb = c->bytes[LP_U_LAG];
-  HIPCHK(hipMemsetAsync(c->f[LP_U_LAG], 0, vb, c->s));
-  HIPCHK(hipMemsetAsync(c->f[LP_V_LAG], 0, vb, c->s));
-  HIPCHK(hipMemsetAsync(c->f[LP_W_LAG], 0, vb, c->s));
+  const long n = (long)(c->bytes[LP_U_LAG] / esz_of(c->fdt));
+  for (int q = 0; q < 3; ++q)
+    FDISPATCH(c, T, hipLaunchKernelGGL(k_fill<T>, grid1(n), dim3(256), 0, c->s, (T *)c->f[LP_U_LAG + q], n, T(0)));
+  HIPCHK(hipGetLastError());
   c->lag_vel_zero = true;
   return LP_OK;
 }
@@ -899,10 +917,11 @@ static int do_lagrange(lp_ctx *c, bool use_u = false) {
     // 256: the kx = 64 column that G_test keeps by rounding) one column per block
     const int tx = c->fdt == F64 ? 8 : 16;
     const int nmain = (nc % tx <= 2 && nc > tx) ? nc / tx * tx : nc;
-    prof_begin(c, LP_K_COL_OP);
+    const LaunchEv ev = prof_attach(c, LP_K_COL_OP);
     HIPCHK(col_op(c->fdt, gg.ny, gg.ny, 0, 1, 1, c->lspec, pc, ps, nmain, kNumLag * gg.nz, 2, o, pc, ps, c->tab,
-                  c->s));
-    prof_end(c, LP_K_COL_OP, (double)kNumLag * gg.nz * gg.ny * (nmain + std::min(nmain, n1) + std::min(nmain, n2)) * 2 * fe);
+                  c->s, ev));
+    prof_attached(c, ev,
+                  (double)kNumLag * gg.nz * gg.ny * (nmain + std::min(nmain, n1) + std::min(nmain, n2)) * 2 * fe);
     if (nmain < nc) {
       ColOut ot[2];
       int nt = 0;
@@ -1113,9 +1132,10 @@ static int do_press(lp_ctx *c) {
   const long ps = c->sp * gg.ny;  // complex elements per spectral plane
   const double tadv1 = 1.5;
   double cst = 1.0 / ((double)gg.nx * gg.ny);
-  // :105-122,133-136 wall and lid rows from divtz(1), divtz(nz) (single-plane passes),
-  // the pivot flag and the BOGUS top levels of p, dpdx, dpdy (:355-357) on the second
-  // stream, beside the H transforms
+  // :105-122,133-136 wall and lid rows from divtz(1), divtz(nz) (single-plane passes) and
+  // the BOGUS top levels of p, dpdx, dpdy (:355-357) on the second stream, beside the H
+  // transforms.  A zero pivot sets d_err, which check_err_flag reads and clears, so the
+  // flag of every step of an lp_step call is reported.
   auto side = [&]() -> int {
     RCHK(r2c_small(c, c->f[LP_DIVTZ], 1, cst, c->bw));
     ColOut ob{c->bw, M_ONE};
@@ -1124,7 +1144,6 @@ static int do_press(lp_ctx *c) {
     RCHK(r2c_small(c, divtz_top, 1, cst, c->tw));
     ColOut ot{c->tw, M_ONE};
     RCHK(col_small(c, c->tw, 1, 1, 0, 1, &ot));
-    HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(int), c->s));
     hipLaunchKernelGGL(k_fill<double>, grid1(np), dim3(256), 0, c->s, (double *)c->f[LP_P] + (long)gg.nz * np, np,
                        LP_BOGUS);
     FDISPATCH(c, T, {
@@ -1309,9 +1328,11 @@ static int do_ke_dudt(lp_ctx *c) {
   return LP_OK;
 }
 
+// reads the zero-pivot flag the Thomas kernels raise, and clears it for the next call
 static int check_err_flag(lp_ctx *c) {
   int h = 0;
   HIPCHK(hipMemcpyAsync(&h, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(int), c->s));
   HIPCHK(hipStreamSynchronize(c->s));
   if (h) return set_err(LP_ERR_PIVOT, "tridag_array: zero pivot");
   return LP_OK;

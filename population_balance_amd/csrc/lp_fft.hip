@@ -8,6 +8,8 @@
 //             real parts of the kx = 0 and kx = M terms (FFTW2 halfcomplex semantics)
 // Every transform is a register-resident four-step FFT (lp_fft4.hpp) with one LDS
 // exchange; loads and stores go straight between HBM and registers.
+#include <hip/hip_ext.h>
+
 #include "lp_fft4.hpp"
 #include "lp_internal.hpp"
 
@@ -622,7 +624,8 @@ hipError_t row_c2r(Dt dtc, Dt dto, int M, const void *in, long in_pitch, long in
 template <typename T, int NYI, int NYO>
 static hipError_t launch_col(int padmode, int fwd, int inv, const void *in, long in_pitch, long in_plane,
                              int ncols, int nplanes, int nout, const ColOut *outs, long out_pitch,
-                             long out_plane, const SpecTables &tab, hipStream_t s, Dt dt, int col0 = 0) {
+                             long out_plane, const SpecTables &tab, hipStream_t s, Dt dt, int col0 = 0,
+                             LaunchEv ev = {}) {
   const void *twI = tw_ptr(NYI, dt), *twO = tw_ptr(NYO, dt);
   if (!twI || !twO || col0 < 0 || col0 >= ncols) return hipErrorInvalidValue;
   // the map is fixed by the lengths: same grid (0), padd (1), unpadd (2)
@@ -643,31 +646,32 @@ static hipError_t launch_col(int padmode, int fwd, int inv, const void *in, long
   if (col0 > 0) {
     if constexpr (NYI == NYO) {
       dim3 grid((unsigned)(ncols - col0), (unsigned)nplanes);
-      hipLaunchKernelGGL((k_col_op<T, NYI, NYO, 1>), grid, dim3(kColLanes), 0, s, (const cx<T> *)in, in_pitch,
-                         in_plane, ncols, fwd, inv, P, out_pitch, out_plane, tab.kx, tab.ky, tab.G1, tab.G2,
-                         tab.lh, tab.ny, (const cx<T> *)twI, (const cx<T> *)twO, col0);
+      hipExtLaunchKernelGGL((k_col_op<T, NYI, NYO, 1>), grid, dim3(kColLanes), 0, s, ev.start, ev.stop, 0,
+                            (const cx<T> *)in, in_pitch, in_plane, ncols, fwd, inv, P, out_pitch, out_plane,
+                            tab.kx, tab.ky, tab.G1, tab.G2, tab.lh, tab.ny, (const cx<T> *)twI,
+                            (const cx<T> *)twO, col0);
       return hipGetLastError();
     } else {
       return hipErrorInvalidValue;
     }
   }
   dim3 grid((unsigned)((ncols + TX - 1) / TX), (unsigned)nplanes);
-  hipLaunchKernelGGL((k_col_op<T, NYI, NYO>), grid, dim3(TX * kColLanes), 0, s, (const cx<T> *)in, in_pitch,
-                     in_plane, ncols, fwd, inv, P, out_pitch, out_plane, tab.kx, tab.ky, tab.G1, tab.G2, tab.lh,
-                     tab.ny, (const cx<T> *)twI, (const cx<T> *)twO, 0);
+  hipExtLaunchKernelGGL((k_col_op<T, NYI, NYO>), grid, dim3(TX * kColLanes), 0, s, ev.start, ev.stop, 0,
+                        (const cx<T> *)in, in_pitch, in_plane, ncols, fwd, inv, P, out_pitch, out_plane, tab.kx,
+                        tab.ky, tab.G1, tab.G2, tab.lh, tab.ny, (const cx<T> *)twI, (const cx<T> *)twO, 0);
   return hipGetLastError();
 }
 
 hipError_t col_op_tail(Dt dt, int ny, const void *in, long in_pitch, long in_plane, int col0, int ncols,
                        int nplanes, int nout, const ColOut *outs, long out_pitch, long out_plane,
-                       const SpecTables &tab, hipStream_t s) {
+                       const SpecTables &tab, hipStream_t s, LaunchEv ev) {
   if (nout < 1 || nout > 3) return hipErrorInvalidValue;
 #define CASE(n)                                                                                          \
   case n:                                                                                                \
     return dt == F64 ? launch_col<double, n, n>(0, 1, 1, in, in_pitch, in_plane, ncols, nplanes, nout,  \
-                                                outs, out_pitch, out_plane, tab, s, dt, col0)           \
+                                                outs, out_pitch, out_plane, tab, s, dt, col0, ev)       \
                      : launch_col<float, n, n>(0, 1, 1, in, in_pitch, in_plane, ncols, nplanes, nout,   \
-                                               outs, out_pitch, out_plane, tab, s, dt, col0);
+                                               outs, out_pitch, out_plane, tab, s, dt, col0, ev);
   switch (ny) {
     CASE(32) CASE(48) CASE(64) CASE(96) CASE(128) CASE(144) CASE(192) CASE(256) CASE(288) CASE(384)
     default: return hipErrorInvalidValue;
@@ -677,20 +681,20 @@ hipError_t col_op_tail(Dt dt, int ny, const void *in, long in_pitch, long in_pla
 
 hipError_t col_op(Dt dt, int nyi, int nyo, int padmode, int fwd, int inv, const void *in, long in_pitch,
                   long in_plane, int ncols, int nplanes, int nout, const ColOut *outs, long out_pitch,
-                  long out_plane, const SpecTables &tab, hipStream_t s) {
+                  long out_plane, const SpecTables &tab, hipStream_t s, LaunchEv ev) {
   if (nout < 1 || nout > 3) return hipErrorInvalidValue;
 #define SAME(n)                                                                                         \
   if (nyi == n && nyo == n)                                                                             \
     return dt == F64 ? launch_col<double, n, n>(padmode, fwd, inv, in, in_pitch, in_plane, ncols,       \
-                                                nplanes, nout, outs, out_pitch, out_plane, tab, s, dt) \
+                                                nplanes, nout, outs, out_pitch, out_plane, tab, s, dt, 0, ev) \
                      : launch_col<float, n, n>(padmode, fwd, inv, in, in_pitch, in_plane, ncols,        \
-                                               nplanes, nout, outs, out_pitch, out_plane, tab, s, dt);
+                                               nplanes, nout, outs, out_pitch, out_plane, tab, s, dt, 0, ev);
 #define PAIR(a, b)                                                                                      \
   if (nyi == a && nyo == b)                                                                             \
     return dt == F64 ? launch_col<double, a, b>(padmode, fwd, inv, in, in_pitch, in_plane, ncols,       \
-                                                nplanes, nout, outs, out_pitch, out_plane, tab, s, dt) \
+                                                nplanes, nout, outs, out_pitch, out_plane, tab, s, dt, 0, ev) \
                      : launch_col<float, a, b>(padmode, fwd, inv, in, in_pitch, in_plane, ncols,        \
-                                               nplanes, nout, outs, out_pitch, out_plane, tab, s, dt);
+                                               nplanes, nout, outs, out_pitch, out_plane, tab, s, dt, 0, ev);
   SAME(32) SAME(48) SAME(64) SAME(96) SAME(128) SAME(144) SAME(192) SAME(256) SAME(288) SAME(384)
   PAIR(32, 48) PAIR(48, 32) PAIR(64, 96) PAIR(96, 64) PAIR(96, 144) PAIR(144, 96)
   PAIR(128, 192) PAIR(192, 128) PAIR(192, 288) PAIR(288, 192) PAIR(256, 384) PAIR(384, 256)

@@ -60,16 +60,23 @@ struct ColOut {
 //  map: padmode 0 (same grid, Nyquist row/column killed, multiplier applied),
 //       1 (zero-pad ny -> ny2 as padd.f90), 2 (truncate ny2 -> ny as unpadd.f90);
 //  inv: inverse DFT of length nyo.
+// Timing events taken by the dispatch of a launch itself (hipExtLaunchKernel): start at
+// the first wave, stop at the end of the kernel, with no marker packet in the stream.
+// Either may be null.
+struct LaunchEv {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+
 hipError_t col_op(Dt dt, int nyi, int nyo, int padmode, int fwd, int inv, const void *in,
                   long in_pitch, long in_plane, int ncols, int nplanes, int nout,
                   const ColOut *outs, long out_pitch, long out_plane, const SpecTables &tab,
-                  hipStream_t s);
+                  hipStream_t s, LaunchEv ev = {});
 
 // col_op (same grid, forward + inverse) on the columns col0..ncols-1 only, one column per
 // block: the few columns past the last full group of a block-wide launch.
 hipError_t col_op_tail(Dt dt, int ny, const void *in, long in_pitch, long in_plane, int col0, int ncols,
                        int nplanes, int nout, const ColOut *outs, long out_pitch, long out_plane,
-                       const SpecTables &tab, hipStream_t s);
+                       const SpecTables &tab, hipStream_t s, LaunchEv ev = {});
 
 // Column pass of the divergence terms: out = inverse_y( i kx fwd_y(A) + i ky fwd_y(B) )
 // with the Nyquist row and column zeroed; A, B, out share pitch / plane; ny == tab.ny.
