@@ -148,3 +148,22 @@ def test_tridag_array(pair):
     m[:, -1] = False
     m[0, 0] = False
     assert rel_l2(u[:, m], ref[:, m]) < TOL
+
+
+def test_tridag_array_zero_pivot(pair):
+    """A zero leading pivot is reported as the reference's stop (tridag_array.f90:35-38):
+    LP_ERR_PIVOT, raised from the host; the flag is cleared, so a regular system solved
+    next reports no error."""
+    from population_balance_amd.lesgo import LesgoError
+    cfg, o, lg = pair
+    g = cfg.grid
+    n = g.nz + 1
+    shp = (n, g.ny, g.lh)
+    a = np.ones(shp)
+    c = np.ones(shp)
+    r = np.ones(shp, complex)
+    u = np.zeros(shp, complex)
+    with pytest.raises(LesgoError, match="pivot"):
+        lg.tridag_array(a, np.zeros(shp), c, r, u)
+    lg.tridag_array(a, -3.0 * np.ones(shp), c, r, u)
+    assert np.all(np.isfinite(u))
