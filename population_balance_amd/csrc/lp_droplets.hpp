@@ -125,11 +125,15 @@ __global__ void k_kc(const T *__restrict__ nut, T *__restrict__ magS, double *__
 // SMART face value (:2877-3475) from the upwind cell, the cell upwind of it and the
 // downwind cell; a zero upwind gradient takes the upwind value
 __device__ __forceinline__ double smart_face(double cup, double cupup, double cdn) {
+  // cup + psi(r) den / 2 with r = num / den, psi(r) = max(0, min(2r, 0.75 r + 0.25, 4)),
+  // without the division: psi(r) den is max(0, min(2 num, 0.75 num + 0.25 den, 4 den))
+  // for den > 0, and min(0, max(...)) of the same three for den < 0
   const double den = cup - cupup;
   if (den == 0.0) return cup;
-  double r = (cdn - cup) / den;
-  r = fmax(0.0, fmin(fmin(2.0 * r, 0.75 * r + 0.25), 4.0));
-  return cup + r * den / 2.0;
+  const double num = cdn - cup;
+  const double a = 2.0 * num, b = 0.75 * num + 0.25 * den, c = 4.0 * den;
+  const double pd = den > 0.0 ? fmax(0.0, fmin(fmin(a, b), c)) : fmin(0.0, fmax(fmax(a, b), c));
+  return cup + pd / 2.0;
 }
 
 __device__ __forceinline__ double sign1(double x) { return signbit(x) ? -1.0 : 1.0; }
