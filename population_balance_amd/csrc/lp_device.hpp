@@ -1,12 +1,6 @@
-// lp_device.hpp -- device-side building blocks for gfx950 (CDNA4, wave64):
-// complex arithmetic and an LDS-resident Stockham FFT with compile-time radix plan.
-//
-// The FFT is the auto-sort Stockham formulation of Govindaraju et al., "High
-// Performance Discrete Fourier Transforms on Graphics Processors" (SC'08): each stage
-// reads R elements strided by N/R, applies twiddles exp(sign*2*pi*i*(j mod Ns)*r/(Ns*R))
-// and an R-point DFT, and writes them at (j/Ns)*Ns*R + (j mod Ns) + r*Ns.  Radices are
-// 4, then 2, then 3 (every length used here is 2^a 3^b).  Twiddles come from a table
-// tw[j] = exp(-2*pi*i*j/N) computed on the host in binary64.
+// lp_device.hpp -- device-side building blocks for gfx950 (CDNA4, wave64): complex
+// arithmetic and the small in-register DFTs (radix 2, 3, 4) that the four-step FFT of
+// lp_fft4.hpp is built from.
 #pragma once
 
 // the reference's BOGUS marker (param.f90) for levels a routine leaves undefined
@@ -71,56 +65,5 @@ template <typename T, int SIGN> struct SmallDft<T, 3, SIGN> {
     v[2] = m - j;
   }
 };
-
-__host__ __device__ constexpr int next_radix(int rem) {
-  return (rem % 4 == 0) ? 4 : (rem % 2 == 0) ? 2 : (rem % 3 == 0) ? 3 : rem;
-}
-
-// One Stockham stage over a transform of length N living in src (LDS), written to dst.
-// nt threads cooperate, this thread is t.
-template <typename T, int N, int Ns, int R, int SIGN>
-__device__ __forceinline__ void stockham_stage(const cx<T> *__restrict__ src, cx<T> *__restrict__ dst,
-                                               int t, int nt, const cx<T> *__restrict__ tw) {
-  constexpr int NB = N / R;
-  for (int j = t; j < NB; j += nt) {
-    cx<T> v[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = src[j + r * NB];
-    const int kk = j % Ns;
-    if constexpr (Ns > 1) {
-      constexpr int step = N / (Ns * R);
-#pragma unroll
-      for (int r = 1; r < R; ++r) {
-        cx<T> w = tw[kk * r * step];
-        if (SIGN > 0) w.y = -w.y;
-        v[r] = cmul(v[r], w);
-      }
-    }
-    SmallDft<T, R, SIGN>::run(v);
-    const int base = (j / Ns) * Ns * R + kk;
-#pragma unroll
-    for (int r = 0; r < R; ++r) dst[base + r * Ns] = v[r];
-  }
-}
-
-// Full transform: data starts in a; buffers a and b ping-pong; returns the buffer
-// holding the result.  Every thread of the block must call it (it synchronises).
-template <typename T, int N, int Ns, int SIGN>
-__device__ __forceinline__ cx<T> *stockham_run(cx<T> *a, cx<T> *b, int t, int nt,
-                                               const cx<T> *__restrict__ tw) {
-  if constexpr (Ns >= N) {
-    return a;
-  } else {
-    constexpr int R = next_radix(N / Ns);
-    stockham_stage<T, N, Ns, R, SIGN>(a, b, t, nt, tw);
-    __syncthreads();
-    return stockham_run<T, N, Ns * R, SIGN>(b, a, t, nt, tw);
-  }
-}
-
-template <typename T, int N, int SIGN>
-__device__ __forceinline__ cx<T> *fft_lds(cx<T> *a, cx<T> *b, int t, int nt, const cx<T> *__restrict__ tw) {
-  return stockham_run<T, N, 1, SIGN>(a, b, t, nt, tw);
-}
 
 }  // namespace lp

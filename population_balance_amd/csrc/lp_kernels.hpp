@@ -662,10 +662,4 @@ __global__ void k_reduce_diag(const T *__restrict__ u, const T *__restrict__ v, 
   }
 }
 
-template <typename TI, typename TO>
-__global__ void k_convert(const TI *__restrict__ a, TO *__restrict__ b, long n) {
-  long i = gtid();
-  if (i < n) b[i] = (TO)a[i];
-}
-
 }  // namespace lp
