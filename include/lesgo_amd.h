@@ -159,6 +159,12 @@ void lp_config_default(lp_config *c);
  * scratch.  The droplet arrays of cfg->drop are copied. */
 int lp_create(const lp_config *cfg, lp_ctx **out);
 int lp_destroy(lp_ctx *ctx);
+/* Wait for all work of the context.  The routines enqueue on the context's own HIP
+ * streams; inside lp_step a second stream carries the convective term and the
+ * single-plane passes and is joined into the first before the step ends.  Calls given
+ * host arrays return after their copies back; with device arrays a call may return before
+ * its kernels finish, so synchronise before the results are read on another stream.
+ * lp_step returns when its steps have finished. */
 int lp_sync(lp_ctx *ctx);
 
 /* ---------------- state access ---------------- */

@@ -632,6 +632,8 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
 extern "C" int lp_sync(lp_ctx *c) {
   if (!c) return set_err(LP_ERR_ARG, "null context");
   HIPCHK(hipSetDevice(c->dev));
+  // s2 work is always joined into s, except after an error inside a step
+  HIPCHK(hipStreamSynchronize(c->s2));
   HIPCHK(hipStreamSynchronize(c->s));
   return LP_OK;
 }
