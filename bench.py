@@ -6,8 +6,9 @@ pressure gradient 1, seeded divergence-free k^-5/3 field + log law, Lagrangian
 scale-dependent SGS (inilag, cs_count = 1, DYN_init = 2), wall-stress bottom,
 stress-free lid.  The headline `value` is the FP32/FP64-mixed path; the FP64 path is
 timed beside it on the same initial state, and the mixed-vs-FP64 relative L2 of u, v,
-w, p after the run is reported.  The CPU baseline is the binary64 oracle (a C
-restatement of the reference) on the host cores, on a 256 x 256 x 33 slab sample.
+w, p after 1 and after 10 steps from that state is reported.  The CPU baseline is the
+binary64 oracle (a C restatement of the reference) on the host cores, on a
+256 x 256 x 33 slab sample.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3]
 """
