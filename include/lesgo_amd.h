@@ -223,6 +223,12 @@ int lp_project(lp_ctx *ctx);                    /* forcing.f90:122-294          
 int lp_timestep_conditions(lp_ctx *ctx, double *cfl, double *visc_stab);
                                                 /* scalars_module2.f90:1336-1379   */
 int lp_rmsdiv(lp_ctx *ctx, double *rmsdiv);     /* rmsdiv.f90:1-44                 */
+/* io.f90:1782-1849 (avg_stats, serial case, without theta): the instantaneous plane
+ * statistics that avg_stats time-averages and writes.  out (host, binary64) holds
+ * (nz-1) rows of 10: <u>, <v>, <w>, <Cs_opt2>, <Nu_t>, <u'^2>, <v'^2>, <w'^2>, <u'w'>,
+ * <txz> over x < nx; u', v' on the w-node, all three fluctuations zero on the first
+ * level.  The running averages and the output files stay with the caller. */
+int lp_plane_stats(lp_ctx *ctx, double *out);
 /* scalars_module_crossflow_mod.f90:1494-1786 (pollen_all_in_one2 with
  * pollen_RHS_calc2, calcbeta_pcon, step_pollen2) on the context state; needs npcon > 0. */
 int lp_pollen_all_in_one2(lp_ctx *ctx);

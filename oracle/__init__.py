@@ -70,6 +70,7 @@ def lib():
         L.or_tridag_array.restype = C.c_int
         L.or_timestep_conditions.argtypes = [vp, dp, dp]
         L.or_rmsdiv.argtypes = [vp, dp]
+        L.or_plane_stats.argtypes = [vp, dp]
         L.or_ke_dudt.argtypes = [vp]
         L.or_calculate_probability.argtypes = [vp, C.c_int, C.c_double, C.c_double, vp]
         L.or_step.argtypes = [vp, C.c_int]
@@ -282,6 +283,11 @@ class Oracle:
         a = C.c_double()
         self.L.or_rmsdiv(self.h, C.byref(a))
         return a.value
+
+    def plane_stats(self):
+        out = np.zeros((self.cfg.grid.nz - 1, 10))
+        self.L.or_plane_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
 
     def ke_dudt(self):
         self.L.or_ke_dudt(self.h)

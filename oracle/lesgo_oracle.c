@@ -1218,6 +1218,44 @@ void or_rmsdiv(or_ctx *c, double *rms) {
   *rms = s / ((double)g->nx * g->ny * (g->nz - 1));
 }
 
+/* io.f90:1782-1849 (avg_stats without MPI and theta): per level jz = 1..nz-1, row
+ * k = jz-1 of out[(nz-1)][10] = <u>, <v>, <w>, <Cs_opt2>, <Nu_t>, <u'^2>, <v'^2>, <w'^2>,
+ * <u'w'>, <txz>, plane means over x < nx.  u' and v' are moved onto the w-node (mean of
+ * levels jz and jz-1, each minus its own plane mean), w' = w - <w>; at jz = 1 the
+ * serial branch sets all three to zero (:1811-1815). */
+void or_plane_stats(or_ctx *c, double *out) {
+  const lp_grid *g = &c->g;
+  size_t np = plane(c);
+  const double *u = c->f[LP_U], *v = c->f[LP_V], *w = c->f[LP_W];
+  const double *cs = c->f[LP_CS_OPT2], *nu = c->f[LP_NU_T], *txz = c->f[LP_TXZ];
+  const double nxy = (double)g->nx * g->ny;
+  const int nl = g->nz - 1;
+  for (int k = 0; k < nl; ++k) {
+    double m[5] = {0, 0, 0, 0, 0};
+    for (int y = 0; y < g->ny; ++y)
+      for (int x = 0; x < g->nx; ++x) {
+        size_t i = (size_t)k * np + (size_t)y * g->ld + x;
+        m[0] += u[i]; m[1] += v[i]; m[2] += w[i]; m[3] += cs[i]; m[4] += nu[i];
+      }
+    for (int q = 0; q < 5; ++q) out[k * 10 + q] = m[q] / nxy;
+  }
+  for (int k = 0; k < nl; ++k) {
+    double s[5] = {0, 0, 0, 0, 0};
+    for (int y = 0; y < g->ny; ++y)
+      for (int x = 0; x < g->nx; ++x) {
+        size_t i = (size_t)k * np + (size_t)y * g->ld + x;
+        double up = 0.0, vp = 0.0, wp = 0.0;
+        if (k > 0) {
+          up = 0.5 * (u[i] - out[k * 10 + 0] + u[i - np] - out[(k - 1) * 10 + 0]);
+          vp = 0.5 * (v[i] - out[k * 10 + 1] + v[i - np] - out[(k - 1) * 10 + 1]);
+          wp = w[i] - out[k * 10 + 2];
+        }
+        s[0] += up * up; s[1] += vp * vp; s[2] += wp * wp; s[3] += up * wp; s[4] += txz[i];
+      }
+    for (int q = 0; q < 5; ++q) out[k * 10 + 5 + q] = s[q] / nxy;
+  }
+}
+
 /* ------------------------------------------------------------------------- */
 /* main.f90:864-898: ke = energy(); filt_da(ke); ddz_uv(ke); dudt += grad ke      */
 /* (the acceleration the droplet inertia term reads at the next step)          */

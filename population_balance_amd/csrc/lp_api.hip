@@ -133,6 +133,7 @@ struct lp_ctx {
   double *gam = nullptr;      // Thomas scratch (nz+1) x lh x ny
   void *pspec[3] = {};        // binary64 spectra for the p inverse
   int *d_err = nullptr;
+  double *d_stats = nullptr;  // lp_plane_stats: [nz-1][10]
   double *d_part = nullptr;
   int nred = 0;
   int jt = 1, jt_total = 1;
@@ -483,7 +484,7 @@ extern "C" int lp_destroy(lp_ctx *c) {
   for (auto p : c->Hs) fr(p);
   fr(c->bw); fr(c->tw); fr(c->gam); fr(c->phat);
   for (auto p : c->pspec) fr(p);
-  fr(c->d_err); fr(c->d_part);
+  fr(c->d_err); fr(c->d_part); fr(c->d_stats);
   fr(c->d_ws); fr(c->d_qsrc); fr(c->d_vfac); fr(c->d_foot); fr(c->d_bins); fr(c->d_pbeA);
   fr(c->ui0); fr(c->vi0); fr(c->kcell);
   for (auto e : c->ev) (void)hipEventDestroy(e);
@@ -614,6 +615,7 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
   for (auto &p : c->pspec) CK(dalloc(&p, sizeof(double) * 2 * (size_t)c->sp * g.ny * (g.nz + 1)));
   CK(dalloc(&c->phat, sizeof(double) * 2 * (size_t)c->sp * g.ny * (g.nz + 1)));
   CK(dalloc((void **)&c->d_err, sizeof(int) * 4));
+  CK(dalloc((void **)&c->d_stats, sizeof(double) * 10 * (size_t)(g.nz - 1)));
   c->nred = 512;
   CK(dalloc((void **)&c->d_part, sizeof(double) * 3 * c->nred));
   if (cfg->drop.npcon > 0) {
@@ -1567,6 +1569,25 @@ extern "C" int lp_rmsdiv(lp_ctx *c, double *rms) {
   double s = 0.0;
   for (int b = 0; b < c->nred; ++b) s += h[3 * b + 2];
   *rms = s / ((double)c->g.nx * c->g.ny * (c->g.nz - 1));
+  return LP_OK;
+}
+
+extern "C" int lp_plane_stats(lp_ctx *c, double *out) {
+  if (!c || !out) return set_err(LP_ERR_ARG, "lp_plane_stats: null argument");
+  HIPCHK(hipSetDevice(c->dev));
+  Geo g = geo(c);
+  const int nl = g.nz - 1;
+  FDISPATCH(c, T, {
+    hipLaunchKernelGGL((k_plane_stats<T, 0>), dim3(nl), dim3(256), 0, c->s, (const T *)c->f[LP_U],
+                       (const T *)c->f[LP_V], (const T *)c->f[LP_W], (const T *)c->f[LP_CS_OPT2],
+                       (const T *)c->f[LP_NU_T], (const T *)c->f[LP_TXZ], g, c->d_stats);
+    hipLaunchKernelGGL((k_plane_stats<T, 1>), dim3(nl), dim3(256), 0, c->s, (const T *)c->f[LP_U],
+                       (const T *)c->f[LP_V], (const T *)c->f[LP_W], (const T *)c->f[LP_CS_OPT2],
+                       (const T *)c->f[LP_NU_T], (const T *)c->f[LP_TXZ], g, c->d_stats);
+  });
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, c->d_stats, sizeof(double) * 10 * (size_t)nl, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
   return LP_OK;
 }
 

@@ -629,6 +629,63 @@ static __global__ void k_tridag_arrays(const double *__restrict__ a, const doubl
   }
 }
 
+// io.f90:1782-1849 (avg_stats): per-level plane sums of NQ quantities over x < nx, one
+// block of 256 threads per level k = blockIdx.x (levels 0..nz-2).  Pass 0: u, v, w,
+// Cs_opt2, Nu_t, txz.  Pass 1: u'^2, v'^2, w'^2, u'w' with u', v' on the w-node (mean of
+// levels k and k-1, each minus its plane mean) and w' = w - <w>; zero at k = 0 (the
+// serial branch of :1811-1815).  out[k*10 + q]: the plane means of the statistics.
+template <typename T, int PASS>
+__global__ __launch_bounds__(256) void k_plane_stats(const T *__restrict__ u, const T *__restrict__ v,
+                                                     const T *__restrict__ w, const T *__restrict__ cs,
+                                                     const T *__restrict__ nut, const T *__restrict__ txz, Geo g,
+                                                     double *__restrict__ out) {
+  constexpr int NQ = PASS == 0 ? 6 : 4;
+  __shared__ double red[NQ][256];
+  const int k = blockIdx.x;
+  const long base = (long)k * g.np;
+  double m_u = 0.0, m_v = 0.0, m_w = 0.0, m_um = 0.0, m_vm = 0.0;
+  if (PASS == 1) {
+    m_u = out[k * 10 + 0]; m_v = out[k * 10 + 1]; m_w = out[k * 10 + 2];
+    if (k > 0) { m_um = out[(k - 1) * 10 + 0]; m_vm = out[(k - 1) * 10 + 1]; }
+  }
+  double acc[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+  const int npts = g.nx * g.ny;
+  if (PASS == 0 || k > 0) {
+    for (int p = threadIdx.x; p < npts; p += 256) {
+      const long i = base + (long)(p / g.nx) * g.ld + p % g.nx;
+      if (PASS == 0) {
+        acc[0] += (double)u[i]; acc[1] += (double)v[i]; acc[2] += (double)w[i];
+        acc[3] += (double)cs[i]; acc[4] += (double)nut[i]; acc[NQ - 1] += (double)txz[i];
+      } else {
+        const double up = 0.5 * ((double)u[i] - m_u + (double)u[i - g.np] - m_um);
+        const double vp = 0.5 * ((double)v[i] - m_v + (double)v[i - g.np] - m_vm);
+        const double wp = (double)w[i] - m_w;
+        acc[0] += up * up; acc[1] += vp * vp; acc[2] += wp * wp; acc[3] += up * wp;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) red[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double nxy = (double)npts;
+    if (PASS == 0) {
+      for (int q = 0; q < 5; ++q) out[k * 10 + q] = red[q][0] / nxy;
+      out[k * 10 + 9] = red[5][0] / nxy;
+    } else {
+      for (int q = 0; q < 4; ++q) out[k * 10 + 5 + q] = red[q][0] / nxy;
+    }
+  }
+}
+
 // scalars_module2.f90:1336-1379 and rmsdiv.f90:27-36 partial reductions (levels 1..nz-1)
 template <typename T>
 __global__ void k_reduce_diag(const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,

@@ -13,6 +13,7 @@ module lesgo_amd
   public :: lp_divstress_uv, lp_divstress_w, lp_tridag_array
   public :: lp_wallstress, lp_sgs_stag, lp_lagrange_sdep, lp_interpolag_sdep, lp_convec
   public :: lp_press_stag_array, lp_project, lp_timestep_conditions, lp_rmsdiv, lp_step
+  public :: lp_plane_stats
   public :: lp_pollen_all_in_one2, lp_calculate_probability
   public :: lp_pollen_rhs_calc2, lp_calcbeta_pcon, lp_step_pollen2
   public :: LP_OK, LP_FP64, LP_MIXED, LP_LBC_WALL, LP_LBC_STRESS_FREE
@@ -239,6 +240,12 @@ module lesgo_amd
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: rms
+    end function
+    ! (10, nz-1) in Fortran order: the per-level statistics of avg_stats (io.f90:1782-1849)
+    integer(c_int) function lp_plane_stats(ctx, stats) bind(C, name="lp_plane_stats")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: stats(*)
     end function
     integer(c_int) function lp_pollen_all_in_one2(ctx) bind(C, name="lp_pollen_all_in_one2")
       import :: c_int, c_ptr

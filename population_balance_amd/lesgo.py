@@ -70,6 +70,7 @@ def load():
         "lp_project": ([vp], C.c_int),
         "lp_timestep_conditions": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
         "lp_rmsdiv": ([vp, C.POINTER(C.c_double)], C.c_int),
+        "lp_plane_stats": ([vp, C.POINTER(C.c_double)], C.c_int),
         "lp_step": ([vp, i32], C.c_int),
         "lp_pollen_all_in_one2": ([vp], C.c_int),
         "lp_pollen_rhs_calc2": ([vp], C.c_int),
@@ -267,6 +268,12 @@ class Lesgo:
         a = C.c_double()
         self._check(self.L.lp_rmsdiv(self.h, C.byref(a)), "rmsdiv")
         return a.value
+
+    def plane_stats(self):
+        """(nz-1, 10) per-level statistics of avg_stats (see lp_plane_stats)."""
+        out = np.zeros((self.g.nz - 1, 10))
+        self._check(self.L.lp_plane_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_double))), "plane_stats")
+        return out
 
     def pollen_all_in_one2(self):
         self._check(self.L.lp_pollen_all_in_one2(self.h), "pollen_all_in_one2")

@@ -204,6 +204,32 @@ def test_timestep_conditions_and_rmsdiv(prec):
     assert abs(r_g - r_o) <= 1e-12 * r_o
 
 
+@pytest.mark.parametrize("prec", [abi.LP_FP64, abi.LP_MIXED])
+def test_plane_stats(prec):
+    """io.f90:1782-1849 (avg_stats' per-level statistics) as device reductions: each column
+    within 1e-12 of the oracle, relative to the field's rms (means) or the moment's size."""
+    cfg = abi.make_config(64, 64, 17, precision=prec)
+    cfg64 = abi.make_config(64, 64, 17)
+    from oracle import Oracle
+    from population_balance_amd.lesgo import Lesgo
+    print(f"\n[case] plane_stats 64x64x17 prec={prec}", flush=True)
+    o, lg = Oracle(cfg64), Lesgo(cfg)
+    st = seeded_state(cfg)
+    st.update(random_fields(cfg, ["Cs_opt2", "Nu_t", "txz"], seed=9))
+    if prec == abi.LP_MIXED:  # identical inputs on both sides
+        st = {k: v.astype(np.float32).astype(np.float64) for k, v in st.items()}
+    load_oracle(o, st); load_gpu(lg, st)
+    so, sg = o.plane_stats(), lg.plane_stats()
+    nx = cfg.grid.nx
+    rms = lambda k: float(np.sqrt(np.mean(st[k][..., :nx] ** 2)))
+    scale = [rms("u"), rms("v"), rms("w"), rms("Cs_opt2"), rms("Nu_t"),
+             so[:, 5].max(), so[:, 6].max(), so[:, 7].max(), np.abs(so[:, 8]).max(), rms("txz")]
+    for q in range(10):
+        assert np.max(np.abs(sg[:, q] - so[:, q])) <= 1e-12 * scale[q], q
+    assert np.all(sg[0, 5:9] == 0.0)
+    o.close(); lg.close()
+
+
 @pytest.mark.parametrize("nx", [32, 128])
 def test_interpolag_entry(nx):
     """interpolag_Sdep.f90:62-369 on its own: F_* replaced by the interpolated averages,

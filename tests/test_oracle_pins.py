@@ -512,3 +512,49 @@ def test_timestep_conditions_closed_form():
     delta = min(g.dx, g.dy, g.dz)
     assert math.isclose(cfl, 5.0 * cfg.dt / delta, rel_tol=1e-14)
     assert math.isclose(visc, cfg.dt * 0.01 / delta ** 2, rel_tol=1e-14)
+
+
+# ---------------- avg_stats plane statistics: closed form ----------------
+def test_plane_stats_closed_form():
+    """io.f90:1782-1849 on u = U_k + a_k cos x, v = V_k + b_k sin y, w = W_k + c_k cos x:
+    the plane means are U_k, V_k, W_k; on the w-node of level k >= 1, u' = A cos x with
+    A = (a_k + a_{k-1})/2 (the average of the two levels), v' = B sin y likewise and
+    w' = c_k cos x, so <u'^2> = A^2/2, <v'^2> = B^2/2, <w'^2> = c_k^2/2, <u'w'> = A c_k/2;
+    all four vanish on the first level (serial branch).  Amplitudes differ per level, so
+    a missing w-node average is caught."""
+    from oracle import Oracle
+    cfg = abi.make_config(16, 16, 7)
+    g = cfg.grid
+    o = Oracle(cfg)
+    x = np.arange(g.nx) * g.dx
+    y = np.arange(g.ny) * g.dy
+    a = np.linspace(0.7, 1.3, g.nz)
+    b = np.linspace(0.3, 0.1, g.nz)
+    c = np.linspace(0.5, 0.9, g.nz)
+    U = np.linspace(1.0, 2.0, g.nz)
+    V = np.linspace(-0.5, 0.25, g.nz)
+    W = np.linspace(0.1, 0.4, g.nz)
+    f = {k: np.zeros((g.nz, g.ny, g.ld)) for k in ("u", "v", "w", "Cs_opt2", "Nu_t", "txz")}
+    f["u"][:, :, :g.nx] = U[:, None, None] + a[:, None, None] * np.cos(x)[None, None, :]
+    f["v"][:, :, :g.nx] = V[:, None, None] + b[:, None, None] * np.sin(y)[None, :, None]
+    f["w"][:, :, :g.nx] = W[:, None, None] + c[:, None, None] * np.cos(x)[None, None, :]
+    f["Cs_opt2"][:, :, :g.nx] = 0.01 * np.arange(g.nz)[:, None, None]
+    f["Nu_t"][:, :, :g.nx] = 2.0
+    f["txz"][:, :, :g.nx] = -np.arange(g.nz)[:, None, None]
+    f["u"][:, :, g.nx:] = 99.0  # padding columns stay out of the means
+    for k, arr in f.items():
+        o.set_field(k, arr)
+    s = o.plane_stats()
+    nl = g.nz - 1
+    tol = dict(rtol=1e-13, atol=1e-13)
+    assert np.allclose(s[:, 0], U[:nl], **tol) and np.allclose(s[:, 1], V[:nl], **tol)
+    assert np.allclose(s[:, 2], W[:nl], **tol)
+    assert np.allclose(s[:, 3], 0.01 * np.arange(nl), **tol) and np.allclose(s[:, 4], 2.0, **tol)
+    assert np.allclose(s[:, 9], -np.arange(nl), **tol)
+    assert np.all(s[0, 5:9] == 0.0)
+    A = 0.5 * (a[1:nl] + a[:nl - 1])
+    B = 0.5 * (b[1:nl] + b[:nl - 1])
+    C_ = c[1:nl]
+    assert np.allclose(s[1:, 5], A * A / 2, **tol) and np.allclose(s[1:, 6], B * B / 2, **tol)
+    assert np.allclose(s[1:, 7], C_ * C_ / 2, **tol) and np.allclose(s[1:, 8], A * C_ / 2, **tol)
+    o.close()
