@@ -229,6 +229,14 @@ int lp_rmsdiv(lp_ctx *ctx, double *rmsdiv);     /* rmsdiv.f90:1-44              
  * <txz> over x < nx; u', v' on the w-node, all three fluctuations zero on the first
  * level.  The running averages and the output files stay with the caller. */
 int lp_plane_stats(lp_ctx *ctx, double *out);
+/* io.f90:1643-1679 (spectrum), called on every level as avg_stats (io.f90:1846-1848,
+ * E11 before its division by z) and post_spec (io.f90:1587-1594) do: out (host,
+ * binary64) holds (nz-1) rows of nx/2, row k the y-average of the streamwise power of
+ * level k, sum_y w_m |X_m|^2 / ny with X the forward transform of u(0:nx-1, y)/nx,
+ * w_0 = 1/2 (real part), w_m = 1; no Nyquist term.  field: a volume of the context
+ * (not LP_P, LP_USTAR); bin selects the size bin of a per-bin volume (LP_PCON ...),
+ * 0 otherwise.  The transform runs in the field's own type. */
+int lp_spectrum(lp_ctx *ctx, int32_t field, int32_t bin, double *out);
 /* scalars_module_crossflow_mod.f90:1494-1786 (pollen_all_in_one2 with
  * pollen_RHS_calc2, calcbeta_pcon, step_pollen2) on the context state; needs npcon > 0. */
 int lp_pollen_all_in_one2(lp_ctx *ctx);

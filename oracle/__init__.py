@@ -71,6 +71,8 @@ def lib():
         L.or_timestep_conditions.argtypes = [vp, dp, dp]
         L.or_rmsdiv.argtypes = [vp, dp]
         L.or_plane_stats.argtypes = [vp, dp]
+        L.or_spectrum.argtypes = [vp, C.c_int, C.c_int, dp]
+        L.or_spectrum.restype = C.c_int
         L.or_ke_dudt.argtypes = [vp]
         L.or_calculate_probability.argtypes = [vp, C.c_int, C.c_double, C.c_double, vp]
         L.or_step.argtypes = [vp, C.c_int]
@@ -287,6 +289,15 @@ class Oracle:
     def plane_stats(self):
         out = np.zeros((self.cfg.grid.nz - 1, 10))
         self.L.or_plane_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def spectrum(self, name, bin=0):
+        """(nz-1, nx/2) streamwise energy spectrum of every level (io.f90:1643-1679)."""
+        g = self.cfg.grid
+        out = np.zeros((g.nz - 1, g.nx // 2))
+        rc = self.L.or_spectrum(self.h, abi.FIELD[name], bin, out.ctypes.data_as(C.POINTER(C.c_double)))
+        if rc != 0:
+            raise ValueError(f"spectrum: field {name!r} bin {bin} is not a volume of the context")
         return out
 
     def ke_dudt(self):

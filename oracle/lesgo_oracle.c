@@ -1257,6 +1257,41 @@ void or_plane_stats(or_ctx *c, double *out) {
 }
 
 /* ------------------------------------------------------------------------- */
+/* io.f90:1643-1679 (spectrum) on levels 1..nz-1 of a volume (bin `bin` of a    */
+/* per-bin volume): each row u(1:nx, y)/nx is transformed forward; with X_m its */
+/* m-th coefficient, E(m) = sum_y w_m |X_m|^2 / ny for m < nx/2, w_0 = 1/2 on   */
+/* the real part, w_m = 1 otherwise; the Nyquist term is left out.  out[k*nh+m] */
+/* ------------------------------------------------------------------------- */
+int or_spectrum(or_ctx *c, int field, int bin, double *out) {
+  const lp_grid *g = &c->g;
+  if (field < 0 || field >= LP_NFIELDS || field == LP_P || field == LP_USTAR || bin < 0 ||
+      (size_t)(bin + 1) * vol(c) > c->n[field])
+    return 1;
+  const int nh = g->nx / 2, nl = g->nz - 1;
+  const double *f = c->f[field] + (size_t)bin * vol(c);
+  fplan P;
+  fplan_init(&P, g->nx);
+  cplx *row = (cplx *)malloc(sizeof(cplx) * (size_t)g->nx);
+  cplx *X = (cplx *)malloc(sizeof(cplx) * (size_t)g->nx);
+  for (int k = 0; k < nl; ++k) {
+    double *e = out + (size_t)k * nh;
+    for (int m = 0; m < nh; ++m) e[m] = 0.0;
+    for (int y = 0; y < g->ny; ++y) {
+      const double *r = f + (size_t)k * plane(c) + (size_t)y * g->ld;
+      for (int x = 0; x < g->nx; ++x) row[x] = r[x] / (double)g->nx;
+      dft(&P, -1, X, row);
+      e[0] += 0.5 * creal(X[0]) * creal(X[0]);
+      for (int m = 1; m < nh; ++m) e[m] += creal(X[m]) * creal(X[m]) + cimag(X[m]) * cimag(X[m]);
+    }
+    for (int m = 0; m < nh; ++m) e[m] /= (double)g->ny;
+  }
+  free(row);
+  free(X);
+  fplan_free(&P);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
 /* main.f90:864-898: ke = energy(); filt_da(ke); ddz_uv(ke); dudt += grad ke      */
 /* (the acceleration the droplet inertia term reads at the next step)          */
 /* ------------------------------------------------------------------------- */

@@ -134,6 +134,7 @@ struct lp_ctx {
   void *pspec[3] = {};        // binary64 spectra for the p inverse
   int *d_err = nullptr;
   double *d_stats = nullptr;  // lp_plane_stats: [nz-1][10]
+  double *d_espec = nullptr;  // lp_spectrum: [nz-1][nx/2]
   double *d_part = nullptr;
   int nred = 0;
   int jt = 1, jt_total = 1;
@@ -484,7 +485,7 @@ extern "C" int lp_destroy(lp_ctx *c) {
   for (auto p : c->Hs) fr(p);
   fr(c->bw); fr(c->tw); fr(c->gam); fr(c->phat);
   for (auto p : c->pspec) fr(p);
-  fr(c->d_err); fr(c->d_part); fr(c->d_stats);
+  fr(c->d_err); fr(c->d_part); fr(c->d_stats); fr(c->d_espec);
   fr(c->d_ws); fr(c->d_qsrc); fr(c->d_vfac); fr(c->d_foot); fr(c->d_bins); fr(c->d_pbeA);
   fr(c->ui0); fr(c->vi0); fr(c->kcell);
   for (auto e : c->ev) (void)hipEventDestroy(e);
@@ -616,6 +617,7 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
   CK(dalloc(&c->phat, sizeof(double) * 2 * (size_t)c->sp * g.ny * (g.nz + 1)));
   CK(dalloc((void **)&c->d_err, sizeof(int) * 4));
   CK(dalloc((void **)&c->d_stats, sizeof(double) * 10 * (size_t)(g.nz - 1)));
+  CK(dalloc((void **)&c->d_espec, sizeof(double) * (size_t)(g.nx / 2) * (g.nz - 1)));
   c->nred = 512;
   CK(dalloc((void **)&c->d_part, sizeof(double) * 3 * c->nred));
   if (cfg->drop.npcon > 0) {
@@ -1587,6 +1589,36 @@ extern "C" int lp_plane_stats(lp_ctx *c, double *out) {
   });
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, c->d_stats, sizeof(double) * 10 * (size_t)nl, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return LP_OK;
+}
+
+// io.f90:1643-1679 on levels 1..nz-1: the rows of the volume (bin `bin` of a per-bin
+// volume) go through the x transform, scaled by 1/nx, in the field's own type, then
+// k_spectrum forms the y-averaged power per wavenumber.
+extern "C" int lp_spectrum(lp_ctx *c, int32_t field, int32_t bin, double *out) {
+  if (!c || !out) return set_err(LP_ERR_ARG, "lp_spectrum: null argument");
+  const long nv = vol_el(c);
+  if (field < 0 || field >= LP_NFIELDS || field == LP_P || field == LP_USTAR || bin < 0 ||
+      c->bytes[field] < (size_t)(bin + 1) * nv * c->esz[field])
+    return set_err(LP_ERR_ARG, "lp_spectrum: field %d bin %d is not a volume of the context", field, bin);
+  HIPCHK(hipSetDevice(c->dev));
+  const lp_grid &g = c->g;
+  const int nl = g.nz - 1, nh = g.nx / 2;
+  const bool f64 = c->esz[field] == 8;
+  const Dt dt = f64 ? F64 : c->fdt;
+  void *spec = f64 ? c->pspec[0] : c->spec[0];  // (nz+1) resp. nz levels of (sp, ny) complex
+  const char *in = (const char *)c->f[field] + (size_t)bin * nv * c->esz[field];
+  HIPCHK(row_r2c(dt, nh, in, g.ld, plane_el(c), g.ny, nl, 1.0 / g.nx, spec, c->sp, c->sp * g.ny, nh, c->s));
+  const dim3 grid((nh + 63) / 64, nl);
+  if (f64)
+    hipLaunchKernelGGL(k_spectrum<double>, grid, dim3(64), 0, c->s, (const double *)spec, c->sp, g.ny, nh,
+                       c->d_espec);
+  else
+    hipLaunchKernelGGL(k_spectrum<float>, grid, dim3(64), 0, c->s, (const float *)spec, c->sp, g.ny, nh,
+                       c->d_espec);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, c->d_espec, sizeof(double) * (size_t)nh * nl, hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
   return LP_OK;
 }

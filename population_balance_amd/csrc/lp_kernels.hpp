@@ -686,6 +686,25 @@ __global__ __launch_bounds__(256) void k_plane_stats(const T *__restrict__ u, co
   }
 }
 
+// io.f90:1643-1679 (spectrum) from the rows' half spectra X (rows already divided by nx;
+// complex pairs of T, row pitch `pitch` complex values, ny rows per level): thread m of
+// level blockIdx.y sums w_m |X_m|^2 over the rows in order and divides by ny; w_0 = 1/2 on
+// the real part (all of X_0 for a real row), w_m = 1 for 0 < m < nh; no Nyquist term.
+template <typename T>
+__global__ __launch_bounds__(64) void k_spectrum(const T *__restrict__ spec, long pitch, int ny, int nh,
+                                                 double *__restrict__ out) {
+  const int m = blockIdx.x * 64 + threadIdx.x;
+  const int k = blockIdx.y;
+  if (m >= nh) return;
+  const T *p = spec + 2 * ((long)k * ny * pitch + m);
+  double acc = 0.0;
+  for (int y = 0; y < ny; ++y, p += 2 * pitch) {
+    const double re = (double)p[0], im = (double)p[1];
+    acc += m == 0 ? 0.5 * re * re : re * re + im * im;
+  }
+  out[(long)k * nh + m] = acc / (double)ny;
+}
+
 // scalars_module2.f90:1336-1379 and rmsdiv.f90:27-36 partial reductions (levels 1..nz-1)
 template <typename T>
 __global__ void k_reduce_diag(const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,

@@ -13,7 +13,7 @@ module lesgo_amd
   public :: lp_divstress_uv, lp_divstress_w, lp_tridag_array
   public :: lp_wallstress, lp_sgs_stag, lp_lagrange_sdep, lp_interpolag_sdep, lp_convec
   public :: lp_press_stag_array, lp_project, lp_timestep_conditions, lp_rmsdiv, lp_step
-  public :: lp_plane_stats
+  public :: lp_plane_stats, lp_spectrum
   public :: lp_pollen_all_in_one2, lp_calculate_probability
   public :: lp_pollen_rhs_calc2, lp_calcbeta_pcon, lp_step_pollen2
   public :: LP_OK, LP_FP64, LP_MIXED, LP_LBC_WALL, LP_LBC_STRESS_FREE
@@ -246,6 +246,13 @@ module lesgo_amd
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: stats(*)
+    end function
+    ! (nx/2, nz-1) in Fortran order: spectrum (io.f90:1643-1679) of every level of a field
+    integer(c_int) function lp_spectrum(ctx, field, bin, spec) bind(C, name="lp_spectrum")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: field, bin
+      real(c_double), intent(out) :: spec(*)
     end function
     integer(c_int) function lp_pollen_all_in_one2(ctx) bind(C, name="lp_pollen_all_in_one2")
       import :: c_int, c_ptr

@@ -71,6 +71,7 @@ def load():
         "lp_timestep_conditions": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
         "lp_rmsdiv": ([vp, C.POINTER(C.c_double)], C.c_int),
         "lp_plane_stats": ([vp, C.POINTER(C.c_double)], C.c_int),
+        "lp_spectrum": ([vp, i32, i32, C.POINTER(C.c_double)], C.c_int),
         "lp_step": ([vp, i32], C.c_int),
         "lp_pollen_all_in_one2": ([vp], C.c_int),
         "lp_pollen_rhs_calc2": ([vp], C.c_int),
@@ -273,6 +274,14 @@ class Lesgo:
         """(nz-1, 10) per-level statistics of avg_stats (see lp_plane_stats)."""
         out = np.zeros((self.g.nz - 1, 10))
         self._check(self.L.lp_plane_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_double))), "plane_stats")
+        return out
+
+    def spectrum(self, name, bin=0):
+        """(nz-1, nx/2) streamwise energy spectrum of every level (see lp_spectrum); avg_stats'
+        E11(k1, z)/z is row k divided by (k + 1/2) dz."""
+        out = np.zeros((self.g.nz - 1, self.g.nx // 2))
+        rc = self.L.lp_spectrum(self.h, abi.FIELD[name], bin, out.ctypes.data_as(C.POINTER(C.c_double)))
+        self._check(rc, "spectrum")
         return out
 
     def pollen_all_in_one2(self):

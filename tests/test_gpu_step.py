@@ -230,6 +230,39 @@ def test_plane_stats(prec):
     o.close(); lg.close()
 
 
+@pytest.mark.parametrize("prec", [abi.LP_FP64, abi.LP_MIXED])
+def test_spectrum(prec):
+    """io.f90:1643-1679 on every level of u, v, w and of one PCon bin (binary64 in both
+    precisions): per level, every wavenumber within tol x the level's largest value of the
+    oracle's spectrum (1e-12 with binary64 transforms, 1e-5 with binary32 ones)."""
+    from oracle import Oracle
+    from population_balance_amd import droplets
+    from population_balance_amd.lesgo import Lesgo
+    cfg = abi.make_config(64, 64, 9, precision=prec)
+    cfg64 = abi.make_config(64, 64, 9)
+    for c in (cfg, cfg64):
+        droplets.DropletSet(3).fill(c)
+    print(f"\n[case] spectrum 64x64x9 prec={prec}", flush=True)
+    o, lg = Oracle(cfg64), Lesgo(cfg)
+    st = seeded_state(cfg)
+    if prec == abi.LP_MIXED:
+        st = {k: v.astype(np.float32).astype(np.float64) for k, v in st.items()}
+    load_oracle(o, st); load_gpu(lg, st)
+    g = cfg.grid
+    pc = np.random.default_rng(4).standard_normal((3, g.nz, g.ny, g.ld))
+    o.set_field("PCon", pc); lg.upload("PCon", pc)
+    for name, bn, tol in [("u", 0, None), ("v", 0, None), ("w", 0, None), ("PCon", 2, 1e-12)]:
+        tol = tol or (1e-12 if prec == abi.LP_FP64 else 1e-5)
+        Eo, Eg = o.spectrum(name, bn), lg.spectrum(name, bn)
+        assert Eg.shape == Eo.shape == (g.nz - 1, g.nx // 2)
+        # a level that is zero (w on the wall) must come out zero
+        err = np.max(np.abs(Eg - Eo), axis=1) / np.maximum(np.max(Eo, axis=1), 1e-300)
+        assert np.all(err <= tol), (name, err.max())
+    with pytest.raises(RuntimeError):
+        lg.spectrum("PCon", 3)
+    o.close(); lg.close()
+
+
 @pytest.mark.parametrize("nx", [32, 128])
 def test_interpolag_entry(nx):
     """interpolag_Sdep.f90:62-369 on its own: F_* replaced by the interpolated averages,

@@ -558,3 +558,63 @@ def test_plane_stats_closed_form():
     assert np.allclose(s[1:, 5], A * A / 2, **tol) and np.allclose(s[1:, 6], B * B / 2, **tol)
     assert np.allclose(s[1:, 7], C_ * C_ / 2, **tol) and np.allclose(s[1:, 8], A * C_ / 2, **tol)
     o.close()
+
+
+def test_spectrum_closed_form():
+    """io.f90:1643-1679 on u = U_k + a_k(y) cos(m1 x + phi_y) + b_k sin(m2 x) + d_k (-1)^i:
+    the transform of a row over nx is U_k at m = 0, a e^{i phi}/2 at m1 and b/(2i) at m2, so
+    E(0) = U_k^2/2, E(m1) = <a_k^2>_y / 4, E(m2) = b_k^2/4 and every other m < nx/2 is zero:
+    the Nyquist wave d_k (-1)^i is left out, and the padding columns are not read.  A lost
+    1/nx, a missing 1/2 on the mean, a missing y-average or a Nyquist term each fail."""
+    cfg = abi.make_config(16, 8, 6)
+    g = cfg.grid
+    o = Oracle(cfg)
+    i = np.arange(g.nx)
+    x = 2 * np.pi * i / g.nx
+    m1, m2 = 3, 5
+    U = np.linspace(0.5, 2.0, g.nz)
+    b = np.linspace(0.2, 0.9, g.nz)
+    d = np.linspace(1.0, 3.0, g.nz)
+    a = 0.3 + 0.1 * np.arange(g.nz)[:, None] + 0.05 * np.arange(g.ny)[None, :]  # (nz, ny)
+    phi = 0.4 * np.arange(g.ny)
+    u = np.full((g.nz, g.ny, g.ld), 77.0)
+    u[:, :, :g.nx] = (U[:, None, None] + a[:, :, None] * np.cos(m1 * x[None, None, :] + phi[None, :, None])
+                      + b[:, None, None] * np.sin(m2 * x)[None, None, :]
+                      + d[:, None, None] * ((-1.0) ** i)[None, None, :])
+    o.set_field("u", u)
+    E = o.spectrum("u")
+    nl = g.nz - 1
+    want = np.zeros((nl, g.nx // 2))
+    want[:, 0] = 0.5 * U[:nl] ** 2
+    want[:, m1] = np.mean(a[:nl] ** 2, axis=1) / 4
+    want[:, m2] = b[:nl] ** 2 / 4
+    assert E.shape == (nl, g.nx // 2)
+    assert np.allclose(E, want, rtol=1e-13, atol=1e-15)
+    o.close()
+
+
+def test_spectrum_parseval_and_bins():
+    """Parseval on random rows: 2 sum_m E(m) + <X_{nx/2}^2>_y = <u^2> per level, with
+    X_{nx/2} = mean_x u (-1)^i the left-out Nyquist coefficient; the bin argument picks that
+    bin's volume of PCon, and a field that is no volume (p) or a bin past npcon is refused."""
+    from population_balance_amd import droplets
+    cfg = abi.make_config(24, 6, 5)
+    ds = droplets.DropletSet(3)
+    ds.fill(cfg)
+    g = cfg.grid
+    o = Oracle(cfg)
+    rng = np.random.default_rng(17)
+    pc = rng.standard_normal((3, g.nz, g.ny, g.ld))
+    o.set_field("PCon", pc)
+    sgn = (-1.0) ** np.arange(g.nx)
+    nl = g.nz - 1
+    for bn in range(3):
+        E = o.spectrum("PCon", bin=bn)
+        r = pc[bn, :nl, :, :g.nx]
+        nyq = np.mean(np.mean(r * sgn, axis=2) ** 2, axis=1)
+        assert np.allclose(2 * E.sum(axis=1) + nyq, np.mean(r ** 2, axis=(1, 2)), rtol=1e-13)
+    with pytest.raises(ValueError):
+        o.spectrum("PCon", bin=3)
+    with pytest.raises(ValueError):
+        o.spectrum("p")
+    o.close()
