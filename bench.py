@@ -217,6 +217,21 @@ def main():
         for k, v_ in breakdown.items():
             log(f"[bench]   {k:14s} {v_['ms_per_step']:8.3f} ms/step  {v_['launches_per_step']:6.0f} "
                 f"launches  {v_['GBps']:7.0f} GB/s")
+    # a host that takes u, v, w out and puts them back every step (pageable numpy
+    # buffers through lp_download / lp_upload): the step time plus that exchange
+    reps = 3
+    t = time.perf_counter()
+    for _ in range(reps):
+        hv = [lg_m.download(n) for n in ("u", "v", "w")]
+        for n, h in zip(("u", "v", "w"), hv):
+            lg_m.upload(n, h)
+    lg_m.sync()
+    ms_x = (time.perf_counter() - t) / reps * 1e3
+    pcie = {"value": round(pts / ((ms_m + ms_x) * 1e-3) / 1e6, 2), "unit": "Mpt-steps/s",
+            "exchange_ms": round(ms_x, 3),
+            "exchange": "u, v, w downloaded and uploaded once per step, pageable host memory"}
+    log(f"[bench] with a host exchange of u, v, w per step: {pcie['value']} Mpt-steps/s "
+        f"({ms_x:.3f} ms exchange)")
 
     result = {
         "metric": METRIC, "value": round(val_m, 2), "unit": "Mpt-steps/s", "n_gpus": 1,
@@ -237,6 +252,7 @@ def main():
         "step_roofline": {"bytes_per_pt": B_PT["mixed"],
                           "achieved_GBps": round(B_PT["mixed"] * pts / (ms_m * 1e-3) / 1e9, 1),
                           "frac": round(B_PT["mixed"] * pts / (ms_m * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)},
+        "pcie_inclusive": pcie,
     }
     if breakdown:
         result["kernel_classes"] = breakdown
