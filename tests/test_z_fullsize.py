@@ -1,0 +1,73 @@
+"""The throughput workload at its full size: 256 x 256 x 257 levels, 2pi x 2pi x 1,
+dt = 2e-4, mean pressure gradient 1, the seeded turbulent initial condition with the log
+law.  Two whole binary64 steps against the oracle (u, v, w, p, RHS within 1e-12 relative
+L2; the oracle needs about 15 s per step on 16 host cores), and the mixed path within 1e-5
+of the binary64 path after one and after ten steps from the same state."""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from population_balance_amd import abi, ic
+
+pytestmark = pytest.mark.gpu
+
+NX, NZ = 256, 257
+KW = dict(use_mean_p_force=1, mean_p_force=1.0)
+
+
+@pytest.fixture(scope="module")
+def state():
+    cfg = abi.make_config(NX, NX, NZ, **KW)
+    return ic.turbulent_ic(cfg.grid, seed=ic.DEFAULT_SEED)
+
+
+def _levels(name):
+    return slice(0, NZ) if name == "p" else slice(0, NZ - 1)
+
+
+def test_fullsize_fp64_steps_against_oracle(state):
+    from oracle import Oracle
+    from population_balance_amd.lesgo import Lesgo
+    cfg = abi.make_config(NX, NX, NZ, **KW)
+    print(f"\n[case] full size fp64 {NX}x{NX}x{NZ}, 2 steps against the oracle", flush=True)
+    o, lg = Oracle(cfg), Lesgo(cfg)
+    for n, a in zip(("u", "v", "w"), state):
+        o.set_field(n, a)
+        lg.upload(n, a)
+    for s in range(2):
+        t = time.perf_counter()
+        o.step(1)
+        t_o = time.perf_counter() - t
+        lg.step(1)
+        errs = {n: rel_l2(lg.download(n)[_levels(n)], o.field(n)[_levels(n)], NX)
+                for n in ("u", "v", "w", "p", "RHSx", "RHSy", "RHSz")}
+        print(f"  step {s + 1} (oracle {t_o:.1f} s): " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()),
+              flush=True)
+        bad = {k: v for k, v in errs.items() if not v < 1e-12}
+        assert not bad, bad
+    assert lg.time() == o.time()
+    o.close()
+    lg.close()
+
+
+def test_fullsize_mixed_within_1e5_of_fp64(state):
+    from population_balance_amd.lesgo import Lesgo
+    print(f"\n[case] full size mixed vs fp64 {NX}x{NX}x{NZ}, after 1 and 10 steps", flush=True)
+    runs = {}
+    for prec in (abi.LP_MIXED, abi.LP_FP64):
+        lg = Lesgo(abi.make_config(NX, NX, NZ, precision=prec, **KW))
+        for n, a in zip(("u", "v", "w"), state):
+            lg.upload(n, a)
+        out = {}
+        for target in (1, 10):
+            lg.step(target - lg.time()[0] + 1)
+            out[target] = {n: lg.download(n)[_levels(n), :, :NX].astype(np.float64) for n in ("u", "v", "w", "p")}
+        runs[prec] = out
+        lg.close()
+    for target in (1, 10):
+        errs = {n: rel_l2(runs[abi.LP_MIXED][target][n], runs[abi.LP_FP64][target][n])
+                for n in ("u", "v", "w", "p")}
+        print(f"  after {target} steps: " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
+        assert all(v <= 1e-5 for v in errs.values()), errs
