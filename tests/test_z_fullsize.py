@@ -71,3 +71,57 @@ def test_fullsize_mixed_within_1e5_of_fp64(state):
                 for n in ("u", "v", "w", "p")}
         print(f"  after {target} steps: " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
         assert all(v <= 1e-5 for v in errs.values()), errs
+
+
+def test_fullsize_droplet_steps_against_oracle():
+    """The droplet workload at its full size: 192 x 192 x 128 levels, 15 log-spaced bins,
+    settling, inertia, PBE breakup, buoyancy, the Gaussian source.  Two steps of the oracle;
+    the binary64 path within 1e-12 (state, RHS, PCon, RHS_PCon, Rhs_pbe, beta_pcon, magS)
+    and the mixed path within 1e-5 of it on the state, PCon and beta_pcon (RHS_PCon within
+    1e-4, for the reason given below)."""
+    from oracle import Oracle
+    from population_balance_amd import droplets
+    from population_balance_amd.lesgo import Lesgo
+    nx, nz, nb = 192, 128, 15
+    cfgs, sets = {}, {}  # the configs point into their sets' arrays: keep both alive
+    for prec in (abi.LP_FP64, abi.LP_MIXED):
+        cfgs[prec] = abi.make_config(nx, nx, nz, precision=prec, **KW)
+        sets[prec] = droplets.DropletSet(nb)
+        sets[prec].fill(cfgs[prec])
+    ds = sets[abi.LP_FP64]
+    g = cfgs[abi.LP_FP64].grid
+    print(f"\n[case] full size droplets {nx}x{nx}x{nz}, {nb} bins, 2 steps against the oracle", flush=True)
+    u, v, w = ic.turbulent_ic(g, seed=ic.DEFAULT_SEED)
+    C = droplets.initial_concentration(cfgs[abi.LP_FP64], ds)
+    o = Oracle(cfgs[abi.LP_FP64])
+    for n, a in (("u", u), ("v", v), ("w", w), ("PCon", C)):
+        o.set_field(n, a)
+    t = time.perf_counter()
+    o.step(2)
+    print(f"  oracle: {time.perf_counter() - t:.1f} s", flush=True)
+    checks = {abi.LP_FP64: (1e-12, ["u", "v", "w", "RHSx", "RHSy", "RHSz", "p", "PCon", "RHS_PCon", "Rhs_pbe",
+                                    "beta_pcon", "magS"]),
+              abi.LP_MIXED: (1e-5, ["u", "v", "w", "p", "PCon", "beta_pcon"])}
+    # RHS_PCon of the mixed path is the flux divergence of binary64 concentrations carried
+    # by binary32 velocities: the divergence weights the velocity's rounding noise (high
+    # wavenumbers) against the smooth plume, so this tendency differs by about 10x the
+    # state's difference (measured 1.4e-5 while PCon itself is at 6e-7); its bound is 1e-4.
+    tend_tol = {abi.LP_FP64: 1e-12, abi.LP_MIXED: 1e-4}
+    for prec, (tol, names) in checks.items():
+        lg = Lesgo(cfgs[prec])
+        for n, a in (("u", u), ("v", v), ("w", w), ("PCon", C)):
+            lg.upload(n, a)
+        lg.step(2)
+        errs = {}
+        for n in names + ([] if prec == abi.LP_FP64 else ["RHS_PCon"]):
+            a, b = lg.download(n), o.field(n)
+            if n in ("u", "v", "w", "RHSx", "RHSy", "RHSz"):
+                a, b = a[:nz - 1], b[:nz - 1]
+            elif n == "p":
+                a, b = a[:nz], b[:nz]
+            errs[n] = rel_l2(a, b, nx)
+        lg.close()
+        print(f"  prec={prec}: " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
+        bad = {k: v for k, v in errs.items() if not v < (tol if k in names else tend_tol[prec])}
+        assert not bad, bad
+    o.close()
