@@ -2,7 +2,8 @@
 ! ISO_C_BINDING module, with arrays declared exactly as the reference declares them
 ! (real(rprec) f(ld,ny,nz), column-major).  Without arguments it only checks the
 ! structure layout; with "--run" it creates a context on the GPU and checks filt_da
-! (filt_da.f90 argument order) and ddz_uv on closed-form fields.
+! (filt_da.f90 argument order), ddz_uv and the per-level spectrum of io.f90:1643-1679 on
+! closed-form fields.
 program drop_in_demo
   use iso_c_binding
   use lesgo_amd
@@ -13,6 +14,8 @@ program drop_in_demo
   integer(c_int) :: rc
   integer :: jx, jy, jz, nx, ny, nz, ld
   real(rprec), allocatable :: f(:,:,:), dfdx(:,:,:), dfdy(:,:,:), dfdz(:,:,:)
+  real(rprec), allocatable, target :: fu(:,:,:)
+  real(rprec), allocatable :: spec(:,:)
   real(rprec) :: x, y, err, pi
   character(len=16) :: arg
 
@@ -61,7 +64,35 @@ program drop_in_demo
   if (rc /= LP_OK) stop 3
   err = maxval(abs(dfdz(1:nx, :, 2:nz-1) - 1.0_rprec/cfg%grid%dz))
   write (*, '(a,es10.3)') 'ddz_uv max error = ', err*cfg%grid%dz
-  rc = lp_destroy(ctx)
   if (err*cfg%grid%dz > 1e-12) stop 4
+  ! spectrum of u = sin(3x) cos(5y) + jz: the row mean jz gives E(k=0) = jz**2/2, the
+  ! sin(3x) wave of amplitude cos(5y) gives E(k=3) = <cos(5y)**2>/4 = 1/8, nothing else
+  allocate (fu(ld, ny, nz), spec(nx/2, nz-1))
+  fu = 0
+  do jz = 1, nz
+    do jy = 1, ny
+      do jx = 1, nx
+        x = (jx - 1)*cfg%grid%dx
+        y = (jy - 1)*cfg%grid%dy
+        fu(jx, jy, jz) = sin(3*x)*cos(5*y) + jz
+      end do
+    end do
+  end do
+  rc = lp_upload(ctx, LP_U, c_loc(fu), int(8, c_size_t)*ld*ny*nz)
+  if (rc /= LP_OK) stop 5
+  rc = lp_spectrum(ctx, LP_U, 0_c_int32_t, spec)
+  if (rc /= LP_OK) stop 6
+  err = 0
+  do jz = 1, nz - 1
+    do jx = 1, nx/2
+      x = 0
+      if (jx == 1) x = 0.5_rprec*jz*jz
+      if (jx == 4) x = 0.125_rprec
+      err = max(err, abs(spec(jx, jz) - x))
+    end do
+  end do
+  write (*, '(a,es10.3)') 'spectrum max error = ', err
+  rc = lp_destroy(ctx)
+  if (err > 1e-12) stop 7
   write (*, '(a)') 'drop-in demo ok'
 end program drop_in_demo
