@@ -35,6 +35,7 @@ template <typename T> struct LagPair {
   const LagIn<T> &in;
   const long np;
   const int k;
+  T cu, cv;  // per-level shifts of u, v on the w-node (zero in binary64)
   __device__ __forceinline__ static v2 ld2(const T *f, long o) { return *reinterpret_cast<const v2 *>(f + o); }
   __device__ __forceinline__ static v2 mul(v2 a, v2 b) { return v2{a.x * b.x, a.y * b.y}; }
   // branch-free (loads from clamped offsets, then a select) so that the loads of a row
@@ -44,6 +45,10 @@ template <typename T> struct LagPair {
     const v2 b = ld2(f, i - (k > 0 ? np : 0));
     const v2 h{T(0.5) * (a.x + b.x), T(0.5) * (a.y + b.y)};
     return k == 0 ? a : h;
+  }
+  __device__ __forceinline__ v2 hs(const T *f, T c, long i) const {  // hv(f, i) - c
+    const v2 a = hv(f, i);
+    return v2{a.x - c, a.y - c};
   }
   __device__ __forceinline__ v2 wn(long i) const {  // w; at jz = 1 a quarter of w(2)
     const v2 a = ld2(in.w, i + (k == 0 ? np : 0));
@@ -66,17 +71,26 @@ __device__ __forceinline__ void row_pairs(Fn f, cx<T> (&z)[Q], long base, int t)
 template <typename T, int Q, int P>
 __device__ __forceinline__ void lag_row(int q, const LagIn<T> &in, const Geo &g, int k, long base, int t,
                                         cx<T> (&z)[Q]) {
-  const LagPair<T> L{in, g.np, k};
+  // Binary32: u and v enter shifted by their w-node value at the level's first point.  Only
+  // L_ij = bar(u_i u_j) - bar(u_i) bar(u_j) reads fields 0-8, and it is unchanged by a
+  // constant shift (both test filters pass kx = ky = 0 with gain 1); without the mean flow
+  // in the products that difference no longer cancels |U|^2 / |u'|^2 of binary32 rounding.
+  // Binary64 keeps the reference's products (shift 0, so every value is unchanged).
+  LagPair<T> L{in, g.np, k, T(0), T(0)};
+  if constexpr (sizeof(T) == 4) {
+    L.cu = L.hv(in.u, (long)k * g.np).x;
+    L.cv = L.hv(in.v, (long)k * g.np).x;
+  }
   using LP = LagPair<T>;
   switch (q) {  // q is uniform over the wavefront
-    case 0: row_pairs<T, Q, P>([&](long i) { return L.hv(in.u, i); }, z, base, t); break;
-    case 1: row_pairs<T, Q, P>([&](long i) { return L.hv(in.v, i); }, z, base, t); break;
+    case 0: row_pairs<T, Q, P>([&](long i) { return L.hs(in.u, L.cu, i); }, z, base, t); break;
+    case 1: row_pairs<T, Q, P>([&](long i) { return L.hs(in.v, L.cv, i); }, z, base, t); break;
     case 2: row_pairs<T, Q, P>([&](long i) { return L.wn(i); }, z, base, t); break;
-    case 3: row_pairs<T, Q, P>([&](long i) { auto a = L.hv(in.u, i); return LP::mul(a, a); }, z, base, t); break;
-    case 4: row_pairs<T, Q, P>([&](long i) { return LP::mul(L.hv(in.u, i), L.hv(in.v, i)); }, z, base, t); break;
-    case 5: row_pairs<T, Q, P>([&](long i) { return LP::mul(L.hv(in.u, i), L.wn(i)); }, z, base, t); break;
-    case 6: row_pairs<T, Q, P>([&](long i) { auto a = L.hv(in.v, i); return LP::mul(a, a); }, z, base, t); break;
-    case 7: row_pairs<T, Q, P>([&](long i) { return LP::mul(L.hv(in.v, i), L.wn(i)); }, z, base, t); break;
+    case 3: row_pairs<T, Q, P>([&](long i) { auto a = L.hs(in.u, L.cu, i); return LP::mul(a, a); }, z, base, t); break;
+    case 4: row_pairs<T, Q, P>([&](long i) { return LP::mul(L.hs(in.u, L.cu, i), L.hs(in.v, L.cv, i)); }, z, base, t); break;
+    case 5: row_pairs<T, Q, P>([&](long i) { return LP::mul(L.hs(in.u, L.cu, i), L.wn(i)); }, z, base, t); break;
+    case 6: row_pairs<T, Q, P>([&](long i) { auto a = L.hs(in.v, L.cv, i); return LP::mul(a, a); }, z, base, t); break;
+    case 7: row_pairs<T, Q, P>([&](long i) { return LP::mul(L.hs(in.v, L.cv, i), L.wn(i)); }, z, base, t); break;
     case 8: row_pairs<T, Q, P>([&](long i) { auto a = L.wn(i); return LP::mul(a, a); }, z, base, t); break;
     default:
       if (q < 15) {

@@ -71,7 +71,9 @@ def test_mixed_routines():
         o.call("sgs_stag"); lg.sgs_stag()
         e = _errs(o, lg, ["Cs_opt2", "Nu_t", "txx", "txy", "txz", "tyy", "tyz", "tzz"], levels=lev)
         print(f"sgs_stag jt={jt}", {k: f"{v:.2e}" for k, v in e.items()})
-        assert max(e.values()) < 1e-3
+        # the dynamic model's binary32 Germano identity (mean flow shifted out of the
+        # products): measured 7e-6 on Cs_opt2 at jt = 2; 2.5e-4 with the mean left in
+        assert max(e.values()) < 1e-4
     o.call("convec"); lg.convec()
     e = _errs(o, lg, ["RHSx", "RHSy", "RHSz"], levels=lev)
     print("convec", {k: f"{v:.2e}" for k, v in e.items()})
