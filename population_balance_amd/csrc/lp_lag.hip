@@ -78,8 +78,10 @@ __device__ __forceinline__ void lag_row(int q, const LagIn<T> &in, const Geo &g,
   // Binary64 keeps the reference's products (shift 0, so every value is unchanged).
   LagPair<T> L{in, g.np, k, T(0), T(0)};
   if constexpr (sizeof(T) == 4) {
-    L.cu = L.hv(in.u, (long)k * g.np).x;
-    L.cv = L.hv(in.v, (long)k * g.np).x;
+    if (q <= 7 && q != 2) {  // the fields that read u or v
+      L.cu = L.hv(in.u, (long)k * g.np).x;
+      L.cv = L.hv(in.v, (long)k * g.np).x;
+    }
   }
   using LP = LagPair<T>;
   switch (q) {  // q is uniform over the wavefront
