@@ -485,22 +485,6 @@ def test_step_projected_field_is_divergence_free():
     assert np.all(o.field("w")[0] == 0) and np.all(o.field("w")[-1] == 0)
 
 
-def test_step_euler_start_and_ab2():
-    """jt = 1 without initu: RHS_f = RHS (Euler); the next step uses 1.5 RHS - 0.5 RHS_f
-    with RHS_f = the previous step's RHS including -grad p (main.f90:252-254,577-584,716-728)."""
-    cfg = abi.make_config(32, 32, 9)
-    o = Oracle(cfg)
-    g = cfg.grid
-    u, v, w = ic.turbulent_ic(g)
-    o.set_field("u", u); o.set_field("v", v); o.set_field("w", w)
-    o.step(1)
-    rhs1 = o.field("RHSx").copy()
-    assert np.array_equal(o.field("RHSx_f")[:-1, :, :g.nx] + 0, o.field("RHSx_f")[:-1, :, :g.nx])
-    o.step(1)
-    assert np.array_equal(o.field("RHSx_f")[:-1], rhs1[:-1])
-    assert o.time() == (3, 3)
-
-
 def test_timestep_conditions_closed_form():
     cfg = abi.make_config(32, 32, 9)
     o = Oracle(cfg)
