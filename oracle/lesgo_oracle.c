@@ -136,6 +136,9 @@ struct or_ctx {
   double *big[7];
   /* droplet bins (copied from the configuration) */
   double *diam, *ws, *qsrc;
+  /* frozen-coefficient steps (or_set_frozen): Nu_t (volume) and the wall u* (plane) of a
+   * base run, used instead of the SGS model and the wall law's |U| */
+  const double *frz_nut, *frz_ust;
 };
 
 static size_t plane(const or_ctx *c) { return (size_t)c->g.ld * c->g.ny; }
@@ -582,6 +585,22 @@ int or_wallstress(or_ctx *c) {
   const double psi_m = 0.0, phi_m = 1.0;
   double zl = 0.5 * g->dz - cf->d0;
   double denom = log(zl / cf->z0) - psi_m;                                  /* :143 */
+  if (c->frz_ust) {
+    /* frozen coefficient: txz = -(ustar^2 / |U|) u1 with ustar^2 / |U| = ustar kappa / denom
+     * of the base run; dudz = ustar / (zl kappa) u / |U| = u / (zl denom) */
+    for (int y = 0; y < g->ny; ++y)
+      for (int x = 0; x < g->nx; ++x) {
+        size_t i = (size_t)y * g->ld + x;
+        double q = -c->frz_ust[i] * cf->vonk / denom;
+        txz[i] = q * u1[i];
+        tyz[i] = q * v1[i];
+        dudz[i] = u[i] / (zl * denom);
+        dvdz[i] = v[i] / (zl * denom);
+        ust[i] = c->frz_ust[i];
+      }
+    free(u1); free(v1);
+    return 0;
+  }
   for (int y = 0; y < g->ny; ++y)
     for (int x = 0; x < g->nx; ++x) {
       size_t i = (size_t)y * g->ld + x;
@@ -898,6 +917,11 @@ int or_sgs_stag(or_ctx *c) {
         S[3][i] = vy; S[4][i] = 0.5 * (vz + wy); S[5][i] = wz;
       }
   double *lz = (double *)malloc(sizeof(double) * nz);
+  if (c->frz_nut) {
+    /* frozen coefficient: Nu_t of the base run (no model update, no lag accumulation) */
+    for (size_t i = 0; i < nv; ++i) Nu_t[i] = c->frz_nut[i];
+    goto stresses;
+  }
   if (cf->model == 4 || cf->model == 5) {                                     /* :130-134 */
     double *ul = c->f[LP_U_LAG], *vl = c->f[LP_V_LAG], *wl = c->f[LP_W_LAG];
     for (size_t i = 0; i < nv; ++i) { ul[i] += u[i]; vl[i] += v[i]; wl[i] += w[i]; }
@@ -934,6 +958,7 @@ int or_sgs_stag(or_ctx *c) {
         dissip[i] = -(s * s * s) * Cs[i] * l2;
         magS[i] = s;
       }
+stresses:
   /* :254-274 wall level */
   for (int y = 0; y < ny; ++y)
     for (int x = 0; x < nx; ++x) {
@@ -1875,5 +1900,438 @@ int or_step(or_ctx *c, int nsteps) {
     c->jt += 1;
     c->jt_total += 1;
   }
+  return 0;
+}
+
+/* ========================================================================= */
+/* Discrete adjoint of the frozen-coefficient step (test infrastructure: the */
+/* CPU restatement the HIP adjoint is checked against).                       */
+/*                                                                           */
+/* The frozen-coefficient step is step_once with Nu_t and the wall u* taken   */
+/* from a base run (or_set_frozen) instead of the SGS model and the wall law: */
+/* the map from (u, v, w, RHS_f) to (u, v, w, RHS) is then polynomial, and    */
+/* or_step_adjoint applies the transpose of its linearisation at the state    */
+/* held in the context.  Every transform below is the transpose of the        */
+/* forward one: with the FFTW2 conventions (unnormalised, c2r using only the  */
+/* real parts of the kx = 0 and kx = nx/2 terms) r2c^T = c2r after halving the */
+/* interior kx columns, c2r^T = r2c followed by doubling them, the forward y  */
+/* DFT's adjoint is the unnormalised inverse; the spectral derivatives with   */
+/* the Nyquist row and column removed are antisymmetric, the filters and the  */
+/* Nyquist removal symmetric; padding and truncation to and from the 3/2 grid */
+/* are each other's transposes up to (nx2 ny2)/(nx ny).                       */
+/* ========================================================================= */
+void or_set_frozen(or_ctx *c, const double *nut, const double *ust) {
+  c->frz_nut = nut;
+  c->frz_ust = ust;
+}
+
+/* S <- W rfft2_fwd(f): the transpose of rfft2_inv (interior kx columns doubled) */
+static void adj_c2r(or_ctx *c, const double *f, cplx *S) {
+  const lp_grid *g = &c->g;
+  or_rfft2_fwd(c, 0, f, g->ld, (double *)S);
+  for (int y = 0; y < g->ny; ++y)
+    for (int k = 1; k < g->lh - 1; ++k) S[(size_t)y * g->lh + k] *= 2.0;
+}
+/* f <- rfft2_inv(S / W): the transpose of rfft2_fwd (S is modified) */
+static void adj_r2c(or_ctx *c, cplx *S, double *f) {
+  const lp_grid *g = &c->g;
+  for (int y = 0; y < g->ny; ++y)
+    for (int k = 1; k < g->lh - 1; ++k) S[(size_t)y * g->lh + k] *= 0.5;
+  or_rfft2_inv(c, 0, (const double *)S, f, g->ld);
+}
+
+/* adjoint of one frozen-coefficient step (main.f90:252-826 with the frozen Nu_t, u*).
+ * Base: the context's u, v, w before the step and its jt (jt == 1 without initu: the
+ * Euler start).  ub: 3 volumes, on entry the adjoint of the step's u, v, w, on return
+ * that of the u, v, w it starts from; rb: 3 volumes, on entry the adjoint of the RHS the
+ * step leaves, on return that of the RHS_f it reads.  The context's fields are used as
+ * scratch (their values are undefined afterwards); droplets and the inflow are not
+ * part of it. */
+int or_step_adjoint(or_ctx *c, const double *nut, const double *ust, double *ub, double *rb) {
+  const lp_grid *g = &c->g;
+  const lp_config *cf = &c->cfg;
+  const int nx = g->nx, ny = g->ny, nz = g->nz, ld = g->ld, lh = g->lh;
+  const size_t np = plane(c), nv = vol(c), nh = (size_t)lh * ny;
+  const size_t npb = (size_t)g->ld_big * g->ny2;
+  const double dz = g->dz, dt = cf->dt, cst = 1.0 / ((double)nx * ny);
+  const int first = c->jt == 1 && !cf->initu;
+  const int wall = cf->lbc_mom == LP_LBC_WALL;
+  const double zl = 0.5 * dz - cf->d0, den = log(zl / cf->z0);
+#define NEWV(n) double *n = (double *)calloc(nv, sizeof(double))
+#define X_ALL(k, y, x) for (int k = 0; k < nz; ++k) for (int y = 0; y < ny; ++y) for (int x = 0; x < nx; ++x)
+#define IX(k, y, x) ((size_t)(k) * np + (size_t)(y) * ld + (x))
+  /* ---- base intermediates of the step from u, v, w ---- */
+  NEWV(uf); NEWV(vf); NEWV(wf);
+  memcpy(uf, c->f[LP_U], nv * sizeof(double));
+  memcpy(vf, c->f[LP_V], nv * sizeof(double));
+  memcpy(wf, c->f[LP_W], nv * sizeof(double));
+  NEWV(ux); NEWV(uy); NEWV(vx); NEWV(vy); NEWV(wx); NEWV(wy); NEWV(uz); NEWV(vz); NEWV(wz);
+  or_filt_da(c, uf, ux, uy, nz);
+  or_filt_da(c, vf, vx, vy, nz);
+  or_filt_da(c, wf, wx, wy, nz);
+  or_ddz_uv(c, uz, uf);
+  or_ddz_uv(c, vz, vf);
+  or_ddz_w(c, wz, wf);
+  for (int y = 0; y < ny; ++y)
+    for (int x = 0; x < nx; ++x) {
+      size_t i = (size_t)y * ld + x;
+      uz[i] = wall ? uf[i] / (zl * den) : 0.0;
+      vz[i] = wall ? vf[i] / (zl * den) : 0.0;
+    }
+  NEWV(om1); NEWV(om2); NEWV(om3);
+  X_ALL(k, y, x) {
+    size_t i = IX(k, y, x);
+    if (k == 0) {
+      om1[i] = 0.5 * (wy[i] + wy[i + np]) - vz[i];
+      om2[i] = uz[i] - 0.5 * (wx[i] + wx[i + np]);
+    } else {
+      om1[i] = wy[i] - vz[i];
+      om2[i] = uz[i] - wx[i];
+    }
+    om3[i] = vx[i] - uy[i];
+  }
+  double *U[3], *O[3], *aU[3], *aO[3];
+  const double *srcU[3] = {uf, vf, wf}, *srcO[3] = {om1, om2, om3};
+  for (int q = 0; q < 3; ++q) {
+    U[q] = (double *)calloc(npb * nz, sizeof(double));
+    O[q] = (double *)calloc(npb * nz, sizeof(double));
+    aU[q] = (double *)calloc(npb * nz, sizeof(double));
+    aO[q] = (double *)calloc(npb * nz, sizeof(double));
+    or_dealias1(c, srcU[q], U[q], nz);
+    or_dealias1(c, srcO[q], O[q], nz);
+  }
+  /* ---- adjoint ---- */
+  double *au = ub, *av = ub + nv, *aw = ub + 2 * nv;
+  double *rbx = rb, *rby = rb + nv, *rbz = rb + 2 * nv;
+  NEWV(auf); NEWV(avf); NEWV(awf);
+  /* forcing.f90:215-292 boundary values */
+  for (size_t i = 0; i < np; ++i) {
+    size_t top = (size_t)(nz - 1) * np + i, bel = (size_t)(nz - 2) * np + i;
+    aw[i] = 0.0;
+    aw[top] = 0.0;
+    if (cf->lbc_mom == LP_LBC_STRESS_FREE) {
+      au[np + i] += au[i]; au[i] = 0.0;
+      av[np + i] += av[i]; av[i] = 0.0;
+    }
+    if (cf->use_mean_p_force) {
+      au[bel] += au[top]; au[top] = 0.0;
+      av[bel] += av[top]; av[top] = 0.0;
+    } else {  /* the top level is filt_da's value */
+      auf[top] += au[top]; au[top] = 0.0;
+      avf[top] += av[top]; av[top] = 0.0;
+    }
+  }
+  /* projection (forcing.f90:145-191) and RHS -= grad p (main.f90:795-803) */
+  NEWV(aus); NEWV(avs); NEWV(aws); NEWV(apx); NEWV(apy); NEWV(apz); NEWV(arx); NEWV(ary); NEWV(arz);
+  for (int k = 0; k < nz - 1; ++k)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        size_t i = IX(k, y, x);
+        aus[i] = au[i]; apx[i] = -1.5 * dt * au[i] - rbx[i]; arx[i] = rbx[i];
+        avs[i] = av[i]; apy[i] = -1.5 * dt * av[i] - rby[i]; ary[i] = rby[i];
+        if (k >= 1) { aws[i] = aw[i]; apz[i] = -1.5 * dt * aw[i]; }
+        apz[i] -= rbz[i];
+        arz[i] = rbz[i];
+      }
+  /* dpdz = (p(k+1) - p(k))/dz (main.f90:789-791) */
+  double *ap = (double *)calloc(np * (nz + 1), sizeof(double));
+  for (int k = 0; k < nz - 1; ++k)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        size_t i = IX(k, y, x);
+        ap[i + np] += apz[i] / dz;
+        ap[i] -= apz[i] / dz;
+      }
+  /* p, dpdx, dpdy from p_hat (press_stag_array.f90:339-352) */
+  cplx *aph = (cplx *)calloc(nh * (nz + 1), sizeof(cplx));
+  {
+    cplx *S = (cplx *)malloc(sizeof(cplx) * nh);
+    for (int j = 0; j < nz; ++j) {
+      adj_c2r(c, ap + (size_t)j * np, S);
+      for (size_t m = 0; m < nh; ++m) aph[(size_t)j * nh + m] += S[m];
+      if (j >= 1) {
+        adj_c2r(c, apx + (size_t)(j - 1) * np, S);
+        for (size_t m = 0; m < nh; ++m) aph[(size_t)j * nh + m] += -I * c->kx[m] * S[m];
+        adj_c2r(c, apy + (size_t)(j - 1) * np, S);
+        for (size_t m = 0; m < nh; ++m) aph[(size_t)j * nh + m] += -I * c->ky[m] * S[m];
+      }
+    }
+    free(S);
+  }
+  /* the solve (press_stag_array.f90:149-329, tridag_array.f90): transposed systems */
+  cplx *aH[3];
+  for (int q = 0; q < 3; ++q) aH[q] = (cplx *)calloc(nh * (nz + 1), sizeof(cplx));
+  cplx *abw = (cplx *)calloc(nh, sizeof(cplx)), *atw = (cplx *)calloc(nh, sizeof(cplx));
+  {
+    const double idz2 = 1.0 / (dz * dz);
+    const int n = nz + 1;
+    double *ga = (double *)malloc(sizeof(double) * n), *gb = (double *)malloc(sizeof(double) * n);
+    double *gc = (double *)malloc(sizeof(double) * n), *gam = (double *)malloc(sizeof(double) * n);
+    cplx *lam = (cplx *)malloc(sizeof(cplx) * n);
+    for (size_t m = 0; m < nh; ++m) {
+      int jx = (int)(m % lh), jy = (int)(m / lh);
+      if (jy == ny / 2 || jx == lh - 1) continue;
+      if (m == 0) { /* mean mode :303-312 */
+        cplx acc = 0.0;
+        for (int j = nz; j >= 2; --j) {
+          acc += aph[(size_t)j * nh];
+          if (j <= nz - 1) aH[2][(size_t)j * nh] += dz * acc;
+        }
+        acc += aph[nh];
+        abw[0] += -dz * acc;
+        continue;
+      }
+      double bi = -(c->kx[m] * c->kx[m] + c->ky[m] * c->ky[m] + 2.0 / (dz * dz));
+      /* A: row 0 (b,c) = (-1, 1); rows 1..nz-1 (idz2, bi, idz2); row nz (a,b) = (-1, 1).
+       * A^T: a'_j = c_{j-1}, b'_j = b_j, c'_j = a_{j+1} */
+      for (int j = 0; j < n; ++j) {
+        double aj = (j == 0) ? 0.0 : (j == nz ? -1.0 : idz2);
+        double bj = (j == 0) ? -1.0 : (j == nz ? 1.0 : bi);
+        double cj = (j == 0) ? 1.0 : (j == nz ? 0.0 : idz2);
+        (void)aj; (void)cj;
+        gb[j] = bj;
+      }
+      for (int j = 0; j < n; ++j) {
+        double cprev = (j == 0) ? 0.0 : (j - 1 == 0 ? 1.0 : idz2);     /* c_{j-1} */
+        double anext = (j == nz) ? 0.0 : (j + 1 == nz ? -1.0 : idz2);  /* a_{j+1} */
+        ga[j] = cprev;
+        gc[j] = anext;
+      }
+      double bet = gb[0];
+      lam[0] = aph[m] / bet;
+      for (int j = 1; j < n; ++j) {
+        gam[j] = gc[j - 1] / bet;
+        bet = gb[j] - ga[j] * gam[j];
+        lam[j] = (aph[(size_t)j * nh + m] - ga[j] * lam[j - 1]) / bet;
+      }
+      for (int j = n - 2; j >= 0; --j) lam[j] -= gam[j + 1] * lam[j + 1];
+      abw[m] += -dz * lam[0];
+      atw[m] += -dz * lam[nz];
+      for (int j = 1; j <= nz - 1; ++j) {
+        aH[0][(size_t)j * nh + m] += -I * c->kx[m] * lam[j];
+        aH[1][(size_t)j * nh + m] += -I * c->ky[m] * lam[j];
+        aH[2][(size_t)j * nh + m] -= lam[j] / dz;
+        if (j + 1 <= nz - 1) aH[2][(size_t)(j + 1) * nh + m] += lam[j] / dz;
+      }
+    }
+    free(ga); free(gb); free(gc); free(gam); free(lam);
+  }
+  kill_oddballs(abw, lh, ny);
+  kill_oddballs(atw, lh, ny);
+  NEWV(adz);  /* adjoint of divtz */
+  {
+    cplx *S = (cplx *)malloc(sizeof(cplx) * nh);
+    double *f = (double *)malloc(sizeof(double) * np);
+    double *dst[3] = {aus, avs, aws};
+    for (int q = 0; q < 3; ++q)
+      for (int k = 0; k < nz - 1; ++k) {  /* H level k+1 from level k */
+        memcpy(S, aH[q] + (size_t)(k + 1) * nh, sizeof(cplx) * nh);
+        kill_oddballs(S, lh, ny);
+        adj_r2c(c, S, f);
+        for (size_t i = 0; i < np; ++i) dst[q][(size_t)k * np + i] += cst / (1.5 * dt) * f[i];
+      }
+    memcpy(S, abw, sizeof(cplx) * nh);
+    adj_r2c(c, S, f);
+    for (size_t i = 0; i < np; ++i) adz[i] += cst * f[i];
+    memcpy(S, atw, sizeof(cplx) * nh);
+    adj_r2c(c, S, f);
+    for (size_t i = 0; i < np; ++i) adz[(size_t)(nz - 1) * np + i] += cst * f[i];
+    free(S); free(f);
+  }
+  /* AB2 predictor (main.f90:716-728), RHS assembly (:433-439), Euler start (:577-584) */
+  for (int k = 0; k < nz - 1; ++k)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        size_t i = IX(k, y, x);
+        const double tr = first ? 1.0 : 1.5;
+        auf[i] += aus[i]; avf[i] += avs[i]; awf[i] += aws[i];
+        arx[i] += tr * dt * aus[i]; ary[i] += tr * dt * avs[i]; arz[i] += tr * dt * aws[i];
+        rbx[i] = first ? 0.0 : -0.5 * dt * aus[i];
+        rby[i] = first ? 0.0 : -0.5 * dt * avs[i];
+        rbz[i] = first ? 0.0 : -0.5 * dt * aws[i];
+      }
+  for (int q = 0; q < 3; ++q)
+    for (size_t i = (size_t)(nz - 1) * np; i < nv; ++i) rb[(size_t)q * nv + i] = 0.0;
+  NEWV(acx); NEWV(acy); NEWV(acz); NEWV(adx); NEWV(ady);
+  for (int k = 0; k < nz - 1; ++k)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        size_t i = IX(k, y, x);
+        acx[i] = -arx[i]; adx[i] = -arx[i];
+        acy[i] = -ary[i]; ady[i] = -ary[i];
+        if (k >= 1) acz[i] = -arz[i];
+        adz[i] += -arz[i];
+      }
+  /* convec.f90: products on the 3/2 grid (transposes of the padding and truncation) */
+  NEWV(aux); NEWV(auy); NEWV(auz); NEWV(avx); NEWV(avy); NEWV(avz); NEWV(awx); NEWV(awy); NEWV(awz);
+  {
+    const double tt = ((double)nx * ny) / ((double)g->nx2 * g->ny2);  /* T^T = tt P */
+    double *pb[3];
+    const double *ac[3] = {acx, acy, acz};
+    for (int q = 0; q < 3; ++q) {
+      pb[q] = (double *)calloc(npb * nz, sizeof(double));
+      or_dealias1(c, ac[q], pb[q], nz);
+      for (size_t i = 0; i < npb * nz; ++i) pb[q][i] *= tt;
+    }
+    for (int k = 0; k < nz - 1; ++k)
+      for (size_t i = 0; i < npb; ++i) {
+        size_t a = (size_t)k * npb + i, b = a + npb, m = a - npb;
+        double px = pb[0][a], py = pb[1][a], pz = pb[2][a];
+        /* px = -U2 O3 + (U3(k+1) O2(k+1) + [k>=1] U3 O2)/2 */
+        aU[1][a] += -O[2][a] * px; aO[2][a] += -U[1][a] * px;
+        aU[2][b] += 0.5 * O[1][b] * px; aO[1][b] += 0.5 * U[2][b] * px;
+        /* py = U1 O3 - (U3(k+1) O1(k+1) + [k>=1] U3 O1)/2 */
+        aU[0][a] += O[2][a] * py; aO[2][a] += U[0][a] * py;
+        aU[2][b] += -0.5 * O[0][b] * py; aO[0][b] += -0.5 * U[2][b] * py;
+        if (k >= 1) {
+          aU[2][a] += 0.5 * O[1][a] * px; aO[1][a] += 0.5 * U[2][a] * px;
+          aU[2][a] += -0.5 * O[0][a] * py; aO[0][a] += -0.5 * U[2][a] * py;
+          /* pz = (-(U1 + U1(k-1)) O2 + (U2 + U2(k-1)) O1)/2 */
+          aU[0][a] += -0.5 * O[1][a] * pz; aU[0][m] += -0.5 * O[1][a] * pz;
+          aO[1][a] += -0.5 * (U[0][a] + U[0][m]) * pz;
+          aU[1][a] += 0.5 * O[0][a] * pz; aU[1][m] += 0.5 * O[0][a] * pz;
+          aO[0][a] += 0.5 * (U[1][a] + U[1][m]) * pz;
+        }
+      }
+    for (int q = 0; q < 3; ++q) free(pb[q]);
+    const double pt = ((double)g->nx2 * g->ny2) / ((double)nx * ny);  /* P^T = pt T */
+    double *aUf[3] = {auf, avf, awf};
+    NEWV(tmp);
+    NEWV(ao1); NEWV(ao2); NEWV(ao3);
+    double *aom[3] = {ao1, ao2, ao3};
+    for (int q = 0; q < 3; ++q) {
+      or_dealias2(c, tmp, aU[q], nz);
+      for (size_t i = 0; i < nv; ++i) aUf[q][i] += pt * tmp[i];
+      or_dealias2(c, aom[q], aO[q], nz);
+      for (size_t i = 0; i < nv; ++i) aom[q][i] *= pt;
+    }
+    free(tmp);
+    /* vorticity (convec.f90:100-133; jz = 1 on the uvp node) */
+    X_ALL(k, y, x) {
+      size_t i = IX(k, y, x);
+      if (k == 0) {
+        awy[i] += 0.5 * ao1[i]; awy[i + np] += 0.5 * ao1[i];
+        awx[i] -= 0.5 * ao2[i]; awx[i + np] -= 0.5 * ao2[i];
+      } else {
+        awy[i] += ao1[i];
+        awx[i] -= ao2[i];
+      }
+      avz[i] -= ao1[i];
+      auz[i] += ao2[i];
+      avx[i] += ao3[i];
+      auy[i] -= ao3[i];
+    }
+    free(ao1); free(ao2); free(ao3);
+  }
+  /* divstress_uv / divstress_w: the x, y derivatives are antisymmetric */
+  NEWV(atxx); NEWV(atxy); NEWV(atxz); NEWV(atyy); NEWV(atyz); NEWV(atzz);
+  {
+    NEWV(t1);
+    or_ddx(c, t1, adx, nz); for (size_t i = 0; i < nv; ++i) atxx[i] -= t1[i];
+    or_ddy(c, t1, adx, nz); for (size_t i = 0; i < nv; ++i) atxy[i] -= t1[i];
+    or_ddx(c, t1, ady, nz); for (size_t i = 0; i < nv; ++i) atxy[i] -= t1[i];
+    or_ddy(c, t1, ady, nz); for (size_t i = 0; i < nv; ++i) atyy[i] -= t1[i];
+    or_ddx(c, t1, adz, nz); for (size_t i = 0; i < nv; ++i) atxz[i] -= t1[i];
+    or_ddy(c, t1, adz, nz); for (size_t i = 0; i < nv; ++i) atyz[i] -= t1[i];
+    free(t1);
+    for (int k = 0; k < nz - 1; ++k)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          size_t i = IX(k, y, x);
+          atxz[i + np] += adx[i] / dz; atxz[i] -= adx[i] / dz;
+          atyz[i + np] += ady[i] / dz; atyz[i] -= ady[i] / dz;
+          if (k >= 1) { atzz[i] += adz[i] / dz; atzz[i - np] -= adz[i] / dz; }
+        }
+  }
+  /* stresses with the frozen Nu_t (sgs_stag.f90:254-325) */
+  NEWV(aS11); NEWV(aS12); NEWV(aS13); NEWV(aS22); NEWV(aS23); NEWV(aS33);
+  X_ALL(k, y, x) {
+    size_t i = IX(k, y, x);
+    if (k == 0) {
+      aS11[i] += -2.0 * nut[i] * atxx[i]; aS12[i] += -2.0 * nut[i] * atxy[i];
+      aS22[i] += -2.0 * nut[i] * atyy[i]; aS33[i] += -2.0 * nut[i] * atzz[i];
+    } else if (k <= nz - 2) {
+      double cc = 0.5 * (nut[i] + nut[i + np]);
+      aS11[i] += -cc * atxx[i]; aS11[i + np] += -cc * atxx[i];
+      aS12[i] += -cc * atxy[i]; aS12[i + np] += -cc * atxy[i];
+      aS22[i] += -cc * atyy[i]; aS22[i + np] += -cc * atyy[i];
+      aS33[i] += -cc * atzz[i]; aS33[i + np] += -cc * atzz[i];
+      aS13[i] += -2.0 * nut[i] * atxz[i];
+      aS23[i] += -2.0 * nut[i] * atyz[i];
+    }
+  }
+  /* strain rates (sgs_stag.f90:49-125) */
+  X_ALL(k, y, x) {
+    size_t i = IX(k, y, x), mm = i - np;
+    if (k == 0) {
+      aux[i] += aS11[i];
+      auy[i] += 0.5 * aS12[i]; avx[i] += 0.5 * aS12[i];
+      auz[i] += 0.5 * aS13[i]; awx[i] += 0.25 * aS13[i]; awx[i + np] += 0.25 * aS13[i];
+      avy[i] += aS22[i];
+      avz[i] += 0.5 * aS23[i]; awy[i] += 0.25 * aS23[i]; awy[i + np] += 0.25 * aS23[i];
+      awz[i] += aS33[i];
+    } else {
+      aux[i] += 0.5 * aS11[i]; aux[mm] += 0.5 * aS11[i];
+      auy[i] += 0.25 * aS12[i]; auy[mm] += 0.25 * aS12[i];
+      avx[i] += 0.25 * aS12[i]; avx[mm] += 0.25 * aS12[i];
+      auz[i] += 0.5 * aS13[i]; awx[i] += 0.5 * aS13[i];
+      avy[i] += 0.5 * aS22[i]; avy[mm] += 0.5 * aS22[i];
+      avz[i] += 0.5 * aS23[i]; awy[i] += 0.5 * aS23[i];
+      awz[i] += 0.5 * aS33[i]; awz[mm] += 0.5 * aS33[i];
+    }
+  }
+  /* z derivatives (ddz_uv, ddz_w) and the wall level (wallstress.f90 with frozen u*) */
+  for (int k = 0; k < nz - 1; ++k)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        size_t i = IX(k, y, x);
+        if (k >= 1) {
+          auf[i] += auz[i] / dz; auf[i - np] -= auz[i] / dz;
+          avf[i] += avz[i] / dz; avf[i - np] -= avz[i] / dz;
+        }
+        awf[i + np] += awz[i] / dz; awf[i] -= awz[i] / dz;
+      }
+  if (wall) {
+    double *qu = (double *)calloc(np, sizeof(double)), *qv = (double *)calloc(np, sizeof(double));
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        size_t i = (size_t)y * ld + x;
+        double q = -ust[i] * cf->vonk / den;
+        auf[i] += auz[i] / (zl * den);
+        avf[i] += avz[i] / (zl * den);
+        qu[i] = q * atxz[i];
+        qv[i] = q * atyz[i];
+      }
+    test_filter_plane(c, qu, c->G_test);
+    test_filter_plane(c, qv, c->G_test);
+    for (size_t i = 0; i < np; ++i) { auf[i] += qu[i]; avf[i] += qv[i]; }
+    free(qu); free(qv);
+  }
+  /* filt_da: u -> (F u, Dx F u, Dy F u); transpose F(au) - Dx(aux) - Dy(auy) */
+  {
+    double *af[3] = {auf, avf, awf}, *ax[3] = {aux, avx, awx}, *ay[3] = {auy, avy, awy};
+    NEWV(t1); NEWV(t2);
+    for (int q = 0; q < 3; ++q) {
+      double *out = ub + (size_t)q * nv;
+      memcpy(out, af[q], nv * sizeof(double));
+      or_filt_da(c, out, t1, t2, nz);
+      or_ddx(c, t1, ax[q], nz);
+      or_ddy(c, t2, ay[q], nz);
+      for (size_t i = 0; i < nv; ++i) out[i] -= t1[i] + t2[i];
+      for (int k = 0; k < nz; ++k)
+        for (int y = 0; y < ny; ++y)
+          for (int x = nx; x < ld; ++x) out[IX(k, y, x)] = 0.0;
+    }
+    free(t1); free(t2);
+  }
+  double *fr[] = {uf, vf, wf, ux, uy, vx, vy, wx, wy, uz, vz, wz, om1, om2, om3, auf, avf, awf, aus, avs, aws,
+                  apx, apy, apz, arx, ary, arz, ap, adz, acx, acy, acz, adx, ady, aux, auy, auz, avx, avy, avz,
+                  awx, awy, awz, atxx, atxy, atxz, atyy, atyz, atzz, aS11, aS12, aS13, aS22, aS23, aS33};
+  for (size_t q = 0; q < sizeof fr / sizeof fr[0]; ++q) free(fr[q]);
+  for (int q = 0; q < 3; ++q) { free(U[q]); free(O[q]); free(aU[q]); free(aO[q]); free(aH[q]); }
+  free(aph); free(abw); free(atw);
+#undef NEWV
+#undef X_ALL
+#undef IX
   return 0;
 }

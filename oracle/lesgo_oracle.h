@@ -71,6 +71,9 @@ int or_calcbeta_pcon(or_ctx *c);
 int or_step_pollen2(or_ctx *c);
 void or_ke_dudt(or_ctx *c);
 void or_calculate_probability(const double *d, int n, double sigma, double d_min, double *beta);
+/* frozen-coefficient steps and their discrete adjoint */
+void or_set_frozen(or_ctx *c, const double *nut, const double *ust);
+int or_step_adjoint(or_ctx *c, const double *nut, const double *ust, double *ub, double *rb);
 
 #ifdef __cplusplus
 }
