@@ -123,25 +123,87 @@ def droplet_leg(steps, warmup, seed, nx=192, nz=128, nbins=15):
     return out
 
 
-def cpu_baseline(nx, seed):
+def adjoint_leg(seed, nx=128, nz=65, nsteps=20, every=5):
+    """gradient of a 20-step binary32 run at 128x128x65 (checkpoints every 5 steps) with
+    respect to its initial velocity, by the binary64 discrete adjoint of the
+    frozen-coefficient step; objective: u summed over a 4^3 probe box at every step.
+    Timed after one untimed forward + gradient."""
+    import torch
+    from population_balance_amd import adjoint
+    cfg = abi.make_config(nx, nx, nz, precision=abi.LP_MIXED, use_mean_p_force=1, mean_p_force=1.0)
+    g = cfg.grid
+    u0, v0, w0 = (torch.from_numpy(a).to("cuda") for a in ic.turbulent_ic(g, seed=seed))
+    probe = adjoint.Probe(0, k0=nz // 4, j0=g.ny // 2, i0=3 * g.nx // 4)
+    run = adjoint.CheckpointedRun(cfg, nsteps, probe, every=every)
+    run.forward(u0, v0, w0)
+    run.gradient()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run.forward(u0, v0, w0)
+    run.gradient()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return {"workload": f"gradient of a {nsteps}-step run {nx}x{nx}x{nz} (mixed forward, binary64 adjoint), "
+                        f"checkpoints every {every} steps",
+            "ms_per_adjoint_step": round(run.t_adjoint / run.adjoint_steps, 4),
+            "forward_ms_per_step": round(run.t_forward / nsteps, 4),
+            "gradient_wall_s": round(wall, 3),
+            "adjoint_steps": run.adjoint_steps}
+
+
+def host_cpu():
+    """the host's CPU as lscpu reports it, and the cores this process may use"""
+    info = {}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[k] = v
+    except Exception as e:  # lscpu missing: report what os knows
+        info["lscpu"] = f"unavailable ({e})"
+    try:
+        info["usable_cpus"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        info["usable_cpus"] = os.cpu_count()
+    return info
+
+
+def _oracle_rate(nx, nz, seed, threads, nsteps):
     from oracle import Oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
-    nz = 33
     cfg = abi.make_config(nx, nx, nz, use_mean_p_force=1, mean_p_force=1.0)
     u, v, w = ic.turbulent_ic(cfg.grid, seed=seed)
     o = Oracle(cfg, threads=threads)
     o.set_field("u", u); o.set_field("v", v); o.set_field("w", w)
     o.step(1)                      # jt = 1 (constant Cs^2)
-    nsteps = 5                     # about 10 s of wall time on 16 host cores
     t0 = time.perf_counter()
-    o.step(nsteps)                 # jt = 2 .. 6 (Lagrangian model active)
+    o.step(nsteps)                 # jt = 2 .. (Lagrangian model active)
     dt = time.perf_counter() - t0
-    pts = nx * nx * (nz - 1) * nsteps
     o.close()
-    return {"value": pts / dt / 1e6, "unit": "Mpt-steps/s", "cores": threads, "kind": "port",
+    return nx * nx * (nz - 1) * nsteps / dt / 1e6, dt
+
+
+def cpu_baseline(nx, seed):
+    """The binary64 oracle on the host: every core this process may use (OMP_NUM_THREADS
+    when set, as the GPU host's job share sets it, else the affinity mask; one OpenMP
+    thread per core, bound to cores), and one core, on a slab of the same workload."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    cpu = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or int(cpu.get("usable_cpus") or 1)
+    nz = 33
+    nsteps = 5                     # about 10 s of wall time on 16 host cores
+    val, dt = _oracle_rate(nx, nz, seed, threads, nsteps)
+    val1, dt1 = _oracle_rate(nx, nz, seed, 1, 1)  # one core, one step of the same slab
+    return {"value": val, "unit": "Mpt-steps/s", "cores": threads, "kind": "port",
             "sample": f"binary64 oracle (C restatement of the reference path), {nx}x{nx}x{nz} "
                       f"levels, steps 2-{1 + nsteps} (Lagrangian SGS active), {dt:.1f} s wall, "
-                      f"{threads} OpenMP threads"}
+                      f"{threads} OpenMP threads bound one per core",
+            "one_core": {"value": val1, "unit": "Mpt-steps/s", "cores": 1,
+                         "sample": f"the same slab, step 2 only, {dt1:.1f} s wall"},
+            "host": cpu}
 
 
 def measured_traffic(kernel_class, nx, nz, precision):
@@ -188,6 +250,9 @@ def main():
     ap.add_argument("--breakdown", action="store_true", help="per-kernel-class timing to stderr")
     ap.add_argument("--roofline-class", default="col_op", choices=list(abi.KERNEL_CLASSES))
     ap.add_argument("--copy-probe", action="store_true", help="only run a plain copy of known size")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the 128x128x65 adjoint gradient")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only the mixed step: warmup + steps, nothing else (a target for profilers)")
     a = ap.parse_args()
     if a.copy_probe:
         copy_probe()
@@ -205,6 +270,11 @@ def main():
     log(f"[bench] initial condition {nx}x{nx}x{nz} generated in {time.perf_counter() - t:.1f} s")
     pts = nx * nx * (nz - 1)
     cls = abi.KERNEL_CLASSES[a.roofline_class]
+    if a.profile_only:
+        lg_m, t_m, _ = run_gpu(cfg_m, u, v, w, a.steps, a.warmup)
+        lg_m.close()
+        log(f"[bench] profile target: {a.warmup} + {a.steps} mixed steps, {t_m / a.steps * 1e3:.3f} ms/step")
+        return
 
     lg_m, t_m, prof = run_gpu(cfg_m, u, v, w, a.steps, a.warmup, prof_class=cls)
     ms_m = t_m / a.steps * 1e3
@@ -287,6 +357,11 @@ def main():
         dl = droplet_leg(a.steps, a.warmup, a.seed)
         log(f"[bench] droplets: {dl['ms_per_step']:.3f} ms/step, {dl['value']:.1f} Mpt-steps/s")
         result["droplet_step"] = dl
+    if not a.no_adjoint:
+        ad = adjoint_leg(a.seed)
+        log(f"[bench] adjoint: {ad['ms_per_adjoint_step']:.3f} ms per adjoint step, gradient in "
+            f"{ad['gradient_wall_s']:.2f} s")
+        result["adjoint"] = ad
     if not a.no_cpu:
         log("[bench] CPU baseline (oracle) ...")
         cb = cpu_baseline(nx, a.seed)

@@ -15,7 +15,8 @@
  *   - "big" (3/2-padded) fields are (ld_big, ny2, nz).
  *   - spectra overlay the same memory as complex(lh,ny) (interleaved re/im).
  *   - element type: double for an LP_FP64 context, float for an LP_MIXED context
- *     (the exceptions, always double, are listed with the field ids below).
+ *     (the exceptions, always double, are listed with the field ids below).  The
+ *     explicit-array routines (lp_filt_da ... lp_divstress_w) take arrays of this type.
  *   - every array argument may be host memory or device memory: device pointers are
  *     used in place, host pointers are staged through the context (synchronous).
  *
@@ -40,9 +41,11 @@ enum lp_status {
   LP_ERR_ARG = 1,         /* bad argument / unsupported configuration            */
   LP_ERR_HIP = 2,         /* HIP runtime error                                   */
   LP_ERR_PIVOT = 3,       /* zero pivot in the tridiagonal solve                 */
-  LP_ERR_NAN = 4,         /* non-finite value detected by lp_check_finite        */
+  LP_ERR_NAN = 4,         /* non-finite value: a velocity written by lp_step, or
+                             any state field in lp_check_finite                 */
   LP_ERR_NOMEM = 5,       /* device allocation failed                            */
-  LP_ERR_NODEVICE = 6     /* no HIP device visible                               */
+  LP_ERR_NODEVICE = 6,    /* no HIP device visible                               */
+  LP_ERR_CFL = 7          /* CFL above the limit set by lp_set_cfl_limit         */
 };
 
 enum lp_precision {
@@ -121,9 +124,11 @@ typedef struct lp_config {
 } lp_config;
 
 /* State field ids.  Shapes are (ld,ny,nz) unless noted.  In an LP_MIXED context the
- * fields marked [f64] stay binary64, all others are binary32. */
+ * fields marked [f64] stay binary64, all others are binary32.  The velocities are
+ * binary64 in both: the binary32 transforms take u and v less a per-level value that is
+ * added back in binary64, so the state keeps the mean flow without binary32 rounding. */
 enum lp_field {
-  LP_U = 0, LP_V, LP_W,
+  LP_U = 0, LP_V, LP_W,          /* [f64]                                              */
   LP_DUDX, LP_DUDY, LP_DUDZ, LP_DVDX, LP_DVDY, LP_DVDZ, LP_DWDX, LP_DWDY, LP_DWDZ,
   LP_RHSX, LP_RHSY, LP_RHSZ, LP_RHSX_F, LP_RHSY_F, LP_RHSZ_F,
   LP_TXX, LP_TXY, LP_TXZ, LP_TYY, LP_TYZ, LP_TZZ,
@@ -170,6 +175,7 @@ int lp_sync(lp_ctx *ctx);
 /* ---------------- state access ---------------- */
 int lp_field_info(lp_ctx *ctx, int32_t field, void **dev_ptr, size_t *bytes,
                   int32_t *elem_size);
+/* copy a whole field from / to `host` (host or device memory; synchronous) */
 int lp_upload(lp_ctx *ctx, int32_t field, const void *host, size_t bytes);
 int lp_download(lp_ctx *ctx, int32_t field, void *host, size_t bytes);
 int lp_set_time(lp_ctx *ctx, int32_t jt, int32_t jt_total);
@@ -250,8 +256,37 @@ int lp_step_pollen2(lp_ctx *ctx);
  * = probability that a drop of bin j breaks into bin i (host routine, no device). */
 int lp_calculate_probability(const double *diameter, int32_t npcon, double sigma_s,
                              double d_min, double *beta);
-/* One or more device-resident time steps of the hot loop (main.f90:207-1004). */
+/* One or more device-resident time steps of the hot loop (main.f90:207-1004).
+ * Status, checked once per call after its last step (call it with nsteps = 1 to check
+ * every step): LP_ERR_PIVOT if a pressure solve met a zero pivot, LP_ERR_NAN if a step
+ * wrote a non-finite u, v or w, LP_ERR_CFL if a limit is set and the CFL of the final
+ * state (timestep_conditions) exceeds it.  The steps of the call have all run. */
 int lp_step(lp_ctx *ctx, int32_t nsteps);
+/* The reference reports CFL and divergence every wbase steps (main.f90:902-926) and does
+ * not stop on them; with cfl_max > 0 lp_step returns LP_ERR_CFL above it (0: off). */
+int lp_set_cfl_limit(lp_ctx *ctx, double cfl_max);
+/* Scan u, v, w, RHS, RHS_f, p, F_LM/MM/QN/NN, Cs_opt2 (and the droplet state) for
+ * non-finite values: LP_ERR_NAN naming the first such field, else LP_OK. */
+int lp_check_finite(lp_ctx *ctx);
+/* The context's enum lp_precision.  The explicit-array routines take arrays of the
+ * context's field type: binary64 for LP_FP64, binary32 for LP_MIXED. */
+int lp_get_precision(lp_ctx *ctx, int32_t *precision);
+
+/* ---------------- frozen-coefficient steps and the discrete adjoint ----------------
+ * No counterpart in the reference (which has no adjoint).  With nut (a volume) and ust (a
+ * plane), device arrays of the field type, lp_step runs frozen-coefficient steps: the
+ * eddy viscosity Nu_t and the wall u* are the given ones instead of the SGS model's
+ * (sgs_stag.f90:137-227) and the wall law's (wallstress.f90:145-162), so the step is a
+ * quadratic map of (u, v, w, RHS_f).  Null for both restores the model. */
+int lp_set_frozen(lp_ctx *ctx, const void *nut, const void *ust);
+/* The transpose of that step's linearisation at the context's state (u, v, w before the
+ * step, jt; jt == 1 without initu is the Euler start), binary64 only (LP_FP64 context;
+ * the base states of an LP_MIXED run carry over, u, v, w being binary64 in both).  nut,
+ * ust: the frozen coefficients of the step (binary64 device arrays).  ub (3 volumes
+ * u, v, w) and rb (3 volumes RHSx..z), device arrays: on entry the adjoints of the step's
+ * outputs, on return those of its inputs (ub of u, v, w, rb of RHS_f).  The context's
+ * state is left unchanged; its first call allocates the adjoint's scratch. */
+int lp_step_adjoint(lp_ctx *ctx, const double *nut, const double *ust, double *ub, double *rb);
 
 /* ---------------- measurement ----------------
  * Kernel classes whose launches can be bracketed by HIP events on the context's

@@ -9,7 +9,7 @@ import math
 LP_ABI_VERSION = 1
 
 # status codes
-LP_OK, LP_ERR_ARG, LP_ERR_HIP, LP_ERR_PIVOT, LP_ERR_NAN, LP_ERR_NOMEM, LP_ERR_NODEVICE = range(7)
+LP_OK, LP_ERR_ARG, LP_ERR_HIP, LP_ERR_PIVOT, LP_ERR_NAN, LP_ERR_NOMEM, LP_ERR_NODEVICE, LP_ERR_CFL = range(8)
 LP_FP64, LP_MIXED = 0, 1
 LP_LBC_WALL, LP_LBC_STRESS_FREE = 0, 1
 
@@ -33,7 +33,7 @@ FIELD_NAMES = [
 FIELD = {n: i for i, n in enumerate(FIELD_NAMES)}
 LP_NFIELDS = len(FIELD_NAMES)
 # fields that stay binary64 in an LP_MIXED context
-FIELDS_F64_IN_MIXED = {"p", "F_LM", "F_MM", "F_QN", "F_NN", "PCon", "RHS_PCon",
+FIELDS_F64_IN_MIXED = {"u", "v", "w", "p", "F_LM", "F_MM", "F_QN", "F_NN", "PCon", "RHS_PCon",
                        "RHS_PCon_f", "Rhs_pbe"}
 BOGUS = -1234567890.0
 KERNEL_CLASSES = {"row_r2c": 0, "col_op": 1, "row_c2r": 2, "lag_relax": 3, "press_tridiag": 4,
@@ -126,6 +126,15 @@ def make_config(nx, ny, nz, L_x=2 * math.pi, L_y=2 * math.pi, L_z=1.0, dt=2e-4,
     for k, v in kw.items():
         if not hasattr(c, k):
             raise AttributeError(f"lp_config has no field {k!r}")
+        setattr(c, k, v)
+    return c
+
+
+def copy_config(cfg, **kw):
+    """a copy of an lp_config with some members replaced (the droplet arrays it points to
+    are shared, not copied)"""
+    c = lp_config.from_buffer_copy(cfg)
+    for k, v in kw.items():
         setattr(c, k, v)
     return c
 

@@ -38,22 +38,27 @@ FC = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
 
 
 def build_fortran(verbose=False):
-    """Compile the ISO_C_BINDING module and the drop-in demo host with amdflang (the
+    """Compile the ISO_C_BINDING module, the reference-signature shims
+    (lesgo_dropin.f90) and the two Fortran hosts that use them with amdflang (the
     reference's language).  Skipped (returns None) when no Fortran compiler exists."""
     if not os.path.exists(FC):
         return None
-    out = os.path.join(FORTRAN, "drop_in_demo")
     mod_dir = os.path.join(OBJ, "fortran")
     os.makedirs(mod_dir, exist_ok=True)
-    cmd = [FC, "-O2", f"-J{mod_dir}", os.path.join(FORTRAN, "lesgo_amd_iface.f90"),
-           os.path.join(FORTRAN, "drop_in_demo.f90"), "-L" + HERE, "-llesgo_hip",
-           "-Wl,-rpath," + HERE, "-o", out]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"amdflang failed:\n{r.stdout}\n{r.stderr}")
-    if verbose:
-        print(f"built {out}")
-    return out
+    outs = []
+    for prog, mods in (("drop_in_demo", ["lesgo_amd_iface.f90"]),
+                       ("dropin_check", ["lesgo_amd_iface.f90", "lesgo_dropin.f90"])):
+        out = os.path.join(FORTRAN, prog)
+        cmd = [FC, "-O2", f"-J{mod_dir}", *[os.path.join(FORTRAN, m) for m in mods],
+               os.path.join(FORTRAN, prog + ".f90"), "-L" + HERE, "-llesgo_hip",
+               "-Wl,-rpath," + HERE, "-o", out]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang failed for {prog}:\n{r.stdout}\n{r.stderr}")
+        if verbose:
+            print(f"built {out}")
+        outs.append(out)
+    return outs
 
 
 def build(jobs=None, verbose=False):

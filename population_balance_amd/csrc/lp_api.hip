@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <algorithm>
 #include <map>
@@ -17,6 +18,7 @@
 #include "lp_droplets.hpp"
 #include "lp_lag.hpp"
 #include "lp_convec.hpp"
+#include "lp_adjoint.hpp"
 
 using namespace lp;
 
@@ -28,6 +30,15 @@ static int set_err(int code, const char *fmt, ...) {
   va_end(ap);
   return code;
 }
+namespace lp {
+int set_error(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+}  // namespace lp
 #define HIPCHK(expr)                                                                        \
   do {                                                                                      \
     hipError_t e_ = (expr);                                                                 \
@@ -119,6 +130,11 @@ struct lp_ctx {
   long sp = 0, sb = 0;
   void *phat = nullptr;       // binary64 p-hat (0..nz) in the sp layout
   void *uvw_spec[3] = {};     // x-spectra of the filtered u, v, w left by the step's filt_da
+  // binary32 context: the per-level values that u and v entered filt_da's transform
+  // shifted by ([2][nz]); the convective products' level means include them again
+  double *ushift = nullptr;
+  double *cmean = nullptr;  // [6][nz] level means of the padded convec fields (binary32)
+  double *nyq = nullptr;    // filt_da of binary64 velocities: A [nz][ny], B [nz][nx], C [nz]
   void *cvb = nullptr;        // convec: y-padded half spectra of u1 u2 u3 w1 w2 w3 [6][nz][ny2][cvp_pitch]
   void *cvp = nullptr;        // convec: half spectra of the 3/2-grid products [3][nz][ny2][cvp_pitch]
   long cv_pitch = 0;          // kx columns kept on the 3/2 grid (nx/2) rounded up to 16
@@ -132,7 +148,17 @@ struct lp_ctx {
   void *bw = nullptr, *tw = nullptr;  // wall / lid divergence spectra
   double *gam = nullptr;      // Thomas scratch (nz+1) x lh x ny
   void *pspec[3] = {};        // binary64 spectra for the p inverse
+  // d_err[0]: zero pivot of a Thomas solve; d_err[1]: non-finite velocity written by a
+  // step; d_err[2]: lp_check_finite's scratch flag
   int *d_err = nullptr;
+  double cfl_max = 0.0;  // lp_set_cfl_limit (0: no check, as the reference only reports it)
+  // frozen-coefficient steps (lp_set_frozen): Nu_t (volume) and the wall u* (plane) of a
+  // base run, device arrays of the field type, used instead of the SGS model / wall law
+  const void *frz_nut = nullptr, *frz_ust = nullptr;
+  // lp_step_adjoint scratch (allocated on its first call): binary64 volumes, 3/2-grid
+  // volumes and the transposed solve's lambda
+  double *adj_block = nullptr;
+  size_t adj_bytes = 0;
   double *d_stats = nullptr;  // lp_plane_stats: [nz-1][10]
   double *d_espec = nullptr;  // lp_spectrum: [nz-1][nx/2]
   double *d_part = nullptr;
@@ -219,6 +245,7 @@ static dim3 grid1(long n) { return dim3((unsigned)((n + 255) / 256)); }
 static bool field_is_f64(const lp_ctx *c, int id) {
   if (c->fdt == F64) return true;
   switch (id) {
+    case LP_U: case LP_V: case LP_W:
     case LP_P: case LP_F_LM: case LP_F_MM: case LP_F_QN: case LP_F_NN:
     case LP_PCON: case LP_RHS_PCON: case LP_RHS_PCON_F: case LP_RHS_PBE:
       return true;
@@ -485,7 +512,7 @@ extern "C" int lp_destroy(lp_ctx *c) {
   for (auto p : c->Hs) fr(p);
   fr(c->bw); fr(c->tw); fr(c->gam); fr(c->phat);
   for (auto p : c->pspec) fr(p);
-  fr(c->d_err); fr(c->d_part); fr(c->d_stats); fr(c->d_espec);
+  fr(c->d_err); fr(c->d_part); fr(c->d_stats); fr(c->d_espec); fr(c->ushift); fr(c->cmean); fr(c->nyq); fr(c->adj_block);
   fr(c->d_ws); fr(c->d_qsrc); fr(c->d_vfac); fr(c->d_foot); fr(c->d_bins); fr(c->d_pbeA);
   fr(c->ui0); fr(c->vi0); fr(c->kcell);
   for (auto e : c->ev) (void)hipEventDestroy(e);
@@ -494,7 +521,7 @@ extern "C" int lp_destroy(lp_ctx *c) {
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->ev_aux_fork) (void)hipEventDestroy(c->ev_aux_fork);
   if (c->ev_aux) (void)hipEventDestroy(c->ev_aux);
-  if (c->s2) (void)hipStreamDestroy(c->s2);
+  if (c->s2 && c->s2 != c->s) (void)hipStreamDestroy(c->s2);
   if (c->s) (void)hipStreamDestroy(c->s);
   delete c;
   return LP_OK;
@@ -536,7 +563,12 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
     delete c;
     return set_err(LP_ERR_HIP, "hipStreamCreate failed");
   }
-  if (hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking) != hipSuccess ||
+  // LP_SERIAL_STREAMS=1 (profiling): the second stream's work is issued on the first, so
+  // that every kernel of a step runs alone and its trace duration is its own
+  const char *serial = std::getenv("LP_SERIAL_STREAMS");
+  const bool one_stream = serial && serial[0] == '1';
+  if (one_stream) c->s2 = c->s;
+  if ((!one_stream && hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking) != hipSuccess) ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_aux_fork, hipEventDisableTiming) != hipSuccess ||
@@ -616,6 +648,9 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
   for (auto &p : c->pspec) CK(dalloc(&p, sizeof(double) * 2 * (size_t)c->sp * g.ny * (g.nz + 1)));
   CK(dalloc(&c->phat, sizeof(double) * 2 * (size_t)c->sp * g.ny * (g.nz + 1)));
   CK(dalloc((void **)&c->d_err, sizeof(int) * 4));
+  CK(dalloc((void **)&c->ushift, sizeof(double) * 2 * (size_t)g.nz));
+  CK(dalloc((void **)&c->cmean, sizeof(double) * 6 * (size_t)g.nz));
+  CK(dalloc((void **)&c->nyq, sizeof(double) * (size_t)g.nz * (g.ny + g.nx + 1)));
   CK(dalloc((void **)&c->d_stats, sizeof(double) * 10 * (size_t)(g.nz - 1)));
   CK(dalloc((void **)&c->d_espec, sizeof(double) * (size_t)(g.nx / 2) * (g.nz - 1)));
   c->nred = 512;
@@ -657,7 +692,7 @@ extern "C" int lp_upload(lp_ctx *c, int32_t field, const void *host, size_t byte
     return set_err(LP_ERR_ARG, "lp_upload: field %d expects %zu bytes, got %zu", field, c->bytes[field], bytes);
   HIPCHK(hipSetDevice(c->dev));
   if (field == LP_U_LAG || field == LP_V_LAG || field == LP_W_LAG) c->lag_vel_zero = false;
-  HIPCHK(hipMemcpyAsync(c->f[field], host, bytes, hipMemcpyHostToDevice, c->s));
+  HIPCHK(hipMemcpyAsync(c->f[field], host, bytes, hipMemcpyDefault, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
   return LP_OK;
 }
@@ -667,7 +702,7 @@ extern "C" int lp_download(lp_ctx *c, int32_t field, void *host, size_t bytes) {
   if (bytes != c->bytes[field])
     return set_err(LP_ERR_ARG, "lp_download: field %d holds %zu bytes, asked %zu", field, c->bytes[field], bytes);
   HIPCHK(hipSetDevice(c->dev));
-  HIPCHK(hipMemcpyAsync(host, c->f[field], bytes, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemcpyAsync(host, c->f[field], bytes, hipMemcpyDefault, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
   return LP_OK;
 }
@@ -701,22 +736,26 @@ static double bytes_col(int nyi, int nyo, int ncols, int nlev, int nout, size_t 
 
 // ncols_out < lh writes only the leading kx columns (a spectrum that the caller knows to
 // be zero beyond them)
-static int r2c_small(lp_ctx *c, const void *in, int nlev, double scale, void *spec, int ncols_out = 0) {
+// ri: the rows' type and shift (RowIn; default: rows of the field type, no shift)
+static int r2c_small(lp_ctx *c, const void *in, int nlev, double scale, void *spec, int ncols_out = 0,
+                     RowIn ri = {}) {
   const lp_grid &g = c->g;
   if (ncols_out <= 0) ncols_out = g.lh;
+  const size_t ei = ri.din < 0 ? esz_of(c->fdt) : esz_of((Dt)ri.din);
   prof_begin(c, LP_K_ROW_R2C);
   HIPCHK(row_r2c(c->fdt, g.nx / 2, in, g.ld, plane_el(c), g.ny, nlev, scale, spec, c->sp,
-                 c->sp * g.ny, ncols_out, c->s));
-  prof_end(c, LP_K_ROW_R2C, bytes_r2c(g.nx, g.ny, nlev, ncols_out, esz_of(c->fdt)));
+                 c->sp * g.ny, ncols_out, c->s, ri));
+  prof_end(c, LP_K_ROW_R2C, (double)g.ny * nlev * ((double)g.nx * ei + (double)ncols_out * 2 * esz_of(c->fdt)));
   return LP_OK;
 }
-// ikx: c->kx (its first row is kx of every column) to store the x-derivative instead
+// ikx: c->kx (its first row is kx of every column) to store the x-derivative instead;
+// shift: per-plane values added back in binary64 (dto F64 from a binary32 spectrum)
 static int c2r_small(lp_ctx *c, const void *spec, int ncols, int nlev, void *out, Dt dto,
-                     const double *ikx = nullptr) {
+                     const double *ikx = nullptr, const double *shift = nullptr) {
   const lp_grid &g = c->g;
   prof_begin(c, LP_K_ROW_C2R);
   HIPCHK(row_c2r(c->fdt, dto, g.nx / 2, spec, c->sp, c->sp * g.ny, ncols, g.ny, nlev, out, g.ld,
-                 plane_el(c), c->s, ikx));
+                 plane_el(c), c->s, ikx, shift));
   prof_end(c, LP_K_ROW_C2R, bytes_c2r(g.nx, g.ny, nlev, ncols, esz_of(c->fdt), esz_of(dto)));
   return LP_OK;
 }
@@ -734,8 +773,8 @@ static int col_small(lp_ctx *c, const void *in, int nlev, int fwd, int inv, int 
 
 // one input field -> derivative / filter outputs (multipliers), each output real field
 static int spectral_apply(lp_ctx *c, const void *f, int nlev, double scale, int nout, const int *mults,
-                          void *const *outs) {
-  RCHK(r2c_small(c, f, nlev, scale, c->spec[0]));
+                          void *const *outs, RowIn ri = {}) {
+  RCHK(r2c_small(c, f, nlev, scale, c->spec[0], 0, ri));
   ColOut o[3];
   for (int j = 0; j < nout; ++j) o[j] = ColOut{c->spec[j], mults[j]};
   RCHK(col_small(c, c->spec[0], nlev, 1, 1, nout, o));
@@ -788,28 +827,66 @@ static int from_big(lp_ctx *c, const void *fb, int nlev, double scale, void *out
 // convection's 3/2 padding of the same field starts from.  kx is uniform along a column,
 // so the (kx, y) spectrum of df/dx is i kx times that of f: the column pass produces f
 // and df/dy only, and df/dx is the back-transform of f's spectrum times i kx.
-static int do_filt_da(lp_ctx *c, void *f, void *dfdx, void *dfdy, int nlev, void *keep = nullptr) {
+// vel: f is a binary64 velocity of the context (in a binary32 context the transforms
+// still run in binary32); shift (binary32 context only, with vel): the planes enter the
+// transform less their first value, recorded in shift[plane] and added back to the
+// filtered f in binary64 (filt_da(f - s) + s = filt_da(f), since a constant has only the
+// kx = ky = 0 mode).  keep then holds the spectrum of f - s.
+static int do_filt_da(lp_ctx *c, void *f, void *dfdx, void *dfdy, int nlev, void *keep = nullptr, bool vel = false,
+                      double *shift = nullptr) {
   double cst = 1.0 / ((double)c->g.nx * c->g.ny);
-  RCHK(r2c_small(c, f, nlev, cst, c->spec[0]));
+  RowIn ri;
+  if (vel) ri.din = F64;
+  if (vel && shift && c->fdt == F32) {
+    ri.shift = true;
+    ri.shift_out = shift;
+  } else {
+    shift = nullptr;
+  }
+  RCHK(r2c_small(c, f, nlev, cst, c->spec[0], 0, ri));
   void *s0 = keep ? keep : c->spec[0];
   ColOut o[2] = {{s0, M_ONE}, {c->spec[2], M_IKY}};
   RCHK(col_small(c, c->spec[0], nlev, 1, 1, 2, o));
-  RCHK(c2r_small(c, s0, c->g.lh, nlev, f, c->fdt));
+  if (vel && c->fdt == F32) {
+    // the filtered velocity in binary64: its oddball modes subtracted exactly (a binary32
+    // round trip would put about 3e-7 of the fluctuations' size back into u, v, w at
+    // every step)
+    Geo g = geo(c);
+    double *A = c->nyq, *B = A + (size_t)c->g.nz * g.ny, *Cc = B + (size_t)c->g.nz * g.nx;
+    hipLaunchKernelGGL(k_nyq_rows, dim3((unsigned)(((long)g.ny * nlev + 3) / 4)), dim3(256), 0, c->s,
+                       (const double *)f, A, g, nlev);
+    hipLaunchKernelGGL(k_nyq_cols, dim3((unsigned)((g.nx + 63) / 64), (unsigned)nlev), dim3(256), 0, c->s,
+                       (const double *)f, (const double *)A, B, Cc, g);
+    hipLaunchKernelGGL(k_nyq_remove, grid1(g.np * nlev), dim3(256), 0, c->s, (double *)f, (const double *)A,
+                       (const double *)B, (const double *)Cc, g, nlev);
+    HIPCHK(hipGetLastError());
+  } else {
+    RCHK(c2r_small(c, s0, c->g.lh, nlev, f, vel ? F64 : c->fdt, nullptr, shift));
+  }
   RCHK(c2r_small(c, s0, c->g.lh, nlev, dfdx, c->fdt, c->kx));
   return c2r_small(c, c->spec[2], c->g.lh, nlev, dfdy, c->fdt);
 }
 
-static int do_ddz_uv(lp_ctx *c, void *dfdz, const void *f) {
+// vel: f is a binary64 velocity of the context (the difference is formed in binary64)
+static int do_ddz_uv(lp_ctx *c, void *dfdz, const void *f, bool vel = false) {
   Geo g = geo(c);
-  FDISPATCH(c, T, hipLaunchKernelGGL(k_ddz_uv<T>, grid1(g.np * g.nz), dim3(256), 0, c->s, (T *)dfdz,
-                                     (const T *)f, g, (T)(1.0 / c->g.dz)));
+  if (vel)
+    FDISPATCH(c, T, hipLaunchKernelGGL((k_ddz_uv<T, double>), grid1(g.np * g.nz), dim3(256), 0, c->s, (T *)dfdz,
+                                       (const double *)f, g, 1.0 / c->g.dz));
+  else
+    FDISPATCH(c, T, hipLaunchKernelGGL(k_ddz_uv<T>, grid1(g.np * g.nz), dim3(256), 0, c->s, (T *)dfdz,
+                                       (const T *)f, g, (T)(1.0 / c->g.dz)));
   HIPCHK(hipGetLastError());
   return LP_OK;
 }
-static int do_ddz_w(lp_ctx *c, void *dfdz, const void *f) {
+static int do_ddz_w(lp_ctx *c, void *dfdz, const void *f, bool vel = false) {
   Geo g = geo(c);
-  FDISPATCH(c, T, hipLaunchKernelGGL(k_ddz_w<T>, grid1(g.np * g.nz), dim3(256), 0, c->s, (T *)dfdz,
-                                     (const T *)f, g, (T)(1.0 / c->g.dz)));
+  if (vel)
+    FDISPATCH(c, T, hipLaunchKernelGGL((k_ddz_w<T, double>), grid1(g.np * g.nz), dim3(256), 0, c->s, (T *)dfdz,
+                                       (const double *)f, g, 1.0 / c->g.dz));
+  else
+    FDISPATCH(c, T, hipLaunchKernelGGL(k_ddz_w<T>, grid1(g.np * g.nz), dim3(256), 0, c->s, (T *)dfdz,
+                                       (const T *)f, g, (T)(1.0 / c->g.dz)));
   HIPCHK(hipGetLastError());
   return LP_OK;
 }
@@ -829,12 +906,23 @@ static int do_wallstress(lp_ctx *c) {
   // wallstress.f90:103-104,139-140: test-filtered copies of u(1), v(1)
   const int m[1] = {M_G1};
   void *o1[1] = {c->tmp[0]}, *o2[1] = {c->tmp[1]};
-  RCHK(spectral_apply(c, c->f[LP_U], 1, 1.0, 1, m, o1));
-  RCHK(spectral_apply(c, c->f[LP_V], 1, 1.0, 1, m, o2));
+  RowIn vel;
+  vel.din = F64;
+  RCHK(spectral_apply(c, c->f[LP_U], 1, 1.0, 1, m, o1, vel));
+  RCHK(spectral_apply(c, c->f[LP_V], 1, 1.0, 1, m, o2, vel));
   double zl = 0.5 * c->g.dz - cf.d0;
   double denom = std::log(zl / cf.z0) - 0.0;
+  if (c->frz_ust) {  // frozen coefficient: the base run's u*
+    FDISPATCH(c, T, hipLaunchKernelGGL(k_wallstress_frozen<T>, grid1(g.np), dim3(256), 0, c->s,
+                                       (const T *)c->tmp[0], (const T *)c->tmp[1], (const double *)c->f[LP_U],
+                                       (const double *)c->f[LP_V], (T *)c->f[LP_TXZ], (T *)c->f[LP_TYZ],
+                                       (T *)c->f[LP_DUDZ], (T *)c->f[LP_DVDZ], (T *)c->f[LP_USTAR],
+                                       (const T *)c->frz_ust, g, denom, zl, cf.vonk));
+    HIPCHK(hipGetLastError());
+    return LP_OK;
+  }
   FDISPATCH(c, T, hipLaunchKernelGGL(k_wallstress<T>, grid1(g.np), dim3(256), 0, c->s, (const T *)c->tmp[0],
-                                     (const T *)c->tmp[1], (const T *)c->f[LP_U], (const T *)c->f[LP_V],
+                                     (const T *)c->tmp[1], (const double *)c->f[LP_U], (const double *)c->f[LP_V],
                                      (T *)c->f[LP_TXZ], (T *)c->f[LP_TYZ], (T *)c->f[LP_DUDZ],
                                      (T *)c->f[LP_DVDZ], (T *)c->f[LP_USTAR], g, denom, zl, cf.vonk));
   HIPCHK(hipGetLastError());
@@ -964,7 +1052,7 @@ static int do_lagrange(lp_ctx *c, bool use_u = false) {
   prof_begin(c, LP_K_LAG_RELAX);
   const int lv[3] = {use_u ? LP_U : LP_U_LAG, use_u ? LP_V : LP_V_LAG, use_u ? LP_W : LP_W_LAG};
   HIPCHK(lag_relax(c->fdt, c->lcon, Fold, Fnew, c->f[lv[0]], c->f[lv[1]], c->f[lv[2]], c->f[LP_CS_OPT2],
-                   c->f[LP_BETA], g, P, I, c->s));
+                   c->f[LP_BETA], g, P, I, c->s, use_u));
   prof_end(c, LP_K_LAG_RELAX, vol * (8.0 * 8 + 9.0 * fe));
   swap_F(c);
   if (!use_u) RCHK(reset_lag_velocities(c));
@@ -985,6 +1073,15 @@ static int do_sgs_stag(lp_ctx *c) {
                                      (T *)c->S[1], (T *)c->S[2], (T *)c->S[3], (T *)c->S[4], (T *)c->S[5],
                                      (T *)c->tmp[3], g));
   HIPCHK(hipGetLastError());
+  if (c->frz_nut) {  // frozen coefficient: the base run's Nu_t, no model update
+    FDISPATCH(c, T, hipLaunchKernelGGL(k_tau_frozen<T>, grid1(nv), dim3(256), 0, c->s, (const T *)c->S[0],
+                                       (const T *)c->S[1], (const T *)c->S[2], (const T *)c->S[3],
+                                       (const T *)c->S[4], (const T *)c->S[5], (const T *)c->frz_nut,
+                                       (T *)c->f[LP_NU_T], (T *)c->f[LP_TXX], (T *)c->f[LP_TXY], (T *)c->f[LP_TXZ],
+                                       (T *)c->f[LP_TYY], (T *)c->f[LP_TYZ], (T *)c->f[LP_TZZ], g));
+    HIPCHK(hipGetLastError());
+    return LP_OK;
+  }
   double delta = cf.filter_size * std::pow(gg.dx * gg.dy * gg.dz, 1.0 / 3.0);
   if (cf.model == 5) {
     const bool init_cs = c->jt == 1 && cf.inilag;  // sgs_stag.f90:178-180
@@ -994,8 +1091,8 @@ static int do_sgs_stag(lp_ctx *c) {
       RCHK(do_lagrange(c, true));
     } else {
       FDISPATCH(c, T, hipLaunchKernelGGL(k_lag_acc<T>, grid1(nv), dim3(256), 0, c->s, (T *)c->f[LP_U_LAG],
-                                         (T *)c->f[LP_V_LAG], (T *)c->f[LP_W_LAG], (const T *)c->f[LP_U],
-                                         (const T *)c->f[LP_V], (const T *)c->f[LP_W], nv));
+                                         (T *)c->f[LP_V_LAG], (T *)c->f[LP_W_LAG], (const double *)c->f[LP_U],
+                                         (const double *)c->f[LP_V], (const double *)c->f[LP_W], nv));
       HIPCHK(hipGetLastError());
       c->lag_vel_zero = false;
       if (init_cs) {
@@ -1066,12 +1163,14 @@ static int do_divstress_all(lp_ctx *c) {
 
 // dealias1.f90 from the x-spectrum on: the column pass that pads it to ny2 and transforms
 // it back along y, into field q of c->cvb; scale multiplies the padded spectrum
-static int pad_y(lp_ctx *c, const void *spec, int q, double scale = 1.0) {
+// dc: the per-level kx = ky = 0 entries are stored there and left out of the padded field
+static int pad_y(lp_ctx *c, const void *spec, int q, double scale = 1.0, double *dc = nullptr) {
   const lp_grid &g = c->g;
   const long ps = c->sp * g.ny, pb = c->cv_pitch * g.ny2;
   const size_t fe = esz_of(c->fdt);
   ColOut o{(char *)c->cvb + (size_t)q * pb * g.nz * 2 * fe, M_ONE};
   o.scale = scale;
+  o.dc = dc;
   prof_begin(c, LP_K_COL_PAD);
   HIPCHK(col_op(c->fdt, g.ny, g.ny2, 1, 1, 1, spec, c->sp, ps, g.nx / 2, g.nz, 1, &o, c->cv_pitch, pb, c->tab,
                 c->s));
@@ -1086,12 +1185,22 @@ static int do_convec(lp_ctx *c, bool kept) {
   Geo g = geo(c);
   const size_t fe = esz_of(c->fdt);
   const double cst = 1.0 / ((double)gg.nx * gg.ny);
-  // convec.f90:76-98
+  // convec.f90:76-98.  Binary32: every padded field leaves its level means (the kx = ky
+  // = 0 entries) in c->cmean and enters the products without them (k_convec_cross adds
+  // their part back); u and v are transformed less their first value per level (ushift),
+  // as filt_da transforms them.
+  const bool split = c->fdt == F32;
   const int vel[3] = {LP_U, LP_V, LP_W};
   for (int q = 0; q < 3; ++q) {
-    if (!kept) RCHK(r2c_small(c, c->f[vel[q]], gg.nz, cst, c->uvw_spec[q]));
+    if (!kept) {
+      RowIn vin;
+      vin.din = F64;
+      vin.shift = split && q < 2;
+      vin.shift_out = vin.shift ? c->ushift + (size_t)q * gg.nz : nullptr;
+      RCHK(r2c_small(c, c->f[vel[q]], gg.nz, cst, c->uvw_spec[q], 0, vin));
+    }
     // filt_da's spectrum went through an unnormalised y round trip: ny times the r2c
-    RCHK(pad_y(c, c->uvw_spec[q], q, kept ? 1.0 / gg.ny : 1.0));
+    RCHK(pad_y(c, c->uvw_spec[q], q, kept ? 1.0 / gg.ny : 1.0, split ? c->cmean + (size_t)q * gg.nz : nullptr));
   }
   // convec.f90:100-133: each vorticity component formed inside its x transform
   const int va[3] = {LP_DWDY, LP_DWDX, LP_DVDX}, vb[3] = {LP_DVDZ, LP_DUDZ, LP_DUDY};
@@ -1100,7 +1209,7 @@ static int do_convec(lp_ctx *c, bool kept) {
     HIPCHK(row_r2c_vort(c->fdt, gg.nx / 2, q, c->f[va[q]], c->f[vb[q]], gg.ld, plane_el(c), gg.ny, gg.nz, cst,
                         c->cv_spec[0], c->sp, c->sp * gg.ny, gg.nx / 2, c->s));
     prof_end(c, LP_K_ROW_R2C, bytes_r2c(gg.nx, gg.ny, gg.nz, gg.nx / 2, fe) + (double)vol_el(c) * fe);
-    RCHK(pad_y(c, c->cv_spec[0], 3 + q));
+    RCHK(pad_y(c, c->cv_spec[0], 3 + q, 1.0, split ? c->cmean + (size_t)(3 + q) * gg.nz : nullptr));
   }
   // convec.f90:140-158, 180-221, 241-279 products (normalisation folded in) between the
   // x back- and forward transforms of the 3/2 grid, kx < nx/2 kept (dealias2)
@@ -1120,6 +1229,14 @@ static int do_convec(lp_ctx *c, bool kept) {
                   c->cv_pitch, pb, gg.nx / 2, gg.nz - 1, 1, &o, c->sp, ps, c->tab, c->s));
     prof_end(c, LP_K_COL_PAD, bytes_col(gg.ny2, gg.ny, gg.nx / 2, gg.nz - 1, 1, fe));
     RCHK(c2r_small(c, c->cv_spec[1], gg.nx / 2, gg.nz - 1, outs[q], c->fdt));
+  }
+  if (split) {
+    hipLaunchKernelGGL(k_convec_cross<float>, grid1(g.np * (gg.nz - 1)), dim3(256), 0, c->s, (float *)outs[0],
+                       (float *)outs[1], (float *)outs[2], (const double *)c->f[LP_U], (const double *)c->f[LP_V],
+                       (const double *)c->f[LP_W], (const float *)c->f[LP_DUDY], (const float *)c->f[LP_DUDZ],
+                       (const float *)c->f[LP_DVDX], (const float *)c->f[LP_DVDZ], (const float *)c->f[LP_DWDX],
+                       (const float *)c->f[LP_DWDY], (const double *)c->cmean, (const double *)c->ushift, g);
+    HIPCHK(hipGetLastError());
   }
   // convec.f90:306-308 top level BOGUS
   for (int q = 0; q < 3; ++q) {
@@ -1170,10 +1287,17 @@ static int do_press(lp_ctx *c) {
   std::swap(c->s, c->s2);
   RCHK(rc_side);
   HIPCHK(hipEventRecord(c->ev_aux, c->s2));
-  // press_stag_array.f90:75-87,126-131: H = FFT(u*/(1.5 dt))/(nx ny), oddballs zeroed
+  // press_stag_array.f90:75-87,126-131: H = FFT(u*/(1.5 dt))/(nx ny), oddballs zeroed.
+  // Binary32 transforms take u* and v* less their value at each level's first point:
+  // that moves only the kx = ky = 0 mode of H_x and H_y, which the solve never reads
+  // (the mean mode integrates H_z and the wall / lid rows), and it keeps the mean flow's
+  // magnitude out of the transforms' rounding.
   const int fid[3] = {LP_U, LP_V, LP_W};
   for (int q = 0; q < 3; ++q) {
-    RCHK(r2c_small(c, c->f[fid[q]], gg.nz - 1, cst / tadv1 / cf.dt, c->Hs[q]));
+    RowIn ri;
+    ri.din = F64;
+    ri.shift = c->fdt == F32 && q < 2;
+    RCHK(r2c_small(c, c->f[fid[q]], gg.nz - 1, cst / tadv1 / cf.dt, c->Hs[q], 0, ri));
     ColOut o{c->Hs[q], M_ONE};
     RCHK(col_small(c, c->Hs[q], gg.nz - 1, 1, 0, 1, &o));
   }
@@ -1235,9 +1359,9 @@ static int do_pollen_rhs(lp_ctx *c, const double *prev, bool fuse_step = false, 
   double *PC = (double *)c->f[LP_PCON], *RH = (double *)c->f[LP_RHS_PCON];
   FDISPATCH(c, T, {
     hipLaunchKernelGGL(k_face_u<T>, dim3((unsigned)(((long)gg.ny * gg.nz + 3) / 4)), dim3(256), 0, c->s,
-                       (const T *)c->f[LP_U], (const T *)c->f[LP_DUDX], c->ui0, dg, gg.dx);
+                       (const double *)c->f[LP_U], (const T *)c->f[LP_DUDX], c->ui0, dg, gg.dx);
     hipLaunchKernelGGL(k_face_v<T>, dim3((unsigned)((gg.nx + 63) / 64), (unsigned)gg.nz), dim3(64 * kFaceSeg), 0,
-                       c->s, (const T *)c->f[LP_V], (const T *)c->f[LP_DVDY], c->vi0, dg, gg.dy);
+                       c->s, (const double *)c->f[LP_V], (const T *)c->f[LP_DVDY], c->vi0, dg, gg.dy);
     hipLaunchKernelGGL(k_kc<T>, grid1(dg.nv), dim3(256), 0, c->s, (const T *)c->f[LP_NU_T], (T *)c->f[LP_MAGS],
                        c->kcell, dg, D.Sc);
   });
@@ -1257,7 +1381,7 @@ static int do_pollen_rhs(lp_ctx *c, const double *prev, bool fuse_step = false, 
   prof_begin(c, LP_K_POLLEN_RHS);
   const dim3 pgrid((unsigned)((dg.nv + 255) / 256), (unsigned)((dg.npc + kPollenBins - 1) / kPollenBins));
   FDISPATCH(c, T, hipLaunchKernelGGL(k_pollen_rhs<T>, pgrid, dim3(256), 0, c->s, PC, RH, prev, c->ui0,
-                                     c->vi0, (const T *)c->f[LP_W], (const T *)c->f[LP_DUDT],
+                                     c->vi0, (const double *)c->f[LP_W], (const T *)c->f[LP_DUDT],
                                      (const T *)c->f[LP_DVDT], (const T *)c->f[LP_DWDT], c->kcell, c->d_ws,
                                      c->d_qsrc, c->d_foot, dg, P));
   HIPCHK(hipGetLastError());
@@ -1321,8 +1445,9 @@ static int do_pollen(lp_ctx *c) {
 static int do_ke_dudt(lp_ctx *c) {
   const lp_grid &gg = c->g;
   Geo g = geo(c);
-  FDISPATCH(c, T, hipLaunchKernelGGL(k_energy<T>, grid1(g.np * g.nz), dim3(256), 0, c->s, (const T *)c->f[LP_U],
-                                     (const T *)c->f[LP_V], (const T *)c->f[LP_W], (T *)c->tmp[0], g));
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_energy<T>, grid1(g.np * g.nz), dim3(256), 0, c->s,
+                                     (const double *)c->f[LP_U], (const double *)c->f[LP_V],
+                                     (const double *)c->f[LP_W], (T *)c->tmp[0], g));
   HIPCHK(hipGetLastError());
   const int m[2] = {M_IKX, M_IKY};
   void *outs[2] = {c->tmp[1], c->tmp[2]};
@@ -1334,13 +1459,15 @@ static int do_ke_dudt(lp_ctx *c) {
   return LP_OK;
 }
 
-// reads the zero-pivot flag the Thomas kernels raise, and clears it for the next call
+// reads the flags the kernels raise (zero pivot of the Thomas solves, a non-finite
+// velocity written by a step) and clears them for the next call
 static int check_err_flag(lp_ctx *c) {
-  int h = 0;
-  HIPCHK(hipMemcpyAsync(&h, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->s));
-  HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(int), c->s));
+  int h[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(h, c->d_err, sizeof h, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof h, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
-  if (h) return set_err(LP_ERR_PIVOT, "tridag_array: zero pivot");
+  if (h[0]) return set_err(LP_ERR_PIVOT, "tridag_array: zero pivot");
+  if (h[1]) return set_err(LP_ERR_NAN, "lp_step: a step wrote a non-finite velocity");
   return LP_OK;
 }
 
@@ -1348,9 +1475,9 @@ static int check_err_flag(lp_ctx *c) {
 static int project_bc(lp_ctx *c) {
   const lp_config &cf = c->cfg;
   Geo g = geo(c);
-  FDISPATCH(c, T, hipLaunchKernelGGL(k_project_bc<T>, grid1(g.np), dim3(256), 0, c->s, (T *)c->f[LP_U],
-                                     (T *)c->f[LP_V], (T *)c->f[LP_W], g, cf.use_mean_p_force,
-                                     cf.lbc_mom == LP_LBC_STRESS_FREE ? 1 : 0));
+  hipLaunchKernelGGL(k_project_bc<double>, grid1(g.np), dim3(256), 0, c->s, (double *)c->f[LP_U],
+                     (double *)c->f[LP_V], (double *)c->f[LP_W], g, cf.use_mean_p_force,
+                     cf.lbc_mom == LP_LBC_STRESS_FREE ? 1 : 0);
   HIPCHK(hipGetLastError());
   return LP_OK;
 }
@@ -1358,24 +1485,24 @@ static int project_bc(lp_ctx *c) {
 static int do_project(lp_ctx *c) {
   const lp_config &cf = c->cfg;
   Geo g = geo(c);
-  FDISPATCH(c, T, hipLaunchKernelGGL(k_project<T>, grid1(g.np * (g.nz - 1)), dim3(256), 0, c->s, (T *)c->f[LP_U],
-                                     (T *)c->f[LP_V], (T *)c->f[LP_W], (const T *)c->f[LP_DPDX],
-                                     (const T *)c->f[LP_DPDY], (const T *)c->f[LP_DPDZ], g, (T)cf.dt));
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_project<T>, grid1(g.np * (g.nz - 1)), dim3(256), 0, c->s,
+                                     (double *)c->f[LP_U], (double *)c->f[LP_V], (double *)c->f[LP_W],
+                                     (const T *)c->f[LP_DPDX], (const T *)c->f[LP_DPDY], (const T *)c->f[LP_DPDZ], g,
+                                     cf.dt));
   HIPCHK(hipGetLastError());
   return project_bc(c);
 }
 
-// main.f90:252-826
-static int step_once(lp_ctx *c) {
+// main.f90:252-826 after RHS_f = RHS (step_once)
+static int step_body(lp_ctx *c) {
   const lp_config &cf = c->cfg;
   const lp_grid &gg = c->g;
   Geo g = geo(c);
   long n1 = g.np * (g.nz - 1);
-  // :252-254 RHS_f = RHS: the buffers exchange roles; convec rewrites every level of RHS
-  for (int q = 0; q < 3; ++q) std::swap(c->f[LP_RHSX + q], c->f[LP_RHSX_F + q]);
-  RCHK(do_filt_da(c, c->f[LP_U], c->f[LP_DUDX], c->f[LP_DUDY], gg.nz, c->uvw_spec[0]));  // :277-279
-  RCHK(do_filt_da(c, c->f[LP_V], c->f[LP_DVDX], c->f[LP_DVDY], gg.nz, c->uvw_spec[1]));
-  RCHK(do_filt_da(c, c->f[LP_W], c->f[LP_DWDX], c->f[LP_DWDY], gg.nz, c->uvw_spec[2]));
+  // :277-279 (u and v enter the binary32 transforms less a per-level value)
+  RCHK(do_filt_da(c, c->f[LP_U], c->f[LP_DUDX], c->f[LP_DUDY], gg.nz, c->uvw_spec[0], true, c->ushift));
+  RCHK(do_filt_da(c, c->f[LP_V], c->f[LP_DVDX], c->f[LP_DVDY], gg.nz, c->uvw_spec[1], true, c->ushift + gg.nz));
+  RCHK(do_filt_da(c, c->f[LP_W], c->f[LP_DWDX], c->f[LP_DWDY], gg.nz, c->uvw_spec[2], true));
   // :315 wallstress (level 1 of txz, tyz, dudz, dvdz from the test-filtered u(1), v(1):
   // single-plane passes) on the second stream, beside the z-derivatives of the other levels
   HIPCHK(hipEventRecord(c->ev_aux_fork, c->s));
@@ -1387,9 +1514,9 @@ static int step_once(lp_ctx *c) {
   std::swap(c->s, c->s2);
   RCHK(rc_wall);
   HIPCHK(hipEventRecord(c->ev_aux, c->s2));
-  RCHK(do_ddz_uv(c, c->f[LP_DUDZ], c->f[LP_U]));  // :284-288 (levels 2..nz-1)
-  RCHK(do_ddz_uv(c, c->f[LP_DVDZ], c->f[LP_V]));
-  RCHK(do_ddz_w(c, c->f[LP_DWDZ], c->f[LP_W]));
+  RCHK(do_ddz_uv(c, c->f[LP_DUDZ], c->f[LP_U], true));  // :284-288 (levels 2..nz-1)
+  RCHK(do_ddz_uv(c, c->f[LP_DVDZ], c->f[LP_V], true));
+  RCHK(do_ddz_w(c, c->f[LP_DWDZ], c->f[LP_W], true));
   HIPCHK(hipStreamWaitEvent(c->s, c->ev_aux, 0));  // sgs_stag reads level 1
   c->fork_pending = true;  // convec (:419) starts on the second stream inside sgs_stag
   const int rc_sgs = do_sgs_stag(c);  // :327
@@ -1419,8 +1546,8 @@ static int step_once(lp_ctx *c) {
                                      (T *)c->f[LP_DVDT], (T *)c->f[LP_DWDT],
                                      (const T *)(drops && cf.drop.active ? c->f[LP_BETA_PCON] : nullptr),
                                      (T *)c->f[LP_RHSX_F], (T *)c->f[LP_RHSY_F], (T *)c->f[LP_RHSZ_F], first ? 1 : 0,
-                                     (T *)c->f[LP_U], (T *)c->f[LP_V], (T *)c->f[LP_W], n1, (T)cf.dt, (T)force,
-                                     (T)cf.mean_p_force));
+                                     (double *)c->f[LP_U], (double *)c->f[LP_V], (double *)c->f[LP_W], n1, cf.dt,
+                                     force, (T)cf.mean_p_force));
   HIPCHK(hipGetLastError());
   if (first) {  // :577-584 also copies the (BOGUS) top level
     const size_t fe = esz_of(c->fdt), off = (size_t)n1 * fe, pb = (size_t)g.np * fe;
@@ -1434,13 +1561,26 @@ static int step_once(lp_ctx *c) {
                                      (const double *)c->f[LP_P], (T *)c->f[LP_DPDZ], (const T *)c->f[LP_DPDX],
                                      (const T *)c->f[LP_DPDY], (T *)c->f[LP_RHSX], (T *)c->f[LP_RHSY],
                                      (T *)c->f[LP_RHSZ], (T *)c->f[LP_DUDT], (T *)c->f[LP_DVDT],
-                                     (T *)c->f[LP_DWDT], (T *)c->f[LP_U], (T *)c->f[LP_V], (T *)c->f[LP_W], g,
-                                     gg.dz, (T)cf.dt));
+                                     (T *)c->f[LP_DWDT], (double *)c->f[LP_U], (double *)c->f[LP_V],
+                                     (double *)c->f[LP_W], g, gg.dz, cf.dt, c->d_err + 1));
   HIPCHK(hipGetLastError());
   RCHK(project_bc(c));
   if (drops) RCHK(do_ke_dudt(c));  // :864-898
   return LP_OK;
 }
+
+// main.f90:252-826.  :252-254 RHS_f = RHS: the buffers exchange roles (convec rewrites
+// every level of RHS); if the step fails before its end they are exchanged back, so that
+// RHS_f again holds the last completed step's RHS.
+static int step_once(lp_ctx *c) {
+  for (int q = 0; q < 3; ++q) std::swap(c->f[LP_RHSX + q], c->f[LP_RHSX_F + q]);
+  const int rc = step_body(c);
+  if (rc != LP_OK)
+    for (int q = 0; q < 3; ++q) std::swap(c->f[LP_RHSX + q], c->f[LP_RHSX_F + q]);
+  return rc;
+}
+
+static int cfl_of(lp_ctx *c, double *cfl, double *visc);
 
 extern "C" int lp_step(lp_ctx *c, int32_t nsteps) {
   if (!c || nsteps < 0) return set_err(LP_ERR_ARG, "lp_step: bad argument");
@@ -1452,7 +1592,55 @@ extern "C" int lp_step(lp_ctx *c, int32_t nsteps) {
   }
   int rc = check_err_flag(c);
   prof_collect(c);
+  if (rc == LP_OK && c->cfl_max > 0.0 && nsteps > 0) {
+    double cfl = 0.0, visc = 0.0;
+    RCHK(cfl_of(c, &cfl, &visc));
+    if (!(cfl <= c->cfl_max))
+      return set_err(LP_ERR_CFL, "lp_step: CFL %.4g after step jt_total=%d exceeds the limit %.4g", cfl,
+                     c->jt_total - 1, c->cfl_max);
+  }
   return rc;
+}
+
+extern "C" int lp_set_cfl_limit(lp_ctx *c, double cfl_max) {
+  if (!c || !(cfl_max >= 0.0)) return set_err(LP_ERR_ARG, "lp_set_cfl_limit: bad argument");
+  c->cfl_max = cfl_max;
+  return LP_OK;
+}
+
+extern "C" int lp_get_precision(lp_ctx *c, int32_t *precision) {
+  if (!c || !precision) return set_err(LP_ERR_ARG, "lp_get_precision: null argument");
+  *precision = c->cfg.precision;
+  return LP_OK;
+}
+
+extern "C" int lp_check_finite(lp_ctx *c) {
+  if (!c) return set_err(LP_ERR_ARG, "lp_check_finite: null context");
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->s2));
+  static const char *names[] = {"u", "v", "w", "RHSx", "RHSy", "RHSz", "RHSx_f", "RHSy_f", "RHSz_f", "p",
+                                "F_LM", "F_MM", "F_QN", "F_NN", "Cs_opt2", "PCon", "RHS_PCon", "RHS_PCon_f"};
+  static const int ids[] = {LP_U, LP_V, LP_W, LP_RHSX, LP_RHSY, LP_RHSZ, LP_RHSX_F, LP_RHSY_F, LP_RHSZ_F, LP_P,
+                            LP_F_LM, LP_F_MM, LP_F_QN, LP_F_NN, LP_CS_OPT2, LP_PCON, LP_RHS_PCON, LP_RHS_PCON_F};
+  const int nchk = (int)(sizeof ids / sizeof ids[0]) - (c->cfg.drop.npcon > 0 ? 0 : 3);
+  std::vector<int> flags(nchk, 0);
+  for (int q = 0; q < nchk; ++q) {
+    HIPCHK(hipMemsetAsync(c->d_err + 2, 0, sizeof(int), c->s));
+    const int id = ids[q];
+    const long n = (long)(c->bytes[id] / c->esz[id]);
+    if (c->esz[id] == 8)
+      hipLaunchKernelGGL(k_nonfinite<double>, dim3(c->nred), dim3(256), 0, c->s, (const double *)c->f[id], n,
+                         c->d_err + 2);
+    else
+      hipLaunchKernelGGL(k_nonfinite<float>, dim3(c->nred), dim3(256), 0, c->s, (const float *)c->f[id], n,
+                         c->d_err + 2);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&flags[q], c->d_err + 2, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  }
+  HIPCHK(hipStreamSynchronize(c->s));
+  for (int q = 0; q < nchk; ++q)
+    if (flags[q]) return set_err(LP_ERR_NAN, "lp_check_finite: %s holds a non-finite value", names[q]);
+  return LP_OK;
 }
 
 extern "C" int lp_prof_enable(lp_ctx *c, int32_t kernel_class) {
@@ -1539,9 +1727,14 @@ extern "C" int lp_press_stag_array(lp_ctx *c) {
 extern "C" int lp_timestep_conditions(lp_ctx *c, double *cfl, double *visc_stab) {
   if (!c || !cfl || !visc_stab) return set_err(LP_ERR_ARG, "lp_timestep_conditions: null argument");
   HIPCHK(hipSetDevice(c->dev));
+  return cfl_of(c, cfl, visc_stab);
+}
+
+// scalars_module2.f90:1336-1379
+static int cfl_of(lp_ctx *c, double *cfl, double *visc_stab) {
   Geo g = geo(c);
-  FDISPATCH(c, T, hipLaunchKernelGGL(k_reduce_diag<T>, dim3(c->nred), dim3(256), 0, c->s, (const T *)c->f[LP_U],
-                                     (const T *)c->f[LP_V], (const T *)c->f[LP_W], (const T *)c->f[LP_NU_T],
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_reduce_diag<T>, dim3(c->nred), dim3(256), 0, c->s, (const double *)c->f[LP_U],
+                                     (const double *)c->f[LP_V], (const double *)c->f[LP_W], (const T *)c->f[LP_NU_T],
                                      (const T *)c->f[LP_DUDX], (const T *)c->f[LP_DVDY], (const T *)c->f[LP_DWDZ], g,
                                      c->d_part));
   HIPCHK(hipGetLastError());
@@ -1560,8 +1753,8 @@ extern "C" int lp_rmsdiv(lp_ctx *c, double *rms) {
   if (!c || !rms) return set_err(LP_ERR_ARG, "lp_rmsdiv: null argument");
   HIPCHK(hipSetDevice(c->dev));
   Geo g = geo(c);
-  FDISPATCH(c, T, hipLaunchKernelGGL(k_reduce_diag<T>, dim3(c->nred), dim3(256), 0, c->s, (const T *)c->f[LP_U],
-                                     (const T *)c->f[LP_V], (const T *)c->f[LP_W], (const T *)c->f[LP_NU_T],
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_reduce_diag<T>, dim3(c->nred), dim3(256), 0, c->s, (const double *)c->f[LP_U],
+                                     (const double *)c->f[LP_V], (const double *)c->f[LP_W], (const T *)c->f[LP_NU_T],
                                      (const T *)c->f[LP_DUDX], (const T *)c->f[LP_DVDY], (const T *)c->f[LP_DWDZ], g,
                                      c->d_part));
   HIPCHK(hipGetLastError());
@@ -1580,11 +1773,11 @@ extern "C" int lp_plane_stats(lp_ctx *c, double *out) {
   Geo g = geo(c);
   const int nl = g.nz - 1;
   FDISPATCH(c, T, {
-    hipLaunchKernelGGL((k_plane_stats<T, 0>), dim3(nl), dim3(256), 0, c->s, (const T *)c->f[LP_U],
-                       (const T *)c->f[LP_V], (const T *)c->f[LP_W], (const T *)c->f[LP_CS_OPT2],
+    hipLaunchKernelGGL((k_plane_stats<T, 0>), dim3(nl), dim3(256), 0, c->s, (const double *)c->f[LP_U],
+                       (const double *)c->f[LP_V], (const double *)c->f[LP_W], (const T *)c->f[LP_CS_OPT2],
                        (const T *)c->f[LP_NU_T], (const T *)c->f[LP_TXZ], g, c->d_stats);
-    hipLaunchKernelGGL((k_plane_stats<T, 1>), dim3(nl), dim3(256), 0, c->s, (const T *)c->f[LP_U],
-                       (const T *)c->f[LP_V], (const T *)c->f[LP_W], (const T *)c->f[LP_CS_OPT2],
+    hipLaunchKernelGGL((k_plane_stats<T, 1>), dim3(nl), dim3(256), 0, c->s, (const double *)c->f[LP_U],
+                       (const double *)c->f[LP_V], (const double *)c->f[LP_W], (const T *)c->f[LP_CS_OPT2],
                        (const T *)c->f[LP_NU_T], (const T *)c->f[LP_TXZ], g, c->d_stats);
   });
   HIPCHK(hipGetLastError());
@@ -1893,4 +2086,189 @@ extern "C" int lp_tridag_array(lp_ctx *c, const double *a, const double *b, cons
   HIPCHK(hipGetLastError());
   RCHK(finish_all(c, {&A, &B, &Cc, &R, &U}));
   return check_err_flag(c);
+}
+
+// ------------------------------------------------------------------------------------
+// frozen-coefficient steps and their discrete adjoint
+// ------------------------------------------------------------------------------------
+extern "C" int lp_set_frozen(lp_ctx *c, const void *nut, const void *ust) {
+  if (!c) return set_err(LP_ERR_ARG, "lp_set_frozen: null context");
+  if ((nut == nullptr) != (ust == nullptr))
+    return set_err(LP_ERR_ARG, "lp_set_frozen: give both Nu_t and u*, or neither");
+  if (nut && (!is_device_ptr(nut) || !is_device_ptr(ust)))
+    return set_err(LP_ERR_ARG, "lp_set_frozen: Nu_t and u* must be device arrays");
+  c->frz_nut = nut;
+  c->frz_ust = ust;
+  return LP_OK;
+}
+
+namespace {
+// the adjoint's binary64 scratch: NV volumes and NB 3/2-grid volumes, then lambda
+constexpr int kAdjNV = 52, kAdjNB = 15;
+}  // namespace
+
+// main.f90:252-826 with the frozen Nu_t / u*, transposed (the oracle's or_step_adjoint,
+// kernel for kernel): ub / rb hold on entry the adjoints of the step's u, v, w and of the
+// RHS it leaves, on return those of the u, v, w it starts from and of the RHS_f it reads.
+static int do_step_adjoint(lp_ctx *c, const double *nut, const double *ust, double *ub, double *rb) {
+  const lp_grid &gg = c->g;
+  const lp_config &cf = c->cfg;
+  const long nv = vol_el(c), np = plane_el(c);
+  const long npb = (long)gg.ld_big * gg.ny2, nvb = npb * gg.nz;
+  const long nh = (long)gg.lh * gg.ny;
+  const size_t need = sizeof(double) * ((size_t)kAdjNV * nv + (size_t)kAdjNB * nvb) + sizeof(double) * 2 * nh * (gg.nz + 1);
+  if (!c->adj_block) {
+    RCHK(dalloc((void **)&c->adj_block, need));
+    c->adj_bytes = need;
+  }
+  double *V[kAdjNV], *B[kAdjNB];
+  for (int q = 0; q < kAdjNV; ++q) V[q] = c->adj_block + (size_t)q * nv;
+  for (int q = 0; q < kAdjNB; ++q) B[q] = c->adj_block + (size_t)kAdjNV * nv + (size_t)q * nvb;
+  cx<double> *lam = (cx<double> *)(c->adj_block + (size_t)kAdjNV * nv + (size_t)kAdjNB * nvb);
+  AdjGeo ag{gg.nx, gg.ny, gg.nz, gg.ld, np};
+  const dim3 gv = grid1(nv);
+  const double dz = gg.dz, dt = cf.dt, cst = 1.0 / ((double)gg.nx * gg.ny);
+  const int first = c->jt == 1 && !cf.initu;
+  const int wall = cf.lbc_mom == LP_LBC_WALL;
+  const double zl = 0.5 * dz - cf.d0, den = std::log(zl / cf.z0);
+  const double *au = ub, *av = ub + nv, *aw = ub + 2 * nv;
+  double *rbx = rb, *rby = rb + nv, *rbz = rb + 2 * nv;
+  // ---- base intermediates from the context's u, v, w (left untouched) ----
+  double *uf = V[0], *vf = V[1], *wf = V[2], *ux = V[3], *uy = V[4], *vx = V[5], *vy = V[6], *wx = V[7];
+  double *wy = V[8], *uz = V[9], *vz = V[10], *wz = V[11], *om1 = V[12], *om2 = V[13], *om3 = V[14];
+  HIPCHK(hipMemcpyAsync(uf, c->f[LP_U], sizeof(double) * nv, hipMemcpyDeviceToDevice, c->s));
+  HIPCHK(hipMemcpyAsync(vf, c->f[LP_V], sizeof(double) * nv, hipMemcpyDeviceToDevice, c->s));
+  HIPCHK(hipMemcpyAsync(wf, c->f[LP_W], sizeof(double) * nv, hipMemcpyDeviceToDevice, c->s));
+  RCHK(do_filt_da(c, uf, ux, uy, gg.nz));
+  RCHK(do_filt_da(c, vf, vx, vy, gg.nz));
+  RCHK(do_filt_da(c, wf, wx, wy, gg.nz));
+  RCHK(do_ddz_uv(c, uz, uf));
+  RCHK(do_ddz_uv(c, vz, vf));
+  RCHK(do_ddz_w(c, wz, wf));
+  hipLaunchKernelGGL(k_adj_base, gv, dim3(256), 0, c->s, uf, vf, uz, vz, uy, vx, wx, wy, om1, om2, om3, ag,
+                     1.0 / (zl * den), wall);
+  HIPCHK(hipGetLastError());
+  double *U1 = B[0], *U2 = B[1], *U3 = B[2], *O1 = B[3], *O2 = B[4], *O3 = B[5];
+  RCHK(to_big(c, uf, gg.nz, U1));
+  RCHK(to_big(c, vf, gg.nz, U2));
+  RCHK(to_big(c, wf, gg.nz, U3));
+  RCHK(to_big(c, om1, gg.nz, O1));
+  RCHK(to_big(c, om2, gg.nz, O2));
+  RCHK(to_big(c, om3, gg.nz, O3));
+  // ---- projection, boundary values, RHS -= grad p ----
+  double *usx = V[15], *usy = V[16], *usz = V[17], *apx = V[18], *apy = V[19], *apz = V[20];
+  double *arx = V[21], *ary = V[22], *arz = V[23], *aft = V[24], *aftv = V[25], *ap = V[26];
+  hipLaunchKernelGGL(k_adj_project, gv, dim3(256), 0, c->s, au, av, aw, rbx, rby, rbz, usx, usy, usz, apx, apy, apz,
+                     arx, ary, arz, aft, aftv, ag, dt, cf.use_mean_p_force, cf.lbc_mom == LP_LBC_STRESS_FREE ? 1 : 0);
+  hipLaunchKernelGGL(k_adj_dpdz, gv, dim3(256), 0, c->s, apz, ap, ag, 1.0 / dz);
+  HIPCHK(hipGetLastError());
+  // ---- pressure: c2r^T of p, dpdx, dpdy, the transposed solves, r2c^T of H, wall, lid ----
+  const long ps = c->sp * gg.ny;
+  RCHK(r2c_small(c, ap, gg.nz, 1.0, c->spec[0]));
+  RCHK(r2c_small(c, apx, gg.nz - 1, 1.0, c->spec[1]));
+  RCHK(r2c_small(c, apy, gg.nz - 1, 1.0, c->spec[2]));
+  for (int q = 0; q < 3; ++q) {
+    ColOut o{c->spec[q], M_ONE};
+    RCHK(col_small(c, c->spec[q], q == 0 ? gg.nz : gg.nz - 1, 1, 0, 1, &o));
+  }
+  const dim3 gm((unsigned)((nh + 63) / 64));
+  hipLaunchKernelGGL(k_press_adj, gm, dim3(64), 0, c->s, (const cx<double> *)c->spec[0],
+                     (const cx<double> *)c->spec[1], (const cx<double> *)c->spec[2], c->kx, c->ky,
+                     (cx<double> *)c->Hs[0], (cx<double> *)c->Hs[1], (cx<double> *)c->Hs[2], (cx<double> *)c->bw,
+                     (cx<double> *)c->tw, c->gam, lam, gg.lh, gg.ny, gg.nz, c->sp, dz, cst / (1.5 * dt), cst);
+  HIPCHK(hipGetLastError());
+  double *hx = V[27], *hy = V[28], *hz = V[29], *bwb = V[30], *twb = V[31];
+  double *hout[3] = {hx, hy, hz};
+  for (int q = 0; q < 3; ++q) {
+    ColOut o{c->Hs[q], M_ONE};
+    RCHK(col_small(c, c->Hs[q], gg.nz - 1, 0, 1, 1, &o));
+    RCHK(c2r_small(c, c->Hs[q], gg.lh, gg.nz - 1, hout[q], F64));
+  }
+  void *wl[2] = {c->bw, c->tw};
+  double *wlo[2] = {bwb, twb};
+  for (int q = 0; q < 2; ++q) {
+    ColOut o{wl[q], M_ONE};
+    RCHK(col_small(c, wl[q], 1, 0, 1, 1, &o));
+    RCHK(c2r_small(c, wl[q], gg.lh, 1, wlo[q], F64));
+  }
+  (void)ps;
+  // ---- AB2, RHS assembly, Euler start ----
+  double *afu = V[32], *afv = V[33], *afw = V[34], *acx = V[35], *acy = V[36], *acz = V[37];
+  double *adx = V[38], *ady = V[39], *adz = V[40];
+  hipLaunchKernelGGL(k_adj_ab2, gv, dim3(256), 0, c->s, usx, usy, usz, hx, hy, hz, arx, ary, arz, aft, aftv, bwb, twb,
+                     afu, afv, afw, acx, acy, acz, adx, ady, adz, rbx, rby, rbz, ag, dt, first);
+  HIPCHK(hipGetLastError());
+  // ---- convection: P^T, the products transposed, T^T ----
+  double *pbx = B[6], *pby = B[7], *pbz = B[8];
+  RCHK(to_big(c, acx, gg.nz, pbx));
+  RCHK(to_big(c, acy, gg.nz, pby));
+  RCHK(to_big(c, acz, gg.nz, pbz));
+  double *aU1 = B[9], *aU2 = B[10], *aU3 = B[11], *aO1 = B[12], *aO2 = B[13], *aO3 = B[14];
+  const double tt = ((double)gg.nx * gg.ny) / ((double)gg.nx2 * gg.ny2);
+  hipLaunchKernelGGL(k_adj_products, grid1(nvb), dim3(256), 0, c->s, pbx, pby, pbz, U1, U2, U3, O1, O2, O3, aU1, aU2,
+                     aU3, aO1, aO2, aO3, npb, gg.nz, tt);
+  HIPCHK(hipGetLastError());
+  double *t1 = V[41], *ao1 = V[42], *ao2 = V[43], *ao3 = V[44];
+  double *aUb[3] = {aU1, aU2, aU3}, *afq[3] = {afu, afv, afw};
+  for (int q = 0; q < 3; ++q) {  // P^T = (nx2 ny2)/(nx ny) T: truncation scaled by 1/(nx ny)
+    RCHK(from_big(c, aUb[q], gg.nz, cst, t1));
+    hipLaunchKernelGGL(k_add_plane, gv, dim3(256), 0, c->s, afq[q], t1, nv);
+  }
+  RCHK(from_big(c, aO1, gg.nz, cst, ao1));
+  RCHK(from_big(c, aO2, gg.nz, cst, ao2));
+  RCHK(from_big(c, aO3, gg.nz, cst, ao3));
+  // ---- stress divergences: the x, y derivatives of their adjoints ----
+  double *dxa = V[45], *dya = V[46], *dxb = V[47], *dyb = V[48], *dxc = V[49], *dyc = V[50];
+  const int mxy[2] = {M_IKX, M_IKY};
+  void *oa[2] = {dxa, dya}, *ob[2] = {dxb, dyb}, *oc[2] = {dxc, dyc};
+  RCHK(spectral_apply(c, adx, gg.nz, cst, 2, mxy, oa));
+  RCHK(spectral_apply(c, ady, gg.nz, cst, 2, mxy, ob));
+  RCHK(spectral_apply(c, adz, gg.nz, cst, 2, mxy, oc));
+  // ---- stresses with the frozen Nu_t -> strain-rate adjoints (reuse the base volumes) ----
+  double *aS11 = V[15], *aS12 = V[16], *aS13 = V[17], *aS22 = V[18], *aS23 = V[19], *aS33 = V[20];
+  double *wtx = V[21], *wty = V[22];
+  hipLaunchKernelGGL(k_adj_tau, gv, dim3(256), 0, c->s, adx, ady, adz, dxa, dya, dxb, dyb, dxc, dyc, nut, ust, aS11,
+                     aS12, aS13, aS22, aS23, aS33, wtx, wty, ag, 1.0 / dz, cf.vonk / den, wall);
+  HIPCHK(hipGetLastError());
+  if (wall) {  // the wall law's test filter (symmetric) on the txz(1), tyz(1) adjoints
+    const int mg[1] = {M_G1};
+    void *o1[1] = {t1};
+    RCHK(spectral_apply(c, wtx, 1, 1.0, 1, mg, o1));
+    hipLaunchKernelGGL(k_add_plane, grid1(np), dim3(256), 0, c->s, afu, t1, np);
+    RCHK(spectral_apply(c, wty, 1, 1.0, 1, mg, o1));
+    hipLaunchKernelGGL(k_add_plane, grid1(np), dim3(256), 0, c->s, afv, t1, np);
+    HIPCHK(hipGetLastError());
+  }
+  double *aux = V[3], *auy = V[4], *avx = V[5], *avy = V[6], *awx = V[7], *awy = V[8];
+  hipLaunchKernelGGL(k_adj_strain, gv, dim3(256), 0, c->s, aS11, aS12, aS13, aS22, aS23, aS33, ao1, ao2, ao3, aux, auy,
+                     avx, avy, awx, awy, afu, afv, afw, ag, 1.0 / dz, 1.0 / (zl * den), wall);
+  HIPCHK(hipGetLastError());
+  // ---- filt_da transposed: F(af) - Dx(a*x) - Dy(a*y) ----
+  double *ff = V[9], *gx = V[10], *gy = V[11];
+  double *axq[3] = {aux, avx, awx}, *ayq[3] = {auy, avy, awy};
+  const int m1[1] = {M_ONE}, mx[1] = {M_IKX}, my[1] = {M_IKY};
+  for (int q = 0; q < 3; ++q) {
+    void *of[1] = {ff}, *ox[1] = {gx}, *oy[1] = {gy};
+    RCHK(spectral_apply(c, afq[q], gg.nz, cst, 1, m1, of));
+    RCHK(spectral_apply(c, axq[q], gg.nz, cst, 1, mx, ox));
+    RCHK(spectral_apply(c, ayq[q], gg.nz, cst, 1, my, oy));
+    hipLaunchKernelGGL(k_adj_combine, gv, dim3(256), 0, c->s, ub + (size_t)q * nv, ff, gx, gy, ag);
+    HIPCHK(hipGetLastError());
+  }
+  return LP_OK;
+}
+
+extern "C" int lp_step_adjoint(lp_ctx *c, const double *nut, const double *ust, double *ub, double *rb) {
+  if (!c || !nut || !ust || !ub || !rb) return set_err(LP_ERR_ARG, "lp_step_adjoint: null argument");
+  if (c->fdt != F64)
+    return set_err(LP_ERR_ARG, "lp_step_adjoint runs in binary64: use an LP_FP64 context (the base states of a "
+                               "binary32 run carry over, u, v, w being binary64 in both)");
+  if (c->cfg.drop.npcon > 0 && c->cfg.drop.active)
+    return set_err(LP_ERR_ARG, "lp_step_adjoint: active droplets (buoyancy feedback) are not part of the adjoint");
+  HIPCHK(hipSetDevice(c->dev));
+  if (!is_device_ptr(nut) || !is_device_ptr(ust) || !is_device_ptr(ub) || !is_device_ptr(rb))
+    return set_err(LP_ERR_ARG, "lp_step_adjoint: device arrays expected");
+  RCHK(do_step_adjoint(c, nut, ust, ub, rb));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return LP_OK;
 }

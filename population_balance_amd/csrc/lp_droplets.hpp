@@ -4,7 +4,7 @@
 // feeds its inertia term (main.f90:864-898).
 //
 // The droplet state (PCon, RHS_PCon, RHS_PCon_f, Rhs_pbe) and all droplet arithmetic are
-// binary64 in both precisions; T is the velocity-field type.  Bin b of a droplet field
+// binary64 in both precisions, and so are u, v, w; T is the type of the other fields.  Bin b of a droplet field
 // is the (ld, ny, nz) volume at offset b*nv.
 #pragma once
 #include "lp_kernels.hpp"
@@ -24,13 +24,13 @@ struct DropGeo {
 // i.e. the exclusive prefix sum of dx dudx shifted to the mean of u.  u_int0(nx+1) =
 // u_int0(1).  One 64-lane wavefront per (y, z) row, four rows per block.
 template <typename T>
-__global__ __launch_bounds__(256) void k_face_u(const T *__restrict__ u, const T *__restrict__ dudx,
+__global__ __launch_bounds__(256) void k_face_u(const double *__restrict__ u, const T *__restrict__ dudx,
                                                 double *__restrict__ ui0, DropGeo g, double dx) {
   const int lane = threadIdx.x & 63;
   const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= (long)g.ny * g.nz) return;
   const int k = (int)(row / g.ny), j = (int)(row % g.ny);
-  const T *ur = u + (long)k * g.np + (long)j * g.ld;
+  const double *ur = u + (long)k * g.np + (long)j * g.ld;
   const T *er = dudx + (long)k * g.np + (long)j * g.ld;
   double *dst = ui0 + (long)k * g.fp + (long)j * g.ld;
   double su = 0.0, sp = 0.0, carry = 0.0;
@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) void k_face_u(const T *__restrict__ u, const T
 // are combined through LDS.
 static constexpr int kFaceSeg = 8;
 template <typename T>
-__global__ __launch_bounds__(64 * kFaceSeg) void k_face_v(const T *__restrict__ v, const T *__restrict__ dvdy,
+__global__ __launch_bounds__(64 * kFaceSeg) void k_face_v(const double *__restrict__ v, const T *__restrict__ dvdy,
                                                          double *__restrict__ vi0, DropGeo g, double dy) {
   __shared__ double E[kFaceSeg][64], SU[kFaceSeg][64], SP[kFaceSeg][64];
   const int x = threadIdx.x & 63, seg = threadIdx.x >> 6;
@@ -72,7 +72,7 @@ __global__ __launch_bounds__(64 * kFaceSeg) void k_face_v(const T *__restrict__ 
   const bool on = i < g.nx;
   const int L = (g.ny + kFaceSeg - 1) / kFaceSeg;
   const int m0 = seg * L, m1 = m0 + L < g.ny ? m0 + L : g.ny;
-  const T *vc = v + (long)k * g.np + (on ? i : 0);
+  const double *vc = v + (long)k * g.np + (on ? i : 0);
   const T *ec = dvdy + (long)k * g.np + (on ? i : 0);
   auto e_at = [&](int m) -> double { return (m < g.ny - 1) ? dy * (double)ec[(long)m * g.ld] : 0.0; };
   double P = 0.0, su = 0.0, sp = 0.0;  // segment-local exclusive prefix and sums
@@ -164,7 +164,7 @@ static constexpr int kPollenBins = 8;  // bins per thread
 template <typename T>
 __global__ __launch_bounds__(256) void k_pollen_rhs(
     const double *__restrict__ PC, double *__restrict__ RH, const double *__restrict__ RF,
-    const double *__restrict__ ui0, const double *__restrict__ vi0, const T *__restrict__ w,
+    const double *__restrict__ ui0, const double *__restrict__ vi0, const double *__restrict__ w,
     const T *__restrict__ dudt, const T *__restrict__ dvdt, const T *__restrict__ dwdt, const double *__restrict__ kc,
     const double *__restrict__ wsb, const double *__restrict__ qsrc, const double *__restrict__ foot, DropGeo g,
     PollenParams P) {
@@ -495,7 +495,7 @@ __global__ void k_beta_step(double *__restrict__ PC, const double *__restrict__ 
 
 // energy.f90:23-31 kinetic energy on the uvp nodes (zero outside the cells)
 template <typename T>
-__global__ void k_energy(const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+__global__ void k_energy(const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
                          T *__restrict__ ke, Geo g) {
   const long i = gtid();
   if (i >= g.np * g.nz) return;
@@ -504,8 +504,8 @@ __global__ void k_energy(const T *__restrict__ u, const T *__restrict__ v, const
     ke[i] = T(0);
     return;
   }
-  const T tw = T(0.5) * (w[i] + w[i + g.np]);
-  ke[i] = T(0.5) * (u[i] * u[i] + v[i] * v[i] + tw * tw);
+  const double tw = 0.5 * (w[i] + w[i + g.np]);
+  ke[i] = (T)(0.5 * (u[i] * u[i] + v[i] * v[i] + tw * tw));
 }
 
 // main.f90:883-885 with ddz_uv(dkedz, ke) folded in (ddz_uv.f90:39-57: level 1 and the

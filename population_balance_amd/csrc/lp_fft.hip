@@ -44,14 +44,19 @@ static unsigned r2c_grid(long ngroups, bool persistent) {
 // derivative volumes a, b (convec.f90:100-133): 1 w_y - v_z (a = dwdy, b = dvdz), 2
 // u_z - w_x (a = dwdx, b = dudz), 3 v_x - u_y (a = dvdx, b = dudy); on plane 0 (jz = 1)
 // w_y and w_x are averaged with plane 1 onto the uvp node.  a, b have in's layout.
-template <typename T, int M, int PRO = 0>
-__global__ __launch_bounds__(kR2cBlock) void k_row_r2c(const T *__restrict__ in, long in_pitch, long in_plane,
-                                                       int rows, int nplanes, T scale, cx<T> *__restrict__ out,
+// TI is the type of `in` (binary64 velocities into a binary32 transform).  SH: every row
+// of a plane enters shifted by the plane's first value, (in - in[plane, 0, 0]) * scale,
+// which only moves the plane's kx = ky = 0 mode; the row y = 0 stores that value in
+// shift_out[plane] when shift_out is given.
+template <typename T, int M, int PRO = 0, typename TI = T, bool SH = false>
+__global__ __launch_bounds__(kR2cBlock) void k_row_r2c(const TI *__restrict__ in, long in_pitch, long in_plane,
+                                                       int rows, int nplanes, TI scale, cx<T> *__restrict__ out,
                                                        long out_pitch, long out_plane, int ncols_out,
                                                        const cx<T> *__restrict__ twM,
                                                        const cx<T> *__restrict__ twN,
                                                        const T *__restrict__ pa = nullptr,
-                                                       const T *__restrict__ pb = nullptr) {
+                                                       const T *__restrict__ pb = nullptr,
+                                                       double *__restrict__ shift_out = nullptr) {
   constexpr int P = row_lanes(M);
   using F = Fft4<T, M, P, -1>;
   constexpr int Q = F::Q;
@@ -69,11 +74,20 @@ __global__ __launch_bounds__(kR2cBlock) void k_row_r2c(const T *__restrict__ in,
     const bool valid = grow < total;
     const long plane = valid ? grow / rows : 0, y = valid ? grow % rows : 0;
     if constexpr (PRO == 0) {
-      const T *src = in + plane * in_plane + y * in_pitch;
+      using vi2 = typename V2<TI>::t;
+      const TI *src = in + plane * in_plane + y * in_pitch;
+      TI s0 = TI(0);
+      if constexpr (SH) s0 = in[plane * in_plane];
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
-        const v2 p = *reinterpret_cast<const v2 *>(src + 2 * (t + P * q));
-        v[q] = mk<T>(p.x * scale, p.y * scale);
+        const vi2 p = *reinterpret_cast<const vi2 *>(src + 2 * (t + P * q));
+        if constexpr (SH)
+          v[q] = mk<T>((T)((p.x - s0) * scale), (T)((p.y - s0) * scale));
+        else
+          v[q] = mk<T>((T)(p.x * scale), (T)(p.y * scale));
+      }
+      if constexpr (SH) {
+        if (shift_out && valid && y == 0 && t == 0) shift_out[plane] = (double)s0;
       }
     } else {
       const long o = plane * in_plane + y * in_pitch;
@@ -160,6 +174,8 @@ template <typename T> __device__ __forceinline__ cx<T> merge_c2r(cx<T> xa, cx<T>
 // on levels 1..nz-2.  The padding columns are zero (BOGUS on the uv top level).  tz has the
 // layout of out.  ikx (optional): the input is multiplied by i*ikx[kx] first, which turns
 // the (kx, y) spectrum of f into that of df/dx (ddx.f90:15-24; kx is uniform along y).
+// shift (optional, EPI 0): shift[plane] is added to every stored value of the plane in
+// binary64 (the inverse of k_row_r2c's SH; padding columns stay zero).
 template <typename T, typename TO, int M, int EPI>
 __global__ __launch_bounds__(kRowBlock) void k_row_c2r(const cx<T> *__restrict__ in, long in_pitch, long in_plane,
                                                        int ncols_in, int rows, int nplanes,
@@ -167,7 +183,8 @@ __global__ __launch_bounds__(kRowBlock) void k_row_c2r(const cx<T> *__restrict__
                                                        const cx<T> *__restrict__ twM,
                                                        const cx<T> *__restrict__ twN,
                                                        const TO *__restrict__ tz, TO cst,
-                                                       const double *__restrict__ ikx) {
+                                                       const double *__restrict__ ikx,
+                                                       const double *__restrict__ shift) {
   constexpr int P = row_lanes(M);
   using F = Fft4<T, M, P, +1>;
   constexpr int Q = F::Q;
@@ -208,6 +225,7 @@ __global__ __launch_bounds__(kRowBlock) void k_row_c2r(const cx<T> *__restrict__
   const bool top = plane == nplanes - 1;
   const bool mid = plane >= 1 && plane <= nplanes - 2;
   const long up = top ? 0 : out_plane, dn = mid ? out_plane : 0;  // neighbour levels, clamped
+  const double sh = (EPI == 0 && shift && valid) ? shift[plane] : 0.0;
   auto fin = [&](T v, int x) -> TO {
     if constexpr (EPI == 1) {
       const TO d = cst * (tzr[x + up] - tzr[x]);
@@ -218,6 +236,7 @@ __global__ __launch_bounds__(kRowBlock) void k_row_c2r(const cx<T> *__restrict__
       if (mid) r += d;
       return r;
     } else {
+      if (shift) return (TO)((double)v + sh);  // the plane's shift back, in binary64
       return (TO)v;
     }
   };
@@ -254,6 +273,7 @@ struct ColSlot {
   int tpitch;  // row length of the multiplier tables (lh of the small grid)
   int ncols;   // columns stored (kx < ncols)
   double scale;  // factor on the padded / truncated spectrum (padmode 1, 2)
+  double *dc;    // padmode 1: per-plane kx = ky = 0 entry, removed from the output
 };
 struct ColParams {
   ColSlot outs[3];
@@ -371,6 +391,13 @@ __global__ __launch_bounds__(TX * kColLanes) void k_col_op(
           else if (yo > NYO / 2) x = sp[q + DQ];
         }
         v[q] = scaled ? mk<T>(x.x * osc, x.y * osc) : x;
+      }
+      if constexpr (PM == 1) {
+        double *dc = Pr.outs[jo].dc;
+        if (dc && kx == 0 && t == 0) {  // lane 0 of column 0 holds ky = 0 in v[0]
+          dc[plane] = (double)v[0].x;
+          v[0] = mk<T>(0, 0);
+        }
       }
     }
     // addresses are formed from per-output values so that they are not hoisted out of
@@ -503,19 +530,20 @@ bool fft_len_supported(int n) {
   }
 }
 
-template <typename T, int M, int PRO = 0>
+template <typename T, int M, int PRO = 0, typename TI = T, bool SH = false>
 static hipError_t launch_r2c(const void *in, long in_pitch, long in_plane, int rows, int nplanes,
                              double scale, void *out, long out_pitch, long out_plane, int ncols_out,
-                             hipStream_t s, Dt dt, const void *pa = nullptr, const void *pb = nullptr) {
+                             hipStream_t s, Dt dt, const void *pa = nullptr, const void *pb = nullptr,
+                             double *shift_out = nullptr) {
   const void *twM = tw_ptr(M, dt), *twN = tw_ptr(2 * M, dt);
   if (!twM || !twN) return hipErrorInvalidValue;
   constexpr int RPB = kR2cBlock / row_lanes(M);
   long total = (long)rows * nplanes;
   if (total < 1) return hipErrorInvalidValue;
   dim3 grid(r2c_grid((total + RPB - 1) / RPB, r2c_persistent<T>()));
-  hipLaunchKernelGGL((k_row_r2c<T, M, PRO>), grid, dim3(kR2cBlock), 0, s, (const T *)in, in_pitch, in_plane, rows,
-                     nplanes, (T)scale, (cx<T> *)out, out_pitch, out_plane, ncols_out,
-                     (const cx<T> *)twM, (const cx<T> *)twN, (const T *)pa, (const T *)pb);
+  hipLaunchKernelGGL((k_row_r2c<T, M, PRO, TI, SH>), grid, dim3(kR2cBlock), 0, s, (const TI *)in, in_pitch,
+                     in_plane, rows, nplanes, (TI)scale, (cx<T> *)out, out_pitch, out_plane, ncols_out,
+                     (const cx<T> *)twM, (const cx<T> *)twN, (const T *)pa, (const T *)pb, shift_out);
   return hipGetLastError();
 }
 
@@ -547,13 +575,25 @@ hipError_t row_r2c_vort(Dt dt, int M, int comp, const void *a, const void *b, lo
 }
 
 hipError_t row_r2c(Dt dt, int M, const void *in, long in_pitch, long in_plane, int rows, int nplanes,
-                   double scale, void *out, long out_pitch, long out_plane, int ncols_out, hipStream_t s) {
+                   double scale, void *out, long out_pitch, long out_plane, int ncols_out, hipStream_t s,
+                   RowIn ri) {
+  const Dt din = ri.din < 0 ? dt : (Dt)ri.din;
+  if (ri.shift && !(dt == F32 && din == F64)) return hipErrorInvalidValue;
+  if (dt == F64 && din == F32) return hipErrorInvalidValue;
 #define CASE(m)                                                                                        \
   case m:                                                                                              \
-    return dt == F64 ? launch_r2c<double, m>(in, in_pitch, in_plane, rows, nplanes, scale, out,        \
-                                             out_pitch, out_plane, ncols_out, s, dt)                   \
-                     : launch_r2c<float, m>(in, in_pitch, in_plane, rows, nplanes, scale, out,         \
-                                            out_pitch, out_plane, ncols_out, s, dt);
+    if (dt == F64)                                                                                     \
+      return launch_r2c<double, m>(in, in_pitch, in_plane, rows, nplanes, scale, out, out_pitch,       \
+                                   out_plane, ncols_out, s, dt);                                       \
+    if (din == F32)                                                                                    \
+      return launch_r2c<float, m>(in, in_pitch, in_plane, rows, nplanes, scale, out, out_pitch,        \
+                                  out_plane, ncols_out, s, dt);                                        \
+    if (ri.shift)                                                                                      \
+      return launch_r2c<float, m, 0, double, true>(in, in_pitch, in_plane, rows, nplanes, scale, out,  \
+                                                   out_pitch, out_plane, ncols_out, s, dt, nullptr,    \
+                                                   nullptr, ri.shift_out);                             \
+    return launch_r2c<float, m, 0, double, false>(in, in_pitch, in_plane, rows, nplanes, scale, out,   \
+                                                  out_pitch, out_plane, ncols_out, s, dt);
   switch (M) {
     LP_ROW_LENGTHS(CASE)
     default: return hipErrorInvalidValue;
@@ -564,7 +604,8 @@ hipError_t row_r2c(Dt dt, int M, const void *in, long in_pitch, long in_plane, i
 template <typename T, typename TO, int M, int EPI = 0>
 static hipError_t launch_c2r(const void *in, long in_pitch, long in_plane, int ncols_in, int rows,
                              int nplanes, void *out, long out_pitch, long out_plane, hipStream_t s, Dt dt,
-                             const void *tz = nullptr, double cst = 0.0, const double *ikx = nullptr) {
+                             const void *tz = nullptr, double cst = 0.0, const double *ikx = nullptr,
+                             const double *shift = nullptr) {
   const void *twM = tw_ptr(M, dt), *twN = tw_ptr(2 * M, dt);
   if (!twM || !twN) return hipErrorInvalidValue;
   constexpr int RPB = kRowBlock / row_lanes(M);
@@ -572,7 +613,7 @@ static hipError_t launch_c2r(const void *in, long in_pitch, long in_plane, int n
   dim3 grid((unsigned)((total + RPB - 1) / RPB));
   hipLaunchKernelGGL((k_row_c2r<T, TO, M, EPI>), grid, dim3(kRowBlock), 0, s, (const cx<T> *)in, in_pitch,
                      in_plane, ncols_in, rows, nplanes, (TO *)out, out_pitch, out_plane, (const cx<T> *)twM,
-                     (const cx<T> *)twN, (const TO *)(EPI ? tz : out), (TO)cst, ikx);
+                     (const cx<T> *)twN, (const TO *)(EPI ? tz : out), (TO)cst, ikx, shift);
   return hipGetLastError();
 }
 
@@ -600,7 +641,8 @@ hipError_t row_c2r_div(Dt dt, int M, const void *in, long in_pitch, long in_plan
 
 hipError_t row_c2r(Dt dtc, Dt dto, int M, const void *in, long in_pitch, long in_plane, int ncols_in,
                    int rows, int nplanes, void *out, long out_pitch, long out_plane, hipStream_t s,
-                   const double *ikx) {
+                   const double *ikx, const double *shift) {
+  if (shift && !(dtc == F32 && dto == F64)) return hipErrorInvalidValue;
 #define CASE(m)                                                                                         \
   case m:                                                                                               \
     if (dtc == F64 && dto == F64)                                                                       \
@@ -612,6 +654,9 @@ hipError_t row_c2r(Dt dtc, Dt dto, int M, const void *in, long in_pitch, long in
     if (dtc == F32 && dto == F32)                                                                       \
       return launch_c2r<float, float, m>(in, in_pitch, in_plane, ncols_in, rows, nplanes, out,          \
                                          out_pitch, out_plane, s, dtc, nullptr, 0.0, ikx);              \
+    if (dtc == F32 && dto == F64)                                                                       \
+      return launch_c2r<float, double, m>(in, in_pitch, in_plane, ncols_in, rows, nplanes, out,         \
+                                          out_pitch, out_plane, s, dtc, nullptr, 0.0, ikx, shift);      \
     return hipErrorInvalidValue;
   switch (M) {
     LP_ROW_LENGTHS(CASE)
@@ -635,13 +680,14 @@ static hipError_t launch_col(int padmode, int fwd, int inv, const void *in, long
   P.nout = nout;
   for (int j = 0; j < 3; ++j) {
     if (j >= nout) {
-      P.outs[j] = ColSlot{nullptr, 0, 0, 0, 0, 0, 1.0};
+      P.outs[j] = ColSlot{nullptr, 0, 0, 0, 0, 0, 1.0, nullptr};
       continue;
     }
     const ColOut &o = outs[j];
     if (padmode == 0 && o.scale != 1.0) return hipErrorInvalidValue;
+    if (padmode != 1 && o.dc) return hipErrorInvalidValue;
     P.outs[j] = ColSlot{o.ptr, o.pitch ? o.pitch : out_pitch, o.plane ? o.plane : out_plane, o.mult, tab.lh,
-                        o.ncols ? o.ncols : ncols, o.scale};
+                        o.ncols ? o.ncols : ncols, o.scale, o.dc};
   }
   if (col0 > 0) {
     if constexpr (NYI == NYO) {

@@ -25,19 +25,32 @@ struct SpecTables {
   int lh, ny;                        // small-grid sizes
 };
 
+// Input of a row pass: din is the type of the rows (-1: the transform's type; F64 into an
+// F32 transform reads binary64 velocities).  shift (F64 into F32 only): each plane enters
+// less its first value, stored in shift_out[plane] when shift_out is given; only the
+// plane's kx = ky = 0 mode moves, by that value.
+struct RowIn {
+  int din = -1;
+  bool shift = false;
+  double *shift_out = nullptr;
+};
+
 // Row pass: real rows of length 2M (scaled by `scale`) -> complex half spectra
 // (M+1 values, only the first ncols_out written).
 hipError_t row_r2c(Dt dt, int M, const void *in, long in_pitch, long in_plane,
                    int rows, int nplanes, double scale, void *out, long out_pitch,
-                   long out_plane, int ncols_out, hipStream_t s);
+                   long out_plane, int ncols_out, hipStream_t s, RowIn ri = {});
 
 // Row pass: complex half spectra (ncols_in columns present, rest zero) -> real rows
 // of length 2M; columns [2M, out_pitch) set to zero.  Compute type dtc, store type dto.
 // ikx (optional, M+1 values, zero at kx = 0 and kx = M): the spectra are multiplied by
 // i*ikx[kx] first, so the rows are the x-derivative of what they would be without it.
+// shift (optional, dtc F32 -> dto F64): shift[plane] is added to every value of the plane
+// in binary64, undoing row_r2c's shift.
 hipError_t row_c2r(Dt dtc, Dt dto, int M, const void *in, long in_pitch, long in_plane,
                    int ncols_in, int rows, int nplanes, void *out, long out_pitch,
-                   long out_plane, hipStream_t s, const double *ikx = nullptr);
+                   long out_plane, hipStream_t s, const double *ikx = nullptr,
+                   const double *shift = nullptr);
 
 // row_r2c of a vorticity component formed from two derivative volumes (comp 0: w_y - v_z
 // from a = dwdy, b = dvdz; 1: u_z - w_x from a = dwdx, b = dudz; 2: v_x - u_y from
@@ -53,6 +66,8 @@ struct ColOut {
   long plane = 0;
   int ncols = 0;
   double scale = 1.0;  // padmode 1 and 2 only: factor on the output spectrum
+  double *dc = nullptr;  // padmode 1 only: the kx = ky = 0 entry of each plane is stored
+                         // here (real part, after scale) and left out of the output
 };
 
 // Column pass over ncols kx-columns of nplanes spectra:
@@ -92,5 +107,8 @@ hipError_t row_c2r_div(Dt dt, int M, const void *in, long in_pitch, long in_plan
                        int is_w, hipStream_t s);
 
 bool fft_len_supported(int n);
+
+// record the message that lp_last_error() returns; returns code
+int set_error(int code, const char *fmt, ...);
 
 }  // namespace lp

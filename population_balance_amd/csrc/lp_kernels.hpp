@@ -1,7 +1,7 @@
 // lp_kernels.hpp -- pointwise / stencil / gather / tridiagonal kernels of the LES step
 // (everything that is not an FFT pass).  Each kernel cites the reference lines it
 // implements.  T is the field type (double for LP_FP64, float for LP_MIXED); the
-// Lagrangian F_* state, the pressure solve and p are always double.
+// Lagrangian F_* state, the pressure solve, p and the velocities u, v, w are always double.
 #pragma once
 #include "lp_device.hpp"
 
@@ -14,30 +14,95 @@ struct Geo {
 
 __device__ __forceinline__ long gtid() { return (long)blockIdx.x * blockDim.x + threadIdx.x; }
 
-// ddz_uv.f90:39-57 ; level 1 untouched
-template <typename T>
-__global__ void k_ddz_uv(T *__restrict__ dfdz, const T *__restrict__ f, Geo g, T cst) {
+// ddz_uv.f90:39-57 ; level 1 untouched.  TI: type of f (binary64 velocities), the
+// difference is formed in TI and stored in T.
+template <typename T, typename TI = T>
+__global__ void k_ddz_uv(T *__restrict__ dfdz, const TI *__restrict__ f, Geo g, TI cst) {
   long i = gtid();
   if (i >= g.np * g.nz) return;
   int k = (int)(i / g.np), x = (int)(i % g.ld);
   if (k == g.nz - 1) dfdz[i] = T(0);
-  else if (k >= 1 && x < g.nx) dfdz[i] = cst * (f[i] - f[i - g.np]);
+  else if (k >= 1 && x < g.nx) dfdz[i] = (T)(cst * (f[i] - f[i - g.np]));
 }
 
 // ddz_w.f90:24-44
-template <typename T>
-__global__ void k_ddz_w(T *__restrict__ dfdz, const T *__restrict__ f, Geo g, T cst) {
+template <typename T, typename TI = T>
+__global__ void k_ddz_w(T *__restrict__ dfdz, const TI *__restrict__ f, Geo g, TI cst) {
   long i = gtid();
   if (i >= g.np * g.nz) return;
   int k = (int)(i / g.np), x = (int)(i % g.ld);
   if (k == g.nz - 1) dfdz[i] = T(0);
-  else if (x < g.nx) dfdz[i] = cst * (f[i + g.np] - f[i]);
+  else if (x < g.nx) dfdz[i] = (T)(cst * (f[i + g.np] - f[i]));
+}
+
+// filt_da.f90's zeroing of the kx = nx/2 and ky = ny/2 modes ("oddballs") on a binary64
+// field, exactly, without a round trip through binary32 transforms.  Per level those
+// modes sum to
+//   (-1)^x A(y) + (-1)^y B(x) - (-1)^(x+y) C,
+//   A(y) = sum_x (-1)^x f(x, y) / nx,  B(x) = sum_y (-1)^y f(x, y) / ny,
+//   C = sum_y (-1)^y A(y) / ny  (the corner mode, counted in both sums).
+// k_nyq_rows: one wave per (row, level) -> A.
+static __global__ __launch_bounds__(256) void k_nyq_rows(const double *__restrict__ f, double *__restrict__ A, Geo g,
+                                                  int nlev) {
+  const long row = (long)blockIdx.x * 4 + threadIdx.x / 64;  // y + ny * level
+  const int lane = threadIdx.x % 64;
+  if (row >= (long)g.ny * nlev) return;
+  const double *r = f + (row / g.ny) * g.np + (row % g.ny) * g.ld;
+  double acc = 0.0;
+  for (int x = lane; x < g.nx; x += 64) acc += (x & 1) ? -r[x] : r[x];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (lane == 0) A[row] = acc / g.nx;
+}
+
+// k_nyq_cols: one block per (64 columns, level), four waves over interleaved rows -> B;
+// the block of column 0 also forms C from A.
+static __global__ __launch_bounds__(256) void k_nyq_cols(const double *__restrict__ f, const double *__restrict__ A,
+                                                  double *__restrict__ B, double *__restrict__ Cc, Geo g) {
+  __shared__ double part[4][64];
+  const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+  const int x = blockIdx.x * 64 + lane;
+  const int k = blockIdx.y;
+  const double *p = f + (long)k * g.np;
+  double acc = 0.0;
+  if (x < g.nx) {
+    int y = wv;
+    for (; y + 12 < g.ny; y += 16) {  // four rows in flight per lane
+      const double a0 = p[(long)y * g.ld + x], a1 = p[(long)(y + 4) * g.ld + x];
+      const double a2 = p[(long)(y + 8) * g.ld + x], a3 = p[(long)(y + 12) * g.ld + x];
+      acc += (y & 1) ? -(a0 + a1 + a2 + a3) : (a0 + a1 + a2 + a3);  // y, y+4, ... share parity
+    }
+    for (; y < g.ny; y += 4) acc += (y & 1) ? -p[(long)y * g.ld + x] : p[(long)y * g.ld + x];
+  }
+  part[wv][lane] = acc;
+  __syncthreads();
+  if (wv == 0 && x < g.nx) B[(long)k * g.nx + x] = (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]) / g.ny;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const double *a = A + (long)k * g.ny;
+    double c = 0.0;
+    for (int y = 0; y < g.ny; ++y) c += (y & 1) ? -a[y] : a[y];
+    Cc[k] = c / g.ny;
+  }
+}
+
+// f -= the oddball modes (padding columns x >= nx set to zero, as a back-transform leaves them)
+static __global__ void k_nyq_remove(double *__restrict__ f, const double *__restrict__ A, const double *__restrict__ B,
+                             const double *__restrict__ Cc, Geo g, int nlev) {
+  const long i = gtid();
+  if (i >= g.np * nlev) return;
+  const int x = (int)(i % g.ld), y = (int)((i / g.ld) % g.ny), k = (int)(i / g.np);
+  if (x >= g.nx) {
+    f[i] = 0.0;
+    return;
+  }
+  const double sx = (x & 1) ? -1.0 : 1.0, sy = (y & 1) ? -1.0 : 1.0;
+  f[i] = f[i] - (sx * A[(long)k * g.ny + y] + sy * B[(long)k * g.nx + x] - sx * sy * Cc[k]);
 }
 
 // wallstress.f90:101-185 ('wall', neutral psi_m = 0, phi_m = 1, uniform z0, d0)
 template <typename T>
-__global__ void k_wallstress(const T *__restrict__ u1, const T *__restrict__ v1, const T *__restrict__ u,
-                             const T *__restrict__ v, T *__restrict__ txz, T *__restrict__ tyz,
+__global__ void k_wallstress(const T *__restrict__ u1, const T *__restrict__ v1, const double *__restrict__ u,
+                             const double *__restrict__ v, T *__restrict__ txz, T *__restrict__ tyz,
                              T *__restrict__ dudz, T *__restrict__ dvdz, T *__restrict__ ust, Geo g,
                              double denom, double zl, double vonk) {
   long i = gtid();
@@ -102,13 +167,13 @@ __global__ void k_strain(const T *__restrict__ dudx, const T *__restrict__ dudy,
 
 // sgs_stag.f90:130-134  u_lag = u_lag + u (all columns)
 template <typename T>
-__global__ void k_lag_acc(T *__restrict__ ul, T *__restrict__ vl, T *__restrict__ wl, const T *__restrict__ u,
-                          const T *__restrict__ v, const T *__restrict__ w, long n) {
+__global__ void k_lag_acc(T *__restrict__ ul, T *__restrict__ vl, T *__restrict__ wl, const double *__restrict__ u,
+                          const double *__restrict__ v, const double *__restrict__ w, long n) {
   long i = gtid();
   if (i >= n) return;
-  ul[i] += u[i];
-  vl[i] += v[i];
-  wl[i] += w[i];
+  ul[i] = (T)((double)ul[i] + u[i]);
+  vl[i] = (T)((double)vl[i] + v[i]);
+  wl[i] = (T)((double)wl[i] + w[i]);
 }
 
 
@@ -356,14 +421,16 @@ __device__ __forceinline__ void relax_point(double LM, double MM, double QN, dou
 }
 
 // main.f90:433-439 (+458-460) and 716-728 in one pass: RHS = -conv - divt (+ beta_pcon),
-// dudt = -divt (+ mean_p_force on x), then the AB2 predictor.  first: RHS_f = RHS
-// (main.f90:577-584) for the levels written here.
+// dudt = -divt (+ mean_p_force on x), then the AB2 predictor (binary64 velocities, the
+// update formed in binary64).  first: RHS_f = RHS (main.f90:577-584) for the levels
+// written here.
 template <typename T>
 __global__ void k_rhs_ab2(T *__restrict__ Rx, T *__restrict__ Ry, T *__restrict__ Rz, const T *__restrict__ dx_,
                           const T *__restrict__ dy_, const T *__restrict__ dz_, T *__restrict__ ax,
                           T *__restrict__ ay, T *__restrict__ az, const T *__restrict__ beta_pcon,
-                          T *__restrict__ Fx, T *__restrict__ Fy, T *__restrict__ Fz, int first, T *__restrict__ u,
-                          T *__restrict__ v, T *__restrict__ w, long n, T dt, T force, T mpf) {
+                          T *__restrict__ Fx, T *__restrict__ Fy, T *__restrict__ Fz, int first,
+                          double *__restrict__ u, double *__restrict__ v, double *__restrict__ w, long n, double dt,
+                          double force, T mpf) {
   long i = gtid();
   if (i >= n) return;
   const T rx = -Rx[i] - dx_[i];
@@ -383,10 +450,59 @@ __global__ void k_rhs_ab2(T *__restrict__ Rx, T *__restrict__ Ry, T *__restrict_
   } else {
     fx = Fx[i]; fy = Fy[i]; fz = Fz[i];
   }
-  const T t1 = T(1.5), t2 = T(1) - T(1.5);
-  u[i] = u[i] + dt * (t1 * rx + t2 * fx + force);
-  v[i] = v[i] + dt * (t1 * ry + t2 * fy);
-  w[i] = w[i] + dt * (t1 * rz + t2 * fz);
+  const double t1 = 1.5, t2 = 1.0 - 1.5;
+  u[i] = u[i] + dt * (t1 * (double)rx + t2 * (double)fx + force);
+  v[i] = v[i] + dt * (t1 * (double)ry + t2 * (double)fy);
+  w[i] = w[i] + dt * (t1 * (double)rz + t2 * (double)fz);
+}
+
+// convec.f90:140-296 in binary32 with the level means split off.  The 3/2-grid fields
+// entered the products with their kx = ky = 0 mode removed (a' = a - A per level, A
+// recorded by the padding pass), so the transforms carry only the fluctuations.  For
+// every product a b the rest, A b + B a - A B, is added here on the small grid in
+// binary64: b and a are band-limited (filtered velocities, spectral x/y derivatives, z
+// differences of filtered fields above the wall), so their padding and truncation leave
+// them unchanged and the sum equals the dealiased product.  The wall level's vorticity
+// (wall-model dudz, dvdz) enters no product.
+// A: [6][nz] means of u, v, w, omega1, omega2, omega3 as padded (u, v: less ush, the
+// shift their transforms took out, which is added back here).
+template <typename T>
+__global__ void k_convec_cross(T *__restrict__ cx_, T *__restrict__ cy_, T *__restrict__ cz_,
+                               const double *__restrict__ u, const double *__restrict__ v,
+                               const double *__restrict__ w, const T *__restrict__ dudy,
+                               const T *__restrict__ dudz, const T *__restrict__ dvdx, const T *__restrict__ dvdz,
+                               const T *__restrict__ dwdx, const T *__restrict__ dwdy,
+                               const double *__restrict__ A, const double *__restrict__ ush, Geo g) {
+  const long i = gtid();
+  if (i >= g.np * (g.nz - 1)) return;
+  const int x = (int)(i % g.ld);
+  if (x >= g.nx) return;
+  const int k = (int)(i / g.np);
+  const long up = i + g.np, dn = i - g.np;
+  const int nz = g.nz;
+  auto Au = [&](int l) { return A[l] + ush[l]; };
+  auto Av = [&](int l) { return A[nz + l] + ush[nz + l]; };
+  auto Aw = [&](int l) { return A[2 * nz + l]; };
+  auto O1 = [&](int l) { return A[3 * nz + l]; };
+  auto O2 = [&](int l) { return A[4 * nz + l]; };
+  auto O3 = [&](int l) { return A[5 * nz + l]; };
+  // om1 = dwdy - dvdz, om2 = dudz - dwdx (w nodes above the wall), om3 = dvdx - dudy
+  const double om3 = (double)dvdx[i] - (double)dudy[i];
+  const double om1u = (double)dwdy[up] - (double)dvdz[up], om2u = (double)dudz[up] - (double)dwdx[up];
+  // A b + B a - A B
+  auto X = [](double Aa, double b, double Bb, double a) { return Aa * b + Bb * a - Aa * Bb; };
+  double rx = -X(Av(k), om3, O3(k), v[i]) + 0.5 * X(Aw(k + 1), om2u, O2(k + 1), w[up]);
+  double ry = X(Au(k), om3, O3(k), u[i]) - 0.5 * X(Aw(k + 1), om1u, O1(k + 1), w[up]);
+  if (k >= 1) {
+    const double om1 = (double)dwdy[i] - (double)dvdz[i], om2 = (double)dudz[i] - (double)dwdx[i];
+    rx += 0.5 * X(Aw(k), om2, O2(k), w[i]);
+    ry -= 0.5 * X(Aw(k), om1, O1(k), w[i]);
+    const double su = Au(k) + Au(k - 1), sv = Av(k) + Av(k - 1);
+    const double rz = 0.5 * (-X(su, om2, O2(k), u[i] + u[dn]) + X(sv, om1, O1(k), v[i] + v[dn]));
+    cz_[i] = (T)((double)cz_[i] + rz);
+  }
+  cx_[i] = (T)((double)cx_[i] + rx);
+  cy_[i] = (T)((double)cy_[i] + ry);
 }
 
 // main.f90:789-803 (dpdz from p in binary64, RHS -= grad p, dudt -= grad p) and the
@@ -395,12 +511,14 @@ template <typename T>
 __global__ void k_dpdz_project(const double *__restrict__ p, T *__restrict__ dpdz, const T *__restrict__ dpdx,
                                const T *__restrict__ dpdy, T *__restrict__ Rx, T *__restrict__ Ry,
                                T *__restrict__ Rz, T *__restrict__ ax, T *__restrict__ ay, T *__restrict__ az,
-                               T *__restrict__ u, T *__restrict__ v, T *__restrict__ w, Geo g, double dz, T dt) {
+                               double *__restrict__ u, double *__restrict__ v, double *__restrict__ w, Geo g,
+                               double dz, double dt, int *__restrict__ nonfinite) {
   long i = gtid();
   if (i >= g.np * g.nz) return;
   int k = (int)(i / g.np), x = (int)(i % g.ld);
   if (k == g.nz - 1) { dpdz[i] = (T)LP_BOGUS; return; }
-  const T d = (x < g.nx) ? (T)((p[i + g.np] - p[i]) / dz) : T(0);
+  const double dd = (x < g.nx) ? (p[i + g.np] - p[i]) / dz : 0.0;  // binary64 for the correction
+  const T d = (T)dd;
   const T px = dpdx[i], py = dpdy[i];
   dpdz[i] = d;
   Rx[i] -= px;
@@ -410,28 +528,45 @@ __global__ void k_dpdz_project(const double *__restrict__ p, T *__restrict__ dpd
   ay[i] -= py;
   az[i] -= d;
   if (x >= g.nx) return;
-  const T t1 = T(1.5);
-  u[i] = (u[i] + dt * (-t1 * px + T(0)));
-  v[i] = (v[i] + dt * (-t1 * py + T(0)));
-  if (k >= 1) w[i] = (w[i] + dt * (-t1 * d + T(0)));
+  const double t1 = 1.5;
+  const double un = (u[i] + dt * (-t1 * (double)px + 0.0));
+  const double vn = (v[i] + dt * (-t1 * (double)py + 0.0));
+  u[i] = un;
+  v[i] = vn;
+  double wn = 0.0;
+  if (k >= 1) {
+    wn = (w[i] + dt * (-t1 * dd + 0.0));
+    w[i] = wn;
+  }
+  // the step's velocities are all written here: flag a non-finite one for the host
+  if (!isfinite(un) || !isfinite(vn) || !isfinite(wn)) atomicOr(nonfinite, 1);
+}
+
+// any non-finite value among a[0..n) (columns x >= nx included) raises *flag
+template <typename T>
+__global__ void k_nonfinite(const T *__restrict__ a, long n, int *__restrict__ flag) {
+  bool bad = false;
+  for (long i = gtid(); i < n; i += (long)gridDim.x * blockDim.x) bad |= !isfinite(a[i]);
+  if (bad) atomicOr(flag, 1);
 }
 
 // forcing.f90:145-191 velocity correction
 template <typename T>
-__global__ void k_project(T *__restrict__ u, T *__restrict__ v, T *__restrict__ w, const T *__restrict__ dpdx,
-                          const T *__restrict__ dpdy, const T *__restrict__ dpdz, Geo g, T dt) {
+__global__ void k_project(double *__restrict__ u, double *__restrict__ v, double *__restrict__ w,
+                          const T *__restrict__ dpdx, const T *__restrict__ dpdy, const T *__restrict__ dpdz, Geo g,
+                          double dt) {
   long i = gtid();
   if (i >= g.np * (g.nz - 1)) return;
   int k = (int)(i / g.np), x = (int)(i % g.ld);
   if (x >= g.nx) return;
-  const T t1 = T(1.5);
-  T R = -t1 * dpdx[i];
-  u[i] = (u[i] + dt * (R + T(0)));
-  R = -t1 * dpdy[i];
-  v[i] = (v[i] + dt * (R + T(0)));
+  const double t1 = 1.5;
+  double R = -t1 * (double)dpdx[i];
+  u[i] = (u[i] + dt * (R + 0.0));
+  R = -t1 * (double)dpdy[i];
+  v[i] = (v[i] + dt * (R + 0.0));
   if (k >= 1) {
-    R = -t1 * dpdz[i];
-    w[i] = (w[i] + dt * (R + T(0)));
+    R = -t1 * (double)dpdz[i];
+    w[i] = (w[i] + dt * (R + 0.0));
   }
 }
 
@@ -635,8 +770,8 @@ static __global__ void k_tridag_arrays(const double *__restrict__ a, const doubl
 // levels k and k-1, each minus its plane mean) and w' = w - <w>; zero at k = 0 (the
 // serial branch of :1811-1815).  out[k*10 + q]: the plane means of the statistics.
 template <typename T, int PASS>
-__global__ __launch_bounds__(256) void k_plane_stats(const T *__restrict__ u, const T *__restrict__ v,
-                                                     const T *__restrict__ w, const T *__restrict__ cs,
+__global__ __launch_bounds__(256) void k_plane_stats(const double *__restrict__ u, const double *__restrict__ v,
+                                                     const double *__restrict__ w, const T *__restrict__ cs,
                                                      const T *__restrict__ nut, const T *__restrict__ txz, Geo g,
                                                      double *__restrict__ out) {
   constexpr int NQ = PASS == 0 ? 6 : 4;
@@ -707,7 +842,7 @@ __global__ __launch_bounds__(64) void k_spectrum(const T *__restrict__ spec, lon
 
 // scalars_module2.f90:1336-1379 and rmsdiv.f90:27-36 partial reductions (levels 1..nz-1)
 template <typename T>
-__global__ void k_reduce_diag(const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ w,
+__global__ void k_reduce_diag(const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
                               const T *__restrict__ nut, const T *__restrict__ du, const T *__restrict__ dv,
                               const T *__restrict__ dw, Geo g, double *__restrict__ part) {
   __shared__ double s_u[256], s_n[256], s_d[256];

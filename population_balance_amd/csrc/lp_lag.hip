@@ -16,7 +16,7 @@
 namespace lp {
 
 template <typename T> struct LagIn {
-  const T *u, *v, *w;
+  const double *u, *v, *w;  // the velocities are binary64 in both precisions
   const T *S[6];
   const T *magS;
 };
@@ -32,28 +32,32 @@ struct F4 {
 // returns the values at the two points i, i+1 (i even) of level k.
 template <typename T> struct LagPair {
   using v2 = typename V2<T>::t;
+  using d2 = typename V2<double>::t;
   const LagIn<T> &in;
   const long np;
   const int k;
-  T cu, cv;  // per-level shifts of u, v on the w-node (zero in binary64)
+  double cu, cv;  // per-level shifts of u, v on the w-node (zero in binary64)
   __device__ __forceinline__ static v2 ld2(const T *f, long o) { return *reinterpret_cast<const v2 *>(f + o); }
+  __device__ __forceinline__ static d2 ldd(const double *f, long o) {
+    return *reinterpret_cast<const d2 *>(f + o);
+  }
   __device__ __forceinline__ static v2 mul(v2 a, v2 b) { return v2{a.x * b.x, a.y * b.y}; }
   // branch-free (loads from clamped offsets, then a select) so that the loads of a row
   // are issued together
-  __device__ __forceinline__ v2 hv(const T *f, long i) const {  // u, v on the w-node (jz = 1 stays)
-    const v2 a = ld2(f, i);
-    const v2 b = ld2(f, i - (k > 0 ? np : 0));
-    const v2 h{T(0.5) * (a.x + b.x), T(0.5) * (a.y + b.y)};
+  __device__ __forceinline__ d2 hv(const double *f, long i) const {  // u, v on the w-node (jz = 1 stays)
+    const d2 a = ldd(f, i);
+    const d2 b = ldd(f, i - (k > 0 ? np : 0));
+    const d2 h{0.5 * (a.x + b.x), 0.5 * (a.y + b.y)};
     return k == 0 ? a : h;
   }
-  __device__ __forceinline__ v2 hs(const T *f, T c, long i) const {  // hv(f, i) - c
-    const v2 a = hv(f, i);
-    return v2{a.x - c, a.y - c};
+  __device__ __forceinline__ v2 hs(const double *f, double c, long i) const {  // hv(f, i) - c, rounded to T
+    const d2 a = hv(f, i);
+    return v2{(T)(a.x - c), (T)(a.y - c)};
   }
   __device__ __forceinline__ v2 wn(long i) const {  // w; at jz = 1 a quarter of w(2)
-    const v2 a = ld2(in.w, i + (k == 0 ? np : 0));
-    const v2 q{T(0.25) * a.x, T(0.25) * a.y};
-    return k == 0 ? q : a;
+    const d2 a = ldd(in.w, i + (k == 0 ? np : 0));
+    const d2 q{0.25 * a.x, 0.25 * a.y};
+    return k == 0 ? v2{(T)q.x, (T)q.y} : v2{(T)a.x, (T)a.y};
   }
 };
 
@@ -76,7 +80,7 @@ __device__ __forceinline__ void lag_row(int q, const LagIn<T> &in, const Geo &g,
   // constant shift (both test filters pass kx = ky = 0 with gain 1); without the mean flow
   // in the products that difference no longer cancels |U|^2 / |u'|^2 of binary32 rounding.
   // Binary64 keeps the reference's products (shift 0, so every value is unchanged).
-  LagPair<T> L{in, g.np, k, T(0), T(0)};
+  LagPair<T> L{in, g.np, k, 0.0, 0.0};
   if constexpr (sizeof(T) == 4) {
     if (q <= 7 && q != 2) {  // the fields that read u or v
       L.cu = L.hv(in.u, (long)k * g.np).x;
@@ -260,11 +264,11 @@ constexpr int kHx = kRx + 2, kHy = kRy + 2;
 // The tile of the last x column also relaxes the two padding columns of its rows, which
 // interpolag_Sdep.f90 leaves alone (their old values enter the relaxation).  The
 // interpolation is interp_point's, term for term.
-template <typename T, bool UNIT>
+template <typename T, bool UNIT, typename TV = T>
 __global__ __launch_bounds__(256) void k_lag_relax(const T *__restrict__ lm, const T *__restrict__ mm,
                                                    const T *__restrict__ qn, const T *__restrict__ nn,
-                                                   F4c Fold, F4 Fnew, const T *__restrict__ ul,
-                                                   const T *__restrict__ vl, const T *__restrict__ wl,
+                                                   F4c Fold, F4 Fnew, const TV *__restrict__ ul,
+                                                   const TV *__restrict__ vl, const TV *__restrict__ wl,
                                                    T *__restrict__ Cs, T *__restrict__ Beta, Geo g, LagParams P,
                                                    LagInterp I, int ntx, int cg) {
   __shared__ double sF[4][3][kHy][kHx];
@@ -383,7 +387,7 @@ static hipError_t launch_lag_r2c(const LagInputs &in, Geo g, void *spec, int nc,
   const void *twM = tw_ptr(M, dt), *twN = tw_ptr(2 * M, dt);
   if (!twM || !twN || nc < 1 || nc > M + 1 || pitch < nc) return hipErrorInvalidValue;
   LagIn<T> a;
-  a.u = (const T *)in.u; a.v = (const T *)in.v; a.w = (const T *)in.w;
+  a.u = (const double *)in.u; a.v = (const double *)in.v; a.w = (const double *)in.w;
   for (int q = 0; q < 6; ++q) a.S[q] = (const T *)in.S[q];
   a.magS = (const T *)in.magS;
   constexpr int RB = 64 / row_lanes(M);
@@ -441,7 +445,7 @@ hipError_t lag_c2r_contract(Dt dt, const void *bar, const void *hat, int n1, int
 #undef CASE
 }
 
-template <typename T>
+template <typename T, typename TV>
 static hipError_t launch_lag_relax(void *const C[4], const double *const Fold[4], double *const Fnew[4],
                                    const void *ul, const void *vl, const void *wl, void *Cs, void *Beta, Geo g,
                                    const LagParams &P, const LagInterp &I, hipStream_t s) {
@@ -452,21 +456,22 @@ static hipError_t launch_lag_relax(void *const C[4], const double *const Fold[4]
   const int ntx = (g.nx + kRx - 1) / kRx, cg = (ntx * (g.ny / kRy) + 7) / 8;
   const dim3 grid((unsigned)(8L * cg * g.nz));
   if (I.cs == 1.0)
-    hipLaunchKernelGGL((k_lag_relax<T, true>), grid, dim3(256), 0, s, (const T *)C[0], (const T *)C[1],
-                       (const T *)C[2], (const T *)C[3], fo, fn, (const T *)ul, (const T *)vl, (const T *)wl, (T *)Cs,
-                       (T *)Beta, g, P, I, ntx, cg);
+    hipLaunchKernelGGL((k_lag_relax<T, true, TV>), grid, dim3(256), 0, s, (const T *)C[0], (const T *)C[1],
+                       (const T *)C[2], (const T *)C[3], fo, fn, (const TV *)ul, (const TV *)vl, (const TV *)wl,
+                       (T *)Cs, (T *)Beta, g, P, I, ntx, cg);
   else
-    hipLaunchKernelGGL((k_lag_relax<T, false>), grid, dim3(256), 0, s, (const T *)C[0], (const T *)C[1],
-                       (const T *)C[2], (const T *)C[3], fo, fn, (const T *)ul, (const T *)vl, (const T *)wl, (T *)Cs,
-                       (T *)Beta, g, P, I, ntx, cg);
+    hipLaunchKernelGGL((k_lag_relax<T, false, TV>), grid, dim3(256), 0, s, (const T *)C[0], (const T *)C[1],
+                       (const T *)C[2], (const T *)C[3], fo, fn, (const TV *)ul, (const TV *)vl, (const TV *)wl,
+                       (T *)Cs, (T *)Beta, g, P, I, ntx, cg);
   return hipGetLastError();
 }
 
 hipError_t lag_relax(Dt dt, void *const C[4], const double *const Fold[4], double *const Fnew[4], const void *ul,
                      const void *vl, const void *wl, void *Cs, void *Beta, Geo g, const LagParams &P,
-                     const LagInterp &I, hipStream_t s) {
-  return dt == F64 ? launch_lag_relax<double>(C, Fold, Fnew, ul, vl, wl, Cs, Beta, g, P, I, s)
-                   : launch_lag_relax<float>(C, Fold, Fnew, ul, vl, wl, Cs, Beta, g, P, I, s);
+                     const LagInterp &I, hipStream_t s, bool vel_f64) {
+  if (dt == F64) return launch_lag_relax<double, double>(C, Fold, Fnew, ul, vl, wl, Cs, Beta, g, P, I, s);
+  return vel_f64 ? launch_lag_relax<float, double>(C, Fold, Fnew, ul, vl, wl, Cs, Beta, g, P, I, s)
+                 : launch_lag_relax<float, float>(C, Fold, Fnew, ul, vl, wl, Cs, Beta, g, P, I, s);
 }
 
 }  // namespace lp

@@ -15,7 +15,7 @@
 namespace lp {
 
 struct LagInputs {
-  const void *u, *v, *w;
+  const void *u, *v, *w;  // binary64 in both precisions
   const void *S[6];
   const void *magS;  // |S| of the strain rates (same value sgs_stag computes)
 };
@@ -34,9 +34,10 @@ hipError_t lag_r2c(Dt dt, const LagInputs &in, Geo g, void *spec, int nc, long p
 hipError_t lag_c2r_contract(Dt dt, const void *bar, const void *hat, int n1, int n2, long p1, long p2,
                             void *const C[4], Geo g, const LagParams &P, hipStream_t s);
 
-// C as above; Fold are read (gathered), Fnew written
+// C as above; Fold are read (gathered), Fnew written.  ul, vl, wl: the lag velocities
+// (vel_f64 false: the field type) or u, v, w themselves (vel_f64 true: binary64).
 hipError_t lag_relax(Dt dt, void *const C[4], const double *const Fold[4], double *const Fnew[4], const void *ul,
                      const void *vl, const void *wl, void *Cs, void *Beta, Geo g, const LagParams &P,
-                     const LagInterp &I, hipStream_t s);
+                     const LagInterp &I, hipStream_t s, bool vel_f64);
 
 }  // namespace lp

@@ -60,6 +60,7 @@ program drop_in_demo
     end do
   end do
   write (*, '(a,es10.3)') 'filt_da max error = ', err
+  if (err > 1e-12) stop 8
   rc = lp_ddz_uv(ctx, dfdz, f)
   if (rc /= LP_OK) stop 3
   err = maxval(abs(dfdz(1:nx, :, 2:nz-1) - 1.0_rprec/cfg%grid%dz))

@@ -16,6 +16,7 @@ module lesgo_amd
   public :: lp_plane_stats, lp_spectrum
   public :: lp_pollen_all_in_one2, lp_calculate_probability
   public :: lp_pollen_rhs_calc2, lp_calcbeta_pcon, lp_step_pollen2
+  public :: lp_set_cfl_limit, lp_check_finite, lp_get_precision
   public :: LP_OK, LP_FP64, LP_MIXED, LP_LBC_WALL, LP_LBC_STRESS_FREE
   public :: LP_U, LP_V, LP_W, LP_P, LP_RHSX, LP_RHSY, LP_RHSZ
   public :: LP_PCON, LP_RHS_PCON, LP_RHS_PCON_F, LP_RHS_PBE, LP_BETA_PCON
@@ -282,6 +283,21 @@ module lesgo_amd
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: nsteps
+    end function
+    integer(c_int) function lp_set_cfl_limit(ctx, cfl_max) bind(C, name="lp_set_cfl_limit")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: cfl_max
+    end function
+    integer(c_int) function lp_check_finite(ctx) bind(C, name="lp_check_finite")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    ! the explicit-array routines above take binary64 arrays: an LP_FP64 context
+    integer(c_int) function lp_get_precision(ctx, precision) bind(C, name="lp_get_precision")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: precision
     end function
   end interface
 

@@ -19,7 +19,11 @@ _lib = None
 
 
 class LesgoError(RuntimeError):
-    pass
+    """a non-zero lp_status; `code` holds it (None for errors raised on the host side)"""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 def load():
@@ -30,6 +34,11 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise LesgoError(f"{LIB_PATH} is missing: build it with "
                          "`python -m population_balance_amd.build`")
+    # One HIP runtime per process: torch ships its own libamdhip64 (soname
+    # libamdhip64.so.7).  Loaded first, it also satisfies this library's dependency on
+    # that soname; loaded after this library, torch would open a second runtime, which
+    # then finds no GPU.
+    import torch  # noqa: F401
     L = C.CDLL(LIB_PATH)
     vp, i32 = C.c_void_p, C.c_int32
     sig = {
@@ -73,6 +82,11 @@ def load():
         "lp_plane_stats": ([vp, C.POINTER(C.c_double)], C.c_int),
         "lp_spectrum": ([vp, i32, i32, C.POINTER(C.c_double)], C.c_int),
         "lp_step": ([vp, i32], C.c_int),
+        "lp_set_cfl_limit": ([vp, C.c_double], C.c_int),
+        "lp_check_finite": ([vp], C.c_int),
+        "lp_get_precision": ([vp, C.POINTER(i32)], C.c_int),
+        "lp_set_frozen": ([vp, vp, vp], C.c_int),
+        "lp_step_adjoint": ([vp, vp, vp, vp, vp], C.c_int),
         "lp_pollen_all_in_one2": ([vp], C.c_int),
         "lp_pollen_rhs_calc2": ([vp], C.c_int),
         "lp_calcbeta_pcon": ([vp], C.c_int),
@@ -130,7 +144,7 @@ class Lesgo:
 
     def _check(self, rc, what):
         if rc != abi.LP_OK:
-            raise LesgoError(f"{what} failed ({rc}): {self.L.lp_last_error().decode()}")
+            raise LesgoError(f"{what} failed ({rc}): {self.L.lp_last_error().decode()}", rc)
 
     def close(self):
         if getattr(self, "h", None):
@@ -168,6 +182,38 @@ class Lesgo:
         out = np.empty(n, dt)
         self._check(self.L.lp_download(self.h, abi.FIELD[name], _ptr(out), nbytes), "lp_download")
         return out.reshape(shape) if int(np.prod(shape)) == n else out
+
+    def get(self, name):
+        """a field as a torch tensor on the device (a copy)"""
+        import torch
+        _, nbytes, esz = self.field_info(name)
+        shape = abi.field_shape(self.g, name, self.cfg.drop.npcon)
+        t = torch.empty(shape, dtype=torch.float64 if esz == 8 else torch.float32, device="cuda")
+        if t.numel() * esz != nbytes:
+            raise LesgoError(f"{name}: shape {shape} does not hold {nbytes} bytes")
+        self._check(self.L.lp_download(self.h, abi.FIELD[name], C.c_void_p(t.data_ptr()), nbytes), "lp_download")
+        return t
+
+    def put(self, name, t):
+        """set a field from a torch tensor on the device (converted to the field's type)"""
+        import torch
+        _, nbytes, esz = self.field_info(name)
+        t = t.to(torch.float64 if esz == 8 else torch.float32).contiguous()
+        if t.numel() * esz != nbytes:
+            raise LesgoError(f"{name}: expected {nbytes} bytes, got {t.numel() * esz}")
+        self._check(self.L.lp_upload(self.h, abi.FIELD[name], C.c_void_p(t.data_ptr()), nbytes), "lp_upload")
+
+    def set_frozen(self, nut, ust):
+        """frozen-coefficient steps with this Nu_t (volume) and wall u* (plane), device
+        tensors of the field type; (None, None) restores the SGS model and the wall law"""
+        self._frozen = (nut, ust)  # keep the tensors alive while the context uses them
+        self._check(self.L.lp_set_frozen(self.h, _ptr(nut), _ptr(ust)), "set_frozen")
+
+    def step_adjoint(self, nut, ust, ub, rb):
+        """the adjoint of one frozen-coefficient step at the context's state (binary64
+        device tensors: nut, ust the step's coefficients; ub (3 volumes), rb (3 volumes)
+        updated in place from the adjoints of its outputs to those of its inputs)"""
+        self._check(self.L.lp_step_adjoint(self.h, _ptr(nut), _ptr(ust), _ptr(ub), _ptr(rb)), "step_adjoint")
 
     def set_time(self, jt, jt_total=None):
         self._check(self.L.lp_set_time(self.h, jt, jt if jt_total is None else jt_total), "lp_set_time")
@@ -264,6 +310,19 @@ class Lesgo:
         a, b = C.c_double(), C.c_double()
         self._check(self.L.lp_timestep_conditions(self.h, C.byref(a), C.byref(b)), "timestep_conditions")
         return a.value, b.value
+
+    def set_cfl_limit(self, cfl_max):
+        """lp_step raises (LP_ERR_CFL) when the CFL after its steps exceeds cfl_max (0: off)."""
+        self._check(self.L.lp_set_cfl_limit(self.h, float(cfl_max)), "set_cfl_limit")
+
+    def check_finite(self):
+        """raises LesgoError (LP_ERR_NAN) naming the first state field with a non-finite value."""
+        self._check(self.L.lp_check_finite(self.h), "check_finite")
+
+    def precision(self):
+        a = C.c_int32()
+        self._check(self.L.lp_get_precision(self.h, C.byref(a)), "get_precision")
+        return a.value
 
     def rmsdiv(self):
         a = C.c_double()

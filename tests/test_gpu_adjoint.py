@@ -1,0 +1,193 @@
+"""The discrete adjoint of the frozen-coefficient step on the GPU (lp_step_adjoint):
+against the oracle's restatement (or_step_adjoint) on the same base state and seeds at
+1e-12, against central differences of the GPU's own frozen-coefficient step, and the
+checkpointed 20-step gradient of a probe objective at 128 x 128 x 65 against central
+differences on 32 seeded random directions (epsilon 1e-4), with the gradient of the
+binary32 run compared to that of the binary64 run."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from population_balance_amd import abi, ic
+
+pytestmark = pytest.mark.gpu
+VEL = ("u", "v", "w")
+RHS = ("RHSx", "RHSy", "RHSz")
+
+
+def _base(cfg, steps_before, seed=5):
+    """state after `steps_before` ordinary GPU steps and the Nu_t, u* of the next step"""
+    import torch
+    from population_balance_amd.lesgo import Lesgo
+    lg = Lesgo(cfg)
+    for n, a in zip(VEL, ic.turbulent_ic(cfg.grid, seed=seed)):
+        lg.upload(n, a)
+    lg.step(steps_before)
+    state = {n: lg.download(n) for n in VEL + RHS}
+    t = lg.time()
+    lg.step(1)
+    nut, ust = lg.download("Nu_t"), lg.download("ustar")
+    lg.close()
+    return state, t, nut, ust
+
+
+def _phys(g, a):
+    out = np.zeros_like(a)
+    out[..., :g.nx] = a[..., :g.nx]
+    return out
+
+
+@pytest.mark.parametrize("nx,nz,steps_before,kw", [
+    (32, 9, 0, {}), (32, 9, 2, {}), (64, 17, 3, {"use_mean_p_force": 1, "mean_p_force": 1.0}),
+    (32, 9, 2, {"lbc_mom": abi.LP_LBC_STRESS_FREE})])
+def test_step_adjoint_matches_oracle(nx, nz, steps_before, kw):
+    import torch
+    from oracle import Oracle
+    from population_balance_amd.lesgo import Lesgo
+    print(f"\n[case] step adjoint vs oracle {nx}x{nx}x{nz} after {steps_before} steps {kw}", flush=True)
+    cfg = abi.make_config(nx, nx, nz, **kw)
+    g = cfg.grid
+    state, t, nut, ust = _base(cfg, steps_before)
+    rng = np.random.default_rng(3)
+    shp = abi.field_shape(g, "u")
+    ub0 = np.stack([_phys(g, rng.standard_normal(shp)) for _ in VEL])
+    rb0 = np.stack([_phys(g, rng.standard_normal(shp)) for _ in RHS])
+    rb0[:, -1] = 0.0
+    # oracle
+    o = Oracle(cfg)
+    dp = C.POINTER(C.c_double)
+    o.L.or_step_adjoint.argtypes = [C.c_void_p, dp, dp, dp, dp]
+    for n in VEL + RHS:
+        o.set_field(n, state[n])
+    o.set_time(*t)
+    ub_o, rb_o = np.ascontiguousarray(ub0.copy()), np.ascontiguousarray(rb0.copy())
+    p = lambda a: a.ctypes.data_as(dp)
+    nut_c, ust_c = np.ascontiguousarray(nut), np.ascontiguousarray(ust)
+    assert o.L.or_step_adjoint(o.h, p(nut_c), p(ust_c), p(ub_o), p(rb_o)) == 0
+    o.close()
+    # GPU
+    A = Lesgo(cfg)
+    for n in VEL:
+        A.upload(n, state[n])
+    A.set_time(*t)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    ub, rb, tn, tu = d(ub0), d(rb0), d(nut), d(ust)
+    A.step_adjoint(tn, tu, ub, rb)
+    u_after = A.download("u")
+    A.close()
+    assert np.array_equal(u_after, state["u"])  # the base state is left as it was
+    ub_g, rb_g = ub.cpu().numpy(), rb.cpu().numpy()
+    for q in range(3):
+        e_u = rel_l2(ub_g[q], ub_o[q], g.nx)
+        print(f"  {VEL[q]}: {e_u:.2e}", end="")
+        assert e_u < 1e-12, (q, e_u)
+        if steps_before > 0:
+            e_r = rel_l2(rb_g[q][:nz - 1], rb_o[q][:nz - 1], g.nx)
+            print(f"  RHS_f{q}: {e_r:.2e}", end="")
+            assert e_r < 1e-12, (q, e_r)
+    print()
+
+
+def test_step_adjoint_against_gpu_central_differences():
+    """the frozen GPU step is quadratic in its state: a central difference with a large
+    step is exact up to rounding"""
+    import torch
+    from population_balance_amd.lesgo import Lesgo
+    nx, nz = 64, 17
+    print(f"\n[case] step adjoint vs GPU central differences {nx}x{nx}x{nz}", flush=True)
+    cfg = abi.make_config(nx, nx, nz)
+    g = cfg.grid
+    state, t, nut, ust = _base(cfg, 2)
+    rng = np.random.default_rng(7)
+    shp = abi.field_shape(g, "u")
+    su = [_phys(g, rng.standard_normal(shp)) for _ in VEL]
+    sr = [_phys(g, rng.standard_normal(shp)) for _ in RHS]
+    for a in sr:
+        a[-1] = 0.0
+    du = [_phys(g, rng.standard_normal(shp)) for _ in VEL]
+    dr = [_phys(g, rng.standard_normal(shp)) for _ in RHS]
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    tn, tu = d(nut), d(ust)
+
+    def J(eps):
+        lg = Lesgo(cfg)
+        for n, dd in zip(VEL, du):
+            lg.upload(n, state[n] + eps * dd)
+        for n, dd in zip(RHS, dr):
+            lg.upload(n, state[n] + eps * dd)
+        lg.set_time(*t)
+        lg.set_frozen(tn, tu)
+        lg.step(1)
+        tot = sum(np.sum(a[..., :g.nx] * lg.download(n)[..., :g.nx]) for n, a in zip(VEL, su))
+        tot += sum(np.sum(a[:nz - 1, :, :g.nx] * lg.download(n)[:nz - 1, :, :g.nx]) for n, a in zip(RHS, sr))
+        lg.close()
+        return tot
+
+    A = Lesgo(cfg)
+    for n in VEL:
+        A.upload(n, state[n])
+    A.set_time(*t)
+    ub, rb = d(np.stack(su)), d(np.stack(sr))
+    A.step_adjoint(tn, tu, ub, rb)
+    A.close()
+    ub, rb = ub.cpu().numpy(), rb.cpu().numpy()
+    adj = sum(np.sum(ub[q][..., :g.nx] * du[q][..., :g.nx]) + np.sum(rb[q][..., :g.nx] * dr[q][..., :g.nx])
+              for q in range(3))
+    fd = (J(0.5) - J(-0.5)) / 1.0
+    print(f"  adjoint {adj:.15e} central difference {fd:.15e} rel {abs(adj - fd) / abs(fd):.2e}")
+    assert abs(adj - fd) <= 1e-11 * abs(fd)
+
+
+def test_checkpointed_gradient_128_20_steps():
+    """128 x 128 x 65, 20 steps with checkpoints every 5, objective: u summed over a 4^3
+    probe box at every step.  The binary64 gradient against central differences of the
+    frozen-coefficient binary64 run on 32 seeded directions (epsilon 1e-4): <= 1e-4; the
+    gradient of the binary32 run (binary64 adjoint around its states) against it; the
+    autograd wrapper gives the same gradient."""
+    import torch
+    from population_balance_amd import adjoint
+    nx, nz, nsteps = 128, 65, 20
+    print(f"\n[case] checkpointed gradient {nx}x{nx}x{nz}, {nsteps} steps", flush=True)
+    kw = dict(use_mean_p_force=1, mean_p_force=1.0)
+    cfg = abi.make_config(nx, nx, nz, **kw)
+    g = cfg.grid
+    u0, v0, w0 = (torch.from_numpy(a).to("cuda") for a in ic.turbulent_ic(g, seed=ic.DEFAULT_SEED))
+    probe = adjoint.Probe(0, k0=nz // 4, j0=g.ny // 2, i0=3 * g.nx // 4)
+    run = adjoint.CheckpointedRun(cfg, nsteps, probe, every=5)
+    J = run.forward(u0, v0, w0)
+    gu, gv, gw = run.gradient()
+    assert run.adjoint_steps == nsteps and sorted(run.checkpoints) == [0, 5, 10, 15]
+    print(f"  J {J:.10e}; forward {run.t_forward:.1f} ms, adjoint {run.t_adjoint:.1f} ms "
+          f"({run.t_adjoint / nsteps:.3f} ms per adjoint step)", flush=True)
+    frozen = run.frozen_coefficients(torch.float64)
+    rng = np.random.default_rng(20261202)
+    shp = abi.field_shape(g, "u")
+    eps, worst = 1e-4, 0.0
+    for r in range(32):
+        dd = [torch.from_numpy(_phys(g, rng.standard_normal(shp))).to("cuda") for _ in VEL]
+        jp = run.objective_at(u0 + eps * dd[0], v0 + eps * dd[1], w0 + eps * dd[2], frozen)
+        jm = run.objective_at(u0 - eps * dd[0], v0 - eps * dd[1], w0 - eps * dd[2], frozen)
+        fd = (jp - jm) / (2 * eps)
+        ad = float((gu * dd[0]).sum() + (gv * dd[1]).sum() + (gw * dd[2]).sum())
+        e = abs(ad - fd) / abs(fd)
+        worst = max(worst, e)
+        if r < 4:
+            print(f"  direction {r}: adjoint {ad:.10e} central difference {fd:.10e} rel {e:.2e}", flush=True)
+    print(f"  worst over 32 directions: {worst:.2e}", flush=True)
+    assert worst <= 1e-4
+    # the binary32 run: binary64 adjoint around its own states
+    cfg_m = abi.make_config(nx, nx, nz, precision=abi.LP_MIXED, **kw)
+    run_m = adjoint.CheckpointedRun(cfg_m, nsteps, probe, every=5)
+    Jm = run_m.forward(u0, v0, w0)
+    gm = run_m.gradient()
+    e = float(torch.linalg.norm(torch.stack(gm) - torch.stack([gu, gv, gw])) / torch.linalg.norm(torch.stack([gu, gv, gw])))
+    print(f"  mixed run: J {Jm:.10e} (rel {abs(Jm - J) / abs(J):.2e}), gradient vs binary64 {e:.2e}", flush=True)
+    assert e < 1e-3
+    # autograd
+    x = [a.clone().requires_grad_(True) for a in (u0, v0, w0)]
+    Ja = adjoint.RunObjective.apply(x[0], x[1], x[2], adjoint.CheckpointedRun(cfg, nsteps, probe, every=5))
+    Ja.backward()
+    assert abs(float(Ja) - J) <= 1e-12 * abs(J)
+    assert torch.equal(x[0].grad, gu) and torch.equal(x[2].grad, gw)

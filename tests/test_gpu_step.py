@@ -163,6 +163,29 @@ def test_steps_fp64(nx, nz, nsteps):
     assert lg.time() == o.time()
 
 
+def test_steps_fp64_128_after_50():
+    """128 x 128 x 129, the seeded state with the log law: the binary64 path against the
+    oracle field by field after 1 and after 50 whole steps."""
+    nx, nz = 128, 129
+    print(f"\n[case] steps fp64 {nx}x{nx}x{nz}, after 1 and 50 steps", flush=True)
+    cfg = abi.make_config(nx, nx, nz)
+    o, lg = _pair(cfg)
+    st = seeded_state(cfg)
+    load_oracle(o, st); load_gpu(lg, st)
+    done = 0
+    for target in (1, 50):
+        o.step(target - done); lg.step(target - done)
+        done = target
+        out = {}
+        for n in ("u", "v", "w", "RHSx", "RHSy", "RHSz", "p"):
+            lev = slice(0, nz) if n == "p" else slice(0, nz - 1)
+            out[n] = rel_l2(lg.download(n)[lev], o.field(n)[lev], nx)
+        print(f"  after {target}: " + " ".join(f"{k}={v:.2e}" for k, v in out.items()), flush=True)
+        bad = {k: v for k, v in out.items() if not v < TOL}
+        assert not bad, (target, bad)
+    assert lg.time() == o.time()
+
+
 @pytest.mark.parametrize("cs_count", [1, 2])
 def test_steps_lagrangian_state(cs_count):
     """the Lagrangian model's state through whole steps: F_LM/MM/QN/NN, Cs_opt2, Beta and

@@ -73,6 +73,34 @@ def test_fullsize_mixed_within_1e5_of_fp64(state):
         assert all(v <= 1e-5 for v in errs.values()), errs
 
 
+def test_fullsize_mixed_100_steps(state):
+    """100 mixed and 100 binary64 steps at dt = 1e-4 from the same state (at dt = 2e-4
+    the seeded 256^3 state has CFL 1.38 and grows unstable near step 85 in binary64 as
+    well): u, v, w, p of the mixed path within 1e-5 of the binary64 path after 1 and 100
+    steps; the divergence left by each projection is reported."""
+    from population_balance_amd.lesgo import Lesgo
+    print(f"\n[case] full size mixed vs fp64 {NX}x{NX}x{NZ}, dt=1e-4, after 1 and 100 steps", flush=True)
+    runs, div = {}, {}
+    for prec in (abi.LP_MIXED, abi.LP_FP64):
+        lg = Lesgo(abi.make_config(NX, NX, NZ, precision=prec, dt=1e-4, **KW))
+        for n, a in zip(("u", "v", "w"), state):
+            lg.upload(n, a)
+        out = {}
+        for target in (1, 100):
+            lg.step(target - lg.time()[0] + 1)
+            out[target] = {n: lg.download(n)[_levels(n), :, :NX].astype(np.float64) for n in ("u", "v", "w", "p")}
+        div[prec] = lg.rmsdiv()
+        cfl, _ = lg.timestep_conditions()
+        print(f"  prec={prec}: rmsdiv {div[prec]:.2e} cfl {cfl:.3f}", flush=True)
+        runs[prec] = out
+        lg.close()
+    for target in (1, 100):
+        errs = {n: rel_l2(runs[abi.LP_MIXED][target][n], runs[abi.LP_FP64][target][n])
+                for n in ("u", "v", "w", "p")}
+        print(f"  after {target} steps: " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
+        assert all(v <= 1e-5 for v in errs.values()), (target, errs)
+
+
 def test_fullsize_droplet_steps_against_oracle():
     """The droplet workload at its full size: 192 x 192 x 128 levels, 15 log-spaced bins,
     settling, inertia, PBE breakup, buoyancy, the Gaussian source.  Two steps of the oracle;
