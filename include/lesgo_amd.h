@@ -18,7 +18,10 @@
  *     (the exceptions, always double, are listed with the field ids below).  The
  *     explicit-array routines (lp_filt_da ... lp_divstress_w) take arrays of this type.
  *   - every array argument may be host memory or device memory: device pointers are
- *     used in place, host pointers are staged through the context (synchronous).
+ *     used in place, host pointers are staged through the context (synchronous).  The
+ *     context's streams do not wait on other streams: work that the caller queued on
+ *     device arrays it passes must be complete before the call.  Calls return after
+ *     their own work on the arrays has completed.
  *
  * Error behaviour: the reference STOPs on a zero pivot (tridag_array.f90:35-38,70-75)
  * and on an invalid lbc_mom (wallstress.f90:207-209); here those return

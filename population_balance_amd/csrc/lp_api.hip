@@ -853,10 +853,8 @@ static int do_filt_da(lp_ctx *c, void *f, void *dfdx, void *dfdy, int nlev, void
     // every step)
     Geo g = geo(c);
     double *A = c->nyq, *B = A + (size_t)c->g.nz * g.ny, *Cc = B + (size_t)c->g.nz * g.nx;
-    hipLaunchKernelGGL(k_nyq_rows, dim3((unsigned)(((long)g.ny * nlev + 3) / 4)), dim3(256), 0, c->s,
-                       (const double *)f, A, g, nlev);
-    hipLaunchKernelGGL(k_nyq_cols, dim3((unsigned)((g.nx + 63) / 64), (unsigned)nlev), dim3(256), 0, c->s,
-                       (const double *)f, (const double *)A, B, Cc, g);
+    if (g.nx > kNyqMax || g.ny > kNyqMax) return set_err(LP_ERR_ARG, "filt_da: nx, ny above %d", kNyqMax);
+    hipLaunchKernelGGL(k_nyq_sums, dim3((unsigned)nlev), dim3(1024), 0, c->s, (const double *)f, A, B, Cc, g);
     hipLaunchKernelGGL(k_nyq_remove, grid1(g.np * nlev), dim3(256), 0, c->s, (double *)f, (const double *)A,
                        (const double *)B, (const double *)Cc, g, nlev);
     HIPCHK(hipGetLastError());

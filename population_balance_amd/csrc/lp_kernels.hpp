@@ -41,47 +41,58 @@ __global__ void k_ddz_w(T *__restrict__ dfdz, const TI *__restrict__ f, Geo g, T
 //   (-1)^x A(y) + (-1)^y B(x) - (-1)^(x+y) C,
 //   A(y) = sum_x (-1)^x f(x, y) / nx,  B(x) = sum_y (-1)^y f(x, y) / ny,
 //   C = sum_y (-1)^y A(y) / ny  (the corner mode, counted in both sums).
-// k_nyq_rows: one wave per (row, level) -> A.
-static __global__ __launch_bounds__(256) void k_nyq_rows(const double *__restrict__ f, double *__restrict__ A, Geo g,
-                                                  int nlev) {
-  const long row = (long)blockIdx.x * 4 + threadIdx.x / 64;  // y + ny * level
-  const int lane = threadIdx.x % 64;
-  if (row >= (long)g.ny * nlev) return;
-  const double *r = f + (row / g.ny) * g.np + (row % g.ny) * g.ld;
-  double acc = 0.0;
-  for (int x = lane; x < g.nx; x += 64) acc += (x & 1) ? -r[x] : r[x];
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-  if (lane == 0) A[row] = acc / g.nx;
-}
-
-// k_nyq_cols: one block per (64 columns, level), four waves over interleaved rows -> B;
-// the block of column 0 also forms C from A.
-static __global__ __launch_bounds__(256) void k_nyq_cols(const double *__restrict__ f, const double *__restrict__ A,
-                                                  double *__restrict__ B, double *__restrict__ Cc, Geo g) {
-  __shared__ double part[4][64];
-  const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
-  const int x = blockIdx.x * 64 + lane;
-  const int k = blockIdx.y;
+// k_nyq_sums: one block per level reads it once: each of 16 waves takes every 16th row,
+// forms A(y) with a wave reduction and keeps its rows' alternating column sums; the 16
+// partial column sums are added in LDS (fixed order: the result does not depend on timing).
+constexpr int kNyqMax = 384;  // largest nx, ny of the transforms
+static __global__ __launch_bounds__(1024) void k_nyq_sums(const double *__restrict__ f, double *__restrict__ A,
+                                                          double *__restrict__ B, double *__restrict__ Cc, Geo g) {
+  __shared__ double colp[16][kNyqMax];
+  __shared__ double arow[kNyqMax];
+  const int lane = threadIdx.x % 64, grp = threadIdx.x / 64;
+  const int k = blockIdx.x;
   const double *p = f + (long)k * g.np;
-  double acc = 0.0;
-  if (x < g.nx) {
-    int y = wv;
-    for (; y + 12 < g.ny; y += 16) {  // four rows in flight per lane
-      const double a0 = p[(long)y * g.ld + x], a1 = p[(long)(y + 4) * g.ld + x];
-      const double a2 = p[(long)(y + 8) * g.ld + x], a3 = p[(long)(y + 12) * g.ld + x];
-      acc += (y & 1) ? -(a0 + a1 + a2 + a3) : (a0 + a1 + a2 + a3);  // y, y+4, ... share parity
+  constexpr int NQ = kNyqMax / 64;
+  double col[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) col[q] = 0.0;
+  for (int y = grp; y < g.ny; y += 16) {
+    const double *r = p + (long)y * g.ld;
+    const bool odd_y = y & 1;
+    double ra = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int x = lane + 64 * q;
+      if (x < g.nx) {
+        const double v = r[x];
+        ra += (lane & 1) ? -v : v;  // x and lane have the same parity
+        col[q] += odd_y ? -v : v;
+      }
     }
-    for (; y < g.ny; y += 4) acc += (y & 1) ? -p[(long)y * g.ld + x] : p[(long)y * g.ld + x];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) ra += __shfl_xor(ra, o, 64);
+    if (lane == 0) {
+      A[(long)k * g.ny + y] = ra / g.nx;
+      arow[y] = ra / g.nx;
+    }
   }
-  part[wv][lane] = acc;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int x = lane + 64 * q;
+    if (x < g.nx) colp[grp][x] = col[q];
+  }
   __syncthreads();
-  if (wv == 0 && x < g.nx) B[(long)k * g.nx + x] = (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]) / g.ny;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    const double *a = A + (long)k * g.ny;
+  for (int x = threadIdx.x; x < g.nx; x += blockDim.x) {
+    double s = 0.0;
+    for (int r = 0; r < 16; ++r) s += colp[r][x];
+    B[(long)k * g.nx + x] = s / g.ny;
+  }
+  if (grp == 0) {
     double c = 0.0;
-    for (int y = 0; y < g.ny; ++y) c += (y & 1) ? -a[y] : a[y];
-    Cc[k] = c / g.ny;
+    for (int y = lane; y < g.ny; y += 64) c += (y & 1) ? -arow[y] : arow[y];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if (lane == 0) Cc[k] = c / g.ny;
   }
 }
 

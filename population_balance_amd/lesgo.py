@@ -105,6 +105,15 @@ def load():
     return L
 
 
+def _torch_ready(t):
+    """the library's stream does not wait on torch's: work that torch has queued on a
+    tensor (or on the memory a new tensor reuses) must be done before the library reads
+    or writes it"""
+    if t.is_cuda:
+        import torch
+        torch.cuda.current_stream(t.device).synchronize()
+
+
 def _ptr(a):
     """device or host address of a numpy array / torch tensor."""
     if a is None:
@@ -112,6 +121,7 @@ def _ptr(a):
     if hasattr(a, "data_ptr"):
         if not a.is_contiguous():
             raise LesgoError("torch tensors must be contiguous")
+        _torch_ready(a)
         return C.c_void_p(a.data_ptr())
     if not a.flags["C_CONTIGUOUS"]:
         raise LesgoError("numpy arrays must be C-contiguous")
@@ -191,7 +201,7 @@ class Lesgo:
         t = torch.empty(shape, dtype=torch.float64 if esz == 8 else torch.float32, device="cuda")
         if t.numel() * esz != nbytes:
             raise LesgoError(f"{name}: shape {shape} does not hold {nbytes} bytes")
-        self._check(self.L.lp_download(self.h, abi.FIELD[name], C.c_void_p(t.data_ptr()), nbytes), "lp_download")
+        self._check(self.L.lp_download(self.h, abi.FIELD[name], _ptr(t), nbytes), "lp_download")
         return t
 
     def put(self, name, t):
@@ -201,7 +211,7 @@ class Lesgo:
         t = t.to(torch.float64 if esz == 8 else torch.float32).contiguous()
         if t.numel() * esz != nbytes:
             raise LesgoError(f"{name}: expected {nbytes} bytes, got {t.numel() * esz}")
-        self._check(self.L.lp_upload(self.h, abi.FIELD[name], C.c_void_p(t.data_ptr()), nbytes), "lp_upload")
+        self._check(self.L.lp_upload(self.h, abi.FIELD[name], _ptr(t), nbytes), "lp_upload")
 
     def set_frozen(self, nut, ust):
         """frozen-coefficient steps with this Nu_t (volume) and wall u* (plane), device
