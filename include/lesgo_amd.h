@@ -218,6 +218,16 @@ int lp_divstress_w(lp_ctx *ctx, void *divt, const void *tx, const void *ty,
  * binary64; modes jy = ny/2+1, jx = lh and (1,1) are skipped as in the reference. */
 int lp_tridag_array(lp_ctx *ctx, const double *a, const double *b, const double *c,
                     const double *r, double *u);
+/* No counterpart in the reference: the same solve for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No context; device arrays only, laid out as lp_tridag_array's with
+ * (lh, ny, nz+1); enqueued on `stream` (a hipStream_t; null: the default stream) and
+ * returns after the solve (LP_ERR_PIVOT on a zero pivot).  Its scratch -- the gam factors
+ * and the pivot flag -- is one fixed device array of 72 MiB on each device the library
+ * runs on, held until the process ends: lh*ny*(nz+1) <= 9*2^20 - 32 values (256^3 fits),
+ * larger systems return LP_ERR_ARG.  Calls that overlap on one device race. */
+int lp_tridag_array_noalloc(int32_t lh, int32_t ny, int32_t nz, const double *a, const double *b,
+                            const double *c, const double *r, double *u, void *stream);
 
 /* ---------------- physics routines on the context state ----------------
  * The reference's no-argument routines read and write module globals

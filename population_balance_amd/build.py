@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblesgo_hip.so")
 ARCH = os.environ.get("LP_OFFLOAD_ARCH", "gfx950")
-SOURCES = ["lp_fft.hip", "lp_lag.hip", "lp_convec.hip", "lp_api.hip"]
+SOURCES = ["lp_fft.hip", "lp_lag.hip", "lp_convec.hip", "lp_api.hip", "lp_noalloc.hip"]
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
