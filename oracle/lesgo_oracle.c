@@ -1940,6 +1940,368 @@ static void adj_r2c(or_ctx *c, cplx *S, double *f) {
   or_rfft2_inv(c, 0, (const double *)S, f, g->ld);
 }
 
+/* d smart_face / d (cup, cupup, cdn) on the branch the values select (the face is linear
+ * in the three cells on each branch of the limiter) */
+static void smart_coef(double cup, double cupup, double cdn, double *a_up, double *a_upup, double *a_dn) {
+  const double den = cup - cupup;
+  *a_up = 1.0; *a_upup = 0.0; *a_dn = 0.0;
+  if (den == 0.0) return;
+  const double r = (cdn - cup) / den;
+  const double b2 = 2.0 * r, bq = 0.75 * r + 0.25;
+  const double m = fmin(fmin(b2, bq), 4.0);
+  if (!(m > 0.0)) return;                                    /* psi = 0: the upwind cell */
+  if (m == b2) { *a_up = 0.0; *a_dn = 1.0; return; }         /* psi = 2r: the downwind cell */
+  if (m == bq) { *a_up = 0.75; *a_upup = -0.125; *a_dn = 0.375; return; }  /* QUICK */
+  *a_up = 3.0; *a_upup = -2.0;                               /* psi = 4 */
+}
+
+/* Adjoint of the droplet part of one frozen-coefficient step (pollen_all_in_one2,
+ * scalars_module_crossflow_mod.f90:1494-1786 with pollen_RHS_calc2 :1894-4472,
+ * calcbeta_pcon :1795-1843 and step_pollen2 :4481-4508) at the base C (the context's
+ * PCon) and the base filtered velocities uf, vf, wf with ux = dudx, vy = dvdy.  Frozen:
+ * Nu_t (so Kc_t).  In: cb, rcb the adjoints of the step's PCon and RHS_PCon; arz the
+ * adjoint of the assembled RHSz (which receives the buoyancy).  Out: cb, rcb those of the
+ * PCon and RHS_PCon the step starts from; the velocity adjoints gain the transport's part
+ * (auf, avf, awf of the filtered u, v, w; aux, avy of dudx, dvdy). */
+static int adj_droplets(or_ctx *c, const double *nut, const double *uf, const double *vf, const double *wf,
+                        const double *ux, const double *vy, const double *arz, double *cb, double *rcb,
+                        double *auf, double *aux, double *avf, double *avy, double *awf) {
+  const lp_grid *g = &c->g;
+  const lp_droplet_config *D = &c->cfg.drop;
+  const int nx = g->nx, ny = g->ny, nz = g->nz, ld = g->ld, npc = D->npcon;
+  const size_t np = plane(c), nv = vol(c);
+  const double dx = g->dx, dy = g->dy, dz = g->dz, dt = c->cfg.dt;
+  if (D->inertia) return fail(LP_ERR_ARG, "droplet adjoint: inertia (dudt in the face velocity) is not differentiated");
+  if (D->pbe && D->pbe_feedback) return fail(LP_ERR_ARG, "droplet adjoint: the breakup feedback is not differentiated");
+  if (c->jt_total < D->pcon_init) return 0;  /* no droplet step: PCon and RHS_PCon carry over */
+  const int euler = (c->jt_total == D->pcon_init || c->jt_total == D->src_jt0) && !D->init_pcon;
+  const double *PC = c->f[LP_PCON];
+  const size_t fp = (size_t)ld * (ny + 1);
+#define FI(i, j, k) ((size_t)(k) * fp + (size_t)(j) * ld + (i))
+#define SI(i, j, k) ((size_t)(k) * np + (size_t)(j) * ld + (i))
+  /* base face velocities (:2054-2138) */
+  double *ui = (double *)calloc(fp * nz, sizeof(double)), *vi = (double *)calloc(fp * nz, sizeof(double));
+  double *wi = (double *)calloc(fp * nz, sizeof(double));
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j) {
+      double su = 0.0;
+      for (int m = 0; m < nx; ++m) su += uf[SI(m, j, k)];
+      for (int i = 0; i < nx; ++i) {
+        double acc = 0.0;
+        for (int m = 0; m < nx - 1; ++m)
+          acc += (i <= m ? -(nx - (m + 1)) * (1.0 / nx) : (m + 1) * (1.0 / nx)) * dx * ux[SI(m, j, k)];
+        ui[FI(i, j, k)] = acc + su / nx;
+      }
+      ui[FI(nx, j, k)] = ui[FI(0, j, k)];
+    }
+  for (int k = 0; k < nz; ++k)
+    for (int i = 0; i < nx; ++i) {
+      double sv = 0.0;
+      for (int m = 0; m < ny; ++m) sv += vf[SI(i, m, k)];
+      for (int j = 0; j < ny; ++j) {
+        double acc = 0.0;
+        for (int m = 0; m < ny - 1; ++m)
+          acc += (j <= m ? -(ny - (m + 1)) * (1.0 / ny) : (m + 1) * (1.0 / ny)) * dy * vy[SI(i, m, k)];
+        vi[FI(i, j, k)] = acc + sv / ny;
+      }
+      vi[FI(i, ny, k)] = vi[FI(i, 0, k)];
+    }
+  double *Kc = (double *)calloc(nv, sizeof(double));
+  for (int k = 0; k < nz - 1; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        size_t s0 = SI(i, j, k);
+        double kv = (k == 0 || k == nz - 2) ? nut[s0] : 0.5 * (nut[s0] + nut[s0 + np]);
+        Kc[s0] = kv / D->Sc;
+      }
+  /* adjoints of the face velocities, summed over the bins */
+  double *aui = (double *)calloc(fp * nz, sizeof(double)), *avi = (double *)calloc(fp * nz, sizeof(double));
+  double *awi = (double *)calloc(fp * nz, sizeof(double));
+  double *mu = (double *)calloc(nv, sizeof(double)), *ac = (double *)calloc(nv, sizeof(double));
+  double *acx = (double *)calloc(fp * nz, sizeof(double)), *acy = (double *)calloc(fp * nz, sizeof(double));
+  double *acz = (double *)calloc(fp * nz, sizeof(double));
+  const double gh = D->g * (D->z_i / (D->u_star * D->u_star));
+  for (int ip = 0; ip < npc; ++ip) {
+    const double *C = PC + (size_t)ip * nv;
+    double *lb = cb + (size_t)ip * nv, *lr = rcb + (size_t)ip * nv;
+    const double ws = D->settling_vel[ip];
+#define CC(i, j, k) C[SI(i, j, k)]
+    for (size_t q = 0; q < fp * nz; ++q) {
+      wi[q] = 0.0; acx[q] = 0.0; acy[q] = 0.0; acz[q] = 0.0;
+    }
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          wi[FI(i, j, k)] = wf[SI(i, j, k)];
+          if (D->settling && k >= 1 && k <= nz - 2) wi[FI(i, j, k)] -= ws;
+        }
+    /* step_pollen2 (:4481-4508): top level copied from below; the update */
+    memset(mu, 0, nv * sizeof(double));
+    memset(ac, 0, nv * sizeof(double));
+    for (size_t q = 0; q < np; ++q) {
+      lb[(size_t)(nz - 2) * np + q] += lb[(size_t)(nz - 1) * np + q];
+      lb[(size_t)(nz - 1) * np + q] = 0.0;
+    }
+    for (int k = 0; k < nz - 1; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          size_t s0 = SI(i, j, k);
+          mu[s0] = lr[s0] + (euler ? dt : 1.5 * dt) * lb[s0];
+          ac[s0] = lb[s0];                               /* C(k) carried by the update */
+          lr[s0] = euler ? 0.0 : -0.5 * dt * lb[s0];     /* RHS_PCon_f = the RHS_PCon read */
+        }
+    for (size_t q = (size_t)(nz - 1) * np; q < nv; ++q) lr[q] = 0.0;
+    /* calcbeta_pcon (:1795-1843) into RHSz (main.f90:458-460) */
+    if (D->active) {
+      const double dd = D->diameter[ip];
+      const double coef = gh * (D->rho_c / D->rho_w - 1.0) * (PI_D / 6 * (dd * dd * dd));
+      for (int k = 0; k < nz - 1; ++k)
+        for (int j = 0; j < ny; ++j)
+          for (int i = 0; i < nx; ++i) ac[SI(i, j, k)] += coef * arz[SI(i, j, k)];
+    }
+    /* RHS = -div(u C) (:3559-3594): adjoints of the face values and face velocities */
+    for (int k = 0; k < nz - 1; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i <= nx; ++i) {
+          double m = (i < nx ? mu[SI(i, j, k)] : 0.0) - (i >= 1 ? mu[SI(i - 1, j, k)] : 0.0);
+          acx[FI(i, j, k)] += ui[FI(i, j, k)] / dx * m;
+        }
+    for (int k = 0; k < nz - 1; ++k)
+      for (int i = 0; i < nx; ++i)
+        for (int j = 0; j <= ny; ++j) {
+          double m = (j < ny ? mu[SI(i, j, k)] : 0.0) - (j >= 1 ? mu[SI(i, j - 1, k)] : 0.0);
+          acy[FI(i, j, k)] += vi[FI(i, j, k)] / dy * m;
+        }
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i)
+        for (int k = 0; k <= nz - 1; ++k) {
+          double m = (k <= nz - 2 ? mu[SI(i, j, k)] : 0.0) - (k >= 1 ? mu[SI(i, j, k - 1)] : 0.0);
+          acz[FI(i, j, k)] += wi[FI(i, j, k)] / dz * m;
+        }
+    /* the face values, recomputed, for the velocity adjoints; then the faces transposed */
+    for (int k = 0; k < nz - 1; ++k)
+      for (int j = 0; j < ny; ++j) {
+        /* x faces.  Ghosts (:2478-2590) */
+        double gx0 = 0.0, gxL = 0.0, s0 = 0.0, s1 = 0.0;
+        if (D->bc_x == 0) { gx0 = CC(nx - 1, j, k); gxL = CC(0, j, k); }
+        else if (D->bc_x == 1) { gxL = CC(nx - 1, j, k); }
+        else {
+          s0 = signbit(ui[FI(0, j, k)]) ? 1.0 : 0.0;      /* (1 - sign)/2 */
+          s1 = signbit(-ui[FI(0, j, k)]) ? 1.0 : 0.0;
+          gx0 = CC(0, j, k) * s0; gxL = CC(nx - 1, j, k) * s1;
+        }
+        double agx0 = 0.0, agxL = 0.0;
+        for (int i = 1; i < nx; ++i) {
+          const size_t f = FI(i, j, k);
+          const double m = (i < nx ? mu[SI(i, j, k)] : 0.0) - mu[SI(i - 1, j, k)];
+          double cup, cupup, cdn, au, auu, ad;
+          if (ui[f] >= 0.0) {
+            cup = CC(i - 1, j, k); cupup = (i >= 2) ? CC(i - 2, j, k) : gx0; cdn = CC(i, j, k);
+          } else {
+            cup = CC(i, j, k); cupup = (i + 1 <= nx - 1) ? CC(i + 1, j, k) : gxL; cdn = CC(i - 1, j, k);
+          }
+          aui[f] += smart_face(cup, cupup, cdn) / dx * m;
+          smart_coef(cup, cupup, cdn, &au, &auu, &ad);
+          const double a = acx[f];
+          if (ui[f] >= 0.0) {
+            ac[SI(i - 1, j, k)] += au * a; ac[SI(i, j, k)] += ad * a;
+            if (i >= 2) ac[SI(i - 2, j, k)] += auu * a; else agx0 += auu * a;
+          } else {
+            ac[SI(i, j, k)] += au * a; ac[SI(i - 1, j, k)] += ad * a;
+            if (i + 1 <= nx - 1) ac[SI(i + 1, j, k)] += auu * a; else agxL += auu * a;
+          }
+        }
+        {  /* boundary faces 0 and nx */
+          const double m0 = mu[SI(0, j, k)], mL = -mu[SI(nx - 1, j, k)];
+          const double a0 = acx[FI(0, j, k)], aL = acx[FI(nx, j, k)];
+          if (D->bc_x == 0) {
+            double cup, cupup, cdn, au, auu, ad;
+            int iu, iuu, idn;
+            if (ui[FI(0, j, k)] >= 0.0) { iu = nx - 1; iuu = nx - 2; idn = 0; }
+            else { iu = 0; iuu = 1; idn = nx - 1; }
+            cup = CC(iu, j, k); cupup = CC(iuu, j, k); cdn = CC(idn, j, k);
+            const double f0 = smart_face(cup, cupup, cdn);
+            aui[FI(0, j, k)] += f0 / dx * m0;
+            aui[FI(nx, j, k)] += f0 / dx * mL;
+            smart_coef(cup, cupup, cdn, &au, &auu, &ad);
+            const double a = a0 + aL;
+            ac[SI(iu, j, k)] += au * a; ac[SI(iuu, j, k)] += auu * a; ac[SI(idn, j, k)] += ad * a;
+          } else if (D->bc_x == 1) {  /* cx(0) = 0, cx(nx) = C(nx-1) */
+            aui[FI(nx, j, k)] += CC(nx - 1, j, k) / dx * mL;
+            ac[SI(nx - 1, j, k)] += aL;
+          } else {                    /* cx(0) = gx0, cx(nx) = gxL */
+            aui[FI(0, j, k)] += gx0 / dx * m0;
+            aui[FI(nx, j, k)] += gxL / dx * mL;
+            agx0 += a0; agxL += aL;
+          }
+        }
+        /* x diffusion (:3595-3622) with the ghosts */
+        const double idx2 = 1.0 / (dx * dx);
+        for (int i = 0; i < nx; ++i) {
+          const double m = mu[SI(i, j, k)] * idx2;
+          const double Ki = Kc[SI(i, j, k)];
+          const double Kp = 0.5 * (Ki + Kc[SI((i + 1) % nx, j, k)]);
+          const double Km = 0.5 * (Ki + Kc[SI((i + nx - 1) % nx, j, k)]);
+          /* + Kp (C(i+1) - C(i)) - Km (C(i) - C(i-1)) */
+          ac[SI(i, j, k)] -= (Kp + Km) * m;
+          if (i + 1 <= nx - 1) ac[SI(i + 1, j, k)] += Kp * m; else agxL += Kp * m;
+          if (i >= 1) ac[SI(i - 1, j, k)] += Km * m; else agx0 += Km * m;
+        }
+        /* ghosts to cells */
+        if (D->bc_x == 0) { ac[SI(nx - 1, j, k)] += agx0; ac[SI(0, j, k)] += agxL; }
+        else if (D->bc_x == 1) { ac[SI(nx - 1, j, k)] += agxL; }
+        else { ac[SI(0, j, k)] += s0 * agx0; ac[SI(nx - 1, j, k)] += s1 * agxL; }
+      }
+    for (int k = 0; k < nz - 1; ++k)
+      for (int i = 0; i < nx; ++i) {
+        double gy0 = 0.0, gyL = 0.0, s0 = 0.0, s1 = 0.0;
+        if (D->bc_y == 0) { gy0 = CC(i, ny - 1, k); gyL = CC(i, 0, k); }
+        else if (D->bc_y == 1) { gy0 = CC(i, 0, k); }
+        else {
+          s0 = signbit(vi[FI(i, 0, k)]) ? 1.0 : 0.0;
+          s1 = signbit(-vi[FI(i, 0, k)]) ? 1.0 : 0.0;
+          gy0 = CC(i, 0, k) * s0; gyL = CC(i, ny - 1, k) * s1;
+        }
+        double agy0 = 0.0, agyL = 0.0;
+        for (int j = 1; j < ny; ++j) {
+          const size_t f = FI(i, j, k);
+          const double m = mu[SI(i, j, k)] - mu[SI(i, j - 1, k)];
+          double cup, cupup, cdn, au, auu, ad;
+          if (vi[f] >= 0.0) {
+            cup = CC(i, j - 1, k); cupup = (j >= 2) ? CC(i, j - 2, k) : gy0; cdn = CC(i, j, k);
+          } else {
+            cup = CC(i, j, k); cupup = (j + 1 <= ny - 1) ? CC(i, j + 1, k) : gyL; cdn = CC(i, j - 1, k);
+          }
+          avi[f] += smart_face(cup, cupup, cdn) / dy * m;
+          smart_coef(cup, cupup, cdn, &au, &auu, &ad);
+          const double a = acy[f];
+          if (vi[f] >= 0.0) {
+            ac[SI(i, j - 1, k)] += au * a; ac[SI(i, j, k)] += ad * a;
+            if (j >= 2) ac[SI(i, j - 2, k)] += auu * a; else agy0 += auu * a;
+          } else {
+            ac[SI(i, j, k)] += au * a; ac[SI(i, j - 1, k)] += ad * a;
+            if (j + 1 <= ny - 1) ac[SI(i, j + 1, k)] += auu * a; else agyL += auu * a;
+          }
+        }
+        {
+          const double m0 = mu[SI(i, 0, k)], mL = -mu[SI(i, ny - 1, k)];
+          const double a0 = acy[FI(i, 0, k)], aL = acy[FI(i, ny, k)];
+          if (D->bc_y == 0) {
+            double au, auu, ad;
+            int ju, juu, jdn;
+            if (vi[FI(i, 0, k)] >= 0.0) { ju = ny - 1; juu = ny - 2; jdn = 0; }
+            else { ju = 0; juu = 1; jdn = ny - 1; }
+            const double cup = CC(i, ju, k), cupup = CC(i, juu, k), cdn = CC(i, jdn, k);
+            const double f0 = smart_face(cup, cupup, cdn);
+            avi[FI(i, 0, k)] += f0 / dy * m0;
+            avi[FI(i, ny, k)] += f0 / dy * mL;
+            smart_coef(cup, cupup, cdn, &au, &auu, &ad);
+            const double a = a0 + aL;
+            ac[SI(i, ju, k)] += au * a; ac[SI(i, juu, k)] += auu * a; ac[SI(i, jdn, k)] += ad * a;
+          } else if (D->bc_y == 1) {  /* cy(0) = C(0), cy(ny) = 0 */
+            avi[FI(i, 0, k)] += CC(i, 0, k) / dy * m0;
+            ac[SI(i, 0, k)] += a0;
+          } else {
+            avi[FI(i, 0, k)] += gy0 / dy * m0;
+            avi[FI(i, ny, k)] += gyL / dy * mL;
+            agy0 += a0; agyL += aL;
+          }
+        }
+        const double idy2 = 1.0 / (dy * dy);
+        for (int j = 0; j < ny; ++j) {
+          const double m = mu[SI(i, j, k)] * idy2;
+          const double Kj = Kc[SI(i, j, k)];
+          const double Kp = 0.5 * (Kj + Kc[SI(i, (j + 1) % ny, k)]);
+          const double Km = 0.5 * (Kj + Kc[SI(i, (j + ny - 1) % ny, k)]);
+          ac[SI(i, j, k)] -= (Kp + Km) * m;
+          if (j + 1 <= ny - 1) ac[SI(i, j + 1, k)] += Kp * m; else agyL += Kp * m;
+          if (j >= 1) ac[SI(i, j - 1, k)] += Km * m; else agy0 += Km * m;
+        }
+        if (D->bc_y == 0) { ac[SI(i, ny - 1, k)] += agy0; ac[SI(i, 0, k)] += agyL; }
+        else if (D->bc_y == 1) { ac[SI(i, 0, k)] += agy0; }
+        else { ac[SI(i, 0, k)] += s0 * agy0; ac[SI(i, ny - 1, k)] += s1 * agyL; }
+      }
+    /* z faces (:3285-3360) and z diffusion (:3623-3677) */
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        /* cz(nz-1) = cz(nz-2) */
+        acz[FI(i, j, nz - 2)] += acz[FI(i, j, nz - 1)];
+        for (int k = 1; k <= nz - 2; ++k) {
+          const size_t f = FI(i, j, k);
+          double cup, cupup, cdn, au, auu, ad, face;
+          int ku, kuu, kdn;
+          if (k == 1 && wi[f] >= 0.0) {  /* central at the wall side */
+            face = 0.5 * (CC(i, j, 1) + CC(i, j, 0));
+            ku = 1; kdn = 0; kuu = -1; au = 0.5; ad = 0.5; auu = 0.0;
+          } else {
+            if (wi[f] >= 0.0) { ku = k - 1; kuu = k - 2; kdn = k; }
+            else { ku = k; kuu = k + 1; kdn = k - 1; }
+            cup = CC(i, j, ku); cupup = CC(i, j, kuu); cdn = CC(i, j, kdn);
+            face = smart_face(cup, cupup, cdn);
+            smart_coef(cup, cupup, cdn, &au, &auu, &ad);
+          }
+          double m = mu[SI(i, j, k)] * (k <= nz - 2 ? 1.0 : 0.0) - mu[SI(i, j, k - 1)];
+          double at = acz[f];
+          if (k == nz - 2) {  /* face nz-1 holds the same value, carried by w(nz-1) */
+            const double mt = -mu[SI(i, j, nz - 2)];
+            awi[FI(i, j, nz - 1)] += face / dz * mt;
+          }
+          awi[f] += face / dz * m;
+          ac[SI(i, j, ku)] += au * at; ac[SI(i, j, kdn)] += ad * at;
+          if (kuu >= 0) ac[SI(i, j, kuu)] += auu * at;
+        }
+        const double idz2 = 1.0 / (dz * dz);
+        for (int k = 0; k <= nz - 2; ++k) {
+          const double m = mu[SI(i, j, k)] * idz2;
+          if (k <= nz - 3) {  /* + Kp (C(k+1) - C(k)) */
+            const double Kp = 0.5 * (Kc[SI(i, j, k + 1)] + Kc[SI(i, j, k)]);
+            ac[SI(i, j, k + 1)] += Kp * m; ac[SI(i, j, k)] -= Kp * m;
+          }
+          if (k >= 1) {       /* - Km (C(k) - C(k-1)) */
+            const double Km = 0.5 * (Kc[SI(i, j, k)] + Kc[SI(i, j, k - 1)]);
+            ac[SI(i, j, k)] -= Km * m; ac[SI(i, j, k - 1)] += Km * m;
+          }
+        }
+      }
+    memcpy(lb, ac, nv * sizeof(double));
+#undef CC
+  }
+  /* w faces: wi(k) = w(k) (less the settling velocity) */
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) awf[SI(i, j, k)] += awi[FI(i, j, k)];
+  /* u faces: ui0 = M (dx dudx(0..nx-2), sum u), face nx = face 0 */
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j) {
+      double l0 = aui[FI(0, j, k)] + aui[FI(nx, j, k)], lsum = 0.0;
+      for (int i = 0; i < nx; ++i) lsum += (i == 0 ? l0 : aui[FI(i, j, k)]);
+      for (int m = 0; m < nx - 1; ++m) {
+        double acc = 0.0;
+        for (int i = 0; i < nx; ++i)
+          acc += (i <= m ? -(nx - (m + 1)) * (1.0 / nx) : (m + 1) * (1.0 / nx)) * (i == 0 ? l0 : aui[FI(i, j, k)]);
+        aux[SI(m, j, k)] += dx * acc;
+      }
+      for (int m = 0; m < nx; ++m) auf[SI(m, j, k)] += lsum / nx;
+    }
+  for (int k = 0; k < nz; ++k)
+    for (int i = 0; i < nx; ++i) {
+      double l0 = avi[FI(i, 0, k)] + avi[FI(i, ny, k)], lsum = 0.0;
+      for (int j = 0; j < ny; ++j) lsum += (j == 0 ? l0 : avi[FI(i, j, k)]);
+      for (int m = 0; m < ny - 1; ++m) {
+        double acc = 0.0;
+        for (int j = 0; j < ny; ++j)
+          acc += (j <= m ? -(ny - (m + 1)) * (1.0 / ny) : (m + 1) * (1.0 / ny)) * (j == 0 ? l0 : avi[FI(i, j, k)]);
+        avy[SI(i, m, k)] += dy * acc;
+      }
+      for (int m = 0; m < ny; ++m) avf[SI(i, m, k)] += lsum / ny;
+    }
+#undef FI
+#undef SI
+  free(ui); free(vi); free(wi); free(Kc); free(aui); free(avi); free(awi);
+  free(mu); free(ac); free(acx); free(acy); free(acz);
+  return 0;
+}
+
 /* adjoint of one frozen-coefficient step (main.f90:252-826 with the frozen Nu_t, u*).
  * Base: the context's u, v, w before the step and its jt (jt == 1 without initu: the
  * Euler start).  ub: 3 volumes, on entry the adjoint of the step's u, v, w, on return
@@ -1948,6 +2310,14 @@ static void adj_r2c(or_ctx *c, cplx *S, double *f) {
  * scratch (their values are undefined afterwards); droplets and the inflow are not
  * part of it. */
 int or_step_adjoint(or_ctx *c, const double *nut, const double *ust, double *ub, double *rb) {
+  return or_step_adjoint2(c, nut, ust, ub, rb, NULL, NULL);
+}
+
+/* or_step_adjoint with droplets: cb, rcb (npcon volumes each), on entry the adjoints of
+ * the step's PCon and RHS_PCon, on return those of the PCon and RHS_PCon it starts from
+ * (the base C is the context's PCon).  Required when npcon > 0. */
+int or_step_adjoint2(or_ctx *c, const double *nut, const double *ust, double *ub, double *rb, double *cb,
+                     double *rcb) {
   const lp_grid *g = &c->g;
   const lp_config *cf = &c->cfg;
   const int nx = g->nx, ny = g->ny, nz = g->nz, ld = g->ld, lh = g->lh;
@@ -1957,6 +2327,12 @@ int or_step_adjoint(or_ctx *c, const double *nut, const double *ust, double *ub,
   const int first = c->jt == 1 && !cf->initu;
   const int wall = cf->lbc_mom == LP_LBC_WALL;
   const double zl = 0.5 * dz - cf->d0, den = log(zl / cf->z0);
+  if (cf->drop.npcon > 0) {  /* checked before anything is allocated */
+    if (!cb || !rcb) return fail(LP_ERR_ARG, "step adjoint: droplets need the PCon and RHS_PCon adjoints");
+    if (cf->drop.inertia) return fail(LP_ERR_ARG, "droplet adjoint: inertia (dudt in the face velocity) is not differentiated");
+    if (cf->drop.pbe && cf->drop.pbe_feedback)
+      return fail(LP_ERR_ARG, "droplet adjoint: the breakup feedback is not differentiated");
+  }
 #define NEWV(n) double *n = (double *)calloc(nv, sizeof(double))
 #define X_ALL(k, y, x) for (int k = 0; k < nz; ++k) for (int y = 0; y < ny; ++y) for (int x = 0; x < nx; ++x)
 #define IX(k, y, x) ((size_t)(k) * np + (size_t)(y) * ld + (x))
@@ -2223,6 +2599,8 @@ int or_step_adjoint(or_ctx *c, const double *nut, const double *ust, double *ub,
     }
     free(ao1); free(ao2); free(ao3);
   }
+  /* the droplet transport, its buoyancy and its update (main.f90:360-372) */
+  if (cf->drop.npcon > 0) adj_droplets(c, nut, uf, vf, wf, ux, vy, arz, cb, rcb, auf, aux, avf, avy, awf);
   /* divstress_uv / divstress_w: the x, y derivatives are antisymmetric */
   NEWV(atxx); NEWV(atxy); NEWV(atxz); NEWV(atyy); NEWV(atyz); NEWV(atzz);
   {

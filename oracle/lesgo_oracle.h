@@ -74,6 +74,8 @@ void or_calculate_probability(const double *d, int n, double sigma, double d_min
 /* frozen-coefficient steps and their discrete adjoint */
 void or_set_frozen(or_ctx *c, const double *nut, const double *ust);
 int or_step_adjoint(or_ctx *c, const double *nut, const double *ust, double *ub, double *rb);
+int or_step_adjoint2(or_ctx *c, const double *nut, const double *ust, double *ub, double *rb, double *cb,
+                     double *rcb);
 
 #ifdef __cplusplus
 }
