@@ -123,28 +123,34 @@ def droplet_leg(steps, warmup, seed, nx=192, nz=128, nbins=15):
     return out
 
 
-def adjoint_leg(seed, nx=128, nz=65, nsteps=20, every=5):
-    """gradient of a 20-step binary32 run at 128x128x65 (checkpoints every 5 steps) with
-    respect to its initial velocity, by the binary64 discrete adjoint of the
-    frozen-coefficient step; objective: u summed over a 4^3 probe box at every step.
-    Timed after one untimed forward + gradient."""
+def adjoint_leg(seed, nx=128, nz=65, nsteps=20, every=5, nbins=15):
+    """the differentiation workload: gradient of a 20-step binary32 run at 128x128x65 with
+    15 droplet bins (settling, breakup source, buoyancy; no inertia, which the adjoint does
+    not differentiate), checkpoints every 5 steps, with respect to its initial velocity and
+    concentrations, by the binary64 discrete adjoint of the frozen-coefficient step;
+    objective: the concentration of every bin summed over a 4^3 probe box above the source
+    at every step.  Timed after one untimed forward + gradient."""
     import torch
-    from population_balance_amd import adjoint
+    from population_balance_amd import adjoint, droplets
     cfg = abi.make_config(nx, nx, nz, precision=abi.LP_MIXED, use_mean_p_force=1, mean_p_force=1.0)
+    ds = droplets.DropletSet(nbins)
+    ds.fill(cfg, inertia=False)
     g = cfg.grid
     u0, v0, w0 = (torch.from_numpy(a).to("cuda") for a in ic.turbulent_ic(g, seed=seed))
-    probe = adjoint.Probe(0, k0=nz // 4, j0=g.ny // 2, i0=3 * g.nx // 4)
+    c0 = torch.from_numpy(droplets.initial_concentration(cfg, ds)).to("cuda")
+    d = cfg.drop
+    probe = adjoint.Probe("PCon", k0=d.src_k + 2, j0=d.src_j - 2, i0=d.src_i + 3)
     run = adjoint.CheckpointedRun(cfg, nsteps, probe, every=every)
-    run.forward(u0, v0, w0)
+    run.forward(u0, v0, w0, c0)
     run.gradient()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    run.forward(u0, v0, w0)
+    run.forward(u0, v0, w0, c0)
     run.gradient()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    return {"workload": f"gradient of a {nsteps}-step run {nx}x{nx}x{nz} (mixed forward, binary64 adjoint), "
-                        f"checkpoints every {every} steps",
+    return {"workload": f"gradient of a {nsteps}-step run {nx}x{nx}x{nz} with {nbins} droplet bins (mixed forward, "
+                        f"binary64 adjoint), checkpoints every {every} steps, concentration probe",
             "ms_per_adjoint_step": round(run.t_adjoint / run.adjoint_steps, 4),
             "forward_ms_per_step": round(run.t_forward / nsteps, 4),
             "gradient_wall_s": round(wall, 3),

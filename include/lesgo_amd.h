@@ -142,7 +142,8 @@ enum lp_field {
   LP_CS_OPT2, LP_BETA,
   LP_NU_T, LP_DISSIP, LP_MAGS,
   LP_U_LAG, LP_V_LAG, LP_W_LAG,
-  LP_DUDT, LP_DVDT, LP_DWDT,
+  LP_DUDT, LP_DVDT, LP_DWDT,     /* main.f90:433-439,795-803,864-898; lp_step forms them
+                                    only when droplets are on (their only reader)       */
   LP_USTAR,                      /* (ld,ny) wall u* of wallstress                       */
   LP_PCON,                       /* (ld,ny,nz,npcon) [f64]                             */
   LP_RHS_PCON, LP_RHS_PCON_F,    /* (ld,ny,nz,npcon) [f64]                             */
@@ -300,6 +301,14 @@ int lp_set_frozen(lp_ctx *ctx, const void *nut, const void *ust);
  * outputs, on return those of its inputs (ub of u, v, w, rb of RHS_f).  The context's
  * state is left unchanged; its first call allocates the adjoint's scratch. */
 int lp_step_adjoint(lp_ctx *ctx, const double *nut, const double *ust, double *ub, double *rb);
+/* lp_step_adjoint with droplets (required when npcon > 0; the base C is the context's
+ * PCon): cb, rcb (npcon volumes each, binary64 device arrays), on entry the adjoints of
+ * the step's PCon and RHS_PCon, on return those of the PCon and RHS_PCon it starts from
+ * (pollen_all_in_one2, scalars_module_crossflow_mod.f90:1494-1786, with the transport
+ * linear in C on the limiter branch the base C selects).  Inertia (PCon_acc) and the
+ * breakup feedback are not differentiated: LP_ERR_ARG. */
+int lp_step_adjoint2(lp_ctx *ctx, const double *nut, const double *ust, double *ub, double *rb, double *cb,
+                     double *rcb);
 
 /* ---------------- measurement ----------------
  * Kernel classes whose launches can be bracketed by HIP events on the context's

@@ -4,6 +4,7 @@
 // point sums the contributions that reach it), so no atomics are needed.  Levels k are
 // 0-based (Fortran jz = k+1); only the physical columns x < nx carry adjoint values.
 #pragma once
+#include "lp_droplets.hpp"
 #include "lp_kernels.hpp"
 
 namespace lp {
@@ -470,6 +471,343 @@ __global__ void k_tau_frozen(const T *__restrict__ S11, const T *__restrict__ S1
     txz[i] = m2 * nu * S13[i];
     tyz[i] = m2 * nu * S23[i];
   }
+}
+
+// ------------------------------------------------------------------------------------
+// droplets (pollen_all_in_one2, scalars_module_crossflow_mod.f90:1494-1786, with
+// pollen_RHS_calc2 :1894-4472, calcbeta_pcon :1795-1843, step_pollen2 :4481-4508), all
+// bins in one launch per pass, with per-bin face adjoints summed at the end.  Frozen: Kc_t (from Nu_t).  The transport is linear in C on each branch
+// of the SMART limiter: its transpose uses the branch the base C selects.  The three
+// directions are line scans: one thread owns an x row, a y column or a z column and
+// scatters into the cells of its own line only, so the passes need no atomics.
+// ------------------------------------------------------------------------------------
+
+// d smart_face / d (cup, cupup, cdn) on the branch smart_face selects
+__device__ __forceinline__ void smart_coef(double cup, double cupup, double cdn, double &au, double &auu,
+                                           double &ad) {
+  au = 1.0; auu = 0.0; ad = 0.0;
+  const double den = cup - cupup;
+  if (den == 0.0) return;
+  const double num = cdn - cup;
+  const double a = 2.0 * num, b = 0.75 * num + 0.25 * den, c = 4.0 * den;
+  const double pd = den > 0.0 ? fmax(0.0, fmin(fmin(a, b), c)) : fmin(0.0, fmax(fmax(a, b), c));
+  if (pd == 0.0) return;                                              // the upwind cell
+  if (pd == a) { au = 0.0; ad = 1.0; return; }                        // the downwind cell
+  if (pd == b) { au = 0.75; auu = -0.125; ad = 0.375; return; }       // QUICK
+  au = 3.0; auu = -2.0;                                               // psi = 4
+}
+
+// step_pollen2 and calcbeta_pcon transposed (every point of every bin):
+// lb, lr the adjoints of the step's PCon and RHS_PCon (lr becomes that of the RHS_PCon the
+// step reads); mu = adjoint of this step's RHS_PCon; ac = adjoint of the old PCon through
+// the update and the buoyancy (the transport adds to it)
+__global__ void k_adj_drop_update(const double *__restrict__ lb_all, double *__restrict__ lr_all,
+                                  const double *__restrict__ arz, const double *__restrict__ vfac, int active,
+                                  double *__restrict__ mu_all, double *__restrict__ ac_all, DropGeo g, double dt,
+                                  int euler) {
+  const long t = gtid();
+  if (t >= g.nv * g.npc) return;
+  const int b = (int)(t / g.nv);
+  const long s = t % g.nv, ob = (long)b * g.nv;
+  const double *lb = lb_all + ob;
+  double *lr = lr_all + ob, *mu = mu_all + ob, *ac = ac_all + ob;
+  const int k = (int)(s / g.np), x = (int)(s % g.ld);
+  if (k == g.nz - 1 || x >= g.nx) {
+    mu[s] = 0.0; ac[s] = 0.0; lr[s] = 0.0;
+    return;
+  }
+  const double l = lb[s] + (k == g.nz - 2 ? lb[s + g.np] : 0.0);  // the top level copies nz-2
+  mu[s] = lr[s] + (euler ? dt : 1.5 * dt) * l;
+  ac[s] = l + (active ? vfac[b] * arz[s] : 0.0);
+  lr[s] = euler ? 0.0 : -0.5 * dt * l;
+}
+
+struct DropAdjP {
+  double idx, idy, idz, idx2, idy2, idz2;
+  int bc_x, bc_y, settling;
+};
+
+// x: one thread per (y, level < nz-1) row and bin; aui (ld, ny+1, nz) of the bin gains its
+// adjoint of the face velocities
+__global__ void k_adj_drop_x(const double *__restrict__ C, const double *__restrict__ mu, double *__restrict__ ac,
+                             const double *__restrict__ ui0, double *__restrict__ aui, const double *__restrict__ kc,
+                             DropGeo g, DropAdjP P) {
+  const long nl = (long)g.ny * (g.nz - 1);
+  const long t = gtid();
+  if (t >= nl * g.npc) return;
+  const int b = (int)(t / nl);
+  const long row = t % nl;
+  const int k = (int)(row / g.ny), j = (int)(row % g.ny), nx = g.nx;
+  const long o = (long)k * g.np + (long)j * g.ld, ob = (long)b * g.nv;
+  const double *Cr = C + ob + o, *M = mu + ob + o, *K = kc + o;
+  double *A = ac + ob + o;
+  const double *U = ui0 + (long)k * g.fp + (long)j * g.ld;
+  double *AU = aui + (long)b * g.fp * g.nz + (long)k * g.fp + (long)j * g.ld;
+  double gx0 = 0.0, gxL = 0.0, s0 = 0.0, s1 = 0.0;
+  if (P.bc_x == 0) { gx0 = Cr[nx - 1]; gxL = Cr[0]; }
+  else if (P.bc_x == 1) { gxL = Cr[nx - 1]; }
+  else {
+    s0 = signbit(U[0]) ? 1.0 : 0.0;
+    s1 = signbit(-U[0]) ? 1.0 : 0.0;
+    gx0 = Cr[0] * s0; gxL = Cr[nx - 1] * s1;
+  }
+  double agx0 = 0.0, agxL = 0.0;
+  for (int f = 1; f < nx; ++f) {  // interior faces: between cells f-1 and f
+    const double m = M[f] - M[f - 1], uf = U[f], a = uf * P.idx * m;
+    double cup, cupup, cdn, au, auu, ad;
+    const bool pos = uf >= 0.0;
+    if (pos) { cup = Cr[f - 1]; cupup = f >= 2 ? Cr[f - 2] : gx0; cdn = Cr[f]; }
+    else { cup = Cr[f]; cupup = f + 1 <= nx - 1 ? Cr[f + 1] : gxL; cdn = Cr[f - 1]; }
+    AU[f] += smart_face(cup, cupup, cdn) * P.idx * m;
+    smart_coef(cup, cupup, cdn, au, auu, ad);
+    if (pos) {
+      A[f - 1] += au * a; A[f] += ad * a;
+      if (f >= 2) A[f - 2] += auu * a; else agx0 += auu * a;
+    } else {
+      A[f] += au * a; A[f - 1] += ad * a;
+      if (f + 1 <= nx - 1) A[f + 1] += auu * a; else agxL += auu * a;
+    }
+  }
+  {  // faces 0 and nx
+    const double m0 = M[0], mL = -M[nx - 1];
+    const double a0 = U[0] * P.idx * m0, aL = U[nx] * P.idx * mL;
+    if (P.bc_x == 0) {
+      int iu, iuu, idn;
+      if (U[0] >= 0.0) { iu = nx - 1; iuu = nx - 2; idn = 0; }
+      else { iu = 0; iuu = 1; idn = nx - 1; }
+      const double cup = Cr[iu], cupup = Cr[iuu], cdn = Cr[idn];
+      const double f0 = smart_face(cup, cupup, cdn);
+      AU[0] += f0 * P.idx * m0;
+      AU[nx] += f0 * P.idx * mL;
+      double au, auu, ad;
+      smart_coef(cup, cupup, cdn, au, auu, ad);
+      const double a = a0 + aL;
+      A[iu] += au * a; A[iuu] += auu * a; A[idn] += ad * a;
+    } else if (P.bc_x == 1) {  // face 0 carries nothing, face nx the last cell
+      AU[nx] += Cr[nx - 1] * P.idx * mL;
+      A[nx - 1] += aL;
+    } else {                   // the ghosts are the end faces
+      AU[0] += gx0 * P.idx * m0;
+      AU[nx] += gxL * P.idx * mL;
+      agx0 += a0; agxL += aL;
+    }
+  }
+  for (int i = 0; i < nx; ++i) {  // Kc diffusion, ghosts at the row ends
+    const double m = M[i] * P.idx2, Ki = K[i];
+    const double Kp = 0.5 * (Ki + K[i + 1 < nx ? i + 1 : 0]), Km = 0.5 * (Ki + K[i >= 1 ? i - 1 : nx - 1]);
+    A[i] -= (Kp + Km) * m;
+    if (i + 1 <= nx - 1) A[i + 1] += Kp * m; else agxL += Kp * m;
+    if (i >= 1) A[i - 1] += Km * m; else agx0 += Km * m;
+  }
+  if (P.bc_x == 0) { A[nx - 1] += agx0; A[0] += agxL; }
+  else if (P.bc_x == 1) { A[nx - 1] += agxL; }
+  else { A[0] += s0 * agx0; A[nx - 1] += s1 * agxL; }
+}
+
+// y: one thread per (x < nx, level < nz-1) column and bin (consecutive threads: consecutive x)
+__global__ void k_adj_drop_y(const double *__restrict__ C, const double *__restrict__ mu, double *__restrict__ ac,
+                             const double *__restrict__ vi0, double *__restrict__ avi, const double *__restrict__ kc,
+                             DropGeo g, DropAdjP P) {
+  const long nl = (long)g.nx * (g.nz - 1);
+  const long t0 = gtid();
+  if (t0 >= nl * g.npc) return;
+  const int b = (int)(t0 / nl);
+  const long t = t0 % nl;
+  const int k = (int)(t / g.nx), i = (int)(t % g.nx), ny = g.ny;
+  const long o = (long)k * g.np + i, ld = g.ld, ob = (long)b * g.nv;
+  const double *Cc = C + ob + o, *M = mu + ob + o, *K = kc + o;
+  double *A = ac + ob + o;
+  const double *V = vi0 + (long)k * g.fp + i;
+  double *AV = avi + (long)b * g.fp * g.nz + (long)k * g.fp + i;
+#define CY(j) Cc[(long)(j) * ld]
+#define MY(j) M[(long)(j) * ld]
+#define AY(j) A[(long)(j) * ld]
+  double gy0 = 0.0, gyL = 0.0, s0 = 0.0, s1 = 0.0;
+  if (P.bc_y == 0) { gy0 = CY(ny - 1); gyL = CY(0); }
+  else if (P.bc_y == 1) { gy0 = CY(0); }
+  else {
+    s0 = signbit(V[0]) ? 1.0 : 0.0;
+    s1 = signbit(-V[0]) ? 1.0 : 0.0;
+    gy0 = CY(0) * s0; gyL = CY(ny - 1) * s1;
+  }
+  double agy0 = 0.0, agyL = 0.0;
+  for (int f = 1; f < ny; ++f) {
+    const double m = MY(f) - MY(f - 1), vf = V[(long)f * ld], a = vf * P.idy * m;
+    double cup, cupup, cdn, au, auu, ad;
+    const bool pos = vf >= 0.0;
+    if (pos) { cup = CY(f - 1); cupup = f >= 2 ? CY(f - 2) : gy0; cdn = CY(f); }
+    else { cup = CY(f); cupup = f + 1 <= ny - 1 ? CY(f + 1) : gyL; cdn = CY(f - 1); }
+    AV[(long)f * ld] += smart_face(cup, cupup, cdn) * P.idy * m;
+    smart_coef(cup, cupup, cdn, au, auu, ad);
+    if (pos) {
+      AY(f - 1) += au * a; AY(f) += ad * a;
+      if (f >= 2) AY(f - 2) += auu * a; else agy0 += auu * a;
+    } else {
+      AY(f) += au * a; AY(f - 1) += ad * a;
+      if (f + 1 <= ny - 1) AY(f + 1) += auu * a; else agyL += auu * a;
+    }
+  }
+  {
+    const double m0 = MY(0), mL = -MY(ny - 1);
+    const double a0 = V[0] * P.idy * m0, aL = V[(long)ny * ld] * P.idy * mL;
+    if (P.bc_y == 0) {
+      int ju, juu, jdn;
+      if (V[0] >= 0.0) { ju = ny - 1; juu = ny - 2; jdn = 0; }
+      else { ju = 0; juu = 1; jdn = ny - 1; }
+      const double cup = CY(ju), cupup = CY(juu), cdn = CY(jdn);
+      const double f0 = smart_face(cup, cupup, cdn);
+      AV[0] += f0 * P.idy * m0;
+      AV[(long)ny * ld] += f0 * P.idy * mL;
+      double au, auu, ad;
+      smart_coef(cup, cupup, cdn, au, auu, ad);
+      const double a = a0 + aL;
+      AY(ju) += au * a; AY(juu) += auu * a; AY(jdn) += ad * a;
+    } else if (P.bc_y == 1) {  // face 0 is the first cell, face ny carries nothing
+      AV[0] += CY(0) * P.idy * m0;
+      AY(0) += a0;
+    } else {
+      AV[0] += gy0 * P.idy * m0;
+      AV[(long)ny * ld] += gyL * P.idy * mL;
+      agy0 += a0; agyL += aL;
+    }
+  }
+  for (int j = 0; j < ny; ++j) {
+    const double m = MY(j) * P.idy2, Kj = K[(long)j * ld];
+    const double Kp = 0.5 * (Kj + K[(long)(j + 1 < ny ? j + 1 : 0) * ld]);
+    const double Km = 0.5 * (Kj + K[(long)(j >= 1 ? j - 1 : ny - 1) * ld]);
+    AY(j) -= (Kp + Km) * m;
+    if (j + 1 <= ny - 1) AY(j + 1) += Kp * m; else agyL += Kp * m;
+    if (j >= 1) AY(j - 1) += Km * m; else agy0 += Km * m;
+  }
+  if (P.bc_y == 0) { AY(ny - 1) += agy0; AY(0) += agyL; }
+  else if (P.bc_y == 1) { AY(0) += agy0; }
+  else { AY(0) += s0 * agy0; AY(ny - 1) += s1 * agyL; }
+#undef CY
+#undef MY
+#undef AY
+}
+
+// z: one thread per (x < nx, y) column and bin; w faces 1..nz-1 (face 0 carries nothing,
+// face nz-1 repeats nz-2, face 1 is central for upward flow); awi (a volume per bin) gains
+// the bin's adjoint of the w faces
+__global__ void k_adj_drop_z(const double *__restrict__ C, const double *__restrict__ mu, double *__restrict__ ac,
+                             const double *__restrict__ w, double *__restrict__ awi_all, const double *__restrict__ kc,
+                             const double *__restrict__ wsb, DropGeo g, DropAdjP P) {
+  const long nl = (long)g.nx * g.ny;
+  const long t0 = gtid();
+  if (t0 >= nl * g.npc) return;
+  const int b = (int)(t0 / nl);
+  const long t = t0 % nl;
+  const int j = (int)(t / g.nx), i = (int)(t % g.nx), nz = g.nz;
+  const long o = (long)j * g.ld + i, np = g.np, ob = (long)b * g.nv;
+  const double ws = P.settling ? wsb[b] : 0.0;
+  double *awi = awi_all + ob;
+#define CZ(k) C[ob + (long)(k) * np + o]
+#define MZ(k) mu[ob + (long)(k) * np + o]
+#define AZ(k) ac[ob + (long)(k) * np + o]
+#define KZ(k) kc[(long)(k) * np + o]
+  auto wface = [&](int k) { return w[(long)k * np + o] - ((k >= 1 && k <= nz - 2) ? ws : 0.0); };
+  // face nz-1 holds face nz-2's value: its adjoint joins face nz-2's
+  const double atop = wface(nz - 1) * P.idz * (-MZ(nz - 2));
+  for (int k = 1; k <= nz - 2; ++k) {
+    const double wk = wface(k);
+    const double m = MZ(k) - MZ(k - 1);
+    double at = wk * P.idz * m;
+    if (k == nz - 2) at += atop;
+    double au, auu, ad, face;
+    int ku, kuu, kdn;
+    if (k == 1 && wk >= 0.0) {
+      face = 0.5 * (CZ(1) + CZ(0));
+      ku = 1; kdn = 0; kuu = -1; au = 0.5; ad = 0.5; auu = 0.0;
+    } else {
+      if (wk >= 0.0) { ku = k - 1; kuu = k - 2; kdn = k; }
+      else { ku = k; kuu = k + 1; kdn = k - 1; }
+      const double cup = CZ(ku), cupup = CZ(kuu), cdn = CZ(kdn);
+      face = smart_face(cup, cupup, cdn);
+      smart_coef(cup, cupup, cdn, au, auu, ad);
+    }
+    awi[(long)k * np + o] += face * P.idz * m;
+    if (k == nz - 2) awi[(long)(nz - 1) * np + o] += face * P.idz * (-MZ(nz - 2));
+    AZ(ku) += au * at; AZ(kdn) += ad * at;
+    if (kuu >= 0) AZ(kuu) += auu * at;
+  }
+  for (int k = 0; k <= nz - 2; ++k) {
+    const double m = MZ(k) * P.idz2;
+    if (k <= nz - 3) {
+      const double Kp = 0.5 * (KZ(k + 1) + KZ(k));
+      AZ(k + 1) += Kp * m; AZ(k) -= Kp * m;
+    }
+    if (k >= 1) {
+      const double Km = 0.5 * (KZ(k) + KZ(k - 1));
+      AZ(k) -= Km * m; AZ(k - 1) += Km * m;
+    }
+  }
+#undef CZ
+#undef MZ
+#undef AZ
+#undef KZ
+}
+
+// the face-velocity reconstruction transposed, along x (one thread per (y, level) row):
+// ui0(i) = mean(u) + P(i) - mean(P), P(i) = sum_{m<i} dx dudx(m), ui0(nx) = ui0(0).  With
+// l the adjoint of the faces (face nx folded into 0) and S = sum l:
+//   adjoint of dudx(m), m < nx-1: dx (sum_{i>m} l(i) - S (nx-1-m)/nx);  of u(m): S/nx.
+// The same along y (k_adj_face_v, one thread per (x, level) column).
+__global__ void k_adj_face_u(const double *__restrict__ aui, double *__restrict__ aux, double *__restrict__ auf,
+                             DropGeo g, double dx) {
+  const long row = gtid();
+  if (row >= (long)g.ny * g.nz) return;
+  const int k = (int)(row / g.ny), j = (int)(row % g.ny), nx = g.nx;
+  const double *L0 = aui + (long)k * g.fp + (long)j * g.ld;
+  const long fz = g.fp * g.nz;
+  auto L = [&](int i) {  // summed over the bins
+    double v = 0.0;
+    for (int b = 0; b < g.npc; ++b) v += L0[(long)b * fz + i];
+    return v;
+  };
+  const long o = (long)k * g.np + (long)j * g.ld;
+  const double l0 = L(0) + L(nx);
+  double S = l0;
+  for (int i = 1; i < nx; ++i) S += L(i);
+  double tail = S - l0;  // sum_{i > 0} l(i), then sum_{i > m}
+  for (int m = 0; m < nx; ++m) {
+    if (m < nx - 1) aux[o + m] += dx * (tail - S * (double)(nx - 1 - m) / nx);
+    auf[o + m] += S / nx;
+    if (m + 1 < nx) tail -= L(m + 1);
+  }
+}
+__global__ void k_adj_face_v(const double *__restrict__ avi, double *__restrict__ avy, double *__restrict__ avf,
+                             DropGeo g, double dy) {
+  const long t = gtid();
+  if (t >= (long)g.nx * g.nz) return;
+  const int k = (int)(t / g.nx), i = (int)(t % g.nx), ny = g.ny;
+  const long ld = g.ld;
+  const double *L0 = avi + (long)k * g.fp + i;
+  const long fz = g.fp * g.nz;
+  auto L = [&](int j) {  // summed over the bins
+    double v = 0.0;
+    for (int b = 0; b < g.npc; ++b) v += L0[(long)b * fz + (long)j * ld];
+    return v;
+  };
+  const long o = (long)k * g.np + i;
+  const double l0 = L(0) + L(ny);
+  double S = l0;
+  for (int j = 1; j < ny; ++j) S += L(j);
+  double tail = S - l0;
+  for (int m = 0; m < ny; ++m) {
+    if (m < ny - 1) avy[o + (long)m * ld] += dy * (tail - S * (double)(ny - 1 - m) / ny);
+    avf[o + (long)m * ld] += S / ny;
+    if (m + 1 < ny) tail -= L(m + 1);
+  }
+}
+
+// afw += the w-face adjoints summed over the bins
+__global__ void k_adj_face_w(const double *__restrict__ awi, double *__restrict__ afw, DropGeo g) {
+  const long s = gtid();
+  if (s >= g.nv) return;
+  double v = 0.0;
+  for (int b = 0; b < g.npc; ++b) v += awi[(long)b * g.nv + s];
+  afw[s] += v;
 }
 
 }  // namespace lp

@@ -131,10 +131,10 @@ struct lp_ctx {
   void *phat = nullptr;       // binary64 p-hat (0..nz) in the sp layout
   void *uvw_spec[3] = {};     // x-spectra of the filtered u, v, w left by the step's filt_da
   // binary32 context: the per-level values that u and v entered filt_da's transform
-  // shifted by ([2][nz]); the convective products' level means include them again
+  // shifted by ([2][nz]); the convective pass adds them back to the kept spectra
   double *ushift = nullptr;
-  double *cmean = nullptr;  // [6][nz] level means of the padded convec fields (binary32)
   double *nyq = nullptr;    // filt_da of binary64 velocities: A [nz][ny], B [nz][nx], C [nz]
+  double *adj_drop = nullptr;  // droplet adjoint scratch: mu, ac, awi volumes + aui, avi faces
   void *cvb = nullptr;        // convec: y-padded half spectra of u1 u2 u3 w1 w2 w3 [6][nz][ny2][cvp_pitch]
   void *cvp = nullptr;        // convec: half spectra of the 3/2-grid products [3][nz][ny2][cvp_pitch]
   long cv_pitch = 0;          // kx columns kept on the 3/2 grid (nx/2) rounded up to 16
@@ -512,7 +512,7 @@ extern "C" int lp_destroy(lp_ctx *c) {
   for (auto p : c->Hs) fr(p);
   fr(c->bw); fr(c->tw); fr(c->gam); fr(c->phat);
   for (auto p : c->pspec) fr(p);
-  fr(c->d_err); fr(c->d_part); fr(c->d_stats); fr(c->d_espec); fr(c->ushift); fr(c->cmean); fr(c->nyq); fr(c->adj_block);
+  fr(c->d_err); fr(c->d_part); fr(c->d_stats); fr(c->d_espec); fr(c->ushift); fr(c->nyq); fr(c->adj_block); fr(c->adj_drop);
   fr(c->d_ws); fr(c->d_qsrc); fr(c->d_vfac); fr(c->d_foot); fr(c->d_bins); fr(c->d_pbeA);
   fr(c->ui0); fr(c->vi0); fr(c->kcell);
   for (auto e : c->ev) (void)hipEventDestroy(e);
@@ -649,7 +649,6 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
   CK(dalloc(&c->phat, sizeof(double) * 2 * (size_t)c->sp * g.ny * (g.nz + 1)));
   CK(dalloc((void **)&c->d_err, sizeof(int) * 4));
   CK(dalloc((void **)&c->ushift, sizeof(double) * 2 * (size_t)g.nz));
-  CK(dalloc((void **)&c->cmean, sizeof(double) * 6 * (size_t)g.nz));
   CK(dalloc((void **)&c->nyq, sizeof(double) * (size_t)g.nz * (g.ny + g.nx + 1)));
   CK(dalloc((void **)&c->d_stats, sizeof(double) * 10 * (size_t)(g.nz - 1)));
   CK(dalloc((void **)&c->d_espec, sizeof(double) * (size_t)(g.nx / 2) * (g.nz - 1)));
@@ -1161,14 +1160,12 @@ static int do_divstress_all(lp_ctx *c) {
 
 // dealias1.f90 from the x-spectrum on: the column pass that pads it to ny2 and transforms
 // it back along y, into field q of c->cvb; scale multiplies the padded spectrum
-// dc: the per-level kx = ky = 0 entries are stored there and left out of the padded field
-static int pad_y(lp_ctx *c, const void *spec, int q, double scale = 1.0, double *dc = nullptr) {
+static int pad_y(lp_ctx *c, const void *spec, int q, double scale = 1.0) {
   const lp_grid &g = c->g;
   const long ps = c->sp * g.ny, pb = c->cv_pitch * g.ny2;
   const size_t fe = esz_of(c->fdt);
   ColOut o{(char *)c->cvb + (size_t)q * pb * g.nz * 2 * fe, M_ONE};
   o.scale = scale;
-  o.dc = dc;
   prof_begin(c, LP_K_COL_PAD);
   HIPCHK(col_op(c->fdt, g.ny, g.ny2, 1, 1, 1, spec, c->sp, ps, g.nx / 2, g.nz, 1, &o, c->cv_pitch, pb, c->tab,
                 c->s));
@@ -1183,22 +1180,20 @@ static int do_convec(lp_ctx *c, bool kept) {
   Geo g = geo(c);
   const size_t fe = esz_of(c->fdt);
   const double cst = 1.0 / ((double)gg.nx * gg.ny);
-  // convec.f90:76-98.  Binary32: every padded field leaves its level means (the kx = ky
-  // = 0 entries) in c->cmean and enters the products without them (k_convec_cross adds
-  // their part back); u and v are transformed less their first value per level (ushift),
-  // as filt_da transforms them.
-  const bool split = c->fdt == F32;
+  // convec.f90:76-98
   const int vel[3] = {LP_U, LP_V, LP_W};
+  RowIn vin;
+  vin.din = F64;
   for (int q = 0; q < 3; ++q) {
-    if (!kept) {
-      RowIn vin;
-      vin.din = F64;
-      vin.shift = split && q < 2;
-      vin.shift_out = vin.shift ? c->ushift + (size_t)q * gg.nz : nullptr;
-      RCHK(r2c_small(c, c->f[vel[q]], gg.nz, cst, c->uvw_spec[q], 0, vin));
+    if (!kept) RCHK(r2c_small(c, c->f[vel[q]], gg.nz, cst, c->uvw_spec[q], 0, vin));
+    if (kept && c->fdt == F32 && q < 2) {
+      // filt_da transformed u - s, v - s: the row means (kx = 0 column) get s back
+      hipLaunchKernelGGL(k_add_dc<float>, grid1((long)gg.ny * gg.nz), dim3(256), 0, c->s, (cx<float> *)c->uvw_spec[q],
+                         c->sp, gg.ny, gg.nz, (const double *)(c->ushift + (size_t)q * gg.nz));
+      HIPCHK(hipGetLastError());
     }
     // filt_da's spectrum went through an unnormalised y round trip: ny times the r2c
-    RCHK(pad_y(c, c->uvw_spec[q], q, kept ? 1.0 / gg.ny : 1.0, split ? c->cmean + (size_t)q * gg.nz : nullptr));
+    RCHK(pad_y(c, c->uvw_spec[q], q, kept ? 1.0 / gg.ny : 1.0));
   }
   // convec.f90:100-133: each vorticity component formed inside its x transform
   const int va[3] = {LP_DWDY, LP_DWDX, LP_DVDX}, vb[3] = {LP_DVDZ, LP_DUDZ, LP_DUDY};
@@ -1207,7 +1202,7 @@ static int do_convec(lp_ctx *c, bool kept) {
     HIPCHK(row_r2c_vort(c->fdt, gg.nx / 2, q, c->f[va[q]], c->f[vb[q]], gg.ld, plane_el(c), gg.ny, gg.nz, cst,
                         c->cv_spec[0], c->sp, c->sp * gg.ny, gg.nx / 2, c->s));
     prof_end(c, LP_K_ROW_R2C, bytes_r2c(gg.nx, gg.ny, gg.nz, gg.nx / 2, fe) + (double)vol_el(c) * fe);
-    RCHK(pad_y(c, c->cv_spec[0], 3 + q, 1.0, split ? c->cmean + (size_t)(3 + q) * gg.nz : nullptr));
+    RCHK(pad_y(c, c->cv_spec[0], 3 + q));
   }
   // convec.f90:140-158, 180-221, 241-279 products (normalisation folded in) between the
   // x back- and forward transforms of the 3/2 grid, kx < nx/2 kept (dealias2)
@@ -1227,14 +1222,6 @@ static int do_convec(lp_ctx *c, bool kept) {
                   c->cv_pitch, pb, gg.nx / 2, gg.nz - 1, 1, &o, c->sp, ps, c->tab, c->s));
     prof_end(c, LP_K_COL_PAD, bytes_col(gg.ny2, gg.ny, gg.nx / 2, gg.nz - 1, 1, fe));
     RCHK(c2r_small(c, c->cv_spec[1], gg.nx / 2, gg.nz - 1, outs[q], c->fdt));
-  }
-  if (split) {
-    hipLaunchKernelGGL(k_convec_cross<float>, grid1(g.np * (gg.nz - 1)), dim3(256), 0, c->s, (float *)outs[0],
-                       (float *)outs[1], (float *)outs[2], (const double *)c->f[LP_U], (const double *)c->f[LP_V],
-                       (const double *)c->f[LP_W], (const float *)c->f[LP_DUDY], (const float *)c->f[LP_DUDZ],
-                       (const float *)c->f[LP_DVDX], (const float *)c->f[LP_DVDZ], (const float *)c->f[LP_DWDX],
-                       (const float *)c->f[LP_DWDY], (const double *)c->cmean, (const double *)c->ushift, g);
-    HIPCHK(hipGetLastError());
   }
   // convec.f90:306-308 top level BOGUS
   for (int q = 0; q < 3; ++q) {
@@ -1540,8 +1527,8 @@ static int step_body(lp_ctx *c) {
   double force = cf.use_mean_p_force ? cf.mean_p_force : 0.0;
   FDISPATCH(c, T, hipLaunchKernelGGL(k_rhs_ab2<T>, grid1(n1), dim3(256), 0, c->s, (T *)c->f[LP_RHSX],
                                      (T *)c->f[LP_RHSY], (T *)c->f[LP_RHSZ], (const T *)c->f[LP_DIVTX],
-                                     (const T *)c->f[LP_DIVTY], (const T *)c->f[LP_DIVTZ], (T *)c->f[LP_DUDT],
-                                     (T *)c->f[LP_DVDT], (T *)c->f[LP_DWDT],
+                                     (const T *)c->f[LP_DIVTY], (const T *)c->f[LP_DIVTZ],
+                                     (T *)(drops ? c->f[LP_DUDT] : nullptr), (T *)c->f[LP_DVDT], (T *)c->f[LP_DWDT],
                                      (const T *)(drops && cf.drop.active ? c->f[LP_BETA_PCON] : nullptr),
                                      (T *)c->f[LP_RHSX_F], (T *)c->f[LP_RHSY_F], (T *)c->f[LP_RHSZ_F], first ? 1 : 0,
                                      (double *)c->f[LP_U], (double *)c->f[LP_V], (double *)c->f[LP_W], n1, cf.dt,
@@ -1558,7 +1545,7 @@ static int step_body(lp_ctx *c) {
   FDISPATCH(c, T, hipLaunchKernelGGL(k_dpdz_project<T>, grid1(g.np * g.nz), dim3(256), 0, c->s,
                                      (const double *)c->f[LP_P], (T *)c->f[LP_DPDZ], (const T *)c->f[LP_DPDX],
                                      (const T *)c->f[LP_DPDY], (T *)c->f[LP_RHSX], (T *)c->f[LP_RHSY],
-                                     (T *)c->f[LP_RHSZ], (T *)c->f[LP_DUDT], (T *)c->f[LP_DVDT],
+                                     (T *)c->f[LP_RHSZ], (T *)(drops ? c->f[LP_DUDT] : nullptr), (T *)c->f[LP_DVDT],
                                      (T *)c->f[LP_DWDT], (double *)c->f[LP_U], (double *)c->f[LP_V],
                                      (double *)c->f[LP_W], g, gg.dz, cf.dt, c->d_err + 1));
   HIPCHK(hipGetLastError());
@@ -2108,7 +2095,8 @@ constexpr int kAdjNV = 52, kAdjNB = 15;
 // main.f90:252-826 with the frozen Nu_t / u*, transposed (the oracle's or_step_adjoint,
 // kernel for kernel): ub / rb hold on entry the adjoints of the step's u, v, w and of the
 // RHS it leaves, on return those of the u, v, w it starts from and of the RHS_f it reads.
-static int do_step_adjoint(lp_ctx *c, const double *nut, const double *ust, double *ub, double *rb) {
+static int do_step_adjoint(lp_ctx *c, const double *nut, const double *ust, double *ub, double *rb, double *cb,
+                           double *rcb) {
   const lp_grid &gg = c->g;
   const lp_config &cf = c->cfg;
   const long nv = vol_el(c), np = plane_el(c);
@@ -2196,6 +2184,44 @@ static int do_step_adjoint(lp_ctx *c, const double *nut, const double *ust, doub
   hipLaunchKernelGGL(k_adj_ab2, gv, dim3(256), 0, c->s, usx, usy, usz, hx, hy, hz, arx, ary, arz, aft, aftv, bwb, twb,
                      afu, afv, afw, acx, acy, acz, adx, ady, adz, rbx, rby, rbz, ag, dt, first);
   HIPCHK(hipGetLastError());
+  // ---- droplets (main.f90:360-372): update, buoyancy and transport transposed, every
+  // bin in each launch; the face-velocity adjoints reach u, v, w after the strain rates ----
+  const lp_droplet_config &D = cf.drop;
+  const bool drops = D.npcon > 0 && c->jt_total >= D.pcon_init;
+  double *dmu = nullptr, *dac = nullptr, *awi = nullptr, *aui = nullptr, *avi = nullptr;
+  DropGeo dg{};
+  if (drops) {
+    dg = drop_geo(c);
+    const size_t fz = (size_t)dg.fp * gg.nz, nb = (size_t)D.npcon;
+    // mu, ac, awi (a volume per bin) and aui, avi (a face array per bin)
+    if (!c->adj_drop) RCHK(dalloc((void **)&c->adj_drop, sizeof(double) * nb * (3 * (size_t)nv + 2 * fz)));
+    dmu = c->adj_drop; dac = dmu + nb * nv; awi = dac + nb * nv; aui = awi + nb * nv; avi = aui + nb * fz;
+    // base face velocities from the filtered u, v and dudx, dvdy; Kc_t from the frozen Nu_t
+    hipLaunchKernelGGL(k_face_u<double>, dim3((unsigned)(((long)gg.ny * gg.nz + 3) / 4)), dim3(256), 0, c->s, uf, ux,
+                       c->ui0, dg, gg.dx);
+    hipLaunchKernelGGL(k_face_v<double>, dim3((unsigned)((gg.nx + 63) / 64), (unsigned)gg.nz), dim3(64 * kFaceSeg), 0,
+                       c->s, vf, vy, c->vi0, dg, gg.dy);
+    hipLaunchKernelGGL(k_kc<double>, grid1(dg.nv), dim3(256), 0, c->s, nut, (double *)c->f[LP_MAGS], c->kcell, dg,
+                       D.Sc);
+    const long nfill = (long)(nb * (nv + 2 * fz));
+    hipLaunchKernelGGL(k_fill<double>, grid1(nfill), dim3(256), 0, c->s, awi, nfill, 0.0);
+    HIPCHK(hipGetLastError());
+    const int euler = (c->jt_total == D.pcon_init || c->jt_total == D.src_jt0) && !D.init_pcon;
+    DropAdjP P{1.0 / gg.dx, 1.0 / gg.dy, 1.0 / gg.dz, 1.0 / (gg.dx * gg.dx), 1.0 / (gg.dy * gg.dy),
+               1.0 / (gg.dz * gg.dz), D.bc_x, D.bc_y, D.settling};
+    const double *PC = (const double *)c->f[LP_PCON];
+    hipLaunchKernelGGL(k_adj_drop_update, grid1(dg.nv * D.npcon), dim3(256), 0, c->s, (const double *)cb, rcb,
+                       (const double *)arz, (const double *)c->d_vfac, D.active, dmu, dac, dg, dt, euler);
+    hipLaunchKernelGGL(k_adj_drop_x, grid1((long)gg.ny * (gg.nz - 1) * D.npcon), dim3(256), 0, c->s, PC,
+                       (const double *)dmu, dac, (const double *)c->ui0, aui, (const double *)c->kcell, dg, P);
+    hipLaunchKernelGGL(k_adj_drop_y, grid1((long)gg.nx * (gg.nz - 1) * D.npcon), dim3(256), 0, c->s, PC,
+                       (const double *)dmu, dac, (const double *)c->vi0, avi, (const double *)c->kcell, dg, P);
+    hipLaunchKernelGGL(k_adj_drop_z, grid1((long)gg.nx * gg.ny * D.npcon), dim3(256), 0, c->s, PC,
+                       (const double *)dmu, dac, (const double *)wf, awi, (const double *)c->kcell,
+                       (const double *)c->d_ws, dg, P);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(cb, dac, sizeof(double) * nb * nv, hipMemcpyDeviceToDevice, c->s));
+  }
   // ---- convection: P^T, the products transposed, T^T ----
   double *pbx = B[6], *pby = B[7], *pbz = B[8];
   RCHK(to_big(c, acx, gg.nz, pbx));
@@ -2241,6 +2267,14 @@ static int do_step_adjoint(lp_ctx *c, const double *nut, const double *ust, doub
   hipLaunchKernelGGL(k_adj_strain, gv, dim3(256), 0, c->s, aS11, aS12, aS13, aS22, aS23, aS33, ao1, ao2, ao3, aux, auy,
                      avx, avy, awx, awy, afu, afv, afw, ag, 1.0 / dz, 1.0 / (zl * den), wall);
   HIPCHK(hipGetLastError());
+  if (drops) {  // the droplet faces read the filtered u, v, w and dudx, dvdy
+    hipLaunchKernelGGL(k_adj_face_u, grid1((long)gg.ny * gg.nz), dim3(256), 0, c->s, (const double *)aui, aux, afu, dg,
+                       gg.dx);
+    hipLaunchKernelGGL(k_adj_face_v, grid1((long)gg.nx * gg.nz), dim3(256), 0, c->s, (const double *)avi, avy, afv, dg,
+                       gg.dy);
+    hipLaunchKernelGGL(k_adj_face_w, gv, dim3(256), 0, c->s, (const double *)awi, afw, dg);
+    HIPCHK(hipGetLastError());
+  }
   // ---- filt_da transposed: F(af) - Dx(a*x) - Dy(a*y) ----
   double *ff = V[9], *gx = V[10], *gy = V[11];
   double *axq[3] = {aux, avx, awx}, *ayq[3] = {auy, avy, awy};
@@ -2256,17 +2290,27 @@ static int do_step_adjoint(lp_ctx *c, const double *nut, const double *ust, doub
   return LP_OK;
 }
 
-extern "C" int lp_step_adjoint(lp_ctx *c, const double *nut, const double *ust, double *ub, double *rb) {
+extern "C" int lp_step_adjoint2(lp_ctx *c, const double *nut, const double *ust, double *ub, double *rb, double *cb,
+                                double *rcb) {
   if (!c || !nut || !ust || !ub || !rb) return set_err(LP_ERR_ARG, "lp_step_adjoint: null argument");
   if (c->fdt != F64)
     return set_err(LP_ERR_ARG, "lp_step_adjoint runs in binary64: use an LP_FP64 context (the base states of a "
                                "binary32 run carry over, u, v, w being binary64 in both)");
-  if (c->cfg.drop.npcon > 0 && c->cfg.drop.active)
-    return set_err(LP_ERR_ARG, "lp_step_adjoint: active droplets (buoyancy feedback) are not part of the adjoint");
+  const lp_droplet_config &D = c->cfg.drop;
+  if (D.npcon > 0) {
+    if (!cb || !rcb) return set_err(LP_ERR_ARG, "lp_step_adjoint: droplets need the PCon and RHS_PCon adjoints");
+    if (D.inertia) return set_err(LP_ERR_ARG, "lp_step_adjoint: droplet inertia (dudt in the face velocity) is not differentiated");
+    if (D.pbe && D.pbe_feedback) return set_err(LP_ERR_ARG, "lp_step_adjoint: the breakup feedback is not differentiated");
+  }
   HIPCHK(hipSetDevice(c->dev));
-  if (!is_device_ptr(nut) || !is_device_ptr(ust) || !is_device_ptr(ub) || !is_device_ptr(rb))
+  if (!is_device_ptr(nut) || !is_device_ptr(ust) || !is_device_ptr(ub) || !is_device_ptr(rb) ||
+      (D.npcon > 0 && (!is_device_ptr(cb) || !is_device_ptr(rcb))))
     return set_err(LP_ERR_ARG, "lp_step_adjoint: device arrays expected");
-  RCHK(do_step_adjoint(c, nut, ust, ub, rb));
+  RCHK(do_step_adjoint(c, nut, ust, ub, rb, cb, rcb));
   HIPCHK(hipStreamSynchronize(c->s));
   return LP_OK;
+}
+
+extern "C" int lp_step_adjoint(lp_ctx *c, const double *nut, const double *ust, double *ub, double *rb) {
+  return lp_step_adjoint2(c, nut, ust, ub, rb, nullptr, nullptr);
 }

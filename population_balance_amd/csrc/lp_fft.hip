@@ -273,7 +273,6 @@ struct ColSlot {
   int tpitch;  // row length of the multiplier tables (lh of the small grid)
   int ncols;   // columns stored (kx < ncols)
   double scale;  // factor on the padded / truncated spectrum (padmode 1, 2)
-  double *dc;    // padmode 1: per-plane kx = ky = 0 entry, removed from the output
 };
 struct ColParams {
   ColSlot outs[3];
@@ -391,13 +390,6 @@ __global__ __launch_bounds__(TX * kColLanes) void k_col_op(
           else if (yo > NYO / 2) x = sp[q + DQ];
         }
         v[q] = scaled ? mk<T>(x.x * osc, x.y * osc) : x;
-      }
-      if constexpr (PM == 1) {
-        double *dc = Pr.outs[jo].dc;
-        if (dc && kx == 0 && t == 0) {  // lane 0 of column 0 holds ky = 0 in v[0]
-          dc[plane] = (double)v[0].x;
-          v[0] = mk<T>(0, 0);
-        }
       }
     }
     // addresses are formed from per-output values so that they are not hoisted out of
@@ -680,14 +672,13 @@ static hipError_t launch_col(int padmode, int fwd, int inv, const void *in, long
   P.nout = nout;
   for (int j = 0; j < 3; ++j) {
     if (j >= nout) {
-      P.outs[j] = ColSlot{nullptr, 0, 0, 0, 0, 0, 1.0, nullptr};
+      P.outs[j] = ColSlot{nullptr, 0, 0, 0, 0, 0, 1.0};
       continue;
     }
     const ColOut &o = outs[j];
     if (padmode == 0 && o.scale != 1.0) return hipErrorInvalidValue;
-    if (padmode != 1 && o.dc) return hipErrorInvalidValue;
     P.outs[j] = ColSlot{o.ptr, o.pitch ? o.pitch : out_pitch, o.plane ? o.plane : out_plane, o.mult, tab.lh,
-                        o.ncols ? o.ncols : ncols, o.scale, o.dc};
+                        o.ncols ? o.ncols : ncols, o.scale};
   }
   if (col0 > 0) {
     if constexpr (NYI == NYO) {

@@ -66,8 +66,6 @@ struct ColOut {
   long plane = 0;
   int ncols = 0;
   double scale = 1.0;  // padmode 1 and 2 only: factor on the output spectrum
-  double *dc = nullptr;  // padmode 1 only: the kx = ky = 0 entry of each plane is stored
-                         // here (real part, after scale) and left out of the output
 };
 
 // Column pass over ncols kx-columns of nplanes spectra:

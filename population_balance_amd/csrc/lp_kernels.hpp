@@ -35,6 +35,18 @@ __global__ void k_ddz_w(T *__restrict__ dfdz, const TI *__restrict__ f, Geo g, T
   else if (x < g.nx) dfdz[i] = (T)(cst * (f[i + g.np] - f[i]));
 }
 
+// the kx = 0 column of x-spectra in physical y ([nplanes][ny][pitch] complex): add a
+// per-plane value to the real part of every y, i.e. to the plane's mean (the inverse of
+// row_r2c's shift on a spectrum that keeps the row means there)
+template <typename T>
+__global__ void k_add_dc(cx<T> *__restrict__ spec, long pitch, int ny, int nplanes, const double *__restrict__ dc) {
+  long i = gtid();
+  if (i >= (long)ny * nplanes) return;
+  const int k = (int)(i / ny);
+  cx<T> &e = spec[i * pitch];
+  e.x = (T)((double)e.x + dc[k]);
+}
+
 // filt_da.f90's zeroing of the kx = nx/2 and ky = ny/2 modes ("oddballs") on a binary64
 // field, exactly, without a round trip through binary32 transforms.  Per level those
 // modes sum to
@@ -451,9 +463,11 @@ __global__ void k_rhs_ab2(T *__restrict__ Rx, T *__restrict__ Ry, T *__restrict_
   Rx[i] = rx;
   Ry[i] = ry;
   Rz[i] = rz;
-  ax[i] = -dx_[i] + mpf;
-  ay[i] = -dy_[i];
-  az[i] = -dz_[i];
+  if (ax) {  // dudt, dvdt, dwdt: formed when droplets read them (null otherwise)
+    ax[i] = -dx_[i] + mpf;
+    ay[i] = -dy_[i];
+    az[i] = -dz_[i];
+  }
   T fx, fy, fz;
   if (first) {
     fx = rx; fy = ry; fz = rz;
@@ -465,55 +479,6 @@ __global__ void k_rhs_ab2(T *__restrict__ Rx, T *__restrict__ Ry, T *__restrict_
   u[i] = u[i] + dt * (t1 * (double)rx + t2 * (double)fx + force);
   v[i] = v[i] + dt * (t1 * (double)ry + t2 * (double)fy);
   w[i] = w[i] + dt * (t1 * (double)rz + t2 * (double)fz);
-}
-
-// convec.f90:140-296 in binary32 with the level means split off.  The 3/2-grid fields
-// entered the products with their kx = ky = 0 mode removed (a' = a - A per level, A
-// recorded by the padding pass), so the transforms carry only the fluctuations.  For
-// every product a b the rest, A b + B a - A B, is added here on the small grid in
-// binary64: b and a are band-limited (filtered velocities, spectral x/y derivatives, z
-// differences of filtered fields above the wall), so their padding and truncation leave
-// them unchanged and the sum equals the dealiased product.  The wall level's vorticity
-// (wall-model dudz, dvdz) enters no product.
-// A: [6][nz] means of u, v, w, omega1, omega2, omega3 as padded (u, v: less ush, the
-// shift their transforms took out, which is added back here).
-template <typename T>
-__global__ void k_convec_cross(T *__restrict__ cx_, T *__restrict__ cy_, T *__restrict__ cz_,
-                               const double *__restrict__ u, const double *__restrict__ v,
-                               const double *__restrict__ w, const T *__restrict__ dudy,
-                               const T *__restrict__ dudz, const T *__restrict__ dvdx, const T *__restrict__ dvdz,
-                               const T *__restrict__ dwdx, const T *__restrict__ dwdy,
-                               const double *__restrict__ A, const double *__restrict__ ush, Geo g) {
-  const long i = gtid();
-  if (i >= g.np * (g.nz - 1)) return;
-  const int x = (int)(i % g.ld);
-  if (x >= g.nx) return;
-  const int k = (int)(i / g.np);
-  const long up = i + g.np, dn = i - g.np;
-  const int nz = g.nz;
-  auto Au = [&](int l) { return A[l] + ush[l]; };
-  auto Av = [&](int l) { return A[nz + l] + ush[nz + l]; };
-  auto Aw = [&](int l) { return A[2 * nz + l]; };
-  auto O1 = [&](int l) { return A[3 * nz + l]; };
-  auto O2 = [&](int l) { return A[4 * nz + l]; };
-  auto O3 = [&](int l) { return A[5 * nz + l]; };
-  // om1 = dwdy - dvdz, om2 = dudz - dwdx (w nodes above the wall), om3 = dvdx - dudy
-  const double om3 = (double)dvdx[i] - (double)dudy[i];
-  const double om1u = (double)dwdy[up] - (double)dvdz[up], om2u = (double)dudz[up] - (double)dwdx[up];
-  // A b + B a - A B
-  auto X = [](double Aa, double b, double Bb, double a) { return Aa * b + Bb * a - Aa * Bb; };
-  double rx = -X(Av(k), om3, O3(k), v[i]) + 0.5 * X(Aw(k + 1), om2u, O2(k + 1), w[up]);
-  double ry = X(Au(k), om3, O3(k), u[i]) - 0.5 * X(Aw(k + 1), om1u, O1(k + 1), w[up]);
-  if (k >= 1) {
-    const double om1 = (double)dwdy[i] - (double)dvdz[i], om2 = (double)dudz[i] - (double)dwdx[i];
-    rx += 0.5 * X(Aw(k), om2, O2(k), w[i]);
-    ry -= 0.5 * X(Aw(k), om1, O1(k), w[i]);
-    const double su = Au(k) + Au(k - 1), sv = Av(k) + Av(k - 1);
-    const double rz = 0.5 * (-X(su, om2, O2(k), u[i] + u[dn]) + X(sv, om1, O1(k), v[i] + v[dn]));
-    cz_[i] = (T)((double)cz_[i] + rz);
-  }
-  cx_[i] = (T)((double)cx_[i] + rx);
-  cy_[i] = (T)((double)cy_[i] + ry);
 }
 
 // main.f90:789-803 (dpdz from p in binary64, RHS -= grad p, dudt -= grad p) and the
@@ -535,9 +500,11 @@ __global__ void k_dpdz_project(const double *__restrict__ p, T *__restrict__ dpd
   Rx[i] -= px;
   Ry[i] -= py;
   Rz[i] -= d;
-  ax[i] -= px;
-  ay[i] -= py;
-  az[i] -= d;
+  if (ax) {
+    ax[i] -= px;
+    ay[i] -= py;
+    az[i] -= d;
+  }
   if (x >= g.nx) return;
   const double t1 = 1.5;
   const double un = (u[i] + dt * (-t1 * (double)px + 0.0));

@@ -87,6 +87,7 @@ def load():
         "lp_get_precision": ([vp, C.POINTER(i32)], C.c_int),
         "lp_set_frozen": ([vp, vp, vp], C.c_int),
         "lp_step_adjoint": ([vp, vp, vp, vp, vp], C.c_int),
+        "lp_step_adjoint2": ([vp, vp, vp, vp, vp, vp, vp], C.c_int),
         "lp_pollen_all_in_one2": ([vp], C.c_int),
         "lp_pollen_rhs_calc2": ([vp], C.c_int),
         "lp_calcbeta_pcon": ([vp], C.c_int),
@@ -219,11 +220,13 @@ class Lesgo:
         self._frozen = (nut, ust)  # keep the tensors alive while the context uses them
         self._check(self.L.lp_set_frozen(self.h, _ptr(nut), _ptr(ust)), "set_frozen")
 
-    def step_adjoint(self, nut, ust, ub, rb):
+    def step_adjoint(self, nut, ust, ub, rb, cb=None, rcb=None):
         """the adjoint of one frozen-coefficient step at the context's state (binary64
         device tensors: nut, ust the step's coefficients; ub (3 volumes), rb (3 volumes)
-        updated in place from the adjoints of its outputs to those of its inputs)"""
-        self._check(self.L.lp_step_adjoint(self.h, _ptr(nut), _ptr(ust), _ptr(ub), _ptr(rb)), "step_adjoint")
+        and, with droplets, cb, rcb (npcon volumes: PCon, RHS_PCon) updated in place from
+        the adjoints of its outputs to those of its inputs)"""
+        self._check(self.L.lp_step_adjoint2(self.h, _ptr(nut), _ptr(ust), _ptr(ub), _ptr(rb), _ptr(cb), _ptr(rcb)),
+                    "step_adjoint")
 
     def set_time(self, jt, jt_total=None):
         self._check(self.L.lp_set_time(self.h, jt, jt if jt_total is None else jt_total), "lp_set_time")
