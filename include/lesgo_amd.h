@@ -249,6 +249,23 @@ int lp_rmsdiv(lp_ctx *ctx, double *rmsdiv);     /* rmsdiv.f90:1-44              
  * <txz> over x < nx; u', v' on the w-node, all three fluctuations zero on the first
  * level.  The running averages and the output files stay with the caller. */
 int lp_plane_stats(lp_ctx *ctx, double *out);
+/* energy.f90:1-47: kea = mean of (u^2 + v^2 + w_uv^2)/2 over the uv nodes of levels
+ * 1..nz-1 (w_uv = (w(k) + w(k+1))/2), binary64. */
+int lp_energy(lp_ctx *ctx, double *kea);
+/* Running time averages, io_crossflow_mod.f90:2452-2799.  lp_tavg_reset is tavg_init (the
+ * accumulators and tavg_total_time start at zero); lp_tavg_compute(ctx, tavg_dt) is
+ * tavg_compute: every accumulator gains tavg_dt times its quantity at the current state
+ * (the host counts tavg_dt as the reference's output loop does, :255-280).  Quantity q
+ * (binary64 (ld,ny,nz), levels 1..nz): 0 u, 1 v, 2 w_uv, 3 u_w, 4 v_w, 5 w, 6 u2, 7 v2,
+ * 8 w2, 9 uv, 10 uw, 11 vw, 12 txx, 13 tyy, 14 tzz, 15 txy, 16 txz, 17 tyz,
+ * 18 p - (u^2 + w_uv^2 + v^2)/2, 19 dudx, 20 dudy, 21 dudz_uv, 22 dvdx, 23 dvdy, 24 dvdz_uv,
+ * 25 dwdx_uv, 26 dwdy_uv, 27 dwdz; with droplets 28 tot_pc, 29 tot_pc2, 30+2b pcon and
+ * 31+2b pcon2 of bin b (volume fractions pcon pi/6 d^3).  The droplets' diffusive fluxes
+ * nudPCondx/y are not accumulated.  lp_tavg_get copies quantity q (host or device out)
+ * and returns tavg_total_time. */
+int lp_tavg_reset(lp_ctx *ctx);
+int lp_tavg_compute(lp_ctx *ctx, double tavg_dt);
+int lp_tavg_get(lp_ctx *ctx, int32_t q, void *out, size_t bytes, double *total_time);
 /* io.f90:1643-1679 (spectrum), called on every level as avg_stats (io.f90:1846-1848,
  * E11 before its division by z) and post_spec (io.f90:1587-1594) do: out (host,
  * binary64) holds (nz-1) rows of nx/2, row k the y-average of the streamwise power of

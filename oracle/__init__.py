@@ -71,6 +71,9 @@ def lib():
         L.or_timestep_conditions.argtypes = [vp, dp, dp]
         L.or_rmsdiv.argtypes = [vp, dp]
         L.or_plane_stats.argtypes = [vp, dp]
+        L.or_energy.argtypes = [vp, dp]
+        L.or_energy.restype = C.c_double
+        L.or_tavg_compute.argtypes = [vp, C.c_double, dp]
         L.or_spectrum.argtypes = [vp, C.c_int, C.c_int, dp]
         L.or_spectrum.restype = C.c_int
         L.or_ke_dudt.argtypes = [vp]
@@ -285,6 +288,18 @@ class Oracle:
         a = C.c_double()
         self.L.or_rmsdiv(self.h, C.byref(a))
         return a.value
+
+    def energy(self):
+        """energy.f90:1-47: (kea, ke volume)"""
+        g = self.cfg.grid
+        ke = np.zeros((g.nz, g.ny, g.ld))
+        kea = self.L.or_energy(self.h, ke.ctypes.data_as(C.POINTER(C.c_double)))
+        return kea, ke
+
+    def tavg_compute(self, tavg_dt, acc):
+        """io_crossflow_mod.f90:2647-2799 into acc ((nq, nz, ny, ld), binary64, C-contiguous)"""
+        assert acc.dtype == np.float64 and acc.flags["C_CONTIGUOUS"]
+        self.L.or_tavg_compute(self.h, float(tavg_dt), acc.ctypes.data_as(C.POINTER(C.c_double)))
 
     def plane_stats(self):
         out = np.zeros((self.cfg.grid.nz - 1, 10))

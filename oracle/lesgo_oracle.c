@@ -1248,6 +1248,83 @@ void or_rmsdiv(or_ctx *c, double *rms) {
  * <u'w'>, <txz>, plane means over x < nx.  u' and v' are moved onto the w-node (mean of
  * levels jz and jz-1, each minus its own plane mean), w' = w - <w>; at jz = 1 the
  * serial branch sets all three to zero (:1811-1815). */
+/* energy.f90:1-47 (serial): ke = (u^2 + v^2 + w_uv^2)/2 on the uv nodes of levels
+ * 1..nz-1, w_uv = (w(k) + w(k+1))/2, and its mean over those nx ny (nz-1) points, summed in
+ * the reference's loop order (jz, jy, jx; each term divided by the count).  ke (volume,
+ * may be null) receives the field. */
+double or_energy(or_ctx *c, double *ke) {
+  const lp_grid *g = &c->g;
+  const size_t np = plane(c);
+  const double *u = c->f[LP_U], *v = c->f[LP_V], *w = c->f[LP_W];
+  const double denom = (double)g->nx * g->ny * (g->nz - 1);
+  double kea = 0.0;
+  for (int k = 0; k < g->nz - 1; ++k)
+    for (int y = 0; y < g->ny; ++y)
+      for (int x = 0; x < g->nx; ++x) {
+        const size_t i = (size_t)k * np + (size_t)y * g->ld + x;
+        const double tw = 0.5 * (w[i] + w[i + np]);
+        const double e = 0.5 * (u[i] * u[i] + v[i] * v[i] + tw * tw);
+        if (ke) ke[i] = e;
+        kea = kea + e / denom;
+      }
+  return kea;
+}
+
+/* io_crossflow_mod.f90:2647-2799 tavg_compute (serial, lbz = 1, levels 1..nz): acc gains
+ * tavg_dt times each quantity, in the order of the accumulator struct:
+ *   0 u, 1 v, 2 w_uv, 3 u_w, 4 v_w, 5 w, 6 u2, 7 v2, 8 w2, 9 uv, 10 uw, 11 vw,
+ *   12 txx, 13 tyy, 14 tzz, 15 txy, 16 txz, 17 tyz, 18 p (p - (u^2 + w_uv^2 + v^2)/2),
+ *   19 dudx, 20 dudy, 21 dudz_uv, 22 dvdx, 23 dvdy, 24 dvdz_uv, 25 dwdx_uv, 26 dwdy_uv,
+ *   27 dwdz, 28 tot_pc, 29 tot_pc2, then per bin 30 + 2b pcon, 31 + 2b pcon2
+ * (with droplets; the concentrations as volume fractions pcon pi/6 d^3).  Each quantity
+ * is an (ld, ny, nz) volume; columns x >= nx are left alone.  interp_to_uv_grid
+ * (functions.f90:53-128): f_uv(k) = (f(k) + f(k+1))/2, the top level repeats the one below;
+ * interp_to_w_grid (:131-187): f_w(k) = (f(k-1) + f(k))/2, with u_w and v_w zeroed at the
+ * bottom and top levels (:2688-2691).  The diffusive droplet fluxes nudPCondx/y are not
+ * accumulated. */
+void or_tavg_compute(or_ctx *c, double tavg_dt, double *acc) {
+  const lp_grid *g = &c->g;
+  const size_t np = plane(c), nv = vol(c);
+  const int nz = g->nz, npc = c->cfg.drop.npcon;
+  double **f = c->f;
+  const double *u = f[LP_U], *v = f[LP_V], *w = f[LP_W], *p = f[LP_P];
+  for (int k = 0; k < nz; ++k)
+    for (int y = 0; y < g->ny; ++y)
+      for (int x = 0; x < g->nx; ++x) {
+        const size_t i = (size_t)k * np + (size_t)y * g->ld + x;
+        /* w-grid field to the uv grid */
+        const int ku = k < nz - 1 ? k : nz - 2;
+        const size_t iu = (size_t)ku * np + (size_t)y * g->ld + x;
+#define UVG(F) (0.5 * (f[F][iu] + f[F][iu + np]))
+        const double w_uv = 0.5 * (w[iu] + w[iu + np]);
+        const double u_w = (k == 0 || k == nz - 1) ? 0.0 : 0.5 * (u[i - np] + u[i]);
+        const double v_w = (k == 0 || k == nz - 1) ? 0.0 : 0.5 * (v[i - np] + v[i]);
+        const double up = u[i], vp = v[i], wp = w[i];
+        const double pres = p[i + np] - 0.5 * (up * up + w_uv * w_uv + vp * vp);
+        double q[30];
+        q[0] = up; q[1] = vp; q[2] = w_uv; q[3] = u_w; q[4] = v_w; q[5] = wp;
+        q[6] = up * up; q[7] = vp * vp; q[8] = wp * wp; q[9] = up * vp; q[10] = u_w * wp; q[11] = v_w * wp;
+        q[12] = f[LP_TXX][i]; q[13] = f[LP_TYY][i]; q[14] = f[LP_TZZ][i];
+        q[15] = f[LP_TXY][i]; q[16] = f[LP_TXZ][i]; q[17] = f[LP_TYZ][i];
+        q[18] = pres;
+        q[19] = f[LP_DUDX][i]; q[20] = f[LP_DUDY][i]; q[21] = UVG(LP_DUDZ);
+        q[22] = f[LP_DVDX][i]; q[23] = f[LP_DVDY][i]; q[24] = UVG(LP_DVDZ);
+        q[25] = UVG(LP_DWDX); q[26] = UVG(LP_DWDY); q[27] = f[LP_DWDZ][i];
+#undef UVG
+        double tot = 0.0;
+        for (int b = 0; b < npc; ++b) {
+          const double d = c->cfg.drop.diameter[b];
+          const double pc = f[LP_PCON][(size_t)b * nv + i] * PI_D / 6 * (d * d * d);
+          tot = tot + pc;
+          acc[(size_t)(30 + 2 * b) * nv + i] += pc * tavg_dt;
+          acc[(size_t)(31 + 2 * b) * nv + i] += pc * pc * tavg_dt;
+        }
+        q[28] = tot; q[29] = tot * tot;
+        const int nq = npc > 0 ? 30 : 28;
+        for (int m = 0; m < nq; ++m) acc[(size_t)m * nv + i] += q[m] * tavg_dt;
+      }
+}
+
 void or_plane_stats(or_ctx *c, double *out) {
   const lp_grid *g = &c->g;
   size_t np = plane(c);

@@ -63,6 +63,8 @@ int or_project(or_ctx *c);
 void or_timestep_conditions(or_ctx *c, double *cfl, double *visc_stab);
 void or_rmsdiv(or_ctx *c, double *rms);
 void or_plane_stats(or_ctx *c, double *out);
+double or_energy(or_ctx *c, double *ke);
+void or_tavg_compute(or_ctx *c, double tavg_dt, double *acc);
 int or_spectrum(or_ctx *c, int field, int bin, double *out);
 int or_step(or_ctx *c, int nsteps);
 int or_pollen_all_in_one2(or_ctx *c);

@@ -135,6 +135,10 @@ struct lp_ctx {
   double *ushift = nullptr;
   double *nyq = nullptr;    // filt_da of binary64 velocities: A [nz][ny], B [nz][nx], C [nz]
   double *adj_drop = nullptr;  // droplet adjoint scratch: mu, ac, awi volumes + aui, avi faces
+  double *tavg = nullptr;      // running time averages (tavg_nq binary64 volumes)
+  int tavg_nq = 0;
+  double tavg_time = 0.0;      // tavg_total_time
+  double *d_pvol = nullptr;    // per bin: pi/6 d^3 (tavg's volume fractions)
   void *cvb = nullptr;        // convec: y-padded half spectra of u1 u2 u3 w1 w2 w3 [6][nz][ny2][cvp_pitch]
   void *cvp = nullptr;        // convec: half spectra of the 3/2-grid products [3][nz][ny2][cvp_pitch]
   long cv_pitch = 0;          // kx columns kept on the 3/2 grid (nx/2) rounded up to 16
@@ -418,6 +422,9 @@ static int droplet_setup(lp_ctx *c) {
   RCHK(upload_vec((void **)&c->d_ws, D.settling_vel, sizeof(double) * n));
   RCHK(upload_vec((void **)&c->d_qsrc, D.Q_src, sizeof(double) * n));
   RCHK(upload_vec((void **)&c->d_vfac, vfac.data(), sizeof(double) * n));
+  std::vector<double> pvol(n);
+  for (int b = 0; b < n; ++b) pvol[b] = M_PI / 6 * (D.diameter[b] * D.diameter[b] * D.diameter[b]);
+  RCHK(upload_vec((void **)&c->d_pvol, pvol.data(), sizeof(double) * n));
   // :4222-4256 footprint normalised to unit sum
   std::vector<double> foot((size_t)g.nx * g.ny, 0.0);
   if (D.src_jt0 < D.src_jt1) {
@@ -512,7 +519,7 @@ extern "C" int lp_destroy(lp_ctx *c) {
   for (auto p : c->Hs) fr(p);
   fr(c->bw); fr(c->tw); fr(c->gam); fr(c->phat);
   for (auto p : c->pspec) fr(p);
-  fr(c->d_err); fr(c->d_part); fr(c->d_stats); fr(c->d_espec); fr(c->ushift); fr(c->nyq); fr(c->adj_block); fr(c->adj_drop);
+  fr(c->d_err); fr(c->d_part); fr(c->d_stats); fr(c->d_espec); fr(c->ushift); fr(c->nyq); fr(c->adj_block); fr(c->adj_drop); fr(c->tavg); fr(c->d_pvol);
   fr(c->d_ws); fr(c->d_qsrc); fr(c->d_vfac); fr(c->d_foot); fr(c->d_bins); fr(c->d_pbeA);
   fr(c->ui0); fr(c->vi0); fr(c->kcell);
   for (auto e : c->ev) (void)hipEventDestroy(e);
@@ -1768,6 +1775,81 @@ extern "C" int lp_plane_stats(lp_ctx *c, double *out) {
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, c->d_stats, sizeof(double) * 10 * (size_t)nl, hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
+  return LP_OK;
+}
+
+// energy.f90:1-47: the mean kinetic energy of the uv nodes of levels 1..nz-1 (the level
+// sums on the device in binary64, added level by level on the host)
+extern "C" int lp_energy(lp_ctx *c, double *kea) {
+  if (!c || !kea) return set_err(LP_ERR_ARG, "lp_energy: null argument");
+  HIPCHK(hipSetDevice(c->dev));
+  Geo g = geo(c);
+  const int nl = g.nz - 1;
+  hipLaunchKernelGGL(k_level_ke, dim3(nl), dim3(256), 0, c->s, (const double *)c->f[LP_U],
+                     (const double *)c->f[LP_V], (const double *)c->f[LP_W], g, c->d_stats);
+  HIPCHK(hipGetLastError());
+  std::vector<double> lev(nl);
+  HIPCHK(hipMemcpyAsync(lev.data(), c->d_stats, sizeof(double) * nl, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  double s = 0.0;
+  for (int k = 0; k < nl; ++k) s += lev[k];
+  *kea = s / ((double)g.nx * g.ny * nl);
+  return LP_OK;
+}
+
+// io_crossflow_mod.f90:2452-2642 tavg_init (the accumulators start at zero) and
+// :2647-2799 tavg_compute
+static int tavg_alloc(lp_ctx *c) {
+  const long nv = vol_el(c);
+  const int npc = c->cfg.drop.npcon;
+  const int nq = npc > 0 ? 30 + 2 * npc : 28;
+  if (!c->tavg) {
+    RCHK(dalloc((void **)&c->tavg, sizeof(double) * (size_t)nq * nv));
+    c->tavg_nq = nq;
+  }
+  hipLaunchKernelGGL(k_fill<double>, grid1((long)nq * nv), dim3(256), 0, c->s, c->tavg, (long)nq * nv, 0.0);
+  HIPCHK(hipGetLastError());
+  c->tavg_time = 0.0;
+  return LP_OK;
+}
+extern "C" int lp_tavg_reset(lp_ctx *c) {
+  if (!c) return set_err(LP_ERR_ARG, "null context");
+  HIPCHK(hipSetDevice(c->dev));
+  RCHK(tavg_alloc(c));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return LP_OK;
+}
+extern "C" int lp_tavg_compute(lp_ctx *c, double tavg_dt) {
+  if (!c) return set_err(LP_ERR_ARG, "null context");
+  HIPCHK(hipSetDevice(c->dev));
+  if (!c->tavg) RCHK(tavg_alloc(c));
+  Geo g = geo(c);
+  const int npc = c->cfg.drop.npcon;
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_tavg<T>, grid1(g.np * g.nz), dim3(256), 0, c->s, c->tavg,
+                                     (const double *)c->f[LP_U], (const double *)c->f[LP_V],
+                                     (const double *)c->f[LP_W], (const double *)c->f[LP_P],
+                                     (const T *)c->f[LP_TXX], (const T *)c->f[LP_TYY], (const T *)c->f[LP_TZZ],
+                                     (const T *)c->f[LP_TXY], (const T *)c->f[LP_TXZ], (const T *)c->f[LP_TYZ],
+                                     (const T *)c->f[LP_DUDX], (const T *)c->f[LP_DUDY], (const T *)c->f[LP_DUDZ],
+                                     (const T *)c->f[LP_DVDX], (const T *)c->f[LP_DVDY], (const T *)c->f[LP_DVDZ],
+                                     (const T *)c->f[LP_DWDX], (const T *)c->f[LP_DWDY], (const T *)c->f[LP_DWDZ],
+                                     (const double *)(npc > 0 ? c->f[LP_PCON] : nullptr),
+                                     (const double *)c->d_pvol, npc, g, tavg_dt));
+  HIPCHK(hipGetLastError());
+  c->tavg_time += tavg_dt;  // :2795 tavg_total_time
+  HIPCHK(hipStreamSynchronize(c->s));
+  return LP_OK;
+}
+extern "C" int lp_tavg_get(lp_ctx *c, int32_t q, void *out, size_t bytes, double *total_time) {
+  if (!c || !out) return set_err(LP_ERR_ARG, "lp_tavg_get: null argument");
+  if (!c->tavg) return set_err(LP_ERR_ARG, "lp_tavg_get: no averages yet (lp_tavg_reset / lp_tavg_compute)");
+  const size_t vb = sizeof(double) * (size_t)vol_el(c);
+  if (q < 0 || q >= c->tavg_nq) return set_err(LP_ERR_ARG, "lp_tavg_get: quantity %d of %d", q, c->tavg_nq);
+  if (bytes != vb) return set_err(LP_ERR_ARG, "lp_tavg_get: %zu bytes, expected %zu", bytes, vb);
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipMemcpyAsync(out, c->tavg + (size_t)q * vol_el(c), vb, hipMemcpyDefault, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  if (total_time) *total_time = c->tavg_time;
   return LP_OK;
 }
 

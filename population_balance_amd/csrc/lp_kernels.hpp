@@ -481,6 +481,76 @@ __global__ void k_rhs_ab2(T *__restrict__ Rx, T *__restrict__ Ry, T *__restrict_
   w[i] = w[i] + dt * (t1 * (double)rz + t2 * (double)fz);
 }
 
+// energy.f90:1-47: one block per level k < nz-1 sums ke = (u^2 + v^2 + w_uv^2)/2 over
+// the level in binary64 (per-thread partial sums in a fixed order, then a fixed LDS tree)
+static __global__ __launch_bounds__(256) void k_level_ke(const double *__restrict__ u, const double *__restrict__ v,
+                                                  const double *__restrict__ w, Geo g, double *__restrict__ out) {
+  __shared__ double red[256];
+  const int k = blockIdx.x;
+  const long base = (long)k * g.np;
+  double s = 0.0;
+  for (long q = threadIdx.x; q < (long)g.nx * g.ny; q += blockDim.x) {
+    const long i = base + (q / g.nx) * g.ld + q % g.nx;
+    const double tw = 0.5 * (w[i] + w[i + g.np]);
+    s += 0.5 * (u[i] * u[i] + v[i] * v[i] + tw * tw);
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[k] = red[0];
+}
+
+// io_crossflow_mod.f90:2647-2799 tavg_compute on levels 1..nz (0-based k < nz): acc[q] +=
+// tavg_dt * quantity q (the accumulator struct's order: u, v, w_uv, u_w, v_w, w, u2, v2,
+// w2, uv, uw, vw, txx, tyy, tzz, txy, txz, tyz, p - ke, dudx, dudy, dudz_uv, dvdx, dvdy,
+// dvdz_uv, dwdx_uv, dwdy_uv, dwdz, and with droplets tot_pc, tot_pc2, then per bin pcon,
+// pcon2 as volume fractions).  interp_to_uv_grid (functions.f90:53-128) repeats level
+// nz-2 at the top; interp_to_w_grid (:131-187) with u_w, v_w zero at the bottom and top
+// levels (:2688-2691).  p has levels 0..nz: its level k+1 is the uv level k.
+template <typename T>
+__global__ void k_tavg(double *__restrict__ acc, const double *__restrict__ u, const double *__restrict__ v,
+                       const double *__restrict__ w, const double *__restrict__ p, const T *__restrict__ txx,
+                       const T *__restrict__ tyy, const T *__restrict__ tzz, const T *__restrict__ txy,
+                       const T *__restrict__ txz, const T *__restrict__ tyz, const T *__restrict__ dudx,
+                       const T *__restrict__ dudy, const T *__restrict__ dudz, const T *__restrict__ dvdx,
+                       const T *__restrict__ dvdy, const T *__restrict__ dvdz, const T *__restrict__ dwdx,
+                       const T *__restrict__ dwdy, const T *__restrict__ dwdz, const double *__restrict__ PC,
+                       const double *__restrict__ pvol, int npc, Geo g, double dt) {
+  const long nv = g.np * g.nz;
+  const long i = gtid();
+  if (i >= nv) return;
+  const int k = (int)(i / g.np), x = (int)(i % g.ld);
+  if (x >= g.nx) return;
+  const long iu = (k < g.nz - 1) ? i : i - g.np;  // lower node of the uv interpolation
+  auto uvg = [&](const T *f) { return 0.5 * ((double)f[iu] + (double)f[iu + g.np]); };
+  const double up = u[i], vp = v[i], wp = w[i];
+  const double w_uv = 0.5 * (w[iu] + w[iu + g.np]);
+  const bool edge = k == 0 || k == g.nz - 1;
+  const double u_w = edge ? 0.0 : 0.5 * (u[i - g.np] + up);
+  const double v_w = edge ? 0.0 : 0.5 * (v[i - g.np] + vp);
+  const double q[28] = {up, vp, w_uv, u_w, v_w, wp, up * up, vp * vp, wp * wp, up * vp, u_w * wp, v_w * wp,
+                        (double)txx[i], (double)tyy[i], (double)tzz[i], (double)txy[i], (double)txz[i],
+                        (double)tyz[i], p[i + g.np] - 0.5 * (up * up + w_uv * w_uv + vp * vp),
+                        (double)dudx[i], (double)dudy[i], uvg(dudz), (double)dvdx[i], (double)dvdy[i], uvg(dvdz),
+                        uvg(dwdx), uvg(dwdy), (double)dwdz[i]};
+#pragma unroll
+  for (int m = 0; m < 28; ++m) acc[(long)m * nv + i] += q[m] * dt;
+  if (npc > 0) {
+    double tot = 0.0;
+    for (int b = 0; b < npc; ++b) {
+      const double pc = PC[(long)b * nv + i] * pvol[b];
+      tot = tot + pc;
+      acc[(long)(30 + 2 * b) * nv + i] += pc * dt;
+      acc[(long)(31 + 2 * b) * nv + i] += pc * pc * dt;
+    }
+    acc[28 * nv + i] += tot * dt;
+    acc[29 * nv + i] += tot * tot * dt;
+  }
+}
+
 // main.f90:789-803 (dpdz from p in binary64, RHS -= grad p, dudt -= grad p) and the
 // velocity correction of forcing.f90:145-191 in one pass
 template <typename T>

@@ -13,7 +13,8 @@ module lesgo_amd
   public :: lp_divstress_uv, lp_divstress_w, lp_tridag_array
   public :: lp_wallstress, lp_sgs_stag, lp_lagrange_sdep, lp_interpolag_sdep, lp_convec
   public :: lp_press_stag_array, lp_project, lp_timestep_conditions, lp_rmsdiv, lp_step
-  public :: lp_plane_stats, lp_spectrum
+  public :: lp_plane_stats, lp_spectrum, lp_energy
+  public :: lp_tavg_reset, lp_tavg_compute, lp_tavg_get
   public :: lp_pollen_all_in_one2, lp_calculate_probability
   public :: lp_pollen_rhs_calc2, lp_calcbeta_pcon, lp_step_pollen2
   public :: lp_set_cfl_limit, lp_check_finite, lp_get_precision
@@ -254,6 +255,31 @@ module lesgo_amd
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: field, bin
       real(c_double), intent(out) :: spec(*)
+    end function
+    ! energy (energy.f90:1-47): kea
+    integer(c_int) function lp_energy(ctx, kea) bind(C, name="lp_energy")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: kea
+    end function
+    ! tavg_init, tavg_compute (io_crossflow_mod.f90:2452-2799)
+    integer(c_int) function lp_tavg_reset(ctx) bind(C, name="lp_tavg_reset")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function lp_tavg_compute(ctx, tavg_dt) bind(C, name="lp_tavg_compute")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: tavg_dt
+    end function
+    ! accumulator q (0-based, the order of the tavg struct) into out(ld, ny, nz)
+    integer(c_int) function lp_tavg_get(ctx, q, out, nbytes, total_time) bind(C, name="lp_tavg_get")
+      import :: c_int, c_int32_t, c_ptr, c_double, c_size_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: q
+      type(c_ptr), value :: out
+      integer(c_size_t), value :: nbytes
+      real(c_double), intent(out) :: total_time
     end function
     integer(c_int) function lp_pollen_all_in_one2(ctx) bind(C, name="lp_pollen_all_in_one2")
       import :: c_int, c_ptr

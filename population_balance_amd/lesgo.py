@@ -81,6 +81,10 @@ def load():
         "lp_rmsdiv": ([vp, C.POINTER(C.c_double)], C.c_int),
         "lp_plane_stats": ([vp, C.POINTER(C.c_double)], C.c_int),
         "lp_spectrum": ([vp, i32, i32, C.POINTER(C.c_double)], C.c_int),
+        "lp_energy": ([vp, C.POINTER(C.c_double)], C.c_int),
+        "lp_tavg_reset": ([vp], C.c_int),
+        "lp_tavg_compute": ([vp, C.c_double], C.c_int),
+        "lp_tavg_get": ([vp, i32, vp, C.c_size_t, C.POINTER(C.c_double)], C.c_int),
         "lp_step": ([vp, i32], C.c_int),
         "lp_set_cfl_limit": ([vp, C.c_double], C.c_int),
         "lp_check_finite": ([vp], C.c_int),
@@ -104,6 +108,12 @@ def load():
         raise LesgoError("ABI version mismatch between liblesgo_hip.so and abi.py")
     _lib = L
     return L
+
+
+# the accumulators of lp_tavg_get, in order (then "pcon<b>", "pcon2<b>" per droplet bin)
+TAVG_NAMES = ("u", "v", "w_uv", "u_w", "v_w", "w", "u2", "v2", "w2", "uv", "uw", "vw",
+              "txx", "tyy", "tzz", "txy", "txz", "tyz", "p", "dudx", "dudy", "dudz_uv",
+              "dvdx", "dvdy", "dvdz_uv", "dwdx_uv", "dwdy_uv", "dwdz", "tot_pc", "tot_pc2")
 
 
 def _torch_ready(t):
@@ -355,6 +365,30 @@ class Lesgo:
         rc = self.L.lp_spectrum(self.h, abi.FIELD[name], bin, out.ctypes.data_as(C.POINTER(C.c_double)))
         self._check(rc, "spectrum")
         return out
+
+    def energy(self):
+        """kea of energy.f90: the mean of (u^2 + v^2 + w_uv^2)/2 over levels 1..nz-1."""
+        a = C.c_double()
+        self._check(self.L.lp_energy(self.h, C.byref(a)), "energy")
+        return a.value
+
+    def tavg_reset(self):
+        """tavg_init: zero the running time averages and their total time."""
+        self._check(self.L.lp_tavg_reset(self.h), "tavg_reset")
+
+    def tavg_compute(self, tavg_dt):
+        """tavg_compute: add tavg_dt times each averaged quantity of the current state."""
+        self._check(self.L.lp_tavg_compute(self.h, float(tavg_dt)), "tavg_compute")
+
+    def tavg_get(self, q):
+        """(accumulator q as a binary64 (nz, ny, ld) array, tavg_total_time); the
+        quantities' order is that of lp_tavg_get (TAVG_NAMES)."""
+        q = TAVG_NAMES.index(q) if isinstance(q, str) else int(q)
+        out = np.zeros((self.g.nz, self.g.ny, self.g.ld))
+        t = C.c_double()
+        rc = self.L.lp_tavg_get(self.h, q, out.ctypes.data, out.nbytes, C.byref(t))
+        self._check(rc, "tavg_get")
+        return out, t.value
 
     def pollen_all_in_one2(self):
         self._check(self.L.lp_pollen_all_in_one2(self.h), "pollen_all_in_one2")
