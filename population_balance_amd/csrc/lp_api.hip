@@ -1420,10 +1420,14 @@ static int do_beta_step(lp_ctx *c, int do_beta, int do_step) {
 static int do_pollen(lp_ctx *c) {
   const lp_droplet_config &D = c->cfg.drop;
   // :1521 -- RHS_PCon_f takes the old RHS by exchanging the buffers (k_pollen_rhs
-  // rewrites every point of the new RHS_PCon)
+  // rewrites every point of the new RHS_PCon); exchanged back if the RHS fails
   std::swap(c->f[LP_RHS_PCON], c->f[LP_RHS_PCON_F]);
   bool stepped = false;
-  RCHK(do_pollen_rhs(c, (const double *)c->f[LP_RHS_PCON_F], true, &stepped));
+  const int rc = do_pollen_rhs(c, (const double *)c->f[LP_RHS_PCON_F], true, &stepped);
+  if (rc != LP_OK) {
+    std::swap(c->f[LP_RHS_PCON], c->f[LP_RHS_PCON_F]);
+    return rc;
+  }
   if (stepped) return LP_OK;  // the breakup pass did :1660-1665 and the step
   if ((c->jt_total == D.pcon_init || c->jt_total == D.src_jt0) && !D.init_pcon)  // :1660-1665
     HIPCHK(hipMemcpyAsync(c->f[LP_RHS_PCON_F], c->f[LP_RHS_PCON], c->bytes[LP_RHS_PCON], hipMemcpyDeviceToDevice,
