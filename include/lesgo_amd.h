@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define LP_ABI_VERSION 1
+#define LP_ABI_VERSION 2
 
 enum lp_status {
   LP_OK = 0,
@@ -103,6 +103,30 @@ typedef struct lp_droplet_config {
   const double *Q_src;           /* [npcon] source strength (nondimensional)           */
 } lp_droplet_config;
 
+/* Potential temperature theta (S_FLAG): param.f90:244-283, 537, 616 and bottombc.f90:7
+ * re-expressed at run time, for the reference's surface condition: a prescribed,
+ * homogeneous surface flux (lbc = 1, patch_flag = 1, num_patch = 1).  lbc = 0 (surface
+ * temperature through a log law) reads the scalar roughness z_os, which the reference's
+ * sources never assign; lp_create refuses it.  The Stokes drift of the advection
+ * (U_stokes) and the sponge of ubc = 1 are not modelled; the ocean's stress-free bottom
+ * is lbc_mom = LP_LBC_STRESS_FREE. */
+typedef struct lp_scalar_config {
+  int32_t s_flag;                /* S_FLAG: theta on                                   */
+  int32_t passive;               /* passive_scalar: no buoyancy beta_scal in RHSz       */
+  int32_t scal_init;             /* SCAL_init: theta steps from jt >= scal_init        */
+  int32_t initsc;                /* initsc: theta, RHS_T, sgs_t3(:,:,1), psi_m come with
+                                    the state (no Euler start, psi_m kept)              */
+  int32_t lbc;                   /* 1: prescribed surface flux wt_s                     */
+  int32_t ocean;                 /* OCEAN_FLAG: calcbeta's ocean forms                  */
+  int32_t iboussinesq;           /* 1, 2 or 3 with ocean (param.f90:616)                */
+  int32_t pad_;
+  double Pr;                     /* SGS Prandtl number (0.4)                            */
+  double wt_s;                   /* surface flux, K m/s                                 */
+  double T_scale;                /* 300 K                                               */
+  double dTdz_top;               /* top gradient, K/m                                   */
+  double g, u_star, z_i;         /* 9.81, 1, 1                                          */
+} lp_scalar_config;
+
 /* Run-time configuration: replaces param.f90 / sgsmodule.f90 parameters. */
 typedef struct lp_config {
   lp_grid grid;
@@ -124,6 +148,7 @@ typedef struct lp_config {
   double opftime;                /* 1.5 (sgsmodule.f90:7)                              */
   double filter_size;            /* 1 (test_filtermodule.f90:8)                        */
   lp_droplet_config drop;
+  lp_scalar_config scal;
 } lp_config;
 
 /* State field ids.  Shapes are (ld,ny,nz) unless noted.  In an LP_MIXED context the
@@ -149,6 +174,13 @@ enum lp_field {
   LP_RHS_PCON, LP_RHS_PCON_F,    /* (ld,ny,nz,npcon) [f64]                             */
   LP_RHS_PBE,                    /* (ld,ny,nz,npcon) [f64]                             */
   LP_BETA_PCON,                  /* (ld,ny,nz)                                         */
+  LP_THETA,                      /* potential temperature / T_scale [f64]              */
+  LP_RHS_T, LP_RHS_TF,           /* its RHS and the previous step's                    */
+  LP_DTDX, LP_DTDY, LP_DTDZ,     /* dTdz: w nodes, level 1 from the surface flux        */
+  LP_SGS_T3,                     /* vertical SGS flux (sgs_vert); level 1 = the surface
+                                    flux [f64]                                          */
+  LP_BETA_SCAL,                  /* theta buoyancy (calcbeta), added to RHSz            */
+  LP_PSI_M,                      /* (ld,ny) stability correction of the wall law [f64] */
   LP_NFIELDS
 };
 
@@ -243,6 +275,21 @@ int lp_project(lp_ctx *ctx);                    /* forcing.f90:122-294          
 int lp_timestep_conditions(lp_ctx *ctx, double *cfl, double *visc_stab);
                                                 /* scalars_module2.f90:1336-1379   */
 int lp_rmsdiv(lp_ctx *ctx, double *rmsdiv);     /* rmsdiv.f90:1-44                 */
+/* Potential temperature (cfg.scal.s_flag; LP_ERR_ARG without it).
+ * lp_obukhov: scalars_module_crossflow_mod.f90:1170-1403 obukhov for a homogeneous
+ * surface: from u, v, theta and sgs_t3 on level 1 and psi_m, the plane-mean u*, the
+ * Monin-Obukhov length L and the stability functions; psi_m (LP_PSI_M, read by the wall
+ * law) and the u* plane (LP_USTAR) are rewritten with them.  lp_step calls it at the
+ * start of each step (main.f90:257) when theta is on.  lp_obukhov_get returns the last
+ * call's values: out[0..7] = psi_m, psi_h, phi_m, phi_h, u*, L, the mean surface flux,
+ * the mean theta of level 1 (L = 0 and the neutral functions before scal_init).
+ * lp_theta_all_in_one: :141-360 theta_all_in_one (filt_da and ddz_uv of theta,
+ * scalar_RHS_calc :474-1110 with the 3/2-rule advection and the SGS flux divergence,
+ * calcbeta :361-473, step_scalar :1111-1169); lp_step calls it after sgs_stag
+ * (main.f90:340-344) once jt >= scal_init and adds beta_scal to RHSz (:450-453). */
+int lp_obukhov(lp_ctx *ctx);
+int lp_obukhov_get(lp_ctx *ctx, double *out);
+int lp_theta_all_in_one(lp_ctx *ctx);
 /* io.f90:1782-1849 (avg_stats, serial case, without theta): the instantaneous plane
  * statistics that avg_stats time-averages and writes.  out (host, binary64) holds
  * (nz-1) rows of 10: <u>, <v>, <w>, <Cs_opt2>, <Nu_t>, <u'^2>, <v'^2>, <w'^2>, <u'w'>,

@@ -64,13 +64,15 @@ def lib():
         L.or_tridag_array.argtypes = [vp, vp, vp, vp, vp, vp]
         for n in ("or_wallstress", "or_sgs_stag", "or_lagrange_sdep", "or_interpolag_sdep",
                   "or_convec", "or_press_stag_array", "or_project", "or_pollen_all_in_one2",
-                  "or_pollen_rhs_calc2", "or_calcbeta_pcon", "or_step_pollen2"):
+                  "or_pollen_rhs_calc2", "or_calcbeta_pcon", "or_step_pollen2", "or_obukhov",
+                  "or_theta_all_in_one"):
             getattr(L, n).argtypes = [vp]
             getattr(L, n).restype = C.c_int
         L.or_tridag_array.restype = C.c_int
         L.or_timestep_conditions.argtypes = [vp, dp, dp]
         L.or_rmsdiv.argtypes = [vp, dp]
         L.or_plane_stats.argtypes = [vp, dp]
+        L.or_obukhov_get.argtypes = [vp, dp]
         L.or_energy.argtypes = [vp, dp]
         L.or_energy.restype = C.c_double
         L.or_tavg_compute.argtypes = [vp, C.c_double, dp]
@@ -278,6 +280,12 @@ class Oracle:
     def call(self, name):
         rc = getattr(self.L, "or_" + name)(self.h)
         self._check(rc, name)
+
+    def obukhov_get(self):
+        """the last obukhov call: psi_m, psi_h, phi_m, phi_h, u*, L, <sgs_t3(1)>, <theta(1)>"""
+        out = np.zeros(8)
+        self.L.or_obukhov_get(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
 
     def timestep_conditions(self):
         a, b = C.c_double(), C.c_double()

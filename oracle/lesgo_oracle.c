@@ -139,6 +139,9 @@ struct or_ctx {
   /* frozen-coefficient steps (or_set_frozen): Nu_t (volume) and the wall u* (plane) of a
    * base run, used instead of the SGS model and the wall law's |U| */
   const double *frz_nut, *frz_ust;
+  /* the last obukhov call: psi_m, psi_h, phi_m, phi_h, u*, L, <sgs_t3(1)>, <theta(1)>
+   * (the wall law reads phi_m; the theta RHS phi_h) */
+  double obuk[8];
 };
 
 static size_t plane(const or_ctx *c) { return (size_t)c->g.ld * c->g.ny; }
@@ -167,7 +170,7 @@ or_ctx *or_create(const lp_config *cfg) {
   for (int i = 0; i < LP_NFIELDS; ++i) {
     size_t n = vol(c);
     if (i == LP_P) n = plane(c) * (g->nz + 1);
-    if (i == LP_USTAR) n = plane(c);
+    if (i == LP_USTAR || i == LP_PSI_M) n = plane(c);
     if (i == LP_PCON || i == LP_RHS_PCON || i == LP_RHS_PCON_F || i == LP_RHS_PBE)
       n = vol(c) * npc;
     c->n[i] = n;
@@ -175,6 +178,7 @@ or_ctx *or_create(const lp_config *cfg) {
   }
   c->jt = 1;
   c->jt_total = 1;
+  c->obuk[2] = c->obuk[3] = 1.0; /* neutral phi_m, phi_h */
   int np_ = cfg->drop.npcon;
   if (np_ > 0) {
     c->diam = (double *)malloc(sizeof(double) * np_);
@@ -562,8 +566,8 @@ int or_convec(or_ctx *c) {
 }
 
 /* ------------------------------------------------------------------------- */
-/* wallstress.f90:5-215 (OCEAN_FLAG off, WALL_HOMOG_FLAG off, neutral:        */
-/* psi_m = 0, phi_m = 1 from obukhov, scalars_module.f90:1152-1163)           */
+/* wallstress.f90:5-215 (OCEAN_FLAG off, WALL_HOMOG_FLAG off; psi_m (plane)  */
+/* and phi_m from obukhov, zero and one with theta off)                       */
 /* ------------------------------------------------------------------------- */
 int or_wallstress(or_ctx *c) {
   const lp_grid *g = &c->g;
@@ -582,9 +586,11 @@ int or_wallstress(or_ctx *c) {
   memcpy(v1, v, sizeof(double) * np);
   test_filter_plane(c, u1, c->G_test);                                      /* :139-140 */
   test_filter_plane(c, v1, c->G_test);
-  const double psi_m = 0.0, phi_m = 1.0;
+  /* psi_m and phi_m from obukhov: zero and one with theta off */
+  const double *psi = c->f[LP_PSI_M];
+  const double phi_m = cf->scal.s_flag ? c->obuk[2] : 1.0;
   double zl = 0.5 * g->dz - cf->d0;
-  double denom = log(zl / cf->z0) - psi_m;                                  /* :143 */
+  double denom = log(zl / cf->z0);                                          /* :143 */
   if (c->frz_ust) {
     /* frozen coefficient: txz = -(ustar^2 / |U|) u1 with ustar^2 / |U| = ustar kappa / denom
      * of the base run; dudz = ustar / (zl kappa) u / |U| = u / (zl denom) */
@@ -605,7 +611,7 @@ int or_wallstress(or_ctx *c) {
     for (int x = 0; x < g->nx; ++x) {
       size_t i = (size_t)y * g->ld + x;
       double u_avg = sqrt(u1[i] * u1[i] + v1[i] * v1[i]);                   /* :145 */
-      double us = u_avg * cf->vonk / denom;                                 /* :162 */
+      double us = u_avg * cf->vonk / (denom - psi[i]);                      /* :143, 162 */
       double cst = -(us * us) / u_avg;                                      /* :167 */
       txz[i] = cst * u1[i];
       tyz[i] = cst * v1[i];
@@ -1897,6 +1903,257 @@ int or_pollen_all_in_one2(or_ctx *c) {
 }
 
 /* ------------------------------------------------------------------------- */
+/* Potential temperature (S_FLAG): scalars_module_crossflow_mod.f90 for a     */
+/* homogeneous surface with a prescribed flux (patch_flag = 1, num_patch = 1, */
+/* lbc = 1, spatial_flux_flag off, ubc = 0, U_stokes = ucross = 0).           */
+/* ------------------------------------------------------------------------- */
+/* :1170-1403 obukhov.  Homogeneous surface: the plane means of |U(1)|,        */
+/* sgs_t3(1) and theta(1) give one u*, one Obukhov length and one set of       */
+/* stability functions; before scal_init (or with theta off) the neutral       */
+/* functions (:1183-1193).  zo is the uniform z0.                             */
+int or_obukhov(or_ctx *c) {
+  const lp_config *cf = &c->cfg;
+  const lp_scalar_config *S = &cf->scal;
+  const lp_grid *g = &c->g;
+  double **f = c->f;
+  const size_t np = plane(c);
+  const double nxy = (double)g->nx * g->ny;
+  const double *u = f[LP_U], *v = f[LP_V], *th = f[LP_THETA];
+  double *psi = f[LP_PSI_M], *ust = f[LP_USTAR], *t3 = f[LP_SGS_T3];
+  double *ob = c->obuk;
+  double su = 0.0, spsi;
+  for (int y = 0; y < g->ny; ++y)
+    for (int x = 0; x < g->nx; ++x) {
+      const size_t i = (size_t)y * g->ld + x;
+      su += sqrt(u[i] * u[i] + v[i] * v[i]);
+    }
+#define PSUM(a, out)                                                        \
+  do {                                                                      \
+    out = 0.0;                                                              \
+    for (int y = 0; y < g->ny; ++y)                                         \
+      for (int x = 0; x < g->nx; ++x) out += (a)[(size_t)y * g->ld + x];    \
+  } while (0)
+  if (!S->s_flag || c->jt < S->scal_init) {                                 /* :1183-1193 */
+    if (!S->initsc) memset(psi, 0, np * sizeof(double));
+    PSUM(psi, spsi);
+    const double us = su / nxy * cf->vonk / (log(0.5 * g->dz / cf->z0) - spsi / nxy);
+    for (int y = 0; y < g->ny; ++y)
+      for (int x = 0; x < g->nx; ++x) ust[(size_t)y * g->ld + x] = us;
+    ob[0] = spsi / nxy; ob[1] = 0.0; ob[2] = 1.0; ob[3] = 1.0;
+    ob[4] = us; ob[5] = 0.0; ob[6] = 0.0; ob[7] = 0.0;
+    return 0;
+  }
+  if (c->jt == S->scal_init && !S->initsc)                                  /* :1195-1201 */
+    for (size_t i = 0; i < np; ++i) t3[i] = S->wt_s / S->u_star / S->T_scale;
+  const double g_ = S->g / (S->u_star * S->u_star / S->z_i);                /* :1202 */
+  double slz = 0.0;                                                         /* :1205 */
+  for (int q = 0; q < g->nx * g->ny; ++q) slz += log(cf->z0);
+  const double zo_ = exp(slz / nxy);
+  double wt_, th_;
+  PSUM(t3, wt_);
+  wt_ /= nxy;                                                               /* :1206 */
+  PSUM(psi, spsi);
+  const double us = su / nxy * cf->vonk / (log(0.5 * g->dz / zo_) - spsi / nxy); /* :1207-1208 */
+  PSUM(th, th_);
+  th_ /= nxy;                                                               /* :1209 */
+#undef PSUM
+  /* :1210-1218 (homogeneous): wt_avg, ustar_avg, theta_avg are the means; :1232-1272 */
+  const double L = -(us * us * us) / (cf->vonk * g_ / th_ * wt_);
+  double pm = 0.0, ph = 0.0, fm = 1.0, fh = 1.0;
+  if (L < 0.0 && wt_ != 0.0) {
+    const double x = pow(1.0 - 16.0 * g->dz / 2.0 / L, 0.25);
+    pm = 2.0 * log((1.0 + x) / 2.0) + log((1.0 + x * x) / 2.0) - 2.0 * atan(x) + PI_D / 2.0;
+    ph = 2.0 * log((1.0 + x * x) / 2.0);
+    fm = 1.0 / x;
+    fh = 1.0 / (x * x);
+  } else if (L > 0.0 && wt_ != 0.0) {                 /* :1255-1258 (jan_diurnal_run off) */
+    const double zeta = 0.5 * g->dz / L;
+    fm = 1.0 + 4.8 * zeta;
+    fh = 1.0 + 7.8 * zeta;
+    pm = -4.8 * zeta;
+    ph = -7.8 * zeta;
+  }
+  for (int y = 0; y < g->ny; ++y)
+    for (int x = 0; x < g->nx; ++x) {
+      const size_t i = (size_t)y * g->ld + x;
+      psi[i] = pm;
+      ust[i] = us;
+    }
+  ob[0] = pm; ob[1] = ph; ob[2] = fm; ob[3] = fh; ob[4] = us; ob[5] = L; ob[6] = wt_; ob[7] = th_;
+  return 0;
+}
+
+void or_obukhov_get(or_ctx *c, double *out) { memcpy(out, c->obuk, sizeof c->obuk); }
+
+/* :361-473 calcbeta (serial: levels 2..nz-1 get a value, level 1 and nz stay zero) */
+static void or_calcbeta(or_ctx *c) {
+  const lp_scalar_config *S = &c->cfg.scal;
+  const lp_grid *g = &c->g;
+  const size_t np = plane(c);
+  const int nz = g->nz;
+  const double *th = c->f[LP_THETA];
+  double *beta = c->f[LP_BETA_SCAL];
+  double *bar = (double *)malloc(sizeof(double) * nz);
+  const double g_hat = S->g * (S->z_i / (S->u_star * S->u_star));
+  memset(beta, 0, vol(c) * sizeof(double));
+  for (int k = 0; k < nz; ++k) {                                            /* :375-383 */
+    double s = 0.0;
+    for (int y = 0; y < g->ny; ++y)
+      for (int x = 0; x < g->nx; ++x) s += th[(size_t)k * np + (size_t)y * g->ld + x];
+    bar[k] = s / (double)(g->nx * g->ny);
+  }
+  const double at = 2.e-4 * S->T_scale;
+  for (int k = 1; k < nz - 1; ++k)
+    for (int y = 0; y < g->ny; ++y)
+      for (int x = 0; x < g->nx; ++x) {
+        const size_t i = (size_t)k * np + (size_t)y * g->ld + x;
+        const double a = th[i], b = th[i - np];
+        double above, below, r;
+        if (S->ocean && S->iboussinesq == 1) {                              /* :393-401 */
+          above = 1.0 - at * a;
+          below = 1.0 - at * b;
+          r = -g_hat * (above + below) / 2.0;
+        } else if (S->ocean && S->iboussinesq == 2) {                       /* :402-412 */
+          above = at * (a - bar[k]) / (1.0 - at * bar[k]);
+          below = at * (b - bar[k - 1]) / (1.0 - at * bar[k - 1]);
+          r = g_hat * (above + below) / 2.0;
+        } else if (S->ocean) {                                              /* :413-421 */
+          above = at * a;
+          below = at * b;
+          r = g_hat * (above + below) / 2.0;
+        } else {                                                            /* :427-435 */
+          above = (a - bar[k]) / bar[k];
+          below = (b - bar[k - 1]) / bar[k - 1];
+          r = g_hat * (above + below) / 2.0;
+        }
+        beta[i] = r;
+      }
+  free(bar);
+}
+
+/* :141-360 theta_all_in_one with scalar_RHS_calc (:474-1110), calcbeta and
+ * step_scalar (:1111-1169) */
+int or_theta_all_in_one(or_ctx *c) {
+  const lp_config *cf = &c->cfg;
+  const lp_scalar_config *S = &cf->scal;
+  const lp_grid *g = &c->g;
+  double **f = c->f;
+  const size_t np = plane(c), nv = vol(c);
+  const int nz = g->nz, nx = g->nx, ny = g->ny, ld = g->ld;
+  double *th = f[LP_THETA], *dTdx = f[LP_DTDX], *dTdy = f[LP_DTDY], *dTdz = f[LP_DTDZ];
+  double *RHS = f[LP_RHS_T], *nut = f[LP_NU_T], *t3 = f[LP_SGS_T3];
+  or_filt_da(c, th, dTdx, dTdy, nz);                                        /* :160 */
+  or_ddz_uv(c, dTdz, th);                                                   /* :175 */
+  for (size_t i = 0; i < np; ++i)                                           /* :176-178 */
+    dTdz[(size_t)(nz - 1) * np + i] = S->dTdz_top / S->T_scale * S->z_i;
+  memcpy(f[LP_RHS_TF], RHS, nv * sizeof(double));                           /* :199 */
+  /* scalar_RHS_calc :530-575 (patch_flag = 1, one patch type): u* is the plane mean of
+   * the wall law's ustar (a sum of ustar / patchnum); the flux wt_s passes through the default
+   * REAL of surf_flux_current (:145, 489) */
+  const double nxy = (double)(nx * ny);
+  double ustar1 = 0.0;
+  for (int y = 0; y < ny; ++y)
+    for (int x = 0; x < nx; ++x) ustar1 += f[LP_USTAR][(size_t)y * ld + x] / nxy;
+  const double wts = cf->fp32_casts ? (double)(float)S->wt_s : S->wt_s;
+  const double sflux = wts / S->T_scale / S->u_star;
+  const double phi_h = c->obuk[3];
+  for (int y = 0; y < ny; ++y)
+    for (int x = 0; x < nx; ++x)
+      dTdz[(size_t)y * ld + x] = -phi_h * sflux / (ustar1 * cf->vonk * g->dz / 2.0);
+  /* :808-813 dealias1 of u, v, w and the three derivatives; :827-874 products */
+  const size_t npb = (size_t)g->ld_big * g->ny2, nvb = npb * nz;
+  double *B[6];
+  const double *src[6] = {f[LP_U], f[LP_V], f[LP_W], dTdx, dTdy, dTdz};
+  for (int q = 0; q < 6; ++q) {
+    B[q] = (double *)calloc(nvb, sizeof(double));
+    or_dealias1(c, src[q], B[q], nz);
+  }
+  double *Rm = (double *)calloc(nvb, sizeof(double));
+  for (int k = 0; k < nz; ++k)
+    for (int y = 0; y < g->ny2; ++y)
+      for (int x = 0; x < g->nx2; ++x) {
+        const size_t i = (size_t)k * npb + (size_t)y * g->ld_big + x;
+        double r = B[0][i] * B[3][i] + B[1][i] * B[4][i];
+        if (k == 0)
+          r = r + (0.5 * B[2][i + npb]) * B[5][i + npb];                    /* :852-853 */
+        else if (k < nz - 1)
+          r = r + (B[2][i] * B[5][i] + B[2][i + npb] * B[5][i + npb]) / 2.0; /* :837-838 */
+        Rm[i] = r;                                                          /* :866 (top) */
+      }
+  or_dealias2(c, RHS, Rm, nz);                                              /* :879 */
+  for (int q = 0; q < 6; ++q) free(B[q]);
+  free(Rm);
+  /* :889-1060 the SGS flux divergence; temp(nz) of the x and y fluxes is never set by the
+   * reference (a local array), here zero: it reaches only RHS_T(nz), which no step reads */
+  const double inv = 1.0 / S->Pr;
+  double *tmp = (double *)calloc(nv, sizeof(double)), *dt_ = (double *)calloc(nv, sizeof(double));
+  for (int dir = 0; dir < 2; ++dir) {
+    const double *ds = dir == 0 ? dTdx : dTdy;
+    memset(tmp, 0, nv * sizeof(double));
+    for (int k = 0; k < nz - 1; ++k)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          const size_t i = (size_t)k * np + (size_t)y * ld + x;
+          tmp[i] = k == 0 ? inv * nut[i] * ds[i] : inv * 0.5 * (nut[i] + nut[i + np]) * ds[i];
+        }
+    if (dir == 0) {
+      or_ddx(c, dt_, tmp, nz);                                              /* :925 */
+      for (int k = 0; k < nz - 1; ++k)
+        for (int y = 0; y < ny; ++y)
+          for (int x = 0; x < nx; ++x) {
+            const size_t i = (size_t)k * np + (size_t)y * ld + x;
+            RHS[i] = -1.0 * RHS[i] + dt_[i];
+          }
+    } else {
+      or_ddy(c, dt_, tmp, nz);                                              /* :977 */
+      for (int k = 0; k < nz; ++k)
+        for (int y = 0; y < ny; ++y)
+          for (int x = 0; x < nx; ++x) {
+            const size_t i = (size_t)k * np + (size_t)y * ld + x;
+            RHS[i] = RHS[i] + dt_[i];
+          }
+    }
+  }
+  memset(tmp, 0, nv * sizeof(double));
+  for (int k = 0; k < nz; ++k)                                              /* :990-1020 */
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        const size_t i = (size_t)k * np + (size_t)y * ld + x;
+        if (k == 0) {
+          tmp[i] = -1.0 * sflux;
+          t3[i] = sflux;
+        } else {
+          tmp[i] = inv * nut[i] * dTdz[i];
+          t3[i] = -1.0 * tmp[i];
+        }
+      }
+  or_ddz_w(c, dt_, tmp);                                                    /* :1038 */
+  for (int k = 0; k < nz - 1; ++k)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        const size_t i = (size_t)k * np + (size_t)y * ld + x;
+        RHS[i] = RHS[i] + dt_[i];
+      }
+  free(tmp);
+  free(dt_);
+  or_calcbeta(c);                                                           /* :246 */
+  if (c->jt == S->scal_init && !S->initsc) memcpy(f[LP_RHS_TF], RHS, nv * sizeof(double)); /* :272-274 */
+  const double *Rf = f[LP_RHS_TF];                                          /* :1111-1169 */
+  for (int k = 0; k < nz - 1; ++k)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        const size_t i = (size_t)k * np + (size_t)y * ld + x;
+        th[i] = th[i] + cf->dt * (1.5 * RHS[i] - 0.5 * Rf[i]);
+      }
+  const double top = S->dTdz_top / S->T_scale * S->z_i * g->dz;
+  for (size_t i = 0; i < np; ++i) {                                         /* :1155-1166 */
+    const size_t it = (size_t)(nz - 1) * np + i;
+    th[it] = g->L_z > 1.0 ? th[it - np] + top : th[it - np];
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
 /* One time step: main.f90:207-826 (+:864-898 when droplets need dudt)       */
 /* ------------------------------------------------------------------------- */
 static int step_once(or_ctx *c) {
@@ -1909,6 +2166,8 @@ static int step_once(or_ctx *c) {
   memcpy(f[LP_RHSX_F], f[LP_RHSX], nv * sizeof(double));                   /* :252-254 */
   memcpy(f[LP_RHSY_F], f[LP_RHSY], nv * sizeof(double));
   memcpy(f[LP_RHSZ_F], f[LP_RHSZ], nv * sizeof(double));
+  const int theta_on = cf->scal.s_flag && c->jt >= cf->scal.scal_init;
+  if (cf->scal.s_flag) or_obukhov(c);                                       /* :257 */
   or_filt_da(c, f[LP_U], f[LP_DUDX], f[LP_DUDY], nz);                      /* :277-279 */
   or_filt_da(c, f[LP_V], f[LP_DVDX], f[LP_DVDY], nz);
   or_filt_da(c, f[LP_W], f[LP_DWDX], f[LP_DWDY], nz);
@@ -1917,6 +2176,11 @@ static int step_once(or_ctx *c) {
   or_ddz_w(c, f[LP_DWDZ], f[LP_W]);
   if ((rc = or_wallstress(c))) return rc;                                   /* :315 */
   if ((rc = or_sgs_stag(c))) return rc;                                     /* :327 */
+  if (theta_on) {                                                           /* :340-344 */
+    if ((rc = or_theta_all_in_one(c))) return rc;
+  } else if (cf->scal.s_flag) {
+    memset(f[LP_BETA_SCAL], 0, nv * sizeof(double));
+  }
   if (cf->drop.npcon > 0 && c->jt_total >= cf->drop.pcon_init) {                 /* :360-372 */
     if ((rc = or_pollen_all_in_one2(c))) return rc;
   } else {
@@ -1933,6 +2197,7 @@ static int step_once(or_ctx *c) {
     f[LP_DUDT][i] = -f[LP_DIVTX][i];
     f[LP_DVDT][i] = -f[LP_DIVTY][i];
     f[LP_DWDT][i] = -f[LP_DIVTZ][i];
+    if (cf->scal.s_flag && !cf->scal.passive) f[LP_RHSZ][i] += f[LP_BETA_SCAL][i];   /* :450-453 */
     if (cf->drop.npcon > 0 && cf->drop.active) f[LP_RHSZ][i] += f[LP_BETA_PCON][i];  /* :458-460 */
   }
   double force = cf->use_mean_p_force ? cf->mean_p_force : 0.0;             /* :571-575 */

@@ -58,6 +58,9 @@ int or_sgs_stag(or_ctx *c);
 int or_lagrange_sdep(or_ctx *c);
 int or_interpolag_sdep(or_ctx *c);
 int or_convec(or_ctx *c);
+int or_obukhov(or_ctx *c);
+void or_obukhov_get(or_ctx *c, double *out);
+int or_theta_all_in_one(or_ctx *c);
 int or_press_stag_array(or_ctx *c);
 int or_project(or_ctx *c);
 void or_timestep_conditions(or_ctx *c, double *cfl, double *visc_stab);

@@ -6,7 +6,7 @@ sizes against the compiled library.
 import ctypes as C
 import math
 
-LP_ABI_VERSION = 1
+LP_ABI_VERSION = 2
 
 # status codes
 LP_OK, LP_ERR_ARG, LP_ERR_HIP, LP_ERR_PIVOT, LP_ERR_NAN, LP_ERR_NOMEM, LP_ERR_NODEVICE, LP_ERR_CFL = range(8)
@@ -29,12 +29,14 @@ FIELD_NAMES = [
     "ustar",
     "PCon", "RHS_PCon", "RHS_PCon_f", "Rhs_pbe",
     "beta_pcon",
+    "theta", "RHS_T", "RHS_Tf", "dTdx", "dTdy", "dTdz", "sgs_t3", "beta_scal",
+    "psi_m",
 ]
 FIELD = {n: i for i, n in enumerate(FIELD_NAMES)}
 LP_NFIELDS = len(FIELD_NAMES)
 # fields that stay binary64 in an LP_MIXED context
 FIELDS_F64_IN_MIXED = {"u", "v", "w", "p", "F_LM", "F_MM", "F_QN", "F_NN", "PCon", "RHS_PCon",
-                       "RHS_PCon_f", "Rhs_pbe"}
+                       "RHS_PCon_f", "Rhs_pbe", "theta", "sgs_t3", "psi_m"}
 BOGUS = -1234567890.0
 KERNEL_CLASSES = {"row_r2c": 0, "col_op": 1, "row_c2r": 2, "lag_relax": 3, "press_tridiag": 4,
                   "pollen_rhs": 5, "pbe": 6, "convec_rows": 7, "col_pad": 8,
@@ -73,6 +75,16 @@ class lp_droplet_config(C.Structure):
     ]
 
 
+class lp_scalar_config(C.Structure):
+    _fields_ = [
+        ("s_flag", C.c_int32), ("passive", C.c_int32), ("scal_init", C.c_int32),
+        ("initsc", C.c_int32), ("lbc", C.c_int32), ("ocean", C.c_int32),
+        ("iboussinesq", C.c_int32), ("pad_", C.c_int32),
+        ("Pr", C.c_double), ("wt_s", C.c_double), ("T_scale", C.c_double),
+        ("dTdz_top", C.c_double), ("g", C.c_double), ("u_star", C.c_double), ("z_i", C.c_double),
+    ]
+
+
 class lp_config(C.Structure):
     _fields_ = [
         ("grid", lp_grid),
@@ -87,6 +99,7 @@ class lp_config(C.Structure):
         ("mean_p_force", C.c_double),
         ("cs2_init", C.c_double), ("opftime", C.c_double), ("filter_size", C.c_double),
         ("drop", lp_droplet_config),
+        ("scal", lp_scalar_config),
     ]
 
 
@@ -123,11 +136,22 @@ def make_config(nx, ny, nz, L_x=2 * math.pi, L_y=2 * math.pi, L_z=1.0, dt=2e-4,
     c.use_mean_p_force, c.mean_p_force = 0, 1.0
     c.cs2_init, c.opftime, c.filter_size = 0.03, 1.5, 1.0
     c.drop.npcon = 0
+    set_scalar_defaults(c.scal)
     for k, v in kw.items():
         if not hasattr(c, k):
             raise AttributeError(f"lp_config has no field {k!r}")
         setattr(c, k, v)
     return c
+
+
+def set_scalar_defaults(sc, s_flag=0):
+    """param.f90:96, 244-283 (theta off unless s_flag; Pr 0.4, SCAL_init 1, lbc 1 with
+    wt_s = 0, T_scale 300 K, dTdz_top 0, g 9.81, u_star = z_i = 1, atmospheric calcbeta)"""
+    sc.s_flag, sc.passive, sc.scal_init, sc.initsc = s_flag, 0, 1, 0
+    sc.lbc, sc.ocean, sc.iboussinesq = 1, 0, 3
+    sc.Pr, sc.wt_s, sc.T_scale, sc.dTdz_top = 0.4, 0.0, 300.0, 0.0
+    sc.g, sc.u_star, sc.z_i = 9.81, 1.0, 1.0
+    return sc
 
 
 def copy_config(cfg, **kw):
@@ -143,7 +167,7 @@ def field_shape(g, name, npcon=0):
     """numpy shape (C order) of a state field: (levels, ny, ld) etc."""
     if name == "p":
         return (g.nz + 1, g.ny, g.ld)
-    if name == "ustar":
+    if name in ("ustar", "psi_m"):
         return (1, g.ny, g.ld)
     if name in ("PCon", "RHS_PCon", "RHS_PCon_f", "Rhs_pbe"):
         return (max(npcon, 1), g.nz, g.ny, g.ld)

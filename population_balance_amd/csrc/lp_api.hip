@@ -19,6 +19,7 @@
 #include "lp_lag.hpp"
 #include "lp_convec.hpp"
 #include "lp_adjoint.hpp"
+#include "lp_scalar.hpp"
 
 using namespace lp;
 
@@ -139,6 +140,12 @@ struct lp_ctx {
   int tavg_nq = 0;
   double tavg_time = 0.0;      // tavg_total_time
   double *d_pvol = nullptr;    // per bin: pi/6 d^3 (tavg's volume fractions)
+  // potential temperature (cfg.scal.s_flag): obukhov's results (psi_m, psi_h, phi_m,
+  // phi_h, u*, L, <sgs_t3(1)>, <theta(1)>), the per-level shift of theta's binary32
+  // transform, calcbeta's level means and the six 3/2-grid fields of the advection
+  double *d_obuk = nullptr, *tshift = nullptr, *d_tbar = nullptr;
+  void *tbig[6] = {};
+  double lzo_ = 0.0;           // log(0.5 dz / zo_), zo_ = exp(mean(log z0)) (obukhov :1205)
   void *cvb = nullptr;        // convec: y-padded half spectra of u1 u2 u3 w1 w2 w3 [6][nz][ny2][cvp_pitch]
   void *cvp = nullptr;        // convec: half spectra of the 3/2-grid products [3][nz][ny2][cvp_pitch]
   long cv_pitch = 0;          // kx columns kept on the 3/2 grid (nx/2) rounded up to 16
@@ -252,6 +259,7 @@ static bool field_is_f64(const lp_ctx *c, int id) {
     case LP_U: case LP_V: case LP_W:
     case LP_P: case LP_F_LM: case LP_F_MM: case LP_F_QN: case LP_F_NN:
     case LP_PCON: case LP_RHS_PCON: case LP_RHS_PCON_F: case LP_RHS_PBE:
+    case LP_THETA: case LP_SGS_T3: case LP_PSI_M:
       return true;
     default:
       return false;
@@ -302,6 +310,10 @@ extern "C" void lp_config_default(lp_config *c) {
   c->Co = 0.16; c->nnn = 2;
   c->use_mean_p_force = 0; c->mean_p_force = 1.0;
   c->cs2_init = 0.03; c->opftime = 1.5; c->filter_size = 1.0;
+  // param.f90:96, 244-283 (theta off; Pr 0.4, SCAL_init 1, lbc 1, T_scale 300 K, g 9.81)
+  lp_scalar_config &S = c->scal;
+  S.s_flag = 0; S.passive = 0; S.scal_init = 1; S.initsc = 0; S.lbc = 1; S.ocean = 0; S.iboussinesq = 3;
+  S.Pr = 0.4; S.wt_s = 0.0; S.T_scale = 300.0; S.dTdz_top = 0.0; S.g = 9.81; S.u_star = 1.0; S.z_i = 1.0;
 }
 
 static void host_wavenumbers(const lp_grid &g, std::vector<double> &kx, std::vector<double> &ky,
@@ -520,6 +532,8 @@ extern "C" int lp_destroy(lp_ctx *c) {
   fr(c->bw); fr(c->tw); fr(c->gam); fr(c->phat);
   for (auto p : c->pspec) fr(p);
   fr(c->d_err); fr(c->d_part); fr(c->d_stats); fr(c->d_espec); fr(c->ushift); fr(c->nyq); fr(c->adj_block); fr(c->adj_drop); fr(c->tavg); fr(c->d_pvol);
+  fr(c->d_obuk); fr(c->tshift); fr(c->d_tbar);
+  for (auto p : c->tbig) fr(p);
   fr(c->d_ws); fr(c->d_qsrc); fr(c->d_vfac); fr(c->d_foot); fr(c->d_bins); fr(c->d_pbeA);
   fr(c->ui0); fr(c->vi0); fr(c->kcell);
   for (auto e : c->ev) (void)hipEventDestroy(e);
@@ -552,6 +566,16 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
     return set_err(LP_ERR_ARG, "invalid lbc_mom");
   if (cfg->cs_count < 1) return set_err(LP_ERR_ARG, "cs_count must be >= 1");
   RCHK(droplet_check(*cfg));
+  if (cfg->scal.s_flag) {
+    const lp_scalar_config &S = cfg->scal;
+    if (S.lbc != 1)
+      return set_err(LP_ERR_ARG, "lp_create: theta lbc %d not implemented (1: prescribed surface flux; lbc 0 reads "
+                                 "z_os, which the reference never sets)", S.lbc);
+    if (!(S.Pr > 0) || !(S.T_scale > 0) || !(S.u_star > 0) || !(S.z_i > 0))
+      return set_err(LP_ERR_ARG, "lp_create: theta needs Pr, T_scale, u_star, z_i > 0");
+    if (S.ocean && (S.iboussinesq < 1 || S.iboussinesq > 3))
+      return set_err(LP_ERR_ARG, "lp_create: invalid iboussinesq %d", S.iboussinesq);
+  }
   lp_ctx *c = new lp_ctx();
   c->cfg = *cfg;
   c->g = g;
@@ -589,7 +613,8 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
   for (int i = 0; i < LP_NFIELDS; ++i) {
     long n = nv;
     if (i == LP_P) n = np * (g.nz + 1);
-    if (i == LP_USTAR) n = np;
+    if (i == LP_USTAR || i == LP_PSI_M) n = np;
+    if (i >= LP_THETA && i <= LP_BETA_SCAL && !cfg->scal.s_flag) n = 1;
     if (i == LP_PCON || i == LP_RHS_PCON || i == LP_RHS_PCON_F || i == LP_RHS_PBE)
       n = cfg->drop.npcon > 0 ? nv * cfg->drop.npcon : 1;
     if (i == LP_BETA_PCON && cfg->drop.npcon == 0) n = 1;
@@ -658,6 +683,19 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
   CK(dalloc((void **)&c->ushift, sizeof(double) * 2 * (size_t)g.nz));
   CK(dalloc((void **)&c->nyq, sizeof(double) * (size_t)g.nz * (g.ny + g.nx + 1)));
   CK(dalloc((void **)&c->d_stats, sizeof(double) * 10 * (size_t)(g.nz - 1)));
+  if (cfg->scal.s_flag) {
+    CK(dalloc((void **)&c->d_obuk, sizeof(double) * 8));
+    CK(dalloc((void **)&c->tshift, sizeof(double) * (size_t)g.nz));
+    CK(dalloc((void **)&c->d_tbar, sizeof(double) * (size_t)g.nz));
+    for (auto &p : c->tbig) CK(dalloc(&p, (size_t)g.ld_big * g.ny2 * g.nz * fe));
+    const double ob0[8] = {0.0, 0.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0.0};
+    CK(hipMemcpy(c->d_obuk, ob0, sizeof ob0, hipMemcpyHostToDevice) == hipSuccess
+           ? LP_OK
+           : set_err(LP_ERR_HIP, "lp_create: obukhov initial values"));
+    double slz = 0.0;  // :1205 sum(dlog(zo(1:nx,1:ny))) in order, zo uniform
+    for (long q = 0; q < (long)g.nx * g.ny; ++q) slz += std::log(cfg->z0);
+    c->lzo_ = std::log(0.5 * g.dz / std::exp(slz / ((double)g.nx * g.ny)));
+  }
   CK(dalloc((void **)&c->d_espec, sizeof(double) * (size_t)(g.nx / 2) * (g.nz - 1)));
   c->nred = 512;
   CK(dalloc((void **)&c->d_part, sizeof(double) * 3 * c->nred));
@@ -789,10 +827,10 @@ static int spectral_apply(lp_ctx *c, const void *f, int nlev, double scale, int 
 }
 
 // dealias1.f90: small field -> 3/2-grid physical field
-static int to_big(lp_ctx *c, const void *f, int nlev, void *out_big) {
+static int to_big(lp_ctx *c, const void *f, int nlev, void *out_big, RowIn ri = {}) {
   const lp_grid &g = c->g;
   double cst = 1.0 / ((double)g.nx * g.ny);
-  RCHK(r2c_small(c, f, nlev, cst, c->spec[0]));
+  RCHK(r2c_small(c, f, nlev, cst, c->spec[0], 0, ri));
   ColOut o{c->specb[0], M_ONE};
   const long ps = c->sp * g.ny, pb = c->sb * g.ny2;
   HIPCHK(col_op(c->fdt, g.ny, g.ny2, 1, 1, 1, c->spec[0], c->sp, ps, g.nx / 2, nlev, 1, &o, c->sb, pb,
@@ -915,7 +953,7 @@ static int do_wallstress(lp_ctx *c) {
   RCHK(spectral_apply(c, c->f[LP_U], 1, 1.0, 1, m, o1, vel));
   RCHK(spectral_apply(c, c->f[LP_V], 1, 1.0, 1, m, o2, vel));
   double zl = 0.5 * c->g.dz - cf.d0;
-  double denom = std::log(zl / cf.z0) - 0.0;
+  double denom = std::log(zl / cf.z0);  // less psi_m (a plane from obukhov with theta on)
   if (c->frz_ust) {  // frozen coefficient: the base run's u*
     FDISPATCH(c, T, hipLaunchKernelGGL(k_wallstress_frozen<T>, grid1(g.np), dim3(256), 0, c->s,
                                        (const T *)c->tmp[0], (const T *)c->tmp[1], (const double *)c->f[LP_U],
@@ -928,9 +966,113 @@ static int do_wallstress(lp_ctx *c) {
   FDISPATCH(c, T, hipLaunchKernelGGL(k_wallstress<T>, grid1(g.np), dim3(256), 0, c->s, (const T *)c->tmp[0],
                                      (const T *)c->tmp[1], (const double *)c->f[LP_U], (const double *)c->f[LP_V],
                                      (T *)c->f[LP_TXZ], (T *)c->f[LP_TYZ], (T *)c->f[LP_DUDZ],
-                                     (T *)c->f[LP_DVDZ], (T *)c->f[LP_USTAR], g, denom, zl, cf.vonk));
+                                     (T *)c->f[LP_DVDZ], (T *)c->f[LP_USTAR], g, denom, zl, cf.vonk,
+                                     (const double *)(cf.scal.s_flag ? c->f[LP_PSI_M] : nullptr),
+                                     (const double *)c->d_obuk));
   HIPCHK(hipGetLastError());
   return LP_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// potential temperature (scalars_module_crossflow_mod.f90, S_FLAG)
+// ------------------------------------------------------------------------------------
+// :1170-1403 obukhov (homogeneous surface): one workgroup over level 1
+static int do_obukhov(lp_ctx *c) {
+  const lp_config &cf = c->cfg;
+  const lp_scalar_config &S = cf.scal;
+  Geo g = geo(c);
+  ObkP P;
+  P.stab = c->jt >= S.scal_init ? 1 : 0;
+  P.zero_psi = (!P.stab && !S.initsc) ? 1 : 0;
+  P.set_t3 = (P.stab && c->jt == S.scal_init && !S.initsc) ? 1 : 0;
+  P.t3val = S.wt_s / S.u_star / S.T_scale;
+  P.vonk = cf.vonk;
+  P.dz = c->g.dz;
+  P.lz0 = std::log(0.5 * c->g.dz / cf.z0);
+  P.lzo_ = c->lzo_;
+  P.g_ = S.g / (S.u_star * S.u_star / S.z_i);
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_obukhov<T>, dim3(1), dim3(1024), 0, c->s, (const double *)c->f[LP_U],
+                                     (const double *)c->f[LP_V], (const double *)c->f[LP_THETA],
+                                     (double *)c->f[LP_SGS_T3], (double *)c->f[LP_PSI_M], (T *)c->f[LP_USTAR],
+                                     c->d_obuk, g, P));
+  HIPCHK(hipGetLastError());
+  return LP_OK;
+}
+
+// :141-360 theta_all_in_one: filt_da and ddz_uv of theta, scalar_RHS_calc (:474-1110),
+// calcbeta (:361-473), the Euler start (:272-274) and step_scalar (:1111-1169).  RHS_T and
+// RHS_Tf exchange buffers for :199 (scalar_RHS_calc rewrites every level of RHS_T); on a
+// failure they are exchanged back.
+static int theta_body(lp_ctx *c) {
+  const lp_config &cf = c->cfg;
+  const lp_scalar_config &S = cf.scal;
+  const lp_grid &gg = c->g;
+  Geo g = geo(c);
+  const long nv = g.np * g.nz;
+  void *th = c->f[LP_THETA], *dTdx = c->f[LP_DTDX], *dTdy = c->f[LP_DTDY], *dTdz = c->f[LP_DTDZ];
+  // :160, 175-178 (theta enters the binary32 transform less each level's first value)
+  RCHK(do_filt_da(c, th, dTdx, dTdy, gg.nz, nullptr, true, c->tshift));
+  RCHK(do_ddz_uv(c, dTdz, th, true));
+  const size_t fe = esz_of(c->fdt);
+  const double top = S.dTdz_top / S.T_scale * S.z_i;
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_fill<T>, grid1(g.np), dim3(256), 0, c->s,
+                                     (T *)((char *)dTdz + (size_t)(gg.nz - 1) * g.np * fe), g.np, (T)top));
+  // :530-575 the surface flux (wt_s through a default REAL, :145, 489) and dsdz(:,:,1)
+  const double wts = cf.fp32_casts ? (double)(float)S.wt_s : S.wt_s;
+  const double sflux = wts / S.T_scale / S.u_star;
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_theta_sfc<T>, dim3(1), dim3(1024), 0, c->s, (const T *)c->f[LP_USTAR],
+                                     (const double *)c->d_obuk, (T *)dTdz, g, sflux, cf.vonk, gg.dz));
+  HIPCHK(hipGetLastError());
+  // :808-879 dealias1 of u, v, w, dsdx, dsdy, dsdz; the products; dealias2
+  RowIn vel;
+  vel.din = F64;
+  const void *src[6] = {c->f[LP_U], c->f[LP_V], c->f[LP_W], dTdx, dTdy, dTdz};
+  for (int q = 0; q < 6; ++q) RCHK(to_big(c, src[q], gg.nz, c->tbig[q], q < 3 ? vel : RowIn{}));
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_theta_adv<T>, grid1((long)gg.ld_big * gg.ny2 * gg.nz), dim3(256), 0, c->s,
+                                     (T *)c->tbig[0], (const T *)c->tbig[1], (const T *)c->tbig[2],
+                                     (const T *)c->tbig[3], (const T *)c->tbig[4], (const T *)c->tbig[5], gg.nx2,
+                                     gg.ny2, gg.ld_big, gg.nz));
+  HIPCHK(hipGetLastError());
+  RCHK(from_big(c, c->tbig[0], gg.nz, 1.0 / ((double)gg.nx2 * gg.ny2), c->f[LP_RHS_T]));
+  // :889-1060 the SGS flux divergence
+  const double inv = 1.0 / S.Pr;
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_theta_flux_xy<T>, grid1(nv), dim3(256), 0, c->s, (const T *)c->f[LP_NU_T],
+                                     (const T *)dTdx, (const T *)dTdy, (T *)c->tmp[0], (T *)c->tmp[1], g, inv));
+  HIPCHK(hipGetLastError());
+  const double cst = 1.0 / ((double)gg.nx * gg.ny);
+  const int mx[1] = {M_IKX}, my[1] = {M_IKY};
+  void *ox[1] = {c->tmp[2]}, *oy[1] = {c->tmp[3]};
+  RCHK(spectral_apply(c, c->tmp[0], gg.nz, cst, 1, mx, ox));
+  RCHK(spectral_apply(c, c->tmp[1], gg.nz, cst, 1, my, oy));
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_theta_flux_z<T>, grid1(nv), dim3(256), 0, c->s, (const T *)c->f[LP_NU_T],
+                                     (const T *)dTdz, (T *)c->tmp[0], (double *)c->f[LP_SGS_T3], g, inv, sflux));
+  HIPCHK(hipGetLastError());
+  RCHK(do_ddz_w(c, c->tmp[1], c->tmp[0]));
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_theta_rhs<T>, grid1(nv), dim3(256), 0, c->s, (T *)c->f[LP_RHS_T],
+                                     (const T *)c->tmp[2], (const T *)c->tmp[3], (const T *)c->tmp[1], g));
+  HIPCHK(hipGetLastError());
+  // :246 calcbeta
+  hipLaunchKernelGGL(k_level_mean, dim3(gg.nz), dim3(1024), 0, c->s, (const double *)th, g, c->d_tbar);
+  const int mode = S.ocean ? S.iboussinesq : 0;
+  const double g_hat = S.g * (S.z_i / (S.u_star * S.u_star));
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_calcbeta<T>, grid1(nv), dim3(256), 0, c->s, (const double *)th,
+                                     (const double *)c->d_tbar, (T *)c->f[LP_BETA_SCAL], g, mode, g_hat,
+                                     2.e-4 * S.T_scale));
+  HIPCHK(hipGetLastError());
+  if (c->jt == S.scal_init && !S.initsc)  // :272-274
+    HIPCHK(hipMemcpyAsync(c->f[LP_RHS_TF], c->f[LP_RHS_T], c->bytes[LP_RHS_T], hipMemcpyDeviceToDevice, c->s));
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_step_scalar<T>, grid1(g.np * (g.nz - 1)), dim3(256), 0, c->s, (double *)th,
+                                     (const T *)c->f[LP_RHS_T], (const T *)c->f[LP_RHS_TF], g, cf.dt));
+  hipLaunchKernelGGL(k_scalar_top, grid1(g.np), dim3(256), 0, c->s, (double *)th, g, top * gg.dz,
+                     gg.L_z > 1.0 ? 1 : 0);
+  HIPCHK(hipGetLastError());
+  return LP_OK;
+}
+static int do_theta(lp_ctx *c) {
+  std::swap(c->f[LP_RHS_T], c->f[LP_RHS_TF]);
+  const int rc = theta_body(c);
+  if (rc != LP_OK) std::swap(c->f[LP_RHS_T], c->f[LP_RHS_TF]);
+  return rc;
 }
 
 static void swap_F(lp_ctx *c) {
@@ -1495,6 +1637,8 @@ static int step_body(lp_ctx *c) {
   const lp_grid &gg = c->g;
   Geo g = geo(c);
   long n1 = g.np * (g.nz - 1);
+  const bool theta = cf.scal.s_flag != 0;
+  if (theta) RCHK(do_obukhov(c));  // :257, from the unfiltered u(1), v(1)
   // :277-279 (u and v enter the binary32 transforms less a per-level value)
   RCHK(do_filt_da(c, c->f[LP_U], c->f[LP_DUDX], c->f[LP_DUDY], gg.nz, c->uvw_spec[0], true, c->ushift));
   RCHK(do_filt_da(c, c->f[LP_V], c->f[LP_DVDX], c->f[LP_DVDY], gg.nz, c->uvw_spec[1], true, c->ushift + gg.nz));
@@ -1521,6 +1665,15 @@ static int step_body(lp_ctx *c) {
     return rc_sgs;
   }
   RCHK(fork_convec(c));  // no Lagrangian step: start it here
+  if (theta) {  // :340-344
+    if (c->jt >= cf.scal.scal_init) {
+      RCHK(do_theta(c));
+    } else {
+      FDISPATCH(c, T, hipLaunchKernelGGL(k_fill<T>, grid1(g.np * g.nz), dim3(256), 0, c->s,
+                                         (T *)c->f[LP_BETA_SCAL], g.np * g.nz, T(0)));
+      HIPCHK(hipGetLastError());
+    }
+  }
   const bool drops = cf.drop.npcon > 0;
   if (drops) {
     if (c->jt_total >= cf.drop.pcon_init) {  // :360-372
@@ -1543,7 +1696,8 @@ static int step_body(lp_ctx *c) {
                                      (const T *)(drops && cf.drop.active ? c->f[LP_BETA_PCON] : nullptr),
                                      (T *)c->f[LP_RHSX_F], (T *)c->f[LP_RHSY_F], (T *)c->f[LP_RHSZ_F], first ? 1 : 0,
                                      (double *)c->f[LP_U], (double *)c->f[LP_V], (double *)c->f[LP_W], n1, cf.dt,
-                                     force, (T)cf.mean_p_force));
+                                     force, (T)cf.mean_p_force,
+                                     (const T *)(theta && !cf.scal.passive ? c->f[LP_BETA_SCAL] : nullptr)));
   HIPCHK(hipGetLastError());
   if (first) {  // :577-584 also copies the (BOGUS) top level
     const size_t fe = esz_of(c->fdt), off = (size_t)n1 * fe, pb = (size_t)g.np * fe;
@@ -1709,6 +1863,26 @@ extern "C" int lp_lagrange_sdep(lp_ctx *c) {
     return LP_OK;                                                                   \
   }
 DROP_ENTRY(lp_pollen_all_in_one2, do_pollen(c))
+
+#define THETA_ENTRY(name, body)                                                        \
+  extern "C" int name(lp_ctx *c) {                                                     \
+    if (!c) return set_err(LP_ERR_ARG, #name ": null context");                        \
+    if (!c->cfg.scal.s_flag) return set_err(LP_ERR_ARG, #name " needs scal.s_flag"); \
+    HIPCHK(hipSetDevice(c->dev));                                                      \
+    RCHK(body);                                                                        \
+    HIPCHK(hipStreamSynchronize(c->s));                                                \
+    return LP_OK;                                                                      \
+  }
+THETA_ENTRY(lp_obukhov, do_obukhov(c))
+THETA_ENTRY(lp_theta_all_in_one, do_theta(c))
+extern "C" int lp_obukhov_get(lp_ctx *c, double *out) {
+  if (!c || !out) return set_err(LP_ERR_ARG, "lp_obukhov_get: null argument");
+  if (!c->cfg.scal.s_flag) return set_err(LP_ERR_ARG, "lp_obukhov_get needs scal.s_flag");
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipMemcpyAsync(out, c->d_obuk, sizeof(double) * 8, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return LP_OK;
+}
 DROP_ENTRY(lp_pollen_rhs_calc2, do_pollen_rhs(c, nullptr))
 DROP_ENTRY(lp_calcbeta_pcon, do_beta_step(c, 1, 0))
 DROP_ENTRY(lp_step_pollen2, do_beta_step(c, 0, 1))
@@ -2166,6 +2340,9 @@ extern "C" int lp_set_frozen(lp_ctx *c, const void *nut, const void *ust) {
   if (!c) return set_err(LP_ERR_ARG, "lp_set_frozen: null context");
   if ((nut == nullptr) != (ust == nullptr))
     return set_err(LP_ERR_ARG, "lp_set_frozen: give both Nu_t and u*, or neither");
+  if (nut && c->cfg.scal.s_flag)
+    return set_err(LP_ERR_ARG, "lp_set_frozen: the frozen-coefficient step has no temperature (the wall law's "
+                               "stability functions would not be frozen)");
   if (nut && (!is_device_ptr(nut) || !is_device_ptr(ust)))
     return set_err(LP_ERR_ARG, "lp_set_frozen: Nu_t and u* must be device arrays");
   c->frz_nut = nut;
@@ -2382,6 +2559,8 @@ extern "C" int lp_step_adjoint2(lp_ctx *c, const double *nut, const double *ust,
   if (c->fdt != F64)
     return set_err(LP_ERR_ARG, "lp_step_adjoint runs in binary64: use an LP_FP64 context (the base states of a "
                                "binary32 run carry over, u, v, w being binary64 in both)");
+  if (c->cfg.scal.s_flag)
+    return set_err(LP_ERR_ARG, "lp_step_adjoint: the temperature (theta and its buoyancy) is not differentiated");
   const lp_droplet_config &D = c->cfg.drop;
   if (D.npcon > 0) {
     if (!cb || !rcb) return set_err(LP_ERR_ARG, "lp_step_adjoint: droplets need the PCon and RHS_PCon adjoints");

@@ -122,20 +122,22 @@ static __global__ void k_nyq_remove(double *__restrict__ f, const double *__rest
   f[i] = f[i] - (sx * A[(long)k * g.ny + y] + sy * B[(long)k * g.nx + x] - sx * sy * Cc[k]);
 }
 
-// wallstress.f90:101-185 ('wall', neutral psi_m = 0, phi_m = 1, uniform z0, d0)
+// wallstress.f90:101-185 ('wall', uniform z0, d0): denom = log(zl / z0) less psi_m of
+// obukhov (psi: the psi_m plane, ob[2]: phi_m; both null with theta off: 0 and 1)
 template <typename T>
 __global__ void k_wallstress(const T *__restrict__ u1, const T *__restrict__ v1, const double *__restrict__ u,
                              const double *__restrict__ v, T *__restrict__ txz, T *__restrict__ tyz,
                              T *__restrict__ dudz, T *__restrict__ dvdz, T *__restrict__ ust, Geo g,
-                             double denom, double zl, double vonk) {
+                             double denom, double zl, double vonk, const double *__restrict__ psi,
+                             const double *__restrict__ ob) {
   long i = gtid();
   if (i >= g.np) return;
   int x = (int)(i % g.ld);
   if (x >= g.nx) return;
-  const double phi_m = 1.0;
+  const double phi_m = ob ? ob[2] : 1.0;
   double a = u1[i], b = v1[i];
   double u_avg = sqrt(a * a + b * b);
-  double us = u_avg * vonk / denom;
+  double us = u_avg * vonk / (psi ? denom - psi[i] : denom);
   double cst = -(us * us) / u_avg;
   txz[i] = (T)(cst * a);
   tyz[i] = (T)(cst * b);
@@ -453,12 +455,13 @@ __global__ void k_rhs_ab2(T *__restrict__ Rx, T *__restrict__ Ry, T *__restrict_
                           T *__restrict__ ay, T *__restrict__ az, const T *__restrict__ beta_pcon,
                           T *__restrict__ Fx, T *__restrict__ Fy, T *__restrict__ Fz, int first,
                           double *__restrict__ u, double *__restrict__ v, double *__restrict__ w, long n, double dt,
-                          double force, T mpf) {
+                          double force, T mpf, const T *__restrict__ beta_scal) {
   long i = gtid();
   if (i >= n) return;
   const T rx = -Rx[i] - dx_[i];
   const T ry = -Ry[i] - dy_[i];
   T rz = -Rz[i] - dz_[i];
+  if (beta_scal) rz = rz + beta_scal[i];  // main.f90:450-453
   if (beta_pcon) rz = rz + beta_pcon[i];
   Rx[i] = rx;
   Ry[i] = ry;

@@ -21,6 +21,8 @@ module lesgo_amd
   public :: LP_OK, LP_FP64, LP_MIXED, LP_LBC_WALL, LP_LBC_STRESS_FREE
   public :: LP_U, LP_V, LP_W, LP_P, LP_RHSX, LP_RHSY, LP_RHSZ
   public :: LP_PCON, LP_RHS_PCON, LP_RHS_PCON_F, LP_RHS_PBE, LP_BETA_PCON
+  public :: LP_THETA, LP_RHS_T, LP_RHS_TF, LP_SGS_T3, LP_BETA_SCAL, LP_PSI_M
+  public :: lp_scalar_config, lp_obukhov, lp_obukhov_get, lp_theta_all_in_one
 
   integer(c_int), parameter :: LP_OK = 0
   integer(c_int32_t), parameter :: LP_FP64 = 0, LP_MIXED = 1
@@ -30,6 +32,8 @@ module lesgo_amd
   integer(c_int32_t), parameter :: LP_RHSX = 12, LP_RHSY = 13, LP_RHSZ = 14, LP_P = 27
   integer(c_int32_t), parameter :: LP_PCON = 47, LP_RHS_PCON = 48, LP_RHS_PCON_F = 49
   integer(c_int32_t), parameter :: LP_RHS_PBE = 50, LP_BETA_PCON = 51
+  integer(c_int32_t), parameter :: LP_THETA = 52, LP_RHS_T = 53, LP_RHS_TF = 54, LP_SGS_T3 = 58
+  integer(c_int32_t), parameter :: LP_BETA_SCAL = 59, LP_PSI_M = 60
 
   type, bind(C) :: lp_grid
     integer(c_int32_t) :: nx, ny, nz, ld, lh, nx2, ny2, lh_big, ld_big, pad_
@@ -45,6 +49,11 @@ module lesgo_amd
     type(c_ptr) :: diameter, settling_vel, Q_src
   end type lp_droplet_config
 
+  type, bind(C) :: lp_scalar_config
+    integer(c_int32_t) :: s_flag, passive, scal_init, initsc, lbc, ocean, iboussinesq, pad_
+    real(c_double) :: Pr, wt_s, T_scale, dTdz_top, g, u_star, z_i
+  end type lp_scalar_config
+
   type, bind(C) :: lp_config
     type(lp_grid) :: grid
     real(c_double) :: dt
@@ -53,6 +62,7 @@ module lesgo_amd
     integer(c_int32_t) :: nnn, use_mean_p_force
     real(c_double) :: mean_p_force, cs2_init, opftime, filter_size
     type(lp_droplet_config) :: drop
+    type(lp_scalar_config) :: scal
   end type lp_config
 
   interface
@@ -255,6 +265,20 @@ module lesgo_amd
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: field, bin
       real(c_double), intent(out) :: spec(*)
+    end function
+    ! obukhov, theta_all_in_one (scalars_module_crossflow_mod.f90:141-360, 1170-1403)
+    integer(c_int) function lp_obukhov(ctx) bind(C, name="lp_obukhov")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function lp_obukhov_get(ctx, vals) bind(C, name="lp_obukhov_get")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: vals(8)
+    end function
+    integer(c_int) function lp_theta_all_in_one(ctx) bind(C, name="lp_theta_all_in_one")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
     end function
     ! energy (energy.f90:1-47): kea
     integer(c_int) function lp_energy(ctx, kea) bind(C, name="lp_energy")

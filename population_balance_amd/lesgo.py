@@ -82,6 +82,9 @@ def load():
         "lp_plane_stats": ([vp, C.POINTER(C.c_double)], C.c_int),
         "lp_spectrum": ([vp, i32, i32, C.POINTER(C.c_double)], C.c_int),
         "lp_energy": ([vp, C.POINTER(C.c_double)], C.c_int),
+        "lp_obukhov": ([vp], C.c_int),
+        "lp_obukhov_get": ([vp, C.POINTER(C.c_double)], C.c_int),
+        "lp_theta_all_in_one": ([vp], C.c_int),
         "lp_tavg_reset": ([vp], C.c_int),
         "lp_tavg_compute": ([vp, C.c_double], C.c_int),
         "lp_tavg_get": ([vp, i32, vp, C.c_size_t, C.POINTER(C.c_double)], C.c_int),
@@ -371,6 +374,20 @@ class Lesgo:
         a = C.c_double()
         self._check(self.L.lp_energy(self.h, C.byref(a)), "energy")
         return a.value
+
+    def obukhov(self):
+        """obukhov (see lp_obukhov): the stability functions, psi_m and the u* plane."""
+        self._check(self.L.lp_obukhov(self.h), "obukhov")
+
+    def obukhov_get(self):
+        """the last obukhov call: psi_m, psi_h, phi_m, phi_h, u*, L, <sgs_t3(1)>, <theta(1)>"""
+        out = np.zeros(8)
+        self._check(self.L.lp_obukhov_get(self.h, out.ctypes.data_as(C.POINTER(C.c_double))), "obukhov_get")
+        return out
+
+    def theta_all_in_one(self):
+        """theta_all_in_one (see lp_theta_all_in_one): one temperature step."""
+        self._check(self.L.lp_theta_all_in_one(self.h), "theta_all_in_one")
 
     def tavg_reset(self):
         """tavg_init: zero the running time averages and their total time."""

@@ -24,11 +24,12 @@ initial.f90:74-140 reads vel_sc.out (initu and initsc) and :282-297 vel.out (ini
 without initsc) with a prefix of these layouts that depends on the SGS model number and
 inilag (read_prefix).
 
-The context holds u, v, w, the RHS, Cs_opt2, F_LM..F_NN, PCon and RHS_PCon.  The other
-arrays (temperature: theta, RHS_T, sgs_t3, psi_m; droplet diagnostics: deposition,
-Real_dep; Kc_t; the dynamic-Schmidt averages) are the caller's: the writers take them
-from `extras` (zeros where absent) and the readers return them, so a file passes
-through unchanged.  Kc_t is recomputed from Nu_t by every droplet step before it is
+The context holds u, v, w, the RHS, Cs_opt2, F_LM..F_NN, PCon and RHS_PCon, and with
+the temperature on (cfg.scal.s_flag) theta, RHS_T, sgs_t3 (the record's level 1) and
+psi_m.  The other arrays (droplet diagnostics: deposition, Real_dep; Kc_t; the
+dynamic-Schmidt averages; the temperature's arrays for a context without it) are the
+caller's: the writers take them from `extras` (zeros where absent) and the readers
+return them, so a file passes through unchanged.  Kc_t is recomputed from Nu_t by every droplet step before it is
 read (scalars_module_crossflow_mod.f90:3482-3549), so its stored value does not change
 a restarted run.  The droplet inertia's accelerations dudt, dvdt, dwdt are not in any
 layout: a restarted run (the reference's and this one's) starts them at zero.
@@ -60,6 +61,35 @@ LAYOUTS = {
     + _DROP_TAIL,
 }
 CONTEXT_FIELDS = set(VEL_FIELDS) | {"PCon", "RHS_PCon"}
+THETA_FIELDS = {"theta", "RHS_T", "sgs_t3", "psi_m"}  # context fields with the temperature on
+
+
+def _held(lg, name):
+    """whether the context holds record array `name`"""
+    return name in CONTEXT_FIELDS or (name in THETA_FIELDS and lg.cfg.scal.s_flag)
+
+
+def _get(lg, name, shp):
+    """record array `name` (C-order shape shp) from the context"""
+    a = lg.download(name).astype(np.float64)
+    if name == "sgs_t3":
+        return a[0]                      # sgs_t3(:,:,1)
+    if name == "psi_m":
+        return a[0, :, :shp[1]]          # (nx, ny) from the (ld, ny) plane
+    return a.reshape(-1)[: int(np.prod(shp))].reshape(shp)
+
+
+def _put(lg, name, a):
+    """record array `name` into the context"""
+    if name == "sgs_t3":
+        v = lg.download("sgs_t3")
+        v[0] = a
+        a = v
+    elif name == "psi_m":
+        v = np.zeros(lg.download("psi_m").shape)
+        v[0, :, :a.shape[1]] = a
+        a = v
+    lg.upload(name, a)
 
 
 def kind_of(s_flag, pcon_flag):
@@ -178,18 +208,19 @@ def pack_arrays(arrays, names, g, npcon):
 def write_restart(lg, path, kind=None, extras=None, max_sub=_MAX_SUB):
     """checkpoint_final_v2 (io.f90:740-786): the context state (and the caller's
     `extras` for the arrays the context does not hold; zeros where absent) as one record
-    of layout `kind` (default: "vel_pcon" with droplet bins, else "vel")."""
+    of layout `kind` (default: the one io.f90:752-760 picks for the context's
+    temperature and droplet flags)."""
     g = lg.g
     npc = lg.cfg.drop.npcon
-    kind = kind or ("vel_pcon" if npc > 0 else "vel")
+    kind = kind or kind_of(lg.cfg.scal.s_flag, npc > 0)
     extras = extras or {}
     arrays = {}
     for name in LAYOUTS[kind]:
         shp = shape_of(name, g, npc)
-        if name in CONTEXT_FIELDS:
+        if _held(lg, name):
             if name in ("PCon", "RHS_PCon") and npc == 0:
                 raise ValueError(f"restart layout {kind} needs droplet bins")
-            arrays[name] = lg.download(name).astype(np.float64).reshape(-1)[: int(np.prod(shp))].reshape(shp)
+            arrays[name] = _get(lg, name, shp)
         else:
             arrays[name] = extras[name] if name in extras else np.zeros(shp)
     with open(path, "wb") as f:
@@ -202,7 +233,7 @@ def read_restart(lg, path, kind=None, names=None):
     {name: array} of the others)."""
     g = lg.g
     npc = lg.cfg.drop.npcon
-    kind = kind or ("vel_pcon" if npc > 0 else "vel")
+    kind = kind or kind_of(lg.cfg.scal.s_flag, npc > 0)
     names = names or LAYOUTS[kind]
     if npc == 0 and ("PCon" in names or "RHS_PCon" in names):
         raise ValueError(f"restart layout {kind} holds droplet bins; the context has none")
@@ -210,8 +241,8 @@ def read_restart(lg, path, kind=None, names=None):
         arrays = split_record(read_record(f), names, g, npc)
     set_names, extras = [], {}
     for n in names:
-        if n in CONTEXT_FIELDS:
-            lg.upload(n, arrays[n])
+        if _held(lg, n):
+            _put(lg, n, arrays[n])
             set_names.append(n)
         else:
             extras[n] = arrays[n]

@@ -256,3 +256,47 @@ def test_droplet_restart_continues_run(tmp_path, precision):
         assert np.array_equal(a.download(k), c.download(k)), k
     a.close()
     c.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_theta_restart_continues_run(tmp_path, precision):
+    """with the temperature on (an unstable surface flux): 3 steps, a vel_sc file
+    (io.f90:767-770), a new context with initsc reading it (initial_multi.f90:71-74), 2
+    more steps == 5 uninterrupted steps, bit for bit (theta, RHS_T, sgs_t3(:,:,1) and
+    psi_m carry the temperature's state between steps)"""
+    from population_balance_amd import abi, ic
+    from population_balance_amd.lesgo import Lesgo
+    prec = abi.LP_FP64 if precision == "fp64" else abi.LP_MIXED
+    print(f"\n[case] theta restart continuation 32x32x17 {precision}", flush=True)
+
+    def cfg_of(initsc):
+        cfg = abi.make_config(32, 32, 17, precision=prec, L_z=1.5)
+        abi.set_scalar_defaults(cfg.scal, s_flag=1)
+        cfg.scal.wt_s, cfg.scal.dTdz_top, cfg.scal.initsc = 0.1, 0.01, initsc
+        return cfg
+    cfg = cfg_of(0)
+    g = cfg.grid
+    u, v, w = ic.turbulent_ic(g)
+    rng = np.random.default_rng(5)
+    th = 0.95 + 1e-3 * rng.standard_normal((g.nz, g.ny, g.ld))
+    th[..., g.nx:] = 0.0
+    a, b = Lesgo(cfg), Lesgo(cfg)
+    for lg in (a, b):
+        for k, x in (("u", u), ("v", v), ("w", w), ("theta", th)):
+            lg.upload(k, x)
+    a.step(5)
+    b.step(3)
+    f = str(tmp_path / "vel_sc.out")
+    restart.write_restart(b, f)
+    jt, jt_total = b.time()
+    b.close()
+    c = Lesgo(cfg_of(1))
+    names, extras = restart.read_restart(c, f)
+    assert names == restart.LAYOUTS["vel_sc"] and not extras
+    c.set_time(jt, jt_total)
+    c.step(2)
+    for k in ("u", "v", "w", "RHSz", "p", "theta", "RHS_T", "sgs_t3", "psi_m"):
+        assert np.array_equal(a.download(k), c.download(k)), k
+    a.close()
+    c.close()
