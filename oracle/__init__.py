@@ -65,7 +65,8 @@ def lib():
         for n in ("or_wallstress", "or_sgs_stag", "or_lagrange_sdep", "or_interpolag_sdep",
                   "or_convec", "or_press_stag_array", "or_project", "or_pollen_all_in_one2",
                   "or_pollen_rhs_calc2", "or_calcbeta_pcon", "or_step_pollen2", "or_obukhov",
-                  "or_theta_all_in_one"):
+                  "or_theta_all_in_one", "or_std_dynamic", "or_scaledep_dynamic", "or_lagrange_ssim",
+                  "or_interpolag_ssim"):
             getattr(L, n).argtypes = [vp]
             getattr(L, n).restype = C.c_int
         L.or_tridag_array.restype = C.c_int
@@ -73,6 +74,8 @@ def lib():
         L.or_rmsdiv.argtypes = [vp, dp]
         L.or_plane_stats.argtypes = [vp, dp]
         L.or_obukhov_get.argtypes = [vp, dp]
+        L.or_plane_dynamic_level.argtypes = [vp, C.c_int, C.c_int, dp]
+        L.or_dyn_contract.argtypes = [vp, C.c_int, dp, dp]
         L.or_energy.argtypes = [vp, dp]
         L.or_energy.restype = C.c_double
         L.or_tavg_compute.argtypes = [vp, C.c_double, dp]
@@ -280,6 +283,21 @@ class Oracle:
     def call(self, name):
         rc = getattr(self.L, "or_" + name)(self.h)
         self._check(rc, name)
+
+    def plane_dynamic_level(self, model, k):
+        """std_dynamic (2) / scaledep_dynamic (3) of level k from the current strain rates:
+        (a1, b1, c1, d1, e1, a2, b2, c2, d2, e2, beta, Cs^2)"""
+        out = np.zeros(12)
+        self.L.or_plane_dynamic_level(self.h, int(model), int(k), out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def dyn_contract(self, k):
+        """(L.M, M.M) planes of level k (M with 4 |S|bar S_ij bar) from the current strain"""
+        g = self.cfg.grid
+        lm, mm = np.zeros((g.ny, g.ld)), np.zeros((g.ny, g.ld))
+        self.L.or_dyn_contract(self.h, int(k), lm.ctypes.data_as(C.POINTER(C.c_double)),
+                               mm.ctypes.data_as(C.POINTER(C.c_double)))
+        return lm, mm
 
     def obukhov_get(self):
         """the last obukhov call: psi_m, psi_h, phi_m, phi_h, u*, L, <sgs_t3(1)>, <theta(1)>"""

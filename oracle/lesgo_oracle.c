@@ -628,16 +628,19 @@ int or_wallstress(or_ctx *c) {
 /* ------------------------------------------------------------------------- */
 /* interpolag_Sdep.f90:2-373                                                  */
 /* ------------------------------------------------------------------------- */
-int or_interpolag_sdep(or_ctx *c) {
+/* nf of the Lagrangian averages (fid) carried back along the trajectories.  ssim 0:
+ * interpolag_Sdep.f90:2-373 (four averages; the wall's zp doubled and clamped to [-1, 1],
+ * :281-291); ssim 1: interpolag_Ssim.f90:1-160 (F_LM, F_MM; at the wall only
+ * zp(2) += min(zp(2), 0) and zp(1) += max(zp(1), 0), :176-185) */
+static int interpolag(or_ctx *c, int nf, const int *fid, int ssim) {
   const lp_grid *g = &c->g;
   const lp_config *cf = &c->cfg;
   int nx = g->nx, ny = g->ny, nz = g->nz, ld = g->ld;
   size_t np = plane(c);
   size_t fn = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
   double *FF[4];
-  const int fid[4] = {LP_F_LM, LP_F_MM, LP_F_QN, LP_F_NN};
 #define FFI(i, j, k) (((size_t)(k) * (ny + 2) + (j)) * (nx + 2) + (i))
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < nf; ++q) {
     FF[q] = (double *)malloc(sizeof(double) * fn);
     const double *F = c->f[fid[q]];
     /* :62-71 interior; :75-105 periodic x, y wrap and z copy (incl. corners) */
@@ -673,16 +676,21 @@ int or_interpolag_sdep(or_ctx *c) {
     yp[i] = -vl[i] * cf->dt * cs / g->dy;
     zp[i] = -wl[i] * cf->dt * cs / g->dz;
   }
-  for (size_t i = 0; i < np; ++i) {                                                    /* :281-291 */
+  for (size_t i = 0; i < np; ++i) {
     double z2 = zp[np + i];
     z2 = z2 + fmin(z2, 0.0);
-    z2 = fmin(1.0, z2);
-    z2 = fmax(-1.0, z2);
-    zp[np + i] = z2;
-    zp[i] = z2;
+    if (ssim) {                                                                        /* Ssim :180-185 */
+      zp[np + i] = z2;
+      zp[i] = zp[i] + fmax(zp[i], 0.0);
+    } else {                                                                           /* Sdep :281-291 */
+      z2 = fmin(1.0, z2);
+      z2 = fmax(-1.0, z2);
+      zp[np + i] = z2;
+      zp[i] = z2;
+    }
   }
   double *Fo[4];
-  for (int q = 0; q < 4; ++q) Fo[q] = c->f[fid[q]];
+  for (int q = 0; q < nf; ++q) Fo[q] = c->f[fid[q]];
 #pragma omp parallel for
   for (int k = 0; k < nz; ++k)                                                          /* :301-365 */
     for (int j = 0; j < ny; ++j)
@@ -695,7 +703,7 @@ int or_interpolag_sdep(or_ctx *c) {
         int jxa = jjx + addx, jya = jjy + addy, jza = jjz + addz;
         double cx_ = fabs(xp[s]), cy_ = fabs(yp[s]), cz_ = fabs(zp[s]);
         double fx = 1.0 - cx_, fy = 1.0 - cy_, fz = 1.0 - cz_;
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < nf; ++q) {
           const double *F = FF[q];
           Fo[q][s] = fx * fy * (F[FFI(jjx, jjy, jjz)] * fz + F[FFI(jjx, jjy, jza)] * cz_)
                    + fx * cy_ * (F[FFI(jjx, jya, jjz)] * fz + F[FFI(jjx, jya, jza)] * cz_)
@@ -705,9 +713,17 @@ int or_interpolag_sdep(or_ctx *c) {
       }
 #undef FFI
   for (size_t i = 0; i < np * nz; ++i) { ul[i] = 0; vl[i] = 0; wl[i] = 0; }           /* :367-369 */
-  for (int q = 0; q < 4; ++q) free(FF[q]);
+  for (int q = 0; q < nf; ++q) free(FF[q]);
   free(xp); free(yp); free(zp);
   return 0;
+}
+int or_interpolag_sdep(or_ctx *c) {
+  const int fid[4] = {LP_F_LM, LP_F_MM, LP_F_QN, LP_F_NN};
+  return interpolag(c, 4, fid, 0);
+}
+int or_interpolag_ssim(or_ctx *c) {
+  const int fid[2] = {LP_F_LM, LP_F_MM};
+  return interpolag(c, 2, fid, 1);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -880,6 +896,274 @@ int or_lagrange_sdep(or_ctx *c) {
 }
 
 /* ------------------------------------------------------------------------- */
+/* The other dynamic SGS models of sgs_stag.f90:188-198: std_dynamic.f90      */
+/* (model 2), scaledep_dynamic.f90 (3), lagrange_Ssim.f90 (4).  Their plane   */
+/* sums run over the physical columns x < nx (the reference sums the (ld, ny) */
+/* planes, whose padding columns hold zero after the in-place inverse of a    */
+/* spectrum with no Nyquist column).                                          */
+/* ------------------------------------------------------------------------- */
+/* The filtered fields of one level: 2-delta (G_test) L_ij, S_ij, |S| S_ij, |S|bar;
+ * with hat, also the 4-delta ones of scaledep_dynamic.f90:85-152 (Q_ij from the
+ * products of the 2-delta filtered velocities, :100-120). Component order 11 12 13 22 23 33. */
+typedef struct {
+  double *L[6], *Sb[6], *SSb[6], *sbar;
+  double *Q[6], *Sh[6], *SSh[6], *shat;
+  double *buf;
+} dyn_lev;
+
+static void dyn_level(or_ctx *c, int k, int hat, dyn_lev *d) {
+  const size_t np = plane(c);
+  const size_t o = (size_t)k * np;
+  const double *u = c->f[LP_U], *v = c->f[LP_V], *w = c->f[LP_W];
+  double **S = c->S;
+  d->buf = (double *)calloc(np * 48, sizeof(double));
+  double *b = d->buf;
+  double *ub = b, *vb = b + np, *wb = b + 2 * np, *uh = b + 3 * np, *vh = b + 4 * np, *wh = b + 5 * np;
+  for (int q = 0; q < 6; ++q) {
+    d->L[q] = b + (6 + q) * np; d->Q[q] = b + (12 + q) * np;
+    d->Sb[q] = b + (18 + q) * np; d->Sh[q] = b + (24 + q) * np;
+    d->SSb[q] = b + (30 + q) * np; d->SSh[q] = b + (36 + q) * np;
+  }
+  d->sbar = b + 42 * np; d->shat = b + 43 * np;
+  double *Sm = b + 44 * np;
+  for (size_t i = 0; i < np; ++i) {                                /* std_dynamic.f90:36-62 */
+    if (k == 0) { ub[i] = u[i]; vb[i] = v[i]; wb[i] = 0.25 * w[np + i]; }
+    else {
+      ub[i] = 0.5 * (u[o + i] + u[o - np + i]);
+      vb[i] = 0.5 * (v[o + i] + v[o - np + i]);
+      wb[i] = w[o + i];
+    }
+    uh[i] = ub[i]; vh[i] = vb[i]; wh[i] = wb[i];
+    d->L[0][i] = ub[i] * ub[i]; d->L[1][i] = ub[i] * vb[i]; d->L[2][i] = ub[i] * wb[i];
+    d->L[3][i] = vb[i] * vb[i]; d->L[4][i] = vb[i] * wb[i]; d->L[5][i] = wb[i] * wb[i];
+  }
+  test_filter_plane(c, ub, c->G_test); test_filter_plane(c, vb, c->G_test);   /* :64-78 */
+  test_filter_plane(c, wb, c->G_test);
+  for (int q = 0; q < 6; ++q) test_filter_plane(c, d->L[q], c->G_test);
+  for (size_t i = 0; i < np; ++i) {
+    d->L[0][i] -= ub[i] * ub[i]; d->L[1][i] -= ub[i] * vb[i]; d->L[2][i] -= ub[i] * wb[i];
+    d->L[3][i] -= vb[i] * vb[i]; d->L[4][i] -= vb[i] * wb[i]; d->L[5][i] -= wb[i] * wb[i];
+  }
+  if (hat) {                                                       /* scaledep :100-120 */
+    for (size_t i = 0; i < np; ++i) {
+      d->Q[0][i] = ub[i] * ub[i]; d->Q[1][i] = ub[i] * vb[i]; d->Q[2][i] = ub[i] * wb[i];
+      d->Q[3][i] = vb[i] * vb[i]; d->Q[4][i] = vb[i] * wb[i]; d->Q[5][i] = wb[i] * wb[i];
+    }
+    test_filter_plane(c, uh, c->G_test_test); test_filter_plane(c, vh, c->G_test_test);
+    test_filter_plane(c, wh, c->G_test_test);
+    for (int q = 0; q < 6; ++q) test_filter_plane(c, d->Q[q], c->G_test_test);
+    for (size_t i = 0; i < np; ++i) {
+      d->Q[0][i] -= uh[i] * uh[i]; d->Q[1][i] -= uh[i] * vh[i]; d->Q[2][i] -= uh[i] * wh[i];
+      d->Q[3][i] -= vh[i] * vh[i]; d->Q[4][i] -= vh[i] * wh[i]; d->Q[5][i] -= wh[i] * wh[i];
+    }
+  }
+  for (size_t i = 0; i < np; ++i) {                                /* std_dynamic.f90:80-110 */
+    double s11 = S[0][o + i], s12 = S[1][o + i], s13 = S[2][o + i];
+    double s22 = S[3][o + i], s23 = S[4][o + i], s33 = S[5][o + i];
+    Sm[i] = sqrt(2.0 * (s11 * s11 + s22 * s22 + s33 * s33 + 2.0 * (s12 * s12 + s13 * s13 + s23 * s23)));
+    for (int q = 0; q < 6; ++q) {
+      d->Sb[q][i] = S[q][o + i];
+      d->Sh[q][i] = S[q][o + i];
+      d->SSb[q][i] = Sm[i] * S[q][o + i];
+      d->SSh[q][i] = d->SSb[q][i];
+    }
+  }
+  for (int q = 0; q < 6; ++q) {
+    test_filter_plane(c, d->Sb[q], c->G_test);
+    test_filter_plane(c, d->SSb[q], c->G_test);
+    if (hat) {
+      test_filter_plane(c, d->Sh[q], c->G_test_test);
+      test_filter_plane(c, d->SSh[q], c->G_test_test);
+    }
+  }
+#define MAG(A, i) sqrt(2.0 * (A[0][i] * A[0][i] + A[3][i] * A[3][i] + A[5][i] * A[5][i] + \
+                              2.0 * (A[1][i] * A[1][i] + A[2][i] * A[2][i] + A[4][i] * A[4][i])))
+  for (size_t i = 0; i < np; ++i) {
+    d->sbar[i] = MAG(d->Sb, i);
+    if (hat) d->shat[i] = MAG(d->Sh, i);
+  }
+#undef MAG
+}
+
+/* a . b in the reference's order: 11 + 22 + 33 + 2 (12 + 13 + 23) */
+static inline double dot6(double *const *a, double *const *b, size_t i) {
+  return a[0][i] * b[0][i] + a[3][i] * b[3][i] + a[5][i] * b[5][i] +
+         2.0 * (a[1][i] * b[1][i] + a[2][i] * b[2][i] + a[4][i] * b[4][i]);
+}
+
+/* L.M and M.M at one point with M_ij = cst (|S|S_ij bar - 4 b |S|bar S_ij bar) */
+static void lm_mm(const dyn_lev *d, size_t i, double cst, double fourb, double *lm, double *mm) {
+  double M[6];
+  for (int q = 0; q < 6; ++q) M[q] = cst * (d->SSb[q][i] - fourb * d->sbar[i] * d->Sb[q][i]);
+  *lm = d->L[0][i] * M[0] + d->L[3][i] * M[3] + d->L[5][i] * M[5] +
+        2.0 * (d->L[1][i] * M[1] + d->L[2][i] * M[2] + d->L[4][i] * M[4]);
+  *mm = M[0] * M[0] + M[3] * M[3] + M[5] * M[5] + 2.0 * (M[1] * M[1] + M[2] * M[2] + M[4] * M[4]);
+}
+
+/* sgsmodule.f90:47-69 rtnewt: Newton from 7.5 on the quintic A, |dx| < 0.001, else 1 */
+static double rtnewt(const double *A) {
+  double r = 0.5 * (0.0 + 15.0);
+  for (int j = 0; j < 100; ++j) {
+    const double f = A[0] + r * (A[1] + r * (A[2] + r * (A[3] + r * (A[4] + r * A[5]))));
+    const double df = A[1] + r * (2.0 * A[2] + r * (3.0 * A[3] + r * (4.0 * A[4] + r * (5.0 * A[5]))));
+    const double dx = f / df;
+    r = r - dx;
+    if (fabs(dx) < 0.001) return r;
+  }
+  return 1.0;
+}
+
+/* std_dynamic.f90:1-136 (model 2) and scaledep_dynamic.f90:1-230 (model 3): one Cs^2 per
+ * level from plane sums, written over the whole plane (sgs_stag.f90:190-194).  Model 3's
+ * beta, local to scaledep_dynamic (its output file, :231-245, is not written), is left in
+ * Beta, which no other part of these models reads */
+/* one level: out[0..9] = a1, b1, c1, d1, e1, a2, b2, c2, d2, e2 (scaledep_dynamic.f90:156-198;
+ * the 4-delta ones only for model 3), out[10] = beta, out[11] = Cs^2 */
+static void plane_dynamic_level(or_ctx *c, int model, int k, double *out) {
+  const lp_grid *g = &c->g;
+  const lp_config *cf = &c->cfg;
+  const int nx = g->nx, ny = g->ny, ld = g->ld;
+  const double delta = cf->filter_size * pow(g->dx * g->dy * g->dz, 1.0 / 3.0);
+  const double cst = 2.0 * delta * delta;
+  const double nxy = (double)(nx * ny);
+  {
+    dyn_lev d;
+    dyn_level(c, k, model == 3, &d);
+    double beta = 1.0;
+    {                                                              /* scaledep :156-209 */
+      double s[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          const size_t i = (size_t)y * ld + x;
+          const double sb = d.sbar[i], sh = d.shat[i];
+          s[0] += sb * dot6(d.Sb, d.L, i);
+          s[1] += dot6(d.SSb, d.L, i);
+          s[2] += dot6(d.SSb, d.SSb, i);
+          s[3] += 0.5 * (sb * sb) * (sb * sb);
+          s[4] += sb * dot6(d.Sb, d.SSb, i);
+          s[5] += sh * dot6(d.Sh, d.Q, i);
+          s[6] += dot6(d.SSh, d.Q, i);
+          s[7] += dot6(d.SSh, d.SSh, i);
+          s[8] += 0.5 * (sh * sh) * (sh * sh);
+          s[9] += sh * dot6(d.Sh, d.SSh, i);
+        }
+      const double d2 = delta * delta, c2 = (2.0 * d2) * (2.0 * d2);
+      const double a1 = -2.0 * d2 * 4.0 * s[0] / nxy, b1 = -2.0 * d2 * s[1] / nxy;
+      const double c1 = c2 * s[2] / nxy, dd1 = c2 * 16.0 * s[3] / nxy, e1 = 2.0 * c2 * 4.0 * s[4] / nxy;
+      const double a2 = -2.0 * d2 * 16.0 * s[5] / nxy, b2 = -2.0 * d2 * s[6] / nxy;
+      const double cc2 = c2 * s[7] / nxy, dd2 = c2 * 256.0 * s[8] / nxy, e2 = 2.0 * c2 * 16.0 * s[9] / nxy;
+      double A[6];
+      A[0] = b2 * c1 - b1 * cc2;
+      A[1] = a1 * cc2 - b2 * e1;
+      A[2] = b2 * dd1 + b1 * e2 - a2 * c1;
+      A[3] = a2 * e1 - a1 * e2;
+      A[4] = -a2 * dd1 - b1 * dd2;
+      A[5] = a1 * dd2;
+      if (model == 3) beta = rtnewt(A);
+      const double ab[10] = {a1, b1, c1, dd1, e1, a2, b2, cc2, dd2, e2};
+      memcpy(out, ab, sizeof ab);
+    }
+    double slm = 0.0, smm = 0.0;                                   /* std_dynamic :126-130 */
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        double lm, mm;
+        lm_mm(&d, (size_t)y * ld + x, cst, 4.0 * beta, &lm, &mm);
+        slm += lm;
+        smm += mm;
+      }
+    double cs = slm / smm;
+    cs = fmax(0.0, cs);
+    out[10] = beta;
+    out[11] = cs;
+    free(d.buf);
+  }
+}
+static int or_plane_dynamic(or_ctx *c, int model) {
+  const size_t np = plane(c);
+  const int nz = c->g.nz;
+#pragma omp parallel for schedule(dynamic)
+  for (int k = 0; k < nz; ++k) {
+    double out[12];
+    plane_dynamic_level(c, model, k, out);
+    for (size_t i = 0; i < np; ++i) {
+      c->f[LP_CS_OPT2][(size_t)k * np + i] = out[11];
+      if (model == 3) c->f[LP_BETA][(size_t)k * np + i] = out[10];
+    }
+  }
+  return 0;
+}
+/* L.M and M.M (M with 4 |S|bar S_ij bar) at every point of level k (plane arrays; zero in
+ * the padding columns) from the current velocities and strain (after an or_sgs_stag) */
+void or_dyn_contract(or_ctx *c, int k, double *lm, double *mm) {
+  const lp_grid *g = &c->g;
+  const double delta = c->cfg.filter_size * pow(g->dx * g->dy * g->dz, 1.0 / 3.0);
+  dyn_lev d;
+  dyn_level(c, k, 0, &d);
+  for (int y = 0; y < g->ny; ++y)
+    for (int x = 0; x < g->ld; ++x) {
+      const size_t i = (size_t)y * g->ld + x;
+      lm[i] = mm[i] = 0.0;
+      if (x < g->nx) lm_mm(&d, i, 2.0 * delta * delta, 4.0, &lm[i], &mm[i]);
+    }
+  free(d.buf);
+}
+/* the sums, beta and Cs^2 of level k from the current strain (after an or_sgs_stag) */
+void or_plane_dynamic_level(or_ctx *c, int model, int k, double *out) { plane_dynamic_level(c, model, k, out); }
+int or_std_dynamic(or_ctx *c) { return or_plane_dynamic(c, 2); }
+int or_scaledep_dynamic(or_ctx *c) { return or_plane_dynamic(c, 3); }
+
+/* lagrange_Ssim.f90:1-322 (model 4; use_bldg, TREES, inflow off: Beta = 1), with
+ * interpolag_Ssim; the reference's REAL(4) conversions of :281-303 when fp32_casts */
+int or_lagrange_ssim(or_ctx *c) {
+  const lp_grid *g = &c->g;
+  const lp_config *cf = &c->cfg;
+  const int nx = g->nx, ny = g->ny, nz = g->nz, ld = g->ld;
+  const size_t np = plane(c);
+  const int casts = cf->fp32_casts;
+  const double tiny = casts ? (double)1e-32f : 1e-32;
+  const double delta = cf->filter_size * pow(g->dx * g->dy * g->dz, 1.0 / 3.0);
+  const double opftdelta = cf->opftime * delta, powcoeff = -1.0 / 8.0;
+  int rc = or_interpolag_ssim(c);                                          /* :103 */
+  if (rc) return rc;
+  const double lagran_dt = cf->dt * (double)cf->cs_count;
+  const double cst = 2.0 * delta * delta;
+  const int do_init = cf->inilag && !c->F_LM_MM_init && (c->jt == cf->cs_count || c->jt == cf->dyn_init);
+  double *FLM = c->f[LP_F_LM], *FMM = c->f[LP_F_MM], *Cs = c->f[LP_CS_OPT2], *Bt = c->f[LP_BETA];
+#pragma omp parallel for schedule(dynamic)
+  for (int k = 0; k < nz; ++k) {
+    dyn_lev d;
+    dyn_level(c, k, 0, &d);
+    const size_t o = (size_t)k * np;
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < ld; ++x) {
+        const size_t i = (size_t)y * ld + x, s = o + i;
+        const int padc = x >= nx;
+        double lm = 0.0, mm = 0.0;  /* padding columns: the zero Nyquist column */
+        if (!padc) lm_mm(&d, i, cst, 4.0 * 1.0, &lm, &mm);               /* :178-192 */
+        if (do_init) {                                                     /* :193-201 */
+          FMM[s] = mm;
+          FLM[s] = 0.025 * mm;
+          if (x >= ld - 2) { FMM[s] = 1.0; FLM[s] = 1.0; }
+        }
+        double Tn = casts ? (double)fmaxf((float)(FLM[s] * FMM[s]), 1e-32f) : fmax(FLM[s] * FMM[s], tiny); /* :208 */
+        Tn = opftdelta * pow(Tn, powcoeff);
+        const double dum = lagran_dt / Tn, epsi = dum / (1.0 + dum);
+        FLM[s] = (epsi * lm + (1.0 - epsi) * FLM[s]);                     /* :212-213 */
+        FMM[s] = (epsi * mm + (1.0 - epsi) * FMM[s]);
+        FLM[s] = casts ? (double)fmaxf(1e-32f, (float)FLM[s]) : fmax(tiny, FLM[s]);  /* :214 */
+        double cs = FLM[s] / FMM[s];                                       /* :215-218 */
+        if (x >= ld - 2) cs = 0.0;
+        Cs[s] = casts ? (double)fmaxf(1e-32f, (float)cs) : fmax(tiny, cs);
+        Bt[s] = 1.0;
+      }
+    free(d.buf);
+  }
+  if (do_init) c->F_LM_MM_init = 1;
+  return 0;
+}
+
+
+/* ------------------------------------------------------------------------- */
 /* sgs_stag.f90:3-401 (sgs on, molec off, OCEAN/SEABED off)                   */
 /* ------------------------------------------------------------------------- */
 int or_sgs_stag(or_ctx *c) {
@@ -939,17 +1223,19 @@ int or_sgs_stag(or_ctx *c) {
       double n = (double)cf->nnn;
       lz[k] = pow(pow(cf->Co, n) * pow(cf->vonk * zz, -n) + pow(delta, -n), -1.0 / n);
     }
-  } else if (cf->model == 5) {
+  } else if (cf->model >= 2 && cf->model <= 5) {
     for (int k = 0; k < nz; ++k) lz[k] = delta;                               /* :177 */
     if (c->jt == 1 && cf->inilag) {                                           /* :178-180 */
       for (size_t i = 0; i < nv; ++i) Cs[i] = cf->cs2_init;
     } else if ((c->jt >= cf->dyn_init || cf->initu) && (c->jt % cf->cs_count) == 0) {
-      int rc = or_lagrange_sdep(c);                                           /* :200-201 */
+      int rc = cf->model == 5   ? or_lagrange_sdep(c)                         /* :200-201 */
+               : cf->model == 4 ? or_lagrange_ssim(c)                         /* :198-199 */
+                                : or_plane_dynamic(c, cf->model);             /* :188-197 */
       if (rc) { free(lz); return rc; }
     }
   } else {
     free(lz);
-    return fail(LP_ERR_ARG, "unsupported SGS model (1 and 5 are implemented)");
+    return fail(LP_ERR_ARG, "unsupported SGS model (1 to 5 are implemented)");
   }
   /* :215-237 |S|, Nu_t, dissip, magS on 1..nz */
 #pragma omp parallel for

@@ -58,6 +58,12 @@ int or_sgs_stag(or_ctx *c);
 int or_lagrange_sdep(or_ctx *c);
 int or_interpolag_sdep(or_ctx *c);
 int or_convec(or_ctx *c);
+int or_std_dynamic(or_ctx *c);
+void or_plane_dynamic_level(or_ctx *c, int model, int k, double *out);
+void or_dyn_contract(or_ctx *c, int k, double *lm, double *mm);
+int or_scaledep_dynamic(or_ctx *c);
+int or_lagrange_ssim(or_ctx *c);
+int or_interpolag_ssim(or_ctx *c);
 int or_obukhov(or_ctx *c);
 void or_obukhov_get(or_ctx *c, double *out);
 int or_theta_all_in_one(or_ctx *c);

@@ -20,6 +20,7 @@
 #include "lp_convec.hpp"
 #include "lp_adjoint.hpp"
 #include "lp_scalar.hpp"
+#include "lp_sgs_alt.hpp"
 
 using namespace lp;
 
@@ -145,6 +146,10 @@ struct lp_ctx {
   // transform, calcbeta's level means and the six 3/2-grid fields of the advection
   double *d_obuk = nullptr, *tshift = nullptr, *d_tbar = nullptr;
   void *tbig[6] = {};
+  // SGS models 2-4 (lp_sgs_alt.hpp): the 21 filter inputs, their 2-delta and (model 3)
+  // 4-delta filtered volumes, model 3's plane sums [nz][10] and beta [nz]
+  void *dyn_in = nullptr, *dyn_bar = nullptr, *dyn_hat = nullptr;
+  double *dyn_sums = nullptr, *dyn_beta = nullptr;
   double lzo_ = 0.0;           // log(0.5 dz / zo_), zo_ = exp(mean(log z0)) (obukhov :1205)
   void *cvb = nullptr;        // convec: y-padded half spectra of u1 u2 u3 w1 w2 w3 [6][nz][ny2][cvp_pitch]
   void *cvp = nullptr;        // convec: half spectra of the 3/2-grid products [3][nz][ny2][cvp_pitch]
@@ -534,6 +539,7 @@ extern "C" int lp_destroy(lp_ctx *c) {
   fr(c->d_err); fr(c->d_part); fr(c->d_stats); fr(c->d_espec); fr(c->ushift); fr(c->nyq); fr(c->adj_block); fr(c->adj_drop); fr(c->tavg); fr(c->d_pvol);
   fr(c->d_obuk); fr(c->tshift); fr(c->d_tbar);
   for (auto p : c->tbig) fr(p);
+  fr(c->dyn_in); fr(c->dyn_bar); fr(c->dyn_hat); fr(c->dyn_sums); fr(c->dyn_beta);
   fr(c->d_ws); fr(c->d_qsrc); fr(c->d_vfac); fr(c->d_foot); fr(c->d_bins); fr(c->d_pbeA);
   fr(c->ui0); fr(c->vi0); fr(c->kcell);
   for (auto e : c->ev) (void)hipEventDestroy(e);
@@ -560,8 +566,8 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
     return set_err(LP_ERR_ARG, "lp_create: unsupported plane size %dx%d (supported: 32,64,96,128,192,256)",
                    g.nx, g.ny);
   if (g.ld != 2 * (g.nx / 2 + 1) || g.nz < 4) return set_err(LP_ERR_ARG, "lp_create: grid not initialised");
-  if (cfg->model != 1 && cfg->model != 5)
-    return set_err(LP_ERR_ARG, "lp_create: SGS model %d not implemented (1, 5)", cfg->model);
+  if (cfg->model < 1 || cfg->model > 5)
+    return set_err(LP_ERR_ARG, "lp_create: SGS model %d not implemented (1 to 5)", cfg->model);
   if (cfg->lbc_mom != LP_LBC_WALL && cfg->lbc_mom != LP_LBC_STRESS_FREE)
     return set_err(LP_ERR_ARG, "invalid lbc_mom");
   if (cfg->cs_count < 1) return set_err(LP_ERR_ARG, "cs_count must be >= 1");
@@ -672,6 +678,16 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
     CK(dalloc(&c->lhat, pl * pad16(c->ncut2)));
     for (auto &p : c->lcon) CK(dalloc(&p, fe * nv));
     for (auto &p : c->Fi) CK(dalloc((void **)&p, sizeof(double) * nv));
+  } else if (cfg->model >= 2) {
+    CK(dalloc(&c->dyn_in, (size_t)kDynIn * vol_b));
+    CK(dalloc(&c->dyn_bar, (size_t)kDynIn * vol_b));
+    if (cfg->model == 3) {
+      CK(dalloc(&c->dyn_hat, (size_t)kDynIn * vol_b));
+      CK(dalloc((void **)&c->dyn_sums, sizeof(double) * 10 * (size_t)g.nz));
+      CK(dalloc((void **)&c->dyn_beta, sizeof(double) * (size_t)g.nz));
+    }
+    if (cfg->model == 4)
+      for (int q = 0; q < 2; ++q) CK(dalloc((void **)&c->Fi[q], sizeof(double) * nv));
   }
   for (auto &p : c->Hs) CK(dalloc(&p, spec_b));
   CK(dalloc(&c->bw, (size_t)c->sp * g.ny * 2 * fe));
@@ -1095,7 +1111,7 @@ static int reset_lag_velocities(lp_ctx *c) {  // interpolag_Sdep.f90:367-369
 
 // interpolag_Sdep.f90:62-369 on its own: F_LM/MM/QN/NN <- interpolated averages
 static int do_interpolag(lp_ctx *c) {
-  if (!c->Fi[0]) return set_err(LP_ERR_ARG, "interpolag_Sdep needs model 5");
+  if (c->cfg.model != 5 || !c->Fi[3]) return set_err(LP_ERR_ARG, "interpolag_Sdep needs model 5");
   Geo g = geo(c);
   const lp_config &cf = c->cfg;
   FDISPATCH(c, T,
@@ -1207,6 +1223,83 @@ static int do_lagrange(lp_ctx *c, bool use_u = false) {
   return LP_OK;
 }
 
+// sgs_stag.f90:188-198, models 2-4 on the generic spectral path (lp_sgs_alt.hpp):
+// std_dynamic, scaledep_dynamic, lagrange_Ssim with interpolag_Ssim
+static int do_dynamic_alt(lp_ctx *c) {
+  const lp_config &cf = c->cfg;
+  const lp_grid &gg = c->g;
+  Geo g = geo(c);
+  const long nv = g.np * g.nz;
+  const size_t fe = esz_of(c->fdt);
+  const int model = cf.model;
+  const double delta = cf.filter_size * std::pow(gg.dx * gg.dy * gg.dz, 1.0 / 3.0);
+  const double cst = 2.0 * delta * delta;
+  auto vol = [&](void *base, int q) { return (void *)((char *)base + (size_t)q * nv * fe); };
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_dyn_inputs<T>, grid1(nv), dim3(256), 0, c->s, (const double *)c->f[LP_U],
+                                     (const double *)c->f[LP_V], (const double *)c->f[LP_W], (const T *)c->S[0],
+                                     (const T *)c->S[1], (const T *)c->S[2], (const T *)c->S[3],
+                                     (const T *)c->S[4], (const T *)c->S[5], (T *)c->dyn_in, g,
+                                     c->fdt == F32 ? 1 : 0));
+  HIPCHK(hipGetLastError());
+  // the test filters (G_test, G_test_test carry the 1/(nx ny) of the transform)
+  DynVol B{}, H{};
+  for (int q = 0; q < kDynIn; ++q) {
+    const void *src = (q >= 9 && q < 15) ? c->S[q - 9] : vol(c->dyn_in, q);
+    const bool hat = model == 3 && !(q >= 3 && q < 9);
+    const int m[2] = {M_G1, M_G2};
+    void *o[2] = {vol(c->dyn_bar, q), hat ? vol(c->dyn_hat, q) : nullptr};
+    RCHK(spectral_apply(c, src, gg.nz, 1.0, hat ? 2 : 1, m, o));
+    B.p[q] = vol(c->dyn_bar, q);
+    if (model == 3) H.p[q] = vol(c->dyn_hat, q);
+  }
+  if (model == 3) {  // Q_ij: the 2-delta velocities' products, 4-delta filtered (input slots 3-8)
+    FDISPATCH(c, T, hipLaunchKernelGGL(k_dyn_qprod<T>, grid1(nv), dim3(256), 0, c->s, (const T *)B.p[0],
+                                       (const T *)B.p[1], (const T *)B.p[2], (T *)vol(c->dyn_in, 3), nv));
+    HIPCHK(hipGetLastError());
+    for (int q = 3; q < 9; ++q) {
+      const int m[1] = {M_G2};
+      void *o[1] = {vol(c->dyn_hat, q)};
+      RCHK(spectral_apply(c, vol(c->dyn_in, q), gg.nz, 1.0, 1, m, o));
+    }
+    FDISPATCH(c, T, hipLaunchKernelGGL(k_dyn_sums<T>, dim3(gg.nz), dim3(1024), 0, c->s, B, H, g, c->dyn_sums));
+    hipLaunchKernelGGL(k_dyn_beta, grid1(gg.nz), dim3(256), 0, c->s, (const double *)c->dyn_sums, c->dyn_beta,
+                       gg.nz, delta, (double)gg.nx * gg.ny);
+    HIPCHK(hipGetLastError());
+  }
+  if (model != 4) {
+    FDISPATCH(c, T, hipLaunchKernelGGL(k_dyn_cs<T>, dim3(gg.nz), dim3(1024), 0, c->s, B,
+                                       (const double *)(model == 3 ? c->dyn_beta : nullptr), g, cst,
+                                       (T *)c->f[LP_CS_OPT2], (T *)c->f[LP_BETA]));
+    HIPCHK(hipGetLastError());
+    return LP_OK;
+  }
+  // model 4: interpolag_Ssim, then the relaxation of F_LM, F_MM
+  const double cs = (double)cf.cs_count;
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_interpolag_ssim<T>, grid1(nv), dim3(256), 0, c->s,
+                                     (const double *)c->f[LP_F_LM], (const double *)c->f[LP_F_MM], c->Fi[0],
+                                     c->Fi[1], (const T *)c->f[LP_U_LAG], (const T *)c->f[LP_V_LAG],
+                                     (const T *)c->f[LP_W_LAG], g, cs, cf.dt * cs / gg.dx, cf.dt * cs / gg.dy,
+                                     cf.dt * cs / gg.dz));
+  HIPCHK(hipGetLastError());
+  for (int q = 0; q < 2; ++q) {
+    const int fid = q == 0 ? LP_F_LM : LP_F_MM;
+    void *t = c->f[fid];
+    c->f[fid] = c->Fi[q];
+    c->Fi[q] = (double *)t;
+  }
+  RCHK(reset_lag_velocities(c));
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_dyn_contract<T>, grid1(nv), dim3(256), 0, c->s, B, g, cst, (T *)c->tmp[0],
+                                     (T *)c->tmp[1]));
+  const bool init = cf.inilag && !c->F_LM_MM_init && (c->jt == cf.cs_count || c->jt == cf.dyn_init);
+  FDISPATCH(c, T, hipLaunchKernelGGL(k_ssim_relax<T>, grid1(nv), dim3(256), 0, c->s, (const T *)c->tmp[0],
+                                     (const T *)c->tmp[1], (double *)c->f[LP_F_LM], (double *)c->f[LP_F_MM],
+                                     (T *)c->f[LP_CS_OPT2], (T *)c->f[LP_BETA], g, init ? 1 : 0,
+                                     cf.fp32_casts, cf.opftime * delta, cf.dt * cs));
+  HIPCHK(hipGetLastError());
+  if (init) c->F_LM_MM_init = 1;
+  return LP_OK;
+}
+
 static int do_sgs_stag(lp_ctx *c) {
   const lp_config &cf = c->cfg;
   const lp_grid &gg = c->g;
@@ -1248,6 +1341,21 @@ static int do_sgs_stag(lp_ctx *c) {
       } else if (lag_now) {
         RCHK(do_lagrange(c));  // sgs_stag.f90:200-201
       }
+    }
+  } else if (cf.model >= 2) {  // sgs_stag.f90:130-134, 178-198
+    if (cf.model == 4) {
+      FDISPATCH(c, T, hipLaunchKernelGGL(k_lag_acc<T>, grid1(nv), dim3(256), 0, c->s, (T *)c->f[LP_U_LAG],
+                                         (T *)c->f[LP_V_LAG], (T *)c->f[LP_W_LAG], (const double *)c->f[LP_U],
+                                         (const double *)c->f[LP_V], (const double *)c->f[LP_W], nv));
+      HIPCHK(hipGetLastError());
+      c->lag_vel_zero = false;
+    }
+    if (c->jt == 1 && cf.inilag) {
+      FDISPATCH(c, T, hipLaunchKernelGGL(k_fill<T>, grid1(nv), dim3(256), 0, c->s, (T *)c->f[LP_CS_OPT2], nv,
+                                         (T)cf.cs2_init));
+      HIPCHK(hipGetLastError());
+    } else if ((c->jt >= cf.dyn_init || cf.initu) && (c->jt % cf.cs_count) == 0) {
+      RCHK(do_dynamic_alt(c));
     }
   } else {  // model 1: Cs_opt2 = Co^2 (sgs_stag.f90:148)
     FDISPATCH(c, T, hipLaunchKernelGGL(k_fill<T>, grid1(nv), dim3(256), 0, c->s, (T *)c->f[LP_CS_OPT2], nv,
