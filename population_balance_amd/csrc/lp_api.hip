@@ -163,6 +163,9 @@ struct lp_ctx {
   void *Hs[3] = {};           // pressure RHS spectra
   void *bw = nullptr, *tw = nullptr;  // wall / lid divergence spectra
   double *gam = nullptr;      // Thomas scratch (nz+1) x lh x ny
+  // the pressure systems' elimination factors gam(j), 1/bet(j) ((nz+1) x lh x ny), fixed by
+  // the grid: computed once in lp_create (k_press_factors)
+  double *pgam = nullptr, *ibet = nullptr;
   void *pspec[3] = {};        // binary64 spectra for the p inverse
   // d_err[0]: zero pivot of a Thomas solve; d_err[1]: non-finite velocity written by a
   // step; d_err[2]: lp_check_finite's scratch flag
@@ -534,7 +537,7 @@ extern "C" int lp_destroy(lp_ctx *c) {
   for (auto p : c->lcon) fr(p);
   for (auto p : c->Fi) fr(p);
   for (auto p : c->Hs) fr(p);
-  fr(c->bw); fr(c->tw); fr(c->gam); fr(c->phat);
+  fr(c->bw); fr(c->tw); fr(c->gam); fr(c->pgam); fr(c->ibet); fr(c->phat);
   for (auto p : c->pspec) fr(p);
   fr(c->d_err); fr(c->d_part); fr(c->d_stats); fr(c->d_espec); fr(c->ushift); fr(c->nyq); fr(c->adj_block); fr(c->adj_drop); fr(c->tavg); fr(c->d_pvol);
   fr(c->d_obuk); fr(c->tshift); fr(c->d_tbar);
@@ -693,6 +696,16 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
   CK(dalloc(&c->bw, (size_t)c->sp * g.ny * 2 * fe));
   CK(dalloc(&c->tw, (size_t)c->sp * g.ny * 2 * fe));
   CK(dalloc((void **)&c->gam, sizeof(double) * (size_t)g.lh * g.ny * (g.nz + 1)));
+  CK(dalloc((void **)&c->pgam, sizeof(double) * (size_t)g.lh * g.ny * (g.nz + 1)));
+  CK(dalloc((void **)&c->ibet, sizeof(double) * (size_t)g.lh * g.ny * (g.nz + 1)));
+  {
+    const long nmodes = (long)g.lh * g.ny;
+    hipLaunchKernelGGL(k_press_factors, grid1(nmodes), dim3(256), 0, c->s, (const double *)c->kx,
+                       (const double *)c->ky, c->pgam, c->ibet, g.lh, g.ny, g.nz, g.dz);
+    CK(hipGetLastError() == hipSuccess && hipStreamSynchronize(c->s) == hipSuccess
+           ? LP_OK
+           : set_err(LP_ERR_HIP, "lp_create: pressure factors"));
+  }
   for (auto &p : c->pspec) CK(dalloc(&p, sizeof(double) * 2 * (size_t)c->sp * g.ny * (g.nz + 1)));
   CK(dalloc(&c->phat, sizeof(double) * 2 * (size_t)c->sp * g.ny * (g.nz + 1)));
   CK(dalloc((void **)&c->d_err, sizeof(int) * 4));
@@ -1551,13 +1564,13 @@ static int do_press(lp_ctx *c) {
   if (c->fdt == F64)
     hipLaunchKernelGGL(k_press_tridiag<double>, grid_modes, dim3(64), 0, c->s, (const cx<double> *)c->Hs[0],
                        (const cx<double> *)c->Hs[1], (const cx<double> *)c->Hs[2], (const cx<double> *)c->bw,
-                       (const cx<double> *)c->tw, c->kx, c->ky, ph, c->gam, gg.lh, gg.ny, gg.nz, c->sp, gg.dz,
-                       c->d_err);
+                       (const cx<double> *)c->tw, c->kx, c->ky, ph, c->gam, c->ibet, gg.lh, gg.ny, gg.nz, c->sp,
+                       gg.dz, c->d_err);
   else
     hipLaunchKernelGGL(k_press_tridiag<float>, grid_modes, dim3(64), 0, c->s, (const cx<float> *)c->Hs[0],
                        (const cx<float> *)c->Hs[1], (const cx<float> *)c->Hs[2], (const cx<float> *)c->bw,
-                       (const cx<float> *)c->tw, c->kx, c->ky, ph, c->gam, gg.lh, gg.ny, gg.nz, c->sp, gg.dz,
-                       c->d_err);
+                       (const cx<float> *)c->tw, c->kx, c->ky, ph, c->pgam, c->ibet, gg.lh, gg.ny, gg.nz, c->sp,
+                       gg.dz, c->d_err);
   HIPCHK(hipGetLastError());
   // H_x, H_y, H_z read once, p_hat written once (binary64 complex)
   prof_end(c, LP_K_PRESS_TRIDIAG,

@@ -643,8 +643,11 @@ __global__ __launch_bounds__(64) void k_press_tridiag(const cx<TH> *__restrict__
                                 const cx<TH> *__restrict__ Hz, const cx<TH> *__restrict__ bw,
                                 const cx<TH> *__restrict__ tw_, const double *__restrict__ kxt,
                                 const double *__restrict__ kyt, cx<double> *__restrict__ ph,
-                                double *__restrict__ gam, int lh, int ny, int nz, long pitch, double dz,
-                                int *__restrict__ err) {
+                                double *__restrict__ gam_w, const double *__restrict__ ibet, int lh, int ny,
+                                int nz, long pitch, double dz, int *__restrict__ err) {
+  // gam_w: the precomputed gam(j) (binary32 contexts, read only) or the scratch the
+  // forward sweep fills (binary64 contexts)
+  const double *gam = gam_w;
   const long m = gtid();  // mode (kx, ky) = (m % lh, m / lh)
   const long nh = (long)lh * ny;
   if (m >= nh) return;
@@ -676,13 +679,24 @@ __global__ __launch_bounds__(64) void k_press_tridiag(const cx<TH> *__restrict__
   const double kx = kxt[m], ky = kyt[m];
   const double idz2 = 1.0 / (dz * dz);
   const double bi = -(kx * kx + ky * ky + 2.0 / (dz * dz));
-  // row 0 (wall): b = -1, c = 1, r = -dz * bottomw
-  double bet = -1.0;
+  // row 0 (wall): b = -1, c = 1, r = -dz * bottomw.  The elimination factors gam(j) and
+  // 1/bet(j) depend on the mode and the grid only.  Binary32 contexts take them from the
+  // tables of k_press_factors (once per context): the forward sweep is then
+  // u(j) = (r(j) - a u(j-1)) * (1/bet(j)), without a division in its chain.  Binary64
+  // contexts form them on the way with the reference's divisions (tridag_array.f90:60-80),
+  // so that their rounding is the reference's.
+  constexpr bool kTab = sizeof(TH) == 4;
+  double bet = -1.0, cprev = 1.0;
   cx<TH> b0 = bw[o];
   cx<double> r0 = mk<double>(-dz * (double)b0.x, -dz * (double)b0.y);
-  cx<double> uprev = mk<double>(r0.x / bet, r0.y / bet);
+  cx<double> uprev;
+  if constexpr (kTab) {
+    const double ib0 = ibet[m];
+    uprev = mk<double>(r0.x * ib0, r0.y * ib0);
+  } else {
+    uprev = mk<double>(r0.x / bet, r0.y / bet);
+  }
   ph[o] = uprev;
-  double cprev = 1.0;
   bool bad = false;
   // The right-hand sides and the back-substitution inputs do not depend on the
   // recurrence: they are loaded kCh levels at a time, and the loads of the next chunk are
@@ -692,6 +706,7 @@ __global__ __launch_bounds__(64) void k_press_tridiag(const cx<TH> *__restrict__
   constexpr int kCh = sizeof(TH) == 4 ? 16 : 8;
   struct HChunk {
     cx<TH> x[kCh], y[kCh], z[kCh];
+    double ib[kCh];
   };
   auto load_h = [&](int j0, HChunk &c) {
 #pragma unroll
@@ -701,6 +716,7 @@ __global__ __launch_bounds__(64) void k_press_tridiag(const cx<TH> *__restrict__
       c.x[q] = Hx[(long)(lx - 1) * ps + o];
       c.y[q] = Hy[(long)(lx - 1) * ps + o];
       c.z[q] = Hz[(long)(lz - 1) * ps + o];
+      if constexpr (kTab) c.ib[q] = ibet[(long)(j <= nz ? j : nz) * nh + m];
     }
   };
   cx<double> hz_lo = ld_h(Hz, 1);  // Hz(j) for the next row j
@@ -727,11 +743,18 @@ __global__ __launch_bounds__(64) void k_press_tridiag(const cx<TH> *__restrict__
         a = -1.0; b = 1.0; cc = 0.0;
         r = mk<double>(-(double)t.x * dz, -(double)t.y * dz);
       }
-      double gm = cprev / bet;
-      gam[(long)j * nh + m] = gm;
-      bet = b - a * gm;
-      if (bet == 0.0) bad = true;
-      cx<double> uj = mk<double>((r.x - a * uprev.x) / bet, (r.y - a * uprev.y) / bet);
+      cx<double> uj;
+      if constexpr (kTab) {
+        const double ib = c.ib[q];
+        if (isinf(ib)) bad = true;  // bet(j) == 0
+        uj = mk<double>((r.x - a * uprev.x) * ib, (r.y - a * uprev.y) * ib);
+      } else {
+        double gm = cprev / bet;
+        gam_w[(long)j * nh + m] = gm;
+        bet = b - a * gm;
+        if (bet == 0.0) bad = true;
+        uj = mk<double>((r.x - a * uprev.x) / bet, (r.y - a * uprev.y) / bet);
+      }
       ph[(long)j * ps + o] = uj;
       uprev = uj;
       cprev = cc;
@@ -782,6 +805,34 @@ __global__ __launch_bounds__(64) void k_press_tridiag(const cx<TH> *__restrict__
     }
   }
   if (bad) atomicExch(err, 1);
+}
+
+// press_stag_array.f90:149-329 / tridag_array.f90:30-155: the elimination factors of each
+// mode's system (a = c = 1/dz^2, b = -(kx^2 + ky^2 + 2/dz^2) inside; the wall row b = -1,
+// c = 1; the lid row a = -1, b = 1), which depend on the grid only: gam(j) = c(j-1)/bet(j-1)
+// and 1/bet(j), bet(j) = b(j) - a(j) gam(j), in [(nz+1)][lh*ny].  Modes the solve skips
+// are left alone.
+static __global__ void k_press_factors(const double *__restrict__ kxt, const double *__restrict__ kyt,
+                                       double *__restrict__ gam, double *__restrict__ ibet, int lh, int ny, int nz,
+                                       double dz) {
+  const long m = gtid();
+  const long nh = (long)lh * ny;
+  if (m >= nh) return;
+  const int jx = (int)(m % lh), jy = (int)(m / lh);
+  if (jy == ny / 2 || jx == lh - 1 || m == 0) return;
+  const double kx = kxt[m], ky = kyt[m];
+  const double idz2 = 1.0 / (dz * dz);
+  const double bi = -(kx * kx + ky * ky + 2.0 / (dz * dz));
+  double bet = -1.0, cprev = 1.0;
+  ibet[m] = 1.0 / bet;
+  for (int j = 1; j <= nz; ++j) {
+    const double a = j < nz ? idz2 : -1.0, b = j < nz ? bi : 1.0, cc = j < nz ? idz2 : 0.0;
+    const double gm = cprev / bet;
+    gam[(long)j * nh + m] = gm;
+    bet = b - a * gm;
+    ibet[(long)j * nh + m] = 1.0 / bet;
+    cprev = cc;
+  }
 }
 
 // tridag_array.f90:30-155 with explicit a, b, c, r (all binary64); modes skipped as there
