@@ -136,6 +136,7 @@ struct lp_ctx {
   // shifted by ([2][nz]); the convective pass adds them back to the kept spectra
   double *ushift = nullptr;
   double *nyq = nullptr;    // filt_da of binary64 velocities: A [nz][ny], B [nz][nx], C [nz]
+  double *nyq_part = nullptr;  // their partial column sums [nz][kNyqSlices][nx]
   double *adj_drop = nullptr;  // droplet adjoint scratch: mu, ac, awi volumes + aui, avi faces
   double *tavg = nullptr;      // running time averages (tavg_nq binary64 volumes)
   int tavg_nq = 0;
@@ -539,7 +540,7 @@ extern "C" int lp_destroy(lp_ctx *c) {
   for (auto p : c->Hs) fr(p);
   fr(c->bw); fr(c->tw); fr(c->gam); fr(c->pgam); fr(c->ibet); fr(c->phat);
   for (auto p : c->pspec) fr(p);
-  fr(c->d_err); fr(c->d_part); fr(c->d_stats); fr(c->d_espec); fr(c->ushift); fr(c->nyq); fr(c->adj_block); fr(c->adj_drop); fr(c->tavg); fr(c->d_pvol);
+  fr(c->d_err); fr(c->d_part); fr(c->d_stats); fr(c->d_espec); fr(c->ushift); fr(c->nyq); fr(c->nyq_part); fr(c->adj_block); fr(c->adj_drop); fr(c->tavg); fr(c->d_pvol);
   fr(c->d_obuk); fr(c->tshift); fr(c->d_tbar);
   for (auto p : c->tbig) fr(p);
   fr(c->dyn_in); fr(c->dyn_bar); fr(c->dyn_hat); fr(c->dyn_sums); fr(c->dyn_beta);
@@ -711,6 +712,7 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
   CK(dalloc((void **)&c->d_err, sizeof(int) * 4));
   CK(dalloc((void **)&c->ushift, sizeof(double) * 2 * (size_t)g.nz));
   CK(dalloc((void **)&c->nyq, sizeof(double) * (size_t)g.nz * (g.ny + g.nx + 1)));
+  CK(dalloc((void **)&c->nyq_part, sizeof(double) * (size_t)g.nz * kNyqSlices * g.nx));
   CK(dalloc((void **)&c->d_stats, sizeof(double) * 10 * (size_t)(g.nz - 1)));
   if (cfg->scal.s_flag) {
     CK(dalloc((void **)&c->d_obuk, sizeof(double) * 8));
@@ -927,7 +929,10 @@ static int do_filt_da(lp_ctx *c, void *f, void *dfdx, void *dfdy, int nlev, void
     Geo g = geo(c);
     double *A = c->nyq, *B = A + (size_t)c->g.nz * g.ny, *Cc = B + (size_t)c->g.nz * g.nx;
     if (g.nx > kNyqMax || g.ny > kNyqMax) return set_err(LP_ERR_ARG, "filt_da: nx, ny above %d", kNyqMax);
-    hipLaunchKernelGGL(k_nyq_sums, dim3((unsigned)nlev), dim3(1024), 0, c->s, (const double *)f, A, B, Cc, g);
+    hipLaunchKernelGGL(k_nyq_part, dim3((unsigned)(nlev * kNyqSlices)), dim3(1024), 0, c->s, (const double *)f, A,
+                       c->nyq_part, g);
+    hipLaunchKernelGGL(k_nyq_fin, dim3((unsigned)nlev), dim3(256), 0, c->s, (const double *)A,
+                       (const double *)c->nyq_part, B, Cc, g);
     hipLaunchKernelGGL(k_nyq_remove, grid1(g.np * nlev), dim3(256), 0, c->s, (double *)f, (const double *)A,
                        (const double *)B, (const double *)Cc, g, nlev);
     HIPCHK(hipGetLastError());

@@ -53,22 +53,27 @@ __global__ void k_add_dc(cx<T> *__restrict__ spec, long pitch, int ny, int nplan
 //   (-1)^x A(y) + (-1)^y B(x) - (-1)^(x+y) C,
 //   A(y) = sum_x (-1)^x f(x, y) / nx,  B(x) = sum_y (-1)^y f(x, y) / ny,
 //   C = sum_y (-1)^y A(y) / ny  (the corner mode, counted in both sums).
-// k_nyq_sums: one block per level reads it once: each of 16 waves takes every 16th row,
-// forms A(y) with a wave reduction and keeps its rows' alternating column sums; the 16
-// partial column sums are added in LDS (fixed order: the result does not depend on timing).
 constexpr int kNyqMax = 384;  // largest nx, ny of the transforms
-static __global__ __launch_bounds__(1024) void k_nyq_sums(const double *__restrict__ f, double *__restrict__ A,
-                                                          double *__restrict__ B, double *__restrict__ Cc, Geo g) {
+
+// The sums in two passes that read f once: k_nyq_part gives each of kNyqSlices workgroups
+// of a level a contiguous slice of its rows (16 waves, every 16th row of the slice), which
+// yields A(y) of its rows (a wave reduction) and its partial alternating column sums
+// P[k][s][x] (the 16 waves' partials added in LDS); k_nyq_fin adds the slices' partials in
+// slice order for B and forms C from A.  Every sum has a fixed order: the result does not
+// depend on timing.
+constexpr int kNyqSlices = 4;
+static __global__ __launch_bounds__(1024) void k_nyq_part(const double *__restrict__ f, double *__restrict__ A,
+                                                          double *__restrict__ P, Geo g) {
   __shared__ double colp[16][kNyqMax];
-  __shared__ double arow[kNyqMax];
   const int lane = threadIdx.x % 64, grp = threadIdx.x / 64;
-  const int k = blockIdx.x;
+  const int k = blockIdx.x / kNyqSlices, sl = blockIdx.x % kNyqSlices;
+  const int y0 = sl * g.ny / kNyqSlices, y1 = (sl + 1) * g.ny / kNyqSlices;
   const double *p = f + (long)k * g.np;
   constexpr int NQ = kNyqMax / 64;
   double col[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) col[q] = 0.0;
-  for (int y = grp; y < g.ny; y += 16) {
+  for (int y = y0 + grp; y < y1; y += 16) {
     const double *r = p + (long)y * g.ld;
     const bool odd_y = y & 1;
     double ra = 0.0;
@@ -83,10 +88,7 @@ static __global__ __launch_bounds__(1024) void k_nyq_sums(const double *__restri
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) ra += __shfl_xor(ra, o, 64);
-    if (lane == 0) {
-      A[(long)k * g.ny + y] = ra / g.nx;
-      arow[y] = ra / g.nx;
-    }
+    if (lane == 0) A[(long)k * g.ny + y] = ra / g.nx;
   }
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
@@ -97,11 +99,24 @@ static __global__ __launch_bounds__(1024) void k_nyq_sums(const double *__restri
   for (int x = threadIdx.x; x < g.nx; x += blockDim.x) {
     double s = 0.0;
     for (int r = 0; r < 16; ++r) s += colp[r][x];
+    P[((long)k * kNyqSlices + sl) * g.nx + x] = s;
+  }
+}
+static __global__ __launch_bounds__(256) void k_nyq_fin(const double *__restrict__ A, const double *__restrict__ P,
+                                                        double *__restrict__ B, double *__restrict__ Cc, Geo g) {
+  const int k = blockIdx.x;
+  for (int x = threadIdx.x; x < g.nx; x += blockDim.x) {
+    double s = 0.0;
+    for (int sl = 0; sl < kNyqSlices; ++sl) s += P[((long)k * kNyqSlices + sl) * g.nx + x];
     B[(long)k * g.nx + x] = s / g.ny;
   }
-  if (grp == 0) {
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
     double c = 0.0;
-    for (int y = lane; y < g.ny; y += 64) c += (y & 1) ? -arow[y] : arow[y];
+    for (int y = lane; y < g.ny; y += 64) {
+      const double a = A[(long)k * g.ny + y];
+      c += (y & 1) ? -a : a;
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
     if (lane == 0) Cc[k] = c / g.ny;
