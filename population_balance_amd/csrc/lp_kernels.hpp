@@ -366,10 +366,9 @@ __global__ void k_interpolag(const double *__restrict__ F0, const double *__rest
   I0[i] = v[0]; I1[i] = v[1]; I2[i] = v[2]; I3[i] = v[3];
 }
 
-// x^p for the relaxation time scale; the reference's p = -1/8 as three square roots
-__device__ __forceinline__ double pow_coeff(double x, double p) {
-  return p == -0.125 ? 1.0 / sqrt(sqrt(sqrt(x))) : pow(x, p);
-}
+// x^(-1/8) for the relaxation time scale (lagrange_Sdep.f90:85 powcoeff = -1/8, a
+// constant) as three square roots
+__device__ __forceinline__ double pow_m8(double x) { return 1.0 / sqrt(sqrt(sqrt(x))); }
 
 __device__ __forceinline__ double r4max(double a, float b) {  // (double) max(real(a), b)
   return (double)fmaxf((float)a, b);
@@ -378,6 +377,8 @@ __device__ __forceinline__ double r4max(double a, float b) {  // (double) max(re
 // lagrange_Sdep.f90:157-434 after the filters: Germano contractions, Lagrangian
 // relaxation (binary64) with the reference's REAL(4) conversions when casts != 0,
 // scale-dependence beta, Cs^2.  bar[q] = 2-delta filtered field q, hat[q] = 4-delta.
+// (powcoeff and bexp are the reference's constants -1/8 and 1, checked by the launcher:
+// relax_point uses them as such)
 struct LagParams {
   double cst, tf1_2, tf2_2, opftdelta, powcoeff, lagran_dt, bexp, tiny, bclip_min;
   int casts, init_lm, init_qn;
@@ -416,11 +417,11 @@ __device__ __forceinline__ void relax_point(double LM, double MM, double QN, dou
   double Tn, epsi;
   if (P.casts) {
     Tn = (double)fmaxf((float)(flm * fmm), 1e-24f);
-    Tn = P.opftdelta * pow_coeff(Tn, P.powcoeff);
+    Tn = P.opftdelta * pow_m8(Tn);
     Tn = (double)fmaxf(1e-24f, (float)Tn);
   } else {
     Tn = fmax(flm * fmm, tiny);
-    Tn = P.opftdelta * pow_coeff(Tn, P.powcoeff);
+    Tn = P.opftdelta * pow_m8(Tn);
     Tn = fmax(tiny, Tn);
   }
   epsi = P.lagran_dt / (Tn + P.lagran_dt);  // dum / (1 + dum), dum = lagran_dt / Tn
@@ -433,11 +434,11 @@ __device__ __forceinline__ void relax_point(double LM, double MM, double QN, dou
   if (P.init_qn) { fnn = NN; fqn = 0.03 * NN; if (padc) { fnn = 1.0; fqn = 1.0; } }
   if (P.casts) {
     Tn = (double)fmaxf((float)(fqn * fnn), 1e-24f);
-    Tn = P.opftdelta * pow_coeff(Tn, P.powcoeff);
+    Tn = P.opftdelta * pow_m8(Tn);
     Tn = (double)fmaxf(1e-24f, (float)Tn);
   } else {
     Tn = fmax(fqn * fnn, tiny);
-    Tn = P.opftdelta * pow_coeff(Tn, P.powcoeff);
+    Tn = P.opftdelta * pow_m8(Tn);
     Tn = fmax(tiny, Tn);
   }
   epsi = P.lagran_dt / (Tn + P.lagran_dt);  // dum / (1 + dum), dum = lagran_dt / Tn
@@ -447,9 +448,9 @@ __device__ __forceinline__ void relax_point(double LM, double MM, double QN, dou
   double cs4d = fqn / fnn;
   if (padc) cs4d = tiny;
   cs4d = P.casts ? (double)(float)fmax(tiny, (double)(float)cs4d) : fmax(tiny, cs4d);
-  // the exponent log(tf1)/(log(tf2)-log(tf1)) is exactly 1 for the reference's widths 2, 4,
-  // and pow(x, 1) == x
-  double b = P.bexp == 1.0 ? cs4d / cs2d : pow(cs4d / cs2d, P.bexp);
+  // the exponent log(tf1)/(log(tf2)-log(tf1)) (lagrange_Sdep.f90:410) is exactly 1 for the
+  // reference's widths tf1 = 2, tf2 = 4 (:77-78), and pow(x, 1) == x
+  double b = cs4d / cs2d;
   if (top) b = 1.0;
   double bclip = P.casts ? (double)fmaxf((float)b, (float)P.bclip_min) : fmax(b, P.bclip_min);
   double csv = cs2d / bclip;

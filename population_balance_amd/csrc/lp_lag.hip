@@ -9,7 +9,8 @@
 //   k_lag_c2r_contract  transforms the 21 fields of one width back along x for one
 //                (y, z) row in LDS and writes their two Germano contractions per point;
 //   k_lag_relax  interpolates the old averages at the backtraced point and relaxes them
-//                with the four contractions, one thread per point.
+//                with the four contractions, one thread per point, marching a tile
+//                through a run of levels (other workgroups relax the padding columns).
 #include "lp_fft4.hpp"
 #include "lp_lag.hpp"
 
@@ -247,132 +248,225 @@ __global__ __launch_bounds__(256) void k_lag_c2r_contract(const cx<T> *__restric
   }
 }
 
-// k_lag_relax works on tiles of kRx x kRy points of one level.  The old averages it
-// gathers (4 fields, the point's x/y neighbours on both sides, levels k-1..k+1) are
-// staged in LDS first with coalesced loads: slot (sx, sy, sk) holds the value at
-// x = wrap(x0-1+sx), y = wrap(y0-1+sy), level clamp(k-1+sk), so the backtrace corner
-// x + addx of local point lx is slot lx+1+addx, also across the periodic seam and in a
-// partial last tile.
+// k_lag_relax works on tiles of kRx x kRy points and marches each tile through a run of
+// levels (2.5D).  The old averages it gathers (4 fields, the point's x/y neighbours on
+// both sides, levels k-1..k+1) sit in a ring of three level slots in LDS, each level
+// loaded once per block with coalesced loads: slot (sx, sy) of a level holds the value at
+// x = wrap(x0-1+sx), y = wrap(y0-1+sy), so the backtrace corner x + addx of local point
+// lx is slot lx+1+addx, also across the periodic seam and in a partial last tile.
+// Virtual level m (k-1 <= m <= k+1 while level k is relaxed; m = -1 and m = nz occur)
+// lives in ring slot (m+3) % 3 and holds level clamp(m).  While level k is relaxed, the
+// loads of level k+2 and of the point values of level k+1 are in flight.
 constexpr int kRx = 64, kRy = 4;  // kRx = one wavefront, kRy wavefronts per block
 constexpr int kHx = kRx + 2, kHy = kRy + 2;
+constexpr int kLagRows = 4 * kHy;  // staged rows of one level: 4 fields x kHy
+constexpr int kRz = 16;            // levels per block, at most (the runs are balanced)
 
-// lagrange_Sdep.f90:327-434 with interpolag_Sdep.f90:62-365, one thread per point: the
-// old averages are gathered at the backtraced point (from the LDS tile), relaxed with the
-// contractions and written to Fnew.  Blocks take the tiles of a level in XCD-aware order
-// (workgroup ids are dealt round-robin to the 8 XCDs; XCD j takes tiles [j*cg, (j+1)*cg)
-// of every level, level after level, so a tile's neighbouring levels are in its L2).
-// The tile of the last x column also relaxes the two padding columns of its rows, which
-// interpolag_Sdep.f90 leaves alone (their old values enter the relaxation).  The
-// interpolation is interp_point's, term for term.
+// The padding columns x = nx .. ld-1, which interpolag_Sdep.f90 leaves alone: point t of
+// the (ld - nx) ny nz padding points is relaxed from its old values (no interpolation).
+template <typename T>
+__device__ __forceinline__ void relax_pad_point(long t, const T *__restrict__ lm, const T *__restrict__ mm,
+                                                const T *__restrict__ qn, const T *__restrict__ nn,
+                                                const F4c &Fold, const F4 &Fnew, T *__restrict__ Cs,
+                                                T *__restrict__ Beta, const Geo &g, const LagParams &P) {
+  const int npad = g.ld - g.nx;
+  const int px = g.nx + (int)(t % npad);
+  const long row = t / npad;  // k * ny + y
+  const int k = (int)(row / g.ny);
+  const long i = row * g.ld + px;
+  double fi[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) fi[q] = Fold.p[q][i];
+  T cs, b;
+  relax_point<T>((double)lm[i], (double)mm[i], (double)qn[i], (double)nn[i], fi, px >= g.ld - 2, k == g.nz - 1,
+                 P, cs, b);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) Fnew.p[q][i] = fi[q];
+  Cs[i] = cs;
+  Beta[i] = b;
+}
+
+// lagrange_Sdep.f90:327-434 with interpolag_Sdep.f90:62-365, one thread per point and
+// level: the old averages are gathered at the backtraced point (from the LDS ring),
+// relaxed with the contractions and written to Fnew.  Blocks are dealt to the XCDs
+// round-robin by workgroup id; XCD j takes tiles [j*cg, (j+1)*cg) of every level run, so
+// the halo columns and rows a tile shares with its neighbours are read from its L2.
+// The first npb workgroups (a multiple of 8, so that the tiles keep their XCDs) relax the
+// padding columns instead, 256 points each, beside the tiles: a separate branch, so that
+// the tiles' relaxation loop holds no loads of its own (a wait for those would also wait
+// for the next level's loads in flight).  The interpolation is interp_point's, term for
+// term.
 template <typename T, bool UNIT, typename TV = T>
 __global__ __launch_bounds__(256) void k_lag_relax(const T *__restrict__ lm, const T *__restrict__ mm,
                                                    const T *__restrict__ qn, const T *__restrict__ nn,
                                                    F4c Fold, F4 Fnew, const TV *__restrict__ ul,
                                                    const TV *__restrict__ vl, const TV *__restrict__ wl,
                                                    T *__restrict__ Cs, T *__restrict__ Beta, Geo g, LagParams P,
-                                                   LagInterp I, int ntx, int cg) {
-  __shared__ double sF[4][3][kHy][kHx];
+                                                   LagInterp I, int ntx, int cg, int nkc, int npb) {
+  __shared__ double sF[3][kLagRows][kHx];  // [ring slot][field * kHy + sy][sx]
+  if ((int)blockIdx.x < npb) {  // uniform over the block
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    if (t < (long)(g.ld - g.nx) * g.ny * g.nz) relax_pad_point<T>(t, lm, mm, qn, nn, Fold, Fnew, Cs, Beta, g, P);
+    return;
+  }
   const int ntile = ntx * (g.ny / kRy);
-  const int j = (int)(blockIdx.x & 7u), s = (int)(blockIdx.x >> 3);
-  const int k = s / cg, tt = j * cg + s % cg;
-  if (k >= g.nz || tt >= ntile) return;  // uniform over the block
+  const int wg = (int)blockIdx.x - npb;
+  const int j = wg & 7, s = wg >> 3;
+  const int kc = s / cg, tt = j * cg + s % cg;
+  if (kc >= nkc || tt >= ntile) return;  // uniform over the block
+  const int kb = (int)((long)kc * g.nz / nkc), ke = (int)((long)(kc + 1) * g.nz / nkc);
   const int x0 = (tt % ntx) * kRx, y0 = (tt / ntx) * kRy;
-  const long pk = (long)k * g.np;
-  // stage the tile: 4 fields x 3 levels x kHy rows of kHx slots.  Wavefront wv loads rows
-  // wv, wv+4, ... (row index uniform over the wavefront), lane l slot l; slots 64, 65 of
-  // all rows follow.  x0 - 1 <= x <= x0 + 64 < 3 nx; the slots past a partial tile only
-  // need to stay in bounds (no point reads them).
+  const int lane = threadIdx.x % 64, wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
   auto wrapx = [&](int xg) {
     xg = xg < 0 ? xg + g.nx : xg;
     xg = xg >= g.nx ? xg - g.nx : xg;
     return xg >= g.nx ? xg - g.nx : xg;
   };
-  auto row_src = [&](int r) {  // r = (q*3 + sk)*kHy + sy
-    const int q = r / (3 * kHy), sk = (r / kHy) % 3, sy = r % kHy;
-    int yg = y0 - 1 + sy, kg = k - 1 + sk;
+  // staging of one level: wavefront wv loads rows wv, wv+4, ... (row index uniform over
+  // the wavefront), lane l slot l; slots 64, 65 of all rows by threads 0 .. 2 kLagRows - 1.
+  // x0 - 1 <= x <= x0 + 64 < 3 nx; the slots past a partial tile only need to stay in
+  // bounds (no point reads them).
+  auto row_off = [&](int r) {  // r = q*kHy + sy: offset of the row within a level
+    int yg = y0 - 1 + r % kHy;
     yg = yg < 0 ? yg + g.ny : (yg >= g.ny ? yg - g.ny : yg);
-    kg = kg < 0 ? 0 : (kg > g.nz - 1 ? g.nz - 1 : kg);
-    return Fold.p[q] + (long)kg * g.np + (long)yg * g.ld;
+    return (long)yg * g.ld;
   };
-  constexpr int kRows = 4 * 3 * kHy;
-  double *sflat = &sF[0][0][0][0];
-  {
-    const int lane = threadIdx.x % 64, wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
-    const int xs = wrapx(x0 - 1 + lane);
+  const int xs = wrapx(x0 - 1 + lane);
+  const bool extra = threadIdx.x < 2 * kLagRows;
+  const int xr = extra ? (int)(threadIdx.x >> 1) : 0, xsx = 64 + (int)(threadIdx.x & 1);
+  // field pointers by selection (an indexed read of the argument array would be a memory
+  // load, waited for before the loads it addresses)
+  auto fold_q = [&](int q) -> const double * {
+    return q == 0 ? Fold.p[0] : (q == 1 ? Fold.p[1] : (q == 2 ? Fold.p[2] : Fold.p[3]));
+  };
+  constexpr int kPer = kLagRows / 4;
+  const double *rowp[kPer];  // the rows this wavefront stages, within level 0
 #pragma unroll
-    for (int it = 0; it < kRows / 4; ++it) {
-      const int r = 4 * it + wv;
-      sflat[r * kHx + lane] = row_src(r)[xs];
-    }
-    if (threadIdx.x < 2 * kRows) {
-      const int r = threadIdx.x >> 1, sx = 64 + (threadIdx.x & 1);
-      sflat[r * kHx + sx] = row_src(r)[wrapx(x0 - 1 + sx)];
-    }
+  for (int it = 0; it < kPer; ++it) {
+    const int r = 4 * it + wv;
+    rowp[it] = fold_q(r / kHy) + row_off(r);
   }
-  __syncthreads();
+  const double *xrow = fold_q(xr / kHy) + row_off(xr) + wrapx(x0 - 1 + xsx);
+  auto load_level = [&](int m, double (&h)[kPer], double &hx) {
+    const long kg = (long)(m < 0 ? 0 : (m > g.nz - 1 ? g.nz - 1 : m)) * g.np;
+#pragma unroll
+    for (int it = 0; it < kPer; ++it) h[it] = rowp[it][kg + xs];
+    hx = extra ? xrow[kg] : 0.0;
+  };
+  auto store_level = [&](int m, const double (&h)[kPer], double hx) {
+    double(*sl)[kHx] = sF[(m + 3) % 3];
+#pragma unroll
+    for (int it = 0; it < kPer; ++it) sl[4 * it + wv][lane] = h[it];
+    if (extra) sl[xr][xsx] = hx;
+  };
   const int lx = threadIdx.x % kRx, ly = threadIdx.x / kRx;
   const int x = x0 + lx, y = y0 + ly;
-  auto relax_store = [&](long i, int xx, double (&fi)[4]) {
+  const bool pt = x < g.nx;
+  const long ij = (long)y * g.ld + (pt ? x : 0);
+  // the point values of one level: contractions, lag velocities (u, v at the level, w at
+  // the level or, near the wall, at the second w level)
+  struct Pt {
+    T c[4];
+    TV u, v, w;
+  };
+  auto load_pt = [&](int k, Pt &p) {
+    const long i = (long)k * g.np + ij;
+    p.c[0] = lm[i]; p.c[1] = mm[i]; p.c[2] = qn[i]; p.c[3] = nn[i];
+    p.u = ul[i]; p.v = vl[i];
+    p.w = wl[k <= 1 ? (long)g.np + ij : i];
+  };
+  auto dcs = [&](double v) { return UNIT ? v : v / I.cs; };
+  auto relax_store = [&](int k, long i, int xx, const T (&cc)[4], double (&fi)[4]) {
     T cs, b;
-    relax_point<T>((double)lm[i], (double)mm[i], (double)qn[i], (double)nn[i], fi, xx >= g.ld - 2, k == g.nz - 1,
-                   P, cs, b);
+    relax_point<T>((double)cc[0], (double)cc[1], (double)cc[2], (double)cc[3], fi, xx >= g.ld - 2,
+                   k == g.nz - 1, P, cs, b);
 #pragma unroll
     for (int q = 0; q < 4; ++q) Fnew.p[q][i] = fi[q];
     Cs[i] = cs;
     Beta[i] = b;
   };
-  if (x < g.nx) {
-    const long i = pk + (long)y * g.ld + x;
-    auto dcs = [&](double v) { return UNIT ? v : v / I.cs; };
-    // interpolag_Sdep.f90:204-258 lag velocities on w-nodes (u_lag at jz=1 stays on the uvp node)
-    const double ut = dcs((double)ul[i]), vt = dcs((double)vl[i]);
-    double ulag, vlag;
-    if (k == 0) {
-      ulag = ut;
-      vlag = vt;
-    } else {
-      ulag = 0.5 * (ut + dcs((double)ul[i - g.np]));
-      vlag = 0.5 * (vt + dcs((double)vl[i - g.np]));
+  // prologue: virtual levels kb-1, kb, kb+1 in the ring; the point values of kb and, for
+  // the w-node average of the lag velocities, u and v of level kb-1
+  {
+    double h[kPer], hx;
+    for (int m = kb - 1; m <= kb + 1; ++m) {
+      load_level(m, h, hx);
+      store_level(m, h, hx);
     }
-    const double xp = -ulag * I.sx;  // :263-265
-    const double yp = -vlag * I.sy;
-    double zp;
-    if (k <= 1) {  // :281-291
-      const double w2 = dcs((double)wl[(long)g.np + (i - pk)]);
-      double z2 = -w2 * I.sz;
-      z2 = z2 + fmin(z2, 0.0);
-      z2 = fmin(1.0, z2);
-      z2 = fmax(-1.0, z2);
-      zp = z2;
-    } else {
-      const double wlag = dcs((double)wl[i]);
-      zp = -wlag * I.sz;
-    }
-    const int addx = signbit(xp) ? -1 : 1, addy = signbit(yp) ? -1 : 1, addz = signbit(zp) ? -1 : 1;
-    const double cxp = fabs(xp), cyp = fabs(yp), czp = fabs(zp);
-    const double fx = 1.0 - cxp, fy = 1.0 - cyp, fz = 1.0 - czp;
-    // slots of the corners: x, x + addx (periodic); y, y + addy (periodic); k, k + addz
-    // (clamped: slot 1 + addz holds level clamp(k + addz))
-    const int sx0 = lx + 1, sxa = lx + 1 + addx, sy0 = ly + 1, sya = ly + 1 + addy, ska = 1 + addz;
-    double fi[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      fi[q] = fx * fy * (sF[q][1][sy0][sx0] * fz + sF[q][ska][sy0][sx0] * czp)
-            + fx * cyp * (sF[q][1][sya][sx0] * fz + sF[q][ska][sya][sx0] * czp)
-            + cxp * fy * (sF[q][1][sy0][sxa] * fz + sF[q][ska][sy0][sxa] * czp)
-            + cxp * cyp * (sF[q][1][sya][sxa] * fz + sF[q][ska][sya][sxa] * czp);
-    }
-    relax_store(i, x, fi);
   }
-  // the padding columns x = nx .. ld-1 of the tile's rows (old values, no interpolation)
-  const int npad = g.ld - g.nx;
-  if (x0 + kRx >= g.nx && threadIdx.x < npad * kRy) {
-    const int px = g.nx + (int)threadIdx.x % npad, py = y0 + (int)threadIdx.x / npad;
-    const long i = pk + (long)py * g.ld + px;
-    double fi[4];
+  Pt cur;
+  load_pt(kb, cur);
+  TV up = 0, vp = 0;
+  if (kb > 0) {
+    up = ul[(long)(kb - 1) * g.np + ij];
+    vp = vl[(long)(kb - 1) * g.np + ij];
+  }
+  // the prologue's loads complete here: the loop's prefetches are then the only loads in
+  // flight while a level is relaxed, and the counter waits the compiler places in the loop
+  // body cover those alone
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  for (int k = kb; k < ke; ++k) {
+    const bool more = k + 1 < ke;  // uniform over the block
+    double h[kPer], hx = 0.0;
+    Pt nxt;
+    if (more) {
+      load_level(k + 2, h, hx);
+      load_pt(k + 1, nxt);
+    }
+    if (pt) {
+      const long i = (long)k * g.np + ij;
+      // interpolag_Sdep.f90:204-258 lag velocities on w-nodes (u_lag at jz=1 stays on the uvp node)
+      const double ut = dcs((double)cur.u), vt = dcs((double)cur.v);
+      double ulag, vlag;
+      if (k == 0) {
+        ulag = ut;
+        vlag = vt;
+      } else {
+        ulag = 0.5 * (ut + dcs((double)up));
+        vlag = 0.5 * (vt + dcs((double)vp));
+      }
+      const double xp = -ulag * I.sx;  // :263-265
+      const double yp = -vlag * I.sy;
+      double zp;
+      if (k <= 1) {  // :281-291
+        const double w2 = dcs((double)cur.w);
+        double z2 = -w2 * I.sz;
+        z2 = z2 + fmin(z2, 0.0);
+        z2 = fmin(1.0, z2);
+        z2 = fmax(-1.0, z2);
+        zp = z2;
+      } else {
+        const double wlag = dcs((double)cur.w);
+        zp = -wlag * I.sz;
+      }
+      const int addx = signbit(xp) ? -1 : 1, addy = signbit(yp) ? -1 : 1, addz = signbit(zp) ? -1 : 1;
+      const double cxp = fabs(xp), cyp = fabs(yp), czp = fabs(zp);
+      const double fx = 1.0 - cxp, fy = 1.0 - cyp, fz = 1.0 - czp;
+      // slots of the corners: x, x + addx (periodic); y, y + addy (periodic); level k and
+      // virtual level k + addz (holding clamp(k + addz))
+      const int sx0 = lx + 1, sxa = lx + 1 + addx, sy0 = ly + 1, sya = ly + 1 + addy;
+      double(*s0)[kHx] = sF[k % 3];
+      double(*sa)[kHx] = sF[(k + addz + 3) % 3];
+      double fi[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) fi[q] = Fold.p[q][i];
-    relax_store(i, px, fi);
+      for (int q = 0; q < 4; ++q) {
+        const int r0 = q * kHy + sy0, ra = q * kHy + sya;
+        fi[q] = fx * fy * (s0[r0][sx0] * fz + sa[r0][sx0] * czp)
+              + fx * cyp * (s0[ra][sx0] * fz + sa[ra][sx0] * czp)
+              + cxp * fy * (s0[r0][sxa] * fz + sa[r0][sxa] * czp)
+              + cxp * cyp * (s0[ra][sxa] * fz + sa[ra][sxa] * czp);
+      }
+      relax_store(k, i, x, cur.c, fi);
+    }
+    if (more) {
+      __syncthreads();  // every wavefront is done with the slot of virtual level k-1
+      store_level(k + 2, h, hx);
+      __syncthreads();
+      up = cur.u;
+      vp = cur.v;
+      cur = nxt;
+    }
   }
 }
 
@@ -452,17 +546,23 @@ static hipError_t launch_lag_relax(void *const C[4], const double *const Fold[4]
   F4c fo;
   F4 fn;
   for (int q = 0; q < 4; ++q) { fo.p[q] = Fold[q]; fn.p[q] = Fnew[q]; }
-  if (g.ny % kRy != 0 || g.ld - g.nx > 256 / kRy) return hipErrorInvalidValue;
+  // the w level read near the wall (index 1) must exist: nz >= 2; relax_point takes the
+  // reference's exponents as constants
+  if (g.ny % kRy != 0 || g.nz < 2) return hipErrorInvalidValue;
+  if (P.powcoeff != -0.125 || P.bexp != 1.0) return hipErrorInvalidValue;
   const int ntx = (g.nx + kRx - 1) / kRx, cg = (ntx * (g.ny / kRy) + 7) / 8;
-  const dim3 grid((unsigned)(8L * cg * g.nz));
+  const int nkc = (g.nz + kRz - 1) / kRz;  // level runs of nz/nkc levels (rounded)
+  const long npts = (long)(g.ld - g.nx) * g.ny * g.nz;
+  const int npb = (int)((npts + 255) / 256 + 7) / 8 * 8;
+  const dim3 grid((unsigned)(npb + 8L * cg * nkc));
   if (I.cs == 1.0)
     hipLaunchKernelGGL((k_lag_relax<T, true, TV>), grid, dim3(256), 0, s, (const T *)C[0], (const T *)C[1],
                        (const T *)C[2], (const T *)C[3], fo, fn, (const TV *)ul, (const TV *)vl, (const TV *)wl,
-                       (T *)Cs, (T *)Beta, g, P, I, ntx, cg);
+                       (T *)Cs, (T *)Beta, g, P, I, ntx, cg, nkc, npb);
   else
     hipLaunchKernelGGL((k_lag_relax<T, false, TV>), grid, dim3(256), 0, s, (const T *)C[0], (const T *)C[1],
                        (const T *)C[2], (const T *)C[3], fo, fn, (const TV *)ul, (const TV *)vl, (const TV *)wl,
-                       (T *)Cs, (T *)Beta, g, P, I, ntx, cg);
+                       (T *)Cs, (T *)Beta, g, P, I, ntx, cg, nkc, npb);
   return hipGetLastError();
 }
 
