@@ -31,15 +31,17 @@ template <typename T, int M> struct CvCfg {
   static constexpr int NTHR = (NS * P + 63) / 64 * 64;
   using FI = Fft4<T, M, P, +1>;
   using FO = Fft4<T, M, P, -1>;
-  static constexpr int SSI = slot_stride(FI::SCR);
+  static constexpr int SSI = FI::SCR | 1;  // unpadded (odd): room for the twiddles
   static constexpr int SSO = slot_stride(FO::SCR > M ? FO::SCR : M);
   static constexpr int RL = 2 * M;                    // physical row length
   static constexpr int NPT = (L * RL + NTHR - 1) / NTHR;  // product points per thread
   static constexpr size_t mx(size_t a, size_t b) { return a > b ? a : b; }
-  static constexpr size_t lds() {
-    return mx(mx((size_t)NS * SSI * sizeof(cx<T>), (size_t)NS * RL * sizeof(T)),
-              mx((size_t)NF * RL * sizeof(T), (size_t)NF * SSO * sizeof(cx<T>)));
+  static constexpr size_t work() {
+    return (mx(mx((size_t)NS * SSI * sizeof(cx<T>), (size_t)NS * RL * sizeof(T)),
+               mx((size_t)NF * RL * sizeof(T), (size_t)NF * SSO * sizeof(cx<T>))) + 15) / 16 * 16;
   }
+  // the transforms' area, then the twiddles twM[0..M), twN[0..M]
+  static constexpr size_t lds() { return work() + (2 * M + 1) * sizeof(cx<T>); }
 };
 
 template <typename T, int M>
@@ -53,6 +55,8 @@ __global__ __launch_bounds__((CvCfg<T, M>::NTHR)) void k_convec_rows(const cx<T>
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T> *scr = reinterpret_cast<cx<T> *>(smem);  // transform scratch
   T *vals = reinterpret_cast<T *>(smem);         // [NS][RL] rows, then [NF][RL] products
+  cx<T> *tws = reinterpret_cast<cx<T> *>(smem + K::work());
+  for (int i = threadIdx.x; i <= 2 * M; i += K::NTHR) tws[i] = i < M ? twM[i] : twN[i - M];
   const int y = blockIdx.x % ny2;
   const int k0 = (blockIdx.x / ny2) * L;
   const int slot = threadIdx.x / P, t = threadIdx.x % P;
@@ -80,17 +84,25 @@ __global__ __launch_bounds__((CvCfg<T, M>::NTHR)) void k_convec_rows(const cx<T>
   };
   {
     // merge (as row_c2r): Y[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k])
+    cx<T> xa_[Q], xb_[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int kk = t + P * q;
+      xa_[q] = ldx(kk);
+      xb_[q] = ldx(M - kk);
+    }
+    __syncthreads();  // the twiddles
     cx<T> v[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int kk = t + P * q;
-      cx<T> xa = ldx(kk), xb = conjg(ldx(M - kk));
+      cx<T> xa = xa_[q], xb = conjg(xb_[q]);
       cx<T> e = xa + xb, o = xa - xb;
-      cx<T> wo = cmul(conjg(twN[kk]), o);
+      cx<T> wo = cmul(conjg(tws[M + kk]), o);
       v[q] = mk<T>(e.x - wo.y, e.y + wo.x);
     }
     cx<T> *sc = scr + (inv ? slot : 0) * K::SSI;
-    if (inv) K::FI::stage1(v, t, sc, twM);
+    if (inv) K::FI::stage1(v, t, sc, tws);
     slot_sync<P>();
     cx<T> a[K::FI::G][P];
     bool ok[K::FI::G];
@@ -164,7 +176,7 @@ __global__ __launch_bounds__((CvCfg<T, M>::NTHR)) void k_convec_rows(const cx<T>
   }
   __syncthreads();  // products read: the scratch of the forward transforms goes over them
   cx<T> *sc = scr + (fwd ? slot : 0) * K::SSO;
-  if (fwd) K::FO::stage1(z, t, sc, twM);
+  if (fwd) K::FO::stage1(z, t, sc, tws);
   slot_sync<P>();
   cx<T> a[K::FO::G][P];
   bool ok[K::FO::G];
@@ -184,7 +196,7 @@ __global__ __launch_bounds__((CvCfg<T, M>::NTHR)) void k_convec_rows(const cx<T>
     for (int kk = t; kk < ncols; kk += P) {
       cx<T> za = sc[kk % M], zb = conjg(sc[(M - kk) % M]);
       cx<T> e = za + zb, o = za - zb;
-      cx<T> wo = cmul(twN[kk % (2 * M)], o);
+      cx<T> wo = cmul(tws[M + kk], o);
       dst[kk] = mk<T>(T(0.5) * (e.x + wo.y), T(0.5) * (e.y - wo.x));
     }
   }

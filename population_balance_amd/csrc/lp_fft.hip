@@ -63,10 +63,13 @@ __global__ __launch_bounds__(kR2cBlock) void k_row_r2c(const TI *__restrict__ in
   constexpr int RB = kR2cBlock / P;
   constexpr int SS = slot_stride(F::SCR > M ? F::SCR : M);
   __shared__ cx<T> lds[RB * SS];
+  __shared__ cx<T> tws[2 * M + 1];  // twM[0..M), twN[0..M]: read from LDS by every group
   const int t = threadIdx.x % P, r = threadIdx.x / P;
   const long total = (long)rows * nplanes;
   const long ngroups = (total + RB - 1) / RB;
   cx<T> *sc = lds + r * SS;
+  for (int i = threadIdx.x; i <= 2 * M; i += kR2cBlock) tws[i] = i < M ? twM[i] : twN[i - M];
+  __syncthreads();
   using v2 = typename V2<T>::t;
   // input of row group g (rows beyond the last read row 0 and are not stored)
   auto load = [&](long g, cx<T> (&v)[Q]) {
@@ -124,7 +127,7 @@ __global__ __launch_bounds__(kR2cBlock) void k_row_r2c(const TI *__restrict__ in
     const long grow = g * RB + r;
     const bool valid = grow < total;
     const long plane = valid ? grow / rows : 0, y = valid ? grow % rows : 0;
-    F::stage1(v, t, sc, twM);
+    F::stage1(v, t, sc, tws);
     slot_sync<P>();
     cx<T> a[F::G][P];
     bool ok[F::G];
@@ -144,7 +147,7 @@ __global__ __launch_bounds__(kR2cBlock) void k_row_r2c(const TI *__restrict__ in
       for (int k = t; k <= M && k < ncols_out; k += P) {
         cx<T> za = sc[k % M], zb = conjg(sc[(M - k) % M]);
         cx<T> e = za + zb, o = za - zb;
-        cx<T> wo = cmul(twN[k % (2 * M)], o);
+        cx<T> wo = cmul(tws[M + k], o);
         dst[k] = mk<T>(T(0.5) * (e.x + wo.y), T(0.5) * (e.y - wo.x));
       }
     }
@@ -191,17 +194,20 @@ __global__ __launch_bounds__(kRowBlock) void k_row_c2r(const cx<T> *__restrict__
   constexpr int RB = kRowBlock / P;
   constexpr int SS = slot_stride(F::SCR);
   __shared__ cx<T> lds[RB * SS];
+  __shared__ cx<T> tws[2 * M];  // twM[0..M), twN[0..M): read from LDS by every row
   const int t = threadIdx.x % P, r = threadIdx.x / P;
   const long grow = (long)blockIdx.x * RB + r;
   const bool valid = grow < (long)rows * nplanes;
   const long plane = valid ? grow / rows : 0, y = valid ? grow % rows : 0;
   cx<T> *sc = lds + r * SS;
   const cx<T> *src = in + plane * in_plane + y * in_pitch;
+  for (int i = threadIdx.x; i < 2 * M; i += kRowBlock) tws[i] = i < M ? twM[i] : twN[i - M];
   auto ld = [&](int k) -> cx<T> {
     cx<T> x = (valid && k < ncols_in) ? src[k] : mk<T>(0, 0);
     if (k == 0 || k == M) x.y = 0;
     return x;
   };
+  __syncthreads();  // the twiddles
   cx<T> v[Q];
   if (ikx) {  // i kx X[k]: the kx = 0 and kx = M entries of the table are zero
 #pragma unroll
@@ -209,16 +215,16 @@ __global__ __launch_bounds__(kRowBlock) void k_row_c2r(const cx<T> *__restrict__
       const int k = t + P * q;
       const T sa = (T)ikx[k], sb = (T)ikx[M - k];
       const cx<T> ra = ld(k), rb = ld(M - k);
-      v[q] = merge_c2r(mk<T>(-ra.y * sa, ra.x * sa), mk<T>(-rb.y * sb, rb.x * sb), twN[k]);
+      v[q] = merge_c2r(mk<T>(-ra.y * sa, ra.x * sa), mk<T>(-rb.y * sb, rb.x * sb), tws[M + k]);
     }
   } else {
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int k = t + P * q;
-      v[q] = merge_c2r(ld(k), ld(M - k), twN[k]);
+      v[q] = merge_c2r(ld(k), ld(M - k), tws[M + k]);
     }
   }
-  F::stage1(v, t, sc, twM);
+  F::stage1(v, t, sc, tws);
   slot_sync<P>();
   TO *dst = out + plane * out_plane + y * out_pitch;
   const TO *tzr = tz + plane * out_plane + y * out_pitch;
@@ -335,12 +341,22 @@ __global__ __launch_bounds__(TX * kColLanes) void k_col_op(
   constexpr int DQ = (NYI > NYO ? NYI - NYO : NYO - NYI) / PL;
   constexpr int NS = FI::SCR > FO::SCR ? (FI::SCR > NYI ? FI::SCR : NYI) : (FO::SCR > NYI ? FO::SCR : NYI);
   constexpr int SS = col_slot_stride(NS);
-  __shared__ cx<T> lds[TX * SS];
+  // binary32: the twiddles are read from LDS (staged once per workgroup), which also
+  // lowers the register count; binary64 keeps the slots' LDS to itself and reads the tables
+  constexpr bool TWL = sizeof(T) == 4;
+  __shared__ cx<T> lds[TX * SS + (TWL ? NYI + NYO : 0)];
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const int kx = col0 + blockIdx.x * TX + c;
   const bool cv = kx < ncols;
   const int plane = blockIdx.y;
   cx<T> *sc = lds + c * SS;
+  if constexpr (TWL) {
+    cx<T> *tl = lds + TX * SS;
+    for (int i = threadIdx.x; i < NYI + NYO; i += TX * kColLanes) tl[i] = i < NYI ? twI[i] : twO[i - NYI];
+    twI = tl;
+    twO = tl + NYI;
+    __syncthreads();
+  }
   // invalid columns load column ncols-1 and discard it (no divergent loads)
   const cx<T> *src = in + (long)plane * in_plane + (cv ? kx : ncols - 1) + (long)t * in_pitch;
   const long istep = (long)PL * in_pitch;
@@ -436,12 +452,19 @@ __global__ __launch_bounds__(col_tx<T>() * kColLanes) void k_col_div(
   using FO = Fft4<T, NY, PL, +1>;
   constexpr int Q = NY / PL;
   constexpr int SS = col_slot_stride(FO::SCR > NY ? FO::SCR : NY);
-  __shared__ cx<T> lds[TX * SS];
+  constexpr bool TWL = (TX * SS + NY) * sizeof(cx<T>) <= 40960;  // as in k_col_op
+  __shared__ cx<T> lds[TX * SS + (TWL ? NY : 0)];
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
   const int kx = blockIdx.x * TX + c;
   const bool cv = kx < ncols;
   const long plane = blockIdx.y;
   cx<T> *sc = lds + c * SS;
+  if constexpr (TWL) {
+    cx<T> *tl = lds + TX * SS;
+    for (int i = threadIdx.x; i < NY; i += TX * kColLanes) tl[i] = tw[i];
+    tw = tl;
+    __syncthreads();
+  }
   const long off = plane * plane_el + (cv ? kx : ncols - 1) + (long)t * pitch;
   const long istep = (long)PL * pitch;
   cx<T> sa[Q], sb[Q];  // D16

@@ -121,6 +121,9 @@ __global__ __launch_bounds__(256) void k_lag_r2c(LagIn<T> in, Geo g, cx<T> *__re
   constexpr int RB = 64 / P;
   constexpr int SS = slot_stride(F::SCR > M ? F::SCR : M);
   __shared__ cx<T> lds[4 * RB * SS];
+  __shared__ cx<T> tws[2 * M + 1];  // twM[0..M), twN[0..M]
+  for (int i = threadIdx.x; i <= 2 * M; i += 256) tws[i] = i < M ? twM[i] : twN[i - M];
+  __syncthreads();
   const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int r = l / P, t = l % P;
   const long nrows = (long)g.ny * g.nz;
@@ -139,7 +142,7 @@ __global__ __launch_bounds__(256) void k_lag_r2c(LagIn<T> in, Geo g, cx<T> *__re
 #pragma unroll
       for (int m = 0; m < Q; ++m) z[m] = mk<T>(0, 0);
     }
-    F::stage1(z, t, sc, twM);
+    F::stage1(z, t, sc, tws);
     slot_sync<P>();
     cx<T> a[F::G][P];
     bool ok[F::G];
@@ -158,7 +161,7 @@ __global__ __launch_bounds__(256) void k_lag_r2c(LagIn<T> in, Geo g, cx<T> *__re
       for (int kk = t; kk < nc; kk += P) {
         cx<T> za = sc[kk % M], zb = conjg(sc[(M - kk) % M]);
         cx<T> e = za + zb, o = za - zb;
-        cx<T> wo = cmul(twN[kk % (2 * M)], o);
+        cx<T> wo = cmul(tws[M + kk], o);
         dst[kk] = mk<T>(T(0.5) * (e.x + wo.y), T(0.5) * (e.y - wo.x));
       }
     }
@@ -174,19 +177,21 @@ __host__ __device__ constexpr int lag_row_stride(int ld) { return (ld - 16 + 63)
 // lagrange_Sdep.f90:257-326: x back-transform of the 21 fields of one width for one
 // (y, z) row and their Germano contractions.  blockIdx.x = row, blockIdx.y = width
 // (0: 2 delta -> L.M, M.M; 1: 4 delta -> Q.N, N.N).  21 transforms of P lanes each; the
-// LDS of the transforms is reused for the 21 physical rows once every transform has read
-// it.  c0/c1: the two contractions (binary64 volumes) of each width.
+// LDS of the transforms (slots packed without padding: fewer bytes, more workgroups per
+// CU) is reused for the 21 physical rows once every transform has read it; the twiddles
+// sit behind it, so that the transforms issue no global loads.  c0/c1: the two
+// contractions of each width.
 template <typename T, int M>
 __global__ __launch_bounds__(256) void k_lag_c2r_contract(const cx<T> *__restrict__ bar, const cx<T> *__restrict__ hat,
                                                           int n1, int n2, long p1, long p2, T *__restrict__ lm,
                                                           T *__restrict__ mm, T *__restrict__ qn,
                                                           T *__restrict__ nn, Geo g, double cst, double tf1_2,
                                                           double tf2_2, const cx<T> *__restrict__ twM,
-                                                          const cx<T> *__restrict__ twN) {
+                                                          const cx<T> *__restrict__ twN, int tw_off) {
   constexpr int PL = row_lanes(M);
   using F = Fft4<T, M, PL, +1>;
   constexpr int Q = F::Q;
-  constexpr int SS = slot_stride(F::SCR);
+  constexpr int SS = F::SCR;
   using v2 = typename V2<T>::t;
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T> *scr = reinterpret_cast<cx<T> *>(smem);  // [21][SS] during the transforms
@@ -202,23 +207,39 @@ __global__ __launch_bounds__(256) void k_lag_c2r_contract(const cx<T> *__restric
   const int nin = half == 0 ? n1 : n2;
   const long pitch = half == 0 ? p1 : p2;
   const cx<T> *src = spec + (((long)q * g.nz + k) * g.ny + y) * pitch;
-  auto ldx = [&](int kk) -> cx<T> {
-    cx<T> x = (act && kk < nin) ? src[kk] : mk<T>(0, 0);
-    if (kk == 0 || kk == M) x.y = 0;
-    return x;
-  };
-  // merge (as k_row_c2r): Y[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k])
+  // twiddles in LDS, behind the transforms' slots (2M values, read by every lane)
+  cx<T> *tw = reinterpret_cast<cx<T> *>(smem + tw_off);
+  for (int i = threadIdx.x; i < 2 * M; i += blockDim.x) tw[i] = i < M ? twM[i] : twN[i - M];
+  // merge (as k_row_c2r): Y[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k]).  The
+  // loads of a q2 whose columns all lie at or past nin for every lane are skipped by a
+  // uniform branch.
+  cx<T> xa_[Q], xb_[Q];
+#pragma unroll
+  for (int q2 = 0; q2 < Q; ++q2) {
+    const int kk = t + PL * q2;
+    xa_[q2] = mk<T>(0, 0);
+    xb_[q2] = mk<T>(0, 0);
+    if (PL * q2 < nin) {  // uniform
+      if (act && kk < nin) xa_[q2] = src[kk];
+    }
+    if (M - PL * q2 - (PL - 1) < nin) {  // uniform: some lane has M - kk < nin
+      if (act && M - kk < nin) xb_[q2] = src[M - kk];
+    }
+  }
+  __syncthreads();  // the twiddles
   cx<T> v[Q];
 #pragma unroll
   for (int q2 = 0; q2 < Q; ++q2) {
     const int kk = t + PL * q2;
-    cx<T> xa = ldx(kk), xb = conjg(ldx(M - kk));
+    cx<T> xa = xa_[q2], xb = xb_[q2];
+    if (kk == 0) { xa.y = 0; xb.y = 0; }  // X[0] and X[M] (M - kk == M only for kk == 0)
+    xb = conjg(xb);
     cx<T> e = xa + xb, o = xa - xb;
-    cx<T> wo = cmul(conjg(twN[kk]), o);
+    cx<T> wo = cmul(conjg(tw[M + kk]), o);
     v[q2] = mk<T>(e.x - wo.y, e.y + wo.x);
   }
   cx<T> *sc = scr + (act ? slot : 0) * SS;
-  if (act) F::stage1(v, t, sc, twM);
+  if (act) F::stage1(v, t, sc, tw);
   slot_sync<PL>();
   cx<T> a[F::G][PL];
   bool ok[F::G];
@@ -510,9 +531,10 @@ static hipError_t launch_lag_contract(const void *bar, const void *hat, int n1, 
   if (!twM || !twN || n1 < 1 || n2 < 1 || n1 > M + 1 || n2 > M + 1) return hipErrorInvalidValue;
   constexpr int PL = row_lanes(M);
   using F = Fft4<T, M, PL, +1>;
-  const size_t lds_fft = (size_t)21 * slot_stride(F::SCR) * sizeof(cx<T>);
+  const size_t lds_fft = (size_t)21 * F::SCR * sizeof(cx<T>);
   const size_t lds_rows = (size_t)21 * lag_row_stride(g.ld) * sizeof(T);
-  const size_t lds = lds_fft > lds_rows ? lds_fft : lds_rows;
+  const size_t lds_a = ((lds_fft > lds_rows ? lds_fft : lds_rows) + 15) / 16 * 16;
+  const size_t lds = lds_a + 2 * M * sizeof(cx<T>);
   const int threads = (21 * PL + 63) / 64 * 64;
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_lag_c2r_contract<T, M>),
@@ -522,7 +544,7 @@ static hipError_t launch_lag_contract(const void *bar, const void *hat, int n1, 
   hipLaunchKernelGGL((k_lag_c2r_contract<T, M>), dim3((unsigned)(g.ny * g.nz), 2), dim3(threads), lds, s,
                      (const cx<T> *)bar, (const cx<T> *)hat, n1, n2, p1, p2, (T *)C[0], (T *)C[1], (T *)C[2],
                      (T *)C[3], g, P.cst,
-                     P.tf1_2, P.tf2_2, (const cx<T> *)twM, (const cx<T> *)twN);
+                     P.tf1_2, P.tf2_2, (const cx<T> *)twM, (const cx<T> *)twN, (int)lds_a);
   return hipGetLastError();
 }
 
