@@ -15,7 +15,9 @@
 
 namespace lp {
 
-static constexpr int kRowBlock = 256;
+// c2r block: 256 threads; 128 in binary64, whose slots (16-byte values) would otherwise
+// take 88 KB of LDS per block and leave one block per CU
+template <typename T> __host__ __device__ constexpr int row_block() { return sizeof(T) == 8 ? 128 : 256; }
 static constexpr int kR2cBlock = 128;  // r2c: persistent blocks of 128 threads
 
 template <typename T> __host__ __device__ constexpr bool r2c_persistent() { return sizeof(T) == 4; }
@@ -179,8 +181,8 @@ template <typename T> __device__ __forceinline__ cx<T> merge_c2r(cx<T> xa, cx<T>
 // the (kx, y) spectrum of f into that of df/dx (ddx.f90:15-24; kx is uniform along y).
 // shift (optional, EPI 0): shift[plane] is added to every stored value of the plane in
 // binary64 (the inverse of k_row_r2c's SH; padding columns stay zero).
-template <typename T, typename TO, int M, int EPI>
-__global__ __launch_bounds__(kRowBlock) void k_row_c2r(const cx<T> *__restrict__ in, long in_pitch, long in_plane,
+template <typename T, typename TO, int M, int EPI, bool IKX = false>
+__global__ __launch_bounds__(row_block<T>()) void k_row_c2r(const cx<T> *__restrict__ in, long in_pitch, long in_plane,
                                                        int ncols_in, int rows, int nplanes,
                                                        TO *__restrict__ out, long out_pitch, long out_plane,
                                                        const cx<T> *__restrict__ twM,
@@ -191,7 +193,7 @@ __global__ __launch_bounds__(kRowBlock) void k_row_c2r(const cx<T> *__restrict__
   constexpr int P = row_lanes(M);
   using F = Fft4<T, M, P, +1>;
   constexpr int Q = F::Q;
-  constexpr int RB = kRowBlock / P;
+  constexpr int RB = row_block<T>() / P;
   constexpr int SS = slot_stride(F::SCR);
   __shared__ cx<T> lds[RB * SS];
   __shared__ cx<T> tws[2 * M];  // twM[0..M), twN[0..M): read from LDS by every row
@@ -201,7 +203,10 @@ __global__ __launch_bounds__(kRowBlock) void k_row_c2r(const cx<T> *__restrict__
   const long plane = valid ? grow / rows : 0, y = valid ? grow % rows : 0;
   cx<T> *sc = lds + r * SS;
   const cx<T> *src = in + plane * in_plane + y * in_pitch;
-  for (int i = threadIdx.x; i < 2 * M; i += kRowBlock) tws[i] = i < M ? twM[i] : twN[i - M];
+  for (int i = threadIdx.x; i < 2 * M; i += row_block<T>()) tws[i] = i < M ? twM[i] : twN[i - M];
+  __shared__ T ikxs[IKX ? M + 1 : 1];  // the multipliers, in the transform's type
+  if constexpr (IKX)
+    for (int i = threadIdx.x; i <= M; i += row_block<T>()) ikxs[i] = (T)ikx[i];
   auto ld = [&](int k) -> cx<T> {
     cx<T> x = (valid && k < ncols_in) ? src[k] : mk<T>(0, 0);
     if (k == 0 || k == M) x.y = 0;
@@ -209,11 +214,11 @@ __global__ __launch_bounds__(kRowBlock) void k_row_c2r(const cx<T> *__restrict__
   };
   __syncthreads();  // the twiddles
   cx<T> v[Q];
-  if (ikx) {  // i kx X[k]: the kx = 0 and kx = M entries of the table are zero
+  if constexpr (IKX) {  // i kx X[k]: the kx = 0 and kx = M entries of the table are zero
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int k = t + P * q;
-      const T sa = (T)ikx[k], sb = (T)ikx[M - k];
+      const T sa = ikxs[k], sb = ikxs[M - k];
       const cx<T> ra = ld(k), rb = ld(M - k);
       v[q] = merge_c2r(mk<T>(-ra.y * sa, ra.x * sa), mk<T>(-rb.y * sb, rb.x * sb), tws[M + k]);
     }
@@ -623,12 +628,22 @@ static hipError_t launch_c2r(const void *in, long in_pitch, long in_plane, int n
                              const double *shift = nullptr) {
   const void *twM = tw_ptr(M, dt), *twN = tw_ptr(2 * M, dt);
   if (!twM || !twN) return hipErrorInvalidValue;
-  constexpr int RPB = kRowBlock / row_lanes(M);
+  constexpr int RPB = row_block<T>() / row_lanes(M);
   long total = (long)rows * nplanes;
   dim3 grid((unsigned)((total + RPB - 1) / RPB));
-  hipLaunchKernelGGL((k_row_c2r<T, TO, M, EPI>), grid, dim3(kRowBlock), 0, s, (const cx<T> *)in, in_pitch,
-                     in_plane, ncols_in, rows, nplanes, (TO *)out, out_pitch, out_plane, (const cx<T> *)twM,
-                     (const cx<T> *)twN, (const TO *)(EPI ? tz : out), (TO)cst, ikx, shift);
+  // the x derivative is its own kernel: the plain transform then holds no multiplier
+  // registers
+  if (ikx) {
+    if constexpr (EPI != 0) return hipErrorInvalidValue;  // no caller asks for both
+    else
+      hipLaunchKernelGGL((k_row_c2r<T, TO, M, EPI, true>), grid, dim3(row_block<T>()), 0, s, (const cx<T> *)in,
+                         in_pitch, in_plane, ncols_in, rows, nplanes, (TO *)out, out_pitch, out_plane,
+                         (const cx<T> *)twM, (const cx<T> *)twN, (const TO *)out, (TO)cst, ikx, shift);
+  } else
+    hipLaunchKernelGGL((k_row_c2r<T, TO, M, EPI, false>), grid, dim3(row_block<T>()), 0, s, (const cx<T> *)in,
+                       in_pitch, in_plane, ncols_in, rows, nplanes, (TO *)out, out_pitch, out_plane,
+                       (const cx<T> *)twM, (const cx<T> *)twN, (const TO *)(EPI ? tz : out), (TO)cst, nullptr,
+                       shift);
   return hipGetLastError();
 }
 
