@@ -169,18 +169,14 @@ __global__ __launch_bounds__(256) void k_lag_r2c(LagIn<T> in, Geo g, cx<T> *__re
   }
 }
 
-// LDS row stride (elements) of the physical rows in k_lag_c2r_contract: at least ld, and
-// = 16 (mod 64), so the 8-byte row writes of the slots of a wavefront (8 lanes per slot,
-// consecutive pairs) fall on distinct banks
-__host__ __device__ constexpr int lag_row_stride(int ld) { return (ld - 16 + 63) / 64 * 64 + 16; }
-
 // lagrange_Sdep.f90:257-326: x back-transform of the 21 fields of one width for one
 // (y, z) row and their Germano contractions.  blockIdx.x = row, blockIdx.y = width
-// (0: 2 delta -> L.M, M.M; 1: 4 delta -> Q.N, N.N).  21 transforms of P lanes each; the
-// LDS of the transforms (slots packed without padding: fewer bytes, more workgroups per
-// CU) is reused for the 21 physical rows once every transform has read it; the twiddles
-// sit behind it, so that the transforms issue no global loads.  c0/c1: the two
-// contractions of each width.
+// (0: 2 delta -> L.M, M.M; 1: 4 delta -> Q.N, N.N).  21 transforms of P lanes each in
+// LDS slots packed without padding (fewer bytes, more workgroups per CU); each transform
+// writes its physical row into its own slot once it has read it (a slot is long enough:
+// 2 (M + Q) >= ld), so only a wavefront barrier separates the two; the twiddles sit behind
+// the slots, so that the transforms issue no global loads.  c0/c1: the two contractions
+// of each width.
 template <typename T, int M>
 __global__ __launch_bounds__(256) void k_lag_c2r_contract(const cx<T> *__restrict__ bar, const cx<T> *__restrict__ hat,
                                                           int n1, int n2, long p1, long p2, T *__restrict__ lm,
@@ -195,9 +191,9 @@ __global__ __launch_bounds__(256) void k_lag_c2r_contract(const cx<T> *__restric
   using v2 = typename V2<T>::t;
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T> *scr = reinterpret_cast<cx<T> *>(smem);  // [21][SS] during the transforms
-  T *vals = reinterpret_cast<T *>(smem);         // [21][rs] afterwards
+  T *vals = reinterpret_cast<T *>(smem);         // [21][rs]: each slot's row in the slot
   const int ldp = g.ld;
-  const int rs = lag_row_stride(ldp);
+  constexpr int rs = 2 * SS;  // >= ld = 2M + 2 (SS = M + Q)
   const int row = blockIdx.x, half = blockIdx.y;
   const int k = row / g.ny, y = row % g.ny;
   const int slot = threadIdx.x / PL, t = threadIdx.x % PL;
@@ -245,7 +241,7 @@ __global__ __launch_bounds__(256) void k_lag_c2r_contract(const cx<T> *__restric
   bool ok[F::G];
 #pragma unroll
   for (int j = 0; j < F::G; ++j) ok[j] = act && F::stage2(j, t, sc, a[j]);
-  __syncthreads();  // every transform has read its slot: the LDS now holds rows
+  slot_sync<PL>();  // the slot's transform has read it: the slot now holds its row
 #pragma unroll
   for (int j = 0; j < F::G; ++j)
     if (ok[j]) {
@@ -531,9 +527,9 @@ static hipError_t launch_lag_contract(const void *bar, const void *hat, int n1, 
   if (!twM || !twN || n1 < 1 || n2 < 1 || n1 > M + 1 || n2 > M + 1) return hipErrorInvalidValue;
   constexpr int PL = row_lanes(M);
   using F = Fft4<T, M, PL, +1>;
-  const size_t lds_fft = (size_t)21 * F::SCR * sizeof(cx<T>);
-  const size_t lds_rows = (size_t)21 * lag_row_stride(g.ld) * sizeof(T);
-  const size_t lds_a = ((lds_fft > lds_rows ? lds_fft : lds_rows) + 15) / 16 * 16;
+  const size_t lds_fft = (size_t)21 * F::SCR * sizeof(cx<T>);  // the rows live in the slots
+  if (2 * F::SCR < g.ld) return hipErrorInvalidValue;
+  const size_t lds_a = (lds_fft + 15) / 16 * 16;
   const size_t lds = lds_a + 2 * M * sizeof(cx<T>);
   const int threads = (21 * PL + 63) / 64 * 64;
   if (lds > 65536) {
