@@ -83,6 +83,7 @@ int ensure_twiddle(int n) {
   HIPCHK(hipMalloc(&e.f, sizeof(float) * 2 * n));
   HIPCHK(hipMemcpy(e.d, hd.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(e.f, hf.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice));
+  HIPCHK(hipDeviceSynchronize());  // see dalloc: the tables are read on non-blocking streams
   g_tw[key] = e;
   return LP_OK;
 }
@@ -275,10 +276,16 @@ static bool field_is_f64(const lp_ctx *c, int id) {
   }
 }
 
+// Every kernel and copy of a context runs on its own non-blocking streams, which are not
+// ordered after work on the null stream: the zero fill (and any synchronous table upload,
+// whose DMA may still be in flight when hipMemcpy returns from pageable memory) is waited
+// for here, so that a buffer allocated while the context computes (the adjoint's scratch,
+// the running averages) is not zeroed under the kernels that already write it.
 static int dalloc(void **p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes ? bytes : 16);
   if (e != hipSuccess) return set_err(LP_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
   e = hipMemset(*p, 0, bytes ? bytes : 16);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) return set_err(LP_ERR_HIP, "hipMemset failed: %s", hipGetErrorString(e));
   return LP_OK;
 }
@@ -423,6 +430,7 @@ static int droplet_check(const lp_config &cf) {
 static int upload_vec(void **dst, const void *src, size_t bytes) {
   RCHK(dalloc(dst, bytes));
   hipError_t e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipDeviceSynchronize();  // see dalloc
   if (e != hipSuccess) return set_err(LP_ERR_HIP, "table upload failed: %s", hipGetErrorString(e));
   return LP_OK;
 }
@@ -643,7 +651,8 @@ extern "C" int lp_create(const lp_config *cfg, lp_ctx **out) {
       hipMemcpy(c->ky, ky.data(), tb, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->k2, k2.data(), tb, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->G1, G1.data(), tb, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(c->G2, G2.data(), tb, hipMemcpyHostToDevice) != hipSuccess) {
+      hipMemcpy(c->G2, G2.data(), tb, hipMemcpyHostToDevice) != hipSuccess ||
+      hipDeviceSynchronize() != hipSuccess) {  // see dalloc
     lp_destroy(c);
     return set_err(LP_ERR_HIP, "table upload failed");
   }

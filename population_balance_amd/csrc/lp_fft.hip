@@ -18,6 +18,16 @@ namespace lp {
 // c2r block: 256 threads; 128 in binary64, whose slots (16-byte values) would otherwise
 // take 88 KB of LDS per block and leave one block per CU
 template <typename T> __host__ __device__ constexpr int row_block() { return sizeof(T) == 8 ? 128 : 256; }
+
+// XCD-aware plane order for the binary64 passes.  Workgroup ids are dealt round-robin to
+// the 8 XCDs, each with its own L2.  The column pass and the back-transform give plane p to
+// XCD p mod 8, so that a plane written by one is read by the other on the same XCD; the
+// column pass takes an XCD's planes last first, the back-transform first first (the
+// column pass wrote those last).  In binary32 the same order measured slightly slower
+// (its planes are half the size, and more of them are served from the cache behind L2).
+// xcd_planes(n, j): the planes of XCD j.
+__host__ __device__ constexpr int xcd_planes(int nplanes, int j) { return (nplanes - j + 7) / 8; }
+template <typename T> __host__ __device__ constexpr bool xcd_order() { return sizeof(T) == 8; }
 static constexpr int kR2cBlock = 128;  // r2c: persistent blocks of 128 threads
 
 template <typename T> __host__ __device__ constexpr bool r2c_persistent() { return sizeof(T) == 4; }
@@ -189,7 +199,7 @@ __global__ __launch_bounds__(row_block<T>()) void k_row_c2r(const cx<T> *__restr
                                                        const cx<T> *__restrict__ twN,
                                                        const TO *__restrict__ tz, TO cst,
                                                        const double *__restrict__ ikx,
-                                                       const double *__restrict__ shift) {
+                                                       const double *__restrict__ shift, int xorder) {
   constexpr int P = row_lanes(M);
   using F = Fft4<T, M, P, +1>;
   constexpr int Q = F::Q;
@@ -198,7 +208,14 @@ __global__ __launch_bounds__(row_block<T>()) void k_row_c2r(const cx<T> *__restr
   __shared__ cx<T> lds[RB * SS];
   __shared__ cx<T> tws[2 * M];  // twM[0..M), twN[0..M): read from LDS by every row
   const int t = threadIdx.x % P, r = threadIdx.x / P;
-  const long grow = (long)blockIdx.x * RB + r;
+  long blk = blockIdx.x;
+  if (xorder) {  // XCD order, first-written planes first; the grid has 8 max_j(planes of j) gpp blocks
+    const int gpp = rows / RB, xj = (int)(blockIdx.x & 7u);
+    const long xi = blockIdx.x >> 3, kq = xi / gpp;
+    if (kq >= xcd_planes(nplanes, xj)) return;  // uniform over the block
+    blk = (long)(xj + 8 * kq) * gpp + xi % gpp;
+  }
+  const long grow = blk * RB + r;
   const bool valid = grow < (long)rows * nplanes;
   const long plane = valid ? grow / rows : 0, y = valid ? grow % rows : 0;
   cx<T> *sc = lds + r * SS;
@@ -337,7 +354,7 @@ __global__ __launch_bounds__(TX * kColLanes) void k_col_op(
     const cx<T> *__restrict__ in, long in_pitch, long in_plane, int ncols, int fwd, int inv, ColParams Pr,
     long out_pitch, long out_plane, const double *__restrict__ kxt, const double *__restrict__ kyt,
     const double *__restrict__ G1, const double *__restrict__ G2, int lh_s, int ny_s,
-    const cx<T> *__restrict__ twI, const cx<T> *__restrict__ twO, int col0) {
+    const cx<T> *__restrict__ twI, const cx<T> *__restrict__ twO, int col0, int nplanes) {
   constexpr int PL = kColLanes;
   using FI = Fft4<T, NYI, PL, -1>;
   using FO = Fft4<T, NYO, PL, +1>;
@@ -351,9 +368,18 @@ __global__ __launch_bounds__(TX * kColLanes) void k_col_op(
   constexpr bool TWL = sizeof(T) == 4;
   __shared__ cx<T> lds[TX * SS + (TWL ? NYI + NYO : 0)];
   const int c = threadIdx.x % TX, t = threadIdx.x / TX;
-  const int kx = col0 + blockIdx.x * TX + c;
+  int plane = blockIdx.y, cg = blockIdx.x;
+  if constexpr (xcd_order<T>()) {  // last-written planes first; the grid has gx x 8 max_j(planes of j) blocks
+    const int gx = gridDim.x;
+    const long lin = blockIdx.x + (long)gx * blockIdx.y;
+    const int xj = (int)(lin & 7), kq = (int)((lin >> 3) / gx);
+    const int npj = xcd_planes(nplanes, xj);
+    if (kq >= npj) return;  // uniform over the block
+    plane = xj + 8 * (npj - 1 - kq);
+    cg = (int)((lin >> 3) % gx);
+  }
+  const int kx = col0 + cg * TX + c;
   const bool cv = kx < ncols;
-  const int plane = blockIdx.y;
   cx<T> *sc = lds + c * SS;
   if constexpr (TWL) {
     cx<T> *tl = lds + TX * SS;
@@ -630,7 +656,8 @@ static hipError_t launch_c2r(const void *in, long in_pitch, long in_plane, int n
   if (!twM || !twN) return hipErrorInvalidValue;
   constexpr int RPB = row_block<T>() / row_lanes(M);
   long total = (long)rows * nplanes;
-  dim3 grid((unsigned)((total + RPB - 1) / RPB));
+  const int xorder = xcd_order<T>() && rows % RPB == 0 ? 1 : 0;
+  dim3 grid((unsigned)(xorder ? 8L * xcd_planes(nplanes, 0) * (rows / RPB) : (total + RPB - 1) / RPB));
   // the x derivative is its own kernel: the plain transform then holds no multiplier
   // registers
   if (ikx) {
@@ -638,12 +665,12 @@ static hipError_t launch_c2r(const void *in, long in_pitch, long in_plane, int n
     else
       hipLaunchKernelGGL((k_row_c2r<T, TO, M, EPI, true>), grid, dim3(row_block<T>()), 0, s, (const cx<T> *)in,
                          in_pitch, in_plane, ncols_in, rows, nplanes, (TO *)out, out_pitch, out_plane,
-                         (const cx<T> *)twM, (const cx<T> *)twN, (const TO *)out, (TO)cst, ikx, shift);
+                         (const cx<T> *)twM, (const cx<T> *)twN, (const TO *)out, (TO)cst, ikx, shift, xorder);
   } else
     hipLaunchKernelGGL((k_row_c2r<T, TO, M, EPI, false>), grid, dim3(row_block<T>()), 0, s, (const cx<T> *)in,
                        in_pitch, in_plane, ncols_in, rows, nplanes, (TO *)out, out_pitch, out_plane,
                        (const cx<T> *)twM, (const cx<T> *)twN, (const TO *)(EPI ? tz : out), (TO)cst, nullptr,
-                       shift);
+                       shift, xorder);
   return hipGetLastError();
 }
 
@@ -720,20 +747,22 @@ static hipError_t launch_col(int padmode, int fwd, int inv, const void *in, long
   }
   if (col0 > 0) {
     if constexpr (NYI == NYO) {
-      dim3 grid((unsigned)(ncols - col0), (unsigned)nplanes);
+      dim3 grid((unsigned)(ncols - col0), (unsigned)(xcd_order<T>() ? 8 * xcd_planes(nplanes, 0) : nplanes));
       hipExtLaunchKernelGGL((k_col_op<T, NYI, NYO, 1>), grid, dim3(kColLanes), 0, s, ev.start, ev.stop, 0,
                             (const cx<T> *)in, in_pitch, in_plane, ncols, fwd, inv, P, out_pitch, out_plane,
                             tab.kx, tab.ky, tab.G1, tab.G2, tab.lh, tab.ny, (const cx<T> *)twI,
-                            (const cx<T> *)twO, col0);
+                            (const cx<T> *)twO, col0, nplanes);
       return hipGetLastError();
     } else {
       return hipErrorInvalidValue;
     }
   }
-  dim3 grid((unsigned)((ncols + TX - 1) / TX), (unsigned)nplanes);
+  dim3 grid((unsigned)((ncols + TX - 1) / TX),
+            (unsigned)(xcd_order<T>() ? 8 * xcd_planes(nplanes, 0) : nplanes));
   hipExtLaunchKernelGGL((k_col_op<T, NYI, NYO>), grid, dim3(TX * kColLanes), 0, s, ev.start, ev.stop, 0,
                         (const cx<T> *)in, in_pitch, in_plane, ncols, fwd, inv, P, out_pitch, out_plane, tab.kx,
-                        tab.ky, tab.G1, tab.G2, tab.lh, tab.ny, (const cx<T> *)twI, (const cx<T> *)twO, 0);
+                        tab.ky, tab.G1, tab.G2, tab.lh, tab.ny, (const cx<T> *)twI, (const cx<T> *)twO, 0,
+                        nplanes);
   return hipGetLastError();
 }
 
