@@ -1200,11 +1200,9 @@ static int do_lagrange(lp_ctx *c, bool use_u = false) {
     // 256: the kx = 64 column that G_test keeps by rounding) one column per block
     const int tx = c->fdt == F64 ? 8 : 16;
     const int nmain = (nc % tx <= 2 && nc > tx) ? nc / tx * tx : nc;
-    const LaunchEv ev = prof_attach(c, LP_K_COL_OP);
+    // not in the column-pass class: in a step this launch shares the GPU with convec
     HIPCHK(col_op(c->fdt, gg.ny, gg.ny, 0, 1, 1, c->lspec, pc, ps, nmain, kNumLag * gg.nz, 2, o, pc, ps, c->tab,
-                  c->s, ev));
-    prof_attached(c, ev,
-                  (double)kNumLag * gg.nz * gg.ny * (nmain + std::min(nmain, n1) + std::min(nmain, n2)) * 2 * fe);
+                  c->s));
     if (nmain < nc) {
       ColOut ot[2];
       int nt = 0;
@@ -1793,7 +1791,10 @@ static int step_body(lp_ctx *c) {
   RCHK(do_ddz_uv(c, c->f[LP_DVDZ], c->f[LP_V], true));
   RCHK(do_ddz_w(c, c->f[LP_DWDZ], c->f[LP_W], true));
   HIPCHK(hipStreamWaitEvent(c->s, c->ev_aux, 0));  // sgs_stag reads level 1
-  c->fork_pending = true;  // convec (:419) starts on the second stream inside sgs_stag
+  // convec (:419) needs u, v, w, their derivatives and the wall law's level-1 dudz, dvdz
+  // only: it starts on the second stream here and overlaps the whole SGS model
+  c->fork_pending = true;
+  RCHK(fork_convec(c));
   const int rc_sgs = do_sgs_stag(c);  // :327
   if (rc_sgs != LP_OK) {
     c->fork_pending = false;
