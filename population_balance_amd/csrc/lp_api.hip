@@ -938,7 +938,7 @@ static int do_filt_da(lp_ctx *c, void *f, void *dfdx, void *dfdy, int nlev, void
     Geo g = geo(c);
     double *A = c->nyq, *B = A + (size_t)c->g.nz * g.ny, *Cc = B + (size_t)c->g.nz * g.nx;
     if (g.nx > kNyqMax || g.ny > kNyqMax) return set_err(LP_ERR_ARG, "filt_da: nx, ny above %d", kNyqMax);
-    hipLaunchKernelGGL(k_nyq_part, dim3((unsigned)(nlev * kNyqSlices)), dim3(1024), 0, c->s, (const double *)f, A,
+    hipLaunchKernelGGL(k_nyq_part, dim3((unsigned)(nlev * kNyqSlices)), dim3(256), 0, c->s, (const double *)f, A,
                        c->nyq_part, g);
     hipLaunchKernelGGL(k_nyq_fin, dim3((unsigned)nlev), dim3(256), 0, c->s, (const double *)A,
                        (const double *)c->nyq_part, B, Cc, g);

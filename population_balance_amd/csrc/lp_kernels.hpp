@@ -56,15 +56,15 @@ __global__ void k_add_dc(cx<T> *__restrict__ spec, long pitch, int ny, int nplan
 constexpr int kNyqMax = 384;  // largest nx, ny of the transforms
 
 // The sums in two passes that read f once: k_nyq_part gives each of kNyqSlices workgroups
-// of a level a contiguous slice of its rows (16 waves, every 16th row of the slice), which
-// yields A(y) of its rows (a wave reduction) and its partial alternating column sums
-// P[k][s][x] (the 16 waves' partials added in LDS); k_nyq_fin adds the slices' partials in
-// slice order for B and forms C from A.  Every sum has a fixed order: the result does not
-// depend on timing.
-constexpr int kNyqSlices = 4;
-static __global__ __launch_bounds__(1024) void k_nyq_part(const double *__restrict__ f, double *__restrict__ A,
-                                                          double *__restrict__ P, Geo g) {
-  __shared__ double colp[16][kNyqMax];
+// of a level a contiguous slice of its rows (4 waves, each taking every 4th row of the
+// slice, kNyqRU rows at a time with their loads issued together), which yields A(y) of its
+// rows (a wave reduction) and its partial alternating column sums P[k][s][x] (the 4 waves'
+// partials added in LDS); k_nyq_fin adds the slices' partials in slice order for B and
+// forms C from A.  Every sum has a fixed order: the result does not depend on timing.
+constexpr int kNyqSlices = 16, kNyqRU = 4;
+static __global__ __launch_bounds__(256) void k_nyq_part(const double *__restrict__ f, double *__restrict__ A,
+                                                         double *__restrict__ P, Geo g) {
+  __shared__ double colp[4][kNyqMax];
   const int lane = threadIdx.x % 64, grp = threadIdx.x / 64;
   const int k = blockIdx.x / kNyqSlices, sl = blockIdx.x % kNyqSlices;
   const int y0 = sl * g.ny / kNyqSlices, y1 = (sl + 1) * g.ny / kNyqSlices;
@@ -73,22 +73,31 @@ static __global__ __launch_bounds__(1024) void k_nyq_part(const double *__restri
   double col[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) col[q] = 0.0;
-  for (int y = y0 + grp; y < y1; y += 16) {
-    const double *r = p + (long)y * g.ld;
-    const bool odd_y = y & 1;
-    double ra = 0.0;
+  for (int yb = y0 + grp; yb < y1; yb += 4 * kNyqRU) {
+    double v[kNyqRU][NQ];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const int x = lane + 64 * q;
-      if (x < g.nx) {
-        const double v = r[x];
-        ra += (lane & 1) ? -v : v;  // x and lane have the same parity
-        col[q] += odd_y ? -v : v;
+    for (int u = 0; u < kNyqRU; ++u) {
+      const int y = yb + 4 * u;
+      const double *r = p + (long)(y < y1 ? y : y0) * g.ld;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int x = lane + 64 * q;
+        v[u][q] = (y < y1 && x < g.nx) ? r[x] : 0.0;
       }
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) ra += __shfl_xor(ra, o, 64);
-    if (lane == 0) A[(long)k * g.ny + y] = ra / g.nx;
+    for (int u = 0; u < kNyqRU; ++u) {
+      const int y = yb + 4 * u;
+      double ra = 0.0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        ra += (lane & 1) ? -v[u][q] : v[u][q];  // x and lane have the same parity
+        col[q] += (y & 1) ? -v[u][q] : v[u][q];
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) ra += __shfl_xor(ra, o, 64);
+      if (lane == 0 && y < y1) A[(long)k * g.ny + y] = ra / g.nx;
+    }
   }
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
@@ -98,7 +107,7 @@ static __global__ __launch_bounds__(1024) void k_nyq_part(const double *__restri
   __syncthreads();
   for (int x = threadIdx.x; x < g.nx; x += blockDim.x) {
     double s = 0.0;
-    for (int r = 0; r < 16; ++r) s += colp[r][x];
+    for (int r = 0; r < 4; ++r) s += colp[r][x];
     P[((long)k * kNyqSlices + sl) * g.nx + x] = s;
   }
 }
