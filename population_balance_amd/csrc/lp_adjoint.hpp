@@ -753,18 +753,31 @@ __global__ void k_adj_drop_z(const double *__restrict__ C, const double *__restr
 // l the adjoint of the faces (face nx folded into 0) and S = sum l:
 //   adjoint of dudx(m), m < nx-1: dx (sum_{i>m} l(i) - S (nx-1-m)/nx);  of u(m): S/nx.
 // The same along y (k_adj_face_v, one thread per (x, level) column).
-__global__ void k_adj_face_u(const double *__restrict__ aui, double *__restrict__ aux, double *__restrict__ auf,
+// The face adjoints of all bins summed (bin by bin in index order, from zero), for
+// k_adj_face_u and k_adj_face_v: one coalesced pass instead of npc strided loads per face
+// and use inside the line loops.
+__global__ void k_adj_face_bins(const double *__restrict__ aui, const double *__restrict__ avi,
+                                double *__restrict__ lu, double *__restrict__ lv, DropGeo g) {
+  const long fz = g.fp * g.nz;
+  const long s = gtid();
+  if (s >= fz) return;
+  double a = 0.0, b = 0.0;
+  for (int q = 0; q < g.npc; ++q) {
+    a += aui[(long)q * fz + s];
+    b += avi[(long)q * fz + s];
+  }
+  lu[s] = a;
+  lv[s] = b;
+}
+
+// lu, lv: k_adj_face_bins's sums
+__global__ void k_adj_face_u(const double *__restrict__ lu, double *__restrict__ aux, double *__restrict__ auf,
                              DropGeo g, double dx) {
   const long row = gtid();
   if (row >= (long)g.ny * g.nz) return;
   const int k = (int)(row / g.ny), j = (int)(row % g.ny), nx = g.nx;
-  const double *L0 = aui + (long)k * g.fp + (long)j * g.ld;
-  const long fz = g.fp * g.nz;
-  auto L = [&](int i) {  // summed over the bins
-    double v = 0.0;
-    for (int b = 0; b < g.npc; ++b) v += L0[(long)b * fz + i];
-    return v;
-  };
+  const double *L0 = lu + (long)k * g.fp + (long)j * g.ld;
+  auto L = [&](int i) { return L0[i]; };  // summed over the bins
   const long o = (long)k * g.np + (long)j * g.ld;
   const double l0 = L(0) + L(nx);
   double S = l0;
@@ -776,19 +789,14 @@ __global__ void k_adj_face_u(const double *__restrict__ aui, double *__restrict_
     if (m + 1 < nx) tail -= L(m + 1);
   }
 }
-__global__ void k_adj_face_v(const double *__restrict__ avi, double *__restrict__ avy, double *__restrict__ avf,
+__global__ void k_adj_face_v(const double *__restrict__ lv, double *__restrict__ avy, double *__restrict__ avf,
                              DropGeo g, double dy) {
   const long t = gtid();
   if (t >= (long)g.nx * g.nz) return;
   const int k = (int)(t / g.nx), i = (int)(t % g.nx), ny = g.ny;
   const long ld = g.ld;
-  const double *L0 = avi + (long)k * g.fp + i;
-  const long fz = g.fp * g.nz;
-  auto L = [&](int j) {  // summed over the bins
-    double v = 0.0;
-    for (int b = 0; b < g.npc; ++b) v += L0[(long)b * fz + (long)j * ld];
-    return v;
-  };
+  const double *L0 = lv + (long)k * g.fp + i;
+  auto L = [&](int j) { return L0[(long)j * ld]; };  // summed over the bins
   const long o = (long)k * g.np + i;
   const double l0 = L(0) + L(ny);
   double S = l0;

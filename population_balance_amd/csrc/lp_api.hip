@@ -2667,11 +2667,17 @@ static int do_step_adjoint(lp_ctx *c, const double *nut, const double *ust, doub
                      avx, avy, awx, awy, afu, afv, afw, ag, 1.0 / dz, 1.0 / (zl * den), wall);
   HIPCHK(hipGetLastError());
   if (drops) {  // the droplet faces read the filtered u, v, w and dudx, dvdy
-    hipLaunchKernelGGL(k_adj_face_u, grid1((long)gg.ny * gg.nz), dim3(256), 0, c->s, (const double *)aui, aux, afu, dg,
-                       gg.dx);
-    hipLaunchKernelGGL(k_adj_face_v, grid1((long)gg.nx * gg.nz), dim3(256), 0, c->s, (const double *)avi, avy, afv, dg,
-                       gg.dy);
     hipLaunchKernelGGL(k_adj_face_w, gv, dim3(256), 0, c->s, (const double *)awi, afw, dg);
+    // the bin sums of the u and v face adjoints go where mu, ac and awi were (dead now:
+    // 3 npcon nv >= 2 fp nz)
+    const long fz = (long)dg.fp * gg.nz;
+    double *lu = dmu, *lv = dmu + fz;
+    hipLaunchKernelGGL(k_adj_face_bins, grid1(fz), dim3(256), 0, c->s, (const double *)aui, (const double *)avi, lu, lv,
+                       dg);
+    hipLaunchKernelGGL(k_adj_face_u, grid1((long)gg.ny * gg.nz), dim3(256), 0, c->s, (const double *)lu, aux, afu, dg,
+                       gg.dx);
+    hipLaunchKernelGGL(k_adj_face_v, grid1((long)gg.nx * gg.nz), dim3(256), 0, c->s, (const double *)lv, avy, afv, dg,
+                       gg.dy);
     HIPCHK(hipGetLastError());
   }
   // ---- filt_da transposed: F(af) - Dx(a*x) - Dy(a*y) ----
