@@ -477,9 +477,10 @@ __global__ void k_tau_frozen(const T *__restrict__ S11, const T *__restrict__ S1
 // droplets (pollen_all_in_one2, scalars_module_crossflow_mod.f90:1494-1786, with
 // pollen_RHS_calc2 :1894-4472, calcbeta_pcon :1795-1843, step_pollen2 :4481-4508), all
 // bins in one launch per pass, with per-bin face adjoints summed at the end.  Frozen: Kc_t (from Nu_t).  The transport is linear in C on each branch
-// of the SMART limiter: its transpose uses the branch the base C selects.  The three
-// directions are line scans: one thread owns an x row, a y column or a z column and
-// scatters into the cells of its own line only, so the passes need no atomics.
+// of the SMART limiter: its transpose uses the branch the base C selects.  x and y are
+// gathers (a thread per cell adds what the scan would scatter into it, in the scan's
+// order); z is a line scan (a thread owns a z column and scatters into its own cells),
+// so no pass needs atomics.
 // ------------------------------------------------------------------------------------
 
 // d smart_face / d (cup, cupup, cdn) on the branch smart_face selects
@@ -527,164 +528,109 @@ struct DropAdjP {
   int bc_x, bc_y, settling;
 };
 
-// x: one thread per (y, level < nz-1) row and bin; aui (ld, ny+1, nz) of the bin gains its
-// adjoint of the face velocities
-__global__ void k_adj_drop_x(const double *__restrict__ C, const double *__restrict__ mu, double *__restrict__ ac,
-                             const double *__restrict__ ui0, double *__restrict__ aui, const double *__restrict__ kc,
-                             DropGeo g, DropAdjP P) {
-  const long nl = (long)g.ny * (g.nz - 1);
+// x (Y false) and y (Y true) transport transposed in gather form, one thread per cell of
+// every bin (consecutive threads: consecutive x).  A cell adds the terms the line scan of
+// the fluxes transposed scatters into it, in the scan's order: the interior faces c-1, c,
+// c+1, c+2, then the end faces, the Kc diffusion of cells c-1, c, c+1 (ghosts at the line
+// ends), and last the folds of the ghosts' adjoints at the two end cells.  The terms of a
+// face are recomputed by each cell that reads them; cell c = f >= 1 writes face f's
+// adjoint into auv (the bin's (ld, ny+1, nz) face array), cell 0 or n-1 the end faces.
+template <bool Y>
+__global__ void k_adj_drop_line(const double *__restrict__ C, const double *__restrict__ mu,
+                                double *__restrict__ ac, const double *__restrict__ uv0, double *__restrict__ auv,
+                                const double *__restrict__ kc, DropGeo g, DropAdjP P) {
+  const long nl = (long)g.nx * g.ny * (g.nz - 1);
   const long t = gtid();
   if (t >= nl * g.npc) return;
   const int b = (int)(t / nl);
-  const long row = t % nl;
-  const int k = (int)(row / g.ny), j = (int)(row % g.ny), nx = g.nx;
-  const long o = (long)k * g.np + (long)j * g.ld, ob = (long)b * g.nv;
-  const double *Cr = C + ob + o, *M = mu + ob + o, *K = kc + o;
-  double *A = ac + ob + o;
-  const double *U = ui0 + (long)k * g.fp + (long)j * g.ld;
-  double *AU = aui + (long)b * g.fp * g.nz + (long)k * g.fp + (long)j * g.ld;
-  double gx0 = 0.0, gxL = 0.0, s0 = 0.0, s1 = 0.0;
-  if (P.bc_x == 0) { gx0 = Cr[nx - 1]; gxL = Cr[0]; }
-  else if (P.bc_x == 1) { gxL = Cr[nx - 1]; }
+  const long r = t % nl;
+  const int i = (int)(r % g.nx), j = (int)((r / g.nx) % g.ny), k = (int)(r / ((long)g.nx * g.ny));
+  const int n = Y ? g.ny : g.nx, c = Y ? j : i, bc = Y ? P.bc_y : P.bc_x;
+  const long st = Y ? g.ld : 1, ol = Y ? (long)i : (long)j * g.ld, ob = (long)b * g.nv;
+  const double *Cl = C + ob + (long)k * g.np + ol, *Ml = mu + ob + (long)k * g.np + ol, *Kl = kc + (long)k * g.np + ol;
+  const double *U = uv0 + (long)k * g.fp + ol;
+  double *AU = auv + (long)b * g.fp * g.nz + (long)k * g.fp + ol;
+  const double id = Y ? P.idy : P.idx, id2 = Y ? P.idy2 : P.idx2;
+  auto Cv = [&](int q) { return Cl[q * st]; };
+  auto Mv = [&](int q) { return Ml[q * st]; };
+  auto Kv = [&](int q) { return Kl[q * st]; };
+  const double u0 = U[0];
+  double g0 = 0.0, gL = 0.0, s0 = 0.0, s1 = 0.0;  // the ghost cells
+  if (bc == 0) { g0 = Cv(n - 1); gL = Cv(0); }
+  else if (bc == 1) { if (Y) g0 = Cv(0); else gL = Cv(n - 1); }
   else {
-    s0 = signbit(U[0]) ? 1.0 : 0.0;
-    s1 = signbit(-U[0]) ? 1.0 : 0.0;
-    gx0 = Cr[0] * s0; gxL = Cr[nx - 1] * s1;
+    s0 = signbit(u0) ? 1.0 : 0.0;
+    s1 = signbit(-u0) ? 1.0 : 0.0;
+    g0 = Cv(0) * s0; gL = Cv(n - 1) * s1;
   }
-  double agx0 = 0.0, agxL = 0.0;
-  for (int f = 1; f < nx; ++f) {  // interior faces: between cells f-1 and f
-    const double m = M[f] - M[f - 1], uf = U[f], a = uf * P.idx * m;
-    double cup, cupup, cdn, au, auu, ad;
+  // interior face f (between cells f-1 and f): a = u_f m / d, the SMART branch coefficients
+  auto face = [&](int f, double &a, double &au, double &auu, double &ad, double &cup, double &cupup, double &cdn) {
+    const double uf = U[f * st];
+    a = uf * id * (Mv(f) - Mv(f - 1));
     const bool pos = uf >= 0.0;
-    if (pos) { cup = Cr[f - 1]; cupup = f >= 2 ? Cr[f - 2] : gx0; cdn = Cr[f]; }
-    else { cup = Cr[f]; cupup = f + 1 <= nx - 1 ? Cr[f + 1] : gxL; cdn = Cr[f - 1]; }
-    AU[f] += smart_face(cup, cupup, cdn) * P.idx * m;
+    if (pos) { cup = Cv(f - 1); cupup = f >= 2 ? Cv(f - 2) : g0; cdn = Cv(f); }
+    else { cup = Cv(f); cupup = f + 1 <= n - 1 ? Cv(f + 1) : gL; cdn = Cv(f - 1); }
     smart_coef(cup, cupup, cdn, au, auu, ad);
-    if (pos) {
-      A[f - 1] += au * a; A[f] += ad * a;
-      if (f >= 2) A[f - 2] += auu * a; else agx0 += auu * a;
-    } else {
-      A[f] += au * a; A[f - 1] += ad * a;
-      if (f + 1 <= nx - 1) A[f + 1] += auu * a; else agxL += auu * a;
+    return pos;
+  };
+  auto diff = [&](int q, double &Kp, double &Km) {  // returns the cell's m / d^2
+    const double Kq = Kv(q);
+    Kp = 0.5 * (Kq + Kv(q + 1 < n ? q + 1 : 0));
+    Km = 0.5 * (Kq + Kv(q >= 1 ? q - 1 : n - 1));
+    return Mv(q) * id2;
+  };
+  double A = ac[ob + (long)k * g.np + ol + c * st];
+  double a, au, auu, ad, cu, cuu, cd;
+  if (c - 1 >= 1 && !face(c - 1, a, au, auu, ad, cu, cuu, cd)) A += auu * a;
+  if (c >= 1) {
+    const bool pos = face(c, a, au, auu, ad, cu, cuu, cd);
+    A += pos ? ad * a : au * a;
+    AU[c * st] += smart_face(cu, cuu, cd) * id * (Mv(c) - Mv(c - 1));
+  }
+  if (c + 1 <= n - 1) {
+    const bool pos = face(c + 1, a, au, auu, ad, cu, cuu, cd);
+    A += pos ? au * a : ad * a;
+  }
+  if (c + 2 <= n - 1 && face(c + 2, a, au, auu, ad, cu, cuu, cd)) A += auu * a;
+  // faces 0 and n
+  const double m0 = Mv(0), mL = -Mv(n - 1);
+  const double a0 = u0 * id * m0, aL = U[n * st] * id * mL;
+  if (bc == 0) {
+    int iu, iuu, idn;
+    if (u0 >= 0.0) { iu = n - 1; iuu = n - 2; idn = 0; }
+    else { iu = 0; iuu = 1; idn = n - 1; }
+    if (c == iu || c == iuu || c == idn) {
+      smart_coef(Cv(iu), Cv(iuu), Cv(idn), au, auu, ad);
+      A += (c == iu ? au : c == iuu ? auu : ad) * (a0 + aL);
     }
-  }
-  {  // faces 0 and nx
-    const double m0 = M[0], mL = -M[nx - 1];
-    const double a0 = U[0] * P.idx * m0, aL = U[nx] * P.idx * mL;
-    if (P.bc_x == 0) {
-      int iu, iuu, idn;
-      if (U[0] >= 0.0) { iu = nx - 1; iuu = nx - 2; idn = 0; }
-      else { iu = 0; iuu = 1; idn = nx - 1; }
-      const double cup = Cr[iu], cupup = Cr[iuu], cdn = Cr[idn];
-      const double f0 = smart_face(cup, cupup, cdn);
-      AU[0] += f0 * P.idx * m0;
-      AU[nx] += f0 * P.idx * mL;
-      double au, auu, ad;
-      smart_coef(cup, cupup, cdn, au, auu, ad);
-      const double a = a0 + aL;
-      A[iu] += au * a; A[iuu] += auu * a; A[idn] += ad * a;
-    } else if (P.bc_x == 1) {  // face 0 carries nothing, face nx the last cell
-      AU[nx] += Cr[nx - 1] * P.idx * mL;
-      A[nx - 1] += aL;
-    } else {                   // the ghosts are the end faces
-      AU[0] += gx0 * P.idx * m0;
-      AU[nx] += gxL * P.idx * mL;
-      agx0 += a0; agxL += aL;
+    if (c == 0) {
+      const double f0 = smart_face(Cv(iu), Cv(iuu), Cv(idn));
+      AU[0] += f0 * id * m0;
+      AU[n * st] += f0 * id * mL;
     }
+  } else if (bc == 1) {  // x: face 0 carries nothing, face n the last cell; y: the mirror
+    if (Y && c == 0) { AU[0] += Cv(0) * id * m0; A += a0; }
+    if (!Y && c == n - 1) { AU[n * st] += Cv(n - 1) * id * mL; A += aL; }
+  } else if (c == 0) {   // the ghosts are the end faces
+    AU[0] += g0 * id * m0;
+    AU[n * st] += gL * id * mL;
   }
-  for (int i = 0; i < nx; ++i) {  // Kc diffusion, ghosts at the row ends
-    const double m = M[i] * P.idx2, Ki = K[i];
-    const double Kp = 0.5 * (Ki + K[i + 1 < nx ? i + 1 : 0]), Km = 0.5 * (Ki + K[i >= 1 ? i - 1 : nx - 1]);
-    A[i] -= (Kp + Km) * m;
-    if (i + 1 <= nx - 1) A[i + 1] += Kp * m; else agxL += Kp * m;
-    if (i >= 1) A[i - 1] += Km * m; else agx0 += Km * m;
+  double Kp, Km;
+  if (c >= 1) { const double m = diff(c - 1, Kp, Km); A += Kp * m; }
+  { const double m = diff(c, Kp, Km); A -= (Kp + Km) * m; }
+  if (c + 1 <= n - 1) { const double m = diff(c + 1, Kp, Km); A += Km * m; }
+  if (c == 0 || c == n - 1) {  // the ghosts' adjoints, summed in the scan's order
+    double ag0 = 0.0, agL = 0.0;
+    if (face(1, a, au, auu, ad, cu, cuu, cd)) ag0 += auu * a;
+    if (!face(n - 1, a, au, auu, ad, cu, cuu, cd)) agL += auu * a;
+    if (bc == 2) { ag0 += a0; agL += aL; }
+    { const double m = diff(0, Kp, Km); ag0 += Km * m; }
+    { const double m = diff(n - 1, Kp, Km); agL += Kp * m; }
+    if (bc == 0) A += c == n - 1 ? ag0 : agL;
+    else if (bc == 1) { if (Y && c == 0) A += ag0; if (!Y && c == n - 1) A += agL; }
+    else A += c == 0 ? s0 * ag0 : s1 * agL;
   }
-  if (P.bc_x == 0) { A[nx - 1] += agx0; A[0] += agxL; }
-  else if (P.bc_x == 1) { A[nx - 1] += agxL; }
-  else { A[0] += s0 * agx0; A[nx - 1] += s1 * agxL; }
-}
-
-// y: one thread per (x < nx, level < nz-1) column and bin (consecutive threads: consecutive x)
-__global__ void k_adj_drop_y(const double *__restrict__ C, const double *__restrict__ mu, double *__restrict__ ac,
-                             const double *__restrict__ vi0, double *__restrict__ avi, const double *__restrict__ kc,
-                             DropGeo g, DropAdjP P) {
-  const long nl = (long)g.nx * (g.nz - 1);
-  const long t0 = gtid();
-  if (t0 >= nl * g.npc) return;
-  const int b = (int)(t0 / nl);
-  const long t = t0 % nl;
-  const int k = (int)(t / g.nx), i = (int)(t % g.nx), ny = g.ny;
-  const long o = (long)k * g.np + i, ld = g.ld, ob = (long)b * g.nv;
-  const double *Cc = C + ob + o, *M = mu + ob + o, *K = kc + o;
-  double *A = ac + ob + o;
-  const double *V = vi0 + (long)k * g.fp + i;
-  double *AV = avi + (long)b * g.fp * g.nz + (long)k * g.fp + i;
-#define CY(j) Cc[(long)(j) * ld]
-#define MY(j) M[(long)(j) * ld]
-#define AY(j) A[(long)(j) * ld]
-  double gy0 = 0.0, gyL = 0.0, s0 = 0.0, s1 = 0.0;
-  if (P.bc_y == 0) { gy0 = CY(ny - 1); gyL = CY(0); }
-  else if (P.bc_y == 1) { gy0 = CY(0); }
-  else {
-    s0 = signbit(V[0]) ? 1.0 : 0.0;
-    s1 = signbit(-V[0]) ? 1.0 : 0.0;
-    gy0 = CY(0) * s0; gyL = CY(ny - 1) * s1;
-  }
-  double agy0 = 0.0, agyL = 0.0;
-  for (int f = 1; f < ny; ++f) {
-    const double m = MY(f) - MY(f - 1), vf = V[(long)f * ld], a = vf * P.idy * m;
-    double cup, cupup, cdn, au, auu, ad;
-    const bool pos = vf >= 0.0;
-    if (pos) { cup = CY(f - 1); cupup = f >= 2 ? CY(f - 2) : gy0; cdn = CY(f); }
-    else { cup = CY(f); cupup = f + 1 <= ny - 1 ? CY(f + 1) : gyL; cdn = CY(f - 1); }
-    AV[(long)f * ld] += smart_face(cup, cupup, cdn) * P.idy * m;
-    smart_coef(cup, cupup, cdn, au, auu, ad);
-    if (pos) {
-      AY(f - 1) += au * a; AY(f) += ad * a;
-      if (f >= 2) AY(f - 2) += auu * a; else agy0 += auu * a;
-    } else {
-      AY(f) += au * a; AY(f - 1) += ad * a;
-      if (f + 1 <= ny - 1) AY(f + 1) += auu * a; else agyL += auu * a;
-    }
-  }
-  {
-    const double m0 = MY(0), mL = -MY(ny - 1);
-    const double a0 = V[0] * P.idy * m0, aL = V[(long)ny * ld] * P.idy * mL;
-    if (P.bc_y == 0) {
-      int ju, juu, jdn;
-      if (V[0] >= 0.0) { ju = ny - 1; juu = ny - 2; jdn = 0; }
-      else { ju = 0; juu = 1; jdn = ny - 1; }
-      const double cup = CY(ju), cupup = CY(juu), cdn = CY(jdn);
-      const double f0 = smart_face(cup, cupup, cdn);
-      AV[0] += f0 * P.idy * m0;
-      AV[(long)ny * ld] += f0 * P.idy * mL;
-      double au, auu, ad;
-      smart_coef(cup, cupup, cdn, au, auu, ad);
-      const double a = a0 + aL;
-      AY(ju) += au * a; AY(juu) += auu * a; AY(jdn) += ad * a;
-    } else if (P.bc_y == 1) {  // face 0 is the first cell, face ny carries nothing
-      AV[0] += CY(0) * P.idy * m0;
-      AY(0) += a0;
-    } else {
-      AV[0] += gy0 * P.idy * m0;
-      AV[(long)ny * ld] += gyL * P.idy * mL;
-      agy0 += a0; agyL += aL;
-    }
-  }
-  for (int j = 0; j < ny; ++j) {
-    const double m = MY(j) * P.idy2, Kj = K[(long)j * ld];
-    const double Kp = 0.5 * (Kj + K[(long)(j + 1 < ny ? j + 1 : 0) * ld]);
-    const double Km = 0.5 * (Kj + K[(long)(j >= 1 ? j - 1 : ny - 1) * ld]);
-    AY(j) -= (Kp + Km) * m;
-    if (j + 1 <= ny - 1) AY(j + 1) += Kp * m; else agyL += Kp * m;
-    if (j >= 1) AY(j - 1) += Km * m; else agy0 += Km * m;
-  }
-  if (P.bc_y == 0) { AY(ny - 1) += agy0; AY(0) += agyL; }
-  else if (P.bc_y == 1) { AY(0) += agy0; }
-  else { AY(0) += s0 * agy0; AY(ny - 1) += s1 * agyL; }
-#undef CY
-#undef MY
-#undef AY
+  ac[ob + (long)k * g.np + ol + c * st] = A;
 }
 
 // z: one thread per (x < nx, y) column and bin; w faces 1..nz-1 (face 0 carries nothing,

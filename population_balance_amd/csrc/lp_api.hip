@@ -2611,10 +2611,11 @@ static int do_step_adjoint(lp_ctx *c, const double *nut, const double *ust, doub
     const double *PC = (const double *)c->f[LP_PCON];
     hipLaunchKernelGGL(k_adj_drop_update, grid1(dg.nv * D.npcon), dim3(256), 0, c->s, (const double *)cb, rcb,
                        (const double *)arz, (const double *)c->d_vfac, D.active, dmu, dac, dg, dt, euler);
-    hipLaunchKernelGGL(k_adj_drop_x, grid1((long)gg.ny * (gg.nz - 1) * D.npcon), dim3(256), 0, c->s, PC,
-                       (const double *)dmu, dac, (const double *)c->ui0, aui, (const double *)c->kcell, dg, P);
-    hipLaunchKernelGGL(k_adj_drop_y, grid1((long)gg.nx * (gg.nz - 1) * D.npcon), dim3(256), 0, c->s, PC,
-                       (const double *)dmu, dac, (const double *)c->vi0, avi, (const double *)c->kcell, dg, P);
+    const long ncell = (long)gg.nx * gg.ny * (gg.nz - 1) * D.npcon;
+    hipLaunchKernelGGL(k_adj_drop_line<false>, grid1(ncell), dim3(256), 0, c->s, PC, (const double *)dmu, dac,
+                       (const double *)c->ui0, aui, (const double *)c->kcell, dg, P);
+    hipLaunchKernelGGL(k_adj_drop_line<true>, grid1(ncell), dim3(256), 0, c->s, PC, (const double *)dmu, dac,
+                       (const double *)c->vi0, avi, (const double *)c->kcell, dg, P);
     hipLaunchKernelGGL(k_adj_drop_z, grid1((long)gg.nx * gg.ny * D.npcon), dim3(256), 0, c->s, PC,
                        (const double *)dmu, dac, (const double *)wf, awi, (const double *)c->kcell,
                        (const double *)c->d_ws, dg, P);
