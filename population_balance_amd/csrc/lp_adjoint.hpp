@@ -477,10 +477,9 @@ __global__ void k_tau_frozen(const T *__restrict__ S11, const T *__restrict__ S1
 // droplets (pollen_all_in_one2, scalars_module_crossflow_mod.f90:1494-1786, with
 // pollen_RHS_calc2 :1894-4472, calcbeta_pcon :1795-1843, step_pollen2 :4481-4508), all
 // bins in one launch per pass, with per-bin face adjoints summed at the end.  Frozen: Kc_t (from Nu_t).  The transport is linear in C on each branch
-// of the SMART limiter: its transpose uses the branch the base C selects.  x and y are
-// gathers (a thread per cell adds what the scan would scatter into it, in the scan's
-// order); z is a line scan (a thread owns a z column and scatters into its own cells),
-// so no pass needs atomics.
+// of the SMART limiter: its transpose uses the branch the base C selects.  The three
+// directions are gathers (a thread per cell adds what the line scan of the fluxes
+// transposed would scatter into it, in the scan's order), so no pass needs atomics.
 // ------------------------------------------------------------------------------------
 
 // d smart_face / d (cup, cupup, cdn) on the branch smart_face selects
@@ -529,22 +528,20 @@ struct DropAdjP {
 };
 
 // x (Y false) and y (Y true) transport transposed in gather form, one thread per cell of
-// every bin (consecutive threads: consecutive x).  A cell adds the terms the line scan of
+// every bin (blocks of 64 x by 4 y cells of one level and bin).  A cell adds the terms the line scan of
 // the fluxes transposed scatters into it, in the scan's order: the interior faces c-1, c,
 // c+1, c+2, then the end faces, the Kc diffusion of cells c-1, c, c+1 (ghosts at the line
 // ends), and last the folds of the ghosts' adjoints at the two end cells.  The terms of a
 // face are recomputed by each cell that reads them; cell c = f >= 1 writes face f's
 // adjoint into auv (the bin's (ld, ny+1, nz) face array), cell 0 or n-1 the end faces.
 template <bool Y>
-__global__ void k_adj_drop_line(const double *__restrict__ C, const double *__restrict__ mu,
-                                double *__restrict__ ac, const double *__restrict__ uv0, double *__restrict__ auv,
-                                const double *__restrict__ kc, DropGeo g, DropAdjP P) {
-  const long nl = (long)g.nx * g.ny * (g.nz - 1);
-  const long t = gtid();
-  if (t >= nl * g.npc) return;
-  const int b = (int)(t / nl);
-  const long r = t % nl;
-  const int i = (int)(r % g.nx), j = (int)((r / g.nx) % g.ny), k = (int)(r / ((long)g.nx * g.ny));
+__global__ void __launch_bounds__(256) k_adj_drop_line(const double *__restrict__ C, const double *__restrict__ mu,
+                                                       double *__restrict__ ac, const double *__restrict__ uv0,
+                                                       double *__restrict__ auv, const double *__restrict__ kc,
+                                                       DropGeo g, DropAdjP P) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;  // grid z: level + (nz-1) bin
+  if (i >= g.nx || j >= g.ny) return;
+  const int b = (int)blockIdx.z / (g.nz - 1), k = (int)blockIdx.z - b * (g.nz - 1);
   const int n = Y ? g.ny : g.nx, c = Y ? j : i, bc = Y ? P.bc_y : P.bc_x;
   const long st = Y ? g.ld : 1, ol = Y ? (long)i : (long)j * g.ld, ob = (long)b * g.nv;
   const double *Cl = C + ob + (long)k * g.np + ol, *Ml = mu + ob + (long)k * g.np + ol, *Kl = kc + (long)k * g.np + ol;
@@ -554,24 +551,49 @@ __global__ void k_adj_drop_line(const double *__restrict__ C, const double *__re
   auto Cv = [&](int q) { return Cl[q * st]; };
   auto Mv = [&](int q) { return Ml[q * st]; };
   auto Kv = [&](int q) { return Kl[q * st]; };
-  const double u0 = U[0];
+  // the cell's stencil, loaded together: C at c-3 .. c+3, mu at c-2 .. c+2, Kc at c-2 .. c+2
+  // (periodic), faces c-1 .. c+2; indices off the line are clamped (their values go unused)
+  double Cw[7], Mw[5], Kw[5], Uw[4];
+#pragma unroll
+  for (int d = 0; d < 7; ++d) Cw[d] = Cv(min(max(c + d - 3, 0), n - 1));
+#pragma unroll
+  for (int d = 0; d < 5; ++d) Mw[d] = Mv(min(max(c + d - 2, 0), n - 1));
+#pragma unroll
+  for (int d = 0; d < 5; ++d) {
+    const int q = c + d - 2;
+    Kw[d] = Kv(q < 0 ? q + n : (q >= n ? q - n : q));
+  }
+#pragma unroll
+  for (int d = 0; d < 4; ++d) Uw[d] = U[min(max(c + d - 1, 0), n) * st];
+  const double u0 = U[0], Cf = Cv(0), Cn = Cv(n - 1);
+  double A = ac[ob + (long)k * g.np + ol + c * st];
   double g0 = 0.0, gL = 0.0, s0 = 0.0, s1 = 0.0;  // the ghost cells
-  if (bc == 0) { g0 = Cv(n - 1); gL = Cv(0); }
-  else if (bc == 1) { if (Y) g0 = Cv(0); else gL = Cv(n - 1); }
+  if (bc == 0) { g0 = Cn; gL = Cf; }
+  else if (bc == 1) { if (Y) g0 = Cf; else gL = Cn; }
   else {
     s0 = signbit(u0) ? 1.0 : 0.0;
     s1 = signbit(-u0) ? 1.0 : 0.0;
-    g0 = Cv(0) * s0; gL = Cv(n - 1) * s1;
+    g0 = Cf * s0; gL = Cn * s1;
   }
-  // interior face f (between cells f-1 and f): a = u_f m / d, the SMART branch coefficients
+  // interior face f (between cells f-1 and f): a = u_f m / d, the SMART branch coefficients;
+  // facew takes face c + df from the stencil, face any face from memory
+  auto pick = [&](int f, double uf, double &au, double &auu, double &ad, double &cup, double &cupup, double &cdn,
+                  double cm2, double cm1, double c0, double cp1) {
+    const bool pos = uf >= 0.0;
+    if (pos) { cup = cm1; cupup = f >= 2 ? cm2 : g0; cdn = c0; }
+    else { cup = c0; cupup = f + 1 <= n - 1 ? cp1 : gL; cdn = cm1; }
+    smart_coef(cup, cupup, cdn, au, auu, ad);
+    return pos;
+  };
+  auto facew = [&](int df, double &a, double &au, double &auu, double &ad, double &cup, double &cupup, double &cdn) {
+    const double uf = Uw[df + 1];
+    a = uf * id * (Mw[df + 2] - Mw[df + 1]);
+    return pick(c + df, uf, au, auu, ad, cup, cupup, cdn, Cw[df + 1], Cw[df + 2], Cw[df + 3], Cw[df + 4]);
+  };
   auto face = [&](int f, double &a, double &au, double &auu, double &ad, double &cup, double &cupup, double &cdn) {
     const double uf = U[f * st];
     a = uf * id * (Mv(f) - Mv(f - 1));
-    const bool pos = uf >= 0.0;
-    if (pos) { cup = Cv(f - 1); cupup = f >= 2 ? Cv(f - 2) : g0; cdn = Cv(f); }
-    else { cup = Cv(f); cupup = f + 1 <= n - 1 ? Cv(f + 1) : gL; cdn = Cv(f - 1); }
-    smart_coef(cup, cupup, cdn, au, auu, ad);
-    return pos;
+    return pick(f, uf, au, auu, ad, cup, cupup, cdn, Cv(max(f - 2, 0)), Cv(f - 1), Cv(f), Cv(min(f + 1, n - 1)));
   };
   auto diff = [&](int q, double &Kp, double &Km) {  // returns the cell's m / d^2
     const double Kq = Kv(q);
@@ -579,46 +601,54 @@ __global__ void k_adj_drop_line(const double *__restrict__ C, const double *__re
     Km = 0.5 * (Kq + Kv(q >= 1 ? q - 1 : n - 1));
     return Mv(q) * id2;
   };
-  double A = ac[ob + (long)k * g.np + ol + c * st];
+  auto diffw = [&](int dq, double &Kp, double &Km) {
+    const double Kq = Kw[dq + 2];
+    Kp = 0.5 * (Kq + Kw[dq + 3]);
+    Km = 0.5 * (Kq + Kw[dq + 1]);
+    return Mw[dq + 2] * id2;
+  };
   double a, au, auu, ad, cu, cuu, cd;
-  if (c - 1 >= 1 && !face(c - 1, a, au, auu, ad, cu, cuu, cd)) A += auu * a;
+  if (c - 1 >= 1 && !facew(-1, a, au, auu, ad, cu, cuu, cd)) A += auu * a;
   if (c >= 1) {
-    const bool pos = face(c, a, au, auu, ad, cu, cuu, cd);
+    const bool pos = facew(0, a, au, auu, ad, cu, cuu, cd);
     A += pos ? ad * a : au * a;
-    AU[c * st] += smart_face(cu, cuu, cd) * id * (Mv(c) - Mv(c - 1));
+    AU[c * st] += smart_face(cu, cuu, cd) * id * (Mw[2] - Mw[1]);
   }
   if (c + 1 <= n - 1) {
-    const bool pos = face(c + 1, a, au, auu, ad, cu, cuu, cd);
+    const bool pos = facew(1, a, au, auu, ad, cu, cuu, cd);
     A += pos ? au * a : ad * a;
   }
-  if (c + 2 <= n - 1 && face(c + 2, a, au, auu, ad, cu, cuu, cd)) A += auu * a;
-  // faces 0 and n
-  const double m0 = Mv(0), mL = -Mv(n - 1);
-  const double a0 = u0 * id * m0, aL = U[n * st] * id * mL;
-  if (bc == 0) {
-    int iu, iuu, idn;
-    if (u0 >= 0.0) { iu = n - 1; iuu = n - 2; idn = 0; }
-    else { iu = 0; iuu = 1; idn = n - 1; }
-    if (c == iu || c == iuu || c == idn) {
-      smart_coef(Cv(iu), Cv(iuu), Cv(idn), au, auu, ad);
-      A += (c == iu ? au : c == iuu ? auu : ad) * (a0 + aL);
+  if (c + 2 <= n - 1 && facew(2, a, au, auu, ad, cu, cuu, cd)) A += auu * a;
+  // faces 0 and n: they reach cells 0, 1, n-2 and n-1 only
+  double a0 = 0.0, aL = 0.0;
+  if (c <= 1 || c >= n - 2) {
+    const double m0 = Mv(0), mL = -Mv(n - 1);
+    a0 = u0 * id * m0; aL = U[n * st] * id * mL;
+    if (bc == 0) {
+      int iu, iuu, idn;
+      if (u0 >= 0.0) { iu = n - 1; iuu = n - 2; idn = 0; }
+      else { iu = 0; iuu = 1; idn = n - 1; }
+      if (c == iu || c == iuu || c == idn) {
+        smart_coef(Cv(iu), Cv(iuu), Cv(idn), au, auu, ad);
+        A += (c == iu ? au : c == iuu ? auu : ad) * (a0 + aL);
+      }
+      if (c == 0) {
+        const double f0 = smart_face(Cv(iu), Cv(iuu), Cv(idn));
+        AU[0] += f0 * id * m0;
+        AU[n * st] += f0 * id * mL;
+      }
+    } else if (bc == 1) {  // x: face 0 carries nothing, face n the last cell; y: the mirror
+      if (Y && c == 0) { AU[0] += Cf * id * m0; A += a0; }
+      if (!Y && c == n - 1) { AU[n * st] += Cn * id * mL; A += aL; }
+    } else if (c == 0) {   // the ghosts are the end faces
+      AU[0] += g0 * id * m0;
+      AU[n * st] += gL * id * mL;
     }
-    if (c == 0) {
-      const double f0 = smart_face(Cv(iu), Cv(iuu), Cv(idn));
-      AU[0] += f0 * id * m0;
-      AU[n * st] += f0 * id * mL;
-    }
-  } else if (bc == 1) {  // x: face 0 carries nothing, face n the last cell; y: the mirror
-    if (Y && c == 0) { AU[0] += Cv(0) * id * m0; A += a0; }
-    if (!Y && c == n - 1) { AU[n * st] += Cv(n - 1) * id * mL; A += aL; }
-  } else if (c == 0) {   // the ghosts are the end faces
-    AU[0] += g0 * id * m0;
-    AU[n * st] += gL * id * mL;
   }
   double Kp, Km;
-  if (c >= 1) { const double m = diff(c - 1, Kp, Km); A += Kp * m; }
-  { const double m = diff(c, Kp, Km); A -= (Kp + Km) * m; }
-  if (c + 1 <= n - 1) { const double m = diff(c + 1, Kp, Km); A += Km * m; }
+  if (c >= 1) { const double m = diffw(-1, Kp, Km); A += Kp * m; }
+  { const double m = diffw(0, Kp, Km); A -= (Kp + Km) * m; }
+  if (c + 1 <= n - 1) { const double m = diffw(1, Kp, Km); A += Km * m; }
   if (c == 0 || c == n - 1) {  // the ghosts' adjoints, summed in the scan's order
     double ag0 = 0.0, agL = 0.0;
     if (face(1, a, au, auu, ad, cu, cuu, cd)) ag0 += auu * a;
@@ -633,75 +663,80 @@ __global__ void k_adj_drop_line(const double *__restrict__ C, const double *__re
   ac[ob + (long)k * g.np + ol + c * st] = A;
 }
 
-// z: one thread per (x < nx, y) column and bin; w faces 1..nz-1 (face 0 carries nothing,
-// face nz-1 repeats nz-2, face 1 is central for upward flow); awi (a volume per bin) gains
-// the bin's adjoint of the w faces
-__global__ void k_adj_drop_z(const double *__restrict__ C, const double *__restrict__ mu, double *__restrict__ ac,
-                             const double *__restrict__ w, double *__restrict__ awi_all, const double *__restrict__ kc,
-                             const double *__restrict__ wsb, DropGeo g, DropAdjP P) {
-  const long nl = (long)g.nx * g.ny;
-  const long t0 = gtid();
-  if (t0 >= nl * g.npc) return;
-  const int b = (int)(t0 / nl);
-  const long t = t0 % nl;
-  const int j = (int)(t / g.nx), i = (int)(t % g.nx), nz = g.nz;
+// z in gather form, one thread per cell (x < nx) of every bin (blocks as in x, y): w faces 1..nz-1 (face 0
+// carries nothing, face nz-1 repeats nz-2, face 1 is central for upward flow).  A face
+// adds to three distinct cells, so cell c takes the terms of faces c-1 .. c+2 and then the
+// Kc diffusion terms of levels c-1, c, c+1, in the order of the level scan; cell c = k
+// writes the bin's adjoint of w face k into awi (a volume per bin), cell nz-2 also face nz-1's.
+__global__ void __launch_bounds__(256) k_adj_drop_z(const double *__restrict__ C, const double *__restrict__ mu,
+                                                    double *__restrict__ ac, const double *__restrict__ w,
+                                                    double *__restrict__ awi_all, const double *__restrict__ kc,
+                                                    const double *__restrict__ wsb, DropGeo g, DropAdjP P) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;  // grid z: level + nz bin
+  if (i >= g.nx || j >= g.ny) return;
+  const int nz = g.nz, b = (int)blockIdx.z / nz, c = (int)blockIdx.z - b * nz;
   const long o = (long)j * g.ld + i, np = g.np, ob = (long)b * g.nv;
   const double ws = P.settling ? wsb[b] : 0.0;
-  double *awi = awi_all + ob;
-#define CZ(k) C[ob + (long)(k) * np + o]
-#define MZ(k) mu[ob + (long)(k) * np + o]
-#define AZ(k) ac[ob + (long)(k) * np + o]
-#define KZ(k) kc[(long)(k) * np + o]
-  auto wface = [&](int k) { return w[(long)k * np + o] - ((k >= 1 && k <= nz - 2) ? ws : 0.0); };
-  // face nz-1 holds face nz-2's value: its adjoint joins face nz-2's
-  const double atop = wface(nz - 1) * P.idz * (-MZ(nz - 2));
-  for (int k = 1; k <= nz - 2; ++k) {
-    const double wk = wface(k);
-    const double m = MZ(k) - MZ(k - 1);
+  auto lv = [&](int q) { return (long)min(max(q, 0), nz - 1) * np + o; };  // clamped level q
+  // the cell's stencil, loaded together: C at c-3 .. c+3, mu at c-2 .. c+2, w at c-1 .. c+3,
+  // Kc at c-1 .. c+1; levels off the column are clamped (their values go unused)
+  double Cw[7], Mw[5], Ww[5], Kw[3];
+#pragma unroll
+  for (int d = 0; d < 7; ++d) Cw[d] = C[ob + lv(c + d - 3)];
+#pragma unroll
+  for (int d = 0; d < 5; ++d) Mw[d] = mu[ob + lv(c + d - 2)];
+#pragma unroll
+  for (int d = 0; d < 5; ++d) Ww[d] = w[lv(c + d - 1)];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) Kw[d] = kc[lv(c + d - 1)];
+  double A = ac[ob + (long)c * np + o];
+#pragma unroll
+  for (int dk = -1; dk <= 2; ++dk) {  // faces k = c-1 .. c+2 within 1 .. nz-2
+    const int k = c + dk;
+    if (k < 1 || k > nz - 2) continue;
+    const double wk = Ww[dk + 1] - ws;
+    const double m = Mw[dk + 2] - Mw[dk + 1];
     double at = wk * P.idz * m;
-    if (k == nz - 2) at += atop;
+    if (k == nz - 2) at += (Ww[dk + 2] - 0.0) * P.idz * (-Mw[dk + 2]);  // face nz-1 joins nz-2
     double au, auu, ad, face;
     int ku, kuu, kdn;
     if (k == 1 && wk >= 0.0) {
-      face = 0.5 * (CZ(1) + CZ(0));
+      face = 0.5 * (Cw[dk + 3] + Cw[dk + 2]);
       ku = 1; kdn = 0; kuu = -1; au = 0.5; ad = 0.5; auu = 0.0;
     } else {
-      if (wk >= 0.0) { ku = k - 1; kuu = k - 2; kdn = k; }
-      else { ku = k; kuu = k + 1; kdn = k - 1; }
-      const double cup = CZ(ku), cupup = CZ(kuu), cdn = CZ(kdn);
+      double cup, cupup, cdn;
+      if (wk >= 0.0) { ku = k - 1; kuu = k - 2; kdn = k; cup = Cw[dk + 2]; cupup = Cw[dk + 1]; cdn = Cw[dk + 3]; }
+      else { ku = k; kuu = k + 1; kdn = k - 1; cup = Cw[dk + 3]; cupup = Cw[dk + 4]; cdn = Cw[dk + 2]; }
       face = smart_face(cup, cupup, cdn);
       smart_coef(cup, cupup, cdn, au, auu, ad);
     }
-    awi[(long)k * np + o] += face * P.idz * m;
-    if (k == nz - 2) awi[(long)(nz - 1) * np + o] += face * P.idz * (-MZ(nz - 2));
-    AZ(ku) += au * at; AZ(kdn) += ad * at;
-    if (kuu >= 0) AZ(kuu) += auu * at;
-  }
-  for (int k = 0; k <= nz - 2; ++k) {
-    const double m = MZ(k) * P.idz2;
-    if (k <= nz - 3) {
-      const double Kp = 0.5 * (KZ(k + 1) + KZ(k));
-      AZ(k + 1) += Kp * m; AZ(k) -= Kp * m;
+    if (dk == 0) {
+      double *awi = awi_all + ob;
+      awi[(long)k * np + o] += face * P.idz * m;
+      if (k == nz - 2) awi[(long)(nz - 1) * np + o] += face * P.idz * (-Mw[2]);
     }
-    if (k >= 1) {
-      const double Km = 0.5 * (KZ(k) + KZ(k - 1));
-      AZ(k) -= Km * m; AZ(k - 1) += Km * m;
-    }
+    if (ku == c) A += au * at;
+    else if (kdn == c) A += ad * at;
+    else if (kuu == c) A += auu * at;
   }
-#undef CZ
-#undef MZ
-#undef AZ
-#undef KZ
+  if (c >= 1 && c - 1 <= nz - 3) A += 0.5 * (Kw[1] + Kw[0]) * (Mw[1] * P.idz2);  // level c-1
+  if (c <= nz - 2) {                                                            // level c
+    const double m = Mw[2] * P.idz2;
+    if (c <= nz - 3) A -= 0.5 * (Kw[2] + Kw[1]) * m;
+    if (c >= 1) A -= 0.5 * (Kw[1] + Kw[0]) * m;
+  }
+  if (c + 1 <= nz - 2) A += 0.5 * (Kw[2] + Kw[1]) * (Mw[3] * P.idz2);           // level c+1
+  ac[ob + (long)c * np + o] = A;
 }
 
-// the face-velocity reconstruction transposed, along x (one thread per (y, level) row):
+// the face-velocity reconstruction transposed, along x:
 // ui0(i) = mean(u) + P(i) - mean(P), P(i) = sum_{m<i} dx dudx(m), ui0(nx) = ui0(0).  With
 // l the adjoint of the faces (face nx folded into 0) and S = sum l:
 //   adjoint of dudx(m), m < nx-1: dx (sum_{i>m} l(i) - S (nx-1-m)/nx);  of u(m): S/nx.
-// The same along y (k_adj_face_v, one thread per (x, level) column).
+// The same along y.
 // The face adjoints of all bins summed (bin by bin in index order, from zero), for
-// k_adj_face_u and k_adj_face_v: one coalesced pass instead of npc strided loads per face
-// and use inside the line loops.
+// k_adj_face_line: one coalesced pass instead of npc strided loads per face inside the
+// line passes.
 __global__ void k_adj_face_bins(const double *__restrict__ aui, const double *__restrict__ avi,
                                 double *__restrict__ lu, double *__restrict__ lv, DropGeo g) {
   const long fz = g.fp * g.nz;
@@ -716,42 +751,36 @@ __global__ void k_adj_face_bins(const double *__restrict__ aui, const double *__
   lv[s] = b;
 }
 
-// lu, lv: k_adj_face_bins's sums
-__global__ void k_adj_face_u(const double *__restrict__ lu, double *__restrict__ aux, double *__restrict__ auf,
-                             DropGeo g, double dx) {
-  const long row = gtid();
-  if (row >= (long)g.ny * g.nz) return;
-  const int k = (int)(row / g.ny), j = (int)(row % g.ny), nx = g.nx;
-  const double *L0 = lu + (long)k * g.fp + (long)j * g.ld;
-  auto L = [&](int i) { return L0[i]; };  // summed over the bins
-  const long o = (long)k * g.np + (long)j * g.ld;
-  const double l0 = L(0) + L(nx);
-  double S = l0;
-  for (int i = 1; i < nx; ++i) S += L(i);
-  double tail = S - l0;  // sum_{i > 0} l(i), then sum_{i > m}
-  for (int m = 0; m < nx; ++m) {
-    if (m < nx - 1) aux[o + m] += dx * (tail - S * (double)(nx - 1 - m) / nx);
-    auf[o + m] += S / nx;
-    if (m + 1 < nx) tail -= L(m + 1);
-  }
-}
-__global__ void k_adj_face_v(const double *__restrict__ lv, double *__restrict__ avy, double *__restrict__ avf,
-                             DropGeo g, double dy) {
-  const long t = gtid();
-  if (t >= (long)g.nx * g.nz) return;
-  const int k = (int)(t / g.nx), i = (int)(t % g.nx), ny = g.ny;
-  const long ld = g.ld;
-  const double *L0 = lv + (long)k * g.fp + i;
-  auto L = [&](int j) { return L0[(long)j * ld]; };  // summed over the bins
-  const long o = (long)k * g.np + i;
-  const double l0 = L(0) + L(ny);
-  double S = l0;
-  for (int j = 1; j < ny; ++j) S += L(j);
-  double tail = S - l0;
-  for (int m = 0; m < ny; ++m) {
-    if (m < ny - 1) avy[o + (long)m * ld] += dy * (tail - S * (double)(ny - 1 - m) / ny);
-    avf[o + (long)m * ld] += S / ny;
-    if (m + 1 < ny) tail -= L(m + 1);
+// lsum: k_adj_face_bins's sums.  One line per wave (x: a (y, level) row, y: an (x, level)
+// column, consecutive waves at consecutive x): S by a wave sum, the tails sum_{i>m} l(i)
+// by suffix scans over 64-face chunks from the end of the line.
+template <bool Y>
+__global__ void __launch_bounds__(256) k_adj_face_line(const double *__restrict__ lsum, double *__restrict__ ad,
+                                                       double *__restrict__ af, DropGeo g, double d) {
+  const int lane = threadIdx.x & 63;
+  const long line = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int nlat = Y ? g.nx : g.ny;
+  if (line >= (long)nlat * g.nz) return;  // uniform over the wave
+  const int k = (int)(line / nlat), q = (int)(line % nlat), n = Y ? g.ny : g.nx;
+  const long st = Y ? g.ld : 1, ol = Y ? (long)q : (long)q * g.ld;
+  const double *L0 = lsum + (long)k * g.fp + ol;
+  const long o = (long)k * g.np + ol;
+  double part = 0.0;
+  for (int i = 1 + lane; i < n; i += 64) part += L0[i * st];
+  for (int sh = 32; sh > 0; sh >>= 1) part += __shfl_xor(part, sh, 64);
+  const double S = (L0[0] + L0[(long)n * st]) + part;  // face n folded into face 0
+  double carry = 0.0;
+  for (int base = (n - 1) / 64 * 64; base >= 0; base -= 64) {
+    const int m = base + lane;
+    double v = m + 1 <= n - 1 ? L0[(long)(m + 1) * st] : 0.0;
+    for (int sh = 1; sh < 64; sh <<= 1) {
+      const double y = __shfl_down(v, sh, 64);
+      if (lane + sh < 64) v += y;
+    }
+    const double tail = v + carry;  // sum_{i > m} l(i)
+    carry = __shfl(tail, 0, 64);
+    if (m < n - 1) ad[o + m * st] += d * (tail - S * (double)(n - 1 - m) / n);
+    if (m < n) af[o + m * st] += S / n;
   }
 }
 

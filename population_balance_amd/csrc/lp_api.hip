@@ -2611,14 +2611,17 @@ static int do_step_adjoint(lp_ctx *c, const double *nut, const double *ust, doub
     const double *PC = (const double *)c->f[LP_PCON];
     hipLaunchKernelGGL(k_adj_drop_update, grid1(dg.nv * D.npcon), dim3(256), 0, c->s, (const double *)cb, rcb,
                        (const double *)arz, (const double *)c->d_vfac, D.active, dmu, dac, dg, dt, euler);
-    const long ncell = (long)gg.nx * gg.ny * (gg.nz - 1) * D.npcon;
-    hipLaunchKernelGGL(k_adj_drop_line<false>, grid1(ncell), dim3(256), 0, c->s, PC, (const double *)dmu, dac,
+    // 64 x 4 cells per block, grid z over (level, bin): gg.nz * npcon <= 65 535 levels x bins
+    if ((long)gg.nz * D.npcon > 65535)
+      return set_err(LP_ERR_ARG, "droplet adjoint: nz * npcon = %ld exceeds 65535", (long)gg.nz * D.npcon);
+    const dim3 gdl((unsigned)((gg.nx + 63) / 64), (unsigned)((gg.ny + 3) / 4), (unsigned)((gg.nz - 1) * D.npcon));
+    const dim3 gdz(gdl.x, gdl.y, (unsigned)(gg.nz * D.npcon));
+    hipLaunchKernelGGL(k_adj_drop_line<false>, gdl, dim3(64, 4), 0, c->s, PC, (const double *)dmu, dac,
                        (const double *)c->ui0, aui, (const double *)c->kcell, dg, P);
-    hipLaunchKernelGGL(k_adj_drop_line<true>, grid1(ncell), dim3(256), 0, c->s, PC, (const double *)dmu, dac,
+    hipLaunchKernelGGL(k_adj_drop_line<true>, gdl, dim3(64, 4), 0, c->s, PC, (const double *)dmu, dac,
                        (const double *)c->vi0, avi, (const double *)c->kcell, dg, P);
-    hipLaunchKernelGGL(k_adj_drop_z, grid1((long)gg.nx * gg.ny * D.npcon), dim3(256), 0, c->s, PC,
-                       (const double *)dmu, dac, (const double *)wf, awi, (const double *)c->kcell,
-                       (const double *)c->d_ws, dg, P);
+    hipLaunchKernelGGL(k_adj_drop_z, gdz, dim3(64, 4), 0, c->s, PC, (const double *)dmu, dac, (const double *)wf, awi,
+                       (const double *)c->kcell, (const double *)c->d_ws, dg, P);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(cb, dac, sizeof(double) * nb * nv, hipMemcpyDeviceToDevice, c->s));
   }
@@ -2675,10 +2678,10 @@ static int do_step_adjoint(lp_ctx *c, const double *nut, const double *ust, doub
     double *lu = dmu, *lv = dmu + fz;
     hipLaunchKernelGGL(k_adj_face_bins, grid1(fz), dim3(256), 0, c->s, (const double *)aui, (const double *)avi, lu, lv,
                        dg);
-    hipLaunchKernelGGL(k_adj_face_u, grid1((long)gg.ny * gg.nz), dim3(256), 0, c->s, (const double *)lu, aux, afu, dg,
-                       gg.dx);
-    hipLaunchKernelGGL(k_adj_face_v, grid1((long)gg.nx * gg.nz), dim3(256), 0, c->s, (const double *)lv, avy, afv, dg,
-                       gg.dy);
+    hipLaunchKernelGGL(k_adj_face_line<false>, dim3((unsigned)(((long)gg.ny * gg.nz + 3) / 4)), dim3(256), 0, c->s,
+                       (const double *)lu, aux, afu, dg, gg.dx);
+    hipLaunchKernelGGL(k_adj_face_line<true>, dim3((unsigned)(((long)gg.nx * gg.nz + 3) / 4)), dim3(256), 0, c->s,
+                       (const double *)lv, avy, afv, dg, gg.dy);
     HIPCHK(hipGetLastError());
   }
   // ---- filt_da transposed: F(af) - Dx(a*x) - Dy(a*y) ----
