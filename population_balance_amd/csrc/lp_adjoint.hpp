@@ -527,21 +527,17 @@ struct DropAdjP {
   int bc_x, bc_y, settling;
 };
 
-// x (Y false) and y (Y true) transport transposed in gather form, one thread per cell of
-// every bin (blocks of 64 x by 4 y cells of one level and bin).  A cell adds the terms the line scan of
+// x (Y false) and y (Y true) transport transposed in gather form for one cell of one bin.  A cell adds the terms the line scan of
 // the fluxes transposed scatters into it, in the scan's order: the interior faces c-1, c,
 // c+1, c+2, then the end faces, the Kc diffusion of cells c-1, c, c+1 (ghosts at the line
 // ends), and last the folds of the ghosts' adjoints at the two end cells.  The terms of a
 // face are recomputed by each cell that reads them; cell c = f >= 1 writes face f's
 // adjoint into auv (the bin's (ld, ny+1, nz) face array), cell 0 or n-1 the end faces.
 template <bool Y>
-__global__ void __launch_bounds__(256) k_adj_drop_line(const double *__restrict__ C, const double *__restrict__ mu,
-                                                       double *__restrict__ ac, const double *__restrict__ uv0,
-                                                       double *__restrict__ auv, const double *__restrict__ kc,
-                                                       DropGeo g, DropAdjP P) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;  // grid z: level + (nz-1) bin
-  if (i >= g.nx || j >= g.ny) return;
-  const int b = (int)blockIdx.z / (g.nz - 1), k = (int)blockIdx.z - b * (g.nz - 1);
+__device__ __forceinline__ double drop_line_cell(double A, int i, int j, int k, int b, const double *__restrict__ C,
+                                                 const double *__restrict__ mu, const double *__restrict__ uv0,
+                                                 double *__restrict__ auv, const double *__restrict__ kc,
+                                                 const DropGeo &g, const DropAdjP &P) {
   const int n = Y ? g.ny : g.nx, c = Y ? j : i, bc = Y ? P.bc_y : P.bc_x;
   const long st = Y ? g.ld : 1, ol = Y ? (long)i : (long)j * g.ld, ob = (long)b * g.nv;
   const double *Cl = C + ob + (long)k * g.np + ol, *Ml = mu + ob + (long)k * g.np + ol, *Kl = kc + (long)k * g.np + ol;
@@ -566,7 +562,6 @@ __global__ void __launch_bounds__(256) k_adj_drop_line(const double *__restrict_
 #pragma unroll
   for (int d = 0; d < 4; ++d) Uw[d] = U[min(max(c + d - 1, 0), n) * st];
   const double u0 = U[0], Cf = Cv(0), Cn = Cv(n - 1);
-  double A = ac[ob + (long)k * g.np + ol + c * st];
   double g0 = 0.0, gL = 0.0, s0 = 0.0, s1 = 0.0;  // the ghost cells
   if (bc == 0) { g0 = Cn; gL = Cf; }
   else if (bc == 1) { if (Y) g0 = Cf; else gL = Cn; }
@@ -660,21 +655,19 @@ __global__ void __launch_bounds__(256) k_adj_drop_line(const double *__restrict_
     else if (bc == 1) { if (Y && c == 0) A += ag0; if (!Y && c == n - 1) A += agL; }
     else A += c == 0 ? s0 * ag0 : s1 * agL;
   }
-  ac[ob + (long)k * g.np + ol + c * st] = A;
+  return A;
 }
 
-// z in gather form, one thread per cell (x < nx) of every bin (blocks as in x, y): w faces 1..nz-1 (face 0
+// z in gather form for one cell (x < nx) of one bin: w faces 1..nz-1 (face 0
 // carries nothing, face nz-1 repeats nz-2, face 1 is central for upward flow).  A face
 // adds to three distinct cells, so cell c takes the terms of faces c-1 .. c+2 and then the
 // Kc diffusion terms of levels c-1, c, c+1, in the order of the level scan; cell c = k
 // writes the bin's adjoint of w face k into awi (a volume per bin), cell nz-2 also face nz-1's.
-__global__ void __launch_bounds__(256) k_adj_drop_z(const double *__restrict__ C, const double *__restrict__ mu,
-                                                    double *__restrict__ ac, const double *__restrict__ w,
-                                                    double *__restrict__ awi_all, const double *__restrict__ kc,
-                                                    const double *__restrict__ wsb, DropGeo g, DropAdjP P) {
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;  // grid z: level + nz bin
-  if (i >= g.nx || j >= g.ny) return;
-  const int nz = g.nz, b = (int)blockIdx.z / nz, c = (int)blockIdx.z - b * nz;
+__device__ __forceinline__ double drop_z_cell(double A, int i, int j, int c, int b, const double *__restrict__ C,
+                                              const double *__restrict__ mu, const double *__restrict__ w,
+                                              double *__restrict__ awi_all, const double *__restrict__ kc,
+                                              const double *__restrict__ wsb, const DropGeo &g, const DropAdjP &P) {
+  const int nz = g.nz;
   const long o = (long)j * g.ld + i, np = g.np, ob = (long)b * g.nv;
   const double ws = P.settling ? wsb[b] : 0.0;
   auto lv = [&](int q) { return (long)min(max(q, 0), nz - 1) * np + o; };  // clamped level q
@@ -689,7 +682,6 @@ __global__ void __launch_bounds__(256) k_adj_drop_z(const double *__restrict__ C
   for (int d = 0; d < 5; ++d) Ww[d] = w[lv(c + d - 1)];
 #pragma unroll
   for (int d = 0; d < 3; ++d) Kw[d] = kc[lv(c + d - 1)];
-  double A = ac[ob + (long)c * np + o];
 #pragma unroll
   for (int dk = -1; dk <= 2; ++dk) {  // faces k = c-1 .. c+2 within 1 .. nz-2
     const int k = c + dk;
@@ -726,7 +718,28 @@ __global__ void __launch_bounds__(256) k_adj_drop_z(const double *__restrict__ C
     if (c >= 1) A -= 0.5 * (Kw[1] + Kw[0]) * m;
   }
   if (c + 1 <= nz - 2) A += 0.5 * (Kw[2] + Kw[1]) * (Mw[3] * P.idz2);           // level c+1
-  ac[ob + (long)c * np + o] = A;
+  return A;
+}
+
+// the three directions for one cell of one bin (blocks of 64 x by 4 y cells, grid z over
+// level + nz bin): x, then y (levels below nz-1), then z, each as the passes above
+__global__ void __launch_bounds__(256) k_adj_drop_xyz(const double *__restrict__ C, const double *__restrict__ mu,
+                                                      double *__restrict__ ac, const double *__restrict__ ui0,
+                                                      const double *__restrict__ vi0, const double *__restrict__ w,
+                                                      double *__restrict__ aui, double *__restrict__ avi,
+                                                      double *__restrict__ awi, const double *__restrict__ kc,
+                                                      const double *__restrict__ wsb, DropGeo g, DropAdjP P) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  if (i >= g.nx || j >= g.ny) return;
+  const int b = (int)blockIdx.z / g.nz, k = (int)blockIdx.z - b * g.nz;
+  const long s = (long)b * g.nv + (long)k * g.np + (long)j * g.ld + i;
+  double A = ac[s];
+  if (k < g.nz - 1) {
+    A = drop_line_cell<false>(A, i, j, k, b, C, mu, ui0, aui, kc, g, P);
+    A = drop_line_cell<true>(A, i, j, k, b, C, mu, vi0, avi, kc, g, P);
+  }
+  A = drop_z_cell(A, i, j, k, b, C, mu, w, awi, kc, wsb, g, P);
+  ac[s] = A;
 }
 
 // the face-velocity reconstruction transposed, along x:

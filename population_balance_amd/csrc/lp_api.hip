@@ -2614,13 +2614,9 @@ static int do_step_adjoint(lp_ctx *c, const double *nut, const double *ust, doub
     // 64 x 4 cells per block, grid z over (level, bin): gg.nz * npcon <= 65 535 levels x bins
     if ((long)gg.nz * D.npcon > 65535)
       return set_err(LP_ERR_ARG, "droplet adjoint: nz * npcon = %ld exceeds 65535", (long)gg.nz * D.npcon);
-    const dim3 gdl((unsigned)((gg.nx + 63) / 64), (unsigned)((gg.ny + 3) / 4), (unsigned)((gg.nz - 1) * D.npcon));
-    const dim3 gdz(gdl.x, gdl.y, (unsigned)(gg.nz * D.npcon));
-    hipLaunchKernelGGL(k_adj_drop_line<false>, gdl, dim3(64, 4), 0, c->s, PC, (const double *)dmu, dac,
-                       (const double *)c->ui0, aui, (const double *)c->kcell, dg, P);
-    hipLaunchKernelGGL(k_adj_drop_line<true>, gdl, dim3(64, 4), 0, c->s, PC, (const double *)dmu, dac,
-                       (const double *)c->vi0, avi, (const double *)c->kcell, dg, P);
-    hipLaunchKernelGGL(k_adj_drop_z, gdz, dim3(64, 4), 0, c->s, PC, (const double *)dmu, dac, (const double *)wf, awi,
+    hipLaunchKernelGGL(k_adj_drop_xyz, dim3((unsigned)((gg.nx + 63) / 64), (unsigned)((gg.ny + 3) / 4),
+                       (unsigned)(gg.nz * D.npcon)), dim3(64, 4), 0, c->s, PC, (const double *)dmu, dac,
+                       (const double *)c->ui0, (const double *)c->vi0, (const double *)wf, aui, avi, awi,
                        (const double *)c->kcell, (const double *)c->d_ws, dg, P);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(cb, dac, sizeof(double) * nb * nv, hipMemcpyDeviceToDevice, c->s));
