@@ -259,9 +259,14 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 128x128x65 adjoint gradient")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the mixed step: warmup + steps, nothing else (a target for profilers)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the 128x128x65 adjoint gradient, its record to stdout (a target for profilers)")
     a = ap.parse_args()
     if a.copy_probe:
         copy_probe()
+        return
+    if a.adjoint_only:
+        print(json.dumps(adjoint_leg(a.seed)), flush=True)
         return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
