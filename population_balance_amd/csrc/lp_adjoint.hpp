@@ -646,11 +646,12 @@ __device__ __forceinline__ double drop_line_cell(double A, int i, int j, int k, 
   if (c + 1 <= n - 1) { const double m = diffw(1, Kp, Km); A += Km * m; }
   if (c == 0 || c == n - 1) {  // the ghosts' adjoints, summed in the scan's order
     double ag0 = 0.0, agL = 0.0;
-    if (face(1, a, au, auu, ad, cu, cuu, cd)) ag0 += auu * a;
-    if (!face(n - 1, a, au, auu, ad, cu, cuu, cd)) agL += auu * a;
+    const bool lo = c == 0;  // the near end's face and diffusion come from the stencil
+    if (lo ? facew(1, a, au, auu, ad, cu, cuu, cd) : face(1, a, au, auu, ad, cu, cuu, cd)) ag0 += auu * a;
+    if (!(lo ? face(n - 1, a, au, auu, ad, cu, cuu, cd) : facew(0, a, au, auu, ad, cu, cuu, cd))) agL += auu * a;
     if (bc == 2) { ag0 += a0; agL += aL; }
-    { const double m = diff(0, Kp, Km); ag0 += Km * m; }
-    { const double m = diff(n - 1, Kp, Km); agL += Kp * m; }
+    { const double m = lo ? diffw(0, Kp, Km) : diff(0, Kp, Km); ag0 += Km * m; }
+    { const double m = lo ? diff(n - 1, Kp, Km) : diffw(0, Kp, Km); agL += Kp * m; }
     if (bc == 0) A += c == n - 1 ? ag0 : agL;
     else if (bc == 1) { if (Y && c == 0) A += ag0; if (!Y && c == n - 1) A += agL; }
     else A += c == 0 ? s0 * ag0 : s1 * agL;
