@@ -2498,6 +2498,9 @@ static int do_step_adjoint(lp_ctx *c, const double *nut, const double *ust, doub
                            double *rcb) {
   const lp_grid &gg = c->g;
   const lp_config &cf = c->cfg;
+  // the droplet gather's grid z covers (level, bin): at most 65 535, checked before any launch
+  if ((long)gg.nz * cf.drop.npcon > 65535)
+    return set_err(LP_ERR_ARG, "droplet adjoint: nz * npcon = %ld exceeds 65535", (long)gg.nz * cf.drop.npcon);
   const long nv = vol_el(c), np = plane_el(c);
   const long npb = (long)gg.ld_big * gg.ny2, nvb = npb * gg.nz;
   const long nh = (long)gg.lh * gg.ny;
@@ -2611,9 +2614,7 @@ static int do_step_adjoint(lp_ctx *c, const double *nut, const double *ust, doub
     const double *PC = (const double *)c->f[LP_PCON];
     hipLaunchKernelGGL(k_adj_drop_update, grid1(dg.nv * D.npcon), dim3(256), 0, c->s, (const double *)cb, rcb,
                        (const double *)arz, (const double *)c->d_vfac, D.active, dmu, dac, dg, dt, euler);
-    // 64 x 4 cells per block, grid z over (level, bin): gg.nz * npcon <= 65 535 levels x bins
-    if ((long)gg.nz * D.npcon > 65535)
-      return set_err(LP_ERR_ARG, "droplet adjoint: nz * npcon = %ld exceeds 65535", (long)gg.nz * D.npcon);
+    // 64 x 4 cells per block, grid z over (level, bin) (nz * npcon checked on entry)
     hipLaunchKernelGGL(k_adj_drop_xyz, dim3((unsigned)((gg.nx + 63) / 64), (unsigned)((gg.ny + 3) / 4),
                        (unsigned)(gg.nz * D.npcon)), dim3(64, 4), 0, c->s, PC, (const double *)dmu, dac,
                        (const double *)c->ui0, (const double *)c->vi0, (const double *)wf, aui, avi, awi,
